@@ -1,0 +1,262 @@
+/*
+ * mft.h -- C ABI of the MI355X engine for the MATILDA.FT hot paths.
+ *
+ * Plain C: device and host pointers, sizes and an opaque stream handle
+ * (hipStream_t passed as void*, NULL = the null stream).  Every call returns an
+ * mft_status; mft_last_error() describes the last failure on the calling
+ * thread.  Nothing here exits the process (the reference's die(), die.cu:7-11,
+ * becomes a status code that the drop-in layer maps back to the same message).
+ *
+ * Reference paths below are relative to src/ of MATILDA.FT.
+ */
+#ifndef MFT_H
+#define MFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    MFT_OK = 0,
+    MFT_ERR_INVALID = 1,     /* bad argument or unsupported configuration */
+    MFT_ERR_OOM = 2,         /* device allocation failed or over budget */
+    MFT_ERR_NONFINITE = 3,   /* NaN/Inf detected (FTS_Box.cu:150-152 analogue) */
+    MFT_ERR_DEVICE = 4,      /* HIP runtime error */
+    MFT_ERR_UNSUPPORTED = 5  /* valid reference input this engine does not run */
+} mft_status;
+
+const char* mft_last_error(void);
+int mft_version(void);
+
+/* ------------------------------------------------------------ device ---- */
+/* Thin helpers so that hosts without a GPU framework can move data. */
+int mft_device_count(int* n);
+int mft_malloc(void** dptr, size_t bytes);
+int mft_free(void* dptr);
+int mft_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int mft_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int mft_memset(void* dst, int value, size_t bytes);
+int mft_synchronize(void);
+int mft_stream_create(void** stream);
+int mft_stream_destroy(void* stream);
+int mft_stream_synchronize(void* stream);
+
+/* --------------------------------------------------------------- RNG ---- */
+/* Replaces curand_init/curand_normal (device_utils.cu:465-472,
+ * Box.cu:465-488, fts_make_CLnoise.cu:7-24): Philox4x32-10 blocks for
+ * counters (i, c1, c2, c3), i = 0..nblocks-1, keyed by seed.  raw: 4 uint32 per
+ * block; normal: 4 float normals per block (Box-Muller on 24-bit uniforms). */
+int mft_philox_raw(uint32_t* out, int64_t nblocks, uint64_t seed, uint32_t c1, uint32_t c2,
+                   uint32_t c3, void* stream);
+int mft_philox_normal(float* out, int64_t nblocks, uint64_t seed, uint32_t c1, uint32_t c2,
+                      uint32_t c3, void* stream);
+
+/* --------------------------------------------------------------- FFT ---- */
+/* Replaces cufftPlan3d/cufftExecC2C (PS_Box_initialization.cu:369-372,
+ * Box.cu:107-127) and cufftExecZ2Z (FTS_Box.cu:466-469, Box.cu:76-104).
+ * Complex interleaved float, x fastest.  dir = 1: forward scaled by 1/M (the
+ * reference wrappers' convention); dir = -1: inverse, unscaled.  nz = 1 for 2D.
+ * batch grids are consecutive.  Every axis length must be one of
+ * 2,3,4,5,6,8,10,12,16,20,24,32,40,48,64,80,96,128,160,192,256. */
+typedef struct mft_fft_plan mft_fft_plan;
+int mft_fft_plan_create(mft_fft_plan** plan, int nx, int ny, int nz, int batch);
+int mft_fft_exec(mft_fft_plan* plan, const void* in, void* out, int dir, void* stream);
+int mft_fft_plan_destroy(mft_fft_plan* plan);
+int mft_fft_supported(int n);
+
+/* ============================================================== TILD ==== */
+/* Per-kernel entry points (device pointers, reference layouts: positions
+ * x[i*3+j], grids x-fastest, vector grid fields AoS F[g*3+j]).  Dim = 3. */
+
+/* d_calcGridWeights (ps_grid_utils.cu:74-204): (p+1)^3 grid indices and
+ * weights/gvol per particle; indices identical to the reference's. */
+int mft_tild_grid_weights(float* gridW, int* gridInds, const float* x, int ns, const int Nx[3],
+                          const float dx[3], int pmeorder, void* stream);
+/* d_fillDensityGrid / d_fillChargeDensityGrid (device_utils.cu:282-352) from
+ * the tables above for the listed sites (ns particles in total, M grid
+ * points); rho (and rhoq when q != NULL) are accumulated into, like the
+ * reference (zero them first).  Deterministic (integer accumulation). */
+int mft_tild_fill_density(float* rho, float* rhoq, const float* q, const int* sites, int nsites,
+                          int ns, const int* gridInds, const float* gridW, int G, int M,
+                          void* stream);
+/* d_mapGridForcesToPartics / d_mapGridChargeForcesToPartics
+ * (device_utils.cu:358-461): f[p] += sum_g F[g*3+j] W gvol (x q_p if q). */
+int mft_tild_map_forces(float* f, const float* gridF, const float* q, const int* sites,
+                        int nsites, int ns, const int* gridInds, const float* gridW, int G,
+                        float gvol, int M, void* stream);
+/* d_bonds (ps_deviceBonds.cu:7-79).  The per-type tables bondReq, bondK,
+ * bondStyle [nbondtypes] are host arrays; the rest are device arrays. */
+int mft_tild_bonds(float* f, const int* nBonds, const int* bondedTo, const int* bondType,
+                   const float* bondReq, const float* bondK, const int* bondStyle, int nbondtypes,
+                   const float* x, const float L[3], int ns, int maxbonds, void* stream);
+
+/* Potential kinds (ps_potential.cu:294-333). */
+enum { MFT_POT_GAUSSIAN = 0, MFT_POT_ERF2 = 1, MFT_POT_ERFG = 2, MFT_POT_CHARGES = 3 };
+/* Integrators (ps_integrator.cu:51-69). */
+enum { MFT_INT_GJF = 0, MFT_INT_EM = 1, MFT_INT_NONE = 2 };
+
+typedef struct {
+    int kind;          /* MFT_POT_* */
+    uint32_t maskI;    /* species bitmask of group I (ignored for charges) */
+    uint32_t maskJ;    /* species bitmask of group J */
+    float A;           /* amplitude; l_B for charges */
+    float sigma;       /* Gaussian width (ps_potentialGaussian.cu:21-23) */
+    float Rp;          /* erf radius (ps_potentialErf2.cu:91-92) */
+    int ramp;          /* PS_Potential::ramp_check_input (ps_potential.cu:253-283) */
+    float A_final;
+} mft_tild_potential;
+
+/* Everything PS_Box::readInput/finishInitialization (PS_Box_initialization.cu:
+ * 32-569) derives for the hot loop.  Host pointers; copied at create. */
+typedef struct {
+    int nx, ny, nz;
+    float lx, ly, lz;
+    int pmeorder;            /* 0..4 (p = 4 reproduces ps_grid_utils.cu:57) */
+    int nparticles;
+    int nspecies;            /* <= 16 */
+    int maxbonds;            /* MAXBONDS (PS_Box_initialization.cu:40) */
+    int nbondtypes;
+    const float* bond_k;     /* [nbondtypes] */
+    const float* bond_r0;
+    const int* bond_style;   /* 0 harmonic, 1 FENE */
+    int npotentials;         /* <= 32 */
+    const mft_tild_potential* potentials;
+    int integrator;          /* MFT_INT_* on group "all" */
+    float delt;
+    const float* mass;       /* [nspecies] (ps_species.cu:13-38) */
+    const float* mobility;
+    uint64_t seed;
+    int has_charges;
+} mft_tild_desc;
+
+typedef struct mft_tild mft_tild;
+
+/* Engine with device-resident state.  create copies the description; the
+ * state arrays below are host pointers copied in (set_*) or out (get_*). */
+int mft_tild_create(mft_tild** eng, const mft_tild_desc* desc);
+int mft_tild_destroy(mft_tild* eng);
+void* mft_tild_stream(mft_tild* eng);
+/* Topology (PS_Box::makeLinear layout, PS_Box_moleculeMakers.cu:15-311):
+ * type[N], charge[N] (may be NULL), nBonds[N], bondedTo/bondType[N*maxbonds]. */
+int mft_tild_set_topology(mft_tild* eng, const int* type, const float* charge,
+                          const int* nBonds, const int* bondedTo, const int* bondType);
+/* Positions, previous positions and previous noise (GJF state,
+ * ps_integratorGJF.cu:43-51); xo/noise_old may be NULL (x_old = x, 0). */
+int mft_tild_set_state(mft_tild* eng, const float* x, const float* xo, const float* noise_old);
+int mft_tild_get_state(mft_tild* eng, float* x, float* xo, float* noise_old);
+/* Forces at the current positions (PS_Box.cu:96-128 + forces() :240-278)
+ * into f[3N] (host). */
+int mft_tild_compute_forces(mft_tild* eng, float* f);
+/* Species densities rho[nspecies*M] and charge density rhoq[M] (may be NULL)
+ * from the last force evaluation (group densities are sums over species). */
+int mft_tild_get_densities(mft_tild* eng, float* rho, float* rhoq);
+/* Total grid force field of species s, AoS [M*3] (the group's d_gridForce,
+ * ps_groups.cu:164-178); s = nspecies gives the charge field before x q. */
+int mft_tild_get_grid_force(mft_tild* eng, int s, float* F);
+/* nsteps of PS_Box::doTimeStep (PS_Box.cu:73-207) without I/O; step counters
+ * advance the RNG and potential ramps. Asynchronous on the engine stream. */
+int mft_tild_step(mft_tild* eng, int nsteps);
+int mft_tild_synchronize(mft_tild* eng);
+/* Energies of computeThermoProps (PS_Box.cu:282-317): per-potential
+ * energies [npotentials] (charges include the 1/2, ps_potentialCharges.cu:115)
+ * and bond energy (x 1/2, PS_Box.cu:292, with the reference's 0.5*Lh minimum
+ * image of ps_deviceBonds.cu:132-133). */
+int mft_tild_energies(mft_tild* eng, double* pot_energy, double* bond_energy);
+/* Per-kernel timing over the next step() calls: kernel ids below, average ms
+ * per launch and launch counts since enable. */
+enum {
+    MFT_TK_BIN = 0, MFT_TK_SCAN, MFT_TK_SCATTER, MFT_TK_DENSITY, MFT_TK_FFT_X_FWD,
+    MFT_TK_FFT_Y_FWD, MFT_TK_CONV_Z, MFT_TK_FFT_Y_INV, MFT_TK_FFT_X_INV, MFT_TK_GATHER,
+    MFT_TK_INTEGRATE, MFT_TK_COUNT
+};
+int mft_tild_timing(mft_tild* eng, int enable, double* avg_ms, int64_t* launches);
+/* Algorithmic HBM bytes per launch of each kernel id at this configuration. */
+int mft_tild_kernel_bytes(mft_tild* eng, double* bytes);
+int mft_tild_num_outputs(mft_tild* eng, int* nout);
+
+/* =============================================================== FTS ==== */
+/* Per-kernel entry points (complex interleaved float fields [M]). */
+
+/* LinearMolec bond factor exp(-k^2/(Nr-1)) (fts_molecule_linear.cu:109-122),
+ * as complex values like the reference's d_bond_fft. */
+int mft_fts_bond_factor(float* g2, const int Nx[3], const float L[3], double Nr, void* stream);
+
+enum { MFT_FTS_HELFAND = 0, MFT_FTS_FLORY = 1, MFT_FTS_INCOMPRESS = 2 };
+
+typedef struct {
+    int kind;             /* MFT_FTS_* (fts_potential.cu:25-55) */
+    int I, J;             /* species indices (Flory) */
+    double chiN;          /* Flory chi N (fts_potential_flory.cu:49) */
+    double kappaN;        /* Helfand kappa N (fts_potential_helfand.cu:26) */
+    double delt_pl;       /* time step of w+ (delt for Helfand/Incompress) */
+    double delt_mi;       /* time step of w- (Flory) */
+} mft_fts_potential;
+
+typedef struct {
+    int nblocks;          /* <= 16 (fts_molecule_linear.cu:26-47) */
+    int N[16];            /* beads per block */
+    int species[16];      /* species index per block */
+    double phi;           /* volume fraction (canonical); nmolecs derived as in
+                             fts_molecule_linear.cu:433-435 */
+} mft_fts_molecule;
+
+typedef struct {
+    int nx, ny, nz;
+    float lx, ly, lz;     /* box lengths in Rg */
+    double Nr;            /* reference chain length (no default in the reference) */
+    double C;             /* dimensionless concentration (FTS_Box.cu:486-497) */
+    int nspecies;         /* <= 8 */
+    int npotentials;      /* <= 8 */
+    mft_fts_potential potentials[8];
+    int nmolecules;       /* <= 4 */
+    mft_fts_molecule molecules[4];
+    int cl;               /* 1: complex Langevin (noise), 0: SCFT (no noise) */
+    uint64_t seed;        /* noise key (plus the request id in batches) */
+} mft_fts_desc;
+
+typedef struct mft_fts mft_fts;
+
+/* One CL-FTS system (a batch of one).  Fields are complex float host arrays
+ * of M values: wpl/wmi per potential (wmi only for Flory). */
+int mft_fts_create(mft_fts** sys, const mft_fts_desc* desc);
+int mft_fts_destroy(mft_fts* sys);
+void* mft_fts_stream(mft_fts* sys);
+int mft_fts_set_fields(mft_fts* sys, int pot, const float* wpl, const float* wmi);
+int mft_fts_get_fields(mft_fts* sys, int pot, float* wpl, float* wmi);
+/* FTS_Box::finishInitialization density pass (FTS_Box.cu:193-256). */
+int mft_fts_init(mft_fts* sys);
+/* niter CL iterations of FTS_Box::doTimeStep (FTS_Box.cu:19-47). */
+int mft_fts_step(mft_fts* sys, int niter);
+int mft_fts_synchronize(mft_fts* sys);
+/* Species densities rho[nspecies*M] (complex) after the last iteration. */
+int mft_fts_get_densities(mft_fts* sys, float* rho);
+/* Species fields W[nspecies*M] (complex, fts_species.cu:53-132). */
+int mft_fts_get_species_fields(mft_fts* sys, float* W);
+/* Propagators of molecule m: q[Ntot*M], qdag[Ntot*M] in the reference's
+ * q-form (fts_molecule_linear.cu:144-269), reconstructed from the stored
+ * symmetric form; either may be NULL. */
+int mft_fts_get_propagators(mft_fts* sys, int m, float* q, float* qdag);
+/* Q[m] (complex), H (complex) and its parts: potentials then molecules
+ * (FTS_Box::computeHamiltonian, FTS_Box.cu:590-602), FP64 reductions. */
+int mft_fts_observables(mft_fts* sys, double* Q, double* H, double* parts);
+
+/* Batched CL-FTS: many independent replicas of uneven sizes advanced by the
+ * same launches.  Replica r uses descs[r] (seed keyed by replica id). */
+typedef struct mft_fts_batch mft_fts_batch;
+int mft_fts_batch_create(mft_fts_batch** b, const mft_fts_desc* descs, int nrep);
+int mft_fts_batch_destroy(mft_fts_batch* b);
+int mft_fts_batch_set_fields(mft_fts_batch* b, int rep, int pot, const float* wpl,
+                             const float* wmi);
+int mft_fts_batch_init(mft_fts_batch* b);
+int mft_fts_batch_step(mft_fts_batch* b, int niter);
+int mft_fts_batch_synchronize(mft_fts_batch* b);
+int mft_fts_batch_observables(mft_fts_batch* b, int rep, double* Q, double* H, double* parts);
+int mft_fts_batch_get_densities(mft_fts_batch* b, int rep, float* rho);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,137 @@
+"""Synthetic TILD and CL-FTS workloads with the shapes of the benchmark configs.
+
+Particle topology follows PS_Box::makeLinear (PS_Box_moleculeMakers.cu:15-311):
+beads are numbered molecule-major then block-major; bead i bonds first to i-1
+(unless it starts a chain) then to i+1 (unless it ends one), with bond type
+blockBondType-1 of its own block.  Coordinates are a synthetic input: chain
+heads uniform in the box, then a random walk with N(0, 1/3) steps per axis,
+wrapped into [0, L).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def make_linear(nmolecs, blocks, block_charge=None, block_bond_type=None, maxbonds=2, start=0):
+    """blocks = [(N_b, type_b)]; returns type, charge, nBonds, bondedTo, bondType,
+    mID arrays for nmolecs new molecules whose first bead has index `start`."""
+    Ns = np.array([n for n, _ in blocks], np.int64)
+    Ntot = int(Ns.sum())
+    nb = len(blocks)
+    if block_bond_type is None:
+        block_bond_type = [1] * nb
+    block_of_bead = np.repeat(np.arange(nb), Ns)
+    pos_in_chain = np.arange(Ntot)
+    ns = nmolecs * Ntot
+    bead = np.tile(pos_in_chain, nmolecs)
+    blk = np.tile(block_of_bead, nmolecs)
+    gidx = start + np.arange(ns, dtype=np.int64)
+    types = np.array([t for _, t in blocks], np.int32)[blk]
+    charge = np.zeros(ns, np.float32)
+    if block_charge is not None:
+        charge = np.asarray(block_charge, np.float32)[blk]
+    btype = np.asarray(block_bond_type, np.int32)[blk] - 1
+    has_prev = bead > 0
+    has_next = bead < Ntot - 1
+    nBonds = has_prev.astype(np.int32) + has_next.astype(np.int32)
+    bondedTo = np.zeros((ns, maxbonds), np.int32)
+    bondType = np.zeros((ns, maxbonds), np.int32)
+    # slot 0 holds the previous bead when there is one, else the next bead
+    bondedTo[:, 0] = np.where(has_prev, gidx - 1, np.where(has_next, gidx + 1, 0))
+    bondType[:, 0] = np.where(has_prev | has_next, btype, 0)
+    both = has_prev & has_next
+    bondedTo[:, 1] = np.where(both, gidx + 1, 0)
+    bondType[:, 1] = np.where(both, btype, 0)
+    mID = np.repeat(np.arange(nmolecs, dtype=np.int32), Ntot)
+    return dict(type=types, charge=charge, nBonds=nBonds, bondedTo=bondedTo.reshape(-1),
+                bondType=bondType.reshape(-1), mID=mID)
+
+
+def concat_topologies(parts):
+    out = {}
+    for k in ("type", "charge", "nBonds", "bondedTo", "bondType"):
+        out[k] = np.concatenate([p[k] for p in parts])
+    return out
+
+
+def random_walk(nmolecs, Ntot, L, rng):
+    L = np.asarray(L, np.float64)
+    heads = rng.random((nmolecs, 1, 3)) * L
+    steps = rng.normal(0.0, np.sqrt(1.0 / 3.0), (nmolecs, Ntot, 3))
+    steps[:, 0, :] = 0.0
+    x = heads + np.cumsum(steps, axis=1)
+    x = np.mod(x, L)
+    x = x.astype(np.float32).reshape(-1, 3)
+    # float rounding of mod can land exactly on L; the reference keeps [0, L]
+    return x
+
+
+def gaussian_sigma(a):
+    """sigma of the gaussian potential for per-particle smearing a (sigma^2 = 2a^2)."""
+    return float(np.sqrt(2.0) * a)
+
+
+def tild_diblock(nchains=2000, NA=10, NB=10, box=20.0, grid=32, pmeorder=3, kappa=50.0, chi=1.0, a=0.5,
+                 bond_k=1.5, delt=0.005, seed=1234):
+    """Plain TILD AB diblock melt (config 1 shape by default)."""
+    rng = np.random.default_rng(seed)
+    topo = make_linear(nchains, [(NA, 0), (NB, 1)])
+    N = topo["type"].size
+    L = (box, box, box)
+    rho0 = N / box ** 3
+    sig = gaussian_sigma(a)
+    pots = [
+        dict(kind="gaussian", maskI=1, maskJ=1, A=kappa / (2 * rho0), sigma=sig),
+        dict(kind="gaussian", maskI=2, maskJ=2, A=kappa / (2 * rho0), sigma=sig),
+        dict(kind="gaussian", maskI=1, maskJ=2, A=(kappa + chi) / rho0, sigma=sig),
+    ]
+    x = random_walk(nchains, NA + NB, L, rng).reshape(-1)
+    return dict(name="tild_diblock", Nx=(grid, grid, grid), L=L, pmeorder=pmeorder, species=["A", "B"],
+                topo=topo, x=x, potentials=pots, bond_k=[bond_k], bond_r0=[0.0], bond_style=[0],
+                delt=delt, mass=[1.0, 1.0], mobility=[1.0, 1.0], has_charges=False, seed=seed)
+
+
+def tild_coacervate(nchains=40000, Nchain=20, nsolvent=200000, box=60.0, grid=128, pmeorder=3, kappa=50.0,
+                    chi_sp=0.5, lB=2.0, charge_smear=0.5, a=0.5, bond_k=1.5, delt=0.005, seed=1234):
+    """Large TILD polyelectrolyte coacervate (config 2 shape by default):
+    nchains polycations + nchains polyanions (charge +-1 per bead) + solvent."""
+    rng = np.random.default_rng(seed)
+    t_pos = make_linear(nchains, [(Nchain, 0)], block_charge=[1.0])
+    t_neg = make_linear(nchains, [(Nchain, 1)], block_charge=[-1.0], start=nchains * Nchain)
+    t_sol = make_linear(nsolvent, [(1, 2)], start=2 * nchains * Nchain)
+    topo = concat_topologies([t_pos, t_neg, t_sol])
+    N = topo["type"].size
+    L = (box, box, box)
+    rho0 = N / box ** 3
+    sig = gaussian_sigma(a)
+    A_self = kappa / (2 * rho0)
+    pots = [
+        dict(kind="gaussian", maskI=1, maskJ=1, A=A_self, sigma=sig),
+        dict(kind="gaussian", maskI=2, maskJ=2, A=A_self, sigma=sig),
+        dict(kind="gaussian", maskI=4, maskJ=4, A=A_self, sigma=sig),
+        dict(kind="gaussian", maskI=1, maskJ=2, A=kappa / rho0, sigma=sig),
+        dict(kind="gaussian", maskI=1, maskJ=4, A=(kappa + chi_sp) / rho0, sigma=sig),
+        dict(kind="gaussian", maskI=2, maskJ=4, A=(kappa + chi_sp) / rho0, sigma=sig),
+        dict(kind="charges", A=lB, sigma=charge_smear),
+    ]
+    xp = random_walk(nchains, Nchain, L, rng)
+    xn = random_walk(nchains, Nchain, L, rng)
+    xs = (rng.random((nsolvent, 3)) * np.asarray(L)).astype(np.float32)
+    x = np.concatenate([xp, xn, xs]).reshape(-1)
+    return dict(name="tild_coacervate", Nx=(grid, grid, grid), L=L, pmeorder=pmeorder, species=["P", "N", "S"],
+                topo=topo, x=x, potentials=pots, bond_k=[bond_k], bond_r0=[0.0], bond_style=[0],
+                delt=delt, mass=[1.0] * 3, mobility=[1.0] * 3, has_charges=True, seed=seed)
+
+
+def tild_engine(cfg, integrator="GJF"):
+    """Build a TildEngine for a config dict from this module."""
+    from . import TildEngine
+    topo = cfg["topo"]
+    eng = TildEngine(cfg["Nx"], cfg["L"], cfg["pmeorder"], len(cfg["species"]), topo["type"].size,
+                     cfg["potentials"], cfg["bond_k"], cfg["bond_r0"], cfg["bond_style"], integrator=integrator,
+                     delt=cfg["delt"], mass=cfg["mass"], mobility=cfg["mobility"], seed=cfg["seed"],
+                     has_charges=cfg["has_charges"])
+    eng.set_topology(topo["type"], topo["charge"] if cfg["has_charges"] else None, topo["nBonds"],
+                     topo["bondedTo"], topo["bondType"])
+    eng.set_state(cfg["x"])
+    return eng

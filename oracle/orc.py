@@ -1,0 +1,639 @@
+"""CPU oracle for the MATILDA.FT hot paths (TEST INFRASTRUCTURE ONLY).
+
+Only the test suite, the smoke check and the benchmark's CPU-baseline leg may
+import this module; it is the checker, never the thing measured or shipped.
+
+The C primitives in this folder restate the reference kernels; this module
+binds them with ctypes and restates the reference's per-step orchestration:
+
+* TILD step: PS_Box::doTimeStep (PS_Box.cu:73-207) and PS_Box::forces
+  (PS_Box.cu:240-278): grid weights -> per-group densities -> zero forces ->
+  bonds -> every potential's CalcForces into group grid forces -> mapForces per
+  group -> integrator.
+* CL-FTS iteration: FTS_Box::doTimeStep (FTS_Box.cu:19-123): every potential's
+  updateFields (from the previous densities) -> species fields
+  (fts_species.cu:53-132) -> molecule densities (fts_molecule_linear.cu:278-363).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBS = {}
+
+_i = C.c_int
+_d = C.c_double
+_p = C.c_void_p
+
+
+def lib(bits: int = 64):
+    """Load liboracle64.so (FP64 oracle) or liboracle32.so (FP32 host build)."""
+    if bits in _LIBS:
+        return _LIBS[bits]
+    path = os.path.join(_HERE, f"liboracle{bits}.so")
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} missing: build the oracle (make -C oracle)")
+    L = C.CDLL(path)
+    sig = {
+        "orc_philox4x32_10": (None, [_p, _p, _p]),
+        "orc_philox_normals": (None, [_p, C.c_long, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
+        "orc_philox_normal4": (None, [_p, _p, _p]),
+        "orc_fft1d": (None, [_p, _i, _i, _i]),
+        "orc_fft3d": (None, [_p, _i, _i, _i, _i]),
+        "orc_box_fft": (None, [_p, _p, _i, _i]),
+        "orc_kvec_f": (C.c_float, [_i, _p, _p, _i, _p]),
+        "orc_kvec_d": (C.c_double, [_i, _p, _p, _i, _p]),
+        "orc_spline_weights": (None, [_d if bits == 64 else C.c_float, _i, _p]),
+        "orc_grid_weights": (None, [_p, _i, _p, _p, _i, _i, _p, _p]),
+        "orc_fill_density": (None, [_p, _p, _p, _p, _i, _p, _p, _i]),
+        "orc_uk_table": (None, [_i, _d, _d, _d, _p, _p, _i, _p]),
+        "orc_potential_grid_force": (None, [_p, _p, _p, _p, _i, _d, _p]),
+        "orc_potential_energy": (C.c_double, [_p, _p, _p, _p, _i, _d]),
+        "orc_map_forces": (None, [_p, _p, _p, _p, _i, _p, _p, _i, _d, _i]),
+        "orc_bonds": (None, [_p] * 10 + [_i, _i, _i]),
+        "orc_bond_energy": (None, [_p] * 10 + [_i, _i, _i]),
+        "orc_gjf_coeffs": (None, [_d, _d, _d, _p]),
+        "orc_gjf_step": (None, [_p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p]),
+        "orc_em_step": (None, [_p, _p, _d, _p, _p, _i, _i, _p]),
+        "orc_fts_bond_fft": (None, [_p, _p, _i, _d, _p]),
+        "orc_fts_flory_force": (None, [_p] * 6 + [_d, _d, _d, _i]),
+        "orc_fts_helfand_force": (None, [_p, _p, _p, _d, _d, _d, _i]),
+        "orc_fts_incompress_force": (None, [_p, _p, _p, _d, _d, _i]),
+        "orc_fts_update_em": (None, [_p, _p, _p, _d, _i]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIBS[bits] = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _rdt(bits):
+    return np.float64 if bits == 64 else np.float32
+
+
+def _cdt(bits):
+    return np.complex128 if bits == 64 else np.complex64
+
+
+def _ia(v):
+    return np.ascontiguousarray(np.asarray(v, dtype=np.int32))
+
+
+def _fa(v):
+    return np.ascontiguousarray(np.asarray(v, dtype=np.float32))
+
+
+# ------------------------------------------------------------------ RNG ----
+def philox(ctr, key, bits=64):
+    out = np.zeros(4, np.uint32)
+    lib(bits).orc_philox4x32_10(_ptr(np.asarray(ctr, np.uint32)), _ptr(np.asarray(key, np.uint32)), _ptr(out))
+    return out
+
+
+def normals(n, seed, c1, c2, c3, bits=64):
+    out = np.zeros(n, np.float64)
+    lib(bits).orc_philox_normals(_ptr(out), n, seed, c1, c2, c3)
+    return out
+
+
+# ------------------------------------------------------------------ FFT ----
+def fft3d(a, sign, bits=64):
+    """Unnormalised transform of an x-fastest grid array shaped (nz, ny, nx)."""
+    a = np.ascontiguousarray(a, dtype=_cdt(bits)).copy()
+    nz, ny, nx = a.shape
+    lib(bits).orc_fft3d(_ptr(a), nx, ny, nz, sign)
+    return a
+
+
+def box_fft(a, Nx, direction, bits=64):
+    a = np.ascontiguousarray(a, dtype=_cdt(bits)).copy()
+    lib(bits).orc_box_fft(_ptr(a), _ptr(_ia(Nx)), len(Nx), direction)
+    return a
+
+
+def kvec_f(i, Nx, L):
+    k = np.zeros(3, np.float32)
+    k2 = lib().orc_kvec_f(i, _ptr(_ia(list(Nx) + [1] * (3 - len(Nx)))), _ptr(_fa(list(L) + [1] * (3 - len(L)))), len(Nx), _ptr(k))
+    return k2, k
+
+
+def kvec_d(i, Nx, L):
+    k = np.zeros(3, np.float64)
+    k2 = lib().orc_kvec_d(i, _ptr(_ia(list(Nx) + [1] * (3 - len(Nx)))), _ptr(_fa(list(L) + [1] * (3 - len(L)))), len(Nx), _ptr(k))
+    return k2, k
+
+
+# ----------------------------------------------------------------- TILD ----
+def spline_weights(s, p, bits=64):
+    W = np.zeros(6, _rdt(bits))
+    lib(bits).orc_spline_weights(s, p, _ptr(W))
+    return W[: p + 1]
+
+
+def grid_weights(x, Nx, dx, p, bits=64):
+    x = _fa(x).reshape(-1)
+    Dim = len(Nx)
+    ns = x.size // Dim
+    G = (p + 1) ** Dim
+    inds = np.zeros(ns * G, np.int32)
+    W = np.zeros(ns * G, _rdt(bits))
+    lib(bits).orc_grid_weights(_ptr(x), ns, _ptr(_ia(list(Nx) + [1])), _ptr(_fa(list(dx) + [1])), p, Dim, _ptr(inds), _ptr(W))
+    return inds, W
+
+
+def fill_density(M, sites, inds, W, G, q=None, bits=64):
+    rho = np.zeros(M, _rdt(bits))
+    rhoq = np.zeros(M, _rdt(bits)) if q is not None else None
+    sites = _ia(sites)
+    lib(bits).orc_fill_density(_ptr(rho), _ptr(rhoq), _ptr(_fa(q) if q is not None else None), _ptr(sites), sites.size, _ptr(inds), _ptr(W), G)
+    return rho, rhoq
+
+
+POT_KIND = {"gaussian": 0, "erf2": 1, "erfG": 2, "charges": 3}
+
+
+def uk_table(kind, A, sigma, Rp, Nx, L, bits=64):
+    M = int(np.prod(Nx))
+    uk = np.zeros(M, _rdt(bits))
+    lib(bits).orc_uk_table(POT_KIND[kind], A, sigma, Rp, _ptr(_ia(list(Nx) + [1])), _ptr(_fa(list(L) + [1])), len(Nx), _ptr(uk))
+    return uk
+
+
+def potential_grid_force(rhoJ, uk, Nx, L, factor, gridF, bits=64):
+    lib(bits).orc_potential_grid_force(_ptr(rhoJ), _ptr(uk), _ptr(_ia(list(Nx) + [1])), _ptr(_fa(list(L) + [1])), len(Nx), factor, _ptr(gridF))
+
+
+def potential_energy(rhoI, rhoJ, uk, Nx, gvol, bits=64):
+    return lib(bits).orc_potential_energy(_ptr(rhoI), _ptr(rhoJ), _ptr(uk), _ptr(_ia(list(Nx) + [1])), len(Nx), gvol)
+
+
+def map_forces(f, gridF, sites, inds, W, G, gvol, Dim, q=None, bits=64):
+    sites = _ia(sites)
+    lib(bits).orc_map_forces(_ptr(f), _ptr(gridF), _ptr(_fa(q) if q is not None else None), _ptr(sites), sites.size, _ptr(inds), _ptr(W), G, gvol, Dim)
+
+
+def bonds(f, topo, x, L, bits=64):
+    Lf = _fa(L)
+    Lh = _fa(0.5 * Lf)
+    ns = topo["nBonds"].size
+    lib(bits).orc_bonds(_ptr(f), _ptr(topo["nBonds"]), _ptr(topo["bondedTo"]), _ptr(topo["bondType"]),
+                        _ptr(_fa(topo["bondReq"])), _ptr(_fa(topo["bondK"])), _ptr(_ia(topo["bondStyle"])),
+                        _ptr(_fa(x)), _ptr(Lf), _ptr(Lh), ns, topo["maxbonds"], len(L))
+
+
+def bond_energy(topo, x, L, bits=64):
+    Lf = _fa(L)
+    Lh = _fa(0.5 * Lf)
+    ns = topo["nBonds"].size
+    e = np.zeros(ns, _rdt(bits))
+    lib(bits).orc_bond_energy(_ptr(e), _ptr(topo["nBonds"]), _ptr(topo["bondedTo"]), _ptr(topo["bondType"]),
+                              _ptr(_fa(topo["bondReq"])), _ptr(_fa(topo["bondK"])), _ptr(_ia(topo["bondStyle"])),
+                              _ptr(_fa(x)), _ptr(Lf), _ptr(Lh), ns, topo["maxbonds"], len(L))
+    return e
+
+
+def gjf_coeffs(delt, mass, mobility):
+    out = np.zeros(5, np.float64)
+    lib().orc_gjf_coeffs(delt, mass, mobility, _ptr(out))
+    return out
+
+
+def make_linear(nmolecs, blocks, species_index, block_bond_type=None, charges=None, maxbonds=2, start=0):
+    """Topology of PS_Box::makeLinear (PS_Box_moleculeMakers.cu:15-311).
+
+    blocks = [(N_b, species_name)], species_index maps names to integer types.
+    Returns intSpecies, nBonds, bondedTo, bondType, charge, mID arrays for the new
+    molecules; bead order is molecule-major then block-major.
+    """
+    nb = len(blocks)
+    if block_bond_type is None:
+        block_bond_type = [1] * nb
+    Ntot = sum(n for n, _ in blocks)
+    ns = nmolecs * Ntot
+    intSpecies = np.zeros(ns, np.int32)
+    nBonds = np.zeros(ns, np.int32)
+    bondedTo = np.zeros(ns * maxbonds, np.int32)
+    bondType = np.zeros(ns * maxbonds, np.int32)
+    q = np.zeros(ns, np.float32)
+    mID = np.zeros(ns, np.int32)
+    ind = 0
+    for m in range(nmolecs):
+        for j, (Nb, sp) in enumerate(blocks):
+            for s in range(Nb):
+                gi = start + ind
+                intSpecies[ind] = species_index[sp]
+                if charges is not None:
+                    q[ind] = charges[j]
+                if j != 0 or s != 0:
+                    bondedTo[ind * maxbonds + nBonds[ind]] = gi - 1
+                    bondType[ind * maxbonds + nBonds[ind]] = block_bond_type[j] - 1
+                    nBonds[ind] += 1
+                if j != nb - 1 or s != Nb - 1:
+                    bondedTo[ind * maxbonds + nBonds[ind]] = gi + 1
+                    bondType[ind * maxbonds + nBonds[ind]] = block_bond_type[j] - 1
+                    nBonds[ind] += 1
+                mID[ind] = m
+                ind += 1
+    return dict(intSpecies=intSpecies, nBonds=nBonds, bondedTo=bondedTo, bondType=bondType, charge=q, mID=mID)
+
+
+@dataclass
+class TildPotential:
+    kind: str            # gaussian | erf2 | erfG | charges
+    I: str = "charges"   # group names (species names, "all", or custom groups)
+    J: str = "charges"
+    A: float = 0.0       # amplitude (l_B for charges)
+    sigma: float = 0.0
+    Rp: float = 0.0
+
+
+@dataclass
+class TildSystem:
+    """Reference PS_Box state needed by one time step (FP32 positions)."""
+    Nx: tuple
+    L: tuple
+    pmeorder: int
+    species: list        # names, index = integer type
+    x: np.ndarray        # [ns*Dim] float32
+    topo: dict           # make_linear output merged over molecules
+    bond_k: list
+    bond_r0: list
+    bond_style: list
+    potentials: list
+    mass: list = None
+    mobility: list = None
+    delt: float = 0.005
+    integrator: str = "GJF"
+    groups: dict = field(default_factory=dict)   # extra named groups -> list of species names
+    xo: np.ndarray = None
+    noise_old: np.ndarray = None
+
+    @property
+    def Dim(self):
+        return len(self.Nx)
+
+    @property
+    def M(self):
+        return int(np.prod(self.Nx))
+
+    @property
+    def ns(self):
+        return self.topo["intSpecies"].size
+
+    def dxf(self):
+        # PS_Box_initialization.cu:378 (double) then :727-731 (cast to float)
+        return np.array([np.float32(float(np.float32(l)) / n) for l, n in zip(self.L, self.Nx)], np.float32)
+
+    def gvol(self):
+        # the device kernels use the float product of the float spacings
+        g = np.float32(1.0)
+        for d in self.dxf():
+            g = np.float32(g * d)
+        return float(g)
+
+    def group_sites(self, name):
+        t = self.topo["intSpecies"]
+        if name in ("all", "All"):
+            return np.arange(self.ns, dtype=np.int32)
+        if name == "charges":
+            return np.nonzero(self.topo["charge"] != 0.0)[0].astype(np.int32)
+        if name in self.species:
+            return np.nonzero(t == self.species.index(name))[0].astype(np.int32)
+        if name in self.groups:
+            ids = [self.species.index(s) for s in self.groups[name]]
+            return np.nonzero(np.isin(t, ids))[0].astype(np.int32)
+        raise KeyError(name)
+
+
+def tild_forces(sys: TildSystem, bits=64, want_energy=False):
+    """Particle forces at the current positions (PS_Box.cu:96-128, 240-278)."""
+    Dim, M = sys.Dim, sys.M
+    dx = sys.dxf()
+    G = (sys.pmeorder + 1) ** Dim
+    inds, W = grid_weights(sys.x, sys.Nx, dx, sys.pmeorder, bits)
+    gvol_dev = sys.gvol()
+    names = ["all"] + list(sys.species) + list(sys.groups)
+    has_q = bool(np.any(sys.topo["charge"] != 0.0))
+    if has_q:
+        names.append("charges")
+    rho, rhoq, gridF = {}, None, {}
+    for g in names:
+        sites = sys.group_sites(g)
+        if g == "charges":
+            rho[g], rhoq = fill_density(M, sites, inds, W, G, q=sys.topo["charge"], bits=bits)
+        else:
+            rho[g], _ = fill_density(M, sites, inds, W, G, bits=bits)
+    f = np.zeros(sys.ns * Dim, _rdt(bits))
+    bonds(f, dict(sys.topo, bondReq=sys.bond_r0, bondK=sys.bond_k, bondStyle=sys.bond_style, maxbonds=2), sys.x, sys.L, bits)
+    energies = []
+    for pot in sys.potentials:
+        uk = uk_table(pot.kind, pot.A, pot.sigma, pot.Rp, sys.Nx, sys.L, bits)
+        if pot.kind == "charges":
+            gf = gridF.setdefault("charges", np.zeros(M * Dim, _rdt(bits)))
+            potential_grid_force(rhoq, uk, sys.Nx, sys.L, 1.0, gf, bits)
+            if want_energy:
+                energies.append(0.5 * potential_energy(rhoq, rhoq, uk, sys.Nx, gvol_dev, bits))
+            continue
+        gI = gridF.setdefault(pot.I, np.zeros(M * Dim, _rdt(bits)))
+        potential_grid_force(rho[pot.J], uk, sys.Nx, sys.L, 2.0 if pot.I == pot.J else 1.0, gI, bits)
+        if pot.I != pot.J:
+            gJ = gridF.setdefault(pot.J, np.zeros(M * Dim, _rdt(bits)))
+            potential_grid_force(rho[pot.I], uk, sys.Nx, sys.L, 1.0, gJ, bits)
+        if want_energy:
+            energies.append(potential_energy(rho[pot.I], rho[pot.J], uk, sys.Nx, gvol_dev, bits))
+    for g, gf in gridF.items():
+        q = sys.topo["charge"] if g == "charges" else None
+        map_forces(f, gf, sys.group_sites(g), inds, W, G, gvol_dev, Dim, q=q, bits=bits)
+    out = dict(f=f, inds=inds, W=W, rho=rho, rhoq=rhoq, gridF=gridF)
+    if want_energy:
+        out["energies"] = energies
+    return out
+
+
+def philox_np(c0, c1, c2, c3, seed):
+    """Vectorised Philox4x32-10 over counter arrays (same algorithm as
+    orc_philox4x32_10; cross-checked in the tests)."""
+    M0, M1 = np.uint64(0xD2511F53), np.uint64(0xCD9E8D57)
+    k0 = np.uint64(seed & 0xFFFFFFFF)
+    k1 = np.uint64((seed >> 32) & 0xFFFFFFFF)
+    mask = np.uint64(0xFFFFFFFF)
+    n = max(np.size(c0), np.size(c1), np.size(c2), np.size(c3))
+    c = [np.broadcast_to(np.asarray(v, np.uint64), (n,)).copy() for v in (c0, c1, c2, c3)]
+    for _ in range(10):
+        p0 = M0 * c[0]
+        p1 = M1 * c[2]
+        hi0, lo0 = p0 >> np.uint64(32), p0 & mask
+        hi1, lo1 = p1 >> np.uint64(32), p1 & mask
+        c = [(hi1 ^ c[1] ^ k0) & mask, lo1, (hi0 ^ c[3] ^ k1) & mask, lo0]
+        k0 = (k0 + np.uint64(0x9E3779B9)) & mask
+        k1 = (k1 + np.uint64(0xBB67AE85)) & mask
+    return [v.astype(np.uint32) for v in c]
+
+
+def normal4_np(c0, c1, c2, c3, seed):
+    """Box-Muller of philox_np output -> array [n, 4] of unit normals."""
+    r = philox_np(c0, c1, c2, c3, seed)
+    out = []
+    for h in range(2):
+        u1 = ((r[2 * h] >> np.uint32(8)).astype(np.float64) + 1.0) * 2.0 ** -24
+        u2 = (r[2 * h + 1] >> np.uint32(8)).astype(np.float64) * 2.0 ** -24
+        rad = np.sqrt(-2.0 * np.log(u1))
+        out += [rad * np.cos(2.0 * np.pi * u2), rad * np.sin(2.0 * np.pi * u2)]
+    return np.stack(out, axis=1)
+
+
+TAG_TILD = 0x54494C44  # 'TILD'
+TAG_FTS = 0x4654534E   # 'FTSN'
+
+
+def tild_noise(ns, seed, step, Dim=3):
+    """Unit normals for the integrator: counter (particle, step, 'TILD', 0),
+    normals 0..Dim-1 of each Philox block."""
+    z = normal4_np(np.arange(ns, dtype=np.uint64), step, TAG_TILD, 0, seed)
+    return np.ascontiguousarray(z[:, :Dim]).reshape(-1)
+
+
+def tild_step(sys: TildSystem, xi, bits=64):
+    """One PS_Box::doTimeStep with the GJF or EM integrator on group 'all'."""
+    out = tild_forces(sys, bits)
+    f = out["f"]
+    sites = np.arange(sys.ns, dtype=np.int32)
+    Lf = _fa(sys.L)
+    x = sys.x.astype(_rdt(bits))
+    if sys.integrator == "GJF":
+        coef = np.concatenate([gjf_coeffs(sys.delt, m, D) for m, D in zip(sys.mass, sys.mobility)])
+        xo = sys.xo.astype(_rdt(bits))
+        no = sys.noise_old.astype(_rdt(bits))
+        lib(bits).orc_gjf_step(_ptr(x), _ptr(xo), _ptr(no), _ptr(f), _ptr(coef), _ptr(sys.topo["intSpecies"]), _ptr(Lf), _ptr(sites), sys.ns, sys.Dim, _ptr(np.ascontiguousarray(xi, np.float64)))
+        out.update(x=x, xo=xo, noise_old=no)
+    else:
+        lib(bits).orc_em_step(_ptr(x), _ptr(f), sys.delt, _ptr(Lf), _ptr(sites), sys.ns, sys.Dim, _ptr(np.ascontiguousarray(xi, np.float64)))
+        out.update(x=x)
+    return out
+
+
+# ------------------------------------------------------------------ FTS ----
+def fts_bond_fft(Nx, L, Nr, bits=64):
+    g = np.zeros(int(np.prod(Nx)), _cdt(bits))
+    lib(bits).orc_fts_bond_fft(_ptr(_ia(list(Nx) + [1])), _ptr(_fa(list(L) + [1])), len(Nx), Nr, _ptr(g))
+    return g
+
+
+@dataclass
+class FtsPotential:
+    kind: str                 # Helfand | Flory | Incompress
+    I: int = -1               # species indices (Flory)
+    J: int = -1
+    chiN: float = 0.0
+    kappaN: float = 0.0
+    dt_pl: float = 0.0
+    dt_mi: float = 0.0
+    wpl: np.ndarray = None
+    wmi: np.ndarray = None
+
+
+@dataclass
+class FtsMolecule:
+    blocks: list              # [(N_b, species index)]
+    phi: float = -1.0
+    nmolecs: float = -1.0
+
+
+@dataclass
+class FtsSystem:
+    """Reference FTS_Box state for CL iterations (FP64 fields)."""
+    Nx: tuple
+    L: tuple
+    Nr: float
+    C: float
+    nspecies: int
+    potentials: list
+    molecules: list
+    rho: np.ndarray = None    # [nspecies, M] species densities (previous iteration)
+    W: np.ndarray = None      # [nspecies, M] species fields
+    Q: list = None
+
+    @property
+    def M(self):
+        return int(np.prod(self.Nx))
+
+    def V(self):
+        v = 1.0
+        for l in self.L:
+            v *= float(np.float32(l))
+        return v
+
+    def gvol(self):
+        g = 1.0
+        for l, n in zip(self.L, self.Nx):
+            g *= float(np.float32(l)) / float(n)
+        return g
+
+    def init_nmolecs(self):
+        # LinearMolec::computeLinearTerms (fts_molecule_linear.cu:433-435)
+        for m in self.molecules:
+            Ntot = sum(n for n, _ in m.blocks)
+            m.nmolecs = self.C * m.phi * self.V() * self.Nr / float(Ntot)
+
+
+def fts_build_species_fields(sys: FtsSystem):
+    """FTS_Species::buildPotentialField (fts_species.cu:53-132)."""
+    M = sys.M
+    W = np.zeros((sys.nspecies, M), np.complex128)
+    for s in range(sys.nspecies):
+        for p in sys.potentials:
+            if p.kind in ("Helfand", "Incompress"):
+                W[s] += 1j * p.wpl
+            elif p.kind == "Flory":
+                if s == p.I:
+                    W[s] -= p.wmi
+                    W[s] += 1j * p.wpl
+                if s == p.J:
+                    W[s] += p.wmi
+                    W[s] += 1j * p.wpl
+        W[s] *= 1.0 / sys.Nr
+    sys.W = W
+    return W
+
+
+def fts_molecule_densities(sys: FtsSystem, bits=64, keep=False):
+    """LinearMolec::calcDensity for every molecule (fts_molecule_linear.cu:278-363)."""
+    M = sys.M
+    g = fts_bond_fft(sys.Nx, sys.L, sys.Nr, bits)
+    rho = np.zeros((sys.nspecies, M), _cdt(bits))
+    Ws = np.ascontiguousarray(sys.W, _cdt(bits))
+    sys.Q = []
+    extra = []
+    L = lib(bits)
+    for m in sys.molecules:
+        nb = len(m.blocks)
+        N = _ia([n for n, _ in m.blocks])
+        spec = _ia([s for _, s in m.blocks])
+        Ntot = int(N.sum())
+        q = np.zeros(Ntot * M, _cdt(bits))
+        qd = np.zeros(Ntot * M, _cdt(bits))
+        rb = np.zeros(nb * M, _cdt(bits))
+        Qc = _propagators(L, Ws, nb, N, spec, g, sys.Nx, sys.gvol(), sys.V(), q, qd, bits)
+        factor = m.nmolecs / Qc / sys.V()
+        _density(L, Ws, nb, N, spec, q, qd, factor, M, rb, rho, bits)
+        sys.Q.append(Qc)
+        if keep:
+            extra.append(dict(q=q, qdag=qd, rhoBlock=rb))
+    sys.rho = rho.astype(np.complex128)
+    return extra
+
+
+class _c64(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+class _c32(C.Structure):
+    _fields_ = [("re", C.c_float), ("im", C.c_float)]
+
+
+def _propagators(L, W, nb, N, spec, g, Nx, gvol, V, q, qd, bits):
+    fn = L.orc_fts_propagators
+    fn.restype = _c64 if bits == 64 else _c32
+    fn.argtypes = [_p, _i, _p, _p, _p, _p, _i, _d, _d, _p, _p]
+    r = fn(_ptr(W), nb, _ptr(N), _ptr(spec), _ptr(g), _ptr(_ia(list(Nx) + [1])), len(Nx), gvol, V, _ptr(q), _ptr(qd))
+    return complex(r.re, r.im)
+
+
+def _density(L, W, nb, N, spec, q, qd, factor, M, rb, rho, bits):
+    fn = L.orc_fts_density
+    ct = _c64 if bits == 64 else _c32
+    fn.restype = None
+    fn.argtypes = [_p, _i, _p, _p, _p, _p, ct, _i, _p, _p]
+    fn(_ptr(W), nb, _ptr(N), _ptr(spec), _ptr(q), _ptr(qd), ct(factor.real, factor.imag), M, _ptr(rb), _ptr(rho))
+
+
+def fts_noise(M, seed, it, stream=0):
+    """CL noise normals: counter (site, iteration, 'FTSN', stream); normal j of
+    the block feeds the j-th noisy field in potential declaration order."""
+    return normal4_np(np.arange(M, dtype=np.uint64), it, TAG_FTS, stream, seed)
+
+
+def fts_update_fields(sys: FtsSystem, z=None):
+    """Every potential's updateFields with EM (+ real CL noise when z is given):
+    fts_potential_helfand.cu:77-152, fts_potential_flory.cu:123-167,
+    fts_potential_incompress.cu:84-113."""
+    M = sys.M
+    gvol = sys.gvol()
+    L = lib()
+    rho_tot = sys.rho.sum(axis=0)
+    col = 0
+    for p in sys.potentials:
+        if p.kind == "Helfand":
+            dH = np.zeros(M, np.complex128)
+            L.orc_fts_helfand_force(_ptr(dH), _ptr(p.wpl), _ptr(rho_tot), sys.C, p.kappaN, sys.Nr, M)
+            noise = None
+            if z is not None:
+                noise = (np.sqrt(2.0 * p.dt_pl / gvol) * z[:, col]).astype(np.complex128)
+                col += 1
+            L.orc_fts_update_em(_ptr(p.wpl), _ptr(dH), _ptr(noise), p.dt_pl, M)
+        elif p.kind == "Incompress":
+            dH = np.zeros(M, np.complex128)
+            L.orc_fts_incompress_force(_ptr(dH), _ptr(p.wpl), _ptr(rho_tot), sys.C, sys.Nr, M)
+            L.orc_fts_update_em(_ptr(p.wpl), _ptr(dH), None, p.dt_pl, M)
+        elif p.kind == "Flory":
+            dHp = np.zeros(M, np.complex128)
+            dHm = np.zeros(M, np.complex128)
+            rI = np.ascontiguousarray(sys.rho[p.I])
+            rJ = np.ascontiguousarray(sys.rho[p.J])
+            L.orc_fts_flory_force(_ptr(dHp), _ptr(dHm), _ptr(p.wpl), _ptr(p.wmi), _ptr(rI), _ptr(rJ), sys.C, p.chiN, sys.Nr, M)
+            npl = nmi = None
+            if z is not None:
+                npl = (np.sqrt(2.0 * p.dt_pl / gvol) * z[:, col]).astype(np.complex128)
+                nmi = (np.sqrt(2.0 * p.dt_mi / gvol) * z[:, col + 1]).astype(np.complex128)
+                col += 2
+            L.orc_fts_update_em(_ptr(p.wpl), _ptr(dHp), _ptr(npl), p.dt_pl, M)
+            L.orc_fts_update_em(_ptr(p.wmi), _ptr(dHm), _ptr(nmi), p.dt_mi, M)
+
+
+def fts_hamiltonian(sys: FtsSystem):
+    """FTS_Box::computeHamiltonian (FTS_Box.cu:590-602) with the potential terms
+    of fts_potential_flory.cu:252-274, fts_potential_helfand.cu:254-275,
+    fts_potential_incompress.cu:232-247 and LinearMolec::calcHTerm
+    (fts_molecule_linear.cu:368-382); also returns the parts."""
+    gvol = sys.gvol()
+    H = 0j
+    parts = []
+    for p in sys.potentials:
+        if p.kind == "Flory":
+            h = (np.sum(p.wpl * p.wpl) * gvol) * sys.C / p.chiN + (np.sum(p.wmi * p.wmi) * gvol) * sys.C / p.chiN
+        elif p.kind == "Helfand":
+            h = (np.sum(p.wpl * p.wpl) * gvol) * sys.C / 2.0 / p.kappaN - 1j * sys.C * (np.sum(p.wpl) * gvol)
+        else:
+            h = -1j * sys.C * (np.sum(p.wpl) * gvol)
+        parts.append(h)
+        H += h
+    for m, Q in zip(sys.molecules, sys.Q):
+        h = -m.nmolecs * np.log(Q)
+        parts.append(h)
+        H += h
+    return H, parts
+
+
+def fts_init(sys: FtsSystem, bits=64):
+    """FTS_Box::finishInitialization density pass (FTS_Box.cu:193-256)."""
+    sys.init_nmolecs()
+    fts_build_species_fields(sys)
+    fts_molecule_densities(sys, bits)
+
+
+def fts_step(sys: FtsSystem, z=None, bits=64):
+    """One CL iteration (FTS_Box.cu:19-47): update fields, rebuild species
+    fields, recompute densities."""
+    fts_update_fields(sys, z)
+    fts_build_species_fields(sys)
+    fts_molecule_densities(sys, bits)

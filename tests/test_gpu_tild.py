@@ -1,0 +1,260 @@
+"""GPU parity of the TILD path: the HIP engine (through the C ABI) against the
+FP64 CPU oracle on the same seeded inputs.
+
+Tolerances (TILD, FP32 on the device): grid indices and topology bit-exact;
+densities, grid forces and particle forces within 1e-5 normwise relative
+error, rel(a, b) = max|a - b| / max|b|; positions after a step within 1e-6 of
+the box length (periodic distance); energies within 1e-5 relative.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import orc
+from mft import systems
+from test_oracle_tild import oracle_system, small_system
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def rel(a, b):
+    """max|a - b| / max|b| over real or complex arrays."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    dt = np.complex128 if (np.iscomplexobj(a) or np.iscomplexobj(b)) else np.float64
+    a = a.astype(dt)
+    b = b.astype(dt)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def test_philox_bit_exact(gpu):
+    say("case: philox raw 4096 blocks")
+    got = gpu.philox_raw(4096, (7 << 32) | 12345, 3, orc.TAG_TILD, 9)
+    want = orc.philox_np(np.arange(4096, dtype=np.uint64), 3, orc.TAG_TILD, 9, (7 << 32) | 12345)
+    for j in range(4):
+        assert np.array_equal(got[:, j], want[j])
+
+
+def test_normals_match_oracle(gpu):
+    say("case: philox normals")
+    got = gpu.philox_normal(20000, 99, 5, orc.TAG_FTS, 0)
+    want = orc.normal4_np(np.arange(20000, dtype=np.uint64), 5, orc.TAG_FTS, 0, 99)
+    assert np.max(np.abs(got - want)) < 2e-5
+
+
+@pytest.mark.parametrize("shp", ["4x6x8", "8x12x16", "16x8x24"])
+def test_fft_golden(gpu, shp):
+    say("case: fft golden", shp)
+    G = np.load(os.path.join(os.path.dirname(__file__), "golden", "fft_golden.npz"))
+    x = G[f"x3_{shp}"]
+    M = x.size
+    fwd = gpu.fft3d(x, 1)  # forward scaled by 1/M
+    assert rel(fwd * M, G[f"fwd3_{shp}"]) < 2e-6
+    inv = gpu.fft3d(x, -1)
+    assert rel(inv, G[f"inv3_{shp}"]) < 2e-6
+
+
+@pytest.mark.parametrize("n", [12, 20, 24, 32, 40, 48, 64, 80, 96, 128, 160, 192, 256])
+def test_fft_sizes_against_oracle(gpu, n):
+    say("case: fft size", n)
+    rng = np.random.default_rng(n)
+    shp = (8, n, 8) if n > 64 else (n, 8, n)   # (nz, ny, nx)
+    x = rng.normal(size=shp) + 1j * rng.normal(size=shp)
+    want = orc.fft3d(x, -1) / x.size
+    got = gpu.fft3d(x, 1)
+    assert rel(got, want) < 2e-6
+    back = gpu.fft3d(got, -1)
+    assert rel(back, x) < 2e-6
+
+
+def test_fft_batched(gpu):
+    say("case: fft batch of 3 grids 24x32x16")
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=(3, 16, 32, 24)) + 1j * rng.normal(size=(3, 16, 32, 24))
+    got = gpu.fft3d(x, 1)
+    for b in range(3):
+        assert rel(got[b], orc.fft3d(x[b], -1) / x[b].size) < 2e-6
+
+
+def edge_positions(L, Nx, rng, n):
+    L = np.asarray(L, np.float32)
+    dx = (L / np.asarray(Nx, np.float32)).astype(np.float32)
+    x = (rng.random((n, 3)) * L).astype(np.float32)
+    x[0] = 0.0
+    x[1] = L                                            # x = L exactly
+    x[2] = dx * np.array([3, 5, 7], np.float32)         # grid nodes
+    x[3] = (dx * np.array([3.5, 5.5, 7.5])).astype(np.float32)   # cell midpoints
+    x[4] = np.nextafter(L, np.float32(0))               # just below L
+    x[5] = np.nextafter(dx * 2, np.float32(0))           # just below a node
+    x[6] = np.nextafter(dx * 2, np.float32(10))          # just above a node
+    x[7] = np.float32(1e-7)
+    return x.reshape(-1), dx
+
+
+@pytest.mark.parametrize("p", [0, 1, 2, 3, 4])
+def test_grid_weights_indices_bit_exact(gpu, p):
+    say("case: grid weights p =", p)
+    rng = np.random.default_rng(10 + p)
+    Nx, L = (24, 16, 20), (7.3, 5.1, 6.7)
+    x, dx = edge_positions(L, Nx, rng, 20000)
+    inds_g, W_g = gpu.grid_weights(x, Nx, dx, p)
+    inds_o, W_o = orc.grid_weights(x, Nx, dx, p)
+    assert np.array_equal(inds_g, inds_o)
+    assert rel(W_g, W_o) < TOL
+
+
+def test_table_kernels_match_oracle(gpu):
+    say("case: fill_density / map_forces tables")
+    rng = np.random.default_rng(4)
+    Nx, L = (16, 24, 12), (5.0, 7.0, 4.0)
+    x, dx = edge_positions(L, Nx, rng, 3000)
+    inds, W = orc.grid_weights(x, Nx, dx, 3)
+    M = int(np.prod(Nx))
+    sites = np.arange(0, 3000, 3, dtype=np.int32)
+    q = rng.choice([-1.0, 1.0], 3000).astype(np.float32)
+    rho_o, rhoq_o = orc.fill_density(M, sites, inds, W, 64, q=q)
+    rho_g, rhoq_g = gpu.fill_density(M, sites, inds, W.astype(np.float32), 64, q=q)
+    assert rel(rho_g, rho_o) < TOL
+    assert rel(rhoq_g, rhoq_o) < TOL
+    F = rng.normal(size=3 * M)
+    gvol = float(np.float32(np.float32(dx[0] * dx[1]) * dx[2]))
+    f_o = np.zeros(9000)
+    orc.map_forces(f_o, F, sites, inds, W, 64, gvol, 3, q=q)
+    f_g = gpu.map_forces(3000, F.astype(np.float32), sites, inds, W.astype(np.float32), 64, gvol, q=q)
+    assert rel(f_g, f_o) < TOL
+
+
+def test_bonds_match_oracle(gpu):
+    say("case: bonds")
+    cfg = small_system(False)
+    topo = cfg["topo"]
+    f_g = gpu.bonds(cfg["x"], topo, [1.5, 2.0], [0.0, 0.3], [0, 1], cfg["L"])
+    f_o = np.zeros(cfg["x"].size)
+    orc.bonds(f_o, dict(nBonds=topo["nBonds"], bondedTo=topo["bondedTo"], bondType=topo["bondType"],
+                        bondReq=[0.0, 0.3], bondK=[1.5, 2.0], bondStyle=[0, 1], maxbonds=2), cfg["x"], cfg["L"])
+    assert rel(f_g, f_o) < TOL
+
+
+def _engine_vs_oracle(gpu, cfg, integrator="GJF"):
+    eng = systems.tild_engine(cfg, integrator)
+    sys_o = oracle_system(cfg, integrator)
+    return eng, sys_o
+
+
+@pytest.mark.parametrize("charged", [False, True])
+def test_engine_forces_and_densities(gpu, charged):
+    cfg = small_system(charged)
+    say("case: engine forces", cfg["name"], "N =", cfg["topo"]["type"].size)
+    eng, sys_o = _engine_vs_oracle(gpu, cfg)
+    f_g = eng.compute_forces()
+    out = orc.tild_forces(sys_o)
+    rho_g, rhoq_g = eng.densities(charge=charged)
+    for s, name in enumerate(cfg["species"]):
+        assert rel(rho_g[s], out["rho"][name]) < TOL, name
+    if charged:
+        assert rel(rhoq_g, out["rhoq"]) < TOL
+    # grid force of each species group (ps_groups.cu:164-178)
+    for s, name in enumerate(cfg["species"]):
+        if name in out["gridF"]:
+            assert rel(eng.grid_force(s), out["gridF"][name]) < TOL, name
+    assert rel(f_g, out["f"]) < TOL
+    eng.close()
+
+
+def periodic_dist(a, b, L):
+    d = (np.asarray(a, np.float64) - np.asarray(b, np.float64)).reshape(-1, 3)
+    L = np.asarray(L, np.float64)
+    return np.abs(d - L * np.round(d / L))
+
+
+@pytest.mark.parametrize("integrator", ["GJF", "EM"])
+@pytest.mark.parametrize("charged", [False, True])
+def test_engine_steps_match_oracle(gpu, integrator, charged):
+    cfg = small_system(charged, seed=11)
+    say("case: engine steps", cfg["name"], integrator)
+    eng, sys_o = _engine_vs_oracle(gpu, cfg, integrator)
+    ns = cfg["topo"]["type"].size
+    for step in range(3):
+        eng.step(1)
+        xi = orc.tild_noise(ns, cfg["seed"], step)
+        out = orc.tild_step(sys_o, xi)
+        sys_o.x = out["x"].astype(np.float32)
+        if integrator == "GJF":
+            sys_o.xo = out["xo"].astype(np.float32)
+            sys_o.noise_old = out["noise_old"].astype(np.float32)
+        x_g, xo_g, no_g = eng.get_state()
+        assert np.max(periodic_dist(x_g, out["x"], cfg["L"])) < 1e-6 * max(cfg["L"]), step
+        if integrator == "GJF":
+            assert rel(no_g, out["noise_old"]) < 1e-5
+    eng.close()
+
+
+def test_energies_match_oracle(gpu):
+    cfg = small_system(True, seed=3)
+    say("case: energies", cfg["name"])
+    eng, sys_o = _engine_vs_oracle(gpu, cfg)
+    pe, be = eng.energies()
+    out = orc.tild_forces(sys_o, want_energy=True)
+    for g, o in zip(pe, out["energies"]):
+        assert abs(g - o) <= TOL * max(abs(o), 1e-12), (g, o)
+    topo = sys_o.topo
+    eb = orc.bond_energy(dict(nBonds=topo["nBonds"], bondedTo=topo["bondedTo"], bondType=topo["bondType"],
+                              bondReq=cfg["bond_r0"], bondK=cfg["bond_k"], bondStyle=cfg["bond_style"], maxbonds=2),
+                         sys_o.x, cfg["L"])
+    assert be == pytest.approx(0.5 * eb.sum(), rel=TOL)
+    eng.close()
+
+
+def test_ramp_schedule(gpu):
+    # PS_Potential::update_prefactor: step i uses Ao_i + (Ao_f - Ao_i) i / nsteps
+    cfg = small_system(False, seed=2)
+    say("case: ramp")
+    pots = [dict(p) for p in cfg["potentials"]]
+    pots[2].update(ramp=True, A_final=3 * pots[2]["A"])
+    cfg_r = dict(cfg, potentials=pots)
+    eng = systems.tild_engine(cfg_r, "GJF")
+    eng.step(4)   # one NVT run of 4 steps
+    x_g, _, _ = eng.get_state()
+    sys_o = oracle_system(cfg)
+    ns = cfg["topo"]["type"].size
+    A0 = pots[2]["A"]
+    for step in range(4):
+        sys_o.potentials[2].A = A0 + (3 * A0 - A0) * step / 4.0
+        out = orc.tild_step(sys_o, orc.tild_noise(ns, cfg["seed"], step))
+        sys_o.x, sys_o.xo, sys_o.noise_old = (out[k].astype(np.float32) for k in ("x", "xo", "noise_old"))
+    assert np.max(periodic_dist(x_g, sys_o.x, cfg["L"])) < 1e-5 * max(cfg["L"])
+    eng.close()
+
+
+def test_large_coacervate_invariants(gpu):
+    """Config-2 shape (1.8 M particles, 128^3): size-independent properties --
+    species mass conservation, zero net force, zero net charge density, and
+    bitwise determinism of two identical runs."""
+    cfg = systems.tild_coacervate()
+    say("case: coacervate 1.8M invariants")
+    eng = systems.tild_engine(cfg)
+    f = eng.compute_forces().reshape(-1, 3).astype(np.float64)
+    rho, rhoq = eng.densities(charge=True)
+    gvol = np.prod(np.float32(np.asarray(cfg["L"], np.float32) / np.asarray(cfg["Nx"], np.float32)))
+    counts = np.bincount(cfg["topo"]["type"], minlength=3)
+    for s in range(3):
+        assert rho[s].astype(np.float64).sum() * gvol == pytest.approx(counts[s], rel=1e-6)
+    assert abs(rhoq.astype(np.float64).sum() * gvol) < 1e-3 * counts[0]
+    fmax = np.max(np.abs(f))
+    assert np.max(np.abs(f.sum(axis=0))) < 1e-3 * fmax * np.sqrt(f.shape[0])
+    eng.step(3)
+    x1, _, _ = eng.get_state()
+    eng.close()
+    eng2 = systems.tild_engine(cfg)
+    eng2.step(3)
+    x2, _, _ = eng2.get_state()
+    eng2.close()
+    assert np.array_equal(x1, x2), "two identical runs differ"

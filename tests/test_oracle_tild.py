@@ -1,0 +1,236 @@
+"""Pins the TILD oracle against closed forms, invariants and brute force.
+CPU only."""
+import numpy as np
+import pytest
+
+import orc
+from mft import systems
+from test_oracle_core import bspline
+
+
+def brute_force_stencil(x, L, N, p):
+    """Grid points (index, weight) of one coordinate by direct enumeration: the
+    weight of grid point k (at k*dx) is the cardinal B-spline of order p+1
+    evaluated at its distance from the particle in units of dx."""
+    dx = L / N
+    out = {}
+    for k in range(-N, 2 * N):
+        w = bspline((k * dx - x) / dx, p + 1)
+        if abs(w) > 0:
+            out[k % N] = out.get(k % N, 0.0) + w
+    return out
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_grid_weights_brute_force(p):
+    rng = np.random.default_rng(p)
+    Nx = (8, 6, 10)
+    L = (4.0, 3.3, 5.1)
+    dx = [np.float32(np.float32(l) / n) for l, n in zip(L, Nx)]
+    x = (rng.random((40, 3)) * np.asarray(L)).astype(np.float32)
+    x[0] = [0.0, 0.0, 0.0]                       # particle on a corner
+    x[1] = [np.float32(L[0]), 1.0, 2.0]          # x = L exactly (GJF wraps to [0, L])
+    x[2] = [dx[0] * 3, dx[1] * 2, dx[2] * 5]     # on grid nodes
+    inds, W = orc.grid_weights(x.reshape(-1), Nx, dx, p)
+    G = (p + 1) ** 3
+    gvol = float(np.float32(np.float32(dx[0] * dx[1]) * dx[2]))
+    for i in range(x.shape[0]):
+        got = {}
+        for j in range(G):
+            got[inds[i * G + j]] = got.get(inds[i * G + j], 0.0) + W[i * G + j] * gvol
+        per_axis = [brute_force_stencil(float(x[i, a]), float(np.float32(L[a])), Nx[a], p) for a in range(3)]
+        want = {}
+        for kx, wx in per_axis[0].items():
+            for ky, wy in per_axis[1].items():
+                for kz, wz in per_axis[2].items():
+                    idx = kx + Nx[0] * (ky + Nx[1] * kz)
+                    want[idx] = want.get(idx, 0.0) + wx * wy * wz
+        keys = set(k for k, v in want.items() if v > 1e-12) | set(k for k, v in got.items() if v > 1e-12)
+        for k in keys:
+            assert abs(got.get(k, 0.0) - want.get(k, 0.0)) < 2e-6, (p, i, k)
+
+
+def test_density_conserves_mass():
+    rng = np.random.default_rng(3)
+    Nx, L = (12, 8, 16), (6.0, 4.0, 8.0)
+    dx = [np.float32(np.float32(l) / n) for l, n in zip(L, Nx)]
+    x = (rng.random((500, 3)) * np.asarray(L)).astype(np.float32).reshape(-1)
+    inds, W = orc.grid_weights(x, Nx, dx, 3)
+    M = int(np.prod(Nx))
+    sites = np.arange(0, 500, 2, dtype=np.int32)
+    q = np.where(np.arange(500) % 4 == 0, 1.0, -1.0).astype(np.float32)
+    rho, rhoq = orc.fill_density(M, sites, inds, W, 64, q=q)
+    gvol = float(np.float32(np.float32(dx[0] * dx[1]) * dx[2]))
+    assert abs(rho.sum() * gvol - sites.size) < 1e-5
+    assert abs(rhoq.sum() * gvol - q[sites].sum()) < 1e-5
+
+
+def test_uk_tables_known_values():
+    Nx, L = (16, 16, 16), (8.0, 8.0, 8.0)
+    A, sig, Rp = 2.5, 0.7, 1.1
+    g = orc.uk_table("gaussian", A, sig, 0.0, Nx, L)
+    assert g[0] == pytest.approx(A)
+    k2, k = orc.kvec_f(5, Nx, L)
+    assert g[5] == pytest.approx(A * np.exp(-float(k2) * sig * sig / 2), rel=1e-12)
+    e2 = orc.uk_table("erf2", A, sig, Rp, Nx, L)
+    vol = 4 * np.pi * Rp ** 3 / 3
+    assert e2[0] == pytest.approx(A * vol * vol, rel=1e-12)
+    eg = orc.uk_table("erfG", A, sig, Rp, Nx, L)
+    assert eg[0] == pytest.approx(A * vol, rel=1e-12)
+    # 4 pi (sin x - x cos x)/k^3 is the transform of a sphere: check by quadrature
+    kk = float(np.sqrt(k2))
+    r = np.linspace(0, Rp, 20001)
+    quad = np.trapezoid(4 * np.pi * r * np.sin(kk * r) / kk, r)
+    assert eg[5] == pytest.approx(A * np.exp(-k2 * sig * sig / 2) * quad, rel=1e-7)
+    c = orc.uk_table("charges", 2.0, 0.5, 0.0, Nx, L)
+    assert c[0] == 0.0
+    assert c[5] == pytest.approx(4 * np.pi * 2.0 / float(k2) * np.exp(-float(k2) * 0.25), rel=1e-12)
+
+
+def small_system(charged=False, seed=5):
+    if charged:
+        cfg = systems.tild_coacervate(nchains=30, Nchain=8, nsolvent=200, box=6.0, grid=16, seed=seed)
+    else:
+        cfg = systems.tild_diblock(nchains=60, NA=5, NB=5, box=6.0, grid=16, seed=seed)
+    return cfg
+
+
+def oracle_system(cfg, integrator="GJF"):
+    topo = cfg["topo"]
+    names = cfg["species"]
+    pots = []
+    for p in cfg["potentials"]:
+        if p["kind"] == "charges":
+            pots.append(orc.TildPotential("charges", A=p["A"], sigma=p["sigma"]))
+            continue
+        gI = [n for i, n in enumerate(names) if (p["maskI"] >> i) & 1]
+        gJ = [n for i, n in enumerate(names) if (p["maskJ"] >> i) & 1]
+        assert len(gI) == 1 and len(gJ) == 1
+        pots.append(orc.TildPotential(p["kind"], gI[0], gJ[0], A=p["A"], sigma=p["sigma"], Rp=p.get("Rp", 0.0)))
+    t = dict(intSpecies=topo["type"].astype(np.int32), nBonds=topo["nBonds"].astype(np.int32),
+             bondedTo=topo["bondedTo"].astype(np.int32), bondType=topo["bondType"].astype(np.int32),
+             charge=topo["charge"].astype(np.float32))
+    x = cfg["x"].astype(np.float32)
+    return orc.TildSystem(Nx=cfg["Nx"], L=cfg["L"], pmeorder=cfg["pmeorder"], species=names, x=x, topo=t,
+                          bond_k=cfg["bond_k"], bond_r0=cfg["bond_r0"], bond_style=cfg["bond_style"],
+                          potentials=pots, mass=cfg["mass"], mobility=cfg["mobility"], delt=cfg["delt"],
+                          integrator=integrator, xo=x.copy(), noise_old=np.zeros_like(x))
+
+
+@pytest.mark.parametrize("charged", [False, True])
+def test_net_force_vanishes(charged):
+    # the reference's own logNetForce invariant (PS_Box.cu:323-335)
+    sys = oracle_system(small_system(charged))
+    out = orc.tild_forces(sys)
+    f = out["f"].reshape(-1, 3)
+    assert np.max(np.abs(f.sum(axis=0))) < 1e-9 * np.max(np.abs(f)) * f.shape[0]
+
+
+def test_gather_is_transpose_of_scatter():
+    rng = np.random.default_rng(9)
+    Nx, L = (8, 12, 10), (4.0, 6.0, 5.0)
+    dx = [np.float32(np.float32(l) / n) for l, n in zip(L, Nx)]
+    x = (rng.random((300, 3)) * np.asarray(L)).astype(np.float32).reshape(-1)
+    inds, W = orc.grid_weights(x, Nx, dx, 3)
+    M = int(np.prod(Nx))
+    gvol = float(np.float32(np.float32(dx[0] * dx[1]) * dx[2]))
+    sites = np.arange(300, dtype=np.int32)
+    rho, _ = orc.fill_density(M, sites, inds, W, 64)
+    F = rng.normal(size=3 * M)
+    f = np.zeros(900)
+    orc.map_forces(f, F, sites, inds, W, 64, gvol, 3)
+    lhs = f.reshape(-1, 3).sum(axis=0)
+    rhs = gvol * (F.reshape(-1, 3) * rho[:, None]).sum(axis=0)
+    np.testing.assert_allclose(lhs, rhs, rtol=1e-10)
+
+
+def test_gaussian_pair_force_matches_analytic():
+    # two particles, large fine grid: the grid force approaches -grad of the
+    # Gaussian pair potential smeared twice by the assignment function; with
+    # p = 3 on a fine grid the smearing is small relative to sigma = 1.5
+    Nx, L = (64, 64, 64), (16.0, 16.0, 16.0)
+    A, sig = 3.0, 1.5
+    x = np.array([[7.0, 8.0, 8.0], [9.0, 8.0, 8.0]], np.float32)
+    topo = dict(intSpecies=np.array([0, 1], np.int32), nBonds=np.zeros(2, np.int32),
+                bondedTo=np.zeros(4, np.int32), bondType=np.zeros(4, np.int32), charge=np.zeros(2, np.float32))
+    sys = orc.TildSystem(Nx=Nx, L=L, pmeorder=3, species=["A", "B"], x=x.reshape(-1), topo=topo, bond_k=[0.0],
+                         bond_r0=[0.0], bond_style=[0], potentials=[orc.TildPotential("gaussian", "A", "B", A=A, sigma=sig)],
+                         mass=[1, 1], mobility=[1, 1])
+    f = orc.tild_forces(sys)["f"].reshape(2, 3)
+    r = 2.0
+    # u(r) = A (2 pi s^2)^{-3/2} exp(-r^2 / 2 s^2) with s^2 = sigma^2 plus the
+    # variance dx^2/3 of each particle's cubic assignment spline
+    dx = 16.0 / 64
+    s2 = sig * sig + 2 * dx * dx / 3
+    du = A * (2 * np.pi * s2) ** -1.5 * np.exp(-r * r / (2 * s2)) * (-r / s2)
+    assert f[0, 0] == pytest.approx(du, rel=3e-3)   # pushed away (negative x)
+    assert f[1, 0] == pytest.approx(-du, rel=3e-3)
+    assert abs(f[0, 1]) < 1e-9 and abs(f[0, 2]) < 1e-9
+
+
+def test_bond_force_and_energy_analytic():
+    L = (10.0, 10.0, 10.0)
+    x = np.array([[1.0, 1.0, 1.0], [9.5, 1.0, 1.0]], np.float32)  # min-image distance 1.5
+    topo = dict(nBonds=np.array([1, 1], np.int32), bondedTo=np.array([1, 0, 0, 0], np.int32),
+                bondType=np.zeros(4, np.int32), bondReq=[0.5], bondK=[1.5], bondStyle=[0], maxbonds=2)
+    f = np.zeros(6)
+    orc.bonds(f, topo, x.reshape(-1), L)
+    # u = k (r - r0)^2 -> |f| = 2 k (r - r0) toward the partner
+    mag = 2 * 1.5 * (1.5 - 0.5)
+    # the partner's image sits at x = -0.5, so particle 0 is pulled toward -x
+    np.testing.assert_allclose(f.reshape(2, 3)[0], [-mag, 0, 0], atol=1e-6)
+    np.testing.assert_allclose(f.reshape(2, 3)[1], [mag, 0, 0], atol=1e-6)
+    e = orc.bond_energy(topo, x.reshape(-1), L)
+    assert e[0] == pytest.approx(1.5 * (1.5 - 0.5) ** 2, rel=1e-6)
+    # separation 3 (between L/4 and L/2): forces use the true minimum image,
+    # the logged energy wraps against 0.5*Lh = 2.5 (ps_deviceBonds.cu:132-133)
+    x2 = np.array([[1.0, 1.0, 1.0], [4.0, 1.0, 1.0]], np.float32)
+    f2 = np.zeros(6)
+    orc.bonds(f2, topo, x2.reshape(-1), L)
+    np.testing.assert_allclose(f2.reshape(2, 3)[0], [2 * 1.5 * (3.0 - 0.5), 0, 0], atol=1e-6)
+    e2 = orc.bond_energy(topo, x2.reshape(-1), L)
+    assert e2[0] == pytest.approx(1.5 * (7.0 - 0.5) ** 2, rel=1e-6)
+
+
+def test_gjf_harmonic_configurational_temperature():
+    # GJF samples <x^2> = kT/kappa exactly for a harmonic well at any stable dt
+    # (Gronbech-Jensen & Farago 2013), here with dt = 0.2, kappa = 4
+    n, kappa, dt = 4000, 4.0, 0.2
+    L = np.array([1e6], np.float32)
+    coef = orc.gjf_coeffs(dt, 1.0, 1.0)
+    x = np.zeros(n)
+    xo = np.zeros(n)
+    no = np.zeros(n)
+    types = np.zeros(n, np.int32)
+    sites = np.arange(n, dtype=np.int32)
+    lib = orc.lib()
+    samples = []
+    for step in range(3000):
+        xs = x + 5e5
+        f = -kappa * x
+        xi = orc.normal4_np(np.arange(n, dtype=np.uint64), step, orc.TAG_TILD, 0, 77)[:, 0]
+        xo_s = xo + 5e5
+        lib.orc_gjf_step(orc._ptr(xs), orc._ptr(xo_s), orc._ptr(no), orc._ptr(f), orc._ptr(coef), orc._ptr(types),
+                         orc._ptr(L), orc._ptr(sites), n, 1, orc._ptr(np.ascontiguousarray(xi)))
+        x = xs - 5e5
+        xo = xo_s - 5e5
+        if step > 500:
+            samples.append(np.mean(x * x))
+    assert np.mean(samples) == pytest.approx(1.0 / kappa, rel=0.03)
+
+
+def test_make_linear_topology_known_answer():
+    # two diblocks (2 A + 1 B): beads 0..5; block bond types (1, 2)
+    t = orc.make_linear(2, [(2, "A"), (1, "B")], {"A": 0, "B": 1}, block_bond_type=[1, 2])
+    assert list(t["intSpecies"]) == [0, 0, 1, 0, 0, 1]
+    assert list(t["nBonds"]) == [1, 2, 1, 1, 2, 1]
+    # slot 0 = previous bead when present (PS_Box_moleculeMakers.cu:250-256)
+    assert list(t["bondedTo"]) == [1, 0, 0, 2, 1, 0, 4, 0, 3, 5, 4, 0]
+    assert list(t["bondType"]) == [0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0]
+    # the product-side builder produces the same topology bit-for-bit
+    p = systems.make_linear(2, [(2, 0), (1, 1)], block_bond_type=[1, 2])
+    assert list(p["type"]) == list(t["intSpecies"])
+    assert list(p["nBonds"]) == list(t["nBonds"])
+    mask = (np.arange(12) % 2) < np.repeat(t["nBonds"], 2)
+    assert list(p["bondedTo"][mask]) == list(t["bondedTo"][mask])
+    assert list(p["bondType"][mask]) == list(t["bondType"][mask])
