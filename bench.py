@@ -1,0 +1,136 @@
+#!/usr/bin/env python3
+"""Benchmark of the TILD particle-mesh step on one MI355X.
+
+Workload: the large polyelectrolyte coacervate (40 000 polycations + 40 000
+polyanions of 20 beads with charge +-1, 200 000 solvent beads: 1.8 M particles
+in a 60^3 box, 128^3 grid, cubic splines, Gaussian chi/compressibility terms,
+smeared electrostatics l_B = 2, harmonic bonds k = 1.5, GJF dt = 0.005) on
+synthetic random-walk coordinates.  A step is one PS_Box::doTimeStep without
+I/O, inputs resident in HBM.
+
+Prints one JSON line: particle-steps/s with the roofline of the dominant
+kernel (HIP events on the engine stream over the timed steps) and a CPU
+baseline (the FP64 oracle on this host over a bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "matilda.ft_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def step_bytes(N, M, nchan_grid):
+    """Algorithmic HBM bytes of one TILD step: N*(108 + 4 with charges) +
+    M * channels * 80 (per-particle state streamed once; per channel a density
+    write, one forward and three inverse FFTs as one read + write each, one
+    gather read of three force components)."""
+    return N * 112.0 + M * nchan_grid * 80.0
+
+
+def cpu_baseline(seed):
+    """FP64 oracle (single thread) on a 1/8-volume sub-box of the same
+    workload: same density, composition, grid spacing and potentials."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc  # the checker; only this leg of the benchmark loads it
+    from mft import systems
+    cfg = systems.tild_coacervate(nchains=5000, nsolvent=25000, box=30.0, grid=64, seed=seed)
+    sysc = orc.tild_system_from_config(cfg)
+    ns = sysc.ns
+    out = orc.tild_step(sysc, orc.tild_noise(ns, seed, 0))       # warm-up (allocations)
+    sysc.x, sysc.xo, sysc.noise_old = out["x"].astype("float32"), out["xo"].astype("float32"), out["noise_old"].astype("float32")
+    steps = 6
+    t0 = time.perf_counter()
+    for s in range(1, 1 + steps):
+        out = orc.tild_step(sysc, orc.tild_noise(ns, seed, s))
+        sysc.x, sysc.xo, sysc.noise_old = out["x"].astype("float32"), out["xo"].astype("float32"), out["noise_old"].astype("float32")
+    dt = time.perf_counter() - t0
+    return {"value": ns * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 oracle, 6 timed GJF steps (after 1 warm-up) of a 1/8-volume sub-box of the same "
+                      f"workload: {ns} particles, 30^3 box, 64^3 grid, same density/composition/potentials",
+            "seconds": dt}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build runs the single-GPU path only "
+              "(multi-GPU sharding is designed, not built); rerun with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import mft
+    from mft import systems
+
+    cfg = systems.tild_coacervate(seed=args.seed)
+    N = cfg["topo"]["type"].size
+    M = int(cfg["Nx"][0] * cfg["Nx"][1] * cfg["Nx"][2])
+    eng = systems.tild_engine(cfg, "GJF")
+    eng.step(args.warmup)
+    eng.synchronize()
+
+    eng.timing(True)
+    t0 = time.perf_counter()
+    eng.step(args.steps)
+    eng.synchronize()
+    t1 = time.perf_counter()
+    avg_ms, launches = eng.timing(False)
+    wall = t1 - t0
+    ms_step = 1e3 * wall / args.steps
+    value = N * args.steps / wall
+
+    kbytes = eng.kernel_bytes()
+    per_step_ms = {k: avg_ms[k] * launches[k] / args.steps for k in avg_ms}
+    dom = max(per_step_ms, key=per_step_ms.get)
+    achieved = kbytes[dom] / (avg_ms[dom] * 1e-3) / 1e9
+    nchan = len(cfg["species"]) + (1 if cfg["has_charges"] else 0)
+    hbm_step = step_bytes(N, M, nchan) / (ms_step * 1e-3) / 1e9
+
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", f"pmc_{dom}_tild_coacervate.json")
+    if os.path.exists(prof):
+        with open(prof) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+
+    out = {
+        "metric": "TILD particle-steps/s",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "tild_coacervate_1.8M_128cubed", "particles": N, "grid": list(cfg["Nx"]),
+                   "box": list(cfg["L"]), "pmeorder": cfg["pmeorder"], "integrator": "GJF", "dt": cfg["delt"],
+                   "parallelism": "single_gpu"},
+        "hbm_frac_step": hbm_step / HBM_PEAK_GBS,
+        "step_algorithmic_gbs": hbm_step,
+        "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step_ms.items()},
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes": kbytes[dom],
+                     "avg_launch_ms": avg_ms[dom], "traffic": traffic},
+        "cpu_baseline": None,
+    }
+    eng.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.seed)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

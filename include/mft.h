@@ -165,8 +165,10 @@ int mft_tild_synchronize(mft_tild* eng);
  * and bond energy (x 1/2, PS_Box.cu:292, with the reference's 0.5*Lh minimum
  * image of ps_deviceBonds.cu:132-133). */
 int mft_tild_energies(mft_tild* eng, double* pot_energy, double* bond_energy);
-/* Per-kernel timing over the next step() calls: kernel ids below, average ms
- * per launch and launch counts since enable. */
+/* Per-kernel timing: while enabled, every kernel group of step() is
+ * bracketed by HIP events on the engine stream (no host synchronisation);
+ * each call resolves the recorded pairs and returns the average ms per launch
+ * and the launch counts since timing was last switched on. */
 enum {
     MFT_TK_BIN = 0, MFT_TK_SCAN, MFT_TK_SCATTER, MFT_TK_DENSITY, MFT_TK_FFT_X_FWD,
     MFT_TK_FFT_Y_FWD, MFT_TK_CONV_Z, MFT_TK_FFT_Y_INV, MFT_TK_FFT_X_INV, MFT_TK_GATHER,

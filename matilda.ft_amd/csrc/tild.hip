@@ -905,10 +905,12 @@ struct TildEngine {
     size_t dens_lds = 0;
     int64_t step = 0;
     int have_state = 0;
-    // timing
+    // timing: event pairs recorded around kernel launches on the engine
+    // stream without host synchronisation, resolved when timing is read
     bool timing = false;
-    hipEvent_t ev[MFT_TK_COUNT][2];
-    bool ev_made = false;
+    std::vector<hipEvent_t> evpool;
+    std::vector<int> evkind; // kernel id of pair i
+    size_t evused = 0;
     double tk_ms[MFT_TK_COUNT];
     int64_t tk_n[MFT_TK_COUNT];
 };
@@ -1126,22 +1128,35 @@ static int tild_setup(TildEngine* e)
 struct TimedScope {
     TildEngine* e;
     int k;
+    long slot = -1;
     TimedScope(TildEngine* e_, int k_) : e(e_), k(k_)
     {
-        if (e->timing) hipEventRecord(e->ev[k][0], e->stream);
+        if (e->timing && 2 * (e->evused + 1) <= e->evpool.size()) {
+            slot = (long)e->evused++;
+            e->evkind[slot] = k;
+            (void)hipEventRecord(e->evpool[2 * slot], e->stream);
+        }
     }
     ~TimedScope()
     {
-        if (e->timing) {
-            hipEventRecord(e->ev[k][1], e->stream);
-            hipEventSynchronize(e->ev[k][1]);
-            float ms = 0.f;
-            hipEventElapsedTime(&ms, e->ev[k][0], e->ev[k][1]);
-            e->tk_ms[k] += ms;
-            e->tk_n[k] += 1;
-        }
+        if (slot >= 0) (void)hipEventRecord(e->evpool[2 * slot + 1], e->stream);
     }
 };
+
+static void resolve_timing(TildEngine* e)
+{
+    if (e->evused == 0) return;
+    (void)hipStreamSynchronize(e->stream);
+    for (size_t i = 0; i < e->evused; ++i) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e->evpool[2 * i], e->evpool[2 * i + 1]) == hipSuccess) {
+            e->tk_ms[e->evkind[i]] += ms;
+            e->tk_n[e->evkind[i]] += 1;
+        }
+    }
+    (void)hipGetLastError();
+    e->evused = 0;
+}
 
 template <int N, int SIGN, bool RI, bool RO>
 static void launch_axis_n(const void* in, void* out, AxisGeom g, int nf, long fs, float scale, hipStream_t st)
@@ -1291,6 +1306,7 @@ static int do_steps(TildEngine* e, int nsteps, bool want_f)
     const mft_tild_desc& d = e->d;
     int rc;
     for (int i = 0; i < nsteps; ++i) {
+        if (e->timing && 2 * (e->evused + MFT_TK_COUNT) > e->evpool.size()) resolve_timing(e);
         if ((rc = grid_forces(e))) return rc;
         TimedScope ts(e, MFT_TK_INTEGRATE);
         int nxt = 1 - e->cur;
@@ -1422,11 +1438,7 @@ int mft_tild_destroy(mft_tild* e)
         if (p) (void)hipFree(p);
     for (float* t : e->tables)
         if (t) (void)hipFree(t);
-    if (e->ev_made)
-        for (int k = 0; k < MFT_TK_COUNT; ++k) {
-            hipEventDestroy(e->ev[k][0]);
-            hipEventDestroy(e->ev[k][1]);
-        }
+    for (hipEvent_t ev : e->evpool) (void)hipEventDestroy(ev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return MFT_OK;
@@ -1599,20 +1611,20 @@ int mft_tild_energies(mft_tild* e, double* pot_energy, double* bond_energy)
 int mft_tild_timing(mft_tild* e, int enable, double* avg_ms, int64_t* launches)
 {
     MFT_REQUIRE(e, MFT_ERR_INVALID, "null argument");
-    if (enable && !e->ev_made) {
-        for (int k = 0; k < MFT_TK_COUNT; ++k) {
-            MFT_HIP(hipEventCreate(&e->ev[k][0]));
-            MFT_HIP(hipEventCreate(&e->ev[k][1]));
-        }
-        e->ev_made = true;
+    if (enable && e->evpool.empty()) {
+        const size_t pairs = 8192;
+        e->evpool.resize(2 * pairs);
+        e->evkind.assign(pairs, 0);
+        for (auto& ev : e->evpool) MFT_HIP(hipEventCreate(&ev));
     }
+    resolve_timing(e);
     if (enable && !e->timing) {
         for (int k = 0; k < MFT_TK_COUNT; ++k) {
             e->tk_ms[k] = 0;
             e->tk_n[k] = 0;
         }
     }
-    e->timing = enable != 0 && e->ev_made;
+    e->timing = enable != 0;
     for (int k = 0; k < MFT_TK_COUNT; ++k) {
         if (avg_ms) avg_ms[k] = e->tk_n[k] ? e->tk_ms[k] / (double)e->tk_n[k] : 0.0;
         if (launches) launches[k] = e->tk_n[k];

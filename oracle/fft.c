@@ -80,12 +80,44 @@ static void fft_rec(const cplx* in, int istride, cplx* out, int n, const cplx* w
     }
 }
 
+/* Power-of-two lengths: iterative radix-2 decimation in time on a
+ * bit-reversed copy (same roots table). */
+static void fft_pow2(cplx* a, int n, const cplx* w)
+{
+    for (int i = 1, j = 0; i < n; ++i) {
+        int bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) {
+            cplx t = a[i];
+            a[i] = a[j];
+            a[j] = t;
+        }
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+        int half = len >> 1, step = n / len;
+        for (int i = 0; i < n; i += len) {
+            for (int k = 0; k < half; ++k) {
+                cplx u = a[i + k];
+                cplx v = a[i + k + half] * w[k * step];
+                a[i + k] = u + v;
+                a[i + k + half] = u - v;
+            }
+        }
+    }
+}
+
 static void fft_line(cplx* data, int n, size_t stride, const cplx* w, cplx* buf)
 {
     cplx* in = buf;
     cplx* out = buf + n;
     cplx* tmp = buf + 2 * (size_t)n;
     for (int j = 0; j < n; ++j) in[j] = data[(size_t)j * stride];
+    if ((n & (n - 1)) == 0) {
+        fft_pow2(in, n, w);
+        for (int j = 0; j < n; ++j) data[(size_t)j * stride] = in[j];
+        return;
+    }
     fft_rec(in, 1, out, n, w, n, tmp);
     for (int j = 0; j < n; ++j) data[(size_t)j * stride] = out[j];
 }

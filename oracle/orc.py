@@ -315,6 +315,29 @@ class TildSystem:
         raise KeyError(name)
 
 
+def tild_system_from_config(cfg, integrator="GJF"):
+    """TildSystem for a workload dict of mft.systems (species masks -> group names)."""
+    topo = cfg["topo"]
+    names = cfg["species"]
+    pots = []
+    for p in cfg["potentials"]:
+        if p["kind"] == "charges":
+            pots.append(TildPotential("charges", A=p["A"], sigma=p["sigma"]))
+            continue
+        gI = [n for i, n in enumerate(names) if (p["maskI"] >> i) & 1]
+        gJ = [n for i, n in enumerate(names) if (p["maskJ"] >> i) & 1]
+        assert len(gI) == 1 and len(gJ) == 1
+        pots.append(TildPotential(p["kind"], gI[0], gJ[0], A=p["A"], sigma=p["sigma"], Rp=p.get("Rp", 0.0)))
+    t = dict(intSpecies=topo["type"].astype(np.int32), nBonds=topo["nBonds"].astype(np.int32),
+             bondedTo=topo["bondedTo"].astype(np.int32), bondType=topo["bondType"].astype(np.int32),
+             charge=topo["charge"].astype(np.float32))
+    x = cfg["x"].astype(np.float32)
+    return TildSystem(Nx=cfg["Nx"], L=cfg["L"], pmeorder=cfg["pmeorder"], species=names, x=x, topo=t,
+                      bond_k=cfg["bond_k"], bond_r0=cfg["bond_r0"], bond_style=cfg["bond_style"],
+                      potentials=pots, mass=cfg["mass"], mobility=cfg["mobility"], delt=cfg["delt"],
+                      integrator=integrator, xo=x.copy(), noise_old=np.zeros_like(x))
+
+
 def tild_forces(sys: TildSystem, bits=64, want_energy=False):
     """Particle forces at the current positions (PS_Box.cu:96-128, 240-278)."""
     Dim, M = sys.Dim, sys.M
