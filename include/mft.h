@@ -216,7 +216,7 @@ typedef struct {
     int nmolecules;       /* <= 4 */
     mft_fts_molecule molecules[4];
     int cl;               /* 1: complex Langevin (noise), 0: SCFT (no noise) */
-    uint64_t seed;        /* noise key (plus the request id in batches) */
+    uint64_t seed;        /* noise key; give every replica/request its own seed */
 } mft_fts_desc;
 
 typedef struct mft_fts mft_fts;
@@ -246,7 +246,9 @@ int mft_fts_get_propagators(mft_fts* sys, int m, float* q, float* qdag);
 int mft_fts_observables(mft_fts* sys, double* Q, double* H, double* parts);
 
 /* Batched CL-FTS: many independent replicas of uneven sizes advanced by the
- * same launches.  Replica r uses descs[r] (seed keyed by replica id). */
+ * same launches (replicas with equal grids share every launch).  Replica r
+ * uses descs[r]; its noise depends only on descs[r].seed and its own
+ * iteration count, so a replica gives the same result alone or in a batch. */
 typedef struct mft_fts_batch mft_fts_batch;
 int mft_fts_batch_create(mft_fts_batch** b, const mft_fts_desc* descs, int nrep);
 int mft_fts_batch_destroy(mft_fts_batch* b);

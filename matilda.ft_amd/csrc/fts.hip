@@ -1,0 +1,1162 @@
+// Complex-Langevin FTS step on gfx950 (FTS_Box::doTimeStep, FTS_Box.cu:19-123).
+//
+// Per iteration and replica:
+//   update: every potential's EM step with real CL noise from the previous
+//     densities (fts_potential_*.cu updateFields), species fields
+//     W_s (fts_species.cu:53-132) and E_s = exp(-W_s/2), the first propagator
+//     slice of every chain, and FP64 partial sums of the Hamiltonian field
+//     terms -- one elementwise pass;
+//   propagators in the symmetric form p_s = e^{W_s/2} q_s
+//     (fts_molecule_linear.cu:144-269 uses q_s = e^{-W_s} F^-1[g F q_{s-1}]):
+//     p_s = E_{b(s)} D[E_{b(s-1)} p_{s-1}], D = F^-1 g F applied axis by axis
+//     because g(k) = exp(-k^2/(Nr-1)) factorises over kx, ky, kz; the forward
+//     and complementary chains of every replica share each launch;
+//   density: rho_b ~ sum_s p_s pdag_{N-1-s} (equal to e^{W} sum q qdag of
+//     fts_molecule_linear.cu:301-328) and Q = gvol/V sum E p_{N-1}, FP64
+//     accumulation; the n/(QV) factor is applied by the next update.
+// Replicas with equal grids form one group; every launch covers a group.
+#include <cmath>
+#include <array>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "common.h"
+#include "fft_core.h"
+
+namespace mft {
+
+const float2* fft_twiddles_device(int n); // fft.hip
+
+constexpr int kFtsMaxSpecies = 8;
+constexpr int kFtsMaxPots = 8;
+constexpr int kFtsMaxMol = 4;
+constexpr int kFtsMaxBlocks = 16;
+constexpr int kFtsThreads = 256;
+
+struct FtsPotDev {
+    int kind, I, J;
+    double cpl, cmi;          // force coefficients of w+ and w-
+    double dt_pl, dt_mi;
+    double nmag_pl, nmag_mi;  // sqrt(2 dt / gvol) or 0 (no noise)
+    int col_pl, col_mi;       // noise normal index or -1
+    float2 *wpl, *wmi;
+};
+
+struct FtsUpdDev {
+    int M;
+    int nspecies, npots, nblk;     // nblk = blocks over all molecules
+    FtsPotDev pot[kFtsMaxPots];
+    int blk_species[kFtsMaxMol * kFtsMaxBlocks];
+    int blk_mol[kFtsMaxMol * kFtsMaxBlocks];
+    float2* rhot;                  // [nblk][M] unscaled block densities
+    const double2* Q;              // [nmol]
+    double nmolecs[kFtsMaxMol];
+    double V, C, Nr;
+    float2* W;                     // [nspecies][M]
+    float2* E;                     // [nspecies][M]
+    int nchain;
+    float2* chain_p0[2 * kFtsMaxMol];
+    int chain_species0[2 * kFtsMaxMol];
+    double* Hpart;                 // [gridDim.x][npots*6]
+    uint64_t seed;
+    uint32_t stream;
+    const int64_t* iter;           // device counter (CL iteration)
+    int cl;
+    int first;                     // initialisation pass: no field update
+    int have_rho;                  // densities exist (false before the first density pass)
+};
+
+// Register-array helpers with compile-time indices (runtime indices would
+// spill the arrays to scratch memory).
+template <typename T, int N>
+__device__ __forceinline__ T pick(const T (&a)[N], int k)
+{
+    T v = a[0];
+#pragma unroll
+    for (int j = 1; j < N; ++j)
+        if (j == k) v = a[j];
+    return v;
+}
+
+template <int N>
+__device__ __forceinline__ void add_at(double2 (&a)[N], int k, double x, double y)
+{
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+        if (j == k) {
+            a[j].x += x;
+            a[j].y += y;
+        }
+}
+
+// Elementwise update of one replica (blockIdx.y selects the replica).
+__global__ void __launch_bounds__(kFtsThreads) k_fts_update(const FtsUpdDev* __restrict__ descs)
+{
+    const FtsUpdDev& D = descs[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < D.M;
+    double hs[kFtsMaxPots * 6];
+#pragma unroll
+    for (int k = 0; k < kFtsMaxPots * 6; ++k) hs[k] = 0.0;
+    if (valid) {
+        // species densities of the previous iteration
+        double2 rho[kFtsMaxSpecies];
+#pragma unroll
+        for (int s = 0; s < kFtsMaxSpecies; ++s) rho[s] = make_double2(0.0, 0.0);
+        if (D.have_rho) {
+            for (int b = 0; b < D.nblk; ++b) {
+                int m = D.blk_mol[b];
+                double2 Q = D.Q[m];
+                // factor = n / (Q V)
+                double den = (Q.x * Q.x + Q.y * Q.y) * D.V;
+                double fr = D.nmolecs[m] * Q.x / den, fi = -D.nmolecs[m] * Q.y / den;
+                float2 r = D.rhot[(long)b * D.M + i];
+                add_at(rho, D.blk_species[b], fr * r.x - fi * r.y, fr * r.y + fi * r.x);
+            }
+        }
+        double2 rt = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int s = 0; s < kFtsMaxSpecies; ++s) {
+            rt.x += rho[s].x;
+            rt.y += rho[s].y;
+        }
+        float z[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) z[k] = 0.f;
+        if (D.cl && !D.first) {
+            int64_t it = *D.iter;
+            normal4((uint32_t)i, (uint32_t)it, TAG_FTS, D.stream * 16u, D.seed, z);
+            normal4((uint32_t)i, (uint32_t)it, TAG_FTS, D.stream * 16u + 1u, D.seed, z + 4);
+        }
+        double2 W[kFtsMaxSpecies];
+#pragma unroll
+        for (int s = 0; s < kFtsMaxSpecies; ++s) W[s] = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int p = 0; p < kFtsMaxPots; ++p) {
+            if (p >= D.npots) break;
+            const FtsPotDev& P = D.pot[p];
+            float2 wp = P.wpl[i];
+            double2 w = make_double2(wp.x, wp.y);
+            if (P.kind == MFT_FTS_FLORY) {
+                float2 wm = P.wmi[i];
+                double2 v = make_double2(wm.x, wm.y);
+                if (!D.first) {
+                    // d_makeFloryForce (fts_potential_flory.cu:279-304)
+                    double2 rI = pick(rho, P.I), rJ = pick(rho, P.J);
+                    double fpx = P.cpl * w.x - (rI.y + rJ.y) / D.Nr;
+                    double fpy = P.cpl * w.y + (rI.x + rJ.x) / D.Nr;
+                    double fmx = P.cmi * v.x + (rJ.x - rI.x) / D.Nr;
+                    double fmy = P.cmi * v.y + (rJ.y - rI.y) / D.Nr;
+                    // d_fts_updateEM (fts_updateSchemes.cu:8-29), real noise
+                    w.x -= P.dt_pl * fpx;
+                    w.y -= P.dt_pl * fpy;
+                    v.x -= P.dt_mi * fmx;
+                    v.y -= P.dt_mi * fmy;
+                    if (P.col_pl >= 0) w.x += P.nmag_pl * (double)pick(z, P.col_pl);
+                    if (P.col_mi >= 0) v.x += P.nmag_mi * (double)pick(z, P.col_mi);
+                    float2 wf = make_float2((float)w.x, (float)w.y), vf = make_float2((float)v.x, (float)v.y);
+                    P.wpl[i] = wf;
+                    P.wmi[i] = vf;
+                    w = make_double2(wf.x, wf.y);
+                    v = make_double2(vf.x, vf.y);
+                }
+                // species fields: -w- on I, +w- on J, i w+ on both
+                add_at(W, P.I, -v.x - w.y, -v.y + w.x);
+                add_at(W, P.J, v.x - w.y, v.y + w.x);
+                hs[p * 6 + 0] += w.x * w.x - w.y * w.y;
+                hs[p * 6 + 1] += 2.0 * w.x * w.y;
+                hs[p * 6 + 4] += v.x * v.x - v.y * v.y;
+                hs[p * 6 + 5] += 2.0 * v.x * v.y;
+            } else {
+                if (!D.first) {
+                    double fx, fy;
+                    if (P.kind == MFT_FTS_HELFAND) {
+                        // d_makeHelfandForce (fts_potential_helfand.cu:223-240)
+                        fx = P.cpl * w.x - rt.y / D.Nr;
+                        fy = P.cpl * w.y + rt.x / D.Nr - D.C;
+                    } else {
+                        // d_makeIncompressForce (fts_potential_incompress.cu:202-218)
+                        fx = -rt.y / D.Nr;
+                        fy = rt.x / D.Nr - D.C;
+                    }
+                    w.x -= P.dt_pl * fx;
+                    w.y -= P.dt_pl * fy;
+                    if (P.col_pl >= 0) w.x += P.nmag_pl * (double)pick(z, P.col_pl);
+                    float2 wf = make_float2((float)w.x, (float)w.y);
+                    P.wpl[i] = wf;
+                    w = make_double2(wf.x, wf.y);
+                }
+                // i w+ on every species
+#pragma unroll
+                for (int s = 0; s < kFtsMaxSpecies; ++s) {
+                    W[s].x -= w.y;
+                    W[s].y += w.x;
+                }
+                hs[p * 6 + 0] += w.x * w.x - w.y * w.y;
+                hs[p * 6 + 1] += 2.0 * w.x * w.y;
+                hs[p * 6 + 2] += w.x;
+                hs[p * 6 + 3] += w.y;
+            }
+        }
+        double inv = 1.0 / D.Nr;
+        float2 Es[kFtsMaxSpecies];
+#pragma unroll
+        for (int s = 0; s < kFtsMaxSpecies; ++s) {
+            Es[s] = make_float2(0.f, 0.f);
+            if (s >= D.nspecies) continue;
+            double wr = W[s].x * inv, wi = W[s].y * inv;
+            D.W[(long)s * D.M + i] = make_float2((float)wr, (float)wi);
+            // exp(-W/2)
+            double a = exp(-0.5 * wr), sn, cs;
+            sincos(-0.5 * wi, &sn, &cs);
+            Es[s] = make_float2((float)(a * cs), (float)(a * sn));
+            D.E[(long)s * D.M + i] = Es[s];
+        }
+        for (int c = 0; c < D.nchain; ++c) D.chain_p0[c][i] = pick(Es, D.chain_species0[c]);
+    }
+    // Hamiltonian partial sums (fields after the update), FP64, fixed order
+    __shared__ double red[(kFtsThreads / 64) * kFtsMaxPots * 6];
+#pragma unroll
+    for (int k = 0; k < kFtsMaxPots * 6; ++k) {
+        if (k >= D.npots * 6) break;
+        double v = wave_sum(hs[k]);
+        if ((threadIdx.x & 63) == 0) red[(threadIdx.x >> 6) * kFtsMaxPots * 6 + k] = v;
+    }
+    __syncthreads();
+    const int nh = D.npots * 6;
+    if ((int)threadIdx.x < nh) {
+        double s = 0.0;
+        for (int w = 0; w < kFtsThreads / 64; ++w) s += red[w * kFtsMaxPots * 6 + threadIdx.x];
+        D.Hpart[(long)blockIdx.x * kFtsMaxPots * 6 + threadIdx.x] = s;
+    }
+}
+
+// One propagation slice along one axis for a list of chains.
+struct PropDesc {
+    const float2* src;
+    float2* dst;
+    const float2* pre;   // multiply before (E_{b(s-1)}) or null
+    const float2* post;  // multiply after (E_{b(s)}) or null
+    const float2* g;     // exp(-k_a^2/(Nr-1)) / N_a along this axis as hi + lo
+};
+
+template <int N>
+__global__ void __launch_bounds__(256) k_fts_prop_axis(const PropDesc* __restrict__ descs, int nx, int ny, int nz,
+                                                       int axis, const float2* __restrict__ twg)
+{
+    using P = FftPlan<N>;
+    constexpr int T = P::T, E = P::E, B = 256 / T;
+    __shared__ float2 lds[B * N];
+    __shared__ float2 tw[N];
+    __shared__ float2 gk[N];
+    const PropDesc& D = descs[blockIdx.y];
+    for (int i = threadIdx.x; i < N; i += 256) {
+        tw[i] = twg[i];
+        gk[i] = D.g[i];
+    }
+    AxisGeom geom{nx, ny, nz, axis};
+    const bool inter = axis != 0;
+    const int lline = inter ? (threadIdx.x % B) : (threadIdx.x / T);
+    const int t = inter ? (threadIdx.x / B) : (threadIdx.x % T);
+    const int line = blockIdx.x * B + lline;
+    const bool valid = line < geom.nlines();
+    int ta, tb;
+    const long base = valid ? geom.base(line, &ta, &tb) : 0;
+    const long st = geom.stride();
+    float2 v[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        long o = base + (long)(t + T * s) * st;
+        float2 a = valid ? D.src[o] : make_float2(0.f, 0.f);
+        if (D.pre && valid) a = cmul(a, D.pre[o]);
+        v[s] = a;
+    }
+    __syncthreads();
+    LdsMap map{lline, B, inter, N};
+    line_fft<N, -1>(v, lds, tw, t, map);
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        // two-float factor: the per-axis 1/N of a non-power-of-two length is not
+        // a float, and rounding it once would bias the mean every slice
+        const float2 g = gk[t + T * s];
+        v[s] = make_float2(fmaf(v[s].x, g.y, v[s].x * g.x), fmaf(v[s].y, g.y, v[s].y * g.x));
+    }
+    line_fft<N, +1>(v, lds, tw, t, map);
+    if (!valid) return;
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        long o = base + (long)(t + T * s) * st;
+        float2 a = v[s];
+        if (D.post) a = cmul(a, D.post[o]);
+        D.dst[o] = a;
+    }
+}
+
+struct DensDesc {
+    const float2* p;       // forward chain [Ntot][M]
+    const float2* pd;      // complementary chain (== p when symmetric)
+    const float2* Elast;   // E of the last bead's species (for Q)
+    float2* rhot;          // [nblocks][M]
+    double* Qpart;         // [gridDim.x]
+    int M, Ntot, nblocks;
+    int N[kFtsMaxBlocks];
+};
+
+// rho_b(r) = sum_{s in b} p_s(r) pdag_{Ntot-1-s}(r)  and  Q partial sums.
+__global__ void __launch_bounds__(kFtsThreads) k_fts_density(const DensDesc* __restrict__ descs)
+{
+    const DensDesc& D = descs[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double qr = 0.0, qi = 0.0;
+    if (i < D.M) {
+        int s = 0;
+        for (int b = 0; b < D.nblocks; ++b) {
+            double ar = 0.0, ai = 0.0;
+            for (int k = 0; k < D.N[b]; ++k, ++s) {
+                float2 a = D.p[(long)s * D.M + i];
+                float2 c = D.pd[(long)(D.Ntot - 1 - s) * D.M + i];
+                ar += (double)a.x * c.x - (double)a.y * c.y;
+                ai += (double)a.x * c.y + (double)a.y * c.x;
+            }
+            D.rhot[(long)b * D.M + i] = make_float2((float)ar, (float)ai);
+        }
+        float2 e = D.Elast[i];
+        float2 pl = D.p[(long)(D.Ntot - 1) * D.M + i];
+        qr = (double)e.x * pl.x - (double)e.y * pl.y;
+        qi = (double)e.x * pl.y + (double)e.y * pl.x;
+    }
+    __shared__ double red[2 * (kFtsThreads / 64)];
+    qr = wave_sum(qr);
+    qi = wave_sum(qi);
+    if ((threadIdx.x & 63) == 0) {
+        red[2 * (threadIdx.x >> 6)] = qr;
+        red[2 * (threadIdx.x >> 6) + 1] = qi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < kFtsThreads / 64; ++w) {
+            a += red[2 * w];
+            b += red[2 * w + 1];
+        }
+        D.Qpart[2 * blockIdx.x] = a;
+        D.Qpart[2 * blockIdx.x + 1] = b;
+    }
+}
+
+struct QDesc {
+    const double* Qpart;
+    int nparts;
+    double scale;    // gvol / V
+    double2* Q;
+};
+
+__global__ void k_fts_qreduce(const QDesc* __restrict__ descs)
+{
+    const QDesc& D = descs[blockIdx.x];
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < D.nparts; k += blockDim.x) {
+        a += D.Qpart[2 * k];
+        b += D.Qpart[2 * k + 1];
+    }
+    __shared__ double red[2 * 16];
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if ((threadIdx.x & 63) == 0) {
+        red[2 * (threadIdx.x >> 6)] = a;
+        red[2 * (threadIdx.x >> 6) + 1] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double x = 0.0, y = 0.0;
+        for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) {
+            x += red[2 * w];
+            y += red[2 * w + 1];
+        }
+        *D.Q = make_double2(x * D.scale, y * D.scale);
+    }
+}
+
+__global__ void k_fts_iter_incr(int64_t* it, int n)
+{
+    if (threadIdx.x < n) it[threadIdx.x] += 1;
+}
+
+// Species densities rho_s = sum_b factor_m rho~_b (output only).
+__global__ void k_fts_species_rho(const FtsUpdDev* __restrict__ descs, float2* __restrict__ out)
+{
+    const FtsUpdDev& D = descs[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D.M) return;
+    for (int s = 0; s < D.nspecies; ++s) {
+        double rx = 0.0, ry = 0.0;
+        for (int b = 0; b < D.nblk; ++b) {
+            if (D.blk_species[b] != s) continue;
+            int m = D.blk_mol[b];
+            double2 Q = D.Q[m];
+            double den = (Q.x * Q.x + Q.y * Q.y) * D.V;
+            double fr = D.nmolecs[m] * Q.x / den, fi = -D.nmolecs[m] * Q.y / den;
+            float2 r = D.rhot[(long)b * D.M + i];
+            rx += fr * r.x - fi * r.y;
+            ry += fr * r.y + fi * r.x;
+        }
+        out[(long)s * D.M + i] = make_float2((float)rx, (float)ry);
+    }
+}
+
+// ----------------------------------------------------------- host side --
+struct FtsRep {
+    mft_fts_desc d;
+    int id = 0;
+    int M = 0;
+    double gvol = 0, V = 0;
+    int Ntot[kFtsMaxMol] = {0};
+    bool sym[kFtsMaxMol] = {false};
+    double nmolecs[kFtsMaxMol] = {0};
+    int nblk = 0;
+    float2 *wpl[kFtsMaxPots] = {nullptr}, *wmi[kFtsMaxPots] = {nullptr};
+    float2 *W = nullptr, *E = nullptr, *rhot = nullptr;
+    float2 *p[kFtsMaxMol] = {nullptr}, *pd[kFtsMaxMol] = {nullptr};
+    float2* tmp = nullptr; // [2*nmol][M]
+    float2* g[3] = {nullptr, nullptr, nullptr};
+    double *Hpart = nullptr, *Qpart = nullptr;
+    double2* Q = nullptr;
+    int nHpart = 0, nQpart = 0;
+};
+
+struct FtsGroup {
+    int n[3];
+    std::vector<int> reps;
+    FtsUpdDev* d_upd = nullptr;
+    DensDesc* d_dens = nullptr;
+    int ndens = 0;
+    QDesc* d_q = nullptr;
+    // propagation: per step s (1..maxN-1) and axis, a descriptor list
+    std::vector<PropDesc*> d_prop;   // [step*3 + axis]
+    std::vector<int> nprop;          // chains per step
+    int maxN = 0;
+};
+
+struct FtsBatch {
+    std::vector<FtsRep> reps;
+    std::vector<FtsGroup> groups;
+    hipStream_t stream = nullptr;
+    int64_t* d_iter = nullptr;       // one counter per replica (shared step count)
+    int64_t iter = 0;
+    bool inited = false;
+    std::vector<void*> allocs;
+};
+
+template <typename T>
+static int falloc(FtsBatch* b, T** p, size_t n)
+{
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        set_error(MFT_ERR_OOM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+        return MFT_ERR_OOM;
+    }
+    b->allocs.push_back((void*)*p);
+    return MFT_OK;
+}
+
+static int validate_desc(const mft_fts_desc& d)
+{
+    MFT_REQUIRE(d.nx >= 1 && d.ny >= 1 && d.nz >= 1, MFT_ERR_INVALID, "grid sizes must be positive");
+    for (int n : {d.nx, d.ny, d.nz})
+        MFT_REQUIRE(mft_fft_supported(n), MFT_ERR_UNSUPPORTED, "grid size %d is not a supported FFT length", n);
+    MFT_REQUIRE(d.lx > 0 && d.ly > 0 && d.lz > 0, MFT_ERR_INVALID, "box lengths must be positive");
+    MFT_REQUIRE(d.Nr > 1.0, MFT_ERR_INVALID, "Nr must be given and > 1 (the reference has no default)");
+    MFT_REQUIRE(d.C > 0.0, MFT_ERR_INVALID, "C must be positive");
+    MFT_REQUIRE(d.nspecies >= 1 && d.nspecies <= kFtsMaxSpecies, MFT_ERR_INVALID, "nspecies must be 1..%d",
+                kFtsMaxSpecies);
+    MFT_REQUIRE(d.npotentials >= 0 && d.npotentials <= kFtsMaxPots, MFT_ERR_INVALID, "too many potentials");
+    MFT_REQUIRE(d.nmolecules >= 1 && d.nmolecules <= kFtsMaxMol, MFT_ERR_INVALID, "nmolecules must be 1..%d",
+                kFtsMaxMol);
+    int noisy = 0;
+    for (int p = 0; p < d.npotentials; ++p) {
+        const mft_fts_potential& P = d.potentials[p];
+        MFT_REQUIRE(P.kind >= 0 && P.kind <= MFT_FTS_INCOMPRESS, MFT_ERR_UNSUPPORTED, "FTS potential kind %d",
+                    P.kind);
+        if (P.kind == MFT_FTS_FLORY) {
+            MFT_REQUIRE(P.I >= 0 && P.I < d.nspecies && P.J >= 0 && P.J < d.nspecies && P.I != P.J, MFT_ERR_INVALID,
+                        "Flory species out of range");
+            MFT_REQUIRE(P.chiN != 0.0, MFT_ERR_INVALID, "Flory chiN must be nonzero");
+            noisy += 2;
+        } else if (P.kind == MFT_FTS_HELFAND) {
+            MFT_REQUIRE(P.kappaN != 0.0, MFT_ERR_INVALID, "Helfand kappaN must be nonzero");
+            noisy += 1;
+        }
+    }
+    MFT_REQUIRE(noisy <= 8, MFT_ERR_UNSUPPORTED, "at most 8 noisy fields");
+    for (int m = 0; m < d.nmolecules; ++m) {
+        const mft_fts_molecule& mol = d.molecules[m];
+        MFT_REQUIRE(mol.nblocks >= 1 && mol.nblocks <= kFtsMaxBlocks, MFT_ERR_INVALID, "blocks must be 1..%d",
+                    kFtsMaxBlocks);
+        for (int b = 0; b < mol.nblocks; ++b) {
+            MFT_REQUIRE(mol.N[b] >= 1, MFT_ERR_INVALID, "block length must be positive");
+            MFT_REQUIRE(mol.species[b] >= 0 && mol.species[b] < d.nspecies, MFT_ERR_INVALID,
+                        "species not found!");
+        }
+        MFT_REQUIRE(mol.phi > 0.0, MFT_ERR_INVALID, "molecule phi must be positive");
+    }
+    return MFT_OK;
+}
+
+// exp(-k^2/(Nr-1))/n per axis as an unevaluated float sum hi + lo; k from
+// the reference's float wavevector (Box.cu:182-203)
+static void axis_table(int n, float L, double Nr, std::vector<float2>& g)
+{
+    g.resize(n);
+    for (int i = 0; i < n; ++i) {
+        double k = (double)kcomp(i, n, L);
+        double v = std::exp(-k * k / (Nr - 1.0)) / (double)n;
+        float hi = (float)v;
+        g[i] = make_float2(hi, (float)(v - (double)hi));
+    }
+}
+
+static int rep_setup(FtsBatch* b, FtsRep& r)
+{
+    const mft_fts_desc& d = r.d;
+    int rc;
+    r.M = d.nx * d.ny * d.nz;
+    // FTS_Box.cu:466-476 (dx in double from the float box lengths)
+    r.V = (double)d.lx * (double)d.ly * (double)d.lz;
+    r.gvol = ((double)d.lx / d.nx) * ((double)d.ly / d.ny) * ((double)d.lz / d.nz);
+    const size_t M = r.M;
+    for (int p = 0; p < d.npotentials; ++p) {
+        if ((rc = falloc(b, &r.wpl[p], M))) return rc;
+        MFT_HIP(hipMemset(r.wpl[p], 0, sizeof(float2) * M));
+        if (d.potentials[p].kind == MFT_FTS_FLORY) {
+            if ((rc = falloc(b, &r.wmi[p], M))) return rc;
+            MFT_HIP(hipMemset(r.wmi[p], 0, sizeof(float2) * M));
+        }
+    }
+    if ((rc = falloc(b, &r.W, M * d.nspecies))) return rc;
+    if ((rc = falloc(b, &r.E, M * d.nspecies))) return rc;
+    r.nblk = 0;
+    for (int m = 0; m < d.nmolecules; ++m) {
+        const mft_fts_molecule& mol = d.molecules[m];
+        int Nt = 0;
+        for (int k = 0; k < mol.nblocks; ++k) Nt += mol.N[k];
+        r.Ntot[m] = Nt;
+        // fts_molecule_linear.cu:81-88
+        bool sym = true;
+        for (int j = 0; j < mol.nblocks / 2; ++j)
+            if (mol.species[j] != mol.species[mol.nblocks - 1 - j] || mol.N[j] != mol.N[mol.nblocks - 1 - j])
+                sym = false;
+        r.sym[m] = sym;
+        // fts_molecule_linear.cu:433-435 with Vfree = V
+        r.nmolecs[m] = d.C * mol.phi * r.V * d.Nr / (double)Nt;
+        if ((rc = falloc(b, &r.p[m], M * Nt))) return rc;
+        if (sym) r.pd[m] = r.p[m];
+        else if ((rc = falloc(b, &r.pd[m], M * Nt))) return rc;
+        r.nblk += mol.nblocks;
+    }
+    MFT_REQUIRE(r.nblk <= kFtsMaxMol * kFtsMaxBlocks, MFT_ERR_INVALID, "too many blocks");
+    if ((rc = falloc(b, &r.rhot, M * r.nblk))) return rc;
+    MFT_HIP(hipMemset(r.rhot, 0, sizeof(float2) * M * r.nblk));
+    if ((rc = falloc(b, &r.tmp, M * 2 * d.nmolecules))) return rc;
+    const int ns[3] = {d.nx, d.ny, d.nz};
+    const float Ls[3] = {d.lx, d.ly, d.lz};
+    for (int a = 0; a < 3; ++a) {
+        std::vector<float2> g;
+        axis_table(ns[a], Ls[a], d.Nr, g);
+        if ((rc = falloc(b, &r.g[a], g.size()))) return rc;
+        MFT_HIP(hipMemcpy(r.g[a], g.data(), sizeof(float2) * g.size(), hipMemcpyHostToDevice));
+    }
+    r.nHpart = (r.M + kFtsThreads - 1) / kFtsThreads;
+    r.nQpart = r.nHpart;
+    if ((rc = falloc(b, &r.Hpart, (size_t)r.nHpart * kFtsMaxPots * 6))) return rc;
+    if ((rc = falloc(b, &r.Qpart, (size_t)r.nQpart * 2 * d.nmolecules))) return rc;
+    if ((rc = falloc(b, &r.Q, (size_t)d.nmolecules))) return rc;
+    return MFT_OK;
+}
+
+static void fill_upd(const FtsBatch* b, const FtsRep& r, FtsUpdDev& U, int first, int have_rho)
+{
+    const mft_fts_desc& d = r.d;
+    std::memset(&U, 0, sizeof(U));
+    U.M = r.M;
+    U.nspecies = d.nspecies;
+    U.npots = d.npotentials;
+    int col = 0;
+    for (int p = 0; p < d.npotentials; ++p) {
+        const mft_fts_potential& P = d.potentials[p];
+        FtsPotDev& D = U.pot[p];
+        D.kind = P.kind;
+        D.I = P.I;
+        D.J = P.J;
+        D.wpl = r.wpl[p];
+        D.wmi = r.wmi[p];
+        D.dt_pl = P.delt_pl;
+        D.dt_mi = P.delt_mi;
+        D.col_pl = D.col_mi = -1;
+        if (P.kind == MFT_FTS_FLORY) {
+            D.cpl = D.cmi = 2.0 * d.C / P.chiN;
+            if (d.cl) {
+                D.col_pl = col++;
+                D.col_mi = col++;
+                D.nmag_pl = std::sqrt(2.0 * P.delt_pl / r.gvol);
+                D.nmag_mi = std::sqrt(2.0 * P.delt_mi / r.gvol);
+            }
+        } else if (P.kind == MFT_FTS_HELFAND) {
+            D.cpl = d.C / P.kappaN;
+            if (d.cl) {
+                D.col_pl = col++;
+                D.nmag_pl = std::sqrt(2.0 * P.delt_pl / r.gvol);
+            }
+        }
+    }
+    int k = 0;
+    for (int m = 0; m < d.nmolecules; ++m)
+        for (int bb = 0; bb < d.molecules[m].nblocks; ++bb, ++k) {
+            U.blk_species[k] = d.molecules[m].species[bb];
+            U.blk_mol[k] = m;
+        }
+    U.nblk = k;
+    U.rhot = r.rhot;
+    U.Q = r.Q;
+    for (int m = 0; m < d.nmolecules; ++m) U.nmolecs[m] = r.nmolecs[m];
+    U.V = r.V;
+    U.C = d.C;
+    U.Nr = d.Nr;
+    U.W = r.W;
+    U.E = r.E;
+    U.nchain = 0;
+    for (int m = 0; m < d.nmolecules; ++m) {
+        const mft_fts_molecule& mol = d.molecules[m];
+        U.chain_p0[U.nchain] = r.p[m];
+        U.chain_species0[U.nchain++] = mol.species[0];
+        if (!r.sym[m]) {
+            U.chain_p0[U.nchain] = r.pd[m];
+            U.chain_species0[U.nchain++] = mol.species[mol.nblocks - 1];
+        }
+    }
+    U.Hpart = r.Hpart;
+    U.seed = d.seed;
+    U.stream = 0; // noise keyed by the replica's own seed, never by its slot
+    U.iter = b->d_iter + r.id;
+    U.cl = d.cl;
+    U.first = first;
+    U.have_rho = have_rho;
+}
+
+// species of bead s of molecule m along the forward (c = 0) or the
+// complementary (c = 1) chain
+static int bead_species(const mft_fts_molecule& mol, int Ntot, int s, int c)
+{
+    int bead = c == 0 ? s : Ntot - 1 - s;
+    int acc = 0;
+    for (int b = 0; b < mol.nblocks; ++b) {
+        acc += mol.N[b];
+        if (bead < acc) return mol.species[b];
+    }
+    return mol.species[mol.nblocks - 1];
+}
+
+static int group_setup(FtsBatch* b, FtsGroup& G)
+{
+    int rc;
+    const int nr = (int)G.reps.size();
+    // update descriptors: [0, nr) regular, [nr, 2nr) initial pass
+    std::vector<FtsUpdDev> upd(2 * nr);
+    for (int k = 0; k < nr; ++k) {
+        fill_upd(b, b->reps[G.reps[k]], upd[k], 0, 1);
+        fill_upd(b, b->reps[G.reps[k]], upd[nr + k], 1, 0);
+    }
+    if ((rc = falloc(b, &G.d_upd, upd.size()))) return rc;
+    MFT_HIP(hipMemcpy(G.d_upd, upd.data(), sizeof(FtsUpdDev) * upd.size(), hipMemcpyHostToDevice));
+
+    // density and Q descriptors: one per (replica, molecule)
+    std::vector<DensDesc> dens;
+    std::vector<QDesc> qd;
+    for (int k = 0; k < nr; ++k) {
+        FtsRep& r = b->reps[G.reps[k]];
+        for (int m = 0; m < r.d.nmolecules; ++m) {
+            const mft_fts_molecule& mol = r.d.molecules[m];
+            DensDesc D;
+            std::memset(&D, 0, sizeof(D));
+            D.p = r.p[m];
+            D.pd = r.pd[m];
+            D.Elast = r.E + (size_t)mol.species[mol.nblocks - 1] * r.M;
+            int off = 0;
+            for (int mm = 0; mm < m; ++mm) off += r.d.molecules[mm].nblocks;
+            D.rhot = r.rhot + (size_t)off * r.M;
+            D.Qpart = r.Qpart + (size_t)m * 2 * r.nQpart;
+            D.M = r.M;
+            D.Ntot = r.Ntot[m];
+            D.nblocks = mol.nblocks;
+            for (int bb = 0; bb < mol.nblocks; ++bb) D.N[bb] = mol.N[bb];
+            dens.push_back(D);
+            QDesc q;
+            q.Qpart = D.Qpart;
+            q.nparts = r.nQpart;
+            q.scale = r.gvol / r.V;
+            q.Q = r.Q + m;
+            qd.push_back(q);
+        }
+    }
+    G.ndens = (int)dens.size();
+    if ((rc = falloc(b, &G.d_dens, dens.size()))) return rc;
+    MFT_HIP(hipMemcpy(G.d_dens, dens.data(), sizeof(DensDesc) * dens.size(), hipMemcpyHostToDevice));
+    if ((rc = falloc(b, &G.d_q, qd.size()))) return rc;
+    MFT_HIP(hipMemcpy(G.d_q, qd.data(), sizeof(QDesc) * qd.size(), hipMemcpyHostToDevice));
+
+    // propagation descriptors per step and axis
+    G.maxN = 0;
+    for (int k = 0; k < nr; ++k)
+        for (int m = 0; m < b->reps[G.reps[k]].d.nmolecules; ++m) G.maxN = std::max(G.maxN, b->reps[G.reps[k]].Ntot[m]);
+    G.d_prop.assign((size_t)3 * G.maxN, nullptr);
+    G.nprop.assign(G.maxN, 0);
+    for (int s = 1; s < G.maxN; ++s) {
+        std::vector<PropDesc> ax[3];
+        for (int k = 0; k < nr; ++k) {
+            FtsRep& r = b->reps[G.reps[k]];
+            int chain = 0;
+            for (int m = 0; m < r.d.nmolecules; ++m) {
+                const mft_fts_molecule& mol = r.d.molecules[m];
+                for (int c = 0; c < (r.sym[m] ? 1 : 2); ++c, ++chain) {
+                    if (s >= r.Ntot[m]) continue;
+                    float2* P = c == 0 ? r.p[m] : r.pd[m];
+                    float2* tmp = r.tmp + (size_t)chain * r.M;
+                    const float2* Epre = r.E + (size_t)bead_species(mol, r.Ntot[m], s - 1, c) * r.M;
+                    const float2* Epost = r.E + (size_t)bead_species(mol, r.Ntot[m], s, c) * r.M;
+                    ax[0].push_back(PropDesc{P + (size_t)(s - 1) * r.M, tmp, Epre, nullptr, r.g[0]});
+                    ax[1].push_back(PropDesc{tmp, tmp, nullptr, nullptr, r.g[1]});
+                    ax[2].push_back(PropDesc{tmp, P + (size_t)s * r.M, nullptr, Epost, r.g[2]});
+                }
+            }
+        }
+        G.nprop[s] = (int)ax[0].size();
+        for (int a = 0; a < 3; ++a) {
+            PropDesc* dp = nullptr;
+            if ((rc = falloc(b, &dp, ax[a].size()))) return rc;
+            MFT_HIP(hipMemcpy(dp, ax[a].data(), sizeof(PropDesc) * ax[a].size(), hipMemcpyHostToDevice));
+            G.d_prop[(size_t)3 * s + a] = dp;
+        }
+    }
+    return MFT_OK;
+}
+
+template <int N>
+static void launch_prop_n(const PropDesc* d, int n, const FtsGroup& G, int axis, hipStream_t st)
+{
+    constexpr int B = 256 / FftPlan<N>::T;
+    AxisGeom geom{G.n[0], G.n[1], G.n[2], axis};
+    dim3 grid((geom.nlines() + B - 1) / B, n);
+    k_fts_prop_axis<N><<<grid, 256, 0, st>>>(d, G.n[0], G.n[1], G.n[2], axis, fft_twiddles_device(N));
+}
+
+static int launch_prop(const PropDesc* d, int n, const FtsGroup& G, int axis, hipStream_t st)
+{
+    switch (G.n[axis]) {
+#define MFT_CASE(n_)                                                                                      \
+    case n_: launch_prop_n<n_>(d, n, G, axis, st); break;
+        MFT_FOR_EACH_FFT_SIZE(MFT_CASE)
+#undef MFT_CASE
+    default: set_error(MFT_ERR_UNSUPPORTED, "unsupported FFT length %d", G.n[axis]); return MFT_ERR_UNSUPPORTED;
+    }
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
+// propagators, densities and Q of every replica in the group
+static int group_density(FtsBatch* b, FtsGroup& G)
+{
+    int rc;
+    hipStream_t st = b->stream;
+    for (int s = 1; s < G.maxN; ++s) {
+        if (G.nprop[s] == 0) continue;
+        for (int a = 0; a < 3; ++a)
+            if ((rc = launch_prop(G.d_prop[(size_t)3 * s + a], G.nprop[s], G, a, st))) return rc;
+    }
+    const int M = G.n[0] * G.n[1] * G.n[2];
+    dim3 grid((M + kFtsThreads - 1) / kFtsThreads, G.ndens);
+    k_fts_density<<<grid, kFtsThreads, 0, st>>>(G.d_dens);
+    k_fts_qreduce<<<G.ndens, 256, 0, st>>>(G.d_q);
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
+static int group_update(FtsBatch* b, FtsGroup& G, int first)
+{
+    const int M = G.n[0] * G.n[1] * G.n[2];
+    const int nr = (int)G.reps.size();
+    dim3 grid((M + kFtsThreads - 1) / kFtsThreads, nr);
+    k_fts_update<<<grid, kFtsThreads, 0, b->stream>>>(G.d_upd + (first ? nr : 0));
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
+static int batch_create(FtsBatch** out, const mft_fts_desc* descs, int nrep)
+{
+    MFT_REQUIRE(out && descs && nrep >= 1, MFT_ERR_INVALID, "bad argument");
+    *out = nullptr;
+    for (int r = 0; r < nrep; ++r) {
+        int rc = validate_desc(descs[r]);
+        if (rc) return rc;
+    }
+    FtsBatch* b = new FtsBatch();
+    int rc = MFT_OK;
+    b->reps.resize(nrep);
+    for (int r = 0; r < nrep && !rc; ++r) {
+        b->reps[r].d = descs[r];
+        b->reps[r].id = r;
+    }
+    if (!rc) rc = falloc(b, &b->d_iter, (size_t)nrep);
+    if (!rc) {
+        hipError_t e = hipMemset(b->d_iter, 0, sizeof(int64_t) * nrep);
+        if (e != hipSuccess) rc = MFT_ERR_DEVICE;
+    }
+    for (int r = 0; r < nrep && !rc; ++r) rc = rep_setup(b, b->reps[r]);
+    if (!rc) {
+        std::map<std::array<int, 3>, int> idx;
+        for (int r = 0; r < nrep; ++r) {
+            std::array<int, 3> key{b->reps[r].d.nx, b->reps[r].d.ny, b->reps[r].d.nz};
+            auto it = idx.find(key);
+            if (it == idx.end()) {
+                idx[key] = (int)b->groups.size();
+                FtsGroup G;
+                G.n[0] = key[0];
+                G.n[1] = key[1];
+                G.n[2] = key[2];
+                b->groups.push_back(G);
+                it = idx.find(key);
+            }
+            b->groups[it->second].reps.push_back(r);
+        }
+        for (auto& G : b->groups) {
+            for (int a = 0; a < 3; ++a)
+                if (!fft_twiddles_device(G.n[a])) rc = MFT_ERR_OOM;
+            if (!rc) rc = group_setup(b, G);
+            if (rc) break;
+        }
+    }
+    if (!rc) {
+        hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            set_error(MFT_ERR_DEVICE, "stream: %s", hipGetErrorString(e));
+            rc = MFT_ERR_DEVICE;
+        }
+    }
+    if (rc) {
+        for (void* p : b->allocs) (void)hipFree(p);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return MFT_OK;
+}
+
+static int batch_destroy(FtsBatch* b)
+{
+    if (!b) return MFT_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (void* p : b->allocs) (void)hipFree(p);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return MFT_OK;
+}
+
+static int batch_init(FtsBatch* b)
+{
+    int rc;
+    for (auto& G : b->groups) {
+        if ((rc = group_update(b, G, 1))) return rc;
+        if ((rc = group_density(b, G))) return rc;
+    }
+    b->inited = true;
+    return MFT_OK;
+}
+
+static int batch_step(FtsBatch* b, int niter)
+{
+    MFT_REQUIRE(b->inited, MFT_ERR_INVALID, "call init before stepping");
+    int rc;
+    for (int it = 0; it < niter; ++it) {
+        for (auto& G : b->groups) {
+            if ((rc = group_update(b, G, 0))) return rc;
+            if ((rc = group_density(b, G))) return rc;
+        }
+        {
+            // one counter per replica
+            int n = (int)b->reps.size();
+            for (int o = 0; o < n; o += 256) {
+                int cnt = std::min(256, n - o);
+                k_fts_iter_incr<<<1, 256, 0, b->stream>>>(b->d_iter + o, cnt);
+            }
+        }
+        MFT_LAUNCH_CHECK();
+        b->iter++;
+    }
+    return MFT_OK;
+}
+
+static int batch_set_fields(FtsBatch* b, int rep, int pot, const float* wpl, const float* wmi)
+{
+    MFT_REQUIRE(rep >= 0 && rep < (int)b->reps.size(), MFT_ERR_INVALID, "replica out of range");
+    FtsRep& r = b->reps[rep];
+    MFT_REQUIRE(pot >= 0 && pot < r.d.npotentials, MFT_ERR_INVALID, "potential out of range");
+    MFT_HIP(hipStreamSynchronize(b->stream));
+    const size_t M = r.M;
+    if (wpl) {
+        for (size_t i = 0; i < 2 * M; ++i)
+            MFT_REQUIRE(std::isfinite(wpl[i]), MFT_ERR_NONFINITE, "field value %zu is not finite", i);
+        MFT_HIP(hipMemcpy(r.wpl[pot], wpl, sizeof(float2) * M, hipMemcpyHostToDevice));
+    }
+    if (wmi) {
+        MFT_REQUIRE(r.wmi[pot], MFT_ERR_INVALID, "potential %d has no w- field", pot);
+        for (size_t i = 0; i < 2 * M; ++i)
+            MFT_REQUIRE(std::isfinite(wmi[i]), MFT_ERR_NONFINITE, "field value %zu is not finite", i);
+        MFT_HIP(hipMemcpy(r.wmi[pot], wmi, sizeof(float2) * M, hipMemcpyHostToDevice));
+    }
+    b->inited = false;
+    return MFT_OK;
+}
+
+static int batch_observables(FtsBatch* b, int rep, double* Q, double* H, double* parts)
+{
+    MFT_REQUIRE(rep >= 0 && rep < (int)b->reps.size(), MFT_ERR_INVALID, "replica out of range");
+    FtsRep& r = b->reps[rep];
+    const mft_fts_desc& d = r.d;
+    MFT_HIP(hipStreamSynchronize(b->stream));
+    std::vector<double> hp((size_t)r.nHpart * kFtsMaxPots * 6);
+    std::vector<double2> q(d.nmolecules);
+    MFT_HIP(hipMemcpy(hp.data(), r.Hpart, sizeof(double) * hp.size(), hipMemcpyDeviceToHost));
+    MFT_HIP(hipMemcpy(q.data(), r.Q, sizeof(double2) * d.nmolecules, hipMemcpyDeviceToHost));
+    // partial sums in a fixed order
+    double sums[kFtsMaxPots * 6] = {0};
+    for (int blk = 0; blk < r.nHpart; ++blk)
+        for (int k = 0; k < d.npotentials * 6; ++k) sums[k] += hp[(size_t)blk * kFtsMaxPots * 6 + k];
+    std::complex<double> Ht(0.0, 0.0);
+    const std::complex<double> I(0.0, 1.0);
+    int np = 0;
+    for (int p = 0; p < d.npotentials; ++p) {
+        const mft_fts_potential& P = d.potentials[p];
+        std::complex<double> w2(sums[p * 6 + 0], sums[p * 6 + 1]), w1(sums[p * 6 + 2], sums[p * 6 + 3]),
+            m2(sums[p * 6 + 4], sums[p * 6 + 5]);
+        std::complex<double> h;
+        if (P.kind == MFT_FTS_FLORY)  // fts_potential_flory.cu:252-274
+            h = w2 * r.gvol * d.C / P.chiN + m2 * r.gvol * d.C / P.chiN;
+        else if (P.kind == MFT_FTS_HELFAND)  // fts_potential_helfand.cu:254-275
+            h = w2 * r.gvol * d.C / 2.0 / P.kappaN - I * d.C * (w1 * r.gvol);
+        else  // fts_potential_incompress.cu:232-247
+            h = -I * d.C * (w1 * r.gvol);
+        if (parts) {
+            parts[2 * np] = h.real();
+            parts[2 * np + 1] = h.imag();
+        }
+        ++np;
+        Ht += h;
+    }
+    for (int m = 0; m < d.nmolecules; ++m) {
+        std::complex<double> Qm(q[m].x, q[m].y);
+        // LinearMolec::calcHTerm (fts_molecule_linear.cu:368-382)
+        std::complex<double> h = -r.nmolecs[m] * std::log(Qm);
+        if (Q) {
+            Q[2 * m] = Qm.real();
+            Q[2 * m + 1] = Qm.imag();
+        }
+        if (parts) {
+            parts[2 * np] = h.real();
+            parts[2 * np + 1] = h.imag();
+        }
+        ++np;
+        Ht += h;
+    }
+    if (H) {
+        H[0] = Ht.real();
+        H[1] = Ht.imag();
+    }
+    // FTS_Box.cu:150-152 dies on NaN; here it is a status
+    MFT_REQUIRE(std::isfinite(Ht.real()) && std::isfinite(Ht.imag()), MFT_ERR_NONFINITE,
+                "Found Heff = NaN (replica %d)", rep);
+    return MFT_OK;
+}
+
+static int group_of(FtsBatch* b, int rep)
+{
+    for (size_t g = 0; g < b->groups.size(); ++g)
+        for (size_t k = 0; k < b->groups[g].reps.size(); ++k)
+            if (b->groups[g].reps[k] == rep) return (int)(g * 4096 + k);
+    return -1;
+}
+
+static int batch_densities(FtsBatch* b, int rep, float* rho)
+{
+    MFT_REQUIRE(rep >= 0 && rep < (int)b->reps.size() && rho, MFT_ERR_INVALID, "bad argument");
+    FtsRep& r = b->reps[rep];
+    int gk = group_of(b, rep);
+    FtsGroup& G = b->groups[gk / 4096];
+    int k = gk % 4096;
+    float2* out = nullptr;
+    MFT_HIP(hipMalloc((void**)&out, sizeof(float2) * r.M * r.d.nspecies));
+    dim3 grid((r.M + kFtsThreads - 1) / kFtsThreads, 1);
+    k_fts_species_rho<<<grid, kFtsThreads, 0, b->stream>>>(G.d_upd + k, out);
+    MFT_HIP(hipStreamSynchronize(b->stream));
+    MFT_HIP(hipMemcpy(rho, out, sizeof(float2) * r.M * r.d.nspecies, hipMemcpyDeviceToHost));
+    MFT_HIP(hipFree(out));
+    return MFT_OK;
+}
+
+} // namespace mft
+
+using namespace mft;
+
+struct mft_fts : FtsBatch {};
+struct mft_fts_batch : FtsBatch {};
+
+__global__ void k_fts_bond_factor(float2* g2, int nx, int ny, int nz, float Lx, float Ly, float Lz, double Nr)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long M = (long)nx * ny * nz;
+    if (i >= M) return;
+    int ix = i % nx, iy = (i / nx) % ny, iz = i / ((long)nx * ny);
+    double kx = kcomp(ix, nx, Lx), ky = kcomp(iy, ny, Ly), kz = kcomp(iz, nz, Lz);
+    g2[i] = make_float2((float)exp(-(kx * kx + ky * ky + kz * kz) / (Nr - 1.0)), 0.f);
+}
+
+extern "C" {
+
+int mft_fts_bond_factor(float* g2, const int Nx[3], const float L[3], double Nr, void* stream)
+{
+    MFT_REQUIRE(g2 && Nx && L && Nr > 1.0, MFT_ERR_INVALID, "bad argument");
+    long M = (long)Nx[0] * Nx[1] * Nx[2];
+    k_fts_bond_factor<<<(unsigned)((M + 255) / 256), 256, 0, (hipStream_t)stream>>>((float2*)g2, Nx[0], Nx[1], Nx[2],
+                                                                                    L[0], L[1], L[2], Nr);
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
+int mft_fts_create(mft_fts** sys, const mft_fts_desc* desc)
+{
+    FtsBatch* b = nullptr;
+    int rc = batch_create(&b, desc, 1);
+    if (sys) *sys = (mft_fts*)b;
+    return rc;
+}
+int mft_fts_destroy(mft_fts* s) { return batch_destroy(s); }
+void* mft_fts_stream(mft_fts* s) { return s ? (void*)s->stream : nullptr; }
+int mft_fts_set_fields(mft_fts* s, int pot, const float* wpl, const float* wmi)
+{
+    MFT_REQUIRE(s, MFT_ERR_INVALID, "null system");
+    return batch_set_fields(s, 0, pot, wpl, wmi);
+}
+int mft_fts_get_fields(mft_fts* s, int pot, float* wpl, float* wmi)
+{
+    MFT_REQUIRE(s && pot >= 0 && pot < s->reps[0].d.npotentials, MFT_ERR_INVALID, "bad argument");
+    FtsRep& r = s->reps[0];
+    MFT_HIP(hipStreamSynchronize(s->stream));
+    if (wpl) MFT_HIP(hipMemcpy(wpl, r.wpl[pot], sizeof(float2) * r.M, hipMemcpyDeviceToHost));
+    if (wmi) {
+        MFT_REQUIRE(r.wmi[pot], MFT_ERR_INVALID, "potential %d has no w- field", pot);
+        MFT_HIP(hipMemcpy(wmi, r.wmi[pot], sizeof(float2) * r.M, hipMemcpyDeviceToHost));
+    }
+    return MFT_OK;
+}
+int mft_fts_init(mft_fts* s)
+{
+    MFT_REQUIRE(s, MFT_ERR_INVALID, "null system");
+    return batch_init(s);
+}
+int mft_fts_step(mft_fts* s, int niter)
+{
+    MFT_REQUIRE(s && niter >= 0, MFT_ERR_INVALID, "bad argument");
+    return batch_step(s, niter);
+}
+int mft_fts_synchronize(mft_fts* s)
+{
+    MFT_REQUIRE(s, MFT_ERR_INVALID, "null system");
+    MFT_HIP(hipStreamSynchronize(s->stream));
+    return MFT_OK;
+}
+int mft_fts_get_densities(mft_fts* s, float* rho)
+{
+    MFT_REQUIRE(s, MFT_ERR_INVALID, "null system");
+    return batch_densities(s, 0, rho);
+}
+int mft_fts_get_species_fields(mft_fts* s, float* W)
+{
+    MFT_REQUIRE(s && W, MFT_ERR_INVALID, "bad argument");
+    FtsRep& r = s->reps[0];
+    MFT_HIP(hipStreamSynchronize(s->stream));
+    MFT_HIP(hipMemcpy(W, r.W, sizeof(float2) * r.M * r.d.nspecies, hipMemcpyDeviceToHost));
+    return MFT_OK;
+}
+int mft_fts_get_propagators(mft_fts* s, int m, float* q, float* qdag)
+{
+    MFT_REQUIRE(s && m >= 0 && m < s->reps[0].d.nmolecules, MFT_ERR_INVALID, "bad argument");
+    FtsRep& r = s->reps[0];
+    MFT_HIP(hipStreamSynchronize(s->stream));
+    const mft_fts_molecule& mol = r.d.molecules[m];
+    const int Nt = r.Ntot[m];
+    const size_t M = r.M;
+    std::vector<float2> E(M * r.d.nspecies);
+    MFT_HIP(hipMemcpy(E.data(), r.E, sizeof(float2) * E.size(), hipMemcpyDeviceToHost));
+    for (int c = 0; c < 2; ++c) {
+        float* out = c == 0 ? q : qdag;
+        if (!out) continue;
+        float2* o2 = (float2*)out;
+        MFT_HIP(hipMemcpy(o2, c == 0 ? r.p[m] : r.pd[m], sizeof(float2) * M * Nt, hipMemcpyDeviceToHost));
+        // q_s = E_{b(s)} p_s
+        for (int s = 0; s < Nt; ++s) {
+            int sp = bead_species(mol, Nt, s, c == 0 || r.sym[m] ? 0 : 1);
+            for (size_t i = 0; i < M; ++i) {
+                float2 a = o2[(size_t)s * M + i], e = E[(size_t)sp * M + i];
+                o2[(size_t)s * M + i] = make_float2(a.x * e.x - a.y * e.y, a.x * e.y + a.y * e.x);
+            }
+        }
+    }
+    return MFT_OK;
+}
+int mft_fts_observables(mft_fts* s, double* Q, double* H, double* parts)
+{
+    MFT_REQUIRE(s, MFT_ERR_INVALID, "null system");
+    return batch_observables(s, 0, Q, H, parts);
+}
+
+int mft_fts_batch_create(mft_fts_batch** b, const mft_fts_desc* descs, int nrep)
+{
+    FtsBatch* x = nullptr;
+    int rc = batch_create(&x, descs, nrep);
+    if (b) *b = (mft_fts_batch*)x;
+    return rc;
+}
+int mft_fts_batch_destroy(mft_fts_batch* b) { return batch_destroy(b); }
+int mft_fts_batch_set_fields(mft_fts_batch* b, int rep, int pot, const float* wpl, const float* wmi)
+{
+    MFT_REQUIRE(b, MFT_ERR_INVALID, "null batch");
+    return batch_set_fields(b, rep, pot, wpl, wmi);
+}
+int mft_fts_batch_init(mft_fts_batch* b)
+{
+    MFT_REQUIRE(b, MFT_ERR_INVALID, "null batch");
+    return batch_init(b);
+}
+int mft_fts_batch_step(mft_fts_batch* b, int niter)
+{
+    MFT_REQUIRE(b && niter >= 0, MFT_ERR_INVALID, "bad argument");
+    return batch_step(b, niter);
+}
+int mft_fts_batch_synchronize(mft_fts_batch* b)
+{
+    MFT_REQUIRE(b, MFT_ERR_INVALID, "null batch");
+    MFT_HIP(hipStreamSynchronize(b->stream));
+    return MFT_OK;
+}
+int mft_fts_batch_observables(mft_fts_batch* b, int rep, double* Q, double* H, double* parts)
+{
+    MFT_REQUIRE(b, MFT_ERR_INVALID, "null batch");
+    return batch_observables(b, rep, Q, H, parts);
+}
+int mft_fts_batch_get_densities(mft_fts_batch* b, int rep, float* rho)
+{
+    MFT_REQUIRE(b, MFT_ERR_INVALID, "null batch");
+    return batch_densities(b, rep, rho);
+}
+
+} // extern "C"

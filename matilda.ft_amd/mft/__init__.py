@@ -323,3 +323,173 @@ class TildEngine:
         b = np.zeros(len(TK_NAMES), np.float64)
         check(lib().mft_tild_kernel_bytes(self.h, _ptr(b)))
         return dict(zip(TK_NAMES, b))
+
+
+# --------------------------------------------------------------- FTS -----
+class FtsPotentialC(C.Structure):
+    _fields_ = [("kind", _i), ("I", _i), ("J", _i), ("chiN", _d), ("kappaN", _d), ("delt_pl", _d), ("delt_mi", _d)]
+
+
+class FtsMoleculeC(C.Structure):
+    _fields_ = [("nblocks", _i), ("N", _i * 16), ("species", _i * 16), ("phi", _d)]
+
+
+class FtsDescC(C.Structure):
+    _fields_ = [("nx", _i), ("ny", _i), ("nz", _i), ("lx", _f), ("ly", _f), ("lz", _f), ("Nr", _d), ("C", _d),
+                ("nspecies", _i), ("npotentials", _i), ("potentials", FtsPotentialC * 8), ("nmolecules", _i),
+                ("molecules", FtsMoleculeC * 4), ("cl", _i), ("seed", C.c_uint64)]
+
+
+def fts_desc(spec):
+    """C description from a dict: Nx, L, Nr, C, nspecies, potentials
+    [dict(kind, I, J, chiN, kappaN, dt_pl, dt_mi)], molecules
+    [dict(blocks=[(N, species)], phi)], cl, seed."""
+    d = FtsDescC()
+    d.nx, d.ny, d.nz = spec["Nx"]
+    d.lx, d.ly, d.lz = spec["L"]
+    d.Nr = spec["Nr"]
+    d.C = spec["C"]
+    d.nspecies = spec["nspecies"]
+    d.npotentials = len(spec["potentials"])
+    for k, p in enumerate(spec["potentials"]):
+        d.potentials[k] = FtsPotentialC(FTS_POT[p["kind"]], p.get("I", -1), p.get("J", -1), p.get("chiN", 0.0),
+                                        p.get("kappaN", 0.0), p.get("dt_pl", 0.0), p.get("dt_mi", 0.0))
+    d.nmolecules = len(spec["molecules"])
+    for k, m in enumerate(spec["molecules"]):
+        mc = FtsMoleculeC()
+        mc.nblocks = len(m["blocks"])
+        for j, (n, s) in enumerate(m["blocks"]):
+            mc.N[j] = n
+            mc.species[j] = s
+        mc.phi = m.get("phi", 1.0)
+        d.molecules[k] = mc
+    d.cl = int(spec.get("cl", True))
+    d.seed = spec.get("seed", 1234)
+    return d
+
+
+def _c64(a):
+    return np.ascontiguousarray(a, np.complex64)
+
+
+class FtsSystem:
+    """One CL-FTS system on the device (mft_fts_*)."""
+
+    def __init__(self, spec):
+        self.spec = spec
+        self.h = _p()
+        self._d = fts_desc(spec)
+        check(lib().mft_fts_create(C.byref(self.h), C.byref(self._d)))
+        self.M = int(np.prod(spec["Nx"]))
+        self.npot = len(spec["potentials"])
+        self.nmol = len(spec["molecules"])
+        self.nspecies = spec["nspecies"]
+
+    def close(self):
+        if self.h:
+            lib().mft_fts_destroy(self.h)
+            self.h = _p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_fields(self, pot, wpl=None, wmi=None):
+        a = _c64(wpl) if wpl is not None else None
+        b = _c64(wmi) if wmi is not None else None
+        check(lib().mft_fts_set_fields(self.h, pot, _ptr(a), _ptr(b)))
+
+    def get_fields(self, pot, minus=False):
+        a = np.empty(self.M, np.complex64)
+        b = np.empty(self.M, np.complex64) if minus else None
+        check(lib().mft_fts_get_fields(self.h, pot, _ptr(a), _ptr(b)))
+        return (a, b) if minus else a
+
+    def init(self):
+        check(lib().mft_fts_init(self.h))
+
+    def step(self, n=1):
+        check(lib().mft_fts_step(self.h, n))
+
+    def synchronize(self):
+        check(lib().mft_fts_synchronize(self.h))
+
+    def densities(self):
+        r = np.empty(self.nspecies * self.M, np.complex64)
+        check(lib().mft_fts_get_densities(self.h, _ptr(r)))
+        return r.reshape(self.nspecies, self.M)
+
+    def species_fields(self):
+        r = np.empty(self.nspecies * self.M, np.complex64)
+        check(lib().mft_fts_get_species_fields(self.h, _ptr(r)))
+        return r.reshape(self.nspecies, self.M)
+
+    def propagators(self, m):
+        Nt = sum(n for n, _ in self.spec["molecules"][m]["blocks"])
+        q = np.empty(Nt * self.M, np.complex64)
+        qd = np.empty(Nt * self.M, np.complex64)
+        check(lib().mft_fts_get_propagators(self.h, m, _ptr(q), _ptr(qd)))
+        return q, qd
+
+    def observables(self):
+        Q = np.zeros(2 * self.nmol)
+        H = np.zeros(2)
+        parts = np.zeros(2 * (self.npot + self.nmol))
+        check(lib().mft_fts_observables(self.h, _ptr(Q), _ptr(H), _ptr(parts)))
+        return Q.view(np.complex128), complex(H[0], H[1]), parts.view(np.complex128)
+
+
+class FtsBatch:
+    """Many CL-FTS replicas advanced together (mft_fts_batch_*)."""
+
+    def __init__(self, specs):
+        self.specs = specs
+        arr = (FtsDescC * len(specs))()
+        for k, s in enumerate(specs):
+            arr[k] = fts_desc(s)
+        self._arr = arr
+        self.h = _p()
+        check(lib().mft_fts_batch_create(C.byref(self.h), arr, len(specs)))
+
+    def close(self):
+        if self.h:
+            lib().mft_fts_batch_destroy(self.h)
+            self.h = _p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_fields(self, rep, pot, wpl=None, wmi=None):
+        a = _c64(wpl) if wpl is not None else None
+        b = _c64(wmi) if wmi is not None else None
+        check(lib().mft_fts_batch_set_fields(self.h, rep, pot, _ptr(a), _ptr(b)))
+
+    def init(self):
+        check(lib().mft_fts_batch_init(self.h))
+
+    def step(self, n=1):
+        check(lib().mft_fts_batch_step(self.h, n))
+
+    def synchronize(self):
+        check(lib().mft_fts_batch_synchronize(self.h))
+
+    def observables(self, rep):
+        s = self.specs[rep]
+        nmol, npot = len(s["molecules"]), len(s["potentials"])
+        Q = np.zeros(2 * nmol)
+        H = np.zeros(2)
+        parts = np.zeros(2 * (npot + nmol))
+        check(lib().mft_fts_batch_observables(self.h, rep, _ptr(Q), _ptr(H), _ptr(parts)))
+        return Q.view(np.complex128), complex(H[0], H[1]), parts.view(np.complex128)
+
+    def densities(self, rep):
+        s = self.specs[rep]
+        M = int(np.prod(s["Nx"]))
+        r = np.empty(s["nspecies"] * M, np.complex64)
+        check(lib().mft_fts_batch_get_densities(self.h, rep, _ptr(r)))
+        return r.reshape(s["nspecies"], M)

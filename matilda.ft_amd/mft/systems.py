@@ -135,3 +135,63 @@ def tild_engine(cfg, integrator="GJF"):
                      topo["bondedTo"], topo["bondType"])
     eng.set_state(cfg["x"])
     return eng
+
+
+def fts_diblock(grid=64, L=8.0, Nr=100.0, NA=50, NB=50, chiN=20.0, kappaN=100.0, C=50.0, dt=0.01, sigma=0.1,
+                seed=1234, cl=True, grid_shape=None):
+    """Plain CL-FTS AB diblock melt (config 3 shape by default): Helfand then
+    Flory potentials, real parts of w+ (Helfand), w+ and w- (Flory) from
+    N(0, sigma^2), imaginary parts zero."""
+    Nx = tuple(grid_shape) if grid_shape is not None else (grid, grid, grid)
+    Ls = tuple(L) if np.ndim(L) else (L, L, L)
+    spec = dict(Nx=Nx, L=Ls, Nr=Nr, C=C, nspecies=2, cl=cl, seed=seed,
+                potentials=[dict(kind="Helfand", kappaN=kappaN, dt_pl=dt),
+                            dict(kind="Flory", I=0, J=1, chiN=chiN, dt_pl=dt, dt_mi=dt)],
+                molecules=[dict(blocks=[(NA, 0), (NB, 1)], phi=1.0)])
+    rng = np.random.default_rng(seed)
+    M = int(np.prod(Nx))
+    fields = {
+        (0, "pl"): (rng.normal(0.0, sigma, M)).astype(np.complex64),
+        (1, "pl"): (rng.normal(0.0, sigma, M)).astype(np.complex64),
+        (1, "mi"): (rng.normal(0.0, sigma, M)).astype(np.complex64),
+    }
+    return spec, fields
+
+
+def fts_sweep_replicas(n=256, seed=7, iterations=1000):
+    """Batched sweep (config 4 shape): grids from {24,32,48,64}^3, box
+    4-10 Rg, chiN in [10,20], f_A in [0.3,0.5], Ns in {50,100}, kappaN = 100,
+    C = 50; one seed per replica."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for r in range(n):
+        g = int(rng.choice([24, 32, 48, 64]))
+        Lr = float(rng.uniform(4.0, 10.0))
+        chiN = float(rng.uniform(10.0, 20.0))
+        fA = float(rng.uniform(0.3, 0.5))
+        Ns = int(rng.choice([50, 100]))
+        NA = int(round(fA * Ns))
+        spec, fields = fts_diblock(grid=g, L=Lr, Nr=float(Ns), NA=NA, NB=Ns - NA, chiN=chiN, seed=seed * 100003 + r)
+        out.append((spec, fields))
+    return out
+
+
+def fts_system(spec, fields):
+    """Build and initialise an FtsSystem."""
+    from . import FtsSystem
+    s = FtsSystem(spec)
+    load_fields(s.set_fields, spec, fields)
+    s.init()
+    return s
+
+
+def load_fields(setter, spec, fields, rep=None):
+    for p, pot in enumerate(spec["potentials"]):
+        wpl = fields.get((p, "pl"))
+        wmi = fields.get((p, "mi"))
+        if wpl is None and wmi is None:
+            continue
+        if rep is None:
+            setter(p, wpl, wmi)
+        else:
+            setter(rep, p, wpl, wmi)

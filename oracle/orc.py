@@ -660,3 +660,19 @@ def fts_step(sys: FtsSystem, z=None, bits=64):
     fts_update_fields(sys, z)
     fts_build_species_fields(sys)
     fts_molecule_densities(sys, bits)
+
+
+def fts_system_from_spec(spec, fields):
+    """FtsSystem for a CL-FTS spec of mft.systems (fields as complex128 copies)."""
+    M = int(np.prod(spec["Nx"]))
+    pots = []
+    for k, p in enumerate(spec["potentials"]):
+        fp = FtsPotential(kind=p["kind"], I=p.get("I", -1), J=p.get("J", -1), chiN=p.get("chiN", 0.0),
+                          kappaN=p.get("kappaN", 0.0), dt_pl=p.get("dt_pl", 0.0), dt_mi=p.get("dt_mi", 0.0))
+        fp.wpl = np.array(fields.get((k, "pl"), np.zeros(M)), np.complex128)
+        if p["kind"] == "Flory":
+            fp.wmi = np.array(fields.get((k, "mi"), np.zeros(M)), np.complex128)
+        pots.append(fp)
+    mols = [FtsMolecule(blocks=list(m["blocks"]), phi=m.get("phi", 1.0)) for m in spec["molecules"]]
+    return FtsSystem(Nx=tuple(spec["Nx"]), L=tuple(spec["L"]), Nr=spec["Nr"], C=spec["C"], nspecies=spec["nspecies"],
+                     potentials=pots, molecules=mols)

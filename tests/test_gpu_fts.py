@@ -1,0 +1,180 @@
+"""GPU parity of the CL-FTS path: the HIP engine through the C ABI against
+the FP64 oracle on the same seeded fields and noise.
+
+Tolerances (complex FP32 fields, FP64 reductions): fields and densities
+within 2e-5 normwise relative error, rel(a, b) = max|a - b| / max|b|; log Q
+within 1e-6 * max(1, |log Q|) of the FP64 oracle.  H = sum_i H_i carries
+-n log Q, so its error budget is 1e-6 * sum_i |H_i| plus n times the log Q
+budget.  Each comparison prints the error next to that of the FP32 host build
+of the same oracle.  Trajectories are compared step by step from identical
+state and noise (CL dynamics is chaotic, so long runs are compared by
+invariants).
+"""
+import numpy as np
+import pytest
+
+import orc
+from mft import systems
+from test_oracle_fts import random_fields, tiny_spec
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL = 2e-5
+
+
+def rel(a, b):
+    a = np.asarray(a).astype(np.complex128)
+    b = np.asarray(b).astype(np.complex128)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def logq_err(Qg, Qo):
+    lg, lo = np.log(Qg), np.log(Qo)
+    return abs(lg - lo) / max(1.0, abs(lo))
+
+
+def build(gpu, spec, fields):
+    s = gpu.FtsSystem(spec)
+    systems.load_fields(s.set_fields, spec, {k: np.asarray(v, np.complex64) for k, v in fields.items()})
+    s.init()
+    o = orc.fts_system_from_spec(spec, {k: np.asarray(v, np.complex64).astype(np.complex128) for k, v in fields.items()})
+    orc.fts_init(o)
+    return s, o
+
+
+def compare(s, o, spec, what="", o32=None):
+    Qg, Hg, parts_g = s.observables()
+    for m in range(len(spec["molecules"])):
+        e = logq_err(Qg[m], o.Q[m])
+        e32 = logq_err(o32.Q[m], o.Q[m]) if o32 is not None else float("nan")
+        say(f"  {what}: log Q error device {e:.2e}, FP32 host build {e32:.2e}")
+        assert e < 1e-6, (what, Qg[m], o.Q[m])
+    Ho, parts_o = orc.fts_hamiltonian(o)
+    budget = 1e-6 * sum(abs(h) for h in parts_o)
+    budget += sum(m.nmolecs * 1e-6 * max(1.0, abs(np.log(Q))) for m, Q in zip(o.molecules, o.Q))
+    assert abs(Hg - Ho) <= budget, (what, Hg, Ho, budget)
+    assert rel(s.densities(), o.rho) < FIELD_TOL, what
+    assert rel(s.species_fields(), o.W) < FIELD_TOL, what
+    for k, p in enumerate(spec["potentials"]):
+        if p["kind"] == "Flory":
+            wp, wm = s.get_fields(k, minus=True)
+            assert rel(wm, o.potentials[k].wmi) < FIELD_TOL, what
+        else:
+            wp = s.get_fields(k)
+        assert rel(wp, o.potentials[k].wpl) < FIELD_TOL, what
+
+
+@pytest.mark.parametrize("blocks", [((4, 0), (3, 1)), ((2, 0), (3, 1), (2, 0)), ((6, 1),)])
+def test_init_matches_oracle(gpu, blocks):
+    spec = tiny_spec(Nx=(16, 12, 20), L=(4.0, 3.0, 5.0), blocks=blocks)
+    say("case: fts init", blocks)
+    f = random_fields(spec, 0.2, seed=3)
+    s, o = build(gpu, spec, f)
+    o32 = orc.fts_system_from_spec(spec, {k: np.asarray(v, np.complex64).astype(np.complex128) for k, v in f.items()})
+    orc.fts_init(o32, bits=32)
+    compare(s, o, spec, "init", o32)
+    # propagators in the reference's q-form
+    q, qd = s.propagators(0)
+    ref = orc.fts_molecule_densities(o, keep=True)[0]
+    assert rel(q, ref["q"]) < FIELD_TOL
+    assert rel(qd, ref["qdag"]) < FIELD_TOL
+    s.close()
+
+
+def test_closed_forms_on_device(gpu):
+    say("case: fts closed forms")
+    spec = tiny_spec(Nx=(16, 16, 16), L=(4.0, 4.0, 4.0), pots=[dict(kind="Incompress", dt_pl=0.01)])
+    M = 16 ** 3
+    s = gpu.FtsSystem(spec)
+    s.init()
+    Q, H, _ = s.observables()
+    assert abs(Q[0] - 1.0) < 1e-6
+    c = 0.21
+    s.set_fields(0, np.full(M, -1j * c * spec["Nr"], np.complex64))
+    s.init()
+    Q, H, _ = s.observables()
+    assert abs(np.log(Q[0]) - (-c * 7)) < 1e-6 * 7 * c
+    s.close()
+
+
+@pytest.mark.parametrize("case", ["diblock", "incompress_blend"])
+def test_cl_steps_match_oracle(gpu, case):
+    say("case: fts CL steps", case)
+    if case == "diblock":
+        spec, fields = systems.fts_diblock(grid_shape=(16, 24, 12), L=(3.0, 4.0, 2.5), Nr=20.0, NA=8, NB=12,
+                                           chiN=15.0, kappaN=50.0, C=5.0, dt=0.01, sigma=0.1, seed=21)
+    else:
+        spec = tiny_spec(Nx=(12, 16, 20), L=(3.0, 4.0, 5.0), pots=[dict(kind="Incompress", dt_pl=0.01),
+                                                                dict(kind="Flory", I=0, J=1, chiN=10.0, dt_pl=0.01,
+                                                                     dt_mi=0.01)])
+        spec["molecules"] = [dict(blocks=[(4, 0), (3, 1)], phi=0.6), dict(blocks=[(5, 1)], phi=0.4)]
+        fields = random_fields(spec, 0.1, seed=8, imag=False)
+    s, o = build(gpu, spec, fields)
+    compare(s, o, spec, "init")
+    M = int(np.prod(spec["Nx"]))
+    for it in range(3):
+        s.step(1)
+        orc.fts_step(o, orc.fts_noise(M, spec["seed"], it))
+        compare(s, o, spec, f"step {it}")
+    s.close()
+
+
+def test_batch_equals_alone(gpu):
+    """Replicas of uneven grids and chain lengths advanced together give the
+    same bits as each replica alone."""
+    say("case: fts batch vs alone")
+    reps = []
+    for k, (g, L, NA, NB) in enumerate([((24, 24, 24), 5.0, 10, 10), ((16, 16, 16), 4.0, 6, 4),
+                                         ((24, 24, 24), 6.0, 12, 8), ((32, 32, 32), 7.0, 5, 15)]):
+        reps.append(systems.fts_diblock(grid_shape=g, L=L, Nr=float(NA + NB), NA=NA, NB=NB, chiN=12.0, kappaN=60.0,
+                                        C=4.0, dt=0.01, sigma=0.1, seed=100 + k))
+    b = gpu.FtsBatch([r[0] for r in reps])
+    for k, (spec, fields) in enumerate(reps):
+        systems.load_fields(b.set_fields, spec, fields, rep=k)
+    b.init()
+    b.step(4)
+    for k, (spec, fields) in enumerate(reps):
+        s = systems.fts_system(spec, fields)
+        s.step(4)
+        Qa, Ha, _ = s.observables()
+        Qb, Hb, _ = b.observables(k)
+        assert Qa[0] == Qb[0] and Ha == Hb, k
+        assert np.array_equal(s.densities(), b.densities(k)), k
+        s.close()
+    b.close()
+
+
+def test_config3_size_invariants(gpu):
+    """64^3, 100-bead diblock: mass conservation and finite H over a short run."""
+    say("case: fts 64^3 Ns=100 invariants")
+    spec, fields = systems.fts_diblock()
+    s = systems.fts_system(spec, fields)
+    s.step(5)
+    Q, H, parts = s.observables()
+    assert np.isfinite(H)
+    rho = s.densities()
+    L = spec["L"]
+    gvol = np.prod([float(np.float32(l)) / n for l, n in zip(L, spec["Nx"])])
+    V = np.prod([float(np.float32(l)) for l in L])
+    n = spec["C"] * 1.0 * V * spec["Nr"] / 100
+    assert rho.astype(np.complex128).sum() * gvol == pytest.approx(n * 100, rel=1e-5)
+    s.close()
+
+
+def test_config3_log_q_against_oracle(gpu):
+    """64^3, 100-bead diblock at the benchmark shape: log Q and densities of
+    the initial state against the FP64 oracle (and the FP32 host build)."""
+    say("case: fts 64^3 Ns=100 log Q vs oracle")
+    spec, fields = systems.fts_diblock()
+    s = systems.fts_system(spec, fields)
+    f64 = {k: np.asarray(v, np.complex64).astype(np.complex128) for k, v in fields.items()}
+    o = orc.fts_system_from_spec(spec, f64)
+    orc.fts_init(o)
+    o32 = orc.fts_system_from_spec(spec, f64)
+    orc.fts_init(o32, bits=32)
+    compare(s, o, spec, "C3 init", o32)
+    s.close()

@@ -1,0 +1,171 @@
+"""Pins the CL-FTS oracle with closed forms and invariants.  CPU only."""
+import numpy as np
+import pytest
+
+import orc
+from mft import systems
+
+
+def tiny_spec(Nx=(8, 6, 10), L=(3.0, 2.5, 4.0), blocks=((4, 0), (3, 1)), Nr=8.0, C=3.0, pots=None):
+    if pots is None:
+        pots = [dict(kind="Helfand", kappaN=30.0, dt_pl=0.02),
+                dict(kind="Flory", I=0, J=1, chiN=12.0, dt_pl=0.02, dt_mi=0.03)]
+    return dict(Nx=Nx, L=L, Nr=Nr, C=C, nspecies=2, cl=True, seed=5, potentials=pots,
+                molecules=[dict(blocks=list(blocks), phi=1.0)])
+
+
+def random_fields(spec, scale, seed=1, imag=True):
+    rng = np.random.default_rng(seed)
+    M = int(np.prod(spec["Nx"]))
+    f = {}
+    for k, p in enumerate(spec["potentials"]):
+        f[(k, "pl")] = rng.normal(0, scale, M) + (1j * rng.normal(0, scale, M) if imag else 0)
+        if p["kind"] == "Flory":
+            f[(k, "mi")] = rng.normal(0, scale, M) + (1j * rng.normal(0, scale, M) if imag else 0)
+    return f
+
+
+def test_zero_field_gives_unit_Q_and_uniform_density():
+    spec = tiny_spec()
+    s = orc.fts_system_from_spec(spec, {})
+    orc.fts_init(s)
+    assert abs(s.Q[0] - 1.0) < 1e-12
+    Ntot = 7
+    n = s.molecules[0].nmolecs
+    total = s.rho.sum(axis=0)
+    np.testing.assert_allclose(total, n * Ntot / s.V(), rtol=1e-12)
+
+
+def test_constant_field_closed_form():
+    # Incompress w+ = -i c Nr makes W = c on every species: q_s = e^{-c(s+1)}
+    c = 0.37
+    spec = tiny_spec(pots=[dict(kind="Incompress", dt_pl=0.01)])
+    M = int(np.prod(spec["Nx"]))
+    s = orc.fts_system_from_spec(spec, {(0, "pl"): np.full(M, -1j * c * spec["Nr"])})
+    orc.fts_init(s)
+    assert np.log(s.Q[0]) == pytest.approx(-c * 7, rel=1e-12)
+
+
+def debye_S(k2, Nr, Ntot):
+    g = np.exp(-k2 / (Nr - 1.0))
+    i = np.arange(Ntot)
+    return np.sum(g ** np.abs(i[:, None] - i[None, :]))
+
+
+def test_linear_response_is_discrete_chain_debye_function():
+    # W = eps cos(k x) on every bead: delta rho(r) = -(n/V) S(k) eps cos(k x),
+    # S(k) = sum_ij g^|i-j|, g = exp(-k^2/(Nr-1)), to first order in eps
+    eps = 1e-5
+    Nx, L = (16, 4, 4), (6.0, 2.0, 2.0)
+    spec = tiny_spec(Nx=Nx, L=L, blocks=((9, 0),), Nr=10.0, pots=[dict(kind="Incompress", dt_pl=0.01)])
+    x = np.arange(Nx[0]) * L[0] / Nx[0]
+    k = float(np.float32(np.float32(6.2831853071) * np.float32(1.0)) / np.float32(L[0]))
+    W = eps * np.cos(k * x)
+    Wfull = np.broadcast_to(W[None, None, :], (Nx[2], Nx[1], Nx[0])).reshape(-1)
+    s = orc.fts_system_from_spec(spec, {(0, "pl"): -1j * Wfull * spec["Nr"]})
+    orc.fts_init(s)
+    n = s.molecules[0].nmolecs
+    rho = s.rho.sum(axis=0).reshape(Nx[2], Nx[1], Nx[0])[0, 0, :]
+    drho = rho.real - n * 9 / s.V()
+    want = -(n / s.V()) * debye_S(k * k, spec["Nr"], 9) * eps * np.cos(k * x)
+    np.testing.assert_allclose(drho, want, rtol=0, atol=2e-4 * np.max(np.abs(want)))
+
+
+def test_density_integrates_to_total_sites():
+    spec = tiny_spec()
+    s = orc.fts_system_from_spec(spec, random_fields(spec, 0.3))
+    orc.fts_init(s)
+    total = s.rho.sum() * s.gvol()
+    assert total == pytest.approx(s.molecules[0].nmolecs * 7, rel=1e-10)
+
+
+def test_symmetric_molecule_complementary_equals_forward():
+    spec = tiny_spec(blocks=((2, 0), (3, 1), (2, 0)))
+    s = orc.fts_system_from_spec(spec, random_fields(spec, 0.3))
+    orc.fts_build_species_fields(s)
+    s.init_nmolecs()
+    out = orc.fts_molecule_densities(s, keep=True)[0]
+    np.testing.assert_array_equal(out["q"], out["qdag"])
+    # reversing the block order of an asymmetric chain swaps q and qdag
+    spec2 = tiny_spec(blocks=((4, 0), (3, 1)))
+    spec3 = tiny_spec(blocks=((3, 1), (4, 0)))
+    f = random_fields(spec2, 0.3, seed=4)
+    a = orc.fts_system_from_spec(spec2, f)
+    b = orc.fts_system_from_spec(spec3, f)
+    for t in (a, b):
+        orc.fts_build_species_fields(t)
+        t.init_nmolecs()
+    oa = orc.fts_molecule_densities(a, keep=True)[0]
+    ob = orc.fts_molecule_densities(b, keep=True)[0]
+    np.testing.assert_allclose(oa["q"], ob["qdag"], rtol=1e-12)
+    assert a.Q[0] == pytest.approx(b.Q[0], rel=1e-12)
+
+
+@pytest.mark.parametrize("which", [(0, "pl"), (1, "pl"), (1, "mi")])
+def test_forces_are_field_derivatives_of_H(which):
+    # the CL drift dH/dw(r) (fts_potential_*.cu force kernels) equals
+    # (1/gvol) dH/dw(r) of the Hamiltonian (FTS_Box::computeHamiltonian); H
+    # is holomorphic in w, so a real finite difference gives the derivative
+    spec = tiny_spec()
+    f = random_fields(spec, 0.2, seed=7)
+    s = orc.fts_system_from_spec(spec, f)
+    orc.fts_init(s)
+    M = s.M
+    gvol = s.gvol()
+    rho_tot = s.rho.sum(axis=0)
+    p = s.potentials[which[0]]
+    if p.kind == "Helfand":
+        dH = np.zeros(M, complex)
+        orc.lib().orc_fts_helfand_force(orc._ptr(dH), orc._ptr(p.wpl), orc._ptr(rho_tot), s.C, p.kappaN, s.Nr, M)
+    else:
+        dHp = np.zeros(M, complex)
+        dHm = np.zeros(M, complex)
+        rI = np.ascontiguousarray(s.rho[p.I])
+        rJ = np.ascontiguousarray(s.rho[p.J])
+        orc.lib().orc_fts_flory_force(orc._ptr(dHp), orc._ptr(dHm), orc._ptr(p.wpl), orc._ptr(p.wmi), orc._ptr(rI),
+                                      orc._ptr(rJ), s.C, p.chiN, s.Nr, M)
+        dH = dHp if which[1] == "pl" else dHm
+    h = 1e-5
+    for site in (0, 17, M - 1):
+        vals = []
+        for sgn in (1, -1):
+            f2 = {k: v.copy() for k, v in f.items()}
+            f2[which][site] += sgn * h
+            t = orc.fts_system_from_spec(spec, f2)
+            orc.fts_init(t)
+            vals.append(orc.fts_hamiltonian(t)[0])
+        deriv = (vals[0] - vals[1]) / (2 * h) / gvol
+        assert abs(deriv - dH[site]) < 1e-5 * max(1.0, abs(dH[site])), (site, deriv, dH[site])
+
+
+def test_em_with_zero_step_is_identity():
+    spec = tiny_spec()
+    for p in spec["potentials"]:
+        p["dt_pl"] = 0.0
+        p["dt_mi"] = 0.0
+    f = random_fields(spec, 0.2)
+    s = orc.fts_system_from_spec(spec, f)
+    orc.fts_init(s)
+    orc.fts_update_fields(s, z=None)
+    np.testing.assert_array_equal(s.potentials[0].wpl, f[(0, "pl")])
+    np.testing.assert_array_equal(s.potentials[1].wmi, f[(1, "mi")])
+
+
+def test_cl_noise_is_real_with_variance_2dt_over_gvol():
+    spec = tiny_spec()
+    spec["potentials"][1]["chiN"] = 1e30   # switch off the Flory drift
+    spec["potentials"] = spec["potentials"][1:]
+    M = int(np.prod(spec["Nx"]))
+    s = orc.fts_system_from_spec(spec, {})
+    orc.fts_init(s)
+    s.rho[:] = s.rho.mean()          # uniform densities: zero Flory w- drift
+    w0 = s.potentials[0].wmi.copy()
+    incs = []
+    for it in range(40):
+        z = orc.fts_noise(M, 11, it)
+        orc.fts_update_fields(s, z)
+        incs.append(s.potentials[0].wmi - w0)
+        w0 = s.potentials[0].wmi.copy()
+    incs = np.array(incs)
+    assert np.all(incs.imag == 0)
+    assert np.var(incs.real) == pytest.approx(2 * 0.03 / s.gvol(), rel=0.05)
