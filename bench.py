@@ -55,6 +55,78 @@ def cpu_baseline(seed):
             "seconds": dt}
 
 
+def fts_bytes(M, Ns):
+    """Algorithmic bytes of one CL iteration of an AB diblock with complex
+    FP32 fields: M * 8 * (8 Ns + 12)."""
+    return M * 8.0 * (8 * Ns + 12)
+
+
+def fts_measure(iters_plain=20, iters_batch=5, nrep=256, seed=7):
+    """Plain CL-FTS (64^3, Ns = 100, 2000-iteration solves) and the batched
+    sweep of nrep uneven replicas (1000-iteration solves) against the same
+    replicas run one at a time."""
+    import numpy as np
+    import mft
+    from mft import systems
+
+    out = {}
+    spec, fields = systems.fts_diblock()
+    s = systems.fts_system(spec, fields)
+    s.step(2)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(iters_plain)
+    s.synchronize()
+    t_it = (time.perf_counter() - t0) / iters_plain
+    Q, H, _ = s.observables()
+    s.close()
+    M = int(np.prod(spec["Nx"]))
+    out["plain_64cubed_Ns100"] = {"ms_per_iteration": 1e3 * t_it, "solves_per_hour": 3600.0 / (2000 * t_it),
+                                  "hbm_frac": fts_bytes(M, 100) / t_it / 1e9 / HBM_PEAK_GBS,
+                                  "H_finite": bool(np.isfinite(H))}
+
+    reps = systems.fts_sweep_replicas(nrep, seed=seed)
+    # one replica alone per (grid, Ns) class, timed on the plain path
+    alone = {}
+    for spec_r, fields_r in reps:
+        key = (spec_r["Nx"][0], sum(n for n, _ in spec_r["molecules"][0]["blocks"]))
+        if key in alone:
+            continue
+        a = systems.fts_system(spec_r, fields_r)
+        a.step(2)
+        a.synchronize()
+        t0 = time.perf_counter()
+        a.step(iters_plain)
+        a.synchronize()
+        alone[key] = (time.perf_counter() - t0) / iters_plain
+        a.close()
+    t_alone_total = sum(alone[(sp["Nx"][0], sum(n for n, _ in sp["molecules"][0]["blocks"]))] for sp, _ in reps)
+
+    b = mft.FtsBatch([r[0] for r in reps])
+    for k, (spec_r, fields_r) in enumerate(reps):
+        systems.load_fields(b.set_fields, spec_r, fields_r, rep=k)
+    b.init()
+    b.step(1)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step(iters_batch)
+    b.synchronize()
+    t_batch = (time.perf_counter() - t0) / iters_batch
+    finite = all(np.isfinite(b.observables(k)[1]) for k in range(0, nrep, 17))
+    b.close()
+    bytes_it = sum(fts_bytes(int(np.prod(sp["Nx"])), sum(n for n, _ in sp["molecules"][0]["blocks"])) for sp, _ in reps)
+    out["batched_sweep"] = {
+        "replicas": nrep, "iterations_per_solve": 1000,
+        "ms_per_iteration_all_replicas": 1e3 * t_batch,
+        "solves_per_hour": nrep * 3600.0 / (1000 * t_batch),
+        "plain_one_at_a_time_solves_per_hour": nrep * 3600.0 / (1000 * t_alone_total),
+        "speedup_vs_plain": t_alone_total / t_batch,
+        "hbm_frac": bytes_it / t_batch / 1e9 / HBM_PEAK_GBS,
+        "H_finite_sampled": bool(finite),
+    }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,6 +134,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fts", action="store_true", help="skip the CL-FTS plain/batched measurements")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -127,6 +200,8 @@ def main():
         "cpu_baseline": None,
     }
     eng.close()
+    if not args.no_fts:
+        out["fts"] = fts_measure()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.seed)
     print(json.dumps(out), flush=True)
