@@ -149,6 +149,7 @@ def main():
     N = cfg["topo"]["type"].size
     M = int(cfg["Nx"][0] * cfg["Nx"][1] * cfg["Nx"][2])
     eng = systems.tild_engine(cfg, "GJF")
+    tuned = eng.autotune()
     eng.step(args.warmup)
     eng.synchronize()
 
@@ -194,6 +195,7 @@ def main():
         "hbm_frac_step": hbm_step / HBM_PEAK_GBS,
         "step_algorithmic_gbs": hbm_step,
         "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step_ms.items()},
+        "autotune": {"gather": ["direct", "lds_bricks"][tuned]},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes": kbytes[dom],
                      "avg_launch_ms": avg_ms[dom], "traffic": traffic},

@@ -178,6 +178,10 @@ int mft_tild_timing(mft_tild* eng, int enable, double* avg_ms, int64_t* launches
 /* Algorithmic HBM bytes per launch of each kernel id at this configuration. */
 int mft_tild_kernel_bytes(mft_tild* eng, double* bytes);
 int mft_tild_num_outputs(mft_tild* eng, int* nout);
+/* Times the kernel variants of this configuration on the device (reps
+ * launches each, state unchanged) and keeps the fastest; choice receives the
+ * selected gather variant (0 direct loads, 1 LDS-staged bricks). */
+int mft_tild_autotune(mft_tild* eng, int reps, int* choice);
 
 /* =============================================================== FTS ==== */
 /* Per-kernel entry points (complex interleaved float fields [M]). */

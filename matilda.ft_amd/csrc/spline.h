@@ -54,6 +54,24 @@ __device__ __forceinline__ int base_index_weights(float x, float dx, int p, floa
     return g;
 }
 
+// Same index, weights from s = gdx * (1/dx): only the index needs the
+// correctly rounded division; the weights are compared within tolerance.
+__device__ __forceinline__ int base_index_weights_fast(float x, float dx, float inv_dx, int p, float* W)
+{
+    int g;
+    float gdx;
+    if ((p & 1) == 0) {
+        float s = __fadd_rn(x, __fmul_rn(0.5f, dx));
+        g = (int)__fdiv_rn(s, dx);
+        gdx = fmaf(-(float)g, dx, x);
+    } else {
+        g = (int)__fdiv_rn(x, dx);
+        gdx = fmaf(-((float)g + 0.5f), dx, x);
+    }
+    spline_weights(gdx * inv_dx, p, W);
+    return g;
+}
+
 __device__ __forceinline__ int base_index(float x, float dx, int p)
 {
     if ((p & 1) == 0) return (int)__fdiv_rn(__fadd_rn(x, __fmul_rn(0.5f, dx)), dx);

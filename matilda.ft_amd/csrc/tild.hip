@@ -49,6 +49,8 @@ struct KParams {
     int qchan;          // channel index of the charge density or -1
     int nforce;         // species receiving grid forces
     int force_species[kMaxSpecies];
+    int fold;           // 1: every species has one charge; rho_q = sum_s q_s rho_s
+    float qspec[kMaxSpecies];
     int energy;         // accumulate potential energies
 };
 
@@ -251,26 +253,6 @@ __global__ void k_sum_partials(double* __restrict__ out, const double* __restric
     }
 }
 
-// ------------------------------------------------------------ binning ---
-__device__ __forceinline__ int cell_of(float x0, float x1, float x2, float dx0, float dx1, float dx2, int p,
-                                       int nx, int ny, int nz)
-{
-    int cx = wrap(base_index(x0, dx0, p), nx);
-    int cy = wrap(base_index(x1, dx1, p), ny);
-    int cz = wrap(base_index(x2, dx2, p), nz);
-    return cx + nx * (cy + ny * cz);
-}
-
-__global__ void k_bin_count(const float* __restrict__ x, int ns, int* __restrict__ count, int* __restrict__ cell,
-                            int* __restrict__ rank, float dx0, float dx1, float dx2, int p, int nx, int ny, int nz)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ns) return;
-    int c = cell_of(x[i * 3], x[i * 3 + 1], x[i * 3 + 2], dx0, dx1, dx2, p, nx, ny, nz);
-    cell[i] = c;
-    rank[i] = atomicAdd(&count[c], 1);
-}
-
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanChunk = kScanThreads * kScanItems;
@@ -295,6 +277,26 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* sh, int* total)
     __syncthreads();
     *total = tot;
     return woff + inc - v;
+}
+
+// ------------------------------------------------------------ binning ---
+__device__ __forceinline__ int cell_of(float x0, float x1, float x2, float dx0, float dx1, float dx2, int p,
+                                       int nx, int ny, int nz)
+{
+    int cx = wrap(base_index(x0, dx0, p), nx);
+    int cy = wrap(base_index(x1, dx1, p), ny);
+    int cz = wrap(base_index(x2, dx2, p), nz);
+    return cx + nx * (cy + ny * cz);
+}
+
+__global__ void k_bin_count(const float* __restrict__ x, int ns, int* __restrict__ count, int* __restrict__ cell,
+                            int* __restrict__ rank, float dx0, float dx1, float dx2, int p, int nx, int ny, int nz)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    int c = cell_of(x[i * 3], x[i * 3 + 1], x[i * 3 + 2], dx0, dx1, dx2, p, nx, ny, nz);
+    cell[i] = c;
+    rank[i] = atomicAdd(&count[c], 1);
 }
 
 __global__ void k_scan_reduce(const int* __restrict__ count, int M, int* __restrict__ partial)
@@ -368,7 +370,8 @@ __global__ void k_bin_scatter(const float* __restrict__ x, const float* __restri
 
 // ------------------------------------------------------- brick density ---
 struct BrickGeom {
-    int bx, by, bz;      // brick extent in grid points
+    int bx, by, bz;      // brick extent in grid points (powers of two)
+    int sx, sy, sz;      // log2 of the extents
     int nbx, nby, nbz;   // bricks per axis
 };
 
@@ -400,20 +403,22 @@ __device__ __forceinline__ int axis_window(int g0, int B, int p, int n, int* lo,
 constexpr int kDensThreads = 256;
 constexpr int kMaxSegs = 2 * 40 * 40;
 
+template <int P>
 __global__ void __launch_bounds__(kDensThreads) k_density_brick(
     float* __restrict__ rho, int nchan, int qchan, const float4* __restrict__ spos, const int2* __restrict__ smeta,
-    const int* __restrict__ start, int nx, int ny, int nz, float dx0, float dx1, float dx2, int p, BrickGeom bg,
+    const int* __restrict__ start, int nx, int ny, int nz, float dx0, float dx1, float dx2, BrickGeom bg,
     float inv_gvol)
 {
+    constexpr int p = P;
     extern __shared__ unsigned long long acc[]; // [nchan][bx*by*bz]
     __shared__ int seg_begin[512];
     __shared__ int seg_pref[513];
-    __shared__ int nseg_sh;
 
     const int bid = blockIdx.x;
     const int ibx = bid % bg.nbx, iby = (bid / bg.nbx) % bg.nby, ibz = bid / (bg.nbx * bg.nby);
     const int gx0 = ibx * bg.bx, gy0 = iby * bg.by, gz0 = ibz * bg.bz;
     const int npts = bg.bx * bg.by * bg.bz;
+    const float idx0 = 1.0f / dx0, idx1 = 1.0f / dx1, idx2 = 1.0f / dx2;
 
     for (int i = threadIdx.x; i < nchan * npts; i += blockDim.x) acc[i] = 0ull;
 
@@ -427,7 +432,6 @@ __global__ void __launch_bounds__(kDensThreads) k_density_brick(
     int nrows = nyc * nzc;
 
     // one segment per (row, x run): particles of cells [xlo, xlo+xlen) in row (cy, cz)
-    if (threadIdx.x == 0) nseg_sh = 0;
     __syncthreads();
     for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
         int iy = r % nyc, iz = r / nyc;
@@ -442,15 +446,18 @@ __global__ void __launch_bounds__(kDensThreads) k_density_brick(
         }
     }
     __syncthreads();
-    int nseg = nrows * nxs;
-    if (threadIdx.x == 0) {
-        int run = 0;
-        for (int k = 0; k < nseg; ++k) {
-            int l = seg_pref[k];
-            seg_pref[k] = run;
-            run += l;
-        }
-        seg_pref[nseg] = run;
+    const int nseg = nrows * nxs;
+    {
+        // exclusive prefix of the segment lengths, two entries per thread
+        __shared__ int sh[16];
+        int k0 = 2 * threadIdx.x, k1 = k0 + 1;
+        int l0 = k0 < nseg ? seg_pref[k0] : 0, l1 = k1 < nseg ? seg_pref[k1] : 0;
+        int tot;
+        int ex = block_exclusive_scan(l0 + l1, sh, &tot);
+        __syncthreads();
+        if (k0 < nseg) seg_pref[k0] = ex;
+        if (k1 < nseg) seg_pref[k1] = ex + l0;
+        if (threadIdx.x == 0) seg_pref[nseg] = tot;
     }
     __syncthreads();
     const int total = seg_pref[nseg];
@@ -468,23 +475,33 @@ __global__ void __launch_bounds__(kDensThreads) k_density_brick(
         float4 pp = spos[slot];
         int typ = smeta[slot].y;
         float wx[5], wy[5], wz[5];
-        int g0 = base_index_weights(pp.x, dx0, p, wx);
-        int g1 = base_index_weights(pp.y, dx1, p, wy);
-        int g2 = base_index_weights(pp.z, dx2, p, wz);
+        const int g0 = base_index_weights_fast(pp.x, dx0, idx0, p, wx);
+        const int g1 = base_index_weights_fast(pp.y, dx1, idx1, p, wy);
+        const int g2 = base_index_weights_fast(pp.z, dx2, idx2, p, wz);
+        // brick-local coordinate of the first stencil point, wrapped once
+        const int bx = wrap(g0 - shift - gx0, nx), by = wrap(g1 - shift - gy0, ny), bz = wrap(g2 - shift - gz0, nz);
+        const bool hasq = qchan >= 0 && pp.w != 0.f;
+#pragma unroll
         for (int iz = 0; iz <= p; ++iz) {
-            int lz = wrap(g2 + iz - shift - gz0, nz);
+            int lz = bz + iz;
+            if (lz >= nz) lz -= nz;
             if (lz >= bg.bz) continue;
+#pragma unroll
             for (int iy = 0; iy <= p; ++iy) {
-                int ly = wrap(g1 + iy - shift - gy0, ny);
+                int ly = by + iy;
+                if (ly >= ny) ly -= ny;
                 if (ly >= bg.by) continue;
                 float wyz = wy[iy] * wz[iz];
+                const int rowi = ly + (lz << bg.sy);
+#pragma unroll
                 for (int ix = 0; ix <= p; ++ix) {
-                    int lx = wrap(g0 + ix - shift - gx0, nx);
+                    int lx = bx + ix;
+                    if (lx >= nx) lx -= nx;
                     if (lx >= bg.bx) continue;
                     float w = wx[ix] * wyz;
-                    int li = lx + bg.bx * (ly + bg.by * lz);
+                    int li = lx + (rowi << bg.sx);
                     atomicAdd(&acc[typ * npts + li], (unsigned long long)__float2ll_rn(w * 4294967296.0f));
-                    if (qchan >= 0 && pp.w != 0.f)
+                    if (hasq)
                         atomicAdd(&acc[qchan * npts + li],
                                   (unsigned long long)__float2ll_rn(w * pp.w * 4294967296.0f));
                 }
@@ -494,8 +511,8 @@ __global__ void __launch_bounds__(kDensThreads) k_density_brick(
     __syncthreads();
     const long M = (long)nx * ny * nz;
     for (int i = threadIdx.x; i < nchan * npts; i += blockDim.x) {
-        int c = i / npts, li = i % npts;
-        int lx = li % bg.bx, ly = (li / bg.bx) % bg.by, lz = li / (bg.bx * bg.by);
+        int c = i >> (bg.sx + bg.sy + bg.sz), li = i & (npts - 1);
+        int lx = li & (bg.bx - 1), ly = (li >> bg.sx) & (bg.by - 1), lz = li >> (bg.sx + bg.sy);
         int gx = gx0 + lx, gy = gy0 + ly, gz = gz0 + lz;
         if (gx < nx && gy < ny && gz < nz) {
             double v = (double)(long long)acc[i] * (1.0 / 4294967296.0);
@@ -607,12 +624,13 @@ __global__ void __launch_bounds__(256) k_conv_z(const float2* __restrict__ S, fl
     }
 
     int out = 0;
-    for (int fi = 0; fi < kp.nforce + (kp.charge_pot >= 0 ? 1 : 0); ++fi) {
+    for (int fi = 0; fi < kp.nforce + (kp.qchan >= 0 ? 1 : 0); ++fi) {
         const bool charge = fi == kp.nforce;
         float2 Z[E];
 #pragma unroll
         for (int s = 0; s < E; ++s) Z[s] = make_float2(0.f, 0.f);
         if (charge) {
+            // separate charge field (charges not uniform per species)
             const KPot& Pq = kp.pot[kp.charge_pot];
 #pragma unroll
             for (int s = 0; s < E; ++s) {
@@ -627,7 +645,26 @@ __global__ void __launch_bounds__(256) k_conv_z(const float2* __restrict__ S, fl
             const int sp = kp.force_species[fi];
             for (int pi = 0; pi < kp.npots; ++pi) {
                 const KPot& Pp = kp.pot[pi];
-                if (Pp.kind == MFT_POT_CHARGES) continue;
+                if (Pp.kind == MFT_POT_CHARGES) {
+                    // folded electrostatics: q_s u_q(k) sum_c q_c rho_c(k)
+                    if (!kp.fold || kp.qspec[sp] == 0.f) continue;
+#pragma unroll
+                    for (int s = 0; s < E; ++s) {
+                        float2 src = make_float2(0.f, 0.f);
+#pragma unroll
+                        for (int c = 0; c < NCH; ++c) {
+                            if (c >= kp.nchan) continue;
+                            src.x += kp.qspec[c] * spec[c][s].x;
+                            src.y += kp.qspec[c] * spec[c][s].y;
+                        }
+                        float u = k2[s] > 0.f
+                                      ? kp.qspec[sp] * 12.566370614359172f * Pp.A / k2[s] * expf(-k2[s] * Pp.sig2)
+                                      : 0.f;
+                        Z[s].x += u * src.x;
+                        Z[s].y += u * src.y;
+                    }
+                    continue;
+                }
                 const bool inI = (Pp.maskI >> sp) & 1u, inJ = (Pp.maskJ >> sp) & 1u;
                 if (!inI && !inJ) continue;
 #pragma unroll
@@ -679,8 +716,15 @@ __global__ void __launch_bounds__(256) k_conv_z(const float2* __restrict__ S, fl
                     float u;
                     if (Pp.kind == MFT_POT_CHARGES) {
 #pragma unroll
-                        for (int c = 0; c < NCH; ++c)
-                            if (c == kp.qchan) a = spec[c][s];
+                        for (int c = 0; c < NCH; ++c) {
+                            if (c >= kp.nchan) continue;
+                            if (kp.fold) {
+                                a.x += kp.qspec[c] * spec[c][s].x;
+                                a.y += kp.qspec[c] * spec[c][s].y;
+                            } else if (c == kp.qchan) {
+                                a = spec[c][s];
+                            }
+                        }
                         b = a;
                         u = k2[s] > 0.f ? 0.5f * 12.566370614359172f * Pp.A / k2[s] * expf(-k2[s] * Pp.sig2) : 0.f;
                     } else {
@@ -721,14 +765,17 @@ __global__ void k_energy_sum(double* __restrict__ out, const double* __restrict_
 }
 
 // ------------------------------------------------------------- gather ---
-// Force on each particle from its species' grid force and, when charged, q
-// times the charge field (d_mapGridForcesToPartics and
+// Force on each particle from its species' total grid force (potentials and,
+// with folded charges, q_s times the field) and, when charges are not uniform
+// per species, q times the separate charge field (d_mapGridForcesToPartics and
 // d_mapGridChargeForcesToPartics, device_utils.cu:358-461), in cell order.
-__global__ void __launch_bounds__(256) k_gather(float* __restrict__ fgrid, const float* __restrict__ F,
+template <int P>
+__global__ void __launch_bounds__(256) k_gather(float* __restrict__ fgrid, const float4* __restrict__ F4,
                                                 const int* __restrict__ force_slot, int qslot,
                                                 const float4* __restrict__ spos, const int2* __restrict__ smeta,
-                                                int ns, int nx, int ny, int nz, float dx0, float dx1, float dx2, int p)
+                                                int ns, int nx, int ny, int nz, float dx0, float dx1, float dx2)
 {
+    constexpr int p = P;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ns) return;
     float4 pp = spos[i];
@@ -739,29 +786,44 @@ __global__ void __launch_bounds__(256) k_gather(float* __restrict__ fgrid, const
     float f[3] = {0.f, 0.f, 0.f};
     if (fs >= 0 || charged) {
         float wx[5], wy[5], wz[5];
-        int g0 = base_index_weights(pp.x, dx0, p, wx);
-        int g1 = base_index_weights(pp.y, dx1, p, wy);
-        int g2 = base_index_weights(pp.z, dx2, p, wz);
+        int g0 = base_index_weights_fast(pp.x, dx0, 1.0f / dx0, p, wx);
+        int g1 = base_index_weights_fast(pp.y, dx1, 1.0f / dx1, p, wy);
+        int g2 = base_index_weights_fast(pp.z, dx2, 1.0f / dx2, p, wz);
         const int shift = p / 2;
+        const float4* Fs = F4 + (fs >= 0 ? fs : 0) * M;
+        const float4* Fq = F4 + (qslot >= 0 ? qslot : 0) * M;
         float fq[3] = {0.f, 0.f, 0.f};
+        int xs[p + 1], ys[p + 1], zs[p + 1];
+        {
+            int bx = wrap(g0 - shift, nx), by = wrap(g1 - shift, ny), bz = wrap(g2 - shift, nz);
+#pragma unroll
+            for (int i = 0; i <= p; ++i) {
+                xs[i] = bx + i >= nx ? bx + i - nx : bx + i;
+                ys[i] = by + i >= ny ? by + i - ny : by + i;
+                zs[i] = bz + i >= nz ? bz + i - nz : bz + i;
+            }
+        }
+#pragma unroll
         for (int iz = 0; iz <= p; ++iz) {
-            int n2 = wrap(g2 + iz - shift, nz);
+#pragma unroll
             for (int iy = 0; iy <= p; ++iy) {
-                int n1 = wrap(g1 + iy - shift, ny);
                 float wyz = wy[iy] * wz[iz];
-                long row = (long)nx * (n1 + (long)ny * n2);
+                long row = (long)nx * (ys[iy] + (long)ny * zs[iz]);
+#pragma unroll
                 for (int ix = 0; ix <= p; ++ix) {
-                    long g = row + wrap(g0 + ix - shift, nx);
+                    long g = row + xs[ix];
                     float w = wx[ix] * wyz;
                     if (fs >= 0) {
-                        f[0] += F[(3 * fs + 0) * M + g] * w;
-                        f[1] += F[(3 * fs + 1) * M + g] * w;
-                        f[2] += F[(3 * fs + 2) * M + g] * w;
+                        float4 v = Fs[g];
+                        f[0] += v.x * w;
+                        f[1] += v.y * w;
+                        f[2] += v.z * w;
                     }
                     if (charged) {
-                        fq[0] += F[(3 * qslot + 0) * M + g] * w;
-                        fq[1] += F[(3 * qslot + 1) * M + g] * w;
-                        fq[2] += F[(3 * qslot + 2) * M + g] * w;
+                        float4 v = Fq[g];
+                        fq[0] += v.x * w;
+                        fq[1] += v.y * w;
+                        fq[2] += v.z * w;
                     }
                 }
             }
@@ -776,6 +838,196 @@ __global__ void __launch_bounds__(256) k_gather(float* __restrict__ fgrid, const
     fgrid[pid * 3 + 0] = f[0];
     fgrid[pid * 3 + 1] = f[1];
     fgrid[pid * 3 + 2] = f[2];
+}
+
+// Gather through LDS: a workgroup owns a brick of cells, stages the force
+// fields on the brick's grid window (brick + p points per axis, periodic) in
+// LDS once, then every particle whose base cell lies in the brick gathers its
+// (p+1)^3 stencil from LDS.  Fields are staged in chunks that fit the LDS
+// budget; each particle is handled in the chunk holding its field(s).
+constexpr int kGatherThreads = 256;
+constexpr int kGatherLdsFloats = 16384; // 64 KB
+
+template <int P>
+__global__ void __launch_bounds__(kGatherThreads) k_gather_brick(
+    float* __restrict__ fgrid, const float4* __restrict__ F4, const int* __restrict__ force_slot, int qslot,
+    int nout4, const float4* __restrict__ spos, const int2* __restrict__ smeta, const int* __restrict__ start, int nx,
+    int ny, int nz, float dx0, float dx1, float dx2, BrickGeom bg, int fields_per_chunk)
+{
+    constexpr int p = P;
+    extern __shared__ float tile[]; // [fields_per_chunk][3][window points]
+    __shared__ int seg_begin[256];
+    __shared__ int seg_pref[257];
+    __shared__ long rowbase[256];   // grid offset of window row (ly, lz)
+    __shared__ int xw[32];          // wrapped x of window column lx
+    const int bid = blockIdx.x;
+    const int ibx = bid % bg.nbx, iby = (bid / bg.nbx) % bg.nby, ibz = bid / (bg.nbx * bg.nby);
+    const int cx0 = ibx * bg.bx, cy0 = iby * bg.by, cz0 = ibz * bg.bz;
+    const int ex = min(bg.bx, nx - cx0), ey = min(bg.by, ny - cy0), ez = min(bg.bz, nz - cz0);
+    const int shift = p / 2;
+    // window of grid points: [c0 - shift, c0 + extent - 1 - shift + p]
+    const int wx = ex + p, wy = ey + p, wz = ez + p;
+    const int wpts = wx * wy * wz;
+    const long M = (long)nx * ny * nz;
+    const float idx0 = 1.0f / dx0, idx1 = 1.0f / dx1, idx2 = 1.0f / dx2;
+    for (int r = threadIdx.x; r < wy * wz; r += blockDim.x) {
+        int ly = r % wy, lz = r / wy;
+        int gy = wrap(cy0 - shift + ly, ny), gz = wrap(cz0 - shift + lz, nz);
+        rowbase[r] = (long)nx * (gy + (long)ny * gz);
+    }
+    if ((int)threadIdx.x < wx) xw[threadIdx.x] = wrap(cx0 - shift + (int)threadIdx.x, nx);
+
+    // particles of the brick: one contiguous run per (cy, cz) row
+    const int nrows = ey * ez;
+    for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
+        int cy = cy0 + r % ey, cz = cz0 + r / ey;
+        int c0 = cx0 + nx * (cy + ny * cz);
+        seg_begin[r] = start[c0];
+        seg_pref[r] = start[c0 + ex] - start[c0];
+    }
+    __syncthreads();
+    {
+        __shared__ int sh[16];
+        int l0 = (int)threadIdx.x < nrows ? seg_pref[threadIdx.x] : 0;
+        int tot;
+        int exs = block_exclusive_scan(l0, sh, &tot);
+        __syncthreads();
+        if ((int)threadIdx.x < nrows) seg_pref[threadIdx.x] = exs;
+        if (threadIdx.x == 0) seg_pref[nrows] = tot;
+    }
+    __syncthreads();
+    const int total = seg_pref[nrows];
+    if (total == 0) return;
+
+    for (int f0 = 0; f0 < nout4; f0 += fields_per_chunk) {
+        const int nf = min(fields_per_chunk, nout4 - f0);
+        __syncthreads();
+        // stage fields f0 .. f0+nf-1 as [field][comp][point]
+        // 16 lanes per window row (wx <= 12): lane column lx, row index step 16
+        const int lx = threadIdx.x & 15;
+        const int nrow = wy * wz;
+        for (int f = 0; f < nf; ++f) {
+            const float4* src = F4 + (long)(f0 + f) * M;
+            float* tf = tile + f * 3 * wpts;
+            if (lx < wx) {
+                for (int r = threadIdx.x >> 4; r < nrow; r += blockDim.x >> 4) {
+                    float4 v = src[rowbase[r] + xw[lx]];
+                    int li = lx + wx * r;
+                    tf[li] = v.x;
+                    tf[wpts + li] = v.y;
+                    tf[2 * wpts + li] = v.z;
+                }
+            }
+        }
+        __syncthreads();
+        for (int it = threadIdx.x; it < total; it += blockDim.x) {
+            int lo = 0, hi = nrows - 1;
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (seg_pref[mid] <= it) lo = mid;
+                else hi = mid - 1;
+            }
+            int slot = seg_begin[lo] + (it - seg_pref[lo]);
+            float4 pp = spos[slot];
+            int2 meta = smeta[slot];
+            int fs = force_slot[meta.y];
+            bool charged = qslot >= 0 && pp.w != 0.f;
+            // chunk indices of the particle's fields; the first of them writes,
+            // a later one adds (chunks run in order inside this workgroup)
+            const int k = f0 / fields_per_chunk;
+            const int cf = fs >= 0 ? fs / fields_per_chunk : 0x7fffffff;
+            const int cq = charged ? qslot / fields_per_chunk : 0x7fffffff;
+            const int first = min(cf, cq);
+            const bool hs = cf == k, hq = cq == k;
+            const bool none = first == 0x7fffffff;
+            if (!hs && !hq && !(none && k == 0)) continue;
+            float a[3] = {0.f, 0.f, 0.f};
+            if (hs || hq) {
+                float w0[5], w1[5], w2[5];
+                int g0 = base_index_weights_fast(pp.x, dx0, idx0, p, w0);
+                int g1 = base_index_weights_fast(pp.y, dx1, idx1, p, w1);
+                int g2 = base_index_weights_fast(pp.z, dx2, idx2, p, w2);
+                // local window coordinates of the first stencil point
+                int bx = wrap(g0 - cx0, nx), by = wrap(g1 - cy0, ny), bz = wrap(g2 - cz0, nz);
+                float aq[3] = {0.f, 0.f, 0.f};
+                const float* tfs = tile + (hs ? (fs - f0) : 0) * 3 * wpts;
+                const float* tfq = tile + (hq ? (qslot - f0) : 0) * 3 * wpts;
+#pragma unroll
+                for (int iz = 0; iz <= p; ++iz) {
+#pragma unroll
+                    for (int iy = 0; iy <= p; ++iy) {
+                        float wyz = w1[iy] * w2[iz];
+                        int rowi = wx * ((by + iy) + wy * (bz + iz)) + bx;
+#pragma unroll
+                        for (int ix = 0; ix <= p; ++ix) {
+                            float w = w0[ix] * wyz;
+                            int li = rowi + ix;
+                            if (hs) {
+                                a[0] += tfs[li] * w;
+                                a[1] += tfs[wpts + li] * w;
+                                a[2] += tfs[2 * wpts + li] * w;
+                            }
+                            if (hq) {
+                                aq[0] += tfq[li] * w;
+                                aq[1] += tfq[wpts + li] * w;
+                                aq[2] += tfq[2 * wpts + li] * w;
+                            }
+                        }
+                    }
+                }
+                a[0] += aq[0] * pp.w;
+                a[1] += aq[1] * pp.w;
+                a[2] += aq[2] * pp.w;
+            }
+            const int pid = meta.x;
+            if (none || k == first) {
+                fgrid[pid * 3 + 0] = a[0];
+                fgrid[pid * 3 + 1] = a[1];
+                fgrid[pid * 3 + 2] = a[2];
+            } else {
+                fgrid[pid * 3 + 0] += a[0];
+                fgrid[pid * 3 + 1] += a[1];
+                fgrid[pid * 3 + 2] += a[2];
+            }
+        }
+    }
+}
+
+// Last inverse pass along x for the three components of one force field,
+// keeping real parts and writing the field as float4 (x, y, z, 0).
+template <int N>
+__global__ void __launch_bounds__(256) k_fft_x_inv_vec3(const float2* __restrict__ Y, float4* __restrict__ F4, int ny,
+                                                        int nz, const float2* __restrict__ twg)
+{
+    using P = FftPlan<N>;
+    constexpr int T = P::T, E = P::E, B = 256 / T;
+    __shared__ float2 lds[B * N];
+    __shared__ float2 tw[N];
+    for (int i = threadIdx.x; i < N; i += 256) tw[i] = twg[i];
+    const int lline = threadIdx.x / T;
+    const int t = threadIdx.x % T;
+    const int line = blockIdx.x * B + lline;
+    const bool valid = line < ny * nz;
+    const long M = (long)N * ny * nz;
+    const long base = (long)line * N;
+    const int f = blockIdx.y;
+    float re[3][E];
+    __syncthreads();
+    LdsMap map{lline, B, false, N};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const float2* src = Y + (3 * f + j) * M + base;
+        float2 v[E];
+#pragma unroll
+        for (int s = 0; s < E; ++s) v[s] = valid ? src[t + T * s] : make_float2(0.f, 0.f);
+        line_fft<N, +1>(v, lds, tw, t, map);
+#pragma unroll
+        for (int s = 0; s < E; ++s) re[j][s] = v[s].x;
+    }
+    if (!valid) return;
+    float4* dst = F4 + f * M + base;
+#pragma unroll
+    for (int s = 0; s < E; ++s) dst[t + T * s] = make_float4(re[0][s], re[1][s], re[2][s], 0.f);
 }
 
 // ------------------------------------------------- bonds + integrator ---
@@ -845,16 +1097,6 @@ __global__ void k_copy_f(float* __restrict__ dst, const float* __restrict__ src,
     if (i < n) dst[i] = src[i];
 }
 
-// Interleave three real component fields into AoS [M*3].
-__global__ void k_aos3(float* __restrict__ out, const float* __restrict__ F, long M)
-{
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M) return;
-    out[i * 3 + 0] = F[i];
-    out[i * 3 + 1] = F[M + i];
-    out[i * 3 + 2] = F[2 * M + i];
-}
-
 // ------------------------------------------------------------- engine ---
 template <typename T>
 static int dalloc(T** p, size_t n)
@@ -872,7 +1114,8 @@ struct TildEngine {
     mft_tild_desc d;
     std::vector<mft_tild_potential> pots;
     std::vector<float> mass, mob;
-    int N = 0, M = 0, nchan = 0, qchan = -1, nforce = 0, nout = 0, qslot = -1;
+    int N = 0, M = 0, nchan = 0, qchan = -1, nforce = 0, nout = 0, nout4 = 0, qslot = -1, charge_pot = -1;
+    int configured = 0;
     float dx[3] = {0, 0, 0};
     float gvol = 0;
     hipStream_t stream = nullptr;
@@ -890,8 +1133,8 @@ struct TildEngine {
     // grids
     float* rho = nullptr;   // [nchan][M]
     float2* S = nullptr;    // [nchan][M]
-    float2* Y = nullptr;    // [nout][M]
-    float* F = nullptr;     // [nout][M]
+    float2* Y = nullptr;    // [nout][M] force spectra (3 components per field)
+    float4* F4 = nullptr;   // [nout4][M] force fields (x, y, z, 0)
     int* force_slot = nullptr; // [nspecies] -> force field slot or -1
     std::vector<int> force_slot_h;
     std::vector<float*> tables; // erf tables per potential (device)
@@ -902,6 +1145,9 @@ struct TildEngine {
     GjfParams gp;
     KParams kp;
     BrickGeom bg;
+    BrickGeom gbg;          // gather bricks
+    int gather_fpc = 1;     // force fields staged per chunk
+    int gather_mode = 0;    // 0 direct, 1 LDS-staged (mft_tild_autotune picks)
     size_t dens_lds = 0;
     int64_t step = 0;
     int have_state = 0;
@@ -983,42 +1229,25 @@ static int tild_setup(TildEngine* e)
     e->dx[2] = (float)((double)d.lz / (double)d.nz);
     e->gvol = e->dx[0] * e->dx[1] * e->dx[2];
 
-    // channels and force outputs
-    e->nchan = d.nspecies;
-    int charge_pot = -1;
-    uint32_t forced = 0;
+    // potentials: one charges potential at most; the channel layout is set
+    // once the charges are known (configure_channels)
+    e->charge_pot = -1;
     for (int i = 0; i < d.npotentials; ++i) {
         const mft_tild_potential& P = e->pots[i];
         if (P.kind == MFT_POT_CHARGES) {
-            MFT_REQUIRE(charge_pot < 0, MFT_ERR_UNSUPPORTED, "only one charges potential is supported");
-            charge_pot = i;
+            MFT_REQUIRE(e->charge_pot < 0, MFT_ERR_UNSUPPORTED, "only one charges potential is supported");
+            e->charge_pot = i;
         } else {
             MFT_REQUIRE(P.kind >= 0 && P.kind <= MFT_POT_ERFG, MFT_ERR_UNSUPPORTED, "potential kind %d", P.kind);
-            forced |= P.maskI | P.maskJ;
         }
     }
-    if (charge_pot >= 0) {
+    if (e->charge_pot >= 0)
         MFT_REQUIRE(d.has_charges, MFT_ERR_INVALID, "charges potential without charged particles");
-        e->qchan = e->nchan++;
-    }
-    MFT_REQUIRE(e->nchan <= kMaxChan, MFT_ERR_UNSUPPORTED, "at most %d density channels (species + charge)",
-                kMaxChan);
-    e->force_slot_h.assign(d.nspecies, -1);
-    e->nforce = 0;
-    for (int s = 0; s < d.nspecies; ++s)
-        if ((forced >> s) & 1u) {
-            e->kp.force_species[e->nforce] = s;
-            e->force_slot_h[s] = e->nforce++;
-        }
-    e->qslot = charge_pot >= 0 ? e->nforce : -1;
-    e->nout = 3 * (e->nforce + (charge_pot >= 0 ? 1 : 0));
+    const int charge_pot = e->charge_pot;
 
     KParams& kp = e->kp;
     kp.npots = d.npotentials;
     kp.charge_pot = charge_pot;
-    kp.nchan = e->nchan;
-    kp.qchan = e->qchan;
-    kp.nforce = e->nforce;
     kp.energy = 0;
     e->tables.assign(d.npotentials, nullptr);
     for (int i = 0; i < d.npotentials; ++i) {
@@ -1061,24 +1290,6 @@ static int tild_setup(TildEngine* e)
         e->gp.cn[s] = b * dt / (2.0f * m);
     }
 
-    // brick geometry: 16x8x8 points for up to 4 channels, smaller beyond
-    BrickGeom& bg = e->bg;
-    bg.bx = 16;
-    bg.by = 8;
-    bg.bz = e->nchan <= 4 ? 8 : 4;
-    if (bg.bx > d.nx) bg.bx = d.nx;
-    if (bg.by > d.ny) bg.by = d.ny;
-    if (bg.bz > d.nz) bg.bz = d.nz;
-    bg.nbx = (d.nx + bg.bx - 1) / bg.bx;
-    bg.nby = (d.ny + bg.by - 1) / bg.by;
-    bg.nbz = (d.nz + bg.bz - 1) / bg.bz;
-    e->dens_lds = (size_t)e->nchan * bg.bx * bg.by * bg.bz * sizeof(unsigned long long);
-    {
-        // segment table capacity: rows * 2 runs
-        int wy = bg.by + d.pmeorder, wz = bg.bz + d.pmeorder;
-        MFT_REQUIRE(2 * wy * wz <= 512, MFT_ERR_UNSUPPORTED, "brick halo too large");
-    }
-
     const int N = e->N, M = e->M;
     if ((rc = dalloc(&e->x[0], 3 * (size_t)N))) return rc;
     if ((rc = dalloc(&e->x[1], 3 * (size_t)N))) return rc;
@@ -1098,10 +1309,6 @@ static int tild_setup(TildEngine* e)
     if ((rc = dalloc(&e->partial, (size_t)e->nscan_blocks))) return rc;
     if ((rc = dalloc(&e->spos, (size_t)N))) return rc;
     if ((rc = dalloc(&e->smeta, (size_t)N))) return rc;
-    if ((rc = dalloc(&e->rho, (size_t)e->nchan * M))) return rc;
-    if ((rc = dalloc(&e->S, (size_t)e->nchan * M))) return rc;
-    if ((rc = dalloc(&e->Y, (size_t)(e->nout > 0 ? e->nout : 1) * M))) return rc;
-    if ((rc = dalloc(&e->F, (size_t)(e->nout > 0 ? e->nout : 1) * M))) return rc;
     if ((rc = dalloc(&e->force_slot, (size_t)d.nspecies))) return rc;
     int nconv_blocks = 0;
     {
@@ -1115,12 +1322,127 @@ static int tild_setup(TildEngine* e)
     if ((rc = dalloc(&e->epart, (size_t)nconv_blocks * kMaxPots))) return rc;
     if ((rc = dalloc(&e->eout, (size_t)kMaxPots + 1))) return rc;
     if ((rc = dalloc(&e->bpart, (size_t)nblk(N)))) return rc;
-    MFT_HIP(hipMemcpy(e->force_slot, e->force_slot_h.data(), sizeof(int) * d.nspecies, hipMemcpyHostToDevice));
     MFT_HIP(hipMemset(e->count, 0, sizeof(int) * (size_t)M));
     MFT_HIP(hipMemset(e->noise_old, 0, sizeof(float) * 3 * (size_t)N));
     MFT_HIP(hipMemset(e->nBonds, 0, sizeof(int) * (size_t)N));
     MFT_HIP(hipMemset(e->type, 0, sizeof(int) * (size_t)N));
     MFT_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    return MFT_OK;
+}
+
+
+// Density channels and force fields once the charges are known.  When every
+// species carries a single charge value (or there is no charges potential),
+// the charge density is sum_s q_s rho_s, so electrostatics folds into the
+// per-species k-space kernel and no separate charge channel, transform or
+// field is needed; otherwise the charge density is a channel of its own.
+static int configure_channels(TildEngine* e, const int* type, const float* charge)
+{
+    const mft_tild_desc& d = e->d;
+    int rc;
+    bool fold = true;
+    std::vector<float> q(d.nspecies, 0.f);
+    std::vector<int> seen(d.nspecies, 0);
+    if (e->charge_pot >= 0) {
+        for (size_t i = 0; i < (size_t)e->N; ++i) {
+            int t = type[i];
+            float c = charge ? charge[i] : 0.f;
+            if (!seen[t]) {
+                seen[t] = 1;
+                q[t] = c;
+            } else if (q[t] != c) {
+                fold = false;
+            }
+        }
+    }
+    KParams& kp = e->kp;
+    kp.fold = fold ? 1 : 0;
+    for (int s = 0; s < kMaxSpecies; ++s) kp.qspec[s] = (fold && s < d.nspecies) ? q[s] : 0.f;
+    e->nchan = d.nspecies;
+    e->qchan = -1;
+    if (e->charge_pot >= 0 && !fold) e->qchan = e->nchan++;
+    MFT_REQUIRE(e->nchan <= kMaxChan, MFT_ERR_UNSUPPORTED, "at most %d density channels (species + charge)",
+                kMaxChan);
+    uint32_t forced = 0;
+    for (int i = 0; i < d.npotentials; ++i) {
+        const mft_tild_potential& P = e->pots[i];
+        if (P.kind != MFT_POT_CHARGES) forced |= P.maskI | P.maskJ;
+    }
+    if (e->charge_pot >= 0 && fold)
+        for (int s = 0; s < d.nspecies; ++s)
+            if (q[s] != 0.f) forced |= 1u << s;
+    e->force_slot_h.assign(d.nspecies, -1);
+    e->nforce = 0;
+    for (int s = 0; s < d.nspecies; ++s)
+        if ((forced >> s) & 1u) {
+            kp.force_species[e->nforce] = s;
+            e->force_slot_h[s] = e->nforce++;
+        }
+    e->qslot = (e->charge_pot >= 0 && !fold) ? e->nforce : -1;
+    e->nout4 = e->nforce + (e->qslot >= 0 ? 1 : 0);
+    e->nout = 3 * e->nout4;
+    kp.nchan = e->nchan;
+    kp.qchan = e->qchan;
+    kp.nforce = e->nforce;
+
+    // brick geometry: 16x8x8 points for up to 4 channels, smaller beyond
+    BrickGeom& bg = e->bg;
+    auto pow2_floor = [](int v, int cap) {
+        int b = 1;
+        while (b * 2 <= v && b * 2 <= cap) b *= 2;
+        return b;
+    };
+    auto log2i = [](int v) {
+        int l = 0;
+        while ((1 << l) < v) ++l;
+        return l;
+    };
+    bg.bx = pow2_floor(d.nx, 16);
+    bg.by = pow2_floor(d.ny, 8);
+    bg.bz = pow2_floor(d.nz, e->nchan <= 4 ? 8 : 4);
+    bg.sx = log2i(bg.bx);
+    bg.sy = log2i(bg.by);
+    bg.sz = log2i(bg.bz);
+    bg.nbx = (d.nx + bg.bx - 1) / bg.bx;
+    bg.nby = (d.ny + bg.by - 1) / bg.by;
+    bg.nbz = (d.nz + bg.bz - 1) / bg.bz;
+    e->dens_lds = (size_t)e->nchan * bg.bx * bg.by * bg.bz * sizeof(unsigned long long);
+    {
+        int wy = bg.by + d.pmeorder, wz = bg.bz + d.pmeorder;
+        MFT_REQUIRE(2 * wy * wz <= 512, MFT_ERR_UNSUPPORTED, "brick halo too large");
+    }
+    // gather bricks: 8^3 cells, window (8+p)^3 points x 3 components per field
+    BrickGeom& gg = e->gbg;
+    gg.bx = pow2_floor(d.nx, 8);
+    gg.by = pow2_floor(d.ny, 8);
+    gg.bz = pow2_floor(d.nz, 8);
+    gg.sx = log2i(gg.bx);
+    gg.sy = log2i(gg.by);
+    gg.sz = log2i(gg.bz);
+    gg.nbx = (d.nx + gg.bx - 1) / gg.bx;
+    gg.nby = (d.ny + gg.by - 1) / gg.by;
+    gg.nbz = (d.nz + gg.bz - 1) / gg.bz;
+    {
+        int wpts = (gg.bx + d.pmeorder) * (gg.by + d.pmeorder) * (gg.bz + d.pmeorder);
+        int fpc = kGatherLdsFloats / (3 * wpts);
+        MFT_REQUIRE(fpc >= 1 && gg.by * gg.bz <= 256 && (gg.by + d.pmeorder) * (gg.bz + d.pmeorder) <= 256 &&
+                        gg.bx + d.pmeorder <= 16,
+                    MFT_ERR_UNSUPPORTED, "gather window too large");
+        e->gather_fpc = std::max(1, std::min(fpc, std::max(1, e->nout4)));
+    }
+    const size_t M = e->M;
+    for (void* pbuf : {(void*)e->rho, (void*)e->S, (void*)e->Y, (void*)e->F4})
+        if (pbuf) (void)hipFree(pbuf);
+    e->rho = nullptr;
+    e->S = nullptr;
+    e->Y = nullptr;
+    e->F4 = nullptr;
+    if ((rc = dalloc(&e->rho, (size_t)e->nchan * M))) return rc;
+    if ((rc = dalloc(&e->S, (size_t)e->nchan * M))) return rc;
+    if ((rc = dalloc(&e->Y, (size_t)(e->nout > 0 ? e->nout : 1) * M))) return rc;
+    if ((rc = dalloc(&e->F4, (size_t)(e->nout4 > 0 ? e->nout4 : 1) * M))) return rc;
+    MFT_HIP(hipMemcpy(e->force_slot, e->force_slot_h.data(), sizeof(int) * d.nspecies, hipMemcpyHostToDevice));
+    e->configured = 1;
     return MFT_OK;
 }
 
@@ -1214,6 +1536,65 @@ static int launch_conv(TildEngine* e)
     return MFT_OK;
 }
 
+
+static int launch_x_inv_vec3(TildEngine* e)
+{
+    const mft_tild_desc& d = e->d;
+    switch (d.nx) {
+#define MFT_CASE(n_)                                                                                      \
+    case n_: {                                                                                            \
+        constexpr int B = 256 / FftPlan<n_>::T;                                                           \
+        dim3 grid((d.ny * d.nz + B - 1) / B, e->nout4);                                                   \
+        k_fft_x_inv_vec3<n_><<<grid, 256, 0, e->stream>>>(e->Y, e->F4, d.ny, d.nz, fft_twiddles_device(n_)); \
+        break;                                                                                            \
+    }
+        MFT_FOR_EACH_FFT_SIZE(MFT_CASE)
+#undef MFT_CASE
+    default: return MFT_ERR_UNSUPPORTED;
+    }
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
+
+// gather variants: 0 = direct loads in cell order, 1 = LDS-staged bricks
+static int launch_gather(TildEngine* e, int mode)
+{
+    const mft_tild_desc& d = e->d;
+    if (mode == 0) {
+#define MFT_G0(P_)                                                                                        \
+    k_gather<P_><<<nblk(e->N), kBlock, 0, e->stream>>>(e->fgrid, e->F4, e->force_slot, e->qslot, e->spos, e->smeta, \
+                                                        e->N, d.nx, d.ny, d.nz, e->dx[0], e->dx[1], e->dx[2])
+        switch (d.pmeorder) {
+        case 0: MFT_G0(0); break;
+        case 1: MFT_G0(1); break;
+        case 2: MFT_G0(2); break;
+        case 3: MFT_G0(3); break;
+        default: MFT_G0(4); break;
+        }
+#undef MFT_G0
+    } else {
+        const BrickGeom& gb = e->gbg;
+        unsigned nb = gb.nbx * gb.nby * gb.nbz;
+        size_t lds = sizeof(float) * 3 * (size_t)e->gather_fpc * (gb.bx + d.pmeorder) * (gb.by + d.pmeorder) *
+                     (gb.bz + d.pmeorder);
+#define MFT_G1(P_)                                                                                        \
+    k_gather_brick<P_><<<nb, kGatherThreads, lds, e->stream>>>(e->fgrid, e->F4, e->force_slot, e->qslot, e->nout4, \
+                                                               e->spos, e->smeta, e->start, d.nx, d.ny, d.nz,      \
+                                                               e->dx[0], e->dx[1], e->dx[2], gb, e->gather_fpc)
+        switch (d.pmeorder) {
+        case 0: MFT_G1(0); break;
+        case 1: MFT_G1(1); break;
+        case 2: MFT_G1(2); break;
+        case 3: MFT_G1(3); break;
+        default: MFT_G1(4); break;
+        }
+#undef MFT_G1
+    }
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
 static int bin_current(TildEngine* e)
 {
     const mft_tild_desc& d = e->d;
@@ -1248,9 +1629,18 @@ static int grid_forces(TildEngine* e)
     {
         TimedScope ts(e, MFT_TK_DENSITY);
         unsigned nb = e->bg.nbx * e->bg.nby * e->bg.nbz;
-        k_density_brick<<<nb, kDensThreads, e->dens_lds, e->stream>>>(
-            e->rho, e->nchan, e->qchan, e->spos, e->smeta, e->start, d.nx, d.ny, d.nz, e->dx[0], e->dx[1], e->dx[2],
-            d.pmeorder, e->bg, 1.0f / e->gvol);
+#define MFT_DENS(P_)                                                                                      \
+    k_density_brick<P_><<<nb, kDensThreads, e->dens_lds, e->stream>>>(e->rho, e->nchan, e->qchan, e->spos, e->smeta, \
+                                                                       e->start, d.nx, d.ny, d.nz, e->dx[0],      \
+                                                                       e->dx[1], e->dx[2], e->bg, 1.0f / e->gvol)
+        switch (d.pmeorder) {
+        case 0: MFT_DENS(0); break;
+        case 1: MFT_DENS(1); break;
+        case 2: MFT_DENS(2); break;
+        case 3: MFT_DENS(3); break;
+        default: MFT_DENS(4); break;
+        }
+#undef MFT_DENS
         MFT_LAUNCH_CHECK();
     }
     if (e->nout == 0) {
@@ -1277,13 +1667,11 @@ static int grid_forces(TildEngine* e)
     }
     {
         TimedScope ts(e, MFT_TK_FFT_X_INV);
-        if ((rc = launch_axis<+1, false, true>(e->Y, e->F, gx, e->nout, M, 1.0f, e->stream))) return rc;
+        if ((rc = launch_x_inv_vec3(e))) return rc;
     }
     {
         TimedScope ts(e, MFT_TK_GATHER);
-        k_gather<<<nblk(e->N), kBlock, 0, e->stream>>>(e->fgrid, e->F, e->force_slot, e->qslot, e->spos, e->smeta,
-                                                        e->N, d.nx, d.ny, d.nz, e->dx[0], e->dx[1], e->dx[2],
-                                                        d.pmeorder);
+        if ((rc = launch_gather(e, e->gather_mode))) return rc;
         MFT_LAUNCH_CHECK();
     }
     return MFT_OK;
@@ -1433,7 +1821,7 @@ int mft_tild_destroy(mft_tild* e)
     if (!e) return MFT_OK;
     void* ptrs[] = {e->x[0], e->x[1], e->noise_old, e->fgrid, e->ftot, e->charge, e->type, e->nBonds, e->bondedTo,
                     e->bondType, e->count, e->cell, e->rank, e->start, e->partial, e->spos, e->smeta, e->rho, e->S,
-                    e->Y, e->F, e->force_slot, e->epart, e->eout, e->bpart};
+                    e->Y, e->F4, e->force_slot, e->epart, e->eout, e->bpart};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (float* t : e->tables)
@@ -1462,6 +1850,8 @@ int mft_tild_set_topology(mft_tild* e, const int* type, const float* charge, con
         MFT_REQUIRE(std::fabs(qt) < 1e-3, MFT_ERR_INVALID, "Box net charge is not zero! qtotal: %g", qt);
         MFT_HIP(hipMemcpy(e->charge, charge, sizeof(float) * N, hipMemcpyHostToDevice));
     }
+    int rc = configure_channels(e, type, e->d.has_charges ? charge : nullptr);
+    if (rc) return rc;
     if (nBonds) {
         for (size_t i = 0; i < N; ++i) {
             MFT_REQUIRE(nBonds[i] >= 0 && nBonds[i] <= (int)mb, MFT_ERR_INVALID, "particle %zu has %d bonds", i,
@@ -1482,6 +1872,7 @@ int mft_tild_set_topology(mft_tild* e, const int* type, const float* charge, con
 int mft_tild_set_state(mft_tild* e, const float* x, const float* xo, const float* noise_old)
 {
     MFT_REQUIRE(e && x, MFT_ERR_INVALID, "null argument");
+    MFT_REQUIRE(e->configured, MFT_ERR_INVALID, "set the topology before the state");
     const size_t n3 = 3 * (size_t)e->N;
     const float Ls[3] = {e->d.lx, e->d.ly, e->d.lz};
     for (size_t i = 0; i < n3; ++i)
@@ -1536,8 +1927,18 @@ int mft_tild_get_densities(mft_tild* e, float* rho, float* rhoq)
     const size_t M = e->M;
     if (rho) MFT_HIP(hipMemcpy(rho, e->rho, sizeof(float) * M * e->d.nspecies, hipMemcpyDeviceToHost));
     if (rhoq) {
-        MFT_REQUIRE(e->qchan >= 0, MFT_ERR_INVALID, "no charge density channel");
-        MFT_HIP(hipMemcpy(rhoq, e->rho + (size_t)e->qchan * M, sizeof(float) * M, hipMemcpyDeviceToHost));
+        MFT_REQUIRE(e->charge_pot >= 0, MFT_ERR_INVALID, "no charges potential");
+        if (e->qchan >= 0) {
+            MFT_HIP(hipMemcpy(rhoq, e->rho + (size_t)e->qchan * M, sizeof(float) * M, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> r(M * e->d.nspecies);
+            MFT_HIP(hipMemcpy(r.data(), e->rho, sizeof(float) * r.size(), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < M; ++i) {
+                double v = 0.0;
+                for (int sp = 0; sp < e->d.nspecies; ++sp) v += (double)e->kp.qspec[sp] * r[(size_t)sp * M + i];
+                rhoq[i] = (float)v;
+            }
+        }
     }
     return MFT_OK;
 }
@@ -1551,13 +1952,14 @@ int mft_tild_get_grid_force(mft_tild* e, int s, float* F)
         std::memset(F, 0, sizeof(float) * 3 * M);
         return MFT_OK;
     }
-    float* tmp = nullptr;
-    int rc;
-    if ((rc = dalloc(&tmp, 3 * M))) return rc;
-    k_aos3<<<nblk((long)M), kBlock, 0, e->stream>>>(tmp, e->F + (size_t)3 * slot * M, (long)M);
+    std::vector<float4> tmp(M);
     MFT_HIP(hipStreamSynchronize(e->stream));
-    MFT_HIP(hipMemcpy(F, tmp, sizeof(float) * 3 * M, hipMemcpyDeviceToHost));
-    MFT_HIP(hipFree(tmp));
+    MFT_HIP(hipMemcpy(tmp.data(), e->F4 + (size_t)slot * M, sizeof(float4) * M, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < M; ++i) {
+        F[3 * i + 0] = tmp[i].x;
+        F[3 * i + 1] = tmp[i].y;
+        F[3 * i + 2] = tmp[i].z;
+    }
     return MFT_OK;
 }
 
@@ -1635,7 +2037,7 @@ int mft_tild_timing(mft_tild* e, int enable, double* avg_ms, int64_t* launches)
 int mft_tild_kernel_bytes(mft_tild* e, double* bytes)
 {
     MFT_REQUIRE(e && bytes, MFT_ERR_INVALID, "null argument");
-    const double N = e->N, M = e->M, C = e->nchan, O = e->nout;
+    const double N = e->N, M = e->M, C = e->nchan, O = e->nout, O4 = e->nout4;
     const bool q = e->d.has_charges;
     for (int k = 0; k < MFT_TK_COUNT; ++k) bytes[k] = 0;
     // minimal HBM traffic of each kernel's job (reads + writes once)
@@ -1647,9 +2049,48 @@ int mft_tild_kernel_bytes(mft_tild* e, double* bytes)
     bytes[MFT_TK_FFT_Y_FWD] = M * C * 16;
     bytes[MFT_TK_CONV_Z] = M * 8 * (C + O);
     bytes[MFT_TK_FFT_Y_INV] = M * O * 16;
-    bytes[MFT_TK_FFT_X_INV] = M * O * (8 + 4);
-    bytes[MFT_TK_GATHER] = N * (24 + 12) + M * 4 * O;
+    bytes[MFT_TK_FFT_X_INV] = M * O * 8 + M * O4 * 16;
+    bytes[MFT_TK_GATHER] = N * (24 + 12) + M * 16 * O4;
     bytes[MFT_TK_INTEGRATE] = N * (12 + 12 + 12 + 24 + 12 + 4 + 4 + 8 + 8 + 12) + N * 12;
+    return MFT_OK;
+}
+
+int mft_tild_autotune(mft_tild* e, int reps, int* choice)
+{
+    MFT_REQUIRE(e && e->have_state && reps >= 1, MFT_ERR_INVALID, "engine has no state");
+    // one force evaluation fills the cell-ordered particles and force fields;
+    // the gather only writes the per-particle grid force, so the variants
+    // can be timed in place without changing the state
+    int saved = e->d.integrator;
+    int64_t st = e->step;
+    e->d.integrator = MFT_INT_NONE;
+    int rc = do_steps(e, 1, false);
+    e->d.integrator = saved;
+    e->step = st;
+    if (rc) return rc;
+    hipEvent_t a, b;
+    MFT_HIP(hipEventCreate(&a));
+    MFT_HIP(hipEventCreate(&b));
+    float best = 1e30f;
+    int best_mode = 0;
+    for (int mode = 0; mode < 2; ++mode) {
+        if ((rc = launch_gather(e, mode))) return rc;  // warm
+        MFT_HIP(hipEventRecord(a, e->stream));
+        for (int r = 0; r < reps; ++r)
+            if ((rc = launch_gather(e, mode))) return rc;
+        MFT_HIP(hipEventRecord(b, e->stream));
+        MFT_HIP(hipEventSynchronize(b));
+        float ms = 0.f;
+        MFT_HIP(hipEventElapsedTime(&ms, a, b));
+        if (ms < best) {
+            best = ms;
+            best_mode = mode;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    e->gather_mode = best_mode;
+    if (choice) *choice = best_mode;
     return MFT_OK;
 }
 

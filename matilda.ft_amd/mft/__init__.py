@@ -319,6 +319,11 @@ class TildEngine:
         check(lib().mft_tild_timing(self.h, int(enable), _ptr(ms), _ptr(n)))
         return dict(zip(TK_NAMES, ms)), dict(zip(TK_NAMES, n))
 
+    def autotune(self, reps=5):
+        c = _i(0)
+        check(lib().mft_tild_autotune(self.h, reps, C.byref(c)))
+        return c.value
+
     def kernel_bytes(self):
         b = np.zeros(len(TK_NAMES), np.float64)
         check(lib().mft_tild_kernel_bytes(self.h, _ptr(b)))

@@ -149,10 +149,22 @@ def _engine_vs_oracle(gpu, cfg, integrator="GJF"):
     return eng, sys_o
 
 
-@pytest.mark.parametrize("charged", [False, True])
-def test_engine_forces_and_densities(gpu, charged):
-    cfg = small_system(charged)
-    say("case: engine forces", cfg["name"], "N =", cfg["topo"]["type"].size)
+def mixed_charge_system():
+    """Charged system whose species do not carry one charge each (every other
+    bead of the polycation/polyanion chains is neutral): the engine keeps a
+    separate charge-density channel."""
+    cfg = small_system(True, seed=6)
+    q = cfg["topo"]["charge"].copy()
+    q[(np.arange(q.size) % 2 == 1) & (q != 0)] = 0.0
+    cfg["topo"]["charge"] = q
+    return cfg
+
+
+@pytest.mark.parametrize("case", ["neutral", "charged", "mixed_charges"])
+def test_engine_forces_and_densities(gpu, case):
+    cfg = mixed_charge_system() if case == "mixed_charges" else small_system(case == "charged")
+    charged = case != "neutral"
+    say("case: engine forces", case, "N =", cfg["topo"]["type"].size)
     eng, sys_o = _engine_vs_oracle(gpu, cfg)
     f_g = eng.compute_forces()
     out = orc.tild_forces(sys_o)
@@ -161,10 +173,22 @@ def test_engine_forces_and_densities(gpu, charged):
         assert rel(rho_g[s], out["rho"][name]) < TOL, name
     if charged:
         assert rel(rhoq_g, out["rhoq"]) < TOL
-    # grid force of each species group (ps_groups.cu:164-178)
+    # grid force of each species group (ps_groups.cu:164-178); when every
+    # species has one charge q_s the engine folds q_s times the charges group's
+    # field into the species field
+    M = int(np.prod(cfg["Nx"]))
+    q = cfg["topo"]["charge"]
+    t = cfg["topo"]["type"]
+    folded = charged and all(np.unique(q[t == s]).size <= 1 for s in range(len(cfg["species"])))
     for s, name in enumerate(cfg["species"]):
-        if name in out["gridF"]:
-            assert rel(eng.grid_force(s), out["gridF"][name]) < TOL, name
+        want = np.array(out["gridF"].get(name, np.zeros(3 * M)))
+        if folded:
+            qs = float(q[t == s][0]) if np.any(t == s) else 0.0
+            want = want + qs * out["gridF"]["charges"]
+        if np.any(want != 0):
+            assert rel(eng.grid_force(s), want) < TOL, name
+    if charged and not folded:
+        assert rel(eng.grid_force(len(cfg["species"])), out["gridF"]["charges"]) < TOL
     assert rel(f_g, out["f"]) < TOL
     eng.close()
 
@@ -197,9 +221,10 @@ def test_engine_steps_match_oracle(gpu, integrator, charged):
     eng.close()
 
 
-def test_energies_match_oracle(gpu):
-    cfg = small_system(True, seed=3)
-    say("case: energies", cfg["name"])
+@pytest.mark.parametrize("case", ["charged", "mixed_charges"])
+def test_energies_match_oracle(gpu, case):
+    cfg = small_system(True, seed=3) if case == "charged" else mixed_charge_system()
+    say("case: energies", case)
     eng, sys_o = _engine_vs_oracle(gpu, cfg)
     pe, be = eng.energies()
     out = orc.tild_forces(sys_o, want_energy=True)
