@@ -1,0 +1,144 @@
+// Line-FFT passes of the TILD grid pipeline: x and y transforms of all
+// channels in one launch, and the last inverse x pass that writes the float4
+// force fields.
+#include "tild_impl.h"
+
+namespace mft {
+
+// Line transform of nfield grids along one axis.  Input real (float) or
+// complex, output complex or real part; scale applied on store.
+template <int N, int SIGN, bool REAL_IN, bool REAL_OUT>
+__global__ void __launch_bounds__(256) k_fft_axis(const void* __restrict__ in, void* __restrict__ out, AxisGeom geom,
+                                                  long field_stride, const float2* __restrict__ twg, float scale)
+{
+    using P = FftPlan<N>;
+    constexpr int T = P::T, E = P::E, B = 256 / T;
+    __shared__ float2 lds[B * N];
+    __shared__ float2 tw[N];
+    for (int i = threadIdx.x; i < N; i += 256) tw[i] = twg[i];
+    const bool inter = geom.axis != 0;
+    const int lline = inter ? (threadIdx.x % B) : (threadIdx.x / T);
+    const int t = inter ? (threadIdx.x / B) : (threadIdx.x % T);
+    const int line = blockIdx.x * B + lline;
+    const bool valid = line < geom.nlines();
+    const int field = blockIdx.y;
+    int ta, tb;
+    const long base = valid ? geom.base(line, &ta, &tb) : 0;
+    const long st = geom.stride();
+    float2 v[E];
+    if (REAL_IN) {
+        const float* src = (const float*)in + field * field_stride + base;
+#pragma unroll
+        for (int s = 0; s < E; ++s) v[s] = valid ? make_float2(src[(t + T * s) * st], 0.f) : make_float2(0.f, 0.f);
+    } else {
+        const float2* src = (const float2*)in + field * field_stride + base;
+#pragma unroll
+        for (int s = 0; s < E; ++s) v[s] = valid ? src[(t + T * s) * st] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    LdsMap map{lline, B, inter, N};
+    line_fft<N, SIGN>(v, lds, tw, t, map);
+    if (!valid) return;
+    if (REAL_OUT) {
+        float* dst = (float*)out + field * field_stride + base;
+#pragma unroll
+        for (int s = 0; s < E; ++s) dst[(t + T * s) * st] = v[s].x * scale;
+    } else {
+        float2* dst = (float2*)out + field * field_stride + base;
+#pragma unroll
+        for (int s = 0; s < E; ++s) dst[(t + T * s) * st] = cscale(v[s], scale);
+    }
+}
+
+// Last inverse pass along x for the three components of one force field,
+// keeping real parts and writing the field as float4 (x, y, z, 0).
+template <int N>
+__global__ void __launch_bounds__(256) k_fft_x_inv_vec3(const float2* __restrict__ Y, float4* __restrict__ F4, int ny,
+                                                        int nz, const float2* __restrict__ twg)
+{
+    using P = FftPlan<N>;
+    constexpr int T = P::T, E = P::E, B = 256 / T;
+    __shared__ float2 lds[B * N];
+    __shared__ float2 tw[N];
+    for (int i = threadIdx.x; i < N; i += 256) tw[i] = twg[i];
+    const int lline = threadIdx.x / T;
+    const int t = threadIdx.x % T;
+    const int line = blockIdx.x * B + lline;
+    const bool valid = line < ny * nz;
+    const long M = (long)N * ny * nz;
+    const long base = (long)line * N;
+    const int f = blockIdx.y;
+    float re[3][E];
+    __syncthreads();
+    LdsMap map{lline, B, false, N};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const float2* src = Y + (3 * f + j) * M + base;
+        float2 v[E];
+#pragma unroll
+        for (int s = 0; s < E; ++s) v[s] = valid ? src[t + T * s] : make_float2(0.f, 0.f);
+        line_fft<N, +1>(v, lds, tw, t, map);
+#pragma unroll
+        for (int s = 0; s < E; ++s) re[j][s] = v[s].x;
+    }
+    if (!valid) return;
+    float4* dst = F4 + f * M + base;
+#pragma unroll
+    for (int s = 0; s < E; ++s) dst[t + T * s] = make_float4(re[0][s], re[1][s], re[2][s], 0.f);
+}
+
+template <int N, int SIGN, bool RI, bool RO>
+static void launch_axis_n(const void* in, void* out, AxisGeom g, int nf, long fs, float scale, hipStream_t st)
+{
+    constexpr int B = 256 / FftPlan<N>::T;
+    dim3 grid((g.nlines() + B - 1) / B, nf);
+    k_fft_axis<N, SIGN, RI, RO><<<grid, 256, 0, st>>>(in, out, g, fs, fft_twiddles_device(N), scale);
+}
+
+template <int SIGN, bool RI, bool RO>
+static int launch_axis(const void* in, void* out, AxisGeom g, int nf, long fs, float scale, hipStream_t st)
+{
+    switch (g.length()) {
+#define MFT_CASE(n_)                                                                                      \
+    case n_: launch_axis_n<n_, SIGN, RI, RO>(in, out, g, nf, fs, scale, st); break;
+        MFT_FOR_EACH_FFT_SIZE(MFT_CASE)
+#undef MFT_CASE
+    default:
+        set_error(MFT_ERR_UNSUPPORTED, "unsupported FFT length %d", g.length());
+        return MFT_ERR_UNSUPPORTED;
+    }
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
+// the three variants the engine uses: forward real -> complex, forward and
+// inverse complex -> complex
+int tild_axis(int sign, int real_in, int real_out, const void* in, void* out, AxisGeom g, int nf, long fs,
+              float scale, hipStream_t st)
+{
+    if (sign < 0 && real_in && !real_out) return launch_axis<-1, true, false>(in, out, g, nf, fs, scale, st);
+    if (sign < 0 && !real_in && !real_out) return launch_axis<-1, false, false>(in, out, g, nf, fs, scale, st);
+    if (sign > 0 && !real_in && !real_out) return launch_axis<+1, false, false>(in, out, g, nf, fs, scale, st);
+    set_error(MFT_ERR_UNSUPPORTED, "axis pass variant not built");
+    return MFT_ERR_UNSUPPORTED;
+}
+
+int tild_x_inv_vec3(const float2* Y, float4* F4, int nx, int ny, int nz, int nf, hipStream_t st)
+{
+    switch (nx) {
+#define MFT_CASE(n_)                                                                                      \
+    case n_: {                                                                                            \
+        constexpr int B = 256 / FftPlan<n_>::T;                                                           \
+        dim3 grid((ny * nz + B - 1) / B, nf);                                                             \
+        k_fft_x_inv_vec3<n_><<<grid, 256, 0, st>>>(Y, F4, ny, nz, fft_twiddles_device(n_));               \
+        break;                                                                                            \
+    }
+        MFT_FOR_EACH_FFT_SIZE(MFT_CASE)
+#undef MFT_CASE
+    default: return MFT_ERR_UNSUPPORTED;
+    }
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
+} // namespace mft

@@ -1,0 +1,201 @@
+// Shared state and launchers of the TILD particle-mesh engine.
+//
+// One step (PS_Box::doTimeStep, PS_Box.cu:73-207) runs as
+//   bin (counting sort by (brick, species, sub-cell) key, fused into the
+//        previous step's integrator) -> scan -> scatter into key order
+//   -> tile density: one workgroup per brick of grid points accumulates the
+//      (B+p)^3 stencil footprint of the brick's own particles in LDS (integer
+//      fixed point, order independent), writes the tile, and a reduction sums
+//      the overlapping tiles per grid point in a fixed order
+//   -> FFT x, FFT y (all density channels in one launch each)
+//   -> fused z pass: forward z FFT, every potential applied in k-space as one
+//      per-species kernel sum_c K_tc(k) rho_c(k) times -i k_j, inverse z FFT
+//   -> inverse FFT y, inverse FFT x keeping the real part (float4 fields)
+//   -> gather per (brick, species): the species' (B+p)^3 force window staged
+//      in LDS, each particle sums its (p+1)^3 stencil from it
+//   -> bonds + GJF/EM per particle, writing the next positions and counting
+//      the next step's keys.
+// The reference instead materialises (p+1)^3 index/weight tables per particle,
+// scatters with FP32 atomics per group, and runs 4 or 8 FFTs per potential.
+#pragma once
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+#include "fft_core.h"
+#include "spline.h"
+
+namespace mft {
+
+const float2* fft_twiddles_device(int n); // fft.hip
+
+constexpr int kMaxSpecies = 16;
+constexpr int kMaxPots = 32;
+constexpr int kMaxBondTypes = 16;
+constexpr int kMaxChan = 8;
+
+struct KPot {
+    int kind;
+    uint32_t maskI, maskJ;
+    float A;      // current amplitude (ramped)
+    float sig2;
+    float Rp;
+    const float* table; // u(k)/A for erf kinds, else null
+};
+
+struct KParams {
+    int npots;
+    KPot pot[kMaxPots];
+    int charge_pot;     // index of the charges potential or -1
+    int nchan;          // species channels (+1 for charge density)
+    int qchan;          // channel index of the charge density or -1
+    int nforce;         // species receiving grid forces
+    int force_species[kMaxSpecies];
+    int fold;           // 1: every species has one charge; rho_q = sum_s q_s rho_s
+    float qspec[kMaxSpecies];
+    int energy;         // accumulate potential energies
+};
+
+struct BondParams {
+    int ntypes;
+    float k[kMaxBondTypes];
+    float r0[kMaxBondTypes];
+    int style[kMaxBondTypes];
+};
+
+struct GjfParams {
+    float a[kMaxSpecies], b[kMaxSpecies], nmag[kMaxSpecies], cf[kMaxSpecies], cn[kMaxSpecies];
+};
+
+// Brick decomposition of the grid.  A particle belongs to the brick holding
+// its base grid index g (the index the spline stencil starts from, before the
+// -p/2 shift); its stencil then lies in the brick's tile of b+p points per
+// axis starting at the brick origin - p/2.  Tiles have the fixed pitch BMAX+p
+// so stencil offsets are compile-time constants, and are stored in a
+// halo-expanded grid (brick k at k*pitch per axis) so rows of neighbouring
+// bricks are adjacent.  Sort keys are ((brick * nspecies + species) << 6) |
+// sub-cell (4^3 sub-cells of the brick), so the particles of one brick, and of
+// one species in it, are contiguous.
+struct TileGeom {
+    int b[3];    // brick extent per axis (power of two, <= BMAX)
+    int sb[3];   // log2 of b
+    int nb[3];   // bricks per axis
+    int lbmax;   // log2 BMAX
+    int pitch;   // BMAX + p
+    int nbricks;
+    int nspec;   // species per brick in the key
+    int ex[3];   // halo-expanded tile grid: nb * pitch per axis
+};
+
+__device__ __host__ __forceinline__ int tile_key(int g0, int g1, int g2, int s, const TileGeom& tg)
+{
+    int k0 = g0 >> tg.sb[0], k1 = g1 >> tg.sb[1], k2 = g2 >> tg.sb[2];
+    int sh = tg.lbmax - 2; // sub-cells of (BMAX/4)^3 cells
+    int l0 = (g0 & (tg.b[0] - 1)) >> sh, l1 = (g1 & (tg.b[1] - 1)) >> sh, l2 = (g2 & (tg.b[2] - 1)) >> sh;
+    int brick = k0 + tg.nb[0] * (k1 + tg.nb[1] * k2);
+    return ((brick * tg.nspec + s) << 6) | l0 | (l1 << 2) | (l2 << 4);
+}
+
+// Workgroup -> brick: each of the 8 XCDs (round-robin dispatch) gets a
+// contiguous slab of bricks, so tiles that share halos meet in one L2.
+__device__ __forceinline__ int brick_of_block(int blk, int nbricks)
+{
+    const int per = (nbricks + 7) >> 3;
+    const int x = blk & 7, j = blk >> 3;
+    const int full = nbricks - 8 * (per - 1); // XCDs with `per` bricks
+    // blocks are numbered so XCD x handles bricks [base(x), base(x) + cnt(x))
+    const int base = x < full ? x * per : full * per + (x - full) * (per - 1);
+    return base + j;
+}
+
+struct TildEngine {
+    mft_tild_desc d;
+    std::vector<mft_tild_potential> pots;
+    std::vector<float> mass, mob;
+    int N = 0, M = 0, nchan = 0, qchan = -1, nforce = 0, nout = 0, nout4 = 0, qslot = -1, charge_pot = -1;
+    int configured = 0;
+    float dx[3] = {0, 0, 0};
+    float gvol = 0;
+    hipStream_t stream = nullptr;
+    // particle state
+    float* x[2] = {nullptr, nullptr};
+    int cur = 0;
+    float *noise_old = nullptr, *ftot = nullptr, *charge = nullptr;
+    float4* fgrid = nullptr; // per-particle grid force (x, y, z, 0)
+    int *type = nullptr, *nBonds = nullptr, *bondedTo = nullptr, *bondType = nullptr;
+    // binning
+    TileGeom tg;
+    int nkeys = 0;
+    int *count = nullptr, *cell = nullptr, *rank = nullptr, *start = nullptr, *partial = nullptr;
+    int nscan_blocks = 0;
+    float4* spos = nullptr;
+    int2* smeta = nullptr;
+    bool binned = false; // count/cell/rank hold the current positions
+    // grids
+    float* rho = nullptr;   // [nchan][M]
+    float* tiles = nullptr; // [nchan][halo-expanded grid] density tiles
+    int* tile_tab = nullptr; // per-axis tile offsets contributing to each grid coordinate
+    float2* S = nullptr;    // [nchan][M]
+    float2* Y = nullptr;    // [nout][M] force spectra (3 components per field)
+    float4* F4 = nullptr;   // [nout4][M] force fields (x, y, z, 0)
+    int* force_slot = nullptr; // [nspecies] -> force field slot or -1
+    std::vector<int> force_slot_h;
+    std::vector<float*> tables; // erf tables per potential (device)
+    double* epart = nullptr;
+    double* eout = nullptr;   // [kMaxPots + 1]
+    double* bpart = nullptr;
+    BondParams bp;
+    GjfParams gp;
+    KParams kp;
+    size_t dens_lds = 0;
+    int gather_mode = 1;    // 0 direct, 1 LDS windows (mft_tild_autotune picks)
+    int64_t step = 0;
+    int have_state = 0;
+    // timing: event pairs recorded around kernel launches on the engine
+    // stream without host synchronisation, resolved when timing is read
+    bool timing = false;
+    std::vector<hipEvent_t> evpool;
+    std::vector<int> evkind; // kernel id of pair i
+    size_t evused = 0;
+    double tk_ms[MFT_TK_COUNT];
+    int64_t tk_n[MFT_TK_COUNT];
+};
+
+constexpr int kBlock = 256;
+inline unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanChunk = kScanThreads * kScanItems;
+
+template <typename T>
+static int dalloc(T** p, size_t n)
+{
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        set_error(MFT_ERR_OOM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+        return MFT_ERR_OOM;
+    }
+    return MFT_OK;
+}
+
+// ---- grid passes (tild_grid.hip, tild_conv.hip) ----
+int tild_axis(int sign, int real_in, int real_out, const void* in, void* out, AxisGeom g, int nf, long fs,
+              float scale, hipStream_t st);
+int tild_x_inv_vec3(const float2* Y, float4* F4, int nx, int ny, int nz, int nf, hipStream_t st);
+int tild_conv_blocks(int nx, int ny, int nz);
+int tild_conv(TildEngine* e);
+
+// ---- particle passes (tild_particles.hip) ----
+int tild_tiles_configure(TildEngine* e); // brick geometry, LDS sizes, tile tables
+int tild_bin_count(TildEngine* e);
+int tild_scan(TildEngine* e);
+int tild_scatter(TildEngine* e);
+int tild_density(TildEngine* e);
+int tild_gather(TildEngine* e, int mode);
+int tild_integrate(TildEngine* e, int nxt, bool want_f);
+int tild_bond_energy(TildEngine* e);
+
+} // namespace mft

@@ -1,0 +1,1048 @@
+// TILD particle kernels: per-kernel drop-in entry points (grid weights,
+// table-driven density and force mapping, bonds), binning by brick-major
+// key, tile density and reduction, gathers, and the fused bonds + integrator.
+#include "tild_impl.h"
+
+namespace mft {
+
+// ------------------------------------------------- grid weights (T3) ----
+__global__ void k_grid_weights(float* __restrict__ W, int* __restrict__ inds, const float* __restrict__ x,
+                               int ns, int nx, int ny, int nz, float dx0, float dx1, float dx2, int p)
+{
+    int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= ns) return;
+    float w[3][5];
+    int g[3];
+    float dxs[3] = {dx0, dx1, dx2};
+    int n[3] = {nx, ny, nz};
+    for (int j = 0; j < 3; ++j) g[j] = base_index_weights(x[id * 3 + j], dxs[j], p, w[j]);
+    // the device kernel divides by the float product of the float spacings
+    float gvol = dx0 * dx1 * dx2;
+    int shift = p / 2;
+    int G = (p + 1) * (p + 1) * (p + 1);
+    int ct = 0;
+    for (int ix = 0; ix <= p; ++ix) {
+        int n0 = (g[0] + ix - shift + n[0]) % n[0];
+        for (int iy = 0; iy <= p; ++iy) {
+            int n1 = (g[1] + iy - shift + n[1]) % n[1];
+            for (int iz = 0; iz <= p; ++iz) {
+                int n2 = (g[2] + iz - shift + n[2]) % n[2];
+                long o = (long)id * G + ct;
+                inds[o] = n0 + (n1 + n2 * n[1]) * n[0];
+                W[o] = w[0][ix] * w[1][iy] * w[2][iz] / gvol;
+                ++ct;
+            }
+        }
+    }
+}
+
+// Deterministic table-driven density (T4 drop-in): one thread per grid point
+// would need an inverse map, so this entry point accumulates in int64 fixed
+// point with global atomics; integer sums do not depend on the order.
+__global__ void k_fill_density_tables(unsigned long long* __restrict__ acc, unsigned long long* __restrict__ accq,
+                                      const float* __restrict__ q, const int* __restrict__ sites, int nsites,
+                                      const int* __restrict__ inds, const float* __restrict__ W, int G, int M,
+                                      int ns)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsites) return;
+    int p = sites[s];
+    if (p < 0 || p >= ns) return;
+    float qp = q ? q[p] : 0.f;
+    for (int i = 0; i < G; ++i) {
+        long o = (long)p * G + i;
+        int gi = inds[o];
+        if (gi < 0 || gi >= M) continue;
+        float w = W[o];
+        atomicAdd(&acc[gi], (unsigned long long)__float2ll_rn(w * 4294967296.0f));
+        if (accq) atomicAdd(&accq[gi], (unsigned long long)__float2ll_rn(w * qp * 4294967296.0f));
+    }
+}
+
+__global__ void k_fixed_to_float_add(float* __restrict__ out, const unsigned long long* __restrict__ acc, int M)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    out[i] += (float)((double)(long long)acc[i] * (1.0 / 4294967296.0));
+}
+
+__global__ void k_map_forces_tables(float* __restrict__ f, const float* __restrict__ gridF,
+                                    const float* __restrict__ q, const int* __restrict__ sites, int nsites,
+                                    const int* __restrict__ inds, const float* __restrict__ W, int G, float gvol,
+                                    int M, int ns)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsites) return;
+    int p = sites[s];
+    if (p < 0 || p >= ns) return;
+    for (int j = 0; j < 3; ++j) {
+        float acc = 0.f;
+        for (int i = 0; i < G; ++i) {
+            long o = (long)p * G + i;
+            int gi = inds[o];
+            if (gi < 0 || gi >= M) continue;
+            acc += gridF[(long)gi * 3 + j] * W[o] * gvol;
+        }
+        f[p * 3 + j] += q ? acc * q[p] : acc;
+    }
+}
+
+// ----------------------------------------------------------- bonds ------
+__device__ __forceinline__ float min_image(float d, float L, float half)
+{
+    if (d > half) d -= L;
+    else if (d < -half) d += L;
+    return d;
+}
+
+// d_bonds (ps_deviceBonds.cu:7-79) for one particle.
+__device__ __forceinline__ void bond_force(int i, int ns, const float* __restrict__ x,
+                                           const int* __restrict__ nBonds, const int* __restrict__ bondedTo,
+                                           const int* __restrict__ bondType, int maxbonds, const BondParams& bp, float L0, float L1, float L2,
+                                           float x0, float x1, float x2, float* fo)
+{
+    int nb = nBonds[i];
+    if (nb > maxbonds) nb = maxbonds;
+    for (int b = 0; b < nb; ++b) {
+        int id2 = bondedTo[i * maxbonds + b];
+        int bt = bondType[i * maxbonds + b];
+        if (id2 < 0 || id2 >= ns || bt < 0 || bt >= bp.ntypes) continue;
+        float d0 = min_image(x0 - x[id2 * 3 + 0], L0, 0.5f * L0);
+        float d1 = min_image(x1 - x[id2 * 3 + 1], L1, 0.5f * L1);
+        float d2 = min_image(x2 - x[id2 * 3 + 2], L2, 0.5f * L2);
+        float r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        float mf = 0.f;
+        if (r2 > 1.0e-4f) {
+            float r = sqrtf(r2);
+            if (bp.style[bt] == 0) mf = 2.0f * bp.k[bt] * (r - bp.r0[bt]) / r;
+            else if (bp.style[bt] == 1) {
+                float u = r / bp.r0[bt];
+                mf = 2.0f * bp.k[bt] * r / (1.0f - u * u) / r;
+            }
+        }
+        fo[0] -= mf * d0;
+        fo[1] -= mf * d1;
+        fo[2] -= mf * d2;
+    }
+}
+
+__global__ void k_bonds(float* __restrict__ f, const int* __restrict__ nBonds, const int* __restrict__ bondedTo,
+                        const int* __restrict__ bondType, BondParams bp, const float* __restrict__ x, float L0,
+                        float L1, float L2, int ns, int maxbonds)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    float fo[3] = {0.f, 0.f, 0.f};
+    bond_force(i, ns, x, nBonds, bondedTo, bondType, maxbonds, bp, L0, L1, L2, x[i * 3], x[i * 3 + 1], x[i * 3 + 2],
+               fo);
+    f[i * 3 + 0] += fo[0];
+    f[i * 3 + 1] += fo[1];
+    f[i * 3 + 2] += fo[2];
+}
+
+// d_bondStressEnergy energy part (ps_deviceBonds.cu:85-171), FP64 block sums.
+__global__ void k_bond_energy(double* __restrict__ partial, const int* __restrict__ nBonds,
+                              const int* __restrict__ bondedTo, const int* __restrict__ bondType, BondParams bp,
+                              const float* __restrict__ x, float L0, float L1, float L2, int ns, int maxbonds)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if (i < ns) {
+        float Ls[3] = {L0, L1, L2};
+        for (int b = 0; b < nBonds[i]; ++b) {
+            int id2 = bondedTo[i * maxbonds + b];
+            int bt = bondType[i * maxbonds + b];
+            float r2 = 0.f;
+            for (int j = 0; j < 3; ++j) {
+                // 0.5 * Lh as in ps_deviceBonds.cu:132-133
+                float d = min_image(x[i * 3 + j] - x[id2 * 3 + j], Ls[j], 0.25f * Ls[j]);
+                r2 += d * d;
+            }
+            if (r2 > 1.0e-4f) {
+                float r = sqrtf(r2);
+                if (bp.style[bt] == 0) {
+                    float a = r - bp.r0[bt];
+                    e += (double)(bp.k[bt] * a * a);
+                } else if (bp.style[bt] == 1 && r / bp.r0[bt] < 1.f) {
+                    float u = r / bp.r0[bt];
+                    float arg = 1.f / (1.f - u * u);
+                    e += (double)(2.0f * bp.k[bt] * arg * r);
+                }
+            }
+        }
+    }
+    __shared__ double red[4];
+    e = wave_sum(e);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void k_sum_partials(double* __restrict__ out, const double* __restrict__ partial, int n)
+{
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += partial[i];
+    __shared__ double red[16];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) t += red[w];
+        *out = t;
+    }
+}
+
+
+__device__ __forceinline__ int block_exclusive_scan(int v, int* sh, int* total)
+{
+    // wave inclusive scan then across waves
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+    }
+    if (lane == 63) sh[wid] = inc;
+    __syncthreads();
+    int woff = 0, tot = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+        if (w < wid) woff += sh[w];
+        tot += sh[w];
+    }
+    __syncthreads();
+    *total = tot;
+    return woff + inc - v;
+}
+// ------------------------------------------------------------ binning ---
+// Base grid index of one coordinate wrapped into [0, n): the brick a particle
+// is sorted into and the first point (+ p/2) of its spline stencil.
+__device__ __forceinline__ int wrapped_base(float x, float dx, int p, int n) { return wrap(base_index(x, dx, p), n); }
+
+__device__ __forceinline__ int key_of(float x0, float x1, float x2, int s, float dx0, float dx1, float dx2, int p,
+                                      int nx, int ny, int nz, const TileGeom& tg)
+{
+    return tile_key(wrapped_base(x0, dx0, p, nx), wrapped_base(x1, dx1, p, ny), wrapped_base(x2, dx2, p, nz), s, tg);
+}
+
+__global__ void k_bin_count(const float* __restrict__ x, const int* __restrict__ type, int ns,
+                            int* __restrict__ count, int* __restrict__ cell, int* __restrict__ rank, float dx0,
+                            float dx1, float dx2, int p, int nx, int ny, int nz, TileGeom tg)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    int c = key_of(x[i * 3], x[i * 3 + 1], x[i * 3 + 2], type[i], dx0, dx1, dx2, p, nx, ny, nz, tg);
+    cell[i] = c;
+    rank[i] = atomicAdd(&count[c], 1);
+}
+
+__global__ void k_scan_reduce(const int* __restrict__ count, int M, int* __restrict__ partial)
+{
+    __shared__ int sh[16];
+    long base = (long)blockIdx.x * kScanChunk;
+    int s = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        long i = base + (long)k * kScanThreads + threadIdx.x;
+        if (i < M) s += count[i];
+    }
+    int tot;
+    block_exclusive_scan(s, sh, &tot);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+__global__ void k_scan_partials(int* __restrict__ partial, int n)
+{
+    __shared__ int sh[16];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += blockDim.x) {
+        int i = base + threadIdx.x;
+        int v = i < n ? partial[i] : 0;
+        int tot;
+        int ex = block_exclusive_scan(v, sh, &tot);
+        if (i < n) partial[i] = ex + carry;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += tot;
+        __syncthreads();
+    }
+}
+
+// start[i] = exclusive prefix of count; count cleared for the next binning.
+__global__ void k_scan_apply(int* __restrict__ count, int M, const int* __restrict__ partial,
+                             int* __restrict__ start, int ns)
+{
+    __shared__ int sh[16];
+    long base = (long)blockIdx.x * kScanChunk + (long)threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int s = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        long i = base + k;
+        v[k] = i < M ? count[i] : 0;
+        s += v[k];
+    }
+    int tot;
+    int ex = block_exclusive_scan(s, sh, &tot) + partial[blockIdx.x];
+    for (int k = 0; k < kScanItems; ++k) {
+        long i = base + k;
+        if (i < M) {
+            start[i] = ex;
+            count[i] = 0;
+        }
+        ex += v[k];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) start[M] = ns;
+}
+
+__global__ void k_bin_scatter(const float* __restrict__ x, const float* __restrict__ q, const int* __restrict__ type,
+                              int ns, const int* __restrict__ cell, const int* __restrict__ rank,
+                              const int* __restrict__ start, float4* __restrict__ spos, int2* __restrict__ smeta)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    int slot = start[cell[i]] + rank[i];
+    spos[slot] = make_float4(x[i * 3], x[i * 3 + 1], x[i * 3 + 2], q ? q[i] : 0.f);
+    smeta[slot] = make_int2(i, type[i]);
+}
+
+// --------------------------------------------------------- tile density ---
+constexpr int kTileThreads = 256;
+
+// Spline weights of one particle and its brick-local stencil origin; false
+// when the particle is not in the brick (cannot happen for keys computed from
+// the same positions; the check keeps LDS accesses in bounds regardless).
+template <int P>
+__device__ __forceinline__ bool stencil_in_brick(const float4& pp, float dx0, float dx1, float dx2, float i0,
+                                                 float i1, float i2, int nx, int ny, int nz, int o0, int o1, int o2,
+                                                 const TileGeom& tg, float* wx, float* wy, float* wz, int* l)
+{
+    int g0 = wrap(base_index_weights_fast(pp.x, dx0, i0, P, wx), nx);
+    int g1 = wrap(base_index_weights_fast(pp.y, dx1, i1, P, wy), ny);
+    int g2 = wrap(base_index_weights_fast(pp.z, dx2, i2, P, wz), nz);
+    l[0] = g0 - o0;
+    l[1] = g1 - o1;
+    l[2] = g2 - o2;
+    return (unsigned)l[0] < (unsigned)tg.b[0] && (unsigned)l[1] < (unsigned)tg.b[1] &&
+           (unsigned)l[2] < (unsigned)tg.b[2];
+}
+
+// Upper bound of one (p+1)^3 spline weight w_x w_y w_z.
+__host__ __device__ constexpr float stencil_wmax(int p)
+{
+    return p <= 1 ? 1.0f : (p == 2 ? 0.43f : (p == 3 ? 0.30f : 0.22f));
+}
+
+// Deposit the brick's particles into the LDS tile.  ACC is the fixed-point
+// accumulator word: 32-bit with 2^sc scaling when n w_max 2^sc < 2^32 bounds
+// every sum, else 64-bit with 2^31 (and signed charges).  Integer sums do not
+// depend on the order of the atomics.
+template <int P, int BM, typename ACC>
+__device__ __forceinline__ void deposit_tile(ACC* acc, int qchan, const float4* __restrict__ spos,
+                                             const int2* __restrict__ smeta, int pb, int n, float sc, float dx0,
+                                             float dx1, float dx2, int nx, int ny, int nz, int o0, int o1, int o2,
+                                             const TileGeom& tg)
+{
+    constexpr int TP = BM + P, T3 = TP * TP * TP;
+    const float i0 = 1.0f / dx0, i1 = 1.0f / dx1, i2 = 1.0f / dx2;
+    // particles are in cell order, so neighbouring ones share stencil points:
+    // the 64 lanes of a wave take particles n/64 apart (a 64 x S transpose)
+    // so one atomic instruction rarely hits the same LDS word twice
+    const int S = (n + 63) >> 6;
+    for (int j = threadIdx.x; j < 64 * S; j += kTileThreads) {
+        const int r = (j & 63) * S + (j >> 6);
+        if (r >= n) continue;
+        const int it = pb + r;
+        const float4 pp = spos[it];
+        const int typ = smeta[it].y;
+        float wx[P + 1], wy[P + 1], wz[P + 1];
+        int l[3];
+        if (!stencil_in_brick<P>(pp, dx0, dx1, dx2, i0, i1, i2, nx, ny, nz, o0, o1, o2, tg, wx, wy, wz, l)) continue;
+        ACC* a = acc + typ * T3 + l[0] + TP * (l[1] + TP * l[2]);
+#pragma unroll
+        for (int iz = 0; iz <= P; ++iz) {
+#pragma unroll
+            for (int iy = 0; iy <= P; ++iy) {
+                const float wyz = wy[iy] * wz[iz] * sc;
+#pragma unroll
+                for (int ix = 0; ix <= P; ++ix)
+                    atomicAdd(a + ix + TP * (iy + TP * iz), (ACC)__float2uint_rz(fmaf(wx[ix], wyz, 0.5f)));
+            }
+        }
+        if (sizeof(ACC) == 8 && qchan >= 0 && pp.w != 0.f) {
+            ACC* aq = acc + qchan * T3 + l[0] + TP * (l[1] + TP * l[2]);
+#pragma unroll
+            for (int iz = 0; iz <= P; ++iz) {
+#pragma unroll
+                for (int iy = 0; iy <= P; ++iy) {
+                    const float wyz = wy[iy] * wz[iz] * pp.w * sc;
+#pragma unroll
+                    for (int ix = 0; ix <= P; ++ix)
+                        atomicAdd(aq + ix + TP * (iy + TP * iz), (ACC)__float2ll_rn(wx[ix] * wyz));
+                }
+            }
+        }
+    }
+}
+
+// One workgroup per brick: the brick's particles (contiguous in key order)
+// spread their (p+1)^3 weights over the brick's tile in LDS in fixed point;
+// the tile is written as float times 1/gvol.
+template <int P, int BM>
+__global__ void __launch_bounds__(kTileThreads) k_density_tile(
+    float* __restrict__ tiles, int nchan, int qchan, const float4* __restrict__ spos, const int2* __restrict__ smeta,
+    const int* __restrict__ start, TileGeom tg, int nx, int ny, int nz, float dx0, float dx1, float dx2,
+    double inv_gvol)
+{
+    constexpr int TP = BM + P, T3 = TP * TP * TP;
+    extern __shared__ unsigned long long acc64[]; // [nchan][T3] (32-bit words in the narrow case)
+    unsigned int* acc32 = reinterpret_cast<unsigned int*>(acc64);
+    const int brick = brick_of_block(blockIdx.x, tg.nbricks);
+    const int k0 = brick % tg.nb[0], k1 = (brick / tg.nb[0]) % tg.nb[1], k2 = brick / (tg.nb[0] * tg.nb[1]);
+    const int o0 = k0 << tg.sb[0], o1 = k1 << tg.sb[1], o2 = k2 << tg.sb[2];
+    const int pb = start[(brick * tg.nspec) << 6], pe = start[((brick + 1) * tg.nspec) << 6];
+    const int n = pe - pb;
+    // largest power-of-two scale 2^sc with n (w_max 2^sc + 1/2) < 2^32, the
+    // bound of one point's sum of rounded deposits (checked for sc >= 16)
+    const float bound = (float)n * (stencil_wmax(P) + 1.0f / 65536.0f) + 1.0f;
+    int need = 1;
+    while (need < 31 && (float)(1u << need) <= bound) ++need;
+    const int sc = 32 - need;
+    const bool narrow = qchan < 0 && sc >= 16;
+    // tile rows go to the halo-expanded grid: brick k at k * TP per axis
+    const long cstride = (long)tg.ex[0] * tg.ex[1] * tg.ex[2];
+    float* dst = tiles + (long)k0 * TP + (long)tg.ex[0] * (k1 * TP + (long)tg.ex[1] * (k2 * TP));
+    auto out = [&](int i) -> float* {
+        const int c = i / T3, t = i - c * T3;
+        const int t0 = t % TP, t12 = t / TP, t1 = t12 % TP, t2 = t12 / TP;
+        return dst + c * cstride + t0 + (long)tg.ex[0] * (t1 + (long)tg.ex[1] * t2);
+    };
+    if (narrow) {
+        for (int i = threadIdx.x; i < nchan * T3; i += kTileThreads) acc32[i] = 0u;
+        __syncthreads();
+        deposit_tile<P, BM, unsigned int>(acc32, qchan, spos, smeta, pb, n, (float)(1u << sc), dx0, dx1, dx2, nx,
+                                          ny, nz, o0, o1, o2, tg);
+        __syncthreads();
+        const double scale = inv_gvol / (double)(1u << sc);
+        for (int i = threadIdx.x; i < nchan * T3; i += kTileThreads) *out(i) = (float)((double)acc32[i] * scale);
+    } else {
+        for (int i = threadIdx.x; i < nchan * T3; i += kTileThreads) acc64[i] = 0ull;
+        __syncthreads();
+        deposit_tile<P, BM, unsigned long long>(acc64, qchan, spos, smeta, pb, n, 2147483648.0f, dx0, dx1, dx2, nx,
+                                                ny, nz, o0, o1, o2, tg);
+        __syncthreads();
+        const double scale = inv_gvol / 2147483648.0;
+        for (int i = threadIdx.x; i < nchan * T3; i += kTileThreads)
+            *out(i) = (float)((double)(long long)acc64[i] * scale);
+    }
+}
+
+// rho at each grid point: the tile entries of every brick whose tile covers
+// it, summed in a fixed order.  tab holds, per axis and coordinate, the count
+// and the (brick, tile index) offsets of the covering tiles along that axis.
+// One 64-lane workgroup per (x segment, y, z): the y and z entries are uniform.
+__global__ void __launch_bounds__(64) k_tile_reduce(float* __restrict__ rho, const float* __restrict__ tiles,
+                                                    const int* __restrict__ tab, int nx, int ny, int nz, int nchan,
+                                                    long cstride)
+{
+    const long M = (long)nx * ny * nz;
+    const int gx = blockIdx.x * 64 + threadIdx.x;
+    const int gy = blockIdx.y, gz = blockIdx.z;
+    if (gx >= nx) return;
+    const int* tx = tab + 5 * gx;
+    const int* ty = tab + 5 * (nx + gy);
+    const int* tz = tab + 5 * (nx + ny + gz);
+    const int cx = tx[0], cy = ty[0], cz = tz[0];
+    int ox[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) ox[c] = c < cx ? tx[1 + c] : 0;
+    const long id = gx + (long)nx * (gy + (long)ny * gz);
+    for (int ch = 0; ch < nchan; ++ch) {
+        const float* tc = tiles + ch * cstride;
+        float s = 0.f;
+        for (int a = 0; a < cz; ++a)
+            for (int b = 0; b < cy; ++b) {
+                const float* row = tc + tz[1 + a] + ty[1 + b];
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    if (c < cx) s += row[ox[c]];
+            }
+        rho[ch * M + id] = s;
+    }
+}
+
+// ------------------------------------------------------------- gather ---
+// Force on each particle from its species' total grid force (potentials and,
+// with folded charges, q_s times the field) and, when charges are not uniform
+// per species, q times the separate charge field (d_mapGridForcesToPartics and
+// d_mapGridChargeForcesToPartics, device_utils.cu:358-461).  Direct variant:
+// one thread per particle in key order, stencil loaded from the global fields.
+template <int P>
+__global__ void __launch_bounds__(256) k_gather(float4* __restrict__ fgrid, const float4* __restrict__ F4,
+                                                const int* __restrict__ force_slot, int qslot,
+                                                const float4* __restrict__ spos, const int2* __restrict__ smeta,
+                                                int ns, int nx, int ny, int nz, float dx0, float dx1, float dx2)
+{
+    constexpr int p = P;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    float4 pp = spos[i];
+    int2 meta = smeta[i];
+    const long M = (long)nx * ny * nz;
+    int fs = force_slot[meta.y];
+    bool charged = qslot >= 0 && pp.w != 0.f;
+    float f[3] = {0.f, 0.f, 0.f};
+    if (fs >= 0 || charged) {
+        float wx[5], wy[5], wz[5];
+        int g0 = base_index_weights_fast(pp.x, dx0, 1.0f / dx0, p, wx);
+        int g1 = base_index_weights_fast(pp.y, dx1, 1.0f / dx1, p, wy);
+        int g2 = base_index_weights_fast(pp.z, dx2, 1.0f / dx2, p, wz);
+        const int shift = p / 2;
+        const float4* Fs = F4 + (fs >= 0 ? fs : 0) * M;
+        const float4* Fq = F4 + (qslot >= 0 ? qslot : 0) * M;
+        float fq[3] = {0.f, 0.f, 0.f};
+        int xs[p + 1], ys[p + 1], zs[p + 1];
+        {
+            int bx = wrap(g0 - shift, nx), by = wrap(g1 - shift, ny), bz = wrap(g2 - shift, nz);
+#pragma unroll
+            for (int i = 0; i <= p; ++i) {
+                xs[i] = bx + i >= nx ? bx + i - nx : bx + i;
+                ys[i] = by + i >= ny ? by + i - ny : by + i;
+                zs[i] = bz + i >= nz ? bz + i - nz : bz + i;
+            }
+        }
+#pragma unroll
+        for (int iz = 0; iz <= p; ++iz) {
+#pragma unroll
+            for (int iy = 0; iy <= p; ++iy) {
+                float wyz = wy[iy] * wz[iz];
+                long row = (long)nx * (ys[iy] + (long)ny * zs[iz]);
+#pragma unroll
+                for (int ix = 0; ix <= p; ++ix) {
+                    long g = row + xs[ix];
+                    float w = wx[ix] * wyz;
+                    if (fs >= 0) {
+                        float4 v = Fs[g];
+                        f[0] += v.x * w;
+                        f[1] += v.y * w;
+                        f[2] += v.z * w;
+                    }
+                    if (charged) {
+                        float4 v = Fq[g];
+                        fq[0] += v.x * w;
+                        fq[1] += v.y * w;
+                        fq[2] += v.z * w;
+                    }
+                }
+            }
+        }
+        if (charged) {
+            f[0] += fq[0] * pp.w;
+            f[1] += fq[1] * pp.w;
+            f[2] += fq[2] * pp.w;
+        }
+    }
+    fgrid[meta.x] = make_float4(f[0], f[1], f[2], 0.f);
+}
+
+// Tile variant.  The force field window of a brick ((b+p)^3 points from
+// origin - p/2, periodic) is staged in LDS by 16 lanes per window row, every
+// load of a lane issued before its first LDS store.
+template <int P, int BM>
+__device__ __forceinline__ void stage_window(float4* ft, int* xg, int* yo, int* zo, unsigned char (*rows)[2],
+                                             const float4* __restrict__ F, int o0, int o1, int o2, int e0, int e1,
+                                             int e2, int nx, int ny, int nz)
+{
+    constexpr int TP = BM + P, shift = P / 2;
+    if ((int)threadIdx.x < TP) {
+        const int t = threadIdx.x;
+        xg[t] = wrap(o0 - shift + t, nx);
+        yo[t] = nx * wrap(o1 - shift + t, ny);
+        zo[t] = nx * ny * wrap(o2 - shift + t, nz);
+    }
+    const int nrows = e1 * e2;
+    for (int r = threadIdx.x; r < nrows; r += kTileThreads) {
+        rows[r][0] = (unsigned char)(r % e1);
+        rows[r][1] = (unsigned char)(r / e1);
+    }
+    __syncthreads();
+    constexpr int RMAX = (TP * TP + 15) / 16;
+    const int lane16 = threadIdx.x & 15, row16 = threadIdx.x >> 4;
+    if (lane16 < e0) {
+        const float4* src = F + xg[lane16];
+        float4* dst = ft + lane16;
+        // rows past the window re-load the last row (in bounds) and skip the
+        // store, so the arrays stay in registers
+        float4 v[RMAX];
+        int dofs[RMAX];
+#pragma unroll
+        for (int j = 0; j < RMAX; ++j) {
+            const int r = min(row16 + 16 * j, nrows - 1);
+            const int t1 = rows[r][0], t2 = rows[r][1];
+            v[j] = src[yo[t1] + zo[t2]];
+            dofs[j] = TP * (t1 + TP * t2);
+        }
+#pragma unroll
+        for (int j = 0; j < RMAX; ++j)
+            if (row16 + 16 * j < nrows) dst[dofs[j]] = v[j];
+    }
+    __syncthreads();
+}
+
+// sum_w w F over the particle's stencil, from the staged window
+template <int P, int BM>
+__device__ __forceinline__ float3 stencil_sum(const float4* ft, const int* l, const float* wx, const float* wy,
+                                              const float* wz)
+{
+    constexpr int TP = BM + P;
+    const float4* t = ft + l[0] + TP * (l[1] + TP * l[2]);
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll
+    for (int iz = 0; iz <= P; ++iz) {
+#pragma unroll
+        for (int iy = 0; iy <= P; ++iy) {
+            const float wyz = wy[iy] * wz[iz];
+#pragma unroll
+            for (int ix = 0; ix <= P; ++ix) {
+                const float w = wx[ix] * wyz;
+                const float4 v = t[ix + TP * (iy + TP * iz)];
+                a0 += v.x * w;
+                a1 += v.y * w;
+                a2 += v.z * w;
+            }
+        }
+    }
+    return make_float3(a0, a1, a2);
+}
+
+struct GatherMap {
+    int slot[kMaxSpecies]; // force field of each species, or -1
+};
+
+// One workgroup per (brick, species): the species' particles of the brick are
+// contiguous in key order; the workgroup stages that species' force field
+// window and writes the particles' grid forces (zero for a species no
+// potential acts on).
+template <int P, int BM>
+__global__ void __launch_bounds__(kTileThreads) k_gather_species(
+    float4* __restrict__ fgrid, const float4* __restrict__ F4, GatherMap gm, const float4* __restrict__ spos,
+    const int2* __restrict__ smeta, const int* __restrict__ start, TileGeom tg, int nx, int ny, int nz, float dx0,
+    float dx1, float dx2)
+{
+    constexpr int TP = BM + P, T3 = TP * TP * TP;
+    __shared__ float4 ft[T3];
+    __shared__ int xg[TP], yo[TP], zo[TP];
+    __shared__ unsigned char rows[TP * TP][2];
+    const int brick = brick_of_block(blockIdx.x, tg.nbricks);
+    const int s = blockIdx.y;
+    const int key0 = (brick * tg.nspec + s) << 6;
+    const int pb = start[key0], pe = start[key0 + 64];
+    if (pb == pe) return;
+    const int f = gm.slot[s];
+    if (f < 0) {
+        for (int it = pb + (int)threadIdx.x; it < pe; it += kTileThreads)
+            fgrid[smeta[it].x] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    const int k0 = brick % tg.nb[0], k1 = (brick / tg.nb[0]) % tg.nb[1], k2 = brick / (tg.nb[0] * tg.nb[1]);
+    const int o0 = k0 << tg.sb[0], o1 = k1 << tg.sb[1], o2 = k2 << tg.sb[2];
+    const int e0 = min(tg.b[0], nx - o0) + P, e1 = min(tg.b[1], ny - o1) + P, e2 = min(tg.b[2], nz - o2) + P;
+    const long M = (long)nx * ny * nz;
+    stage_window<P, BM>(ft, xg, yo, zo, rows, F4 + f * M, o0, o1, o2, e0, e1, e2, nx, ny, nz);
+    const float i0 = 1.0f / dx0, i1 = 1.0f / dx1, i2 = 1.0f / dx2;
+    for (int it = pb + (int)threadIdx.x; it < pe; it += kTileThreads) {
+        const float4 pp = spos[it];
+        const int pid = smeta[it].x;
+        float wx[P + 1], wy[P + 1], wz[P + 1];
+        int l[3];
+        float3 a = make_float3(0.f, 0.f, 0.f);
+        if (stencil_in_brick<P>(pp, dx0, dx1, dx2, i0, i1, i2, nx, ny, nz, o0, o1, o2, tg, wx, wy, wz, l))
+            a = stencil_sum<P, BM>(ft, l, wx, wy, wz);
+        fgrid[pid] = make_float4(a.x, a.y, a.z, 0.f);
+    }
+}
+
+// Separate charge field (charges not uniform per species): one workgroup per
+// brick adds q times the field to every charged particle's grid force, after
+// the species pass.
+template <int P, int BM>
+__global__ void __launch_bounds__(kTileThreads) k_gather_charge(
+    float4* __restrict__ fgrid, const float4* __restrict__ Fq, const float4* __restrict__ spos,
+    const int2* __restrict__ smeta, const int* __restrict__ start, TileGeom tg, int nx, int ny, int nz, float dx0,
+    float dx1, float dx2)
+{
+    constexpr int TP = BM + P, T3 = TP * TP * TP;
+    __shared__ float4 ft[T3];
+    __shared__ int xg[TP], yo[TP], zo[TP];
+    __shared__ unsigned char rows[TP * TP][2];
+    const int brick = brick_of_block(blockIdx.x, tg.nbricks);
+    const int pb = start[(brick * tg.nspec) << 6], pe = start[((brick + 1) * tg.nspec) << 6];
+    if (pb == pe) return;
+    const int k0 = brick % tg.nb[0], k1 = (brick / tg.nb[0]) % tg.nb[1], k2 = brick / (tg.nb[0] * tg.nb[1]);
+    const int o0 = k0 << tg.sb[0], o1 = k1 << tg.sb[1], o2 = k2 << tg.sb[2];
+    const int e0 = min(tg.b[0], nx - o0) + P, e1 = min(tg.b[1], ny - o1) + P, e2 = min(tg.b[2], nz - o2) + P;
+    stage_window<P, BM>(ft, xg, yo, zo, rows, Fq, o0, o1, o2, e0, e1, e2, nx, ny, nz);
+    const float i0 = 1.0f / dx0, i1 = 1.0f / dx1, i2 = 1.0f / dx2;
+    for (int it = pb + (int)threadIdx.x; it < pe; it += kTileThreads) {
+        const float4 pp = spos[it];
+        if (pp.w == 0.f) continue;
+        const int pid = smeta[it].x;
+        float wx[P + 1], wy[P + 1], wz[P + 1];
+        int l[3];
+        if (!stencil_in_brick<P>(pp, dx0, dx1, dx2, i0, i1, i2, nx, ny, nz, o0, o1, o2, tg, wx, wy, wz, l)) continue;
+        const float3 a = stencil_sum<P, BM>(ft, l, wx, wy, wz);
+        const float4 o = fgrid[pid];
+        fgrid[pid] = make_float4(o.x + pp.w * a.x, o.y + pp.w * a.y, o.z + pp.w * a.z, 0.f);
+    }
+}
+
+// ------------------------------------------------- bonds + integrator ---
+// Bonds (ps_deviceBonds.cu:7-79) plus GJF (ps_integratorGJF.cu:102-163) or
+// EM (ps_integratorEM.cu:39-82) on group "all".  Reads x (current), writes the
+// next positions into xnext (the buffer that held x_old for GJF), counts the
+// next step's keys.  integ = 2 only writes the total force.
+__global__ void __launch_bounds__(256) k_bonds_integrate(
+    const float* __restrict__ x, float* __restrict__ xnext, float* __restrict__ noise_old,
+    const float4* __restrict__ fgrid, float* __restrict__ fout, const int* __restrict__ type,
+    const int* __restrict__ nBonds, const int* __restrict__ bondedTo, const int* __restrict__ bondType,
+    int maxbonds, BondParams bp, GjfParams gp, int integ, float delt, float L0, float L1, float L2, int ns,
+    uint64_t seed, uint32_t step, int* __restrict__ count, int* __restrict__ cell, int* __restrict__ rank,
+    float dx0, float dx1, float dx2, int p, int nx, int ny, int nz, TileGeom tg)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    float xc[3] = {x[i * 3], x[i * 3 + 1], x[i * 3 + 2]};
+    const float4 fg = fgrid[i];
+    float fo[3] = {fg.x, fg.y, fg.z};
+    bond_force(i, ns, x, nBonds, bondedTo, bondType, maxbonds, bp, L0, L1, L2, xc[0], xc[1], xc[2], fo);
+    if (fout) {
+        fout[i * 3 + 0] = fo[0];
+        fout[i * 3 + 1] = fo[1];
+        fout[i * 3 + 2] = fo[2];
+    }
+    if (integ == MFT_INT_NONE) return;
+    float z[4];
+    normal4((uint32_t)i, step, TAG_TILD, 0u, seed, z);
+    const float Ls[3] = {L0, L1, L2};
+    float xn[3];
+    if (integ == MFT_INT_GJF) {
+        int ty = type[i];
+        float a = gp.a[ty], b = gp.b[ty], nm = gp.nmag[ty], cf = gp.cf[ty], cn = gp.cn[ty];
+        for (int j = 0; j < 3; ++j) {
+            float nn = nm * z[j];
+            float xo = xnext[i * 3 + j];
+            if (xo - xc[j] > Ls[j] / 2.0f) xo -= Ls[j];
+            else if (xo - xc[j] < -Ls[j] / 2.0f) xo += Ls[j];
+            float v = 2.0f * b * xc[j] - a * xo + cf * fo[j] + cn * (nn + noise_old[i * 3 + j]);
+            if (v > Ls[j]) v -= Ls[j];
+            else if (v < 0.0f) v += Ls[j];
+            xn[j] = v;
+            noise_old[i * 3 + j] = nn;
+        }
+    } else {
+        float nm = sqrtf(2.0f * delt);
+        for (int j = 0; j < 3; ++j) {
+            float v = xc[j] + delt * fo[j] + nm * z[j];
+            if (v > Ls[j]) v -= Ls[j];
+            else if (v < 0.0f) v += Ls[j];
+            xn[j] = v;
+        }
+    }
+    xnext[i * 3 + 0] = xn[0];
+    xnext[i * 3 + 1] = xn[1];
+    xnext[i * 3 + 2] = xn[2];
+    if (count) {
+        int c = key_of(xn[0], xn[1], xn[2], type[i], dx0, dx1, dx2, p, nx, ny, nz, tg);
+        cell[i] = c;
+        rank[i] = atomicAdd(&count[c], 1);
+    }
+}
+
+// ----------------------------------------------------------- launchers ---
+static int pow2_floor(int v, int cap)
+{
+    int b = 1;
+    while (b * 2 <= v && b * 2 <= cap) b *= 2;
+    return b;
+}
+
+static int log2i(int v)
+{
+    int l = 0;
+    while ((1 << l) < v) ++l;
+    return l;
+}
+
+// Brick geometry, key space, LDS budgets and the per-axis tables of tile
+// offsets covering each grid coordinate.  Bricks are 8^3 points for up to 4
+// density channels, 4^3 beyond.
+int tild_tiles_configure(TildEngine* e)
+{
+    const mft_tild_desc& d = e->d;
+    const int p = d.pmeorder, shift = p / 2;
+    const int BM = e->nchan <= 4 ? 8 : 4;
+    TileGeom& tg = e->tg;
+    const int n[3] = {d.nx, d.ny, d.nz};
+    for (int a = 0; a < 3; ++a) {
+        tg.b[a] = pow2_floor(n[a], BM);
+        tg.sb[a] = log2i(tg.b[a]);
+        tg.nb[a] = (n[a] + tg.b[a] - 1) / tg.b[a];
+    }
+    tg.lbmax = log2i(BM);
+    tg.pitch = BM + p;
+    tg.nbricks = tg.nb[0] * tg.nb[1] * tg.nb[2];
+    tg.nspec = d.nspecies;
+    for (int a = 0; a < 3; ++a) tg.ex[a] = tg.nb[a] * tg.pitch;
+    const long nkeys = ((long)tg.nbricks * tg.nspec) << 6;
+    const long T3 = (long)tg.pitch * tg.pitch * tg.pitch;
+    MFT_REQUIRE(nkeys < (1L << 30) && (long)tg.nbricks * e->nchan * T3 < (1L << 31), MFT_ERR_UNSUPPORTED,
+                "grid too large for the brick decomposition");
+    e->nkeys = (int)nkeys;
+    e->dens_lds = sizeof(unsigned long long) * (size_t)e->nchan * T3;
+    MFT_REQUIRE(e->dens_lds <= 64 * 1024, MFT_ERR_UNSUPPORTED, "density tile exceeds LDS");
+
+    // per-axis covering tiles: brick k covers coordinates k b - shift + t,
+    // t in [0, len_k + p), periodically; its entry t sits at k pitch + t of
+    // the halo-expanded grid along that axis
+    std::vector<int> tab(5 * (size_t)(n[0] + n[1] + n[2]), 0);
+    const long estride[3] = {1, tg.ex[0], (long)tg.ex[0] * tg.ex[1]};
+    int base = 0;
+    for (int a = 0; a < 3; ++a) {
+        for (int X = 0; X < n[a]; ++X) {
+            int* row = &tab[5 * (size_t)(base + X)];
+            int cnt = 0;
+            for (int k = 0; k < tg.nb[a]; ++k) {
+                const int len = std::min(tg.b[a], n[a] - k * tg.b[a]) + p;
+                int t0 = ((X - k * tg.b[a] + shift) % n[a] + n[a]) % n[a];
+                for (int t = t0; t < len; t += n[a]) {
+                    MFT_REQUIRE(cnt < 4, MFT_ERR_UNSUPPORTED, "tile cover table overflow");
+                    row[1 + cnt++] = (int)(((long)k * tg.pitch + t) * estride[a]);
+                }
+            }
+            row[0] = cnt;
+        }
+        base += n[a];
+    }
+    if (e->tile_tab) (void)hipFree(e->tile_tab);
+    e->tile_tab = nullptr;
+    int rc;
+    if ((rc = dalloc(&e->tile_tab, tab.size()))) return rc;
+    MFT_HIP(hipMemcpy(e->tile_tab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
+    if (e->tiles) (void)hipFree(e->tiles);
+    e->tiles = nullptr;
+    if ((rc = dalloc(&e->tiles, (size_t)tg.nbricks * e->nchan * T3))) return rc;
+    // key-space arrays
+    for (void* pbuf : {(void*)e->count, (void*)e->start, (void*)e->partial})
+        if (pbuf) (void)hipFree(pbuf);
+    e->count = e->start = e->partial = nullptr;
+    if ((rc = dalloc(&e->count, (size_t)e->nkeys))) return rc;
+    if ((rc = dalloc(&e->start, (size_t)e->nkeys + 1))) return rc;
+    e->nscan_blocks = (e->nkeys + kScanChunk - 1) / kScanChunk;
+    if ((rc = dalloc(&e->partial, (size_t)e->nscan_blocks))) return rc;
+    MFT_HIP(hipMemset(e->count, 0, sizeof(int) * (size_t)e->nkeys));
+    e->binned = false;
+    return MFT_OK;
+}
+
+int tild_bin_count(TildEngine* e)
+{
+    const mft_tild_desc& d = e->d;
+    k_bin_count<<<nblk(e->N), kBlock, 0, e->stream>>>(e->x[e->cur], e->type, e->N, e->count, e->cell, e->rank, e->dx[0],
+                                                       e->dx[1], e->dx[2], d.pmeorder, d.nx, d.ny, d.nz, e->tg);
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
+int tild_scan(TildEngine* e)
+{
+    k_scan_reduce<<<e->nscan_blocks, kScanThreads, 0, e->stream>>>(e->count, e->nkeys, e->partial);
+    k_scan_partials<<<1, 1024, 0, e->stream>>>(e->partial, e->nscan_blocks);
+    k_scan_apply<<<e->nscan_blocks, kScanThreads, 0, e->stream>>>(e->count, e->nkeys, e->partial, e->start, e->N);
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
+int tild_scatter(TildEngine* e)
+{
+    k_bin_scatter<<<nblk(e->N), kBlock, 0, e->stream>>>(e->x[e->cur], e->charge, e->type, e->N, e->cell, e->rank,
+                                                         e->start, e->spos, e->smeta);
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
+int tild_density(TildEngine* e)
+{
+    const mft_tild_desc& d = e->d;
+    const TileGeom& tg = e->tg;
+    const int BM = 1 << tg.lbmax;
+    const double inv_gvol = 1.0 / (double)e->gvol;
+#define MFT_DENS(P_, B_)                                                                                  \
+    k_density_tile<P_, B_><<<tg.nbricks, kTileThreads, e->dens_lds, e->stream>>>(                         \
+        e->tiles, e->nchan, e->qchan, e->spos, e->smeta, e->start, tg, d.nx, d.ny, d.nz, e->dx[0], e->dx[1], \
+        e->dx[2], inv_gvol)
+#define MFT_DENS_B(P_)                                                                                    \
+    if (BM == 8) MFT_DENS(P_, 8);                                                                         \
+    else MFT_DENS(P_, 4);
+    switch (d.pmeorder) {
+    case 0: MFT_DENS_B(0); break;
+    case 1: MFT_DENS_B(1); break;
+    case 2: MFT_DENS_B(2); break;
+    case 3: MFT_DENS_B(3); break;
+    default: MFT_DENS_B(4); break;
+    }
+#undef MFT_DENS_B
+#undef MFT_DENS
+    MFT_LAUNCH_CHECK();
+    const long cstride = (long)tg.ex[0] * tg.ex[1] * tg.ex[2];
+    dim3 rgrid((d.nx + 63) / 64, d.ny, d.nz);
+    k_tile_reduce<<<rgrid, 64, 0, e->stream>>>(e->rho, e->tiles, e->tile_tab, d.nx, d.ny, d.nz, e->nchan, cstride);
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
+// gather variants: 0 = direct loads in key order, 1 = LDS tiles
+int tild_gather(TildEngine* e, int mode)
+{
+    const mft_tild_desc& d = e->d;
+    if (mode == 0) {
+#define MFT_G0(P_)                                                                                        \
+    k_gather<P_><<<nblk(e->N), kBlock, 0, e->stream>>>(e->fgrid, e->F4, e->force_slot, e->qslot, e->spos, e->smeta, \
+                                                        e->N, d.nx, d.ny, d.nz, e->dx[0], e->dx[1], e->dx[2])
+        switch (d.pmeorder) {
+        case 0: MFT_G0(0); break;
+        case 1: MFT_G0(1); break;
+        case 2: MFT_G0(2); break;
+        case 3: MFT_G0(3); break;
+        default: MFT_G0(4); break;
+        }
+#undef MFT_G0
+    } else {
+        const TileGeom& tg = e->tg;
+        const int BM = 1 << tg.lbmax;
+        GatherMap gm;
+        for (int sp = 0; sp < kMaxSpecies; ++sp) gm.slot[sp] = sp < d.nspecies ? e->force_slot_h[sp] : -1;
+        dim3 grid(tg.nbricks, d.nspecies);
+        const float4* Fq = e->F4 + (size_t)(e->qslot >= 0 ? e->qslot : 0) * e->M;
+#define MFT_G1(P_, B_)                                                                                    \
+    do {                                                                                                  \
+        k_gather_species<P_, B_><<<grid, kTileThreads, 0, e->stream>>>(e->fgrid, e->F4, gm, e->spos, e->smeta, \
+                                                                         e->start, tg, d.nx, d.ny, d.nz, e->dx[0], \
+                                                                         e->dx[1], e->dx[2]);             \
+        if (e->qslot >= 0)                                                                                \
+            k_gather_charge<P_, B_><<<tg.nbricks, kTileThreads, 0, e->stream>>>(                          \
+                e->fgrid, Fq, e->spos, e->smeta, e->start, tg, d.nx, d.ny, d.nz, e->dx[0], e->dx[1], e->dx[2]); \
+    } while (0)
+#define MFT_G1_B(P_)                                                                                      \
+    if (BM == 8) MFT_G1(P_, 8);                                                                           \
+    else MFT_G1(P_, 4);
+        switch (d.pmeorder) {
+        case 0: MFT_G1_B(0); break;
+        case 1: MFT_G1_B(1); break;
+        case 2: MFT_G1_B(2); break;
+        case 3: MFT_G1_B(3); break;
+        default: MFT_G1_B(4); break;
+        }
+#undef MFT_G1_B
+#undef MFT_G1
+    }
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
+int tild_integrate(TildEngine* e, int nxt, bool want_f)
+{
+    const mft_tild_desc& d = e->d;
+    k_bonds_integrate<<<nblk(e->N), kBlock, 0, e->stream>>>(
+        e->x[e->cur], e->x[nxt], e->noise_old, e->fgrid, want_f ? e->ftot : nullptr, e->type, e->nBonds, e->bondedTo,
+        e->bondType, d.maxbonds, e->bp, e->gp, d.integrator, d.delt, d.lx, d.ly, d.lz, e->N, d.seed, (uint32_t)e->step,
+        e->count, e->cell, e->rank, e->dx[0], e->dx[1], e->dx[2], d.pmeorder, d.nx, d.ny, d.nz, e->tg);
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
+int tild_bond_energy(TildEngine* e)
+{
+    const mft_tild_desc& d = e->d;
+    unsigned nb = nblk(e->N);
+    k_bond_energy<<<nb, kBlock, 0, e->stream>>>(e->bpart, e->nBonds, e->bondedTo, e->bondType, e->bp, e->x[e->cur],
+                                                 d.lx, d.ly, d.lz, e->N, d.maxbonds);
+    k_sum_partials<<<1, 1024, 0, e->stream>>>(e->eout + kMaxPots, e->bpart, nb);
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
+} // namespace mft
+
+using namespace mft;
+
+extern "C" {
+
+int mft_tild_grid_weights(float* gridW, int* gridInds, const float* x, int ns, const int Nx[3], const float dx[3],
+                          int pmeorder, void* stream)
+{
+    MFT_REQUIRE(gridW && gridInds && x && Nx && dx && ns >= 0, MFT_ERR_INVALID, "null argument");
+    MFT_REQUIRE(pmeorder >= 0 && pmeorder <= 4, MFT_ERR_INVALID, "pmeorder must be 0..4");
+    if (ns == 0) return MFT_OK;
+    k_grid_weights<<<nblk(ns), kBlock, 0, (hipStream_t)stream>>>(gridW, gridInds, x, ns, Nx[0], Nx[1], Nx[2], dx[0],
+                                                                   dx[1], dx[2], pmeorder);
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
+int mft_tild_fill_density(float* rho, float* rhoq, const float* q, const int* sites, int nsites, int ns,
+                          const int* gridInds, const float* gridW, int G, int M, void* stream)
+{
+    MFT_REQUIRE(rho && sites && gridInds && gridW && nsites >= 0 && G > 0 && M > 0, MFT_ERR_INVALID,
+                "null argument");
+    MFT_REQUIRE((rhoq == nullptr) == (q == nullptr), MFT_ERR_INVALID, "rhoq and q go together");
+    hipStream_t st = (hipStream_t)stream;
+    unsigned long long* acc = nullptr;
+    int rc;
+    if ((rc = dalloc(&acc, (size_t)M * (rhoq ? 2 : 1)))) return rc;
+    MFT_HIP(hipMemsetAsync(acc, 0, sizeof(unsigned long long) * (size_t)M * (rhoq ? 2 : 1), st));
+    if (nsites > 0)
+        k_fill_density_tables<<<nblk(nsites), kBlock, 0, st>>>(acc, rhoq ? acc + M : nullptr, q, sites, nsites,
+                                                                gridInds, gridW, G, M, ns);
+    k_fixed_to_float_add<<<nblk(M), kBlock, 0, st>>>(rho, acc, M);
+    if (rhoq) k_fixed_to_float_add<<<nblk(M), kBlock, 0, st>>>(rhoq, acc + M, M);
+    MFT_LAUNCH_CHECK();
+    MFT_HIP(hipStreamSynchronize(st));
+    MFT_HIP(hipFree(acc));
+    return MFT_OK;
+}
+
+int mft_tild_map_forces(float* f, const float* gridF, const float* q, const int* sites, int nsites, int ns,
+                        const int* gridInds, const float* gridW, int G, float gvol, int M, void* stream)
+{
+    MFT_REQUIRE(f && gridF && sites && gridInds && gridW && nsites >= 0 && G > 0, MFT_ERR_INVALID, "null argument");
+    if (nsites == 0) return MFT_OK;
+    k_map_forces_tables<<<nblk(nsites), kBlock, 0, (hipStream_t)stream>>>(f, gridF, q, sites, nsites, gridInds, gridW,
+                                                                            G, gvol, M, ns);
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
+int mft_tild_bonds(float* f, const int* nBonds, const int* bondedTo, const int* bondType, const float* bondReq,
+                   const float* bondK, const int* bondStyle, int nbondtypes, const float* x, const float L[3], int ns,
+                   int maxbonds, void* stream)
+{
+    MFT_REQUIRE(f && nBonds && x && L && ns >= 0, MFT_ERR_INVALID, "null argument");
+    MFT_REQUIRE(nbondtypes >= 0 && nbondtypes <= kMaxBondTypes, MFT_ERR_INVALID, "at most %d bond types",
+                kMaxBondTypes);
+    MFT_REQUIRE(nbondtypes == 0 || (bondReq && bondK && bondStyle), MFT_ERR_INVALID, "null bond tables");
+    BondParams bp;
+    std::memset(&bp, 0, sizeof(bp));
+    bp.ntypes = nbondtypes;
+    for (int i = 0; i < nbondtypes; ++i) {
+        bp.k[i] = bondK[i];
+        bp.r0[i] = bondReq[i];
+        bp.style[i] = bondStyle[i];
+    }
+    if (ns == 0) return MFT_OK;
+    k_bonds<<<nblk(ns), kBlock, 0, (hipStream_t)stream>>>(f, nBonds, bondedTo, bondType, bp, x, L[0], L[1], L[2], ns,
+                                                           maxbonds);
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
+} // extern "C"
