@@ -159,6 +159,11 @@ int mft_tild_get_grid_force(mft_tild* eng, int s, float* F);
 /* nsteps of PS_Box::doTimeStep (PS_Box.cu:73-207) without I/O; step counters
  * advance the RNG and potential ramps. Asynchronous on the engine stream. */
 int mft_tild_step(mft_tild* eng, int nsteps);
+/* Declares a run of max_steps steps (PS_Box::NVT's maxSteps, PS_Box.cu:210-231):
+ * potential ramps (PS_Potential::update_prefactor, ps_potential.cu:227-248)
+ * then follow the step index within the run across any number of
+ * mft_tild_step calls.  Without it each mft_tild_step call is a run. */
+int mft_tild_set_run(mft_tild* eng, int64_t max_steps);
 int mft_tild_synchronize(mft_tild* eng);
 /* Energies of computeThermoProps (PS_Box.cu:282-317): per-potential
  * energies [npotentials] (charges include the 1/2, ps_potentialCharges.cu:115)

@@ -164,6 +164,8 @@ static int tild_setup(TildEngine* e)
 // the charge density is sum_s q_s rho_s, so electrostatics folds into the
 // per-species k-space kernel and no separate charge channel, transform or
 // field is needed; otherwise the charge density is a channel of its own.
+static void build_groups(TildEngine* e);
+
 static int configure_channels(TildEngine* e, const int* type, const float* charge)
 {
     const mft_tild_desc& d = e->d;
@@ -209,6 +211,7 @@ static int configure_channels(TildEngine* e, const int* type, const float* charg
     e->qslot = (e->charge_pot >= 0 && !fold) ? e->nforce : -1;
     e->nout4 = e->nforce + (e->qslot >= 0 ? 1 : 0);
     e->nout = 3 * e->nout4;
+    e->npk = (e->nout + 1) / 2;
     kp.nchan = e->nchan;
     kp.qchan = e->qchan;
     kp.nforce = e->nforce;
@@ -222,10 +225,11 @@ static int configure_channels(TildEngine* e, const int* type, const float* charg
     e->F4 = nullptr;
     if ((rc = dalloc(&e->rho, (size_t)e->nchan * M))) return rc;
     if ((rc = dalloc(&e->S, (size_t)e->nchan * M))) return rc;
-    if ((rc = dalloc(&e->Y, (size_t)(e->nout > 0 ? e->nout : 1) * M))) return rc;
+    if ((rc = dalloc(&e->Y, (size_t)(e->npk > 0 ? e->npk : 1) * M))) return rc;
     if ((rc = dalloc(&e->F4, (size_t)(e->nout4 > 0 ? e->nout4 : 1) * M))) return rc;
     MFT_HIP(hipMemcpy(e->force_slot, e->force_slot_h.data(), sizeof(int) * d.nspecies, hipMemcpyHostToDevice));
     if ((rc = tild_tiles_configure(e))) return rc;
+    build_groups(e);
     e->configured = 1;
     return MFT_OK;
 }
@@ -312,17 +316,67 @@ static int grid_forces(TildEngine* e)
     }
     {
         TimedScope ts(e, MFT_TK_FFT_Y_INV);
-        if ((rc = tild_axis(+1, 0, 0, e->Y, e->Y, gy, e->nout, M, 1.0f, e->stream))) return rc;
+        if ((rc = tild_axis(+1, 0, 0, e->Y, e->Y, gy, e->npk, M, 1.0f, e->stream))) return rc;
     }
     {
         TimedScope ts(e, MFT_TK_FFT_X_INV);
-        if ((rc = tild_x_inv_vec3(e->Y, e->F4, d.nx, d.ny, d.nz, e->nout4, e->stream))) return rc;
+        if ((rc = tild_x_inv_packed(e->Y, e->F4, d.nx, d.ny, d.nz, e->nout4, e->stream))) return rc;
     }
     {
         TimedScope ts(e, MFT_TK_GATHER);
         if ((rc = tild_gather(e, e->gather_mode))) return rc;
     }
     return MFT_OK;
+}
+
+// Groups of potentials with one k-space shape and their coupling matrices
+// (from the current, possibly ramped, amplitudes).
+static void build_groups(TildEngine* e)
+{
+    KParams& kp = e->kp;
+    kp.ngroups = 0;
+    std::memset(kp.grp, 0, sizeof(kp.grp));
+    auto find = [&](int kind, float sig2, const float* table) -> int {
+        for (int g = 0; g < kp.ngroups; ++g)
+            if (kp.grp[g].kind == kind && kp.grp[g].sig2 == sig2 && kp.grp[g].table == table) return g;
+        if (kp.ngroups == kMaxGroups) return -1;
+        KGroup& G = kp.grp[kp.ngroups];
+        G.kind = kind;
+        G.sig2 = sig2;
+        G.table = table;
+        return kp.ngroups++;
+    };
+    for (int i = 0; i < kp.npots; ++i) {
+        const KPot& P = kp.pot[i];
+        int g;
+        if (P.kind == MFT_POT_CHARGES) {
+            if (!kp.fold) continue; // separate charge field
+            g = find(MFT_POT_CHARGES, P.sig2, nullptr);
+        } else if (P.kind == MFT_POT_GAUSSIAN) {
+            g = find(MFT_POT_GAUSSIAN, P.sig2, nullptr);
+        } else {
+            g = find(P.kind, 0.f, P.table);
+        }
+        if (g < 0) {
+            // more distinct shapes than kMaxGroups: the kernel evaluates each
+            // potential on its own
+            kp.ngroups = -1;
+            return;
+        }
+        for (int f = 0; f < kp.nforce; ++f) {
+            const int sp = kp.force_species[f];
+            for (int c = 0; c < kp.nchan; ++c) {
+                if (c == kp.qchan) continue;
+                if (P.kind == MFT_POT_CHARGES) {
+                    kp.grp[g].coef[f][c] += 12.566370614359172f * P.A * kp.qspec[sp] * kp.qspec[c];
+                } else {
+                    const float inI = (float)((P.maskI >> sp) & 1u), inJ = (float)((P.maskJ >> sp) & 1u);
+                    kp.grp[g].coef[f][c] +=
+                        P.A * (inI * (float)((P.maskJ >> c) & 1u) + inJ * (float)((P.maskI >> c) & 1u));
+                }
+            }
+        }
+    }
 }
 
 static void update_ramps(TildEngine* e, int64_t i, int64_t nsteps)
@@ -335,9 +389,12 @@ static void update_ramps(TildEngine* e, int64_t i, int64_t nsteps)
         double A = (double)P.A + ((double)P.A_final - (double)P.A) / (double)nsteps * (double)(i + 1);
         e->kp.pot[p].A = (float)A;
     }
+    build_groups(e);
 }
 
-static int do_steps(TildEngine* e, int nsteps, bool want_f)
+// advance = false: a force/energy evaluation at the current state (the step
+// counter and the potential ramps stay where they are)
+static int do_steps(TildEngine* e, int nsteps, bool want_f, bool advance = true)
 {
     const mft_tild_desc& d = e->d;
     int rc;
@@ -352,7 +409,9 @@ static int do_steps(TildEngine* e, int nsteps, bool want_f)
             e->cur = nxt;
             e->binned = true;
         }
-        update_ramps(e, i, nsteps);
+        if (!advance) continue;
+        if (e->run_max > 0) update_ramps(e, e->run_pos++, e->run_max);
+        else update_ramps(e, i, nsteps);
         e->step++;
     }
     return MFT_OK;
@@ -486,7 +545,7 @@ int mft_tild_compute_forces(mft_tild* e, float* f)
     int saved = e->d.integrator;
     e->d.integrator = MFT_INT_NONE;
     int64_t st = e->step;
-    rc = do_steps(e, 1, true);
+    rc = do_steps(e, 1, true, false);
     e->d.integrator = saved;
     e->step = st;
     if (rc) return rc;
@@ -545,6 +604,14 @@ int mft_tild_step(mft_tild* e, int nsteps)
     return do_steps(e, nsteps, false);
 }
 
+int mft_tild_set_run(mft_tild* e, int64_t max_steps)
+{
+    MFT_REQUIRE(e && max_steps >= 0, MFT_ERR_INVALID, "bad argument");
+    e->run_max = max_steps;
+    e->run_pos = 0;
+    return MFT_OK;
+}
+
 int mft_tild_synchronize(mft_tild* e)
 {
     MFT_REQUIRE(e, MFT_ERR_INVALID, "null argument");
@@ -563,7 +630,7 @@ int mft_tild_energies(mft_tild* e, double* pot_energy, double* bond_energy)
         e->d.integrator = MFT_INT_NONE;
         e->kp.energy = 1;
         int64_t st = e->step;
-        rc = do_steps(e, 1, false);
+        rc = do_steps(e, 1, false, false);
         e->kp.energy = 0;
         e->d.integrator = saved;
         e->step = st;
@@ -608,7 +675,7 @@ int mft_tild_timing(mft_tild* e, int enable, double* avg_ms, int64_t* launches)
 int mft_tild_kernel_bytes(mft_tild* e, double* bytes)
 {
     MFT_REQUIRE(e && bytes, MFT_ERR_INVALID, "null argument");
-    const double N = e->N, M = e->M, C = e->nchan, O = e->nout, O4 = e->nout4;
+    const double N = e->N, M = e->M, C = e->nchan, O = e->npk, O4 = e->nout4;
     const bool q = e->d.has_charges;
     for (int k = 0; k < MFT_TK_COUNT; ++k) bytes[k] = 0;
     // minimal HBM traffic of each kernel's job (reads + writes once)
@@ -635,7 +702,7 @@ int mft_tild_autotune(mft_tild* e, int reps, int* choice)
     int saved = e->d.integrator;
     int64_t st = e->step;
     e->d.integrator = MFT_INT_NONE;
-    int rc = do_steps(e, 1, false);
+    int rc = do_steps(e, 1, false, false);
     e->d.integrator = saved;
     e->step = st;
     if (rc) return rc;

@@ -61,7 +61,9 @@ __global__ void __launch_bounds__(256) k_conv_z(const float2* __restrict__ S, fl
         k2[s] = __fadd_rn(__fadd_rn(__fmul_rn(kx, kx), __fmul_rn(ky, ky)), __fmul_rn(kz[s], kz[s]));
     }
 
-    int out = 0;
+    int out = 0;        // packed output field
+    float2 pend[E];     // even component waiting for its partner
+    bool pending = false;
     for (int fi = 0; fi < kp.nforce + (kp.qchan >= 0 ? 1 : 0); ++fi) {
         const bool charge = fi == kp.nforce;
         float2 Z[E];
@@ -78,6 +80,27 @@ __global__ void __launch_bounds__(256) k_conv_z(const float2* __restrict__ S, fl
                 for (int c = 0; c < NCH; ++c)
                     if (c == kp.qchan) sq = spec[c][s];
                 Z[s] = cscale(sq, u);
+            }
+        } else if (kp.ngroups >= 0) {
+            for (int g = 0; g < kp.ngroups; ++g) {
+                const KGroup& G = kp.grp[g];
+#pragma unroll
+                for (int s = 0; s < E; ++s) {
+                    float2 src = make_float2(0.f, 0.f);
+#pragma unroll
+                    for (int c = 0; c < NCH; ++c) {
+                        if (c >= kp.nchan) continue;
+                        const float m = G.coef[fi][c];
+                        src.x += m * spec[c][s].x;
+                        src.y += m * spec[c][s].y;
+                    }
+                    float u;
+                    if (G.kind == MFT_POT_GAUSSIAN) u = expf(-k2[s] * G.sig2 * 0.5f);
+                    else if (G.kind == MFT_POT_CHARGES) u = k2[s] > 0.f ? expf(-k2[s] * G.sig2) / k2[s] : 0.f;
+                    else u = G.table[ix + (long)nx * (iy + (long)ny * (t + T * s))];
+                    Z[s].x += u * src.x;
+                    Z[s].y += u * src.y;
+                }
             }
         } else {
             const int sp = kp.force_species[fi];
@@ -124,19 +147,39 @@ __global__ void __launch_bounds__(256) k_conv_z(const float2* __restrict__ S, fl
             }
         }
         for (int j = 0; j < 3; ++j) {
+            // V = -i k_j Z, with the Nyquist plane of axis j dropped: there
+            // V is anti-Hermitian and adds nothing to the real part the force
+            // field keeps, so V is the spectrum of a real field and two such
+            // components share one complex inverse transform (re, im)
             float2 v[E];
 #pragma unroll
             for (int s = 0; s < E; ++s) {
-                float kj = j == 0 ? kx : (j == 1 ? ky : kz[s]);
-                // -i k_j Z
+                const bool nyq = j == 0 ? 2 * ix == nx : (j == 1 ? 2 * iy == ny : 2 * (t + T * s) == N);
+                const float kj = nyq ? 0.f : (j == 0 ? kx : (j == 1 ? ky : kz[s]));
                 v[s] = make_float2(kj * Z[s].y, -kj * Z[s].x);
             }
+            if (!pending) {
+#pragma unroll
+                for (int s = 0; s < E; ++s) pend[s] = v[s];
+                pending = true;
+                continue;
+            }
+#pragma unroll
+            for (int s = 0; s < E; ++s) v[s] = make_float2(pend[s].x - v[s].y, pend[s].y + v[s].x);
+            pending = false;
             line_fft<N, +1>(v, lds, tw, t, map);
             if (valid) {
 #pragma unroll
                 for (int s = 0; s < E; ++s) Y[out * M + line + (long)(t + T * s) * st] = v[s];
             }
             ++out;
+        }
+    }
+    if (pending) {
+        line_fft<N, +1>(pend, lds, tw, t, map);
+        if (valid) {
+#pragma unroll
+            for (int s = 0; s < E; ++s) Y[out * M + line + (long)(t + T * s) * st] = pend[s];
         }
     }
 

@@ -50,11 +50,15 @@ __global__ void __launch_bounds__(256) k_fft_axis(const void* __restrict__ in, v
     }
 }
 
-// Last inverse pass along x for the three components of one force field,
-// keeping real parts and writing the field as float4 (x, y, z, 0).
+// Last inverse pass along x.  The force components are packed two per
+// complex field (component r of field f = 3f + j is the real part of packed
+// field r/2 for even r, the imaginary part for odd r); one workgroup row
+// transforms the three packed lines of fields 2g and 2g+1 and writes them as
+// float4 (x, y, z, 0).
 template <int N>
-__global__ void __launch_bounds__(256) k_fft_x_inv_vec3(const float2* __restrict__ Y, float4* __restrict__ F4, int ny,
-                                                        int nz, const float2* __restrict__ twg)
+__global__ void __launch_bounds__(256) k_fft_x_inv_packed(const float2* __restrict__ Y, float4* __restrict__ F4,
+                                                          int ny, int nz, int nfield, int npk,
+                                                          const float2* __restrict__ twg)
 {
     using P = FftPlan<N>;
     constexpr int T = P::T, E = P::E, B = 256 / T;
@@ -67,24 +71,29 @@ __global__ void __launch_bounds__(256) k_fft_x_inv_vec3(const float2* __restrict
     const bool valid = line < ny * nz;
     const long M = (long)N * ny * nz;
     const long base = (long)line * N;
-    const int f = blockIdx.y;
-    float re[3][E];
+    const int g = blockIdx.y;
+    float2 pk[3][E];
     __syncthreads();
     LdsMap map{lline, B, false, N};
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        const float2* src = Y + (3 * f + j) * M + base;
-        float2 v[E];
+        const int q = 3 * g + j;
 #pragma unroll
-        for (int s = 0; s < E; ++s) v[s] = valid ? src[t + T * s] : make_float2(0.f, 0.f);
-        line_fft<N, +1>(v, lds, tw, t, map);
-#pragma unroll
-        for (int s = 0; s < E; ++s) re[j][s] = v[s].x;
+        for (int s = 0; s < E; ++s)
+            pk[j][s] = (valid && q < npk) ? Y[q * M + base + t + T * s] : make_float2(0.f, 0.f);
+        line_fft<N, +1>(pk[j], lds, tw, t, map);
     }
     if (!valid) return;
-    float4* dst = F4 + f * M + base;
+    // fields 2g (components in pk0.re, pk0.im, pk1.re) and 2g+1 (pk1.im,
+    // pk2.re, pk2.im)
+    float4* d0 = F4 + (2 * g) * M + base;
 #pragma unroll
-    for (int s = 0; s < E; ++s) dst[t + T * s] = make_float4(re[0][s], re[1][s], re[2][s], 0.f);
+    for (int s = 0; s < E; ++s) d0[t + T * s] = make_float4(pk[0][s].x, pk[0][s].y, pk[1][s].x, 0.f);
+    if (2 * g + 1 < nfield) {
+        float4* d1 = F4 + (2 * g + 1) * M + base;
+#pragma unroll
+        for (int s = 0; s < E; ++s) d1[t + T * s] = make_float4(pk[1][s].y, pk[2][s].x, pk[2][s].y, 0.f);
+    }
 }
 
 template <int N, int SIGN, bool RI, bool RO>
@@ -123,14 +132,15 @@ int tild_axis(int sign, int real_in, int real_out, const void* in, void* out, Ax
     return MFT_ERR_UNSUPPORTED;
 }
 
-int tild_x_inv_vec3(const float2* Y, float4* F4, int nx, int ny, int nz, int nf, hipStream_t st)
+int tild_x_inv_packed(const float2* Y, float4* F4, int nx, int ny, int nz, int nfield, hipStream_t st)
 {
+    const int npk = (3 * nfield + 1) / 2;
     switch (nx) {
 #define MFT_CASE(n_)                                                                                      \
     case n_: {                                                                                            \
         constexpr int B = 256 / FftPlan<n_>::T;                                                           \
-        dim3 grid((ny * nz + B - 1) / B, nf);                                                             \
-        k_fft_x_inv_vec3<n_><<<grid, 256, 0, st>>>(Y, F4, ny, nz, fft_twiddles_device(n_));               \
+        dim3 grid((ny * nz + B - 1) / B, (nfield + 1) / 2);                                               \
+        k_fft_x_inv_packed<n_><<<grid, 256, 0, st>>>(Y, F4, ny, nz, nfield, npk, fft_twiddles_device(n_)); \
         break;                                                                                            \
     }
         MFT_FOR_EACH_FFT_SIZE(MFT_CASE)

@@ -44,8 +44,22 @@ struct KPot {
     const float* table; // u(k)/A for erf kinds, else null
 };
 
+// Potentials sharing one k-space shape: Z_f(k) = sum_g shape_g(k) sum_c
+// coef_g[f][c] rho_c(k) for force slot f and density channel c.  Shapes:
+// Gaussian exp(-k^2 sig2 / 2), folded charges exp(-k^2 sig2) / k^2, erf kinds
+// a table of u(k)/A.
+constexpr int kMaxGroups = 8;
+struct KGroup {
+    int kind;
+    float sig2;
+    const float* table;
+    float coef[kMaxChan][kMaxChan];
+};
+
 struct KParams {
     int npots;
+    int ngroups;        // -1: more shapes than kMaxGroups, evaluate per potential
+    KGroup grp[kMaxGroups];
     KPot pot[kMaxPots];
     int charge_pot;     // index of the charges potential or -1
     int nchan;          // species channels (+1 for charge density)
@@ -56,6 +70,8 @@ struct KParams {
     float qspec[kMaxSpecies];
     int energy;         // accumulate potential energies
 };
+
+static_assert(sizeof(KParams) <= 3800, "KParams is passed as a kernel argument");
 
 struct BondParams {
     int ntypes;
@@ -113,7 +129,7 @@ struct TildEngine {
     mft_tild_desc d;
     std::vector<mft_tild_potential> pots;
     std::vector<float> mass, mob;
-    int N = 0, M = 0, nchan = 0, qchan = -1, nforce = 0, nout = 0, nout4 = 0, qslot = -1, charge_pot = -1;
+    int N = 0, M = 0, nchan = 0, qchan = -1, nforce = 0, nout = 0, nout4 = 0, npk = 0, qslot = -1, charge_pot = -1;
     int configured = 0;
     float dx[3] = {0, 0, 0};
     float gvol = 0;
@@ -137,7 +153,7 @@ struct TildEngine {
     float* tiles = nullptr; // [nchan][halo-expanded grid] density tiles
     int* tile_tab = nullptr; // per-axis tile offsets contributing to each grid coordinate
     float2* S = nullptr;    // [nchan][M]
-    float2* Y = nullptr;    // [nout][M] force spectra (3 components per field)
+    float2* Y = nullptr;    // [npk][M] force spectra, two real components per complex field
     float4* F4 = nullptr;   // [nout4][M] force fields (x, y, z, 0)
     int* force_slot = nullptr; // [nspecies] -> force field slot or -1
     std::vector<int> force_slot_h;
@@ -151,6 +167,7 @@ struct TildEngine {
     size_t dens_lds = 0;
     int gather_mode = 1;    // 0 direct, 1 LDS windows (mft_tild_autotune picks)
     int64_t step = 0;
+    int64_t run_max = 0, run_pos = 0; // declared run (mft_tild_set_run)
     int have_state = 0;
     // timing: event pairs recorded around kernel launches on the engine
     // stream without host synchronisation, resolved when timing is read
@@ -184,7 +201,7 @@ static int dalloc(T** p, size_t n)
 // ---- grid passes (tild_grid.hip, tild_conv.hip) ----
 int tild_axis(int sign, int real_in, int real_out, const void* in, void* out, AxisGeom g, int nf, long fs,
               float scale, hipStream_t st);
-int tild_x_inv_vec3(const float2* Y, float4* F4, int nx, int ny, int nz, int nf, hipStream_t st);
+int tild_x_inv_packed(const float2* Y, float4* F4, int nx, int ny, int nz, int nfield, hipStream_t st);
 int tild_conv_blocks(int nx, int ny, int nz);
 int tild_conv(TildEngine* e);
 

@@ -1,0 +1,429 @@
+// Field-theoretic (CL-FTS / SCFT) box of the drop-in front end: the
+// reference's FTS_Box keywords, potential and molecule classes and log output
+// over the mft_fts engine.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <iostream>
+
+#include "matilda.h"
+
+// FTS_Species (fts_species.cu:10-30): "species A"
+FTS_Species::FTS_Species(std::istringstream& iss, FTS_Box* box)
+{
+    (void)box;
+    if (!(iss >> fts_species)) die("Missing label for species");
+}
+
+// --------------------------------------------------------- potentials ---
+FTS_Potential::FTS_Potential(std::istringstream& iss, FTS_Box* box) : input_command(iss.str()), mybox(box)
+{
+    wpl.assign(box->gridSize(), {0.0, 0.0});
+}
+
+// FTS_Potential::initializeField (fts_potential.cu:62-150): value, random, sine
+void FTS_Potential::initializeField(std::istringstream& iss, std::vector<std::complex<double>>& w)
+{
+    std::string s1;
+    iss >> s1;
+    const long M = mybox->gridSize();
+    if (s1 == "value") {
+        double rv, iv;
+        iss >> rv >> iv;
+        for (auto& v : w) v = {rv, iv};
+    } else if (s1 == "random") {
+        double ra, ia;
+        iss >> ra >> ia;
+        for (long i = 0; i < M; ++i) {
+            double r = ra * mybox->rng.uniform();
+            double im = ia * mybox->rng.uniform();
+            w[i] = {r, im};
+        }
+    } else {
+        die(s1 + " is not a valid initialize option in this build (value, random)");
+    }
+}
+
+void FTS_Potential::parse_options(std::istringstream& iss, std::vector<std::complex<double>>* init_target, bool flory)
+{
+    std::string s1;
+    while (iss >> s1) {
+        if (s1 == "initialize") {
+            std::vector<std::complex<double>> scratch(mybox->gridSize());
+            initializeField(iss, init_target ? *init_target : scratch);
+            if (flory) {
+                // fts_potential_flory.cu:83-88: initialize also sets w+ = -1.5i
+                for (auto& v : wpl) v = {0.0, -1.5};
+            }
+        } else if (s1 == "updateScheme") {
+            iss >> updateScheme;
+            if (updateScheme != "EM") die("updateScheme " + updateScheme + " is not part of this build (EM)");
+        } else if (s1 == "modify") {
+            iss >> s1;
+            if (s1 != "zeromean" && s1 != "zeroMean") die("Invalid modify option on " + potentialStyle + " potential");
+            die("modify zeromean is not part of this build");
+        } else {
+            die("Invalid option " + s1 + " on " + potentialStyle + " potential");
+        }
+    }
+}
+
+void FTS_Potential::writeFields(int potInd) { (void)potInd; }
+
+// PotentialHelfand (fts_potential_helfand.cu:15-70): kappaN delt [options];
+// its initialize never reaches the device (the field stays zero)
+PotentialHelfand::PotentialHelfand(std::istringstream& iss, FTS_Box* box) : FTS_Potential(iss, box)
+{
+    potentialStyle = "Helfand";
+    iss >> kappaN >> delt;
+    parse_options(iss, nullptr, false);
+}
+mft_fts_potential PotentialHelfand::descriptor() const
+{
+    mft_fts_potential d{};
+    d.kind = MFT_FTS_HELFAND;
+    d.kappaN = kappaN;
+    d.delt_pl = delt;
+    return d;
+}
+
+// PotentialFlory (fts_potential_flory.cu:21-120): I J chiN deltPlus deltMinus
+PotentialFlory::PotentialFlory(std::istringstream& iss, FTS_Box* box) : FTS_Potential(iss, box)
+{
+    potentialStyle = "Flory";
+    std::string a, b;
+    iss >> a >> b;
+    actsOn = {a, b};
+    I = box->findSpeciesInteger(a);
+    J = box->findSpeciesInteger(b);
+    iss >> chiN >> deltPlus >> deltMinus;
+    wmi.assign(box->gridSize(), {0.0, 0.0});
+    has_wmi = true;
+    parse_options(iss, &wmi, true);
+}
+mft_fts_potential PotentialFlory::descriptor() const
+{
+    mft_fts_potential d{};
+    d.kind = MFT_FTS_FLORY;
+    d.I = I;
+    d.J = J;
+    d.chiN = chiN;
+    d.delt_pl = deltPlus;
+    d.delt_mi = deltMinus;
+    return d;
+}
+
+// PotentialIncompress (fts_potential_incompress.cu:15-60): delt [options]
+PotentialIncompress::PotentialIncompress(std::istringstream& iss, FTS_Box* box) : FTS_Potential(iss, box)
+{
+    potentialStyle = "Incompress";
+    iss >> delt;
+    parse_options(iss, &wpl, false);
+}
+mft_fts_potential PotentialIncompress::descriptor() const
+{
+    mft_fts_potential d{};
+    d.kind = MFT_FTS_INCOMPRESS;
+    d.delt_pl = delt;
+    return d;
+}
+
+FTS_Potential* FTS_PotentialFactory(std::istringstream& iss, FTS_Box* box)
+{
+    std::string s1;
+    iss >> s1;
+    if (s1 == "Helfand" || s1 == "helfand") return new PotentialHelfand(iss, box);
+    if (s1 == "incompress" || s1 == "Incompress" || s1 == "incompressible") return new PotentialIncompress(iss, box);
+    if (s1 == "Flory" || s1 == "flory") return new PotentialFlory(iss, box);
+    if (s1 == "Edwards" || s1 == "edwards" || s1 == "particle" || s1 == "Particle")
+        die(s1 + " is not part of this build's field-theory path");
+    die(s1 + " is not a valid FTS_Potential");
+}
+
+// ---------------------------------------------------------- molecules ---
+// FTS_Molec::FTS_Molec (fts_molecule.cu:13-46): type, then phi | nmolecs | activity
+FTS_Molec::FTS_Molec(std::istringstream& iss, FTS_Box* box) : input_command(iss.str()), mybox(box)
+{
+    iss >> molec_type;
+    std::string s1;
+    iss >> s1;
+    if (s1 == "phi") iss >> phi;
+    else if (s1 == "nmolecs") iss >> nmolecs;
+    else if (s1 == "activity") iss >> activity;
+    else die(s1 + " is not a valid way to define the number of particles/molecules");
+    if (phi <= 0.0) die("this build's linear molecules are canonical with 'phi' given");
+}
+
+void FTS_Molec::modifyMolecule(std::istringstream& iss)
+{
+    (void)iss;
+    die("molecule modifications are not part of this build");
+}
+
+// LinearMolec (fts_molecule_linear.cu:17-140): numBlocks, (N, species) per block
+LinearMolec::LinearMolec(std::istringstream& iss, FTS_Box* box) : FTS_Molec(iss, box)
+{
+    int numBlocks = 0;
+    iss >> numBlocks;
+    if (numBlocks < 1 || numBlocks > 16) die("linear molecule needs 1..16 blocks");
+    N.resize(numBlocks);
+    blockSpecies.resize(numBlocks);
+    for (int j = 0; j < numBlocks; ++j) {
+        iss >> N[j] >> blockSpecies[j];
+        Ntot += N[j];
+    }
+    std::string s1;
+    if (iss >> s1) {
+        if (s1 == "smear" || s1 == "Smear") die("smeared molecules are not part of this build");
+        die("Invalid option " + s1 + " on molecule linear");
+    }
+}
+
+mft_fts_molecule LinearMolec::descriptor() const
+{
+    mft_fts_molecule d{};
+    d.nblocks = (int)N.size();
+    for (size_t j = 0; j < N.size(); ++j) {
+        d.N[j] = N[j];
+        d.species[j] = mybox->findSpeciesInteger(blockSpecies[j]);
+    }
+    d.phi = phi;
+    return d;
+}
+
+FTS_Molec* FTS_MolecFactory(std::istringstream& iss, FTS_Box* box)
+{
+    std::streampos pos = iss.tellg();
+    std::string s1;
+    iss >> s1;
+    iss.clear();
+    iss.seekg(pos);
+    if (s1 == "linear") return new LinearMolec(iss, box);
+    if (s1 == "HParticle") die("HParticle molecules are not part of this build");
+    die(s1 + " is not a valid FTS_Molec");
+}
+
+// --------------------------------------------------------------- box ----
+FTS_Box::FTS_Box(std::istringstream& iss) : Box(iss)
+{
+    boxStyle = "fts";
+    iss >> ftsStyle;
+    if (ftsStyle != "scft" && ftsStyle != "cl") die("Invalid FTS simulation box style! Must be scft or cl.");
+}
+
+FTS_Box::~FTS_Box()
+{
+    for (auto* p : Potentials) delete p;
+    for (auto* m : Molecs) delete m;
+    if (sys) mft_fts_destroy(sys);
+}
+
+int FTS_Box::findSpeciesInteger(const std::string& s) const
+{
+    for (size_t i = 0; i < Species.size(); ++i)
+        if (Species[i].fts_species == s) return (int)i;
+    die(s + " species not found");
+}
+
+void FTS_Box::readInput(std::ifstream& inp)
+{
+    std::string line, firstWord;
+    bool ended = false;
+    while (!ended && std::getline(inp, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream iss(line);
+        while (iss >> firstWord) {
+            if (firstWord == "endBox") {
+                ended = true;
+                break;
+            }
+            if (firstWord == "boxLengths") read_lengths(iss);
+            else if (firstWord == "C") iss >> C;
+            else if (firstWord == "chemFieldFreq" || firstWord == "chemfieldfreq" || firstWord == "clSampleFreq" ||
+                     firstWord == "clsamplefreq" || firstWord == "densityFieldFreq" ||
+                     firstWord == "densityfieldfreq") {
+                int f;
+                iss >> f;
+            } else if (firstWord == "Dim") {
+                iss >> Dim;
+                if (Dim != 3) die("this build runs 3D boxes (Dim 3)");
+                readDimension = true;
+            } else if (firstWord == "grid") {
+                read_grid(iss);
+            } else if (firstWord == "logFreq" || firstWord == "logfreq") {
+                iss >> logFreq;
+            } else if (firstWord == "molecule") {
+                Molecs.push_back(FTS_MolecFactory(iss, this));
+            } else if (firstWord == "Nr") {
+                iss >> Nr;
+            } else if (firstWord == "Nx" || firstWord == "Ny" || firstWord == "Nz") {
+                int a = firstWord == "Nx" ? 0 : (firstWord == "Ny" ? 1 : 2);
+                iss >> Nx[a];
+                M *= Nx[a];
+            } else if (firstWord == "potential") {
+                Potentials.push_back(FTS_PotentialFactory(iss, this));
+            } else if (firstWord == "randSeed" || firstWord == "RAND_SEED") {
+                long long sd;
+                iss >> sd;
+                RANDSEED = (uint64_t)sd;
+                rng.reseed(RANDSEED);
+            } else if (firstWord == "rho0") {
+                iss >> rho0;
+            } else if (firstWord == "species") {
+                Species.push_back(FTS_Species(iss, this));
+            } else if (firstWord == "threads") {
+                int t;
+                iss >> t;
+            } else if (firstWord == "tolerance") {
+                iss >> tolMetric >> tolerance;
+                if (tolMetric != "Heff") die("tolerance metric " + tolMetric + " is not part of this build (Heff)");
+            } else {
+                die("Invalid keyword " + firstWord + " in FTS_Box::readInput()");
+            }
+        }
+    }
+    finishInitialization();
+}
+
+void FTS_Box::finishInitialization()
+{
+    if (Nr <= 0.0) die("Nr must be defined for FTS boxes");
+    if (C <= 0.0) die("C must be defined for FTS boxes");
+    if (M <= 1) die("grid must be defined");
+    if (Species.empty() || Molecs.empty() || Potentials.empty()) die("FTS box needs species, molecules and potentials");
+    if (Species.size() > 8 || Potentials.size() > 8 || Molecs.size() > 4)
+        die("at most 8 species, 8 potentials and 4 molecule types");
+    mft_fts_desc d{};
+    d.nx = Nx[0];
+    d.ny = Nx[1];
+    d.nz = Nx[2];
+    d.lx = L[0];
+    d.ly = L[1];
+    d.lz = L[2];
+    d.Nr = Nr;
+    d.C = C;
+    d.nspecies = (int)Species.size();
+    d.npotentials = (int)Potentials.size();
+    for (size_t i = 0; i < Potentials.size(); ++i) d.potentials[i] = Potentials[i]->descriptor();
+    d.nmolecules = (int)Molecs.size();
+    for (size_t i = 0; i < Molecs.size(); ++i) d.molecules[i] = Molecs[i]->descriptor();
+    d.cl = ftsStyle == "cl" ? 1 : 0;
+    d.seed = RANDSEED;
+    if (dry_run) {
+        std::printf("{\"box\": \"fts\", \"style\": \"%s\", \"grid\": [%d, %d, %d], \"L\": [%.9g, %.9g, %.9g], "
+                    "\"Nr\": %.9g, \"C\": %.9g, \"nspecies\": %d, \"seed\": %llu, \"potentials\": [",
+                    ftsStyle.c_str(), d.nx, d.ny, d.nz, d.lx, d.ly, d.lz, d.Nr, d.C, d.nspecies,
+                    (unsigned long long)d.seed);
+        for (int i = 0; i < d.npotentials; ++i) {
+            const auto& p = d.potentials[i];
+            std::printf("%s{\"kind\": %d, \"I\": %d, \"J\": %d, \"chiN\": %.9g, \"kappaN\": %.9g, \"delt_pl\": %.9g, "
+                        "\"delt_mi\": %.9g, \"wpl0\": [%.9g, %.9g]}",
+                        i ? ", " : "", p.kind, p.I, p.J, p.chiN, p.kappaN, p.delt_pl, p.delt_mi,
+                        Potentials[i]->wpl[0].real(), Potentials[i]->wpl[0].imag());
+        }
+        std::printf("], \"molecules\": [");
+        for (int m = 0; m < d.nmolecules; ++m) {
+            std::printf("%s{\"phi\": %.9g, \"blocks\": [", m ? ", " : "", d.molecules[m].phi);
+            for (int j = 0; j < d.molecules[m].nblocks; ++j)
+                std::printf("%s[%d, %d]", j ? ", " : "", d.molecules[m].N[j], d.molecules[m].species[j]);
+            std::printf("]}");
+        }
+        std::printf("]}\n");
+        std::fflush(stdout);
+        initialized = true;
+        return;
+    }
+    check_mft(mft_fts_create(&sys, &d), "mft_fts_create");
+    std::vector<float> a(2 * (size_t)M), b(2 * (size_t)M);
+    for (size_t p = 0; p < Potentials.size(); ++p) {
+        // Helfand's initialize never reaches the device (fts_potential_helfand.cu)
+        const bool helfand = Potentials[p]->potentialStyle == "Helfand";
+        for (long i = 0; i < M; ++i) {
+            a[2 * i] = helfand ? 0.f : (float)Potentials[p]->wpl[i].real();
+            a[2 * i + 1] = helfand ? 0.f : (float)Potentials[p]->wpl[i].imag();
+            if (Potentials[p]->has_wmi) {
+                b[2 * i] = (float)Potentials[p]->wmi[i].real();
+                b[2 * i + 1] = (float)Potentials[p]->wmi[i].imag();
+            }
+        }
+        check_mft(mft_fts_set_fields(sys, (int)p, a.data(), Potentials[p]->has_wmi ? b.data() : nullptr),
+                  "mft_fts_set_fields");
+    }
+    check_mft(mft_fts_init(sys), "mft_fts_init");
+    initialized = true;
+}
+
+// computeHamiltonian (FTS_Box.cu:590-602): FP64 reductions on the device
+void FTS_Box::computeHamiltonian()
+{
+    std::vector<double> Q(2 * Molecs.size()), parts(2 * (Potentials.size() + Molecs.size()));
+    double H[2];
+    check_mft(mft_fts_observables(sys, Q.data(), H, parts.data()), "mft_fts_observables");
+    Heff = {H[0], H[1]};
+    for (size_t p = 0; p < Potentials.size(); ++p) Potentials[p]->Hterm = {parts[2 * p], parts[2 * p + 1]};
+    for (size_t m = 0; m < Molecs.size(); ++m) {
+        Molecs[m]->Q = {Q[2 * m], Q[2 * m + 1]};
+        size_t k = Potentials.size() + m;
+        Molecs[m]->Hterm = {parts[2 * k], parts[2 * k + 1]};
+    }
+}
+
+// writeData (FTS_Box.cu:145-190): step Re(H) [Im(H) | error] Hterms -n logQ
+void FTS_Box::writeData(int step)
+{
+    computeHamiltonian();
+    if (std::isnan(Heff.real())) die("Found Heff = NaN, quitting!");
+    std::ofstream otp("fts_data.dat", std::ios_base::app);
+    std::ostringstream line;
+    line << step << " " << Heff.real() << " ";
+    if (ftsStyle == "cl") line << Heff.imag() << " ";
+    else line << error << " ";
+    for (auto* p : Potentials) line << p->Hterm.real() << " ";
+    for (auto* m : Molecs) line << m->calcHTerm().real() << " ";
+    otp << line.str() << std::endl;
+    std::cout << line.str() << std::endl;
+}
+
+// converged (FTS_Box.cu:715-760): SCFT only, Heff change at log steps
+int FTS_Box::converged(int step)
+{
+    if (ftsStyle != "scft") return 0;
+    if (logFreq <= 0 || step % logFreq != 0 || step < logFreq) return 0;
+    error = std::fabs(Heff.real() - Hold);
+    Hold = Heff.real();
+    if (error < tolerance) {
+        std::cout << "SCFT converged on step " << step << " using " << tolMetric << " < " << tolerance << std::endl;
+        return 1;
+    }
+    return 0;
+}
+
+void FTS_Box::doTimeStep(int step)
+{
+    for (auto* p : Potentials) p->updateFields();
+    for (auto* m : Molecs) m->calcDensity();
+    check_mft(mft_fts_step(sys, 1), "mft_fts_step");
+    if (logFreq > 0 && step % logFreq == 0) writeData(step);
+}
+
+// NVT (FTS_Box.cu:127-142): steps between log points run as one engine call
+void FTS_Box::NVT(int nsteps)
+{
+    if (dry_run) return;
+    int i = 0;
+    while (i < nsteps) {
+        int next = nsteps;
+        if (logFreq > 0) next = std::min(nsteps, (i / logFreq + (i % logFreq ? 1 : 0)) * logFreq);
+        if (next > i) {
+            check_mft(mft_fts_step(sys, next - i), "mft_fts_step");
+            totSteps += next - i;
+            i = next;
+            continue;
+        }
+        doTimeStep(i);
+        if (i >= 50 && converged(i)) break;
+        ++totSteps;
+        ++i;
+    }
+    check_mft(mft_fts_synchronize(sys), "mft_fts_synchronize");
+}

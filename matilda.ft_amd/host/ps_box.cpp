@@ -1,0 +1,657 @@
+// Particle (TILD) box of the drop-in front end: the reference's PS_Box input
+// keywords, molecule makers and log output over the mft_tild engine.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <iostream>
+
+#include "matilda.h"
+
+// ----------------------------------------------------------- species ---
+// PS_Species::PS_Species (ps_species.cu:13-38)
+PS_Species::PS_Species(std::istringstream& iss, PS_Box* box)
+{
+    (void)box;
+    if (!(iss >> speciesLabel)) die("Missing label for species");
+    std::string word;
+    while (iss >> word) {
+        if (word == "mass" || word == "Mass") iss >> mass;
+        else if (word == "mobility" || word == "Mobility" || word == "Diffusivity" || word == "diffusivity")
+            iss >> mobility;
+        else die("Invalid species option " + word);
+    }
+}
+
+// ------------------------------------------------------------- groups ---
+// PS_Group::PS_Group (ps_groups.cu:12-60): "group name types A B ..."
+PS_Group::PS_Group(std::istringstream& iss, PS_Box* box)
+{
+    iss >> name;
+    std::string s1;
+    iss >> s1;
+    if (s1 != "types") die("group style " + s1 + " is not supported");
+    while (iss >> s1) speciesMask |= 1u << box->findSpeciesInteger(s1);
+}
+
+PS_Group::PS_Group(const std::string& n, uint32_t mask) : name(n), speciesMask(mask) {}
+
+// --------------------------------------------------------- potentials ---
+PS_Potential::PS_Potential(std::istringstream& iss, PS_Box* box) : input_command(iss.str()), mybox(box) {}
+
+int PS_Potential::ramp_check_input(std::istringstream& iss, float A)
+{
+    if (iss.fail()) die("Error during input script; failed to properly read:\n" + iss.str());
+    std::string convert;
+    if (!(iss >> convert)) return 0;
+    if (convert != "ramp") die("Invalid keyword: " + convert);
+    ramp = true;
+    Ao_initial = A;
+    if (!(iss >> Ao_final)) die("no final prefactor specified");
+    std::cout << "Ramping potential prefactor from " << Ao_initial << " to " << Ao_final << std::endl;
+    return 1;
+}
+
+void PS_Potential::update_prefactor(int step, int maxSteps)
+{
+    // the engine applies the same schedule on the device; the host copy is
+    // kept for the log
+    if (ramp && maxSteps > 0) Ao = Ao_initial + (Ao_final - Ao_initial) / maxSteps * (step + 1);
+}
+
+static mft_tild_potential pair_desc(const PS_Potential& p, PS_Box* box, int kind)
+{
+    mft_tild_potential d{};
+    d.kind = kind;
+    d.maskI = box->groupMask(p.grpI);
+    d.maskJ = box->groupMask(p.grpJ);
+    d.A = p.ramp ? p.Ao_initial : p.Ao;
+    d.sigma = std::sqrt(p.sig2);
+    d.Rp = p.Rp;
+    d.ramp = p.ramp ? 1 : 0;
+    d.A_final = p.Ao_final;
+    return d;
+}
+
+NBGauss::NBGauss(std::istringstream& iss, PS_Box* box) : PS_Potential(iss, box)
+{
+    float sigma;
+    iss >> grpI >> grpJ >> Ao >> sigma;
+    sig2 = sigma * sigma;
+    ramp_check_input(iss, Ao);
+}
+mft_tild_potential NBGauss::descriptor() const { return pair_desc(*this, mybox, MFT_POT_GAUSSIAN); }
+
+NBErf2::NBErf2(std::istringstream& iss, PS_Box* box) : PS_Potential(iss, box)
+{
+    float sigma;
+    iss >> grpI >> grpJ >> Ao >> Rp >> sigma;
+    sig2 = sigma * sigma;
+    ramp_check_input(iss, Ao);
+}
+mft_tild_potential NBErf2::descriptor() const { return pair_desc(*this, mybox, MFT_POT_ERF2); }
+
+NBErfG::NBErfG(std::istringstream& iss, PS_Box* box) : PS_Potential(iss, box)
+{
+    float sigma;
+    iss >> grpI >> grpJ >> Ao >> Rp >> sigma;
+    sig2 = sigma * sigma;
+    ramp_check_input(iss, Ao);
+}
+mft_tild_potential NBErfG::descriptor() const { return pair_desc(*this, mybox, MFT_POT_ERFG); }
+
+NBCharge::NBCharge(std::istringstream& iss, PS_Box* box) : PS_Potential(iss, box)
+{
+    grpI = grpJ = "charges";
+    float sigma;
+    iss >> LB >> sigma;
+    sig2 = sigma * sigma;
+    Ao = LB;
+}
+mft_tild_potential NBCharge::descriptor() const
+{
+    mft_tild_potential d{};
+    d.kind = MFT_POT_CHARGES;
+    d.A = LB;
+    d.sigma = std::sqrt(sig2);
+    return d;
+}
+
+PS_Potential* PSPotentialFactory(std::istringstream& iss, PS_Box* box)
+{
+    std::string s1;
+    iss >> s1;
+    if (s1 == "charges") return new NBCharge(iss, box);
+    if (s1 == "erf2") return new NBErf2(iss, box);
+    if (s1 == "erfG") return new NBErfG(iss, box);
+    if (s1 == "gaussian") return new NBGauss(iss, box);
+    if (s1 == "bias" || s1 == "bias_lc" || s1 == "Langevin" || s1 == "langevin" || s1 == "dpd" ||
+        s1 == "maiersaupe" || s1 == "wall")
+        die("ps_potential: " + s1 + " is not part of this build's particle-mesh path");
+    die("ps_potential.cu: " + s1 + " is not a supported potential");
+}
+
+// -------------------------------------------------------- integrators ---
+Integrator::Integrator(std::istringstream& iss, PS_Box* box) : mybox(box)
+{
+    if (!(iss >> groupName >> name)) die("Integrator needs a group and a style");
+    std::string word;
+    if (iss >> word) {
+        if (word == "delt") iss >> delt;
+        else die("Invalid keyword in Integrator command");
+    }
+}
+
+Integrator* IntegratorFactory(std::istringstream& iss, PS_Box* box)
+{
+    std::streampos pos = iss.tellg();
+    std::string s1;
+    iss >> s1 >> s1;
+    iss.clear();
+    iss.seekg(pos);
+    if (s1 == "EM") return new EM(iss, box);
+    if (s1 == "GJF") return new GJF(iss, box);
+    if (s1 == "VV") die("VV is not part of this build's particle-mesh path");
+    die(s1 + " is not a valid Integrator");
+}
+
+// --------------------------------------------------------------- box ----
+PS_Box::PS_Box(std::istringstream& iss) : Box(iss) { boxStyle = "ps"; }
+
+PS_Box::~PS_Box()
+{
+    for (auto* p : potentials) delete p;
+    for (auto* i : integrators) delete i;
+    if (eng) mft_tild_destroy(eng);
+}
+
+int PS_Box::findSpeciesInteger(const std::string& s) const
+{
+    for (size_t i = 0; i < species.size(); ++i)
+        if (species[i].speciesLabel == s) return (int)i;
+    die("Species " + s + " not found");
+}
+
+int PS_Box::findGroupInteger(const std::string& s) const
+{
+    for (size_t i = 0; i < psGroup.size(); ++i)
+        if (psGroup[i].name == s) return (int)i;
+    die("Group " + s + " not found");
+}
+
+// groups of the reference: "all", one per species label, "charges", and
+// user groups (PS_Box_initialization.cu:571-585)
+uint32_t PS_Box::groupMask(const std::string& s) const
+{
+    if (s == "all" || s == "All") return species.size() >= 32 ? 0xffffffffu : (1u << species.size()) - 1u;
+    for (size_t i = 0; i < species.size(); ++i)
+        if (species[i].speciesLabel == s) return 1u << i;
+    for (const auto& g : psGroup)
+        if (g.name == s) return g.speciesMask;
+    die("Group " + s + " not found");
+}
+
+void PS_Box::grow(int n)
+{
+    x.resize(3 * (size_t)n, 0.f);
+    charges.resize(n, 0.f);
+    intSpecies.resize(n, 0);
+    mID.resize(n, 0);
+    nBonds.resize(n, 0);
+    bondedTo.resize((size_t)n * MAXBONDS, 0);
+    bondType.resize((size_t)n * MAXBONDS, 0);
+}
+
+void PS_Box::enableCharges() { doCharges = 1; }
+
+void PS_Box::readInput(std::ifstream& inp)
+{
+    std::string line, firstWord;
+    bool ended = false;
+    while (!ended && std::getline(inp, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream iss(line);
+        while (iss >> firstWord) {
+            if (firstWord == "endBox") {
+                ended = true;
+                break;
+            }
+            if (firstWord == "angle") die("angles are not part of this build's particle-mesh path");
+            else if (firstWord == "blocksize" || firstWord == "blockSize") { int b; iss >> b; }
+            else if (firstWord == "bond") {
+                int btype;
+                std::string style;
+                iss >> btype >> style;
+                if (btype < 1) die("bond type must be >= 1");
+                if (btype > (int)bondK.size()) {
+                    bondK.resize(btype);
+                    bondReq.resize(btype);
+                    bondStyle.resize(btype);
+                }
+                iss >> bondK[btype - 1] >> bondReq[btype - 1];
+                if (style == "harmonic") bondStyle[btype - 1] = 0;
+                else if (style == "fene" || style == "FENE") bondStyle[btype - 1] = 1;
+                else die("Bond type not supported!");
+            } else if (firstWord == "boxLengths") {
+                read_lengths(iss);
+            } else if (firstWord == "compute") {
+                die("computes are not part of this build's particle-mesh path");
+            } else if (firstWord == "datFileName" || firstWord == "dat_file_name") {
+                iss >> datFileName;
+            } else if (firstWord == "Dim") {
+                iss >> Dim;
+                if (Dim != 3) die("this build runs 3D boxes (Dim 3)");
+                readDimension = true;
+            } else if (firstWord == "doCharges") {
+                enableCharges();
+            } else if (firstWord == "fieldFreq" || firstWord == "field_freq" || firstWord == "gsdFreq" ||
+                       firstWord == "gsd_freq" || firstWord == "trajFreq" || firstWord == "traj_freq") {
+                int f;
+                iss >> f;
+                std::string rest;
+                while (iss >> rest) {}
+                if (f > 0) std::cout << "note: " << firstWord << " output is not written by this build" << std::endl;
+            } else if (firstWord == "gsdName" || firstWord == "gsd_name" || firstWord == "trajFileName" ||
+                       firstWord == "initDataFileName") {
+                std::string s;
+                iss >> s;
+            } else if (firstWord == "grid") {
+                read_grid(iss);
+            } else if (firstWord == "group") {
+                psGroup.push_back(PS_Group(iss, this));
+            } else if (firstWord == "integrator") {
+                integrators.push_back(IntegratorFactory(iss, this));
+            } else if (firstWord == "logFreq" || firstWord == "logfreq") {
+                iss >> logFreq;
+            } else if (firstWord == "MAXANGLES") {
+                int a;
+                iss >> a;
+            } else if (firstWord == "MAXBONDS") {
+                if (nstot > 0) die("MAXBONDS must be set before defining any molecules");
+                iss >> MAXBONDS;
+            } else if (firstWord == "molecule") {
+                std::string next;
+                iss >> next;
+                if (next == "linear") makeLinear(iss);
+                else if (next == "grafted" || next == "sclc")
+                    die("molecule " + next + " is not part of this build's particle-mesh path");
+                else die("PS_Box_init:readInput: invalid molecule type read");
+            } else if (firstWord == "Nr") {
+                double nr;
+                iss >> nr;
+            } else if (firstWord == "NSEXTRA" || firstWord == "nsextra") {
+                int n;
+                iss >> n;
+            } else if (firstWord == "Nx" || firstWord == "Ny" || firstWord == "Nz") {
+                int a = firstWord == "Nx" ? 0 : (firstWord == "Ny" ? 1 : 2);
+                iss >> Nx[a];
+                M *= Nx[a];
+            } else if (firstWord == "pmeorder") {
+                iss >> pmeorder;
+            } else if (firstWord == "potential") {
+                potentials.push_back(PSPotentialFactory(iss, this));
+            } else if (firstWord == "neighbor_list") {
+                die("neighbor lists are not part of this build's particle-mesh path");
+            } else if (firstWord == "randSeed" || firstWord == "RAND_SEED" || firstWord == "RANDSEED") {
+                long long sd;
+                iss >> sd;
+                RANDSEED = (uint64_t)sd;
+                rng.reseed(RANDSEED);
+            } else if (firstWord == "readData") {
+                std::string name;
+                iss >> name;
+                readDataConfig(name);
+            } else if (firstWord == "rho0") {
+                iss >> rho0;
+            } else if (firstWord == "species") {
+                species.push_back(PS_Species(iss, this));
+            } else if (firstWord == "verbose") {
+                int v;
+                iss >> v;
+            } else {
+                die("Invalid keyword " + firstWord + " in FTS_Box::readInput()");
+            }
+        }
+    }
+    finishInitialization();
+}
+
+// PS_Box::makeLinear (PS_Box_moleculeMakers.cu:15-311): chain ends uniform in
+// the (x/y/z)range, each further bead one unit vector from the previous,
+// wrapped once; bonds to the previous and next beads.
+void PS_Box::makeLinear(std::istringstream& iss)
+{
+    if (rho0 < 0.0) die("rho0 must be defined before molecules created!");
+    double phi = -1.0;
+    int nmolecs = -1, numBlocks = 0, Ntot = 0;
+    std::string s1;
+    iss >> s1;
+    if (s1 == "phi") iss >> phi;
+    else if (s1 == "nmolecs") iss >> nmolecs;
+    else die("invalid keyword after molecule linear. must be 'phi' or 'nmolecs'");
+    iss >> numBlocks;
+    if (numBlocks < 1) die("molecule linear needs at least one block");
+    std::vector<int> Nb(numBlocks), bType(numBlocks, 1);
+    std::vector<std::string> sp(numBlocks);
+    std::vector<float> bq(numBlocks, 0.f);
+    bool molCharges = false;
+    float Rmin[3] = {0.f, 0.f, 0.f}, Rmax[3] = {L[0], L[1], L[2]};
+    for (int j = 0; j < numBlocks; ++j) {
+        iss >> Nb[j] >> sp[j];
+        Ntot += Nb[j];
+    }
+    while (iss >> s1) {
+        if (s1 == "charge") {
+            enableCharges();
+            molCharges = true;
+            for (int j = 0; j < numBlocks; ++j) iss >> bq[j];
+        } else if (s1 == "bondType" || s1 == "bondtype") {
+            for (int j = 0; j < numBlocks; ++j) iss >> bType[j];
+        } else if (s1 == "angleType" || s1 == "angletype") {
+            die("angles are not part of this build's particle-mesh path");
+        } else if (s1 == "xrange") {
+            iss >> Rmin[0] >> Rmax[0];
+        } else if (s1 == "yrange") {
+            iss >> Rmin[1] >> Rmax[1];
+        } else if (s1 == "zrange") {
+            iss >> Rmin[2] >> Rmax[2];
+        } else if (s1 == "drude") {
+            die("drude oscillators not implemented!");
+        } else {
+            die("invalid molecule option " + s1);
+        }
+    }
+    if (phi > 0.0) nmolecs = int(rho0 * V * phi / float(Ntot));
+    if (nmolecs < 0) die("molecule linear: negative molecule count");
+    std::cout << "Generating " << nmolecs << " molecules each with " << Ntot << " sites" << std::endl;
+    int ind = nstot;
+    const int molec0 = nstot > 0 ? *std::max_element(mID.begin(), mID.end()) + 1 : 0;
+    nstot += nmolecs * Ntot;
+    grow(nstot);
+    std::vector<int> spi(numBlocks);
+    for (int j = 0; j < numBlocks; ++j) spi[j] = findSpeciesInteger(sp[j]);
+    for (int i = 0; i < nmolecs; ++i) {
+        double ru[3];
+        for (int j = 0; j < numBlocks; ++j) {
+            for (int s = 0; s < Nb[j]; ++s) {
+                intSpecies[ind] = spi[j];
+                if (j == 0 && s == 0) {
+                    for (int a = 0; a < 3; ++a) x[ind * 3 + a] = (float)(rng.uniform() * (Rmax[a] - Rmin[a]) + Rmin[a]);
+                } else {
+                    rng.unit_vector(ru);
+                    for (int a = 0; a < 3; ++a) {
+                        float v = x[(ind - 1) * 3 + a] + (float)ru[a];
+                        if (v > L[a]) v -= L[a];
+                        else if (v < 0.f) v += L[a];
+                        x[ind * 3 + a] = v;
+                    }
+                }
+                if (molCharges) charges[ind] = bq[j];
+                nBonds[ind] = 0;
+                if (j != 0 || s != 0) {
+                    bondedTo[ind * MAXBONDS + nBonds[ind]] = ind - 1;
+                    bondType[ind * MAXBONDS + nBonds[ind]] = bType[j] - 1;
+                    nBonds[ind]++;
+                }
+                if (j != numBlocks - 1 || s != Nb[j] - 1) {
+                    if (nBonds[ind] >= MAXBONDS) die("MAXBONDS too small for linear molecules");
+                    bondedTo[ind * MAXBONDS + nBonds[ind]] = ind + 1;
+                    bondType[ind * MAXBONDS + nBonds[ind]] = bType[j] - 1;
+                    nBonds[ind]++;
+                }
+                mID[ind] = molec0 + i;
+                ++ind;
+            }
+        }
+    }
+}
+
+// PS_Box::readDataConfig (PS_Box_initialization.cu:787-990): LAMMPS-style
+// data file with Masses, Atoms (id mol type [q] x y z), Bonds, Angles.
+void PS_Box::readDataConfig(const std::string& name)
+{
+    if (nstot != 0) die("Must read data file before making additional molecules");
+    std::ifstream in(name);
+    if (!in) die("Failed to open " + name);
+    std::string line;
+    auto next_nonblank = [&](std::string& out) {
+        while (std::getline(in, out))
+            if (out.find_first_not_of(" \t\r") != std::string::npos) return true;
+        return false;
+    };
+    std::getline(in, line); // title
+    int nat = 0, nb = 0, nang = 0, ntypes = 0, nbt = 0, nat_t = 0;
+    double lo, hi;
+    while (next_nonblank(line)) {
+        std::istringstream ls(line);
+        if (line.find("atoms") != std::string::npos && line.find("types") == std::string::npos) ls >> nat;
+        else if (line.find("bonds") != std::string::npos && line.find("types") == std::string::npos) ls >> nb;
+        else if (line.find("angles") != std::string::npos && line.find("types") == std::string::npos) ls >> nang;
+        else if (line.find("atom types") != std::string::npos) ls >> ntypes;
+        else if (line.find("bond types") != std::string::npos) ls >> nbt;
+        else if (line.find("angle types") != std::string::npos) ls >> nat_t;
+        else if (line.find("xlo") != std::string::npos) { ls >> lo >> hi; L[0] = (float)(hi - lo); }
+        else if (line.find("ylo") != std::string::npos) { ls >> lo >> hi; L[1] = (float)(hi - lo); }
+        else if (line.find("zlo") != std::string::npos) { ls >> lo >> hi; L[2] = (float)(hi - lo); }
+        else if (line.find("Masses") != std::string::npos) break;
+    }
+    (void)nat_t;
+    (void)nbt;
+    V = (double)L[0] * L[1] * L[2];
+    if ((int)species.size() < ntypes) die("readData: define every species before readData");
+    for (int i = 0; i < ntypes; ++i) {
+        next_nonblank(line);
+        std::istringstream ls(line);
+        int t;
+        float m;
+        ls >> t >> m;
+        if (t >= 1 && t <= (int)species.size()) species[t - 1].mass = m;
+    }
+    next_nonblank(line); // Atoms
+    nstot = nat;
+    grow(nstot);
+    for (int i = 0; i < nat; ++i) {
+        if (!next_nonblank(line)) die("Premature end of " + name);
+        std::istringstream ls(line);
+        int id, mol, typ;
+        ls >> id >> mol >> typ;
+        if (id < 1 || id > nat) die("readData: atom id out of range");
+        int k = id - 1;
+        mID[k] = mol - 1;
+        intSpecies[k] = typ - 1;
+        if (doCharges) ls >> charges[k];
+        for (int a = 0; a < 3; ++a) ls >> x[k * 3 + a];
+    }
+    if (nb > 0) {
+        next_nonblank(line); // Bonds
+        for (int i = 0; i < nb; ++i) {
+            if (!next_nonblank(line)) die("Premature end of " + name);
+            std::istringstream ls(line);
+            int cnt, bt, i1, i2;
+            ls >> cnt >> bt >> i1 >> i2;
+            --bt;
+            --i1;
+            --i2;
+            if (i2 < i1) std::swap(i1, i2);
+            if (nBonds[i1] >= MAXBONDS || nBonds[i2] >= MAXBONDS) die("readData: MAXBONDS exceeded");
+            bondedTo[i1 * MAXBONDS + nBonds[i1]] = i2;
+            bondType[i1 * MAXBONDS + nBonds[i1]++] = bt;
+            bondedTo[i2 * MAXBONDS + nBonds[i2]] = i1;
+            bondType[i2 * MAXBONDS + nBonds[i2]++] = bt;
+        }
+    }
+    if (nang > 0) die("angles are not part of this build's particle-mesh path");
+    nBondsTot = nb;
+}
+
+// finishInitialization (PS_Box_initialization.cu:352-426, 551-587): groups,
+// the engine description, topology and state on the device.
+void PS_Box::finishInitialization()
+{
+    if (M <= 1 || Nx[0] <= 0) die("grid must be defined");
+    if (L[0] <= 0.f) die("boxLengths must be defined");
+    if (species.empty()) die("no species defined");
+    if (nstot == 0) die("Box created with no particles");
+    if (integrators.size() != 1) die("this build runs exactly one integrator on group all");
+    if (integrators[0]->groupName != "all") die("integrator group must be 'all' in this build");
+    nBondsTot = 0;
+    for (int i = 0; i < nstot; ++i) nBondsTot += nBonds[i];
+    nBondsTot /= 2;
+    // the reference checks neutrality when charges are on (PS_Box_initialization.cu:398-412)
+    std::vector<mft_tild_potential> pd;
+    for (auto* p : potentials) pd.push_back(p->descriptor());
+    std::vector<float> mass, mob;
+    for (const auto& s : species) {
+        mass.push_back(s.mass);
+        mob.push_back(s.mobility);
+    }
+    mft_tild_desc d{};
+    d.nx = Nx[0];
+    d.ny = Nx[1];
+    d.nz = Nx[2];
+    d.lx = L[0];
+    d.ly = L[1];
+    d.lz = L[2];
+    d.pmeorder = pmeorder;
+    d.nparticles = nstot;
+    d.nspecies = (int)species.size();
+    d.maxbonds = MAXBONDS;
+    d.nbondtypes = (int)bondK.size();
+    d.bond_k = bondK.data();
+    d.bond_r0 = bondReq.data();
+    d.bond_style = bondStyle.data();
+    d.npotentials = (int)pd.size();
+    d.potentials = pd.data();
+    d.integrator = integrators[0]->engine_kind();
+    d.delt = integrators[0]->delt;
+    d.mass = mass.data();
+    d.mobility = mob.data();
+    d.seed = RANDSEED;
+    d.has_charges = doCharges;
+    if (dry_run) {
+        std::printf("{\"box\": \"ps\", \"grid\": [%d, %d, %d], \"L\": [%.9g, %.9g, %.9g], \"pmeorder\": %d, "
+                    "\"nparticles\": %d, \"nspecies\": %d, \"maxbonds\": %d, \"nbonds\": %d, \"integrator\": %d, "
+                    "\"delt\": %.9g, \"seed\": %llu, \"has_charges\": %d, \"potentials\": [",
+                    d.nx, d.ny, d.nz, d.lx, d.ly, d.lz, d.pmeorder, d.nparticles, d.nspecies, d.maxbonds, nBondsTot,
+                    d.integrator, d.delt, (unsigned long long)d.seed, d.has_charges);
+        for (size_t i = 0; i < pd.size(); ++i)
+            std::printf("%s{\"kind\": %d, \"maskI\": %u, \"maskJ\": %u, \"A\": %.9g, \"sigma\": %.9g, \"Rp\": %.9g, "
+                        "\"ramp\": %d, \"A_final\": %.9g}",
+                        i ? ", " : "", pd[i].kind, pd[i].maskI, pd[i].maskJ, pd[i].A, pd[i].sigma, pd[i].Rp,
+                        pd[i].ramp, pd[i].A_final);
+        std::printf("], \"bonds\": [");
+        for (size_t i = 0; i < bondK.size(); ++i)
+            std::printf("%s[%d, %.9g, %.9g]", i ? ", " : "", bondStyle[i], bondK[i], bondReq[i]);
+        std::printf("]}\n");
+        std::fflush(stdout);
+        initialized = true;
+        return;
+    }
+    check_mft(mft_tild_create(&eng, &d), "mft_tild_create");
+    check_mft(mft_tild_set_topology(eng, intSpecies.data(), doCharges ? charges.data() : nullptr, nBonds.data(),
+                                    bondedTo.data(), bondType.data()),
+              "mft_tild_set_topology");
+    check_mft(mft_tild_set_state(eng, x.data(), nullptr, nullptr), "mft_tild_set_state");
+    int choice = 0;
+    check_mft(mft_tild_autotune(eng, 3, &choice), "mft_tild_autotune");
+    for (auto* p : potentials) p->initializePotential();
+    for (auto* i : integrators) i->finishInitialization();
+    initialized = true;
+}
+
+// writeData (PS_Box.cu:339-372): step, Ubond (with bonds), potential energies
+void PS_Box::computeThermoProps()
+{
+    std::vector<double> pe(std::max<size_t>(1, potentials.size()), 0.0);
+    double be = 0.0;
+    check_mft(mft_tild_energies(eng, pe.data(), &be), "mft_tild_energies");
+    Ubond = (float)be;
+    Upe = Ubond;
+    for (size_t i = 0; i < potentials.size(); ++i) {
+        potentials[i]->energy = (float)pe[i];
+        Upe += (float)pe[i];
+    }
+}
+
+void PS_Box::writeData(int step)
+{
+    computeThermoProps();
+    std::ofstream otp(datFileName, std::ios_base::app);
+    otp << step;
+    std::cout << "step: " << step << " Upe: " << Upe;
+    if (nBondsTot > 0) {
+        otp << " " << Ubond;
+        std::cout << " Ubond: " << Ubond;
+    }
+    for (size_t i = 0; i < potentials.size(); ++i) {
+        otp << " " << potentials[i]->energy;
+        std::cout << " pot[" << i << "]: " << potentials[i]->energy;
+    }
+    otp << std::endl;
+    std::cout << std::endl;
+}
+
+// doTimeStep (PS_Box.cu:73-207) as one engine step; potentials' host copies
+// follow update_prefactor; the log at step % logFreq == 0.
+void PS_Box::doTimeStep(int step)
+{
+    for (auto* i : integrators) i->Integrate_1();
+    check_mft(mft_tild_step(eng, 1), "mft_tild_step");
+    for (auto* p : potentials) p->update_prefactor(step, maxSteps);
+    if (logFreq > 0 && step % logFreq == 0) writeData(step);
+}
+
+void PS_Box::NVT(int nSteps)
+{
+    if (dry_run) return;
+    writeData(-1);
+    maxSteps = nSteps;
+    check_mft(mft_tild_set_run(eng, nSteps), "mft_tild_set_run");
+    // steps between log points run as one engine call
+    int i = 0;
+    while (i < maxSteps) {
+        int next = maxSteps;
+        if (logFreq > 0) next = std::min(maxSteps, (i / logFreq + (i % logFreq ? 1 : 0)) * logFreq);
+        if (next > i) {
+            check_mft(mft_tild_step(eng, next - i), "mft_tild_step");
+            for (int s = i; s < next; ++s)
+                for (auto* p : potentials) p->update_prefactor(s, maxSteps);
+            totSteps += next - i;
+            i = next;
+            continue;
+        }
+        doTimeStep(i);
+        ++totSteps;
+        ++i;
+    }
+    check_mft(mft_tild_synchronize(eng), "mft_tild_synchronize");
+    std::cout << "NVT Finished, writing final output" << std::endl;
+    writeData(maxSteps);
+    writeDataConfig("final.input.data");
+}
+
+// LAMMPS-style data file of the current state (PS_Box.cu:629-720)
+void PS_Box::writeDataConfig(const std::string& name)
+{
+    std::vector<float> xc(3 * (size_t)nstot);
+    check_mft(mft_tild_get_state(eng, xc.data(), nullptr, nullptr), "mft_tild_get_state");
+    FILE* otp = std::fopen(name.c_str(), "w");
+    if (!otp) die("Failed to open " + name);
+    std::fprintf(otp, "Generated by the MI355X front end\n\n%d atoms\n%d bonds\n0 angles\n\n", nstot, nBondsTot);
+    std::fprintf(otp, "%d atom types\n%d bond types\n0 angle types\n\n", (int)species.size(), (int)bondK.size());
+    std::fprintf(otp, "0 %f xlo xhi\n0 %f ylo yhi\n0 %f zlo zhi\n\nMasses\n\n", L[0], L[1], L[2]);
+    for (size_t i = 0; i < species.size(); ++i) std::fprintf(otp, "%zu %f\n", i + 1, species[i].mass);
+    std::fprintf(otp, "\nAtoms\n\n");
+    for (int i = 0; i < nstot; ++i) {
+        std::fprintf(otp, "%d %d %d ", i + 1, mID[i] + 1, intSpecies[i] + 1);
+        if (doCharges) std::fprintf(otp, "%f ", charges[i]);
+        std::fprintf(otp, "%.9g %.9g %.9g\n", xc[3 * i], xc[3 * i + 1], xc[3 * i + 2]);
+    }
+    std::fprintf(otp, "\nBonds\n\n");
+    int b = 0;
+    for (int i = 0; i < nstot; ++i)
+        for (int k = 0; k < nBonds[i]; ++k) {
+            int j = bondedTo[i * MAXBONDS + k];
+            if (j > i) std::fprintf(otp, "%d %d %d %d\n", ++b, bondType[i * MAXBONDS + k] + 1, i + 1, j + 1);
+        }
+    std::fclose(otp);
+}

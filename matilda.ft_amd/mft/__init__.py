@@ -304,6 +304,11 @@ class TildEngine:
     def step(self, n=1):
         check(lib().mft_tild_step(self.h, n))
 
+    def set_run(self, max_steps):
+        """Declare a run of max_steps steps: potential ramps follow the step
+        index within the run across step() calls."""
+        check(lib().mft_tild_set_run(self.h, C.c_int64(max_steps)))
+
     def synchronize(self):
         check(lib().mft_tild_synchronize(self.h))
 

@@ -1,0 +1,82 @@
+"""The drop-in front end parses the reference's input syntax (the hot-path
+configurations in its v2 grammar) into the engine description; checked with
+--dry-run, which needs no GPU."""
+import json
+import os
+
+import pytest
+
+from frontend_util import INPUTS, MFT_BIN, run_mft
+
+pytestmark = pytest.mark.skipif(not os.path.exists(MFT_BIN), reason="front end not built")
+
+
+def dry(tmp_path, text=None, name=None):
+    if text is not None:
+        p = tmp_path / "input"
+        p.write_text(text)
+        name = str(p)
+    r = run_mft(["-in", name, "--dry-run"], cwd=tmp_path, timeout=120)
+    return r
+
+
+def desc(tmp_path, name):
+    r = dry(tmp_path, name=os.path.join(INPUTS, name))
+    assert r.returncode == 0, r.stdout[-2000:]
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def test_c1_diblock(tmp_path):
+    d = desc(tmp_path, "c1_diblock.in")
+    assert d["grid"] == [32, 32, 32] and d["L"] == [20, 20, 20] and d["pmeorder"] == 3
+    assert d["nparticles"] == 40000 and d["nspecies"] == 2 and d["nbonds"] == 2000 * 19
+    assert d["integrator"] == 0 and d["delt"] == pytest.approx(0.005) and d["seed"] == 1234
+    pots = d["potentials"]
+    assert [(p["maskI"], p["maskJ"]) for p in pots] == [(1, 1), (2, 2), (1, 2)]
+    assert [p["A"] for p in pots] == pytest.approx([5.0, 5.0, 10.2])
+    assert all(p["sigma"] == pytest.approx(0.70711) and p["kind"] == 0 for p in pots)
+    assert d["bonds"] == [[0, 1.5, 0.0]]
+
+
+def test_c2_coacervate(tmp_path):
+    d = desc(tmp_path, "c2_coacervate.in")
+    assert d["nparticles"] == 1800000 and d["nspecies"] == 3 and d["has_charges"] == 1
+    assert d["nbonds"] == 2 * 40000 * 19
+    assert d["potentials"][-1]["kind"] == 3 and d["potentials"][-1]["A"] == pytest.approx(2.0)
+    assert d["potentials"][-1]["sigma"] == pytest.approx(0.5)
+
+
+def test_c3_fts(tmp_path):
+    d = desc(tmp_path, "c3_fts_diblock.in")
+    assert d["style"] == "cl" and d["grid"] == [64, 64, 64] and d["Nr"] == 100 and d["C"] == 50
+    h, f = d["potentials"]
+    assert h["kind"] == 0 and h["kappaN"] == 100 and h["delt_pl"] == pytest.approx(0.01)
+    assert f["kind"] == 1 and (f["I"], f["J"]) == (0, 1) and f["chiN"] == 20
+    assert d["molecules"] == [{"phi": 1.0, "blocks": [[50, 0], [50, 1]]}]
+
+
+def test_flory_initialize_sets_wplus(tmp_path):
+    # fts_potential_flory.cu:83-88: initialize also sets w+ = -1.5i
+    text = open(os.path.join(INPUTS, "c3_fts_diblock.in")).read().replace(
+        "potential Flory A B 20 0.01 0.01", "potential Flory A B 20 0.01 0.01 initialize value 0.2 0.0")
+    r = dry(tmp_path, text)
+    assert r.returncode == 0, r.stdout[-1000:]
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    assert d["potentials"][1]["wpl0"] == [0.0, -1.5]
+
+
+def test_ramp_and_groups(tmp_path):
+    text = open(os.path.join(INPUTS, "c1_diblock.in")).read().replace(
+        "potential gaussian A B 10.2 0.70711", "group AB types A B\npotential gaussian AB all 10.2 0.70711 ramp 20.4")
+    r = dry(tmp_path, text)
+    assert r.returncode == 0, r.stdout[-1000:]
+    p = json.loads(r.stdout.strip().splitlines()[-1])["potentials"][2]
+    assert (p["maskI"], p["maskJ"], p["ramp"]) == (3, 3, 1) and p["A_final"] == pytest.approx(20.4)
+
+
+@pytest.mark.parametrize("bad, msg", [("Dim 3\nbogus 1\n", "Invalid keyword bogus"),
+                                       ("Dim 3\npotential wobble A A 1 1\n", "is not a supported potential"),
+                                       ("Dim 3\nintegrator all XYZ\n", "XYZ is not a valid Integrator")])
+def test_errors_die_like_the_reference(tmp_path, bad, msg):
+    r = dry(tmp_path, "box ps\n" + bad + "endBox\n")
+    assert r.returncode == 1 and msg in r.stdout

@@ -238,6 +238,32 @@ def test_energies_match_oracle(gpu, case):
     eng.close()
 
 
+@pytest.mark.parametrize("case", ["erf_kinds", "many_shapes"])
+def test_engine_potential_shapes(gpu, case):
+    """erf2/erfG tables next to Gaussians (potentials grouped by k-space shape),
+    and more distinct shapes than the kernel groups (each potential evaluated
+    on its own)."""
+    cfg = small_system(False, seed=8)
+    A = cfg["potentials"][0]["A"]
+    if case == "erf_kinds":
+        cfg["potentials"] = [dict(kind="erf2", maskI=1, maskJ=1, A=A, sigma=0.4, Rp=0.6),
+                             dict(kind="erfG", maskI=2, maskJ=2, A=A, sigma=0.5, Rp=0.5),
+                             dict(kind="gaussian", maskI=1, maskJ=2, A=2 * A, sigma=0.6),
+                             dict(kind="gaussian", maskI=1, maskJ=1, A=0.5 * A, sigma=0.6)]
+    else:
+        cfg["potentials"] = [dict(kind="gaussian", maskI=1 + (k % 2), maskJ=1 + (k // 2) % 2, A=A * (1 + 0.1 * k),
+                                  sigma=0.4 + 0.05 * k) for k in range(10)]
+    say("case: potential shapes", case)
+    eng, sys_o = _engine_vs_oracle(gpu, cfg)
+    f_g = eng.compute_forces()
+    out = orc.tild_forces(sys_o, want_energy=True)
+    assert rel(f_g, out["f"]) < TOL
+    pe, _ = eng.energies()
+    for g, o in zip(pe, out["energies"]):
+        assert abs(g - o) <= TOL * max(abs(o), 1e-12), (g, o)
+    eng.close()
+
+
 def test_ramp_schedule(gpu):
     # PS_Potential::update_prefactor: step i uses Ao_i + (Ao_f - Ao_i) i / nsteps
     cfg = small_system(False, seed=2)
