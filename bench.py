@@ -127,6 +127,79 @@ def fts_measure(iters_plain=20, iters_batch=5, nrep=256, seed=7):
     return out
 
 
+def serve_measure(seed=11, window=1.0, rate=50.0, tild_frac=0.2, max_iters=25, budget_frac=0.7, max_pool=16):
+    """Served mixed workload: a Poisson stream (rate per s) of CL-FTS diblock
+    solves (grids 32^3-96^3, chi N in [8, 25], 500-5000 iterations, Ns = 50;
+    at most max_pool solves share the pool at a time)
+    and TILD diblock runs (5k-500k particles at rho0 = 5, grids 32^3-128^3,
+    200 steps) arriving over `window` seconds, served with continuous batching
+    and drained.  Latency = completion - scheduled arrival."""
+    import numpy as np
+    import mft
+    from mft import systems
+
+    rng = np.random.default_rng(seed)
+    reqs = []
+    t = 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > window:
+            break
+        if rng.random() < tild_frac:
+            n = int(rng.uniform(5e3, 5e5)) // 20 * 20
+            grid = int(rng.choice([32, 48, 64, 96, 128]))
+            cfg = systems.tild_diblock(nchains=n // 20, NA=10, NB=10, box=(n / 5.0) ** (1 / 3), grid=grid,
+                                       seed=int(rng.integers(1 << 30)))
+            reqs.append((t, "tild", cfg, 200))
+        else:
+            g = int(rng.choice([32, 48, 64, 96]))
+            # box 0.10-0.16 Rg per grid point (the explicit CL step at dt = 0.01
+            # is unstable on much finer grids)
+            spec, fields = systems.fts_diblock(grid_shape=(g, g, g), L=float(g * rng.uniform(0.10, 0.16)), Nr=50.0, NA=25,
+                                               NB=25, chiN=float(rng.uniform(8, 25)), kappaN=100.0, C=50.0,
+                                               dt=0.01, sigma=0.1, seed=int(rng.integers(1 << 30)))
+            reqs.append((t, "fts", (spec, fields), int(rng.uniform(500, 5000))))
+    free = mft.mem_info()[0]
+    srv = mft.Server(hbm_budget_bytes=budget_frac * free, max_fts_replicas=max_pool)
+    arrival, finish, kind = {}, {}, {}
+    errors = []
+    pending = list(reqs)
+    running = set()
+    t0 = time.perf_counter()
+    while pending or running:
+        now = time.perf_counter() - t0
+        while pending and pending[0][0] <= now:
+            ta, k, payload, n = pending.pop(0)
+            rid = srv.submit_tild(payload, n) if k == "tild" else srv.submit_fts(payload[0], payload[1], n)
+            arrival[rid], kind[rid] = ta, k
+            running.add(rid)
+        if not running:
+            time.sleep(max(0.0, pending[0][0] - (time.perf_counter() - t0)))
+            continue
+        srv.pump(max_iters)
+        now = time.perf_counter() - t0
+        for rid in list(running):
+            st = srv.status(rid)
+            if st in ("done", "failed"):
+                finish[rid] = now if st == "done" else float("nan")
+                if st == "failed":
+                    errors.append(srv.error(rid))
+                running.discard(rid)
+                srv.release(rid)
+    total = time.perf_counter() - t0
+    srv.close()
+    lat = {k: np.array([finish[r] - arrival[r] for r in finish if kind[r] == k]) for k in ("fts", "tild")}
+    allv = np.concatenate([lat["fts"], lat["tild"]])
+    nf = int(np.sum(np.isfinite(lat["fts"])))
+    return {"requests": len(reqs), "fts_requests": int(lat["fts"].size), "tild_requests": int(lat["tild"].size),
+            "arrival_window_s": window, "rate_per_s": rate, "wall_s": total,
+            "p50_latency_s": float(np.nanpercentile(allv, 50)), "p99_latency_s": float(np.nanpercentile(allv, 99)),
+            "fts_p50_s": float(np.nanpercentile(lat["fts"], 50)) if lat["fts"].size else None,
+            "fts_p99_s": float(np.nanpercentile(lat["fts"], 99)) if lat["fts"].size else None,
+            "solves_per_hour": 3600.0 * nf / total,
+            "failed": int(np.sum(~np.isfinite(allv))), "errors": sorted(set(errors))[:5]}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +208,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fts", action="store_true", help="skip the CL-FTS plain/batched measurements")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served mixed workload")
+    ap.add_argument("--serve-window", type=float, default=1.0, help="seconds of Poisson arrivals to serve")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -204,6 +279,8 @@ def main():
     eng.close()
     if not args.no_fts:
         out["fts"] = fts_measure()
+    if not args.no_serve:
+        out["serve"] = serve_measure(window=args.serve_window)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.seed)
     print(json.dumps(out), flush=True)

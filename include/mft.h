@@ -34,6 +34,8 @@ int mft_version(void);
 /* ------------------------------------------------------------ device ---- */
 /* Thin helpers so that hosts without a GPU framework can move data. */
 int mft_device_count(int* n);
+/* Free and total device memory of the current device (hipMemGetInfo). */
+int mft_mem_info(size_t* free_bytes, size_t* total_bytes);
 int mft_malloc(void** dptr, size_t bytes);
 int mft_free(void* dptr);
 int mft_memcpy_h2d(void* dst, const void* src, size_t bytes);
@@ -268,6 +270,44 @@ int mft_fts_batch_step(mft_fts_batch* b, int niter);
 int mft_fts_batch_synchronize(mft_fts_batch* b);
 int mft_fts_batch_observables(mft_fts_batch* b, int rep, double* Q, double* H, double* parts);
 int mft_fts_batch_get_densities(mft_fts_batch* b, int rep, float* rho);
+
+/* ============================================================ serving ==== */
+/* Request loop with continuous batching (no counterpart in the reference,
+ * whose sweeps re-solve one system at a time, FTS_BoxFindSpinodal.cu:25-371):
+ * CL-FTS solves and TILD runs are submitted at any time, admitted while the
+ * device-memory estimate of the admitted work stays within hbm_budget_bytes,
+ * and advanced together: every admitted CL-FTS replica joins the one
+ * continuously batched pool (replicas of one grid share each launch), every
+ * TILD run has its own engine stream, so their work overlaps on the device.
+ * A request's result does not depend on what it was batched with. */
+typedef struct mft_server mft_server;
+enum { MFT_REQ_QUEUED = 0, MFT_REQ_RUNNING = 1, MFT_REQ_DONE = 2, MFT_REQ_FAILED = 3 };
+int mft_server_create(mft_server** srv, double hbm_budget_bytes, int max_fts_replicas);
+int mft_server_destroy(mft_server* srv);
+/* niter iterations from the given initial fields: wpl/wmi are arrays of
+ * desc->npotentials host pointers to M complex values (NULL = zero field). */
+int mft_server_submit_fts(mft_server* srv, const mft_fts_desc* desc, const float* const* wpl,
+                          const float* const* wmi, int niter, int64_t* id);
+/* nsteps of a TILD system; topology and positions as for mft_tild_set_topology
+ * and mft_tild_set_state (all copied at submission). */
+int mft_server_submit_tild(mft_server* srv, const mft_tild_desc* desc, const int* type, const float* charge,
+                           const int* nBonds, const int* bondedTo, const int* bondType, const float* x,
+                           int nsteps, int64_t* id);
+/* One pass of the loop: admit queued requests that fit, run up to max_iters
+ * pool iterations (fewer when an admitted solve needs fewer), retire finished
+ * work.  Returns the number of running and queued requests. */
+int mft_server_pump(mft_server* srv, int max_iters, int* nrunning, int* nqueued);
+int mft_server_status(mft_server* srv, int64_t id, int* state);
+/* Message of a failed request (empty otherwise). */
+const char* mft_server_error(mft_server* srv, int64_t id);
+/* Results of a finished CL-FTS request: Q[2*nmolecules], H[2],
+ * parts[2*(npotentials+nmolecules)] as mft_fts_observables; rho (may be NULL)
+ * receives the species densities [nspecies*M] complex. */
+int mft_server_fts_result(mft_server* srv, int64_t id, double* Q, double* H, double* parts, float* rho);
+/* Results of a finished TILD request: positions x[3N] (may be NULL) and the
+ * energies of mft_tild_energies at the final state. */
+int mft_server_tild_result(mft_server* srv, int64_t id, float* x, double* pot_energy, double* bond_energy);
+int mft_server_release(mft_server* srv, int64_t id);
 
 #ifdef __cplusplus
 }

@@ -52,6 +52,13 @@ const char* mft_last_error(void) { return g_msg; }
 
 int mft_version(void) { return 1; }
 
+int mft_mem_info(size_t* free_bytes, size_t* total_bytes)
+{
+    MFT_REQUIRE(free_bytes && total_bytes, MFT_ERR_INVALID, "null argument");
+    MFT_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    return MFT_OK;
+}
+
 int mft_device_count(int* n)
 {
     MFT_REQUIRE(n, MFT_ERR_INVALID, "null argument");

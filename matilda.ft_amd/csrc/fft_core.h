@@ -155,6 +155,26 @@ MFT_PLAN(192, 16, 4, 4, 4, 3)
 MFT_PLAN(256, 32, 8, 8, 4)
 #undef MFT_PLAN
 
+// Plans with four elements per thread for kernels that keep several lines
+// in registers at once (the fused k-space pass): more threads per line,
+// fewer registers per thread, one more LDS pass.  Sizes without one use the
+// default plan.
+template <int N>
+struct FftPlanLow : FftPlan<N> {};
+#define MFT_PLAN_LOW(N_, T_, ...)                                               \
+    template <>                                                                 \
+    struct FftPlanLow<N_> {                                                     \
+        static constexpr int N = N_;                                            \
+        static constexpr int T = T_;                                            \
+        static constexpr int E = N_ / T_;                                       \
+        using R = Radices<__VA_ARGS__>;                                         \
+    };
+MFT_PLAN_LOW(32, 8, 4, 4, 2)
+MFT_PLAN_LOW(64, 16, 4, 4, 4)
+MFT_PLAN_LOW(128, 32, 4, 4, 4, 2)
+MFT_PLAN_LOW(256, 64, 4, 4, 4, 4)
+#undef MFT_PLAN_LOW
+
 // Expands F(N) for every supported size; used by host dispatch switches.
 #define MFT_FOR_EACH_FFT_SIZE(F)                                                 \
     F(2) F(3) F(4) F(5) F(6) F(8) F(10) F(12) F(16) F(20) F(24) F(32) F(40)    \
@@ -242,6 +262,14 @@ __device__ __forceinline__ void line_fft(float2 (&v)[N / FftPlan<N>::T], float2*
 {
     using P = FftPlan<N>;
     fft_run<N, P::T, SIGN>(v, lds, tw, t, map, typename P::R{});
+}
+
+// The same transform with an explicit plan.
+template <typename Plan, int SIGN>
+__device__ __forceinline__ void line_fft_plan(float2 (&v)[Plan::E], float2* lds, const float2* tw, int t,
+                                              const LdsMap& map)
+{
+    fft_run<Plan::N, Plan::T, SIGN>(v, lds, tw, t, map, typename Plan::R{});
 }
 
 // Geometry of the lines along one axis of an x-fastest (nx, ny, nz) grid.

@@ -16,9 +16,11 @@
 //     accumulation; the n/(QV) factor is applied by the next update.
 // Replicas with equal grids form one group; every launch covers a group.
 #include <cmath>
+#include <algorithm>
 #include <array>
 #include <complex>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <vector>
 
@@ -424,6 +426,9 @@ struct FtsRep {
     double *Hpart = nullptr, *Qpart = nullptr;
     double2* Q = nullptr;
     int nHpart = 0, nQpart = 0;
+    int slot = 0;                 // iteration counter slot
+    bool alive = true;
+    std::vector<void*> allocs;    // device buffers owned by the replica
 };
 
 struct FtsGroup {
@@ -437,20 +442,28 @@ struct FtsGroup {
     std::vector<PropDesc*> d_prop;   // [step*3 + axis]
     std::vector<int> nprop;          // chains per step
     int maxN = 0;
+    bool dirty = false;              // membership changed: rebuild descriptors
+    std::vector<void*> allocs;       // descriptor buffers
 };
 
+// Replicas live in a deque (stable references); replicas of one grid size
+// that are initialised form a group whose launches advance all of them.
+// Replicas can join (after their own first pass) and leave between
+// iterations; a group's descriptors are rebuilt when its membership changes.
 struct FtsBatch {
-    std::vector<FtsRep> reps;
+    std::deque<FtsRep> reps;
     std::vector<FtsGroup> groups;
     hipStream_t stream = nullptr;
-    int64_t* d_iter = nullptr;       // one counter per replica (shared step count)
+    int64_t* d_iter = nullptr;       // iteration counter per slot
+    int nslots = 0;
+    std::vector<int> free_slots;
     int64_t iter = 0;
     bool inited = false;
     std::vector<void*> allocs;
 };
 
 template <typename T>
-static int falloc(FtsBatch* b, T** p, size_t n)
+static int dev_alloc(std::vector<void*>& owner, T** p, size_t n)
 {
     if (n == 0) n = 1;
     hipError_t e = hipMalloc((void**)p, n * sizeof(T));
@@ -458,8 +471,20 @@ static int falloc(FtsBatch* b, T** p, size_t n)
         set_error(MFT_ERR_OOM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
         return MFT_ERR_OOM;
     }
-    b->allocs.push_back((void*)*p);
+    owner.push_back((void*)*p);
     return MFT_OK;
+}
+
+template <typename T>
+static int falloc(FtsBatch* b, T** p, size_t n)
+{
+    return dev_alloc(b->allocs, p, n);
+}
+
+static void free_all(std::vector<void*>& v)
+{
+    for (void* p : v) (void)hipFree(p);
+    v.clear();
 }
 
 static int validate_desc(const mft_fts_desc& d)
@@ -528,15 +553,15 @@ static int rep_setup(FtsBatch* b, FtsRep& r)
     r.gvol = ((double)d.lx / d.nx) * ((double)d.ly / d.ny) * ((double)d.lz / d.nz);
     const size_t M = r.M;
     for (int p = 0; p < d.npotentials; ++p) {
-        if ((rc = falloc(b, &r.wpl[p], M))) return rc;
+        if ((rc = dev_alloc(r.allocs, &r.wpl[p], M))) return rc;
         MFT_HIP(hipMemset(r.wpl[p], 0, sizeof(float2) * M));
         if (d.potentials[p].kind == MFT_FTS_FLORY) {
-            if ((rc = falloc(b, &r.wmi[p], M))) return rc;
+            if ((rc = dev_alloc(r.allocs, &r.wmi[p], M))) return rc;
             MFT_HIP(hipMemset(r.wmi[p], 0, sizeof(float2) * M));
         }
     }
-    if ((rc = falloc(b, &r.W, M * d.nspecies))) return rc;
-    if ((rc = falloc(b, &r.E, M * d.nspecies))) return rc;
+    if ((rc = dev_alloc(r.allocs, &r.W, M * d.nspecies))) return rc;
+    if ((rc = dev_alloc(r.allocs, &r.E, M * d.nspecies))) return rc;
     r.nblk = 0;
     for (int m = 0; m < d.nmolecules; ++m) {
         const mft_fts_molecule& mol = d.molecules[m];
@@ -551,28 +576,28 @@ static int rep_setup(FtsBatch* b, FtsRep& r)
         r.sym[m] = sym;
         // fts_molecule_linear.cu:433-435 with Vfree = V
         r.nmolecs[m] = d.C * mol.phi * r.V * d.Nr / (double)Nt;
-        if ((rc = falloc(b, &r.p[m], M * Nt))) return rc;
+        if ((rc = dev_alloc(r.allocs, &r.p[m], M * Nt))) return rc;
         if (sym) r.pd[m] = r.p[m];
-        else if ((rc = falloc(b, &r.pd[m], M * Nt))) return rc;
+        else if ((rc = dev_alloc(r.allocs, &r.pd[m], M * Nt))) return rc;
         r.nblk += mol.nblocks;
     }
     MFT_REQUIRE(r.nblk <= kFtsMaxMol * kFtsMaxBlocks, MFT_ERR_INVALID, "too many blocks");
-    if ((rc = falloc(b, &r.rhot, M * r.nblk))) return rc;
+    if ((rc = dev_alloc(r.allocs, &r.rhot, M * r.nblk))) return rc;
     MFT_HIP(hipMemset(r.rhot, 0, sizeof(float2) * M * r.nblk));
-    if ((rc = falloc(b, &r.tmp, M * 2 * d.nmolecules))) return rc;
+    if ((rc = dev_alloc(r.allocs, &r.tmp, M * 2 * d.nmolecules))) return rc;
     const int ns[3] = {d.nx, d.ny, d.nz};
     const float Ls[3] = {d.lx, d.ly, d.lz};
     for (int a = 0; a < 3; ++a) {
         std::vector<float2> g;
         axis_table(ns[a], Ls[a], d.Nr, g);
-        if ((rc = falloc(b, &r.g[a], g.size()))) return rc;
+        if ((rc = dev_alloc(r.allocs, &r.g[a], g.size()))) return rc;
         MFT_HIP(hipMemcpy(r.g[a], g.data(), sizeof(float2) * g.size(), hipMemcpyHostToDevice));
     }
     r.nHpart = (r.M + kFtsThreads - 1) / kFtsThreads;
     r.nQpart = r.nHpart;
-    if ((rc = falloc(b, &r.Hpart, (size_t)r.nHpart * kFtsMaxPots * 6))) return rc;
-    if ((rc = falloc(b, &r.Qpart, (size_t)r.nQpart * 2 * d.nmolecules))) return rc;
-    if ((rc = falloc(b, &r.Q, (size_t)d.nmolecules))) return rc;
+    if ((rc = dev_alloc(r.allocs, &r.Hpart, (size_t)r.nHpart * kFtsMaxPots * 6))) return rc;
+    if ((rc = dev_alloc(r.allocs, &r.Qpart, (size_t)r.nQpart * 2 * d.nmolecules))) return rc;
+    if ((rc = dev_alloc(r.allocs, &r.Q, (size_t)d.nmolecules))) return rc;
     return MFT_OK;
 }
 
@@ -639,7 +664,7 @@ static void fill_upd(const FtsBatch* b, const FtsRep& r, FtsUpdDev& U, int first
     U.Hpart = r.Hpart;
     U.seed = d.seed;
     U.stream = 0; // noise keyed by the replica's own seed, never by its slot
-    U.iter = b->d_iter + r.id;
+    U.iter = b->d_iter + r.slot;
     U.cl = d.cl;
     U.first = first;
     U.have_rho = have_rho;
@@ -661,6 +686,10 @@ static int bead_species(const mft_fts_molecule& mol, int Ntot, int s, int c)
 static int group_setup(FtsBatch* b, FtsGroup& G)
 {
     int rc;
+    free_all(G.allocs);
+    G.d_prop.clear();
+    G.nprop.clear();
+    G.dirty = false;
     const int nr = (int)G.reps.size();
     // update descriptors: [0, nr) regular, [nr, 2nr) initial pass
     std::vector<FtsUpdDev> upd(2 * nr);
@@ -668,7 +697,7 @@ static int group_setup(FtsBatch* b, FtsGroup& G)
         fill_upd(b, b->reps[G.reps[k]], upd[k], 0, 1);
         fill_upd(b, b->reps[G.reps[k]], upd[nr + k], 1, 0);
     }
-    if ((rc = falloc(b, &G.d_upd, upd.size()))) return rc;
+    if ((rc = dev_alloc(G.allocs, &G.d_upd, upd.size()))) return rc;
     MFT_HIP(hipMemcpy(G.d_upd, upd.data(), sizeof(FtsUpdDev) * upd.size(), hipMemcpyHostToDevice));
 
     // density and Q descriptors: one per (replica, molecule)
@@ -701,9 +730,9 @@ static int group_setup(FtsBatch* b, FtsGroup& G)
         }
     }
     G.ndens = (int)dens.size();
-    if ((rc = falloc(b, &G.d_dens, dens.size()))) return rc;
+    if ((rc = dev_alloc(G.allocs, &G.d_dens, dens.size()))) return rc;
     MFT_HIP(hipMemcpy(G.d_dens, dens.data(), sizeof(DensDesc) * dens.size(), hipMemcpyHostToDevice));
-    if ((rc = falloc(b, &G.d_q, qd.size()))) return rc;
+    if ((rc = dev_alloc(G.allocs, &G.d_q, qd.size()))) return rc;
     MFT_HIP(hipMemcpy(G.d_q, qd.data(), sizeof(QDesc) * qd.size(), hipMemcpyHostToDevice));
 
     // propagation descriptors per step and axis
@@ -734,7 +763,7 @@ static int group_setup(FtsBatch* b, FtsGroup& G)
         G.nprop[s] = (int)ax[0].size();
         for (int a = 0; a < 3; ++a) {
             PropDesc* dp = nullptr;
-            if ((rc = falloc(b, &dp, ax[a].size()))) return rc;
+            if ((rc = dev_alloc(G.allocs, &dp, ax[a].size()))) return rc;
             MFT_HIP(hipMemcpy(dp, ax[a].data(), sizeof(PropDesc) * ax[a].size(), hipMemcpyHostToDevice));
             G.d_prop[(size_t)3 * s + a] = dp;
         }
@@ -792,49 +821,28 @@ static int group_update(FtsBatch* b, FtsGroup& G, int first)
     return MFT_OK;
 }
 
-static int batch_create(FtsBatch** out, const mft_fts_desc* descs, int nrep)
+static int batch_destroy(FtsBatch* b);
+
+static FtsGroup& group_for(FtsBatch* b, const mft_fts_desc& d)
 {
-    MFT_REQUIRE(out && descs && nrep >= 1, MFT_ERR_INVALID, "bad argument");
-    *out = nullptr;
-    for (int r = 0; r < nrep; ++r) {
-        int rc = validate_desc(descs[r]);
-        if (rc) return rc;
-    }
+    for (auto& G : b->groups)
+        if (G.n[0] == d.nx && G.n[1] == d.ny && G.n[2] == d.nz) return G;
+    FtsGroup G;
+    G.n[0] = d.nx;
+    G.n[1] = d.ny;
+    G.n[2] = d.nz;
+    b->groups.push_back(G);
+    return b->groups.back();
+}
+
+// empty batch with nslots iteration counters
+static int batch_create_empty(FtsBatch** out, int nslots)
+{
     FtsBatch* b = new FtsBatch();
-    int rc = MFT_OK;
-    b->reps.resize(nrep);
-    for (int r = 0; r < nrep && !rc; ++r) {
-        b->reps[r].d = descs[r];
-        b->reps[r].id = r;
-    }
-    if (!rc) rc = falloc(b, &b->d_iter, (size_t)nrep);
+    int rc = falloc(b, &b->d_iter, (size_t)nslots);
     if (!rc) {
-        hipError_t e = hipMemset(b->d_iter, 0, sizeof(int64_t) * nrep);
+        hipError_t e = hipMemset(b->d_iter, 0, sizeof(int64_t) * nslots);
         if (e != hipSuccess) rc = MFT_ERR_DEVICE;
-    }
-    for (int r = 0; r < nrep && !rc; ++r) rc = rep_setup(b, b->reps[r]);
-    if (!rc) {
-        std::map<std::array<int, 3>, int> idx;
-        for (int r = 0; r < nrep; ++r) {
-            std::array<int, 3> key{b->reps[r].d.nx, b->reps[r].d.ny, b->reps[r].d.nz};
-            auto it = idx.find(key);
-            if (it == idx.end()) {
-                idx[key] = (int)b->groups.size();
-                FtsGroup G;
-                G.n[0] = key[0];
-                G.n[1] = key[1];
-                G.n[2] = key[2];
-                b->groups.push_back(G);
-                it = idx.find(key);
-            }
-            b->groups[it->second].reps.push_back(r);
-        }
-        for (auto& G : b->groups) {
-            for (int a = 0; a < 3; ++a)
-                if (!fft_twiddles_device(G.n[a])) rc = MFT_ERR_OOM;
-            if (!rc) rc = group_setup(b, G);
-            if (rc) break;
-        }
     }
     if (!rc) {
         hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
@@ -844,8 +852,118 @@ static int batch_create(FtsBatch** out, const mft_fts_desc* descs, int nrep)
         }
     }
     if (rc) {
-        for (void* p : b->allocs) (void)hipFree(p);
+        free_all(b->allocs);
         delete b;
+        return rc;
+    }
+    b->nslots = nslots;
+    for (int k = nslots - 1; k >= 0; --k) b->free_slots.push_back(k);
+    *out = b;
+    return MFT_OK;
+}
+
+// a new replica: buffers allocated, fields zero, counter at 0; not yet in a
+// group (batch_activate runs its first pass and adds it)
+static int batch_add(FtsBatch* b, const mft_fts_desc& d, int* rep_out)
+{
+    int rc = validate_desc(d);
+    if (rc) return rc;
+    MFT_REQUIRE(!b->free_slots.empty(), MFT_ERR_INVALID, "no free replica slot (%d in use)", b->nslots);
+    for (int a = 0; a < 3; ++a)
+        MFT_REQUIRE(fft_twiddles_device((&d.nx)[a]) != nullptr, MFT_ERR_OOM, "twiddle table");
+    b->reps.emplace_back();
+    FtsRep& r = b->reps.back();
+    r.d = d;
+    r.id = (int)b->reps.size() - 1;
+    r.slot = b->free_slots.back();
+    if ((rc = rep_setup(b, r))) {
+        free_all(r.allocs);
+        r.alive = false;
+        return rc;
+    }
+    b->free_slots.pop_back();
+    MFT_HIP(hipMemsetAsync(b->d_iter + r.slot, 0, sizeof(int64_t), b->stream));
+    *rep_out = r.id;
+    return MFT_OK;
+}
+
+// first pass (FTS_Box::finishInitialization density pass) of the given new
+// replicas, grouped by grid size, then each joins its grid's group
+static int batch_activate(FtsBatch* b, const std::vector<int>& reps)
+{
+    int rc = MFT_OK;
+    std::map<std::array<int, 3>, FtsGroup> tmp;
+    for (int id : reps) {
+        const mft_fts_desc& d = b->reps[id].d;
+        FtsGroup& T = tmp[std::array<int, 3>{d.nx, d.ny, d.nz}];
+        T.n[0] = d.nx;
+        T.n[1] = d.ny;
+        T.n[2] = d.nz;
+        T.reps.push_back(id);
+    }
+    for (auto& kv : tmp) {
+        FtsGroup& T = kv.second;
+        if (!rc) rc = group_setup(b, T);
+        if (!rc) rc = group_update(b, T, 1);
+        if (!rc) rc = group_density(b, T);
+    }
+    // the temporary descriptors are read by the launches just queued
+    hipError_t e = hipStreamSynchronize(b->stream);
+    for (auto& kv : tmp) free_all(kv.second.allocs);
+    if (rc) return rc;
+    if (e != hipSuccess) {
+        set_error(MFT_ERR_DEVICE, "first pass: %s", hipGetErrorString(e));
+        return MFT_ERR_DEVICE;
+    }
+    for (int id : reps) {
+        FtsGroup& G = group_for(b, b->reps[id].d);
+        G.reps.push_back(id);
+        G.dirty = true;
+    }
+    return MFT_OK;
+}
+
+// a replica leaves: out of its group, buffers freed, slot released (the
+// caller has synchronised the stream)
+static int batch_retire(FtsBatch* b, int id)
+{
+    MFT_REQUIRE(id >= 0 && id < (int)b->reps.size() && b->reps[id].alive, MFT_ERR_INVALID, "no such replica");
+    FtsRep& r = b->reps[id];
+    for (auto& G : b->groups) {
+        auto it = std::find(G.reps.begin(), G.reps.end(), id);
+        if (it != G.reps.end()) {
+            G.reps.erase(it);
+            G.dirty = true;
+        }
+    }
+    free_all(r.allocs);
+    r.alive = false;
+    b->free_slots.push_back(r.slot);
+    return MFT_OK;
+}
+
+static int batch_create(FtsBatch** out, const mft_fts_desc* descs, int nrep)
+{
+    MFT_REQUIRE(out && descs && nrep >= 1, MFT_ERR_INVALID, "bad argument");
+    *out = nullptr;
+    for (int r = 0; r < nrep; ++r) {
+        int rc = validate_desc(descs[r]);
+        if (rc) return rc;
+    }
+    FtsBatch* b = nullptr;
+    int rc = batch_create_empty(&b, nrep);
+    if (rc) return rc;
+    for (int r = 0; r < nrep && !rc; ++r) {
+        int id = -1;
+        rc = batch_add(b, descs[r], &id);
+        if (!rc) {
+            FtsGroup& G = group_for(b, descs[r]);
+            G.reps.push_back(id);
+            G.dirty = true;
+        }
+    }
+    if (rc) {
+        batch_destroy(b);
         return rc;
     }
     *out = b;
@@ -856,9 +974,17 @@ static int batch_destroy(FtsBatch* b)
 {
     if (!b) return MFT_OK;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
-    for (void* p : b->allocs) (void)hipFree(p);
+    for (auto& r : b->reps) free_all(r.allocs);
+    for (auto& G : b->groups) free_all(G.allocs);
+    free_all(b->allocs);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
+    return MFT_OK;
+}
+
+static int group_ready(FtsBatch* b, FtsGroup& G)
+{
+    if (G.dirty || G.allocs.empty()) return group_setup(b, G);
     return MFT_OK;
 }
 
@@ -866,6 +992,8 @@ static int batch_init(FtsBatch* b)
 {
     int rc;
     for (auto& G : b->groups) {
+        if (G.reps.empty()) continue;
+        if ((rc = group_ready(b, G))) return rc;
         if ((rc = group_update(b, G, 1))) return rc;
         if ((rc = group_density(b, G))) return rc;
     }
@@ -877,18 +1005,18 @@ static int batch_step(FtsBatch* b, int niter)
 {
     MFT_REQUIRE(b->inited, MFT_ERR_INVALID, "call init before stepping");
     int rc;
+    for (auto& G : b->groups)
+        if (!G.reps.empty() && (rc = group_ready(b, G))) return rc;
     for (int it = 0; it < niter; ++it) {
         for (auto& G : b->groups) {
+            if (G.reps.empty()) continue;
             if ((rc = group_update(b, G, 0))) return rc;
             if ((rc = group_density(b, G))) return rc;
         }
-        {
-            // one counter per replica
-            int n = (int)b->reps.size();
-            for (int o = 0; o < n; o += 256) {
-                int cnt = std::min(256, n - o);
-                k_fts_iter_incr<<<1, 256, 0, b->stream>>>(b->d_iter + o, cnt);
-            }
+        // every slot's counter advances; a slot is reset when a replica takes it
+        for (int o = 0; o < b->nslots; o += 256) {
+            int cnt = std::min(256, b->nslots - o);
+            k_fts_iter_incr<<<1, 256, 0, b->stream>>>(b->d_iter + o, cnt);
         }
         MFT_LAUNCH_CHECK();
         b->iter++;
@@ -898,7 +1026,8 @@ static int batch_step(FtsBatch* b, int niter)
 
 static int batch_set_fields(FtsBatch* b, int rep, int pot, const float* wpl, const float* wmi)
 {
-    MFT_REQUIRE(rep >= 0 && rep < (int)b->reps.size(), MFT_ERR_INVALID, "replica out of range");
+    MFT_REQUIRE(rep >= 0 && rep < (int)b->reps.size() && b->reps[rep].alive, MFT_ERR_INVALID,
+                "replica out of range");
     FtsRep& r = b->reps[rep];
     MFT_REQUIRE(pot >= 0 && pot < r.d.npotentials, MFT_ERR_INVALID, "potential out of range");
     MFT_HIP(hipStreamSynchronize(b->stream));
@@ -920,7 +1049,8 @@ static int batch_set_fields(FtsBatch* b, int rep, int pot, const float* wpl, con
 
 static int batch_observables(FtsBatch* b, int rep, double* Q, double* H, double* parts)
 {
-    MFT_REQUIRE(rep >= 0 && rep < (int)b->reps.size(), MFT_ERR_INVALID, "replica out of range");
+    MFT_REQUIRE(rep >= 0 && rep < (int)b->reps.size() && b->reps[rep].alive, MFT_ERR_INVALID,
+                "replica out of range");
     FtsRep& r = b->reps[rep];
     const mft_fts_desc& d = r.d;
     MFT_HIP(hipStreamSynchronize(b->stream));
@@ -988,11 +1118,14 @@ static int group_of(FtsBatch* b, int rep)
 
 static int batch_densities(FtsBatch* b, int rep, float* rho)
 {
-    MFT_REQUIRE(rep >= 0 && rep < (int)b->reps.size() && rho, MFT_ERR_INVALID, "bad argument");
+    MFT_REQUIRE(rep >= 0 && rep < (int)b->reps.size() && rho && b->reps[rep].alive, MFT_ERR_INVALID, "bad argument");
     FtsRep& r = b->reps[rep];
     int gk = group_of(b, rep);
+    MFT_REQUIRE(gk >= 0, MFT_ERR_INVALID, "replica %d is not active", rep);
     FtsGroup& G = b->groups[gk / 4096];
     int k = gk % 4096;
+    int rc = group_ready(b, G);
+    if (rc) return rc;
     float2* out = nullptr;
     MFT_HIP(hipMalloc((void**)&out, sizeof(float2) * r.M * r.d.nspecies));
     dim3 grid((r.M + kFtsThreads - 1) / kFtsThreads, 1);
@@ -1160,3 +1293,56 @@ int mft_fts_batch_get_densities(mft_fts_batch* b, int rep, float* rho)
 }
 
 } // extern "C"
+
+// ------------------------------------------------- pool for the server ---
+// The serving loop keeps one batch whose membership changes between
+// iterations (serve.hip); these wrap the batch operations for it.
+namespace mft {
+
+int fts_pool_create(FtsBatch** b, int nslots)
+{
+    int rc = batch_create_empty(b, nslots);
+    if (!rc) (*b)->inited = true;
+    return rc;
+}
+int fts_pool_destroy(FtsBatch* b) { return batch_destroy(b); }
+int fts_pool_add(FtsBatch* b, const mft_fts_desc& d, int* rep) { return batch_add(b, d, rep); }
+int fts_pool_set_fields(FtsBatch* b, int rep, int pot, const float* wpl, const float* wmi)
+{
+    int rc = batch_set_fields(b, rep, pot, wpl, wmi);
+    b->inited = true; // the pool's replicas are initialised one by one
+    return rc;
+}
+int fts_pool_activate(FtsBatch* b, const std::vector<int>& reps) { return batch_activate(b, reps); }
+int fts_pool_step(FtsBatch* b, int n) { return batch_step(b, n); }
+hipStream_t fts_pool_stream(FtsBatch* b) { return b->stream; }
+int fts_pool_observables(FtsBatch* b, int rep, double* Q, double* H, double* parts)
+{
+    return batch_observables(b, rep, Q, H, parts);
+}
+int fts_pool_densities(FtsBatch* b, int rep, float* rho) { return batch_densities(b, rep, rho); }
+int fts_pool_retire(FtsBatch* b, int rep) { return batch_retire(b, rep); }
+
+// device bytes one replica holds (rep_setup)
+size_t fts_bytes_estimate(const mft_fts_desc& d)
+{
+    const size_t M = (size_t)d.nx * d.ny * d.nz;
+    size_t fields = 0;
+    for (int p = 0; p < d.npotentials; ++p) fields += d.potentials[p].kind == MFT_FTS_FLORY ? 2 : 1;
+    size_t prop = 0, nblk = 0;
+    for (int m = 0; m < d.nmolecules; ++m) {
+        const mft_fts_molecule& mol = d.molecules[m];
+        int Nt = 0;
+        for (int k = 0; k < mol.nblocks; ++k) Nt += mol.N[k];
+        bool sym = true;
+        for (int j = 0; j < mol.nblocks / 2; ++j)
+            if (mol.species[j] != mol.species[mol.nblocks - 1 - j] || mol.N[j] != mol.N[mol.nblocks - 1 - j])
+                sym = false;
+        prop += (size_t)Nt * (sym ? 1 : 2);
+        nblk += mol.nblocks;
+    }
+    const size_t cplx = fields + 2 * (size_t)d.nspecies + prop + nblk + 2 * (size_t)d.nmolecules;
+    return cplx * M * sizeof(float2) + (M / kFtsThreads + 1) * (kFtsMaxPots * 6 + 8) * sizeof(double) + 4096;
+}
+
+} // namespace mft

@@ -132,27 +132,40 @@ static int tild_setup(TildEngine* e)
     }
 
     const int N = e->N, M = e->M;
-    if ((rc = dalloc(&e->x[0], 3 * (size_t)N))) return rc;
-    if ((rc = dalloc(&e->x[1], 3 * (size_t)N))) return rc;
-    if ((rc = dalloc(&e->noise_old, 3 * (size_t)N))) return rc;
+    const size_t mb = d.maxbonds > 0 ? d.maxbonds : 1;
+    for (int b = 0; b < 2; ++b) {
+        if ((rc = dalloc(&e->X[b], (size_t)N))) return rc;
+        if ((rc = dalloc(&e->XO[b], (size_t)N))) return rc;
+        if ((rc = dalloc(&e->NZ[b], (size_t)N))) return rc;
+        if ((rc = dalloc(&e->META[b], (size_t)N))) return rc;
+        if ((rc = dalloc(&e->NB[b], (size_t)N))) return rc;
+        if ((rc = dalloc(&e->BS[b], (size_t)N * mb))) return rc;
+        if ((rc = dalloc(&e->BT[b], (size_t)N * mb))) return rc;
+    }
+    if ((rc = dalloc(&e->XN, (size_t)N))) return rc;
+    if ((rc = dalloc(&e->NN, (size_t)N))) return rc;
     if ((rc = dalloc(&e->fgrid, (size_t)N))) return rc;
-    if ((rc = dalloc(&e->ftot, 3 * (size_t)N))) return rc;
+    if ((rc = dalloc(&e->ftot, (size_t)N))) return rc;
+    if ((rc = dalloc(&e->key, (size_t)N))) return rc;
+    if ((rc = dalloc(&e->rank, (size_t)N))) return rc;
+    if ((rc = dalloc(&e->map, (size_t)N))) return rc;
+    if ((rc = dalloc(&e->slot_of, (size_t)N))) return rc;
     if ((rc = dalloc(&e->type, (size_t)N))) return rc;
     if (d.has_charges && (rc = dalloc(&e->charge, (size_t)N))) return rc;
     if ((rc = dalloc(&e->nBonds, (size_t)N))) return rc;
-    if ((rc = dalloc(&e->bondedTo, (size_t)N * (d.maxbonds > 0 ? d.maxbonds : 1)))) return rc;
-    if ((rc = dalloc(&e->bondType, (size_t)N * (d.maxbonds > 0 ? d.maxbonds : 1)))) return rc;
-    if ((rc = dalloc(&e->cell, (size_t)N))) return rc;
-    if ((rc = dalloc(&e->rank, (size_t)N))) return rc;
-    if ((rc = dalloc(&e->spos, (size_t)N))) return rc;
-    if ((rc = dalloc(&e->smeta, (size_t)N))) return rc;
+    if ((rc = dalloc(&e->bondedTo, (size_t)N * mb))) return rc;
+    if ((rc = dalloc(&e->bondType, (size_t)N * mb))) return rc;
+    if ((rc = dalloc(&e->pio, 3 * (size_t)N))) return rc;
+    if ((rc = dalloc(&e->pio2, 3 * (size_t)N))) return rc;
+    if ((rc = dalloc(&e->pio3, 3 * (size_t)N))) return rc;
     if ((rc = dalloc(&e->force_slot, (size_t)d.nspecies))) return rc;
     const int nconv_blocks = tild_conv_blocks(d.nx, d.ny, d.nz);
     if ((rc = dalloc(&e->epart, (size_t)nconv_blocks * kMaxPots))) return rc;
     if ((rc = dalloc(&e->eout, (size_t)kMaxPots + 1))) return rc;
     if ((rc = dalloc(&e->bpart, (size_t)nblk(N)))) return rc;
-    MFT_HIP(hipMemset(e->noise_old, 0, sizeof(float) * 3 * (size_t)N));
     MFT_HIP(hipMemset(e->nBonds, 0, sizeof(int) * (size_t)N));
+    MFT_HIP(hipMemset(e->bondedTo, 0, sizeof(int) * (size_t)N * mb));
+    MFT_HIP(hipMemset(e->bondType, 0, sizeof(int) * (size_t)N * mb));
     MFT_HIP(hipMemset(e->type, 0, sizeof(int) * (size_t)N));
     MFT_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     return MFT_OK;
@@ -268,30 +281,11 @@ static void resolve_timing(TildEngine* e)
     e->evused = 0;
 }
 
-static int bin_current(TildEngine* e)
-{
-    TimedScope ts(e, MFT_TK_BIN);
-    int rc = tild_bin_count(e);
-    if (rc) return rc;
-    e->binned = true;
-    return MFT_OK;
-}
-
 // Everything up to the per-particle grid force in fgrid.
 static int grid_forces(TildEngine* e)
 {
     const mft_tild_desc& d = e->d;
     int rc;
-    if (!e->binned && (rc = bin_current(e))) return rc;
-    {
-        TimedScope ts(e, MFT_TK_SCAN);
-        if ((rc = tild_scan(e))) return rc;
-    }
-    {
-        TimedScope ts(e, MFT_TK_SCATTER);
-        if ((rc = tild_scatter(e))) return rc;
-    }
-    e->binned = false;
     {
         TimedScope ts(e, MFT_TK_DENSITY);
         if ((rc = tild_density(e))) return rc;
@@ -401,13 +395,19 @@ static int do_steps(TildEngine* e, int nsteps, bool want_f, bool advance = true)
     for (int i = 0; i < nsteps; ++i) {
         if (e->timing && 2 * (e->evused + MFT_TK_COUNT) > e->evpool.size()) resolve_timing(e);
         if ((rc = grid_forces(e))) return rc;
-        TimedScope ts(e, MFT_TK_INTEGRATE);
-        int nxt = 1 - e->cur;
         const bool last = i == nsteps - 1;
-        if ((rc = tild_integrate(e, nxt, want_f && last))) return rc;
+        {
+            TimedScope ts(e, MFT_TK_INTEGRATE);
+            if ((rc = tild_integrate(e, want_f && last))) return rc;
+        }
         if (d.integrator != MFT_INT_NONE) {
-            e->cur = nxt;
-            e->binned = true;
+            // the next positions into their key order
+            {
+                TimedScope ts(e, MFT_TK_SCAN);
+                if ((rc = tild_scan(e))) return rc;
+            }
+            TimedScope ts(e, MFT_TK_SCATTER);
+            if ((rc = tild_permute(e))) return rc;
         }
         if (!advance) continue;
         if (e->run_max > 0) update_ramps(e, e->run_pos++, e->run_max);
@@ -453,9 +453,11 @@ int mft_tild_create(mft_tild** out, const mft_tild_desc* desc)
 int mft_tild_destroy(mft_tild* e)
 {
     if (!e) return MFT_OK;
-    void* ptrs[] = {e->x[0], e->x[1], e->noise_old, e->fgrid, e->ftot, e->charge, e->type, e->nBonds, e->bondedTo,
-                    e->bondType, e->count, e->cell, e->rank, e->start, e->partial, e->spos, e->smeta, e->rho, e->S,
-                    e->Y, e->F4, e->force_slot, e->epart, e->eout, e->bpart, e->tiles, e->tile_tab};
+    void* ptrs[] = {e->X[0], e->X[1], e->XO[0], e->XO[1], e->NZ[0], e->NZ[1], e->META[0], e->META[1], e->NB[0],
+                    e->NB[1], e->BS[0], e->BS[1], e->BT[0], e->BT[1], e->XN, e->NN, e->fgrid, e->ftot, e->key,
+                    e->rank, e->map, e->slot_of, e->type, e->charge, e->nBonds, e->bondedTo, e->bondType, e->pio,
+                    e->pio2, e->pio3, e->count, e->start, e->partial, e->rho, e->S, e->Y, e->F4, e->force_slot,
+                    e->epart, e->eout, e->bpart, e->tiles, e->tile_tab};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (float* t : e->tables)
@@ -500,6 +502,8 @@ int mft_tild_set_topology(mft_tild* e, const int* type, const float* charge, con
         MFT_HIP(hipMemcpy(e->bondedTo, bondedTo, sizeof(int) * N * mb, hipMemcpyHostToDevice));
         MFT_HIP(hipMemcpy(e->bondType, bondType, sizeof(int) * N * mb, hipMemcpyHostToDevice));
     }
+    // the key-ordered state is built from the topology: set the state again
+    e->have_state = 0;
     return MFT_OK;
 }
 
@@ -514,27 +518,29 @@ int mft_tild_set_state(mft_tild* e, const float* x, const float* xo, const float
                     "position %zu is not finite or far outside the box: %g", i, x[i]);
     MFT_HIP(hipStreamSynchronize(e->stream));
     e->cur = 0;
-    MFT_HIP(hipMemcpy(e->x[0], x, sizeof(float) * n3, hipMemcpyHostToDevice));
-    MFT_HIP(hipMemcpy(e->x[1], xo ? xo : x, sizeof(float) * n3, hipMemcpyHostToDevice));
-    if (noise_old) MFT_HIP(hipMemcpy(e->noise_old, noise_old, sizeof(float) * n3, hipMemcpyHostToDevice));
-    else MFT_HIP(hipMemset(e->noise_old, 0, sizeof(float) * n3));
-    if (e->binned) {
-        // counts of stale positions would be double-counted: clear them
-        MFT_HIP(hipMemset(e->count, 0, sizeof(int) * (size_t)e->nkeys));
-        e->binned = false;
-    }
+    MFT_HIP(hipMemcpy(e->pio, x, sizeof(float) * n3, hipMemcpyHostToDevice));
+    MFT_HIP(hipMemcpy(e->pio2, xo ? xo : x, sizeof(float) * n3, hipMemcpyHostToDevice));
+    if (noise_old) MFT_HIP(hipMemcpy(e->pio3, noise_old, sizeof(float) * n3, hipMemcpyHostToDevice));
+    else MFT_HIP(hipMemset(e->pio3, 0, sizeof(float) * n3));
+    int rc = tild_sort_state(e);
+    if (rc) return rc;
+    MFT_HIP(hipStreamSynchronize(e->stream));
     e->have_state = 1;
     return MFT_OK;
 }
 
 int mft_tild_get_state(mft_tild* e, float* x, float* xo, float* noise_old)
 {
-    MFT_REQUIRE(e, MFT_ERR_INVALID, "null argument");
-    MFT_HIP(hipStreamSynchronize(e->stream));
+    MFT_REQUIRE(e && e->have_state, MFT_ERR_INVALID, "engine has no state");
     const size_t n3 = 3 * (size_t)e->N;
-    if (x) MFT_HIP(hipMemcpy(x, e->x[e->cur], sizeof(float) * n3, hipMemcpyDeviceToHost));
-    if (xo) MFT_HIP(hipMemcpy(xo, e->x[1 - e->cur], sizeof(float) * n3, hipMemcpyDeviceToHost));
-    if (noise_old) MFT_HIP(hipMemcpy(noise_old, e->noise_old, sizeof(float) * n3, hipMemcpyDeviceToHost));
+    float* outs[3] = {x, xo, noise_old};
+    for (int w = 0; w < 3; ++w) {
+        if (!outs[w]) continue;
+        int rc = tild_unsort(e, w);
+        if (rc) return rc;
+        MFT_HIP(hipStreamSynchronize(e->stream));
+        MFT_HIP(hipMemcpy(outs[w], e->pio, sizeof(float) * n3, hipMemcpyDeviceToHost));
+    }
     return MFT_OK;
 }
 
@@ -549,8 +555,11 @@ int mft_tild_compute_forces(mft_tild* e, float* f)
     e->d.integrator = saved;
     e->step = st;
     if (rc) return rc;
-    MFT_HIP(hipStreamSynchronize(e->stream));
-    if (f) MFT_HIP(hipMemcpy(f, e->ftot, sizeof(float) * 3 * (size_t)e->N, hipMemcpyDeviceToHost));
+    if (f) {
+        if ((rc = tild_unsort(e, 3))) return rc;
+        MFT_HIP(hipStreamSynchronize(e->stream));
+        MFT_HIP(hipMemcpy(f, e->pio, sizeof(float) * 3 * (size_t)e->N, hipMemcpyDeviceToHost));
+    }
     return MFT_OK;
 }
 
@@ -679,17 +688,21 @@ int mft_tild_kernel_bytes(mft_tild* e, double* bytes)
     const bool q = e->d.has_charges;
     for (int k = 0; k < MFT_TK_COUNT; ++k) bytes[k] = 0;
     // minimal HBM traffic of each kernel's job (reads + writes once)
-    bytes[MFT_TK_BIN] = N * (12 + 4 + 4 + 4);                 // x, cell, rank, count
+    const double mb = e->d.maxbonds > 0 ? e->d.maxbonds : 1;
+    bytes[MFT_TK_BIN] = 0;                                    // sorting happens when the state is set
     bytes[MFT_TK_SCAN] = (double)e->nkeys * (4 + 4 + 4);      // count read, start write, count clear
-    bytes[MFT_TK_SCATTER] = N * (12 + (q ? 4 : 0) + 4 + 4 + 4 + 4 + 16 + 8);
-    bytes[MFT_TK_DENSITY] = N * 24 + M * 4 + M * 4 * C;       // sorted records, starts, rho
+    // permutation: key, rank, start, next x and noise, x, id/species, bonds
+    // in; x, previous x, noise, id/species, bonds, map out; bond slots remapped
+    bytes[MFT_TK_SCATTER] = N * ((4 + 4 + 4 + 16 + 16 + 16 + 8 + 4 + 4 * mb) + (16 + 16 + 16 + 8 + 4 + 4 * mb + 4) +
+                                 (4 + 4 + 8 * mb + 4 * mb));
+    bytes[MFT_TK_DENSITY] = N * (16 + 8) + M * 4 * C;         // key-ordered records, rho
     bytes[MFT_TK_FFT_X_FWD] = M * C * (4 + 8);
     bytes[MFT_TK_FFT_Y_FWD] = M * C * 16;
     bytes[MFT_TK_CONV_Z] = M * 8 * (C + O);
     bytes[MFT_TK_FFT_Y_INV] = M * O * 16;
     bytes[MFT_TK_FFT_X_INV] = M * O * 8 + M * O4 * 16;
-    bytes[MFT_TK_GATHER] = N * (24 + 16) + M * 16 * O4;
-    bytes[MFT_TK_INTEGRATE] = N * (12 + 12 + 16 + 24 + 12 + 4 + 4 + 8 + 8 + 12) + N * 12;
+    bytes[MFT_TK_GATHER] = N * (16 + 8 + 16) + M * 16 * O4;
+    bytes[MFT_TK_INTEGRATE] = N * ((16 + 16 + 16 + 8 + 4 + 8 * mb + 16) + (16 + 16 + 4 + 4));
     return MFT_OK;
 }
 

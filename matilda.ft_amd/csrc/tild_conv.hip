@@ -21,7 +21,7 @@ __global__ void __launch_bounds__(256) k_conv_z(const float2* __restrict__ S, fl
                                                 float Lx, float Ly, float Lz, KParams kp, const float2* __restrict__ twg,
                                                 float invM, double* __restrict__ epart, float Mgvol)
 {
-    using P = FftPlan<N>;
+    using P = FftPlanLow<N>;
     constexpr int T = P::T, E = P::E, B = 256 / T;
     __shared__ float2 lds[B * N];
     __shared__ float2 tw[N];
@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(256) k_conv_z(const float2* __restrict__ S, fl
 #pragma unroll
         for (int s = 0; s < E; ++s)
             spec[c][s] = valid ? S[c * M + line + (long)(t + T * s) * st] : make_float2(0.f, 0.f);
-        line_fft<N, -1>(spec[c], lds, tw, t, map);
+        line_fft_plan<P, -1>(spec[c], lds, tw, t, map);
 #pragma unroll
         for (int s = 0; s < E; ++s) spec[c][s] = cscale(spec[c][s], invM);
     }
@@ -167,7 +167,7 @@ __global__ void __launch_bounds__(256) k_conv_z(const float2* __restrict__ S, fl
 #pragma unroll
             for (int s = 0; s < E; ++s) v[s] = make_float2(pend[s].x - v[s].y, pend[s].y + v[s].x);
             pending = false;
-            line_fft<N, +1>(v, lds, tw, t, map);
+            line_fft_plan<P, +1>(v, lds, tw, t, map);
             if (valid) {
 #pragma unroll
                 for (int s = 0; s < E; ++s) Y[out * M + line + (long)(t + T * s) * st] = v[s];
@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(256) k_conv_z(const float2* __restrict__ S, fl
         }
     }
     if (pending) {
-        line_fft<N, +1>(pend, lds, tw, t, map);
+        line_fft_plan<P, +1>(pend, lds, tw, t, map);
         if (valid) {
 #pragma unroll
             for (int s = 0; s < E; ++s) Y[out * M + line + (long)(t + T * s) * st] = pend[s];
@@ -248,7 +248,7 @@ __global__ void k_energy_sum(double* __restrict__ out, const double* __restrict_
 int tild_conv_blocks(int nx, int ny, int nz)
 {
     int T = 1;
-#define MFT_T(n_) if (nz == n_) T = FftPlan<n_>::T;
+#define MFT_T(n_) if (nz == n_) T = FftPlanLow<n_>::T;
     MFT_FOR_EACH_FFT_SIZE(MFT_T)
 #undef MFT_T
     int B = 256 / T;
@@ -262,12 +262,15 @@ int tild_conv(TildEngine* e)
     switch (d.nz) {
 #define MFT_CASE(n_)                                                                                      \
     case n_: {                                                                                            \
-        constexpr int B = 256 / FftPlan<n_>::T;                                                           \
+        constexpr int B = 256 / FftPlanLow<n_>::T;                                                        \
         unsigned grid = (d.nx * d.ny + B - 1) / B;                                                        \
         float Mg = (float)((double)e->gvol * e->M);                                                       \
         const float2* tw = fft_twiddles_device(n_);                                                      \
         if (e->nchan <= 2)                                                                                \
             k_conv_z<n_, 2><<<grid, 256, 0, e->stream>>>(e->S, e->Y, d.nx, d.ny, d.lx, d.ly, d.lz, e->kp,  \
+                                                         tw, invM, e->epart, Mg);                         \
+        else if (e->nchan == 3)                                                                           \
+            k_conv_z<n_, 3><<<grid, 256, 0, e->stream>>>(e->S, e->Y, d.nx, d.ny, d.lx, d.ly, d.lz, e->kp,  \
                                                          tw, invM, e->epart, Mg);                         \
         else if (e->nchan <= 4)                                                                           \
             k_conv_z<n_, 4><<<grid, 256, 0, e->stream>>>(e->S, e->Y, d.nx, d.ny, d.lx, d.ly, d.lz, e->kp,  \

@@ -1,8 +1,7 @@
 // Shared state and launchers of the TILD particle-mesh engine.
 //
 // One step (PS_Box::doTimeStep, PS_Box.cu:73-207) runs as
-//   bin (counting sort by (brick, species, sub-cell) key, fused into the
-//        previous step's integrator) -> scan -> scatter into key order
+//   particles live in key order ((brick, species, sub-cell) counting sort);
 //   -> tile density: one workgroup per brick of grid points accumulates the
 //      (B+p)^3 stencil footprint of the brick's own particles in LDS (integer
 //      fixed point, order independent), writes the tile, and a reduction sums
@@ -13,8 +12,10 @@
 //   -> inverse FFT y, inverse FFT x keeping the real part (float4 fields)
 //   -> gather per (brick, species): the species' (B+p)^3 force window staged
 //      in LDS, each particle sums its (p+1)^3 stencil from it
-//   -> bonds + GJF/EM per particle, writing the next positions and counting
-//      the next step's keys.
+//   -> bonds + GJF/EM per particle in key order, counting the next positions'
+//      keys (wave-aggregated atomics) -> scan -> permutation of the particle
+//      state into the next key order (nearly the identity step to step) and
+//      remapping of the bond partner slots.
 // The reference instead materialises (p+1)^3 index/weight tables per particle,
 // scatters with FP32 atomics per group, and runs 4 or 8 FFTs per potential.
 #pragma once
@@ -134,20 +135,33 @@ struct TildEngine {
     float dx[3] = {0, 0, 0};
     float gvol = 0;
     hipStream_t stream = nullptr;
-    // particle state
-    float* x[2] = {nullptr, nullptr};
+    // particle state in key order (brick, species, sub-cell), double-buffered
+    // for the per-step permutation into the next step's order
+    float4* X[2] = {nullptr, nullptr};   // x, y, z, charge
+    float4* XO[2] = {nullptr, nullptr};  // previous positions (GJF)
+    float4* NZ[2] = {nullptr, nullptr};  // previous noise (GJF)
+    int2* META[2] = {nullptr, nullptr};  // particle id, species
+    int* NB[2] = {nullptr, nullptr};     // bond count
+    int* BS[2] = {nullptr, nullptr};     // bond partners as slots [N][maxbonds]
+    int* BT[2] = {nullptr, nullptr};     // bond types [N][maxbonds]
     int cur = 0;
-    float *noise_old = nullptr, *ftot = nullptr, *charge = nullptr;
-    float4* fgrid = nullptr; // per-particle grid force (x, y, z, 0)
+    // step temporaries in slot order
+    float4 *XN = nullptr, *NN = nullptr; // next positions and noise
+    float4* fgrid = nullptr;             // grid force per slot (x, y, z, 0)
+    float4* ftot = nullptr;              // total force per slot (force evaluations)
+    int *key = nullptr, *rank = nullptr, *map = nullptr;
+    // topology and state staging in particle-id order
     int *type = nullptr, *nBonds = nullptr, *bondedTo = nullptr, *bondType = nullptr;
+    float* charge = nullptr;
+    float* pio = nullptr;                // [3N] host-order positions / forces
+    float* pio2 = nullptr;               // [3N]
+    float* pio3 = nullptr;               // [3N]
+    int* slot_of = nullptr;              // particle id -> slot
     // binning
     TileGeom tg;
     int nkeys = 0;
-    int *count = nullptr, *cell = nullptr, *rank = nullptr, *start = nullptr, *partial = nullptr;
+    int *count = nullptr, *start = nullptr, *partial = nullptr;
     int nscan_blocks = 0;
-    float4* spos = nullptr;
-    int2* smeta = nullptr;
-    bool binned = false; // count/cell/rank hold the current positions
     // grids
     float* rho = nullptr;   // [nchan][M]
     float* tiles = nullptr; // [nchan][halo-expanded grid] density tiles
@@ -207,12 +221,13 @@ int tild_conv(TildEngine* e);
 
 // ---- particle passes (tild_particles.hip) ----
 int tild_tiles_configure(TildEngine* e); // brick geometry, LDS sizes, tile tables
-int tild_bin_count(TildEngine* e);
+int tild_sort_state(TildEngine* e);     // particle-id-order staging -> key order
+int tild_unsort(TildEngine* e, int what); // key order -> particle-id-order staging
 int tild_scan(TildEngine* e);
-int tild_scatter(TildEngine* e);
+int tild_permute(TildEngine* e);         // next positions into the next key order
 int tild_density(TildEngine* e);
 int tild_gather(TildEngine* e, int mode);
-int tild_integrate(TildEngine* e, int nxt, bool want_f);
+int tild_integrate(TildEngine* e, bool want_f);
 int tild_bond_energy(TildEngine* e);
 
 } // namespace mft

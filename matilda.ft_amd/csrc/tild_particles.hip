@@ -140,44 +140,6 @@ __global__ void k_bonds(float* __restrict__ f, const int* __restrict__ nBonds, c
     f[i * 3 + 2] += fo[2];
 }
 
-// d_bondStressEnergy energy part (ps_deviceBonds.cu:85-171), FP64 block sums.
-__global__ void k_bond_energy(double* __restrict__ partial, const int* __restrict__ nBonds,
-                              const int* __restrict__ bondedTo, const int* __restrict__ bondType, BondParams bp,
-                              const float* __restrict__ x, float L0, float L1, float L2, int ns, int maxbonds)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double e = 0.0;
-    if (i < ns) {
-        float Ls[3] = {L0, L1, L2};
-        for (int b = 0; b < nBonds[i]; ++b) {
-            int id2 = bondedTo[i * maxbonds + b];
-            int bt = bondType[i * maxbonds + b];
-            float r2 = 0.f;
-            for (int j = 0; j < 3; ++j) {
-                // 0.5 * Lh as in ps_deviceBonds.cu:132-133
-                float d = min_image(x[i * 3 + j] - x[id2 * 3 + j], Ls[j], 0.25f * Ls[j]);
-                r2 += d * d;
-            }
-            if (r2 > 1.0e-4f) {
-                float r = sqrtf(r2);
-                if (bp.style[bt] == 0) {
-                    float a = r - bp.r0[bt];
-                    e += (double)(bp.k[bt] * a * a);
-                } else if (bp.style[bt] == 1 && r / bp.r0[bt] < 1.f) {
-                    float u = r / bp.r0[bt];
-                    float arg = 1.f / (1.f - u * u);
-                    e += (double)(2.0f * bp.k[bt] * arg * r);
-                }
-            }
-        }
-    }
-    __shared__ double red[4];
-    e = wave_sum(e);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
-    __syncthreads();
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
-}
-
 __global__ void k_sum_partials(double* __restrict__ out, const double* __restrict__ partial, int n)
 {
     double s = 0.0;
@@ -226,15 +188,81 @@ __device__ __forceinline__ int key_of(float x0, float x1, float x2, int s, float
     return tile_key(wrapped_base(x0, dx0, p, nx), wrapped_base(x1, dx1, p, ny), wrapped_base(x2, dx2, p, nz), s, tg);
 }
 
-__global__ void k_bin_count(const float* __restrict__ x, const int* __restrict__ type, int ns,
-                            int* __restrict__ count, int* __restrict__ cell, int* __restrict__ rank, float dx0,
-                            float dx1, float dx2, int p, int nx, int ny, int nz, TileGeom tg)
+// Rank of each lane's key among equal keys, one atomic per run of equal keys
+// in consecutive lanes (particles in key order mostly keep their key, so a
+// wave's lanes fall into a few runs).  Every lane of the wave calls this;
+// inactive lanes do not count.
+__device__ __forceinline__ int wave_rank(int* __restrict__ count, int key, bool active)
+{
+    const int lane = threadIdx.x & 63;
+    const int k = active ? key : -1 - lane; // unique, never counted
+    const int prev = __shfl_up(k, 1, 64);
+    const bool head = lane == 0 || k != prev;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int hl = 63 - __clzll(heads & upto);
+    const unsigned long long above = heads & ~upto;
+    const int next = above ? __ffsll((long long)above) - 1 : 64;
+    int base = 0;
+    if (head && active) base = atomicAdd(&count[k], next - lane);
+    base = __shfl(base, hl, 64);
+    return base + (lane - hl);
+}
+
+// Initial sort from particle-id order: key and rank of every particle.
+__global__ void k_sort_count(const float* __restrict__ x, const int* __restrict__ type, int ns,
+                             int* __restrict__ count, int* __restrict__ key, int* __restrict__ rank, float dx0,
+                             float dx1, float dx2, int p, int nx, int ny, int nz, TileGeom tg)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ns) return;
     int c = key_of(x[i * 3], x[i * 3 + 1], x[i * 3 + 2], type[i], dx0, dx1, dx2, p, nx, ny, nz, tg);
-    cell[i] = c;
+    key[i] = c;
     rank[i] = atomicAdd(&count[c], 1);
+}
+
+// Initial sort: every particle's state into its slot.
+__global__ void k_sort_place(const float* __restrict__ x, const float* __restrict__ xo,
+                             const float* __restrict__ noise, const float* __restrict__ q,
+                             const int* __restrict__ type, const int* __restrict__ nBonds,
+                             const int* __restrict__ bondType, int maxbonds, int ns, const int* __restrict__ key,
+                             const int* __restrict__ rank, const int* __restrict__ start, int* __restrict__ slot_of,
+                             float4* __restrict__ X, float4* __restrict__ XO, float4* __restrict__ NZ,
+                             int2* __restrict__ META, int* __restrict__ NB, int* __restrict__ BT)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    const int s = start[key[i]] + rank[i];
+    slot_of[i] = s;
+    X[s] = make_float4(x[i * 3], x[i * 3 + 1], x[i * 3 + 2], q ? q[i] : 0.f);
+    XO[s] = make_float4(xo[i * 3], xo[i * 3 + 1], xo[i * 3 + 2], 0.f);
+    NZ[s] = make_float4(noise[i * 3], noise[i * 3 + 1], noise[i * 3 + 2], 0.f);
+    META[s] = make_int2(i, type[i]);
+    NB[s] = nBonds[i];
+    for (int b = 0; b < maxbonds; ++b) BT[s * maxbonds + b] = bondType[i * maxbonds + b];
+}
+
+// Initial sort: bond partners as slots.
+__global__ void k_sort_bonds(const int* __restrict__ bondedTo, const int* __restrict__ nBonds, int maxbonds, int ns,
+                             const int* __restrict__ slot_of, int* __restrict__ BS)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    const int s = slot_of[i];
+    for (int b = 0; b < maxbonds; ++b) BS[s * maxbonds + b] = b < nBonds[i] ? slot_of[bondedTo[i * maxbonds + b]] : 0;
+}
+
+// key order -> particle-id order: out[3 id + j] = src[slot].j
+__global__ void k_unsort(float* __restrict__ out, const float4* __restrict__ src, const int2* __restrict__ META,
+                         int ns)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const float4 v = src[s];
+    const int id = META[s].x;
+    out[id * 3 + 0] = v.x;
+    out[id * 3 + 1] = v.y;
+    out[id * 3 + 2] = v.z;
 }
 
 __global__ void k_scan_reduce(const int* __restrict__ count, int M, int* __restrict__ partial)
@@ -295,15 +323,36 @@ __global__ void k_scan_apply(int* __restrict__ count, int M, const int* __restri
     if (blockIdx.x == 0 && threadIdx.x == 0) start[M] = ns;
 }
 
-__global__ void k_bin_scatter(const float* __restrict__ x, const float* __restrict__ q, const int* __restrict__ type,
-                              int ns, const int* __restrict__ cell, const int* __restrict__ rank,
-                              const int* __restrict__ start, float4* __restrict__ spos, int2* __restrict__ smeta)
+// Particle state into the next step's key order: the slot's next position
+// and noise (from the integrator), its current position as the next previous
+// position, identity and bonds; map[slot] = new slot.
+__global__ void k_permute(int ns, int maxbonds, const int* __restrict__ key, const int* __restrict__ rank,
+                          const int* __restrict__ start, int* __restrict__ map, const float4* __restrict__ XN,
+                          const float4* __restrict__ NN, const float4* __restrict__ X, const int2* __restrict__ META,
+                          const int* __restrict__ NB, const int* __restrict__ BT, float4* __restrict__ X2,
+                          float4* __restrict__ XO2, float4* __restrict__ NZ2, int2* __restrict__ META2,
+                          int* __restrict__ NB2, int* __restrict__ BT2)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ns) return;
-    int slot = start[cell[i]] + rank[i];
-    spos[slot] = make_float4(x[i * 3], x[i * 3 + 1], x[i * 3 + 2], q ? q[i] : 0.f);
-    smeta[slot] = make_int2(i, type[i]);
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const int n = start[key[s]] + rank[s];
+    map[s] = n;
+    const float4 xc = X[s], xn = XN[s];
+    X2[n] = make_float4(xn.x, xn.y, xn.z, xc.w);
+    XO2[n] = make_float4(xc.x, xc.y, xc.z, 0.f);
+    NZ2[n] = NN[s];
+    META2[n] = META[s];
+    NB2[n] = NB[s];
+    for (int b = 0; b < maxbonds; ++b) BT2[n * maxbonds + b] = BT[s * maxbonds + b];
+}
+
+__global__ void k_remap_bonds(int ns, int maxbonds, const int* __restrict__ map, const int* __restrict__ NB,
+                              const int* __restrict__ BS, int* __restrict__ BS2)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const int n = map[s], nb = NB[s];
+    for (int b = 0; b < maxbonds; ++b) BS2[n * maxbonds + b] = b < nb ? map[BS[s * maxbonds + b]] : 0;
 }
 
 // --------------------------------------------------------- tile density ---
@@ -475,8 +524,9 @@ __global__ void __launch_bounds__(64) k_tile_reduce(float* __restrict__ rho, con
 // Force on each particle from its species' total grid force (potentials and,
 // with folded charges, q_s times the field) and, when charges are not uniform
 // per species, q times the separate charge field (d_mapGridForcesToPartics and
-// d_mapGridChargeForcesToPartics, device_utils.cu:358-461).  Direct variant:
-// one thread per particle in key order, stencil loaded from the global fields.
+// d_mapGridChargeForcesToPartics, device_utils.cu:358-461), written per slot
+// of the key order.  Direct variant: one thread per particle, stencil loaded
+// from the global fields.
 template <int P>
 __global__ void __launch_bounds__(256) k_gather(float4* __restrict__ fgrid, const float4* __restrict__ F4,
                                                 const int* __restrict__ force_slot, int qslot,
@@ -542,7 +592,7 @@ __global__ void __launch_bounds__(256) k_gather(float4* __restrict__ fgrid, cons
             f[2] += fq[2] * pp.w;
         }
     }
-    fgrid[meta.x] = make_float4(f[0], f[1], f[2], 0.f);
+    fgrid[i] = make_float4(f[0], f[1], f[2], 0.f);
 }
 
 // Tile variant.  The force field window of a brick ((b+p)^3 points from
@@ -641,7 +691,7 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_species(
     const int f = gm.slot[s];
     if (f < 0) {
         for (int it = pb + (int)threadIdx.x; it < pe; it += kTileThreads)
-            fgrid[smeta[it].x] = make_float4(0.f, 0.f, 0.f, 0.f);
+            fgrid[it] = make_float4(0.f, 0.f, 0.f, 0.f);
         return;
     }
     const int k0 = brick % tg.nb[0], k1 = (brick / tg.nb[0]) % tg.nb[1], k2 = brick / (tg.nb[0] * tg.nb[1]);
@@ -652,13 +702,12 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_species(
     const float i0 = 1.0f / dx0, i1 = 1.0f / dx1, i2 = 1.0f / dx2;
     for (int it = pb + (int)threadIdx.x; it < pe; it += kTileThreads) {
         const float4 pp = spos[it];
-        const int pid = smeta[it].x;
         float wx[P + 1], wy[P + 1], wz[P + 1];
         int l[3];
         float3 a = make_float3(0.f, 0.f, 0.f);
         if (stencil_in_brick<P>(pp, dx0, dx1, dx2, i0, i1, i2, nx, ny, nz, o0, o1, o2, tg, wx, wy, wz, l))
             a = stencil_sum<P, BM>(ft, l, wx, wy, wz);
-        fgrid[pid] = make_float4(a.x, a.y, a.z, 0.f);
+        fgrid[it] = make_float4(a.x, a.y, a.z, 0.f);
     }
 }
 
@@ -686,7 +735,7 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_charge(
     for (int it = pb + (int)threadIdx.x; it < pe; it += kTileThreads) {
         const float4 pp = spos[it];
         if (pp.w == 0.f) continue;
-        const int pid = smeta[it].x;
+        const int pid = it;
         float wx[P + 1], wy[P + 1], wz[P + 1];
         int l[3];
         if (!stencil_in_brick<P>(pp, dx0, dx1, dx2, i0, i1, i2, nx, ny, nz, o0, o1, o2, tg, wx, wy, wz, l)) continue;
@@ -698,64 +747,144 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_charge(
 
 // ------------------------------------------------- bonds + integrator ---
 // Bonds (ps_deviceBonds.cu:7-79) plus GJF (ps_integratorGJF.cu:102-163) or
-// EM (ps_integratorEM.cu:39-82) on group "all".  Reads x (current), writes the
-// next positions into xnext (the buffer that held x_old for GJF), counts the
-// next step's keys.  integ = 2 only writes the total force.
-__global__ void __launch_bounds__(256) k_bonds_integrate(
-    const float* __restrict__ x, float* __restrict__ xnext, float* __restrict__ noise_old,
-    const float4* __restrict__ fgrid, float* __restrict__ fout, const int* __restrict__ type,
-    const int* __restrict__ nBonds, const int* __restrict__ bondedTo, const int* __restrict__ bondType,
+// EM (ps_integratorEM.cu:39-82) on group "all", one thread per slot of the
+// key order.  Writes the next position and noise of the slot and counts the
+// next key; integ = MFT_INT_NONE only writes the total force.
+__device__ __forceinline__ void bond_force_slots(int s, int ns, const float4* __restrict__ X,
+                                                 const int* __restrict__ NB, const int* __restrict__ BS,
+                                                 const int* __restrict__ BT, int maxbonds, const BondParams& bp,
+                                                 float L0, float L1, float L2, float x0, float x1, float x2,
+                                                 float* fo)
+{
+    int nb = NB[s];
+    if (nb > maxbonds) nb = maxbonds;
+    for (int b = 0; b < nb; ++b) {
+        const int s2 = BS[s * maxbonds + b];
+        const int bt = BT[s * maxbonds + b];
+        if (s2 < 0 || s2 >= ns || bt < 0 || bt >= bp.ntypes) continue;
+        const float4 y = X[s2];
+        const float d0 = min_image(x0 - y.x, L0, 0.5f * L0);
+        const float d1 = min_image(x1 - y.y, L1, 0.5f * L1);
+        const float d2 = min_image(x2 - y.z, L2, 0.5f * L2);
+        const float r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        float mf = 0.f;
+        if (r2 > 1.0e-4f) {
+            const float r = sqrtf(r2);
+            if (bp.style[bt] == 0) mf = 2.0f * bp.k[bt] * (r - bp.r0[bt]) / r;
+            else if (bp.style[bt] == 1) {
+                const float u = r / bp.r0[bt];
+                mf = 2.0f * bp.k[bt] * r / (1.0f - u * u) / r;
+            }
+        }
+        fo[0] -= mf * d0;
+        fo[1] -= mf * d1;
+        fo[2] -= mf * d2;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_integrate(
+    const float4* __restrict__ X, const float4* __restrict__ XO, const float4* __restrict__ NZ,
+    const int2* __restrict__ META, const int* __restrict__ NB, const int* __restrict__ BS, const int* __restrict__ BT,
+    const float4* __restrict__ fgrid, float4* __restrict__ ftot, float4* __restrict__ XN, float4* __restrict__ NN,
     int maxbonds, BondParams bp, GjfParams gp, int integ, float delt, float L0, float L1, float L2, int ns,
-    uint64_t seed, uint32_t step, int* __restrict__ count, int* __restrict__ cell, int* __restrict__ rank,
+    uint64_t seed, uint32_t step, int* __restrict__ count, int* __restrict__ key, int* __restrict__ rank,
     float dx0, float dx1, float dx2, int p, int nx, int ny, int nz, TileGeom tg)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ns) return;
-    float xc[3] = {x[i * 3], x[i * 3 + 1], x[i * 3 + 2]};
-    const float4 fg = fgrid[i];
-    float fo[3] = {fg.x, fg.y, fg.z};
-    bond_force(i, ns, x, nBonds, bondedTo, bondType, maxbonds, bp, L0, L1, L2, xc[0], xc[1], xc[2], fo);
-    if (fout) {
-        fout[i * 3 + 0] = fo[0];
-        fout[i * 3 + 1] = fo[1];
-        fout[i * 3 + 2] = fo[2];
-    }
-    if (integ == MFT_INT_NONE) return;
-    float z[4];
-    normal4((uint32_t)i, step, TAG_TILD, 0u, seed, z);
-    const float Ls[3] = {L0, L1, L2};
-    float xn[3];
-    if (integ == MFT_INT_GJF) {
-        int ty = type[i];
-        float a = gp.a[ty], b = gp.b[ty], nm = gp.nmag[ty], cf = gp.cf[ty], cn = gp.cn[ty];
-        for (int j = 0; j < 3; ++j) {
-            float nn = nm * z[j];
-            float xo = xnext[i * 3 + j];
-            if (xo - xc[j] > Ls[j] / 2.0f) xo -= Ls[j];
-            else if (xo - xc[j] < -Ls[j] / 2.0f) xo += Ls[j];
-            float v = 2.0f * b * xc[j] - a * xo + cf * fo[j] + cn * (nn + noise_old[i * 3 + j]);
-            if (v > Ls[j]) v -= Ls[j];
-            else if (v < 0.0f) v += Ls[j];
-            xn[j] = v;
-            noise_old[i * 3 + j] = nn;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = s < ns;
+    int k = 0;
+    if (active) {
+        const float4 xc = X[s];
+        const float4 fg = fgrid[s];
+        float fo[3] = {fg.x, fg.y, fg.z};
+        bond_force_slots(s, ns, X, NB, BS, BT, maxbonds, bp, L0, L1, L2, xc.x, xc.y, xc.z, fo);
+        if (ftot) ftot[s] = make_float4(fo[0], fo[1], fo[2], 0.f);
+        if (integ != MFT_INT_NONE) {
+            const int2 meta = META[s];
+            float z[4];
+            normal4((uint32_t)meta.x, step, TAG_TILD, 0u, seed, z);
+            const float Ls[3] = {L0, L1, L2};
+            const float xcv[3] = {xc.x, xc.y, xc.z};
+            float xn[3], nn[3] = {0.f, 0.f, 0.f};
+            if (integ == MFT_INT_GJF) {
+                const int ty = meta.y;
+                const float a = gp.a[ty], b = gp.b[ty], nm = gp.nmag[ty], cf = gp.cf[ty], cn = gp.cn[ty];
+                const float4 xo4 = XO[s], no4 = NZ[s];
+                const float xov[3] = {xo4.x, xo4.y, xo4.z}, nov[3] = {no4.x, no4.y, no4.z};
+                for (int j = 0; j < 3; ++j) {
+                    const float nnj = nm * z[j];
+                    float xo = xov[j];
+                    if (xo - xcv[j] > Ls[j] / 2.0f) xo -= Ls[j];
+                    else if (xo - xcv[j] < -Ls[j] / 2.0f) xo += Ls[j];
+                    float v = 2.0f * b * xcv[j] - a * xo + cf * fo[j] + cn * (nnj + nov[j]);
+                    if (v > Ls[j]) v -= Ls[j];
+                    else if (v < 0.0f) v += Ls[j];
+                    xn[j] = v;
+                    nn[j] = nnj;
+                }
+            } else {
+                const float nm = sqrtf(2.0f * delt);
+                for (int j = 0; j < 3; ++j) {
+                    float v = xcv[j] + delt * fo[j] + nm * z[j];
+                    if (v > Ls[j]) v -= Ls[j];
+                    else if (v < 0.0f) v += Ls[j];
+                    xn[j] = v;
+                }
+            }
+            XN[s] = make_float4(xn[0], xn[1], xn[2], 0.f);
+            NN[s] = make_float4(nn[0], nn[1], nn[2], 0.f);
+            k = key_of(xn[0], xn[1], xn[2], meta.y, dx0, dx1, dx2, p, nx, ny, nz, tg);
         }
-    } else {
-        float nm = sqrtf(2.0f * delt);
-        for (int j = 0; j < 3; ++j) {
-            float v = xc[j] + delt * fo[j] + nm * z[j];
-            if (v > Ls[j]) v -= Ls[j];
-            else if (v < 0.0f) v += Ls[j];
-            xn[j] = v;
+    }
+    if (integ == MFT_INT_NONE) return; // uniform over the grid
+    const int r = wave_rank(count, k, active);
+    if (active) {
+        key[s] = k;
+        rank[s] = r;
+    }
+}
+
+// d_bondStressEnergy energy part (ps_deviceBonds.cu:85-171) over slots, FP64
+// block sums, with the reference's 0.5*Lh minimum image (:132-133).
+__global__ void k_bond_energy_slots(double* __restrict__ partial, const float4* __restrict__ X,
+                                    const int* __restrict__ NB, const int* __restrict__ BS,
+                                    const int* __restrict__ BT, BondParams bp, float L0, float L1, float L2, int ns,
+                                    int maxbonds)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if (s < ns) {
+        const float Ls[3] = {L0, L1, L2};
+        const float4 x = X[s];
+        const float xv[3] = {x.x, x.y, x.z};
+        for (int b = 0; b < NB[s]; ++b) {
+            const int s2 = BS[s * maxbonds + b];
+            const int bt = BT[s * maxbonds + b];
+            const float4 y = X[s2];
+            const float yv[3] = {y.x, y.y, y.z};
+            float r2 = 0.f;
+            for (int j = 0; j < 3; ++j) {
+                const float d = min_image(xv[j] - yv[j], Ls[j], 0.25f * Ls[j]);
+                r2 += d * d;
+            }
+            if (r2 > 1.0e-4f) {
+                const float r = sqrtf(r2);
+                if (bp.style[bt] == 0) {
+                    const float a = r - bp.r0[bt];
+                    e += (double)(bp.k[bt] * a * a);
+                } else if (bp.style[bt] == 1 && r / bp.r0[bt] < 1.f) {
+                    const float u = r / bp.r0[bt];
+                    const float arg = 1.f / (1.f - u * u);
+                    e += (double)(2.0f * bp.k[bt] * arg * r);
+                }
+            }
         }
     }
-    xnext[i * 3 + 0] = xn[0];
-    xnext[i * 3 + 1] = xn[1];
-    xnext[i * 3 + 2] = xn[2];
-    if (count) {
-        int c = key_of(xn[0], xn[1], xn[2], type[i], dx0, dx1, dx2, p, nx, ny, nz, tg);
-        cell[i] = c;
-        rank[i] = atomicAdd(&count[c], 1);
-    }
+    __shared__ double red[4];
+    e = wave_sum(e);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
 // ----------------------------------------------------------- launchers ---
@@ -840,15 +969,37 @@ int tild_tiles_configure(TildEngine* e)
     e->nscan_blocks = (e->nkeys + kScanChunk - 1) / kScanChunk;
     if ((rc = dalloc(&e->partial, (size_t)e->nscan_blocks))) return rc;
     MFT_HIP(hipMemset(e->count, 0, sizeof(int) * (size_t)e->nkeys));
-    e->binned = false;
+    e->have_state = 0; // the key order changed: the state is sorted again when set
     return MFT_OK;
 }
 
-int tild_bin_count(TildEngine* e)
+// particle-id-order staging (pio: positions, pio2: previous positions, pio3:
+// previous noise) -> key order, with counts, starts and bond slots
+int tild_sort_state(TildEngine* e)
 {
     const mft_tild_desc& d = e->d;
-    k_bin_count<<<nblk(e->N), kBlock, 0, e->stream>>>(e->x[e->cur], e->type, e->N, e->count, e->cell, e->rank, e->dx[0],
-                                                       e->dx[1], e->dx[2], d.pmeorder, d.nx, d.ny, d.nz, e->tg);
+    const int N = e->N, mb = std::max(1, d.maxbonds), c = e->cur;
+    MFT_HIP(hipMemsetAsync(e->count, 0, sizeof(int) * (size_t)e->nkeys, e->stream));
+    k_sort_count<<<nblk(N), kBlock, 0, e->stream>>>(e->pio, e->type, N, e->count, e->key, e->rank, e->dx[0],
+                                                     e->dx[1], e->dx[2], d.pmeorder, d.nx, d.ny, d.nz, e->tg);
+    MFT_LAUNCH_CHECK();
+    int rc;
+    if ((rc = tild_scan(e))) return rc;
+    k_sort_place<<<nblk(N), kBlock, 0, e->stream>>>(e->pio, e->pio2, e->pio3, e->charge, e->type, e->nBonds,
+                                                     e->bondType, mb, N, e->key, e->rank, e->start, e->slot_of,
+                                                     e->X[c], e->XO[c], e->NZ[c], e->META[c], e->NB[c], e->BT[c]);
+    k_sort_bonds<<<nblk(N), kBlock, 0, e->stream>>>(e->bondedTo, e->nBonds, mb, N, e->slot_of, e->BS[c]);
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
+// key order -> particle-id order into pio: what 0 positions, 1 previous
+// positions, 2 previous noise, 3 total force of the last evaluation
+int tild_unsort(TildEngine* e, int what)
+{
+    const int c = e->cur;
+    const float4* src = what == 0 ? e->X[c] : what == 1 ? e->XO[c] : what == 2 ? e->NZ[c] : e->ftot;
+    k_unsort<<<nblk(e->N), kBlock, 0, e->stream>>>(e->pio, src, e->META[c], e->N);
     MFT_LAUNCH_CHECK();
     return MFT_OK;
 }
@@ -862,11 +1013,16 @@ int tild_scan(TildEngine* e)
     return MFT_OK;
 }
 
-int tild_scatter(TildEngine* e)
+// next positions into the next key order (after the integrator and the scan)
+int tild_permute(TildEngine* e)
 {
-    k_bin_scatter<<<nblk(e->N), kBlock, 0, e->stream>>>(e->x[e->cur], e->charge, e->type, e->N, e->cell, e->rank,
-                                                         e->start, e->spos, e->smeta);
+    const int N = e->N, mb = std::max(1, e->d.maxbonds), c = e->cur, n = 1 - c;
+    k_permute<<<nblk(N), kBlock, 0, e->stream>>>(N, mb, e->key, e->rank, e->start, e->map, e->XN, e->NN, e->X[c],
+                                                  e->META[c], e->NB[c], e->BT[c], e->X[n], e->XO[n], e->NZ[n],
+                                                  e->META[n], e->NB[n], e->BT[n]);
+    k_remap_bonds<<<nblk(N), kBlock, 0, e->stream>>>(N, mb, e->map, e->NB[c], e->BS[c], e->BS[n]);
     MFT_LAUNCH_CHECK();
+    e->cur = n;
     return MFT_OK;
 }
 
@@ -878,7 +1034,7 @@ int tild_density(TildEngine* e)
     const double inv_gvol = 1.0 / (double)e->gvol;
 #define MFT_DENS(P_, B_)                                                                                  \
     k_density_tile<P_, B_><<<tg.nbricks, kTileThreads, e->dens_lds, e->stream>>>(                         \
-        e->tiles, e->nchan, e->qchan, e->spos, e->smeta, e->start, tg, d.nx, d.ny, d.nz, e->dx[0], e->dx[1], \
+        e->tiles, e->nchan, e->qchan, e->X[e->cur], e->META[e->cur], e->start, tg, d.nx, d.ny, d.nz, e->dx[0], e->dx[1], \
         e->dx[2], inv_gvol)
 #define MFT_DENS_B(P_)                                                                                    \
     if (BM == 8) MFT_DENS(P_, 8);                                                                         \
@@ -906,7 +1062,7 @@ int tild_gather(TildEngine* e, int mode)
     const mft_tild_desc& d = e->d;
     if (mode == 0) {
 #define MFT_G0(P_)                                                                                        \
-    k_gather<P_><<<nblk(e->N), kBlock, 0, e->stream>>>(e->fgrid, e->F4, e->force_slot, e->qslot, e->spos, e->smeta, \
+    k_gather<P_><<<nblk(e->N), kBlock, 0, e->stream>>>(e->fgrid, e->F4, e->force_slot, e->qslot, e->X[e->cur], e->META[e->cur], \
                                                         e->N, d.nx, d.ny, d.nz, e->dx[0], e->dx[1], e->dx[2])
         switch (d.pmeorder) {
         case 0: MFT_G0(0); break;
@@ -925,12 +1081,12 @@ int tild_gather(TildEngine* e, int mode)
         const float4* Fq = e->F4 + (size_t)(e->qslot >= 0 ? e->qslot : 0) * e->M;
 #define MFT_G1(P_, B_)                                                                                    \
     do {                                                                                                  \
-        k_gather_species<P_, B_><<<grid, kTileThreads, 0, e->stream>>>(e->fgrid, e->F4, gm, e->spos, e->smeta, \
+        k_gather_species<P_, B_><<<grid, kTileThreads, 0, e->stream>>>(e->fgrid, e->F4, gm, e->X[e->cur], e->META[e->cur], \
                                                                          e->start, tg, d.nx, d.ny, d.nz, e->dx[0], \
                                                                          e->dx[1], e->dx[2]);             \
         if (e->qslot >= 0)                                                                                \
             k_gather_charge<P_, B_><<<tg.nbricks, kTileThreads, 0, e->stream>>>(                          \
-                e->fgrid, Fq, e->spos, e->smeta, e->start, tg, d.nx, d.ny, d.nz, e->dx[0], e->dx[1], e->dx[2]); \
+                e->fgrid, Fq, e->X[e->cur], e->META[e->cur], e->start, tg, d.nx, d.ny, d.nz, e->dx[0], e->dx[1], e->dx[2]); \
     } while (0)
 #define MFT_G1_B(P_)                                                                                      \
     if (BM == 8) MFT_G1(P_, 8);                                                                           \
@@ -949,13 +1105,15 @@ int tild_gather(TildEngine* e, int mode)
     return MFT_OK;
 }
 
-int tild_integrate(TildEngine* e, int nxt, bool want_f)
+int tild_integrate(TildEngine* e, bool want_f)
 {
     const mft_tild_desc& d = e->d;
-    k_bonds_integrate<<<nblk(e->N), kBlock, 0, e->stream>>>(
-        e->x[e->cur], e->x[nxt], e->noise_old, e->fgrid, want_f ? e->ftot : nullptr, e->type, e->nBonds, e->bondedTo,
-        e->bondType, d.maxbonds, e->bp, e->gp, d.integrator, d.delt, d.lx, d.ly, d.lz, e->N, d.seed, (uint32_t)e->step,
-        e->count, e->cell, e->rank, e->dx[0], e->dx[1], e->dx[2], d.pmeorder, d.nx, d.ny, d.nz, e->tg);
+    const int c = e->cur;
+    k_integrate<<<nblk(e->N), kBlock, 0, e->stream>>>(
+        e->X[c], e->XO[c], e->NZ[c], e->META[c], e->NB[c], e->BS[c], e->BT[c], e->fgrid, want_f ? e->ftot : nullptr,
+        e->XN, e->NN, std::max(1, d.maxbonds), e->bp, e->gp, d.integrator, d.delt, d.lx, d.ly, d.lz, e->N, d.seed,
+        (uint32_t)e->step, e->count, e->key, e->rank, e->dx[0], e->dx[1], e->dx[2], d.pmeorder, d.nx, d.ny, d.nz,
+        e->tg);
     MFT_LAUNCH_CHECK();
     return MFT_OK;
 }
@@ -963,9 +1121,10 @@ int tild_integrate(TildEngine* e, int nxt, bool want_f)
 int tild_bond_energy(TildEngine* e)
 {
     const mft_tild_desc& d = e->d;
+    const int c = e->cur;
     unsigned nb = nblk(e->N);
-    k_bond_energy<<<nb, kBlock, 0, e->stream>>>(e->bpart, e->nBonds, e->bondedTo, e->bondType, e->bp, e->x[e->cur],
-                                                 d.lx, d.ly, d.lz, e->N, d.maxbonds);
+    k_bond_energy_slots<<<nb, kBlock, 0, e->stream>>>(e->bpart, e->X[c], e->NB[c], e->BS[c], e->BT[c], e->bp, d.lx,
+                                                       d.ly, d.lz, e->N, std::max(1, d.maxbonds));
     k_sum_partials<<<1, 1024, 0, e->stream>>>(e->eout + kMaxPots, e->bpart, nb);
     MFT_LAUNCH_CHECK();
     return MFT_OK;

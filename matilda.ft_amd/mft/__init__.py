@@ -45,6 +45,7 @@ def lib():
             raise FileNotFoundError(f"{LIB_PATH} missing: build with `make -C matilda.ft_amd`")
         _lib = C.CDLL(LIB_PATH)
         _lib.mft_last_error.restype = C.c_char_p
+        _lib.mft_server_error.restype = C.c_char_p
         for name in ("mft_tild_stream", "mft_fts_stream"):
             if hasattr(_lib, name):
                 getattr(_lib, name).restype = _p
@@ -103,6 +104,13 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+def mem_info():
+    """(free, total) device memory in bytes."""
+    f, t = C.c_size_t(), C.c_size_t()
+    check(lib().mft_mem_info(C.byref(f), C.byref(t)))
+    return f.value, t.value
 
 
 def synchronize():
@@ -224,27 +232,33 @@ def bonds(x, topo, bond_k, bond_r0, bond_style, L, maxbonds=2):
     return f.to_array(np.float32, 3 * ns)
 
 
+def tild_desc(Nx, L, pmeorder, nspecies, nparticles, potentials, bond_k=(), bond_r0=(), bond_style=(),
+              integrator="GJF", delt=0.005, mass=None, mobility=None, seed=1234, has_charges=False, maxbonds=2):
+    """mft_tild_desc and the arrays it points to (keep them alive)."""
+    pots = (TildPotentialC * max(1, len(potentials)))()
+    for i, p in enumerate(potentials):
+        pots[i] = TildPotentialC(POT[p["kind"]], p.get("maskI", 0), p.get("maskJ", 0), p["A"],
+                                 p.get("sigma", 0.0), p.get("Rp", 0.0), int(p.get("ramp", False)),
+                                 p.get("A_final", 0.0))
+    bk = np.asarray(bond_k, np.float32)
+    br = np.asarray(bond_r0, np.float32)
+    bs = np.asarray(bond_style, np.int32)
+    ma = np.asarray(mass if mass is not None else [1.0] * nspecies, np.float32)
+    mo = np.asarray(mobility if mobility is not None else [1.0] * nspecies, np.float32)
+    d = TildDescC(Nx[0], Nx[1], Nx[2], L[0], L[1], L[2], pmeorder, nparticles, nspecies, maxbonds,
+                  bk.size, _ptr(bk), _ptr(br), _ptr(bs), len(potentials), C.cast(pots, _p),
+                  INTEGRATOR[integrator], delt, _ptr(ma), _ptr(mo), seed, int(has_charges))
+    return d, [pots, bk, br, bs, ma, mo]
+
+
 class TildEngine:
     """Device-resident TILD system (mft_tild_*)."""
 
     def __init__(self, Nx, L, pmeorder, nspecies, nparticles, potentials, bond_k=(), bond_r0=(),
                  bond_style=(), integrator="GJF", delt=0.005, mass=None, mobility=None, seed=1234,
                  has_charges=False, maxbonds=2):
-        self._keep = []
-        pots = (TildPotentialC * max(1, len(potentials)))()
-        for i, p in enumerate(potentials):
-            pots[i] = TildPotentialC(POT[p["kind"]], p.get("maskI", 0), p.get("maskJ", 0), p["A"],
-                                     p.get("sigma", 0.0), p.get("Rp", 0.0), int(p.get("ramp", False)),
-                                     p.get("A_final", 0.0))
-        bk = np.asarray(bond_k, np.float32)
-        br = np.asarray(bond_r0, np.float32)
-        bs = np.asarray(bond_style, np.int32)
-        ma = np.asarray(mass if mass is not None else [1.0] * nspecies, np.float32)
-        mo = np.asarray(mobility if mobility is not None else [1.0] * nspecies, np.float32)
-        self._keep += [pots, bk, br, bs, ma, mo]
-        d = TildDescC(Nx[0], Nx[1], Nx[2], L[0], L[1], L[2], pmeorder, nparticles, nspecies, maxbonds,
-                      bk.size, _ptr(bk), _ptr(br), _ptr(bs), len(potentials), C.cast(pots, _p),
-                      INTEGRATOR[integrator], delt, _ptr(ma), _ptr(mo), seed, int(has_charges))
+        d, self._keep = tild_desc(Nx, L, pmeorder, nspecies, nparticles, potentials, bond_k, bond_r0, bond_style,
+                                  integrator, delt, mass, mobility, seed, has_charges, maxbonds)
         self.h = _p()
         check(lib().mft_tild_create(C.byref(self.h), C.byref(d)))
         self.N = nparticles
@@ -503,3 +517,103 @@ class FtsBatch:
         r = np.empty(s["nspecies"] * M, np.complex64)
         check(lib().mft_fts_batch_get_densities(self.h, rep, _ptr(r)))
         return r.reshape(s["nspecies"], M)
+
+
+REQ_STATE = {0: "queued", 1: "running", 2: "done", 3: "failed"}
+
+
+class Server:
+    """Request loop with continuous batching (mft_server_*): CL-FTS solves join
+    one device-resident pool, TILD runs get their own engines; requests are
+    admitted against a device-memory budget."""
+
+    def __init__(self, hbm_budget_bytes, max_fts_replicas=256):
+        self.h = _p()
+        check(lib().mft_server_create(C.byref(self.h), C.c_double(hbm_budget_bytes), max_fts_replicas))
+        self.meta = {}
+
+    def close(self):
+        if self.h:
+            lib().mft_server_destroy(self.h)
+            self.h = _p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit_fts(self, spec, fields, niter):
+        d = fts_desc(spec)
+        npot = len(spec["potentials"])
+        M = int(np.prod(spec["Nx"]))
+        keep = []
+        wpl = (_p * max(1, npot))()
+        wmi = (_p * max(1, npot))()
+        for p in range(npot):
+            for arr, key in ((wpl, "pl"), (wmi, "mi")):
+                f = fields.get((p, key))
+                if f is not None:
+                    a = _c64(f)
+                    assert a.size == M
+                    keep.append(a)
+                    arr[p] = a.ctypes.data
+        rid = C.c_int64()
+        check(lib().mft_server_submit_fts(self.h, C.byref(d), wpl, wmi, int(niter), C.byref(rid)))
+        self.meta[rid.value] = ("fts", spec)
+        return rid.value
+
+    def submit_tild(self, cfg, nsteps, integrator="GJF"):
+        topo = cfg["topo"]
+        n = topo["type"].size
+        d, keep = tild_desc(cfg["Nx"], cfg["L"], cfg["pmeorder"], len(cfg["species"]), n, cfg["potentials"],
+                            cfg["bond_k"], cfg["bond_r0"], cfg["bond_style"], integrator, cfg["delt"], cfg["mass"],
+                            cfg["mobility"], cfg["seed"], cfg["has_charges"])
+        t = np.ascontiguousarray(topo["type"], np.int32)
+        q = np.ascontiguousarray(topo["charge"], np.float32) if cfg["has_charges"] else None
+        nb = np.ascontiguousarray(topo["nBonds"], np.int32)
+        bt = np.ascontiguousarray(topo["bondedTo"], np.int32)
+        ty = np.ascontiguousarray(topo["bondType"], np.int32)
+        x = np.ascontiguousarray(cfg["x"], np.float32)
+        rid = C.c_int64()
+        check(lib().mft_server_submit_tild(self.h, C.byref(d), _ptr(t), _ptr(q), _ptr(nb), _ptr(bt), _ptr(ty),
+                                           _ptr(x), int(nsteps), C.byref(rid)))
+        self.meta[rid.value] = ("tild", cfg)
+        return rid.value
+
+    def pump(self, max_iters=50):
+        nr, nq = C.c_int(), C.c_int()
+        check(lib().mft_server_pump(self.h, int(max_iters), C.byref(nr), C.byref(nq)))
+        return nr.value, nq.value
+
+    def status(self, rid):
+        st = C.c_int()
+        check(lib().mft_server_status(self.h, C.c_int64(rid), C.byref(st)))
+        return REQ_STATE[st.value]
+
+    def error(self, rid):
+        return lib().mft_server_error(self.h, C.c_int64(rid)).decode()
+
+    def fts_result(self, rid):
+        spec = self.meta[rid][1]
+        nmol, npot = len(spec["molecules"]), len(spec["potentials"])
+        M = int(np.prod(spec["Nx"]))
+        Q = np.zeros(2 * nmol)
+        H = np.zeros(2)
+        parts = np.zeros(2 * (npot + nmol))
+        rho = np.empty(spec["nspecies"] * M, np.complex64)
+        check(lib().mft_server_fts_result(self.h, C.c_int64(rid), _ptr(Q), _ptr(H), _ptr(parts), _ptr(rho)))
+        return Q.view(np.complex128), complex(H[0], H[1]), parts.view(np.complex128), rho.reshape(-1, M)
+
+    def tild_result(self, rid):
+        cfg = self.meta[rid][1]
+        n = cfg["topo"]["type"].size
+        x = np.empty(3 * n, np.float32)
+        pe = np.zeros(max(1, len(cfg["potentials"])))
+        be = _d(0)
+        check(lib().mft_server_tild_result(self.h, C.c_int64(rid), _ptr(x), _ptr(pe), C.byref(be)))
+        return x, pe[:len(cfg["potentials"])], be.value
+
+    def release(self, rid):
+        check(lib().mft_server_release(self.h, C.c_int64(rid)))
+        self.meta.pop(rid, None)

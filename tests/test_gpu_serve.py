@@ -1,0 +1,87 @@
+"""Continuous batching in the request loop: CL-FTS solves and TILD runs
+admitted at different times, while others are on the device, give the same
+results as each run alone (bit for bit: a replica's arithmetic and noise do
+not depend on its batch), and admission respects the device-memory budget."""
+import numpy as np
+import pytest
+
+import mft
+from mft import systems
+from test_oracle_tild import small_system
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def fts_case(grid, Ns, chiN, seed):
+    NA = Ns // 2
+    spec, fields = systems.fts_diblock(grid_shape=grid, L=tuple(0.25 * g for g in grid), Nr=float(Ns), NA=NA,
+                                       NB=Ns - NA, chiN=chiN, kappaN=60.0, C=4.0, dt=0.01, sigma=0.1, seed=seed)
+    return spec, fields
+
+
+def alone_fts(spec, fields, niter):
+    s = systems.fts_system(spec, fields)
+    s.step(niter)
+    Q, H, parts = s.observables()
+    rho = s.densities()
+    s.close()
+    return Q, H, parts, rho
+
+
+def test_results_same_alone_and_in_the_mix(gpu):
+    cases = [(fts_case((16, 16, 16), 10, 12.0, 1), 20), (fts_case((24, 24, 24), 12, 15.0, 2), 35),
+             (fts_case((16, 16, 16), 8, 18.0, 3), 15), (fts_case((32, 32, 32), 10, 11.0, 4), 30),
+             (fts_case((24, 24, 24), 12, 9.0, 5), 12)]
+    tcfg = small_system(True, seed=3)
+    srv = mft.Server(hbm_budget_bytes=4e9, max_fts_replicas=8)
+    say("case: serve mix, 5 CL-FTS + 1 TILD")
+    ids = [srv.submit_fts(cases[0][0][0], cases[0][0][1], cases[0][1]),
+           srv.submit_fts(cases[1][0][0], cases[1][0][1], cases[1][1])]
+    srv.pump(10)
+    ids.append(srv.submit_fts(cases[2][0][0], cases[2][0][1], cases[2][1]))
+    tid = srv.submit_tild(tcfg, 9)
+    srv.pump(7)
+    ids += [srv.submit_fts(c[0][0], c[0][1], c[1]) for c in cases[3:]]
+    for _ in range(200):
+        nr, nq = srv.pump(8)
+        if nr == 0 and nq == 0:
+            break
+    assert nr == 0 and nq == 0
+    for rid, ((spec, fields), niter) in zip(ids, cases):
+        assert srv.status(rid) == "done"
+        Q, H, parts, rho = srv.fts_result(rid)
+        Qa, Ha, pa, rhoa = alone_fts(spec, fields, niter)
+        say("  fts", spec["Nx"], niter, "H", H, Ha)
+        assert np.array_equal(Q, Qa) and H == Ha and np.array_equal(parts, pa)
+        assert np.array_equal(rho, rhoa)
+    x, pe, be = srv.tild_result(tid)
+    eng = systems.tild_engine(tcfg, "GJF")
+    eng.step(9)
+    xa, _, _ = eng.get_state()
+    pea, bea = eng.energies()
+    eng.close()
+    assert np.array_equal(x, xa) and np.array_equal(pe, pea) and be == bea
+    srv.close()
+
+
+def test_admission_respects_budget(gpu):
+    (spec, fields) = fts_case((24, 24, 24), 12, 14.0, 7)
+    one = 1.2 * 8 * 24 ** 3 * (3 + 4 + 24 + 2 + 2)   # about one replica's buffers
+    srv = mft.Server(hbm_budget_bytes=one, max_fts_replicas=4)
+    say("case: serve budget")
+    a = srv.submit_fts(spec, fields, 10)
+    b = srv.submit_fts(spec, fields, 10)
+    srv.pump(5)
+    assert srv.status(a) == "running" and srv.status(b) == "queued"
+    for _ in range(20):
+        nr, nq = srv.pump(5)
+        if nr == 0 and nq == 0:
+            break
+    Qa = srv.fts_result(a)[0]
+    Qb = srv.fts_result(b)[0]
+    assert np.array_equal(Qa, Qb)   # same request, same result, whenever it ran
+    srv.close()
