@@ -210,7 +210,20 @@ def main():
     ap.add_argument("--no-fts", action="store_true", help="skip the CL-FTS plain/batched measurements")
     ap.add_argument("--no-serve", action="store_true", help="skip the served mixed workload")
     ap.add_argument("--serve-window", type=float, default=1.0, help="seconds of Poisson arrivals to serve")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run 5 device-to-device copies of BYTES (the known-size kernel that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import ctypes
+        import mft
+        a, b = mft.DeviceBuffer(args.calibrate_copy), mft.DeviceBuffer(args.calibrate_copy)
+        a.zero()
+        for _ in range(5):
+            mft.check(mft.lib().mft_memcpy_d2d(b.ptr, a.ptr, ctypes.c_size_t(args.calibrate_copy)))
+        mft.synchronize()
+        print(json.dumps({"copy_bytes": args.calibrate_copy, "copies": 5}), flush=True)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build runs the single-GPU path only "
@@ -246,10 +259,10 @@ def main():
     hbm_step = step_bytes(N, M, nchan) / (ms_step * 1e-3) / 1e9
 
     traffic = None
-    prof = os.path.join(ROOT, "profiles", f"pmc_{dom}_tild_coacervate.json")
+    prof = os.path.join(ROOT, "profiles", "pmc_traffic_tild_coacervate_1.8M_128.json")
     if os.path.exists(prof):
         with open(prof) as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
+            traffic = json.load(fh).get("bytes_per_launch", {}).get(dom)
 
     out = {
         "metric": "TILD particle-steps/s",

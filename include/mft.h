@@ -40,6 +40,7 @@ int mft_malloc(void** dptr, size_t bytes);
 int mft_free(void* dptr);
 int mft_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int mft_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int mft_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int mft_memset(void* dst, int value, size_t bytes);
 int mft_synchronize(void);
 int mft_stream_create(void** stream);

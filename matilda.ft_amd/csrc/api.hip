@@ -100,6 +100,12 @@ int mft_memcpy_d2h(void* dst, const void* src, size_t bytes)
     return MFT_OK;
 }
 
+int mft_memcpy_d2d(void* dst, const void* src, size_t bytes)
+{
+    MFT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return MFT_OK;
+}
+
 int mft_memset(void* dst, int value, size_t bytes)
 {
     MFT_HIP(hipMemset(dst, value, bytes));

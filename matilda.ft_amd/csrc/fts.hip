@@ -20,6 +20,7 @@
 #include <array>
 #include <complex>
 #include <cstring>
+#include <cstdlib>
 #include <deque>
 #include <map>
 #include <vector>
@@ -296,6 +297,92 @@ __global__ void __launch_bounds__(256) k_fts_prop_axis(const PropDesc* __restric
     }
 }
 
+// x and y diffusion of one z-plane in one workgroup: the plane (times
+// E_{b(s-1)}) is staged in LDS, every row is transformed along x, multiplied
+// by g_x and transformed back in place, then every column along y; the x and y
+// passes of k_fts_prop_axis with the plane kept in LDS between them (the same
+// arithmetic, one global read and write instead of two).  Square planes up to
+// 64 x 64; lines past the plane use a scratch region for their exchanges.
+template <int N>
+__global__ void __launch_bounds__(256) k_fts_prop_xy(const PropDesc* __restrict__ dxs,
+                                                     const PropDesc* __restrict__ dys,
+                                                     const float2* __restrict__ twg)
+{
+    using P = FftPlan<N>;
+    constexpr int T = P::T, E = P::E, B = 256 / T;
+    constexpr int NL = (N + B - 1) / B;
+    __shared__ float2 plane[N * N];
+    __shared__ float2 scratch[(N % B) ? B * N : 1];
+    __shared__ float2 tw[N], gx[N], gy[N];
+    const PropDesc& DX = dxs[blockIdx.y];
+    const PropDesc& DY = dys[blockIdx.y];
+    for (int i = threadIdx.x; i < N; i += 256) {
+        tw[i] = twg[i];
+        gx[i] = DX.g[i];
+        gy[i] = DY.g[i];
+    }
+    const long off = (long)blockIdx.x * N * N;
+    for (int i = threadIdx.x; i < N * N; i += 256) {
+        float2 a = DX.src[off + i];
+        if (DX.pre) a = cmul(a, DX.pre[off + i]);
+        plane[i] = a;
+    }
+    __syncthreads();
+    // rows (x): T consecutive threads per row
+    {
+        const int lx = threadIdx.x / T, t = threadIdx.x % T;
+        for (int round = 0; round < NL; ++round) {
+            const int row = round * B + lx;
+            const bool valid = row < N;
+            float2 v[E];
+#pragma unroll
+            for (int s = 0; s < E; ++s) v[s] = valid ? plane[row * N + t + T * s] : make_float2(0.f, 0.f);
+            float2* base = valid ? plane : scratch;
+            LdsMap map{valid ? row : lx, N, false, N};
+            line_fft<N, -1>(v, base, tw, t, map);
+#pragma unroll
+            for (int s = 0; s < E; ++s) {
+                const float2 g = gx[t + T * s];
+                v[s] = make_float2(fmaf(v[s].x, g.y, v[s].x * g.x), fmaf(v[s].y, g.y, v[s].y * g.x));
+            }
+            line_fft<N, +1>(v, base, tw, t, map);
+            __syncthreads();
+            if (valid) {
+#pragma unroll
+                for (int s = 0; s < E; ++s) plane[row * N + t + T * s] = v[s];
+            }
+            __syncthreads();
+        }
+    }
+    // columns (y): B consecutive threads cover B columns, element stride N
+    {
+        const int lc = threadIdx.x % B, t = threadIdx.x / B;
+        for (int round = 0; round < NL; ++round) {
+            const int col = round * B + lc;
+            const bool valid = col < N;
+            float2 v[E];
+#pragma unroll
+            for (int s = 0; s < E; ++s) v[s] = valid ? plane[(t + T * s) * N + col] : make_float2(0.f, 0.f);
+            float2* base = valid ? plane : scratch;
+            LdsMap map{valid ? col : lc, valid ? N : B, true, N};
+            line_fft<N, -1>(v, base, tw, t, map);
+#pragma unroll
+            for (int s = 0; s < E; ++s) {
+                const float2 g = gy[t + T * s];
+                v[s] = make_float2(fmaf(v[s].x, g.y, v[s].x * g.x), fmaf(v[s].y, g.y, v[s].y * g.x));
+            }
+            line_fft<N, +1>(v, base, tw, t, map);
+            __syncthreads();
+            if (valid) {
+#pragma unroll
+                for (int s = 0; s < E; ++s) plane[(t + T * s) * N + col] = v[s];
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < N * N; i += 256) DY.dst[off + i] = plane[i];
+}
+
 struct DensDesc {
     const float2* p;       // forward chain [Ntot][M]
     const float2* pd;      // complementary chain (== p when symmetric)
@@ -459,6 +546,7 @@ struct FtsBatch {
     std::vector<int> free_slots;
     int64_t iter = 0;
     bool inited = false;
+    bool plane_xy = true;            // x and y passes as one plane kernel where available
     std::vector<void*> allocs;
 };
 
@@ -793,15 +881,55 @@ static int launch_prop(const PropDesc* d, int n, const FtsGroup& G, int axis, hi
     return MFT_OK;
 }
 
+// whether the x and y passes of this group run as one plane kernel
+static bool plane_fused(const FtsGroup& G)
+{
+    if (G.n[0] != G.n[1]) return false;
+    switch (G.n[0]) {
+    case 16: case 20: case 24: case 32: case 40: case 48: case 64: return true;
+    default: return false;
+    }
+}
+
+template <int N>
+static void launch_prop_xy_n(const PropDesc* dx, const PropDesc* dy, int n, const FtsGroup& G, hipStream_t st)
+{
+    dim3 grid(G.n[2], n);
+    k_fts_prop_xy<N><<<grid, 256, 0, st>>>(dx, dy, fft_twiddles_device(N));
+}
+
+static int launch_prop_xy(const PropDesc* dx, const PropDesc* dy, int n, const FtsGroup& G, hipStream_t st)
+{
+    switch (G.n[0]) {
+    case 16: launch_prop_xy_n<16>(dx, dy, n, G, st); break;
+    case 20: launch_prop_xy_n<20>(dx, dy, n, G, st); break;
+    case 24: launch_prop_xy_n<24>(dx, dy, n, G, st); break;
+    case 32: launch_prop_xy_n<32>(dx, dy, n, G, st); break;
+    case 40: launch_prop_xy_n<40>(dx, dy, n, G, st); break;
+    case 48: launch_prop_xy_n<48>(dx, dy, n, G, st); break;
+    case 64: launch_prop_xy_n<64>(dx, dy, n, G, st); break;
+    default: set_error(MFT_ERR_UNSUPPORTED, "no plane kernel for %d", G.n[0]); return MFT_ERR_UNSUPPORTED;
+    }
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
 // propagators, densities and Q of every replica in the group
 static int group_density(FtsBatch* b, FtsGroup& G)
 {
     int rc;
     hipStream_t st = b->stream;
+    const bool fused = plane_fused(G) && b->plane_xy;
     for (int s = 1; s < G.maxN; ++s) {
         if (G.nprop[s] == 0) continue;
-        for (int a = 0; a < 3; ++a)
-            if ((rc = launch_prop(G.d_prop[(size_t)3 * s + a], G.nprop[s], G, a, st))) return rc;
+        if (fused) {
+            if ((rc = launch_prop_xy(G.d_prop[(size_t)3 * s], G.d_prop[(size_t)3 * s + 1], G.nprop[s], G, st)))
+                return rc;
+        } else {
+            for (int a = 0; a < 2; ++a)
+                if ((rc = launch_prop(G.d_prop[(size_t)3 * s + a], G.nprop[s], G, a, st))) return rc;
+        }
+        if ((rc = launch_prop(G.d_prop[(size_t)3 * s + 2], G.nprop[s], G, 2, st))) return rc;
     }
     const int M = G.n[0] * G.n[1] * G.n[2];
     dim3 grid((M + kFtsThreads - 1) / kFtsThreads, G.ndens);
@@ -839,6 +967,10 @@ static FtsGroup& group_for(FtsBatch* b, const mft_fts_desc& d)
 static int batch_create_empty(FtsBatch** out, int nslots)
 {
     FtsBatch* b = new FtsBatch();
+    // MFT_FTS_PLANE_XY=0 keeps the x and y passes as separate line kernels
+    // (same arithmetic; used to check that the plane kernel matches them)
+    const char* pe = std::getenv("MFT_FTS_PLANE_XY");
+    b->plane_xy = !(pe && pe[0] == '0');
     int rc = falloc(b, &b->d_iter, (size_t)nslots);
     if (!rc) {
         hipError_t e = hipMemset(b->d_iter, 0, sizeof(int64_t) * nslots);

@@ -298,7 +298,10 @@ static int grid_forces(TildEngine* e)
     AxisGeom gx{d.nx, d.ny, d.nz, 0}, gy{d.nx, d.ny, d.nz, 1};
     {
         TimedScope ts(e, MFT_TK_FFT_X_FWD);
-        if ((rc = tild_axis(-1, 1, 0, e->rho, e->S, gx, e->nchan, M, 1.0f, e->stream))) return rc;
+        const TileGeom& tg = e->tg;
+        if ((rc = tild_x_fwd_tiles(e->tiles, e->tile_tab, e->S, d.nx, d.ny, d.nz, e->nchan,
+                                   (long)tg.ex[0] * tg.ex[1] * tg.ex[2], e->stream)))
+            return rc;
     }
     {
         TimedScope ts(e, MFT_TK_FFT_Y_FWD);
@@ -566,6 +569,10 @@ int mft_tild_compute_forces(mft_tild* e, float* f)
 int mft_tild_get_densities(mft_tild* e, float* rho, float* rhoq)
 {
     MFT_REQUIRE(e, MFT_ERR_INVALID, "null argument");
+    if (!e->rho_valid) {
+        int rc = tild_density_reduce(e);
+        if (rc) return rc;
+    }
     MFT_HIP(hipStreamSynchronize(e->stream));
     const size_t M = e->M;
     if (rho) MFT_HIP(hipMemcpy(rho, e->rho, sizeof(float) * M * e->d.nspecies, hipMemcpyDeviceToHost));
@@ -696,7 +703,7 @@ int mft_tild_kernel_bytes(mft_tild* e, double* bytes)
     bytes[MFT_TK_SCATTER] = N * ((4 + 4 + 4 + 16 + 16 + 16 + 8 + 4 + 4 * mb) + (16 + 16 + 16 + 8 + 4 + 4 * mb + 4) +
                                  (4 + 4 + 8 * mb + 4 * mb));
     bytes[MFT_TK_DENSITY] = N * (16 + 8) + M * 4 * C;         // key-ordered records, rho
-    bytes[MFT_TK_FFT_X_FWD] = M * C * (4 + 8);
+    bytes[MFT_TK_FFT_X_FWD] = M * C * (4 + 8);              // densities in, spectra out (tiles summed on the fly)
     bytes[MFT_TK_FFT_Y_FWD] = M * C * 16;
     bytes[MFT_TK_CONV_Z] = M * 8 * (C + O);
     bytes[MFT_TK_FFT_Y_INV] = M * O * 16;

@@ -96,6 +96,78 @@ __global__ void __launch_bounds__(256) k_fft_x_inv_packed(const float2* __restri
     }
 }
 
+// First forward pass along x, reading the density tiles: each grid value is
+// the sum of the tile entries that cover it, in the fixed order of
+// k_tile_reduce (z, then y, then x contributions), so the densities are the
+// same bits as the reduced grid.  tab holds per axis and coordinate the count
+// and the offsets of the covering tile entries.
+template <int N>
+__global__ void __launch_bounds__(256) k_fft_x_fwd_tiles(const float* __restrict__ tiles, const int* __restrict__ tab,
+                                                         float2* __restrict__ out, int ny, int nz, long cstride,
+                                                         const float2* __restrict__ twg)
+{
+    using P = FftPlan<N>;
+    constexpr int T = P::T, E = P::E, B = 256 / T;
+    __shared__ float2 lds[B * N];
+    __shared__ float2 tw[N];
+    for (int i = threadIdx.x; i < N; i += 256) tw[i] = twg[i];
+    const int lline = threadIdx.x / T;
+    const int t = threadIdx.x % T;
+    const int line = blockIdx.x * B + lline;
+    const bool valid = line < ny * nz;
+    const int c = blockIdx.y;
+    const long M = (long)N * ny * nz;
+    float2 v[E];
+    if (valid) {
+        const int gy = line % ny, gz = line / ny;
+        const int* ty = tab + 5 * (N + gy);
+        const int* tz = tab + 5 * (N + ny + gz);
+        const int cy = ty[0], cz = tz[0];
+        const float* tc = tiles + c * cstride;
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const int* tx = tab + 5 * (t + T * s);
+            const int cx = tx[0];
+            float sum = 0.f;
+            for (int a = 0; a < cz; ++a)
+                for (int b = 0; b < cy; ++b) {
+                    const float* row = tc + tz[1 + a] + ty[1 + b];
+                    for (int d = 0; d < cx; ++d) sum += row[tx[1 + d]];
+                }
+            v[s] = make_float2(sum, 0.f);
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < E; ++s) v[s] = make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    LdsMap map{lline, B, false, N};
+    line_fft<N, -1>(v, lds, tw, t, map);
+    if (!valid) return;
+    float2* dst = out + c * M + (long)line * N;
+#pragma unroll
+    for (int s = 0; s < E; ++s) dst[t + T * s] = v[s];
+}
+
+int tild_x_fwd_tiles(const float* tiles, const int* tab, float2* S, int nx, int ny, int nz, int nchan, long cstride,
+                     hipStream_t st)
+{
+    switch (nx) {
+#define MFT_CASE(n_)                                                                                      \
+    case n_: {                                                                                            \
+        constexpr int B = 256 / FftPlan<n_>::T;                                                           \
+        dim3 grid((ny * nz + B - 1) / B, nchan);                                                          \
+        k_fft_x_fwd_tiles<n_><<<grid, 256, 0, st>>>(tiles, tab, S, ny, nz, cstride, fft_twiddles_device(n_)); \
+        break;                                                                                            \
+    }
+        MFT_FOR_EACH_FFT_SIZE(MFT_CASE)
+#undef MFT_CASE
+    default: return MFT_ERR_UNSUPPORTED;
+    }
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
 template <int N, int SIGN, bool RI, bool RO>
 static void launch_axis_n(const void* in, void* out, AxisGeom g, int nf, long fs, float scale, hipStream_t st)
 {

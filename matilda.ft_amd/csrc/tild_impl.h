@@ -163,7 +163,8 @@ struct TildEngine {
     int *count = nullptr, *start = nullptr, *partial = nullptr;
     int nscan_blocks = 0;
     // grids
-    float* rho = nullptr;   // [nchan][M]
+    float* rho = nullptr;   // [nchan][M], reduced from the tiles when read
+    bool rho_valid = false;
     float* tiles = nullptr; // [nchan][halo-expanded grid] density tiles
     int* tile_tab = nullptr; // per-axis tile offsets contributing to each grid coordinate
     float2* S = nullptr;    // [nchan][M]
@@ -215,6 +216,8 @@ static int dalloc(T** p, size_t n)
 // ---- grid passes (tild_grid.hip, tild_conv.hip) ----
 int tild_axis(int sign, int real_in, int real_out, const void* in, void* out, AxisGeom g, int nf, long fs,
               float scale, hipStream_t st);
+int tild_x_fwd_tiles(const float* tiles, const int* tab, float2* S, int nx, int ny, int nz, int nchan, long cstride,
+                     hipStream_t st);
 int tild_x_inv_packed(const float2* Y, float4* F4, int nx, int ny, int nz, int nfield, hipStream_t st);
 int tild_conv_blocks(int nx, int ny, int nz);
 int tild_conv(TildEngine* e);
@@ -225,7 +228,8 @@ int tild_sort_state(TildEngine* e);     // particle-id-order staging -> key orde
 int tild_unsort(TildEngine* e, int what); // key order -> particle-id-order staging
 int tild_scan(TildEngine* e);
 int tild_permute(TildEngine* e);         // next positions into the next key order
-int tild_density(TildEngine* e);
+int tild_density(TildEngine* e);          // tiles of the current positions
+int tild_density_reduce(TildEngine* e);   // tiles -> rho (when the densities are read)
 int tild_gather(TildEngine* e, int mode);
 int tild_integrate(TildEngine* e, bool want_f);
 int tild_bond_energy(TildEngine* e);

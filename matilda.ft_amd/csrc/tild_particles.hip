@@ -1049,10 +1049,19 @@ int tild_density(TildEngine* e)
 #undef MFT_DENS_B
 #undef MFT_DENS
     MFT_LAUNCH_CHECK();
+    e->rho_valid = false;
+    return MFT_OK;
+}
+
+int tild_density_reduce(TildEngine* e)
+{
+    const mft_tild_desc& d = e->d;
+    const TileGeom& tg = e->tg;
     const long cstride = (long)tg.ex[0] * tg.ex[1] * tg.ex[2];
     dim3 rgrid((d.nx + 63) / 64, d.ny, d.nz);
     k_tile_reduce<<<rgrid, 64, 0, e->stream>>>(e->rho, e->tiles, e->tile_tab, d.nx, d.ny, d.nz, e->nchan, cstride);
     MFT_LAUNCH_CHECK();
+    e->rho_valid = true;
     return MFT_OK;
 }
 

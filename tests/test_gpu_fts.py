@@ -101,12 +101,16 @@ def test_closed_forms_on_device(gpu):
     s.close()
 
 
-@pytest.mark.parametrize("case", ["diblock", "incompress_blend"])
+@pytest.mark.parametrize("case", ["diblock", "diblock_square", "incompress_blend"])
 def test_cl_steps_match_oracle(gpu, case):
     say("case: fts CL steps", case)
     if case == "diblock":
         spec, fields = systems.fts_diblock(grid_shape=(16, 24, 12), L=(3.0, 4.0, 2.5), Nr=20.0, NA=8, NB=12,
                                            chiN=15.0, kappaN=50.0, C=5.0, dt=0.01, sigma=0.1, seed=21)
+    elif case == "diblock_square":
+        # square x-y planes take the fused plane kernel
+        spec, fields = systems.fts_diblock(grid_shape=(20, 20, 12), L=(3.5, 3.5, 2.5), Nr=20.0, NA=8, NB=12,
+                                           chiN=15.0, kappaN=50.0, C=5.0, dt=0.01, sigma=0.1, seed=23)
     else:
         spec = tiny_spec(Nx=(12, 16, 20), L=(3.0, 4.0, 5.0), pots=[dict(kind="Incompress", dt_pl=0.01),
                                                                 dict(kind="Flory", I=0, J=1, chiN=10.0, dt_pl=0.01,
@@ -178,3 +182,26 @@ def test_config3_log_q_against_oracle(gpu):
     orc.fts_init(o32, bits=32)
     compare(s, o, spec, "C3 init", o32)
     s.close()
+
+
+@pytest.mark.parametrize("grid", [(16, 16, 12), (20, 20, 10), (24, 24, 16), (32, 32, 12), (40, 40, 8),
+                                  (48, 48, 8), (64, 64, 10)])
+def test_plane_kernel_matches_line_kernels(gpu, grid, monkeypatch):
+    """The fused x-y plane kernel against the separate x and y line kernels
+    (same arithmetic) over a few CL steps."""
+    say("case: fts plane kernel", grid)
+    spec, fields = systems.fts_diblock(grid_shape=grid, L=(0.12 * grid[0], 0.12 * grid[1], 0.12 * grid[2]),
+                                       Nr=20.0, NA=8, NB=12, chiN=15.0, kappaN=50.0, C=5.0, dt=0.01, sigma=0.1,
+                                       seed=31)
+    out = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("MFT_FTS_PLANE_XY", flag)
+        s = systems.fts_system(spec, fields)
+        s.step(3)
+        Q, H, _ = s.observables()
+        out.append((Q[0], H, s.densities()))
+        s.close()
+    (Q0, H0, r0), (Q1, H1, r1) = out
+    say(f"  bitwise equal: {np.array_equal(r0, r1)}, rel {rel(r1, r0):.2e}")
+    assert rel(r1, r0) < 1e-6
+    assert abs(np.log(Q1) - np.log(Q0)) < 1e-6 * max(1.0, abs(np.log(Q0)))
