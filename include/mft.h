@@ -183,6 +183,10 @@ enum {
     MFT_TK_INTEGRATE, MFT_TK_COUNT
 };
 int mft_tild_timing(mft_tild* eng, int enable, double* avg_ms, int64_t* launches);
+/* Which kernel groups timing brackets: bit k of mask = MFT_TK k (default all).
+ * Each bracket costs a few microseconds of stream time, so a run that times
+ * one kernel inside a throughput measurement selects only that kernel. */
+int mft_tild_timing_select(mft_tild* eng, uint32_t mask);
 /* Algorithmic HBM bytes per launch of each kernel id at this configuration. */
 int mft_tild_kernel_bytes(mft_tild* eng, double* bytes);
 int mft_tild_num_outputs(mft_tild* eng, int* nout);

@@ -254,7 +254,7 @@ struct TimedScope {
     long slot = -1;
     TimedScope(TildEngine* e_, int k_) : e(e_), k(k_)
     {
-        if (e->timing && 2 * (e->evused + 1) <= e->evpool.size()) {
+        if (e->timing && ((e->tk_mask >> k) & 1u) && 2 * (e->evused + 1) <= e->evpool.size()) {
             slot = (long)e->evused++;
             e->evkind[slot] = k;
             (void)hipEventRecord(e->evpool[2 * slot], e->stream);
@@ -685,6 +685,13 @@ int mft_tild_timing(mft_tild* e, int enable, double* avg_ms, int64_t* launches)
         if (avg_ms) avg_ms[k] = e->tk_n[k] ? e->tk_ms[k] / (double)e->tk_n[k] : 0.0;
         if (launches) launches[k] = e->tk_n[k];
     }
+    return MFT_OK;
+}
+
+int mft_tild_timing_select(mft_tild* e, uint32_t mask)
+{
+    MFT_REQUIRE(e, MFT_ERR_INVALID, "null argument");
+    e->tk_mask = mask;
     return MFT_OK;
 }
 

@@ -100,7 +100,9 @@ __global__ void __launch_bounds__(256) k_fft_x_inv_packed(const float2* __restri
 // the sum of the tile entries that cover it, in the fixed order of
 // k_tile_reduce (z, then y, then x contributions), so the densities are the
 // same bits as the reduced grid.  tab holds per axis and coordinate the count
-// and the offsets of the covering tile entries.
+// and the offsets of the covering tile entries.  The block's B lines are
+// summed into LDS first (consecutive lanes on consecutive x, so the tile rows
+// are read coalesced), then transformed from there.
 template <int N>
 __global__ void __launch_bounds__(256) k_fft_x_fwd_tiles(const float* __restrict__ tiles, const int* __restrict__ tab,
                                                          float2* __restrict__ out, int ny, int nz, long cstride,
@@ -110,41 +112,44 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd_tiles(const float* __restrict
     constexpr int T = P::T, E = P::E, B = 256 / T;
     __shared__ float2 lds[B * N];
     __shared__ float2 tw[N];
+    __shared__ int tabx[5 * N];
     for (int i = threadIdx.x; i < N; i += 256) tw[i] = twg[i];
-    const int lline = threadIdx.x / T;
-    const int t = threadIdx.x % T;
-    const int line = blockIdx.x * B + lline;
-    const bool valid = line < ny * nz;
+    for (int i = threadIdx.x; i < 5 * N; i += 256) tabx[i] = tab[i];
+    __syncthreads();
     const int c = blockIdx.y;
-    const long M = (long)N * ny * nz;
-    float2 v[E];
-    if (valid) {
-        const int gy = line % ny, gz = line / ny;
-        const int* ty = tab + 5 * (N + gy);
-        const int* tz = tab + 5 * (N + ny + gz);
-        const int cy = ty[0], cz = tz[0];
-        const float* tc = tiles + c * cstride;
-#pragma unroll
-        for (int s = 0; s < E; ++s) {
-            const int* tx = tab + 5 * (t + T * s);
-            const int cx = tx[0];
-            float sum = 0.f;
+    const int nlines = ny * nz;
+    const float* tc = tiles + c * cstride;
+    for (int i = threadIdx.x; i < B * N; i += 256) {
+        const int l = i / N, x = i - l * N;
+        const int line = blockIdx.x * B + l;
+        float sum = 0.f;
+        if (line < nlines) {
+            const int gy = line % ny, gz = line / ny;
+            const int* ty = tab + 5 * (N + gy);
+            const int* tz = tab + 5 * (N + ny + gz);
+            const int* tx = tabx + 5 * x;
+            const int cx = tx[0], cy = ty[0], cz = tz[0];
             for (int a = 0; a < cz; ++a)
                 for (int b = 0; b < cy; ++b) {
                     const float* row = tc + tz[1 + a] + ty[1 + b];
-                    for (int d = 0; d < cx; ++d) sum += row[tx[1 + d]];
-                }
-            v[s] = make_float2(sum, 0.f);
-        }
-    } else {
 #pragma unroll
-        for (int s = 0; s < E; ++s) v[s] = make_float2(0.f, 0.f);
+                    for (int d = 0; d < 4; ++d)
+                        if (d < cx) sum += row[tx[1 + d]];
+                }
+        }
+        lds[i] = make_float2(sum, 0.f);
     }
     __syncthreads();
+    const int lline = threadIdx.x / T;
+    const int t = threadIdx.x % T;
+    const int line = blockIdx.x * B + lline;
+    float2 v[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) v[s] = lds[lline * N + t + T * s];
     LdsMap map{lline, B, false, N};
     line_fft<N, -1>(v, lds, tw, t, map);
-    if (!valid) return;
-    float2* dst = out + c * M + (long)line * N;
+    if (line >= nlines) return;
+    float2* dst = out + c * (long)N * nlines + (long)line * N;
 #pragma unroll
     for (int s = 0; s < E; ++s) dst[t + T * s] = v[s];
 }

@@ -180,6 +180,7 @@ struct TildEngine {
     GjfParams gp;
     KParams kp;
     size_t dens_lds = 0;
+    uint32_t tk_mask = ~0u;      // kernel groups bracketed by timing events
     int gather_mode = 1;    // 0 direct, 1 LDS windows (mft_tild_autotune picks)
     int64_t step = 0;
     int64_t run_max = 0, run_pos = 0; // declared run (mft_tild_set_run)

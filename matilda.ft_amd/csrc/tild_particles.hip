@@ -279,29 +279,17 @@ __global__ void k_scan_reduce(const int* __restrict__ count, int M, int* __restr
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-__global__ void k_scan_partials(int* __restrict__ partial, int n)
-{
-    __shared__ int sh[16];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < n; base += blockDim.x) {
-        int i = base + threadIdx.x;
-        int v = i < n ? partial[i] : 0;
-        int tot;
-        int ex = block_exclusive_scan(v, sh, &tot);
-        if (i < n) partial[i] = ex + carry;
-        __syncthreads();
-        if (threadIdx.x == 0) carry += tot;
-        __syncthreads();
-    }
-}
-
 // start[i] = exclusive prefix of count; count cleared for the next binning.
+// Each block adds the totals of the blocks before it (at most a few thousand
+// values, read from L2), so the scan takes two launches.
 __global__ void k_scan_apply(int* __restrict__ count, int M, const int* __restrict__ partial,
                              int* __restrict__ start, int ns)
 {
     __shared__ int sh[16];
+    int c = 0;
+    for (int j = threadIdx.x; j < (int)blockIdx.x; j += blockDim.x) c += partial[j];
+    int carry;
+    block_exclusive_scan(c, sh, &carry);
     long base = (long)blockIdx.x * kScanChunk + (long)threadIdx.x * kScanItems;
     int v[kScanItems];
     int s = 0;
@@ -311,7 +299,7 @@ __global__ void k_scan_apply(int* __restrict__ count, int M, const int* __restri
         s += v[k];
     }
     int tot;
-    int ex = block_exclusive_scan(s, sh, &tot) + partial[blockIdx.x];
+    int ex = block_exclusive_scan(s, sh, &tot) + carry;
     for (int k = 0; k < kScanItems; ++k) {
         long i = base + k;
         if (i < M) {
@@ -346,6 +334,7 @@ __global__ void k_permute(int ns, int maxbonds, const int* __restrict__ key, con
     for (int b = 0; b < maxbonds; ++b) BT2[n * maxbonds + b] = BT[s * maxbonds + b];
 }
 
+// bond partner slots into the new order (after every map entry is written)
 __global__ void k_remap_bonds(int ns, int maxbonds, const int* __restrict__ map, const int* __restrict__ NB,
                               const int* __restrict__ BS, int* __restrict__ BS2)
 {
@@ -782,13 +771,70 @@ __device__ __forceinline__ void bond_force_slots(int s, int ns, const float4* __
     }
 }
 
+struct StepArgs {
+    const float4* XO;
+    const float4* NZ;
+    float4* XN;
+    float4* NN;
+    GjfParams gp;
+    int integ;
+    float delt;
+    float L[3];
+    uint64_t seed;
+    uint32_t step;
+    float dx[3];
+    int p, nx, ny, nz;
+};
+
+// GJF (ps_integratorGJF.cu:102-163) or EM (ps_integratorEM.cu:39-82) update
+// of slot s from its total force fo: writes the next position and noise and
+// returns the next key.
+__device__ __forceinline__ int integrate_slot(int s, float4 xc, int2 meta, const float* fo, const StepArgs& A,
+                                              const TileGeom& tg)
+{
+    float z[4];
+    normal4((uint32_t)meta.x, A.step, TAG_TILD, 0u, A.seed, z);
+    const float xcv[3] = {xc.x, xc.y, xc.z};
+    float xn[3], nn[3] = {0.f, 0.f, 0.f};
+    if (A.integ == MFT_INT_GJF) {
+        const int ty = meta.y;
+        const float a = A.gp.a[ty], b = A.gp.b[ty], nm = A.gp.nmag[ty], cf = A.gp.cf[ty], cn = A.gp.cn[ty];
+        const float4 xo4 = A.XO[s], no4 = A.NZ[s];
+        const float xov[3] = {xo4.x, xo4.y, xo4.z}, nov[3] = {no4.x, no4.y, no4.z};
+        for (int j = 0; j < 3; ++j) {
+            const float nnj = nm * z[j];
+            float xo = xov[j];
+            if (xo - xcv[j] > A.L[j] / 2.0f) xo -= A.L[j];
+            else if (xo - xcv[j] < -A.L[j] / 2.0f) xo += A.L[j];
+            float v = 2.0f * b * xcv[j] - a * xo + cf * fo[j] + cn * (nnj + nov[j]);
+            if (v > A.L[j]) v -= A.L[j];
+            else if (v < 0.0f) v += A.L[j];
+            xn[j] = v;
+            nn[j] = nnj;
+        }
+    } else {
+        const float nm = sqrtf(2.0f * A.delt);
+        for (int j = 0; j < 3; ++j) {
+            float v = xcv[j] + A.delt * fo[j] + nm * z[j];
+            if (v > A.L[j]) v -= A.L[j];
+            else if (v < 0.0f) v += A.L[j];
+            xn[j] = v;
+        }
+    }
+    A.XN[s] = make_float4(xn[0], xn[1], xn[2], 0.f);
+    A.NN[s] = make_float4(nn[0], nn[1], nn[2], 0.f);
+    return key_of(xn[0], xn[1], xn[2], meta.y, A.dx[0], A.dx[1], A.dx[2], A.p, A.nx, A.ny, A.nz, tg);
+}
+
+// Bonds (ps_deviceBonds.cu:7-79) plus the integrator on group "all", one
+// thread per slot of the key order.  Writes the next position and noise of
+// the slot and counts the next key; integ = MFT_INT_NONE only writes the
+// total force.
 __global__ void __launch_bounds__(256) k_integrate(
-    const float4* __restrict__ X, const float4* __restrict__ XO, const float4* __restrict__ NZ,
-    const int2* __restrict__ META, const int* __restrict__ NB, const int* __restrict__ BS, const int* __restrict__ BT,
-    const float4* __restrict__ fgrid, float4* __restrict__ ftot, float4* __restrict__ XN, float4* __restrict__ NN,
-    int maxbonds, BondParams bp, GjfParams gp, int integ, float delt, float L0, float L1, float L2, int ns,
-    uint64_t seed, uint32_t step, int* __restrict__ count, int* __restrict__ key, int* __restrict__ rank,
-    float dx0, float dx1, float dx2, int p, int nx, int ny, int nz, TileGeom tg)
+    const float4* __restrict__ X, const int2* __restrict__ META, const int* __restrict__ NB,
+    const int* __restrict__ BS, const int* __restrict__ BT, const float4* __restrict__ fgrid,
+    float4* __restrict__ ftot, int maxbonds, BondParams bp, StepArgs A, int ns, int* __restrict__ count,
+    int* __restrict__ key, int* __restrict__ rank, TileGeom tg)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = s < ns;
@@ -797,46 +843,11 @@ __global__ void __launch_bounds__(256) k_integrate(
         const float4 xc = X[s];
         const float4 fg = fgrid[s];
         float fo[3] = {fg.x, fg.y, fg.z};
-        bond_force_slots(s, ns, X, NB, BS, BT, maxbonds, bp, L0, L1, L2, xc.x, xc.y, xc.z, fo);
+        bond_force_slots(s, ns, X, NB, BS, BT, maxbonds, bp, A.L[0], A.L[1], A.L[2], xc.x, xc.y, xc.z, fo);
         if (ftot) ftot[s] = make_float4(fo[0], fo[1], fo[2], 0.f);
-        if (integ != MFT_INT_NONE) {
-            const int2 meta = META[s];
-            float z[4];
-            normal4((uint32_t)meta.x, step, TAG_TILD, 0u, seed, z);
-            const float Ls[3] = {L0, L1, L2};
-            const float xcv[3] = {xc.x, xc.y, xc.z};
-            float xn[3], nn[3] = {0.f, 0.f, 0.f};
-            if (integ == MFT_INT_GJF) {
-                const int ty = meta.y;
-                const float a = gp.a[ty], b = gp.b[ty], nm = gp.nmag[ty], cf = gp.cf[ty], cn = gp.cn[ty];
-                const float4 xo4 = XO[s], no4 = NZ[s];
-                const float xov[3] = {xo4.x, xo4.y, xo4.z}, nov[3] = {no4.x, no4.y, no4.z};
-                for (int j = 0; j < 3; ++j) {
-                    const float nnj = nm * z[j];
-                    float xo = xov[j];
-                    if (xo - xcv[j] > Ls[j] / 2.0f) xo -= Ls[j];
-                    else if (xo - xcv[j] < -Ls[j] / 2.0f) xo += Ls[j];
-                    float v = 2.0f * b * xcv[j] - a * xo + cf * fo[j] + cn * (nnj + nov[j]);
-                    if (v > Ls[j]) v -= Ls[j];
-                    else if (v < 0.0f) v += Ls[j];
-                    xn[j] = v;
-                    nn[j] = nnj;
-                }
-            } else {
-                const float nm = sqrtf(2.0f * delt);
-                for (int j = 0; j < 3; ++j) {
-                    float v = xcv[j] + delt * fo[j] + nm * z[j];
-                    if (v > Ls[j]) v -= Ls[j];
-                    else if (v < 0.0f) v += Ls[j];
-                    xn[j] = v;
-                }
-            }
-            XN[s] = make_float4(xn[0], xn[1], xn[2], 0.f);
-            NN[s] = make_float4(nn[0], nn[1], nn[2], 0.f);
-            k = key_of(xn[0], xn[1], xn[2], meta.y, dx0, dx1, dx2, p, nx, ny, nz, tg);
-        }
+        if (A.integ != MFT_INT_NONE) k = integrate_slot(s, xc, META[s], fo, A, tg);
     }
-    if (integ == MFT_INT_NONE) return; // uniform over the grid
+    if (A.integ == MFT_INT_NONE) return; // uniform over the grid
     const int r = wave_rank(count, k, active);
     if (active) {
         key[s] = k;
@@ -1007,7 +1018,6 @@ int tild_unsort(TildEngine* e, int what)
 int tild_scan(TildEngine* e)
 {
     k_scan_reduce<<<e->nscan_blocks, kScanThreads, 0, e->stream>>>(e->count, e->nkeys, e->partial);
-    k_scan_partials<<<1, 1024, 0, e->stream>>>(e->partial, e->nscan_blocks);
     k_scan_apply<<<e->nscan_blocks, kScanThreads, 0, e->stream>>>(e->count, e->nkeys, e->partial, e->start, e->N);
     MFT_LAUNCH_CHECK();
     return MFT_OK;
@@ -1114,15 +1124,38 @@ int tild_gather(TildEngine* e, int mode)
     return MFT_OK;
 }
 
+static StepArgs step_args(const TildEngine* e)
+{
+    const mft_tild_desc& d = e->d;
+    const int c = e->cur;
+    StepArgs A;
+    A.XO = e->XO[c];
+    A.NZ = e->NZ[c];
+    A.XN = e->XN;
+    A.NN = e->NN;
+    A.gp = e->gp;
+    A.integ = d.integrator;
+    A.delt = d.delt;
+    A.L[0] = d.lx;
+    A.L[1] = d.ly;
+    A.L[2] = d.lz;
+    A.seed = d.seed;
+    A.step = (uint32_t)e->step;
+    for (int j = 0; j < 3; ++j) A.dx[j] = e->dx[j];
+    A.p = d.pmeorder;
+    A.nx = d.nx;
+    A.ny = d.ny;
+    A.nz = d.nz;
+    return A;
+}
+
 int tild_integrate(TildEngine* e, bool want_f)
 {
     const mft_tild_desc& d = e->d;
     const int c = e->cur;
-    k_integrate<<<nblk(e->N), kBlock, 0, e->stream>>>(
-        e->X[c], e->XO[c], e->NZ[c], e->META[c], e->NB[c], e->BS[c], e->BT[c], e->fgrid, want_f ? e->ftot : nullptr,
-        e->XN, e->NN, std::max(1, d.maxbonds), e->bp, e->gp, d.integrator, d.delt, d.lx, d.ly, d.lz, e->N, d.seed,
-        (uint32_t)e->step, e->count, e->key, e->rank, e->dx[0], e->dx[1], e->dx[2], d.pmeorder, d.nx, d.ny, d.nz,
-        e->tg);
+    k_integrate<<<nblk(e->N), kBlock, 0, e->stream>>>(e->X[c], e->META[c], e->NB[c], e->BS[c], e->BT[c], e->fgrid,
+                                                       want_f ? e->ftot : nullptr, std::max(1, d.maxbonds), e->bp,
+                                                       step_args(e), e->N, e->count, e->key, e->rank, e->tg);
     MFT_LAUNCH_CHECK();
     return MFT_OK;
 }

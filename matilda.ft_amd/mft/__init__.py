@@ -338,6 +338,11 @@ class TildEngine:
         check(lib().mft_tild_timing(self.h, int(enable), _ptr(ms), _ptr(n)))
         return dict(zip(TK_NAMES, ms)), dict(zip(TK_NAMES, n))
 
+    def timing_select(self, names=None):
+        """Restrict timing events to the named kernel groups (None: all)."""
+        mask = 0xFFFFFFFF if names is None else sum(1 << TK_NAMES.index(n) for n in names)
+        check(lib().mft_tild_timing_select(self.h, C.c_uint32(mask)))
+
     def autotune(self, reps=5):
         c = _i(0)
         check(lib().mft_tild_autotune(self.h, reps, C.byref(c)))
