@@ -129,7 +129,8 @@ def fts_measure(iters_plain=20, iters_batch=5, nrep=256, seed=7):
 
 def serve_measure(seed=11, window=1.0, rate=50.0, tild_frac=0.2, max_iters=25, budget_frac=0.7, max_pool=16):
     """Served mixed workload: a Poisson stream (rate per s) of CL-FTS diblock
-    solves (grids 32^3-96^3, chi N in [8, 25], 500-5000 iterations, Ns = 50;
+    solves (grids 32^3-96^3, chi N in [8, 25], 500-5000 iterations, Ns = 50,
+    dt = 0.005;
     at most max_pool solves share the pool at a time)
     and TILD diblock runs (5k-500k particles at rho0 = 5, grids 32^3-128^3,
     200 steps) arriving over `window` seconds, served with continuous batching
@@ -153,11 +154,12 @@ def serve_measure(seed=11, window=1.0, rate=50.0, tild_frac=0.2, max_iters=25, b
             reqs.append((t, "tild", cfg, 200))
         else:
             g = int(rng.choice([32, 48, 64, 96]))
-            # box 0.10-0.16 Rg per grid point (the explicit CL step at dt = 0.01
-            # is unstable on much finer grids)
+            # box 0.10-0.16 Rg per grid point; explicit CL step dt = 0.005 (at
+            # chi N above ~19 a CL trajectory can still run away: such a
+            # request ends as failed, with the engine's message)
             spec, fields = systems.fts_diblock(grid_shape=(g, g, g), L=float(g * rng.uniform(0.10, 0.16)), Nr=50.0, NA=25,
                                                NB=25, chiN=float(rng.uniform(8, 25)), kappaN=100.0, C=50.0,
-                                               dt=0.01, sigma=0.1, seed=int(rng.integers(1 << 30)))
+                                               dt=0.005, sigma=0.1, seed=int(rng.integers(1 << 30)))
             reqs.append((t, "fts", (spec, fields), int(rng.uniform(500, 5000))))
     free = mft.mem_info()[0]
     srv = mft.Server(hbm_budget_bytes=budget_frac * free, max_fts_replicas=max_pool)

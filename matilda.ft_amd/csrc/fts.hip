@@ -71,15 +71,42 @@ struct FtsUpdDev {
     int have_rho;                  // densities exist (false before the first density pass)
 };
 
-// Register-array helpers with compile-time indices (runtime indices would
-// spill the arrays to scratch memory).
-template <typename T, int N>
-__device__ __forceinline__ T pick(const T (&a)[N], int k)
+// Register-array helpers with compile-time indices.  The selection is a
+// multiply by 0 or 1 rather than a branch: with a wave-uniform index the
+// compiler turns a select chain back into an indexed access to scratch
+// memory.  (Adding 0 leaves a finite value unchanged.)
+template <int N>
+__device__ __forceinline__ float pick(const float (&a)[N], int k)
 {
-    T v = a[0];
+    float v = 0.f;
 #pragma unroll
-    for (int j = 1; j < N; ++j)
-        if (j == k) v = a[j];
+    for (int j = 0; j < N; ++j) v = fmaf(j == k ? 1.f : 0.f, a[j], v);
+    return v;
+}
+
+template <int N>
+__device__ __forceinline__ double2 pick(const double2 (&a)[N], int k)
+{
+    double2 v = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const double m = j == k ? 1.0 : 0.0;
+        v.x = fma(m, a[j].x, v.x);
+        v.y = fma(m, a[j].y, v.y);
+    }
+    return v;
+}
+
+template <int N>
+__device__ __forceinline__ float2 pick(const float2 (&a)[N], int k)
+{
+    float2 v = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const float m = j == k ? 1.f : 0.f;
+        v.x = fmaf(m, a[j].x, v.x);
+        v.y = fmaf(m, a[j].y, v.y);
+    }
     return v;
 }
 
@@ -87,27 +114,30 @@ template <int N>
 __device__ __forceinline__ void add_at(double2 (&a)[N], int k, double x, double y)
 {
 #pragma unroll
-    for (int j = 0; j < N; ++j)
-        if (j == k) {
-            a[j].x += x;
-            a[j].y += y;
-        }
+    for (int j = 0; j < N; ++j) {
+        const double m = j == k ? 1.0 : 0.0;
+        a[j].x = fma(m, x, a[j].x);
+        a[j].y = fma(m, y, a[j].y);
+    }
 }
 
-// Elementwise update of one replica (blockIdx.y selects the replica).
+// Elementwise update of one replica (blockIdx.y selects the replica).  NS
+// and NP bound the species and potentials of every replica of the launch, so
+// the per-thread arrays stay in registers.
+template <int NS, int NP>
 __global__ void __launch_bounds__(kFtsThreads) k_fts_update(const FtsUpdDev* __restrict__ descs)
 {
     const FtsUpdDev& D = descs[blockIdx.y];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < D.M;
-    double hs[kFtsMaxPots * 6];
+    double hs[NP * 6];
 #pragma unroll
-    for (int k = 0; k < kFtsMaxPots * 6; ++k) hs[k] = 0.0;
+    for (int k = 0; k < NP * 6; ++k) hs[k] = 0.0;
     if (valid) {
         // species densities of the previous iteration
-        double2 rho[kFtsMaxSpecies];
+        double2 rho[NS];
 #pragma unroll
-        for (int s = 0; s < kFtsMaxSpecies; ++s) rho[s] = make_double2(0.0, 0.0);
+        for (int s = 0; s < NS; ++s) rho[s] = make_double2(0.0, 0.0);
         if (D.have_rho) {
             for (int b = 0; b < D.nblk; ++b) {
                 int m = D.blk_mol[b];
@@ -121,7 +151,7 @@ __global__ void __launch_bounds__(kFtsThreads) k_fts_update(const FtsUpdDev* __r
         }
         double2 rt = make_double2(0.0, 0.0);
 #pragma unroll
-        for (int s = 0; s < kFtsMaxSpecies; ++s) {
+        for (int s = 0; s < NS; ++s) {
             rt.x += rho[s].x;
             rt.y += rho[s].y;
         }
@@ -133,11 +163,11 @@ __global__ void __launch_bounds__(kFtsThreads) k_fts_update(const FtsUpdDev* __r
             normal4((uint32_t)i, (uint32_t)it, TAG_FTS, D.stream * 16u, D.seed, z);
             normal4((uint32_t)i, (uint32_t)it, TAG_FTS, D.stream * 16u + 1u, D.seed, z + 4);
         }
-        double2 W[kFtsMaxSpecies];
+        double2 W[NS];
 #pragma unroll
-        for (int s = 0; s < kFtsMaxSpecies; ++s) W[s] = make_double2(0.0, 0.0);
+        for (int s = 0; s < NS; ++s) W[s] = make_double2(0.0, 0.0);
 #pragma unroll
-        for (int p = 0; p < kFtsMaxPots; ++p) {
+        for (int p = 0; p < NP; ++p) {
             if (p >= D.npots) break;
             const FtsPotDev& P = D.pot[p];
             float2 wp = P.wpl[i];
@@ -193,7 +223,7 @@ __global__ void __launch_bounds__(kFtsThreads) k_fts_update(const FtsUpdDev* __r
                 }
                 // i w+ on every species
 #pragma unroll
-                for (int s = 0; s < kFtsMaxSpecies; ++s) {
+                for (int s = 0; s < NS; ++s) {
                     W[s].x -= w.y;
                     W[s].y += w.x;
                 }
@@ -204,9 +234,9 @@ __global__ void __launch_bounds__(kFtsThreads) k_fts_update(const FtsUpdDev* __r
             }
         }
         double inv = 1.0 / D.Nr;
-        float2 Es[kFtsMaxSpecies];
+        float2 Es[NS];
 #pragma unroll
-        for (int s = 0; s < kFtsMaxSpecies; ++s) {
+        for (int s = 0; s < NS; ++s) {
             Es[s] = make_float2(0.f, 0.f);
             if (s >= D.nspecies) continue;
             double wr = W[s].x * inv, wi = W[s].y * inv;
@@ -220,18 +250,18 @@ __global__ void __launch_bounds__(kFtsThreads) k_fts_update(const FtsUpdDev* __r
         for (int c = 0; c < D.nchain; ++c) D.chain_p0[c][i] = pick(Es, D.chain_species0[c]);
     }
     // Hamiltonian partial sums (fields after the update), FP64, fixed order
-    __shared__ double red[(kFtsThreads / 64) * kFtsMaxPots * 6];
+    __shared__ double red[(kFtsThreads / 64) * NP * 6];
 #pragma unroll
-    for (int k = 0; k < kFtsMaxPots * 6; ++k) {
+    for (int k = 0; k < NP * 6; ++k) {
         if (k >= D.npots * 6) break;
         double v = wave_sum(hs[k]);
-        if ((threadIdx.x & 63) == 0) red[(threadIdx.x >> 6) * kFtsMaxPots * 6 + k] = v;
+        if ((threadIdx.x & 63) == 0) red[(threadIdx.x >> 6) * NP * 6 + k] = v;
     }
     __syncthreads();
     const int nh = D.npots * 6;
     if ((int)threadIdx.x < nh) {
         double s = 0.0;
-        for (int w = 0; w < kFtsThreads / 64; ++w) s += red[w * kFtsMaxPots * 6 + threadIdx.x];
+        for (int w = 0; w < kFtsThreads / 64; ++w) s += red[w * NP * 6 + threadIdx.x];
         D.Hpart[(long)blockIdx.x * kFtsMaxPots * 6 + threadIdx.x] = s;
     }
 }
@@ -301,8 +331,9 @@ __global__ void __launch_bounds__(256) k_fts_prop_axis(const PropDesc* __restric
 // E_{b(s-1)}) is staged in LDS, every row is transformed along x, multiplied
 // by g_x and transformed back in place, then every column along y; the x and y
 // passes of k_fts_prop_axis with the plane kept in LDS between them (the same
-// arithmetic, one global read and write instead of two).  Square planes up to
-// 64 x 64; lines past the plane use a scratch region for their exchanges.
+// arithmetic, one global read and write instead of two).  Rows are padded by
+// PAD values so the rows a half-wave touches start on different LDS banks;
+// lines past the plane use a scratch region for their exchanges.
 template <int N>
 __global__ void __launch_bounds__(256) k_fts_prop_xy(const PropDesc* __restrict__ dxs,
                                                      const PropDesc* __restrict__ dys,
@@ -311,21 +342,45 @@ __global__ void __launch_bounds__(256) k_fts_prop_xy(const PropDesc* __restrict_
     using P = FftPlan<N>;
     constexpr int T = P::T, E = P::E, B = 256 / T;
     constexpr int NL = (N + B - 1) / B;
-    __shared__ float2 plane[N * N];
+    constexpr int PAD = (N % 16 == 0) ? 8 : 1, RS = N + PAD; // row stride in LDS
+    constexpr int NV = N * N / 2;                            // float4 (two values) per plane
+    constexpr int PER = (NV + 255) / 256;
+    __shared__ float2 plane[N * RS];
     __shared__ float2 scratch[(N % B) ? B * N : 1];
     __shared__ float2 tw[N], gx[N], gy[N];
     const PropDesc& DX = dxs[blockIdx.y];
     const PropDesc& DY = dys[blockIdx.y];
-    for (int i = threadIdx.x; i < N; i += 256) {
-        tw[i] = twg[i];
-        gx[i] = DX.g[i];
-        gy[i] = DY.g[i];
-    }
     const long off = (long)blockIdx.x * N * N;
-    for (int i = threadIdx.x; i < N * N; i += 256) {
-        float2 a = DX.src[off + i];
-        if (DX.pre) a = cmul(a, DX.pre[off + i]);
-        plane[i] = a;
+    {
+        // every load of the plane in flight before the first LDS store
+        const float4* src = reinterpret_cast<const float4*>(DX.src + off);
+        const float4* pre = DX.pre ? reinterpret_cast<const float4*>(DX.pre + off) : nullptr;
+        float4 a[PER], e[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int v = threadIdx.x + 256 * k;
+            const int vc = v < NV ? v : NV - 1;
+            a[k] = src[vc];
+            e[k] = pre ? pre[vc] : make_float4(1.f, 0.f, 1.f, 0.f);
+        }
+        for (int i = threadIdx.x; i < N; i += 256) {
+            tw[i] = twg[i];
+            gx[i] = DX.g[i];
+            gy[i] = DY.g[i];
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int v = threadIdx.x + 256 * k;
+            if (v >= NV) continue;
+            float2 lo = make_float2(a[k].x, a[k].y), hi = make_float2(a[k].z, a[k].w);
+            if (pre) {
+                lo = cmul(lo, make_float2(e[k].x, e[k].y));
+                hi = cmul(hi, make_float2(e[k].z, e[k].w));
+            }
+            const int i0 = 2 * v, r = i0 / N, c = i0 - r * N; // N even: both values on one row
+            plane[r * RS + c] = lo;
+            plane[r * RS + c + 1] = hi;
+        }
     }
     __syncthreads();
     // rows (x): T consecutive threads per row
@@ -336,9 +391,9 @@ __global__ void __launch_bounds__(256) k_fts_prop_xy(const PropDesc* __restrict_
             const bool valid = row < N;
             float2 v[E];
 #pragma unroll
-            for (int s = 0; s < E; ++s) v[s] = valid ? plane[row * N + t + T * s] : make_float2(0.f, 0.f);
+            for (int s = 0; s < E; ++s) v[s] = valid ? plane[row * RS + t + T * s] : make_float2(0.f, 0.f);
             float2* base = valid ? plane : scratch;
-            LdsMap map{valid ? row : lx, N, false, N};
+            LdsMap map{valid ? row : lx, N, false, valid ? RS : N};
             line_fft<N, -1>(v, base, tw, t, map);
 #pragma unroll
             for (int s = 0; s < E; ++s) {
@@ -349,12 +404,12 @@ __global__ void __launch_bounds__(256) k_fts_prop_xy(const PropDesc* __restrict_
             __syncthreads();
             if (valid) {
 #pragma unroll
-                for (int s = 0; s < E; ++s) plane[row * N + t + T * s] = v[s];
+                for (int s = 0; s < E; ++s) plane[row * RS + t + T * s] = v[s];
             }
             __syncthreads();
         }
     }
-    // columns (y): B consecutive threads cover B columns, element stride N
+    // columns (y): B consecutive threads cover B columns, element stride RS
     {
         const int lc = threadIdx.x % B, t = threadIdx.x / B;
         for (int round = 0; round < NL; ++round) {
@@ -362,9 +417,9 @@ __global__ void __launch_bounds__(256) k_fts_prop_xy(const PropDesc* __restrict_
             const bool valid = col < N;
             float2 v[E];
 #pragma unroll
-            for (int s = 0; s < E; ++s) v[s] = valid ? plane[(t + T * s) * N + col] : make_float2(0.f, 0.f);
+            for (int s = 0; s < E; ++s) v[s] = valid ? plane[(t + T * s) * RS + col] : make_float2(0.f, 0.f);
             float2* base = valid ? plane : scratch;
-            LdsMap map{valid ? col : lc, valid ? N : B, true, N};
+            LdsMap map{valid ? col : lc, valid ? RS : B, true, N};
             line_fft<N, -1>(v, base, tw, t, map);
 #pragma unroll
             for (int s = 0; s < E; ++s) {
@@ -375,12 +430,20 @@ __global__ void __launch_bounds__(256) k_fts_prop_xy(const PropDesc* __restrict_
             __syncthreads();
             if (valid) {
 #pragma unroll
-                for (int s = 0; s < E; ++s) plane[(t + T * s) * N + col] = v[s];
+                for (int s = 0; s < E; ++s) plane[(t + T * s) * RS + col] = v[s];
             }
             __syncthreads();
         }
     }
-    for (int i = threadIdx.x; i < N * N; i += 256) DY.dst[off + i] = plane[i];
+    float4* dst = reinterpret_cast<float4*>(DY.dst + off);
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int v = threadIdx.x + 256 * k;
+        if (v >= NV) continue;
+        const int i0 = 2 * v, r = i0 / N, c = i0 - r * N;
+        const float2 lo = plane[r * RS + c], hi = plane[r * RS + c + 1];
+        dst[v] = make_float4(lo.x, lo.y, hi.x, hi.y);
+    }
 }
 
 struct DensDesc {
@@ -547,6 +610,12 @@ struct FtsBatch {
     int64_t iter = 0;
     bool inited = false;
     bool plane_xy = true;            // x and y passes as one plane kernel where available
+    // one CL iteration captured as a HIP graph and replayed while the
+    // launches stay the same (gen counts changes of the groups' launches)
+    bool use_graph = true;
+    uint64_t gen = 0, graph_gen = ~0ull;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
     std::vector<void*> allocs;
 };
 
@@ -774,6 +843,7 @@ static int bead_species(const mft_fts_molecule& mol, int Ntot, int s, int c)
 static int group_setup(FtsBatch* b, FtsGroup& G)
 {
     int rc;
+    ++b->gen;
     free_all(G.allocs);
     G.d_prop.clear();
     G.nprop.clear();
@@ -943,8 +1013,19 @@ static int group_update(FtsBatch* b, FtsGroup& G, int first)
 {
     const int M = G.n[0] * G.n[1] * G.n[2];
     const int nr = (int)G.reps.size();
+    int ns = 1, np = 1;
+    for (int id : G.reps) {
+        ns = std::max(ns, b->reps[id].d.nspecies);
+        np = std::max(np, b->reps[id].d.npotentials);
+    }
+    const int NS = ns <= 2 ? 2 : (ns <= 4 ? 4 : 8), NP = np <= 2 ? 2 : (np <= 4 ? 4 : 8);
     dim3 grid((M + kFtsThreads - 1) / kFtsThreads, nr);
-    k_fts_update<<<grid, kFtsThreads, 0, b->stream>>>(G.d_upd + (first ? nr : 0));
+    const FtsUpdDev* d = G.d_upd + (first ? nr : 0);
+#define MFT_UPD(S_, P_)                                                                                   \
+    if (NS == S_ && NP == P_) k_fts_update<S_, P_><<<grid, kFtsThreads, 0, b->stream>>>(d);
+    MFT_UPD(2, 2) MFT_UPD(2, 4) MFT_UPD(2, 8) MFT_UPD(4, 2) MFT_UPD(4, 4) MFT_UPD(4, 8) MFT_UPD(8, 2)
+    MFT_UPD(8, 4) MFT_UPD(8, 8)
+#undef MFT_UPD
     MFT_LAUNCH_CHECK();
     return MFT_OK;
 }
@@ -971,6 +1052,10 @@ static int batch_create_empty(FtsBatch** out, int nslots)
     // (same arithmetic; used to check that the plane kernel matches them)
     const char* pe = std::getenv("MFT_FTS_PLANE_XY");
     b->plane_xy = !(pe && pe[0] == '0');
+    // MFT_FTS_GRAPH=0 launches every iteration directly (checks the replayed
+    // graph against direct launches)
+    const char* ge = std::getenv("MFT_FTS_GRAPH");
+    b->use_graph = !(ge && ge[0] == '0');
     int rc = falloc(b, &b->d_iter, (size_t)nslots);
     if (!rc) {
         hipError_t e = hipMemset(b->d_iter, 0, sizeof(int64_t) * nslots);
@@ -1052,6 +1137,7 @@ static int batch_activate(FtsBatch* b, const std::vector<int>& reps)
         G.reps.push_back(id);
         G.dirty = true;
     }
+    ++b->gen;
     return MFT_OK;
 }
 
@@ -1068,6 +1154,7 @@ static int batch_retire(FtsBatch* b, int id)
             G.dirty = true;
         }
     }
+    ++b->gen;
     free_all(r.allocs);
     r.alive = false;
     b->free_slots.push_back(r.slot);
@@ -1102,10 +1189,13 @@ static int batch_create(FtsBatch** out, const mft_fts_desc* descs, int nrep)
     return MFT_OK;
 }
 
+static void graph_drop(FtsBatch* b);
+
 static int batch_destroy(FtsBatch* b)
 {
     if (!b) return MFT_OK;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    graph_drop(b);
     for (auto& r : b->reps) free_all(r.allocs);
     for (auto& G : b->groups) free_all(G.allocs);
     free_all(b->allocs);
@@ -1133,6 +1223,63 @@ static int batch_init(FtsBatch* b)
     return MFT_OK;
 }
 
+// one CL iteration of every group: field updates from the previous
+// densities, propagators, densities and Q; then every slot's counter
+// advances (a slot is reset when a replica takes it)
+static int batch_iteration(FtsBatch* b)
+{
+    int rc;
+    for (auto& G : b->groups) {
+        if (G.reps.empty()) continue;
+        if ((rc = group_update(b, G, 0))) return rc;
+        if ((rc = group_density(b, G))) return rc;
+    }
+    for (int o = 0; o < b->nslots; o += 256) {
+        int cnt = std::min(256, b->nslots - o);
+        k_fts_iter_incr<<<1, 256, 0, b->stream>>>(b->d_iter + o, cnt);
+    }
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
+static void graph_drop(FtsBatch* b)
+{
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+}
+
+// Records one iteration (kernel launches only: no memset, copy or host
+// synchronisation happens inside it) into a graph.
+static int graph_capture(FtsBatch* b)
+{
+    hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        set_error(MFT_ERR_DEVICE, "begin capture: %s", hipGetErrorString(e));
+        return MFT_ERR_DEVICE;
+    }
+    const int rc = batch_iteration(b);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(b->stream, &g);
+    if (rc || e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        set_error(MFT_ERR_DEVICE, "capture of the iteration failed");
+        return MFT_ERR_DEVICE;
+    }
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        set_error(MFT_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(e));
+        return MFT_ERR_DEVICE;
+    }
+    b->graph = g;
+    b->gexec = x;
+    return MFT_OK;
+}
+
 static int batch_step(FtsBatch* b, int niter)
 {
     MFT_REQUIRE(b->inited, MFT_ERR_INVALID, "call init before stepping");
@@ -1140,17 +1287,24 @@ static int batch_step(FtsBatch* b, int niter)
     for (auto& G : b->groups)
         if (!G.reps.empty() && (rc = group_ready(b, G))) return rc;
     for (int it = 0; it < niter; ++it) {
-        for (auto& G : b->groups) {
-            if (G.reps.empty()) continue;
-            if ((rc = group_update(b, G, 0))) return rc;
-            if ((rc = group_density(b, G))) return rc;
+        if (b->use_graph && b->graph_gen != b->gen) {
+            // the launches changed: run this iteration directly (it also sets
+            // up anything created on first use) and capture the next one
+            graph_drop(b);
+            b->graph_gen = b->gen;
+            if ((rc = batch_iteration(b))) return rc;
+        } else if (b->use_graph) {
+            if (!b->gexec && graph_capture(b)) {
+                // no graph on this device or runtime: launch directly from now on
+                b->use_graph = false;
+                if ((rc = batch_iteration(b))) return rc;
+                b->iter++;
+                continue;
+            }
+            MFT_HIP(hipGraphLaunch(b->gexec, b->stream));
+        } else if ((rc = batch_iteration(b))) {
+            return rc;
         }
-        // every slot's counter advances; a slot is reset when a replica takes it
-        for (int o = 0; o < b->nslots; o += 256) {
-            int cnt = std::min(256, b->nslots - o);
-            k_fts_iter_incr<<<1, 256, 0, b->stream>>>(b->d_iter + o, cnt);
-        }
-        MFT_LAUNCH_CHECK();
         b->iter++;
     }
     return MFT_OK;

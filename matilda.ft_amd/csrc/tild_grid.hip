@@ -125,17 +125,7 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd_tiles(const float* __restrict
         float sum = 0.f;
         if (line < nlines) {
             const int gy = line % ny, gz = line / ny;
-            const int* ty = tab + 5 * (N + gy);
-            const int* tz = tab + 5 * (N + ny + gz);
-            const int* tx = tabx + 5 * x;
-            const int cx = tx[0], cy = ty[0], cz = tz[0];
-            for (int a = 0; a < cz; ++a)
-                for (int b = 0; b < cy; ++b) {
-                    const float* row = tc + tz[1 + a] + ty[1 + b];
-#pragma unroll
-                    for (int d = 0; d < 4; ++d)
-                        if (d < cx) sum += row[tx[1 + d]];
-                }
+            sum = tile_sum(tc, tab + 5 * (N + ny + gz), tab + 5 * (N + gy), tabx + 5 * x);
         }
         lds[i] = make_float2(sum, 0.f);
     }

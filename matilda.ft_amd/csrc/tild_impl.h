@@ -116,6 +116,43 @@ __device__ __host__ __forceinline__ int tile_key(int g0, int g1, int g2, int s, 
 
 // Workgroup -> brick: each of the 8 XCDs (round-robin dispatch) gets a
 // contiguous slab of bricks, so tiles that share halos meet in one L2.
+// Density at one grid point from the halo tiles that cover it: tz, ty, tx
+// are the point's rows of the cover table (count, then up to 4 offsets into
+// the halo-expanded tile grid) along z, y and x; the entries are added in
+// the fixed order z, then y, then x.  Points covered at most twice along
+// every axis (all but tiny grids) issue their loads together.
+__device__ __forceinline__ float tile_sum(const float* __restrict__ tc, const int* tz, const int* ty, const int* tx)
+{
+    const int cz = tz[0], cy = ty[0], cx = tx[0];
+    float s = 0.f;
+    if (cz <= 2 && cy <= 2 && cx <= 2) {
+        float v[8];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int d = 0; d < 2; ++d) {
+                    const bool in = a < cz && b < cy && d < cx;
+                    v[4 * a + 2 * b + d] = in ? tc[tz[1 + a] + ty[1 + b] + tx[1 + d]] : 0.f;
+                }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int d = 0; d < 2; ++d)
+                    if (a < cz && b < cy && d < cx) s += v[4 * a + 2 * b + d];
+        return s;
+    }
+    for (int a = 0; a < cz; ++a)
+        for (int b = 0; b < cy; ++b) {
+            const float* row = tc + tz[1 + a] + ty[1 + b];
+            for (int d = 0; d < cx; ++d) s += row[tx[1 + d]];
+        }
+    return s;
+}
+
 __device__ __forceinline__ int brick_of_block(int blk, int nbricks)
 {
     const int per = (nbricks + 7) >> 3;

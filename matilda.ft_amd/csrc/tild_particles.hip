@@ -490,23 +490,8 @@ __global__ void __launch_bounds__(64) k_tile_reduce(float* __restrict__ rho, con
     const int* tx = tab + 5 * gx;
     const int* ty = tab + 5 * (nx + gy);
     const int* tz = tab + 5 * (nx + ny + gz);
-    const int cx = tx[0], cy = ty[0], cz = tz[0];
-    int ox[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) ox[c] = c < cx ? tx[1 + c] : 0;
     const long id = gx + (long)nx * (gy + (long)ny * gz);
-    for (int ch = 0; ch < nchan; ++ch) {
-        const float* tc = tiles + ch * cstride;
-        float s = 0.f;
-        for (int a = 0; a < cz; ++a)
-            for (int b = 0; b < cy; ++b) {
-                const float* row = tc + tz[1 + a] + ty[1 + b];
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    if (c < cx) s += row[ox[c]];
-            }
-        rho[ch * M + id] = s;
-    }
+    for (int ch = 0; ch < nchan; ++ch) rho[ch * M + id] = tile_sum(tiles + ch * cstride, tz, ty, tx);
 }
 
 // ------------------------------------------------------------- gather ---
