@@ -127,6 +127,31 @@ def fts_measure(iters_plain=20, iters_batch=5, nrep=256, seed=7):
     return out
 
 
+def fts_profile(which, iters=20):
+    """A few CL iterations of one FTS workload (2 untimed first), for a
+    profiler: the plain 64^3, Ns = 100 solve or the 256-replica sweep."""
+    import mft
+    from mft import systems
+    if which == "plain":
+        spec, fields = systems.fts_diblock()
+        s = systems.fts_system(spec, fields)
+    else:
+        reps = systems.fts_sweep_replicas(256, seed=7)
+        s = mft.FtsBatch([r[0] for r in reps])
+        for k, (spec_r, fields_r) in enumerate(reps):
+            systems.load_fields(s.set_fields, spec_r, fields_r, rep=k)
+        s.init()
+        iters = 5
+    s.step(2)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(iters)
+    s.synchronize()
+    dt = (time.perf_counter() - t0) / iters
+    print(json.dumps({"fts_profile": which, "iterations": iters, "ms_per_iteration": 1e3 * dt}), flush=True)
+    s.close()
+
+
 def serve_measure(seed=11, window=1.0, rate=50.0, tild_frac=0.2, max_iters=25, budget_frac=0.7, max_pool=16):
     """Served mixed workload: a Poisson stream (rate per s) of CL-FTS diblock
     solves (grids 32^3-96^3, chi N in [8, 25], 500-5000 iterations, Ns = 50,
@@ -215,7 +240,13 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run 5 device-to-device copies of BYTES (the known-size kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--fts-profile", choices=["plain", "sweep"], default=None,
+                    help="only run a few CL iterations of the plain 64^3 solve or of the 256-replica sweep "
+                         "(for profilers) and exit")
     args = ap.parse_args()
+    if args.fts_profile:
+        fts_profile(args.fts_profile)
+        return
     if args.calibrate_copy:
         import ctypes
         import mft

@@ -40,11 +40,15 @@ constexpr int kFtsThreads = 256;
 
 struct FtsPotDev {
     int kind, I, J;
+    int scheme;               // MFT_FTS_EM or MFT_FTS_EMPC
     double cpl, cmi;          // force coefficients of w+ and w-
     double dt_pl, dt_mi;
     double nmag_pl, nmag_mi;  // sqrt(2 dt / gvol) or 0 (no noise)
     int col_pl, col_mi;       // noise normal index or -1
     float2 *wpl, *wmi;
+    // EMPC predictor data: fields before the predictor and its forces
+    float2 *wpl_o, *wmi_o;
+    double2 *fpl_o, *fmi_o;
 };
 
 struct FtsUpdDev {
@@ -68,6 +72,7 @@ struct FtsUpdDev {
     const int64_t* iter;           // device counter (CL iteration)
     int cl;
     int first;                     // initialisation pass: no field update
+    int corrector;                 // EMPC corrector pass (fts_potential_flory.cu:198-236)
     int have_rho;                  // densities exist (false before the first density pass)
 };
 
@@ -175,20 +180,41 @@ __global__ void __launch_bounds__(kFtsThreads) k_fts_update(const FtsUpdDev* __r
             if (P.kind == MFT_FTS_FLORY) {
                 float2 wm = P.wmi[i];
                 double2 v = make_double2(wm.x, wm.y);
-                if (!D.first) {
+                if (!D.first && (!D.corrector || P.scheme == MFT_FTS_EMPC)) {
                     // d_makeFloryForce (fts_potential_flory.cu:279-304)
                     double2 rI = pick(rho, P.I), rJ = pick(rho, P.J);
                     double fpx = P.cpl * w.x - (rI.y + rJ.y) / D.Nr;
                     double fpy = P.cpl * w.y + (rI.x + rJ.x) / D.Nr;
                     double fmx = P.cmi * v.x + (rJ.x - rI.x) / D.Nr;
                     double fmy = P.cmi * v.y + (rJ.y - rI.y) / D.Nr;
-                    // d_fts_updateEM (fts_updateSchemes.cu:8-29), real noise
-                    w.x -= P.dt_pl * fpx;
-                    w.y -= P.dt_pl * fpy;
-                    v.x -= P.dt_mi * fmx;
-                    v.y -= P.dt_mi * fmy;
-                    if (P.col_pl >= 0) w.x += P.nmag_pl * (double)pick(z, P.col_pl);
-                    if (P.col_mi >= 0) v.x += P.nmag_mi * (double)pick(z, P.col_mi);
+                    const double npl = P.col_pl >= 0 ? P.nmag_pl * (double)pick(z, P.col_pl) : 0.0;
+                    const double nmi = P.col_mi >= 0 ? P.nmag_mi * (double)pick(z, P.col_mi) : 0.0;
+                    if (!D.corrector) {
+                        if (P.scheme == MFT_FTS_EMPC) {
+                            // storePredictorData (fts_potential_flory.cu:243-249)
+                            P.wpl_o[i] = wp;
+                            P.wmi_o[i] = wm;
+                            P.fpl_o[i] = make_double2(fpx, fpy);
+                            P.fmi_o[i] = make_double2(fmx, fmy);
+                        }
+                        // d_fts_updateEM (fts_updateSchemes.cu:8-29), real noise
+                        w.x -= P.dt_pl * fpx;
+                        w.y -= P.dt_pl * fpy;
+                        v.x -= P.dt_mi * fmx;
+                        v.y -= P.dt_mi * fmy;
+                        w.x += npl;
+                        v.x += nmi;
+                    } else {
+                        // d_fts_updateEMPC (fts_updateSchemes.cu:56-81): the
+                        // predictor's fields, forces and noise
+                        const float2 wo = P.wpl_o[i], vo = P.wmi_o[i];
+                        const double2 fo = P.fpl_o[i], go = P.fmi_o[i];
+                        const double hp = 0.5 * P.dt_pl, hm = 0.5 * P.dt_mi;
+                        w.x = (double)wo.x - hp * (fpx + fo.x) + npl;
+                        w.y = (double)wo.y - hp * (fpy + fo.y);
+                        v.x = (double)vo.x - hm * (fmx + go.x) + nmi;
+                        v.y = (double)vo.y - hm * (fmy + go.y);
+                    }
                     float2 wf = make_float2((float)w.x, (float)w.y), vf = make_float2((float)v.x, (float)v.y);
                     P.wpl[i] = wf;
                     P.wmi[i] = vf;
@@ -203,7 +229,7 @@ __global__ void __launch_bounds__(kFtsThreads) k_fts_update(const FtsUpdDev* __r
                 hs[p * 6 + 4] += v.x * v.x - v.y * v.y;
                 hs[p * 6 + 5] += 2.0 * v.x * v.y;
             } else {
-                if (!D.first) {
+                if (!D.first && (!D.corrector || P.scheme == MFT_FTS_EMPC)) {
                     double fx, fy;
                     if (P.kind == MFT_FTS_HELFAND) {
                         // d_makeHelfandForce (fts_potential_helfand.cu:223-240)
@@ -214,9 +240,22 @@ __global__ void __launch_bounds__(kFtsThreads) k_fts_update(const FtsUpdDev* __r
                         fx = -rt.y / D.Nr;
                         fy = rt.x / D.Nr - D.C;
                     }
-                    w.x -= P.dt_pl * fx;
-                    w.y -= P.dt_pl * fy;
-                    if (P.col_pl >= 0) w.x += P.nmag_pl * (double)pick(z, P.col_pl);
+                    const double npl = P.col_pl >= 0 ? P.nmag_pl * (double)pick(z, P.col_pl) : 0.0;
+                    if (!D.corrector) {
+                        if (P.scheme == MFT_FTS_EMPC) {
+                            P.wpl_o[i] = wp;
+                            P.fpl_o[i] = make_double2(fx, fy);
+                        }
+                        w.x -= P.dt_pl * fx;
+                        w.y -= P.dt_pl * fy;
+                        w.x += npl;
+                    } else {
+                        const float2 wo = P.wpl_o[i];
+                        const double2 fo = P.fpl_o[i];
+                        const double hp = 0.5 * P.dt_pl;
+                        w.x = (double)wo.x - hp * (fx + fo.x) + npl;
+                        w.y = (double)wo.y - hp * (fy + fo.y);
+                    }
                     float2 wf = make_float2((float)w.x, (float)w.y);
                     P.wpl[i] = wf;
                     w = make_double2(wf.x, wf.y);
@@ -569,6 +608,10 @@ struct FtsRep {
     double nmolecs[kFtsMaxMol] = {0};
     int nblk = 0;
     float2 *wpl[kFtsMaxPots] = {nullptr}, *wmi[kFtsMaxPots] = {nullptr};
+    // EMPC predictor data per potential (fields before the predictor, forces)
+    float2 *wpl_o[kFtsMaxPots] = {nullptr}, *wmi_o[kFtsMaxPots] = {nullptr};
+    double2 *fpl_o[kFtsMaxPots] = {nullptr}, *fmi_o[kFtsMaxPots] = {nullptr};
+    bool pc = false;              // some potential uses the predictor-corrector
     float2 *W = nullptr, *E = nullptr, *rhot = nullptr;
     float2 *p[kFtsMaxMol] = {nullptr}, *pd[kFtsMaxMol] = {nullptr};
     float2* tmp = nullptr; // [2*nmol][M]
@@ -583,6 +626,7 @@ struct FtsRep {
 
 struct FtsGroup {
     int n[3];
+    bool pc = false;                 // predictor-corrector replicas (two passes per iteration)
     std::vector<int> reps;
     FtsUpdDev* d_upd = nullptr;
     DensDesc* d_dens = nullptr;
@@ -662,6 +706,8 @@ static int validate_desc(const mft_fts_desc& d)
         const mft_fts_potential& P = d.potentials[p];
         MFT_REQUIRE(P.kind >= 0 && P.kind <= MFT_FTS_INCOMPRESS, MFT_ERR_UNSUPPORTED, "FTS potential kind %d",
                     P.kind);
+        MFT_REQUIRE(P.scheme == MFT_FTS_EM || P.scheme == MFT_FTS_EMPC, MFT_ERR_UNSUPPORTED,
+                    "update scheme %d (EM and EMPC are built)", P.scheme);
         if (P.kind == MFT_FTS_FLORY) {
             MFT_REQUIRE(P.I >= 0 && P.I < d.nspecies && P.J >= 0 && P.J < d.nspecies && P.I != P.J, MFT_ERR_INVALID,
                         "Flory species out of range");
@@ -716,6 +762,15 @@ static int rep_setup(FtsBatch* b, FtsRep& r)
             if ((rc = dev_alloc(r.allocs, &r.wmi[p], M))) return rc;
             MFT_HIP(hipMemset(r.wmi[p], 0, sizeof(float2) * M));
         }
+        if (d.potentials[p].scheme == MFT_FTS_EMPC) {
+            r.pc = true;
+            if ((rc = dev_alloc(r.allocs, &r.wpl_o[p], M))) return rc;
+            if ((rc = dev_alloc(r.allocs, &r.fpl_o[p], M))) return rc;
+            if (d.potentials[p].kind == MFT_FTS_FLORY) {
+                if ((rc = dev_alloc(r.allocs, &r.wmi_o[p], M))) return rc;
+                if ((rc = dev_alloc(r.allocs, &r.fmi_o[p], M))) return rc;
+            }
+        }
     }
     if ((rc = dev_alloc(r.allocs, &r.W, M * d.nspecies))) return rc;
     if ((rc = dev_alloc(r.allocs, &r.E, M * d.nspecies))) return rc;
@@ -758,10 +813,11 @@ static int rep_setup(FtsBatch* b, FtsRep& r)
     return MFT_OK;
 }
 
-static void fill_upd(const FtsBatch* b, const FtsRep& r, FtsUpdDev& U, int first, int have_rho)
+static void fill_upd(const FtsBatch* b, const FtsRep& r, FtsUpdDev& U, int first, int have_rho, int corrector = 0)
 {
     const mft_fts_desc& d = r.d;
     std::memset(&U, 0, sizeof(U));
+    U.corrector = corrector;
     U.M = r.M;
     U.nspecies = d.nspecies;
     U.npots = d.npotentials;
@@ -774,6 +830,11 @@ static void fill_upd(const FtsBatch* b, const FtsRep& r, FtsUpdDev& U, int first
         D.J = P.J;
         D.wpl = r.wpl[p];
         D.wmi = r.wmi[p];
+        D.scheme = P.scheme;
+        D.wpl_o = r.wpl_o[p];
+        D.wmi_o = r.wmi_o[p];
+        D.fpl_o = r.fpl_o[p];
+        D.fmi_o = r.fmi_o[p];
         D.dt_pl = P.delt_pl;
         D.dt_mi = P.delt_mi;
         D.col_pl = D.col_mi = -1;
@@ -849,11 +910,13 @@ static int group_setup(FtsBatch* b, FtsGroup& G)
     G.nprop.clear();
     G.dirty = false;
     const int nr = (int)G.reps.size();
-    // update descriptors: [0, nr) regular, [nr, 2nr) initial pass
-    std::vector<FtsUpdDev> upd(2 * nr);
+    // update descriptors: [0, nr) regular (the predictor with EMPC),
+    // [nr, 2nr) initial pass, [2nr, 3nr) corrector
+    std::vector<FtsUpdDev> upd(3 * nr);
     for (int k = 0; k < nr; ++k) {
         fill_upd(b, b->reps[G.reps[k]], upd[k], 0, 1);
         fill_upd(b, b->reps[G.reps[k]], upd[nr + k], 1, 0);
+        fill_upd(b, b->reps[G.reps[k]], upd[2 * nr + k], 0, 1, 1);
     }
     if ((rc = dev_alloc(G.allocs, &G.d_upd, upd.size()))) return rc;
     MFT_HIP(hipMemcpy(G.d_upd, upd.data(), sizeof(FtsUpdDev) * upd.size(), hipMemcpyHostToDevice));
@@ -1009,6 +1072,8 @@ static int group_density(FtsBatch* b, FtsGroup& G)
     return MFT_OK;
 }
 
+// pass 0: the iteration's update (the predictor with EMPC), 1: the initial
+// pass (no update), 2: the EMPC corrector
 static int group_update(FtsBatch* b, FtsGroup& G, int first)
 {
     const int M = G.n[0] * G.n[1] * G.n[2];
@@ -1020,7 +1085,7 @@ static int group_update(FtsBatch* b, FtsGroup& G, int first)
     }
     const int NS = ns <= 2 ? 2 : (ns <= 4 ? 4 : 8), NP = np <= 2 ? 2 : (np <= 4 ? 4 : 8);
     dim3 grid((M + kFtsThreads - 1) / kFtsThreads, nr);
-    const FtsUpdDev* d = G.d_upd + (first ? nr : 0);
+    const FtsUpdDev* d = G.d_upd + (size_t)first * nr;
 #define MFT_UPD(S_, P_)                                                                                   \
     if (NS == S_ && NP == P_) k_fts_update<S_, P_><<<grid, kFtsThreads, 0, b->stream>>>(d);
     MFT_UPD(2, 2) MFT_UPD(2, 4) MFT_UPD(2, 8) MFT_UPD(4, 2) MFT_UPD(4, 4) MFT_UPD(4, 8) MFT_UPD(8, 2)
@@ -1032,14 +1097,16 @@ static int group_update(FtsBatch* b, FtsGroup& G, int first)
 
 static int batch_destroy(FtsBatch* b);
 
-static FtsGroup& group_for(FtsBatch* b, const mft_fts_desc& d)
+static FtsGroup& group_for(FtsBatch* b, const FtsRep& r)
 {
+    const mft_fts_desc& d = r.d;
     for (auto& G : b->groups)
-        if (G.n[0] == d.nx && G.n[1] == d.ny && G.n[2] == d.nz) return G;
+        if (G.n[0] == d.nx && G.n[1] == d.ny && G.n[2] == d.nz && G.pc == r.pc) return G;
     FtsGroup G;
     G.n[0] = d.nx;
     G.n[1] = d.ny;
     G.n[2] = d.nz;
+    G.pc = r.pc;
     b->groups.push_back(G);
     return b->groups.back();
 }
@@ -1109,13 +1176,14 @@ static int batch_add(FtsBatch* b, const mft_fts_desc& d, int* rep_out)
 static int batch_activate(FtsBatch* b, const std::vector<int>& reps)
 {
     int rc = MFT_OK;
-    std::map<std::array<int, 3>, FtsGroup> tmp;
+    std::map<std::array<int, 4>, FtsGroup> tmp;
     for (int id : reps) {
         const mft_fts_desc& d = b->reps[id].d;
-        FtsGroup& T = tmp[std::array<int, 3>{d.nx, d.ny, d.nz}];
+        FtsGroup& T = tmp[std::array<int, 4>{d.nx, d.ny, d.nz, b->reps[id].pc ? 1 : 0}];
         T.n[0] = d.nx;
         T.n[1] = d.ny;
         T.n[2] = d.nz;
+        T.pc = b->reps[id].pc;
         T.reps.push_back(id);
     }
     for (auto& kv : tmp) {
@@ -1133,7 +1201,7 @@ static int batch_activate(FtsBatch* b, const std::vector<int>& reps)
         return MFT_ERR_DEVICE;
     }
     for (int id : reps) {
-        FtsGroup& G = group_for(b, b->reps[id].d);
+        FtsGroup& G = group_for(b, b->reps[id]);
         G.reps.push_back(id);
         G.dirty = true;
     }
@@ -1176,7 +1244,7 @@ static int batch_create(FtsBatch** out, const mft_fts_desc* descs, int nrep)
         int id = -1;
         rc = batch_add(b, descs[r], &id);
         if (!rc) {
-            FtsGroup& G = group_for(b, descs[r]);
+            FtsGroup& G = group_for(b, b->reps[id]);
             G.reps.push_back(id);
             G.dirty = true;
         }
@@ -1233,6 +1301,11 @@ static int batch_iteration(FtsBatch* b)
         if (G.reps.empty()) continue;
         if ((rc = group_update(b, G, 0))) return rc;
         if ((rc = group_density(b, G))) return rc;
+        if (G.pc) {
+            // FTS_Box.cu:53-81: correct, rebuild species fields, densities again
+            if ((rc = group_update(b, G, 2))) return rc;
+            if ((rc = group_density(b, G))) return rc;
+        }
     }
     for (int o = 0; o < b->nslots; o += 256) {
         int cnt = std::min(256, b->nslots - o);

@@ -57,7 +57,8 @@ void FTS_Potential::parse_options(std::istringstream& iss, std::vector<std::comp
             }
         } else if (s1 == "updateScheme") {
             iss >> updateScheme;
-            if (updateScheme != "EM") die("updateScheme " + updateScheme + " is not part of this build (EM)");
+            if (updateScheme != "EM" && updateScheme != "EMPC")
+                die("updateScheme " + updateScheme + " is not part of this build (EM, EMPC)");
         } else if (s1 == "modify") {
             iss >> s1;
             if (s1 != "zeromean" && s1 != "zeroMean") die("Invalid modify option on " + potentialStyle + " potential");
@@ -82,6 +83,7 @@ mft_fts_potential PotentialHelfand::descriptor() const
 {
     mft_fts_potential d{};
     d.kind = MFT_FTS_HELFAND;
+    d.scheme = updateScheme == "EMPC" ? MFT_FTS_EMPC : MFT_FTS_EM;
     d.kappaN = kappaN;
     d.delt_pl = delt;
     return d;
@@ -105,6 +107,7 @@ mft_fts_potential PotentialFlory::descriptor() const
 {
     mft_fts_potential d{};
     d.kind = MFT_FTS_FLORY;
+    d.scheme = updateScheme == "EMPC" ? MFT_FTS_EMPC : MFT_FTS_EM;
     d.I = I;
     d.J = J;
     d.chiN = chiN;
@@ -124,6 +127,7 @@ mft_fts_potential PotentialIncompress::descriptor() const
 {
     mft_fts_potential d{};
     d.kind = MFT_FTS_INCOMPRESS;
+    d.scheme = updateScheme == "EMPC" ? MFT_FTS_EMPC : MFT_FTS_EM;
     d.delt_pl = delt;
     return d;
 }
@@ -317,8 +321,8 @@ void FTS_Box::finishInitialization()
         for (int i = 0; i < d.npotentials; ++i) {
             const auto& p = d.potentials[i];
             std::printf("%s{\"kind\": %d, \"I\": %d, \"J\": %d, \"chiN\": %.9g, \"kappaN\": %.9g, \"delt_pl\": %.9g, "
-                        "\"delt_mi\": %.9g, \"wpl0\": [%.9g, %.9g]}",
-                        i ? ", " : "", p.kind, p.I, p.J, p.chiN, p.kappaN, p.delt_pl, p.delt_mi,
+                        "\"delt_mi\": %.9g, \"scheme\": %d, \"wpl0\": [%.9g, %.9g]}",
+                        i ? ", " : "", p.kind, p.I, p.J, p.chiN, p.kappaN, p.delt_pl, p.delt_mi, p.scheme,
                         Potentials[i]->wpl[0].real(), Potentials[i]->wpl[0].imag());
         }
         std::printf("], \"molecules\": [");

@@ -356,7 +356,11 @@ class TildEngine:
 
 # --------------------------------------------------------------- FTS -----
 class FtsPotentialC(C.Structure):
-    _fields_ = [("kind", _i), ("I", _i), ("J", _i), ("chiN", _d), ("kappaN", _d), ("delt_pl", _d), ("delt_mi", _d)]
+    _fields_ = [("kind", _i), ("I", _i), ("J", _i), ("chiN", _d), ("kappaN", _d), ("delt_pl", _d), ("delt_mi", _d),
+                ("scheme", _i)]
+
+
+FTS_SCHEME = {"EM": 0, "EMPC": 1}
 
 
 class FtsMoleculeC(C.Structure):
@@ -382,7 +386,8 @@ def fts_desc(spec):
     d.npotentials = len(spec["potentials"])
     for k, p in enumerate(spec["potentials"]):
         d.potentials[k] = FtsPotentialC(FTS_POT[p["kind"]], p.get("I", -1), p.get("J", -1), p.get("chiN", 0.0),
-                                        p.get("kappaN", 0.0), p.get("dt_pl", 0.0), p.get("dt_mi", 0.0))
+                                        p.get("kappaN", 0.0), p.get("dt_pl", 0.0), p.get("dt_mi", 0.0),
+                                        FTS_SCHEME[p.get("scheme", "EM")])
     d.nmolecules = len(spec["molecules"])
     for k, m in enumerate(spec["molecules"]):
         mc = FtsMoleculeC()

@@ -132,6 +132,16 @@ void orc_fts_update_em(cplx* w, const cplx* dHdw, const cplx* noise, double delt
     }
 }
 
+void orc_fts_update_empc(cplx* w, const cplx* wo, const cplx* dHdw, const cplx* dHdwP, const cplx* noise,
+                         double delt, int M)
+{
+    const real h = (real)(0.5 * delt);
+    for (int i = 0; i < M; ++i) {
+        w[i] = wo[i] - h * (dHdw[i] + dHdwP[i]);
+        if (noise) w[i] += noise[i];
+    }
+}
+
 cplx orc_csum_prod(const cplx* a, const cplx* b, int M)
 {
     double _Complex s = 0;

@@ -129,6 +129,10 @@ void orc_fts_incompress_force(cplx* dHdw, const cplx* w, const cplx* rhoTot,
                               double C, double Nr, int M);
 /* d_fts_updateEM (fts_updateSchemes.cu:8-29); noise may be NULL. */
 void orc_fts_update_em(cplx* w, const cplx* dHdw, const cplx* noise, double delt, int M);
+/* d_fts_updateEMPC corrector (fts_updateSchemes.cu:56-81):
+ * w = wo - delt/2 (dHdw + dHdwP) + noise, with the predictor's noise. */
+void orc_fts_update_empc(cplx* w, const cplx* wo, const cplx* dHdw, const cplx* dHdwP, const cplx* noise,
+                         double delt, int M);
 /* sum_r a(r)*b(r) (complex product, FP64) and sum_r a(r). */
 cplx orc_csum_prod(const cplx* a, const cplx* b, int M);
 cplx orc_csum(const cplx* a, int M);

@@ -64,6 +64,7 @@ def lib(bits: int = 64):
         "orc_fts_helfand_force": (None, [_p, _p, _p, _d, _d, _d, _i]),
         "orc_fts_incompress_force": (None, [_p, _p, _p, _d, _d, _i]),
         "orc_fts_update_em": (None, [_p, _p, _p, _d, _i]),
+        "orc_fts_update_empc": (None, [_p, _p, _p, _p, _p, _d, _i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -463,6 +464,8 @@ class FtsPotential:
     dt_mi: float = 0.0
     wpl: np.ndarray = None
     wmi: np.ndarray = None
+    scheme: str = "EM"        # EM | EMPC (updateScheme keyword)
+    saved: dict = None        # EMPC predictor data: fields, forces and noise
 
 
 @dataclass
@@ -586,41 +589,78 @@ def fts_noise(M, seed, it, stream=0):
     return normal4_np(np.arange(M, dtype=np.uint64), it, TAG_FTS, stream, seed)
 
 
+def _forces(sys: FtsSystem, p: FtsPotential):
+    """dH/dw of one potential at its current fields and the current species
+    densities: d_makeHelfandForce (fts_potential_helfand.cu:223-240),
+    d_makeIncompressForce (fts_potential_incompress.cu:202-218),
+    d_makeFloryForce (fts_potential_flory.cu:279-304)."""
+    M = sys.M
+    L = lib()
+    rho_tot = sys.rho.sum(axis=0)
+    if p.kind == "Helfand":
+        dH = np.zeros(M, np.complex128)
+        L.orc_fts_helfand_force(_ptr(dH), _ptr(p.wpl), _ptr(rho_tot), sys.C, p.kappaN, sys.Nr, M)
+        return dH, None
+    if p.kind == "Incompress":
+        dH = np.zeros(M, np.complex128)
+        L.orc_fts_incompress_force(_ptr(dH), _ptr(p.wpl), _ptr(rho_tot), sys.C, sys.Nr, M)
+        return dH, None
+    dHp = np.zeros(M, np.complex128)
+    dHm = np.zeros(M, np.complex128)
+    rI = np.ascontiguousarray(sys.rho[p.I])
+    rJ = np.ascontiguousarray(sys.rho[p.J])
+    L.orc_fts_flory_force(_ptr(dHp), _ptr(dHm), _ptr(p.wpl), _ptr(p.wmi), _ptr(rI), _ptr(rJ), sys.C, p.chiN, sys.Nr, M)
+    return dHp, dHm
+
+
+def _noises(sys: FtsSystem, p: FtsPotential, z, col):
+    """The potential's CL noise fields (real, sqrt(2 dt / gvol) N(0,1)); the
+    incompressible update has none (fts_potential_incompress.cu:105)."""
+    gvol = sys.gvol()
+    if z is None or p.kind == "Incompress":
+        return None, None, col
+    npl = (np.sqrt(2.0 * p.dt_pl / gvol) * z[:, col]).astype(np.complex128)
+    if p.kind == "Helfand":
+        return npl, None, col + 1
+    nmi = (np.sqrt(2.0 * p.dt_mi / gvol) * z[:, col + 1]).astype(np.complex128)
+    return npl, nmi, col + 2
+
+
 def fts_update_fields(sys: FtsSystem, z=None):
     """Every potential's updateFields with EM (+ real CL noise when z is given):
     fts_potential_helfand.cu:77-152, fts_potential_flory.cu:123-167,
-    fts_potential_incompress.cu:84-113."""
+    fts_potential_incompress.cu:84-113.  An EMPC potential first keeps its
+    fields, forces and noise for the corrector (storePredictorData,
+    fts_potential_flory.cu:243-249)."""
     M = sys.M
-    gvol = sys.gvol()
     L = lib()
-    rho_tot = sys.rho.sum(axis=0)
     col = 0
     for p in sys.potentials:
-        if p.kind == "Helfand":
-            dH = np.zeros(M, np.complex128)
-            L.orc_fts_helfand_force(_ptr(dH), _ptr(p.wpl), _ptr(rho_tot), sys.C, p.kappaN, sys.Nr, M)
-            noise = None
-            if z is not None:
-                noise = (np.sqrt(2.0 * p.dt_pl / gvol) * z[:, col]).astype(np.complex128)
-                col += 1
-            L.orc_fts_update_em(_ptr(p.wpl), _ptr(dH), _ptr(noise), p.dt_pl, M)
-        elif p.kind == "Incompress":
-            dH = np.zeros(M, np.complex128)
-            L.orc_fts_incompress_force(_ptr(dH), _ptr(p.wpl), _ptr(rho_tot), sys.C, sys.Nr, M)
-            L.orc_fts_update_em(_ptr(p.wpl), _ptr(dH), None, p.dt_pl, M)
-        elif p.kind == "Flory":
-            dHp = np.zeros(M, np.complex128)
-            dHm = np.zeros(M, np.complex128)
-            rI = np.ascontiguousarray(sys.rho[p.I])
-            rJ = np.ascontiguousarray(sys.rho[p.J])
-            L.orc_fts_flory_force(_ptr(dHp), _ptr(dHm), _ptr(p.wpl), _ptr(p.wmi), _ptr(rI), _ptr(rJ), sys.C, p.chiN, sys.Nr, M)
-            npl = nmi = None
-            if z is not None:
-                npl = (np.sqrt(2.0 * p.dt_pl / gvol) * z[:, col]).astype(np.complex128)
-                nmi = (np.sqrt(2.0 * p.dt_mi / gvol) * z[:, col + 1]).astype(np.complex128)
-                col += 2
-            L.orc_fts_update_em(_ptr(p.wpl), _ptr(dHp), _ptr(npl), p.dt_pl, M)
+        dHp, dHm = _forces(sys, p)
+        npl, nmi, col = _noises(sys, p, z, col)
+        if p.scheme == "EMPC":
+            p.saved = dict(wpl=p.wpl.copy(), wmi=None if p.wmi is None else p.wmi.copy(), dHp=dHp, dHm=dHm,
+                           npl=npl, nmi=nmi)
+        L.orc_fts_update_em(_ptr(p.wpl), _ptr(dHp), _ptr(npl), p.dt_pl, M)
+        if p.kind == "Flory":
             L.orc_fts_update_em(_ptr(p.wmi), _ptr(dHm), _ptr(nmi), p.dt_mi, M)
+
+
+def fts_correct_fields(sys: FtsSystem):
+    """Every EMPC potential's correctFields (fts_potential_flory.cu:198-236,
+    and likewise for Helfand and Incompress): forces at the predicted fields
+    and densities, then w = w_o - dt/2 (F + F_o) + the predictor's noise."""
+    M = sys.M
+    L = lib()
+    for p in sys.potentials:
+        if p.scheme != "EMPC":
+            continue
+        dHp, dHm = _forces(sys, p)
+        sv = p.saved
+        L.orc_fts_update_empc(_ptr(p.wpl), _ptr(sv["wpl"]), _ptr(dHp), _ptr(sv["dHp"]), _ptr(sv["npl"]), p.dt_pl, M)
+        if p.kind == "Flory":
+            L.orc_fts_update_empc(_ptr(p.wmi), _ptr(sv["wmi"]), _ptr(dHm), _ptr(sv["dHm"]), _ptr(sv["nmi"]),
+                                  p.dt_mi, M)
 
 
 def fts_hamiltonian(sys: FtsSystem):
@@ -655,11 +695,17 @@ def fts_init(sys: FtsSystem, bits=64):
 
 
 def fts_step(sys: FtsSystem, z=None, bits=64):
-    """One CL iteration (FTS_Box.cu:19-47): update fields, rebuild species
-    fields, recompute densities."""
+    """One CL iteration (FTS_Box.cu:19-81): update fields, rebuild species
+    fields, recompute densities; with a predictor-corrector potential
+    (PCflag), correct the fields and rebuild species fields and densities
+    again."""
     fts_update_fields(sys, z)
     fts_build_species_fields(sys)
     fts_molecule_densities(sys, bits)
+    if any(p.scheme == "EMPC" for p in sys.potentials):
+        fts_correct_fields(sys)
+        fts_build_species_fields(sys)
+        fts_molecule_densities(sys, bits)
 
 
 def fts_system_from_spec(spec, fields):
@@ -668,7 +714,8 @@ def fts_system_from_spec(spec, fields):
     pots = []
     for k, p in enumerate(spec["potentials"]):
         fp = FtsPotential(kind=p["kind"], I=p.get("I", -1), J=p.get("J", -1), chiN=p.get("chiN", 0.0),
-                          kappaN=p.get("kappaN", 0.0), dt_pl=p.get("dt_pl", 0.0), dt_mi=p.get("dt_mi", 0.0))
+                          kappaN=p.get("kappaN", 0.0), dt_pl=p.get("dt_pl", 0.0), dt_mi=p.get("dt_mi", 0.0),
+                          scheme=p.get("scheme", "EM"))
         fp.wpl = np.array(fields.get((k, "pl"), np.zeros(M)), np.complex128)
         if p["kind"] == "Flory":
             fp.wmi = np.array(fields.get((k, "mi"), np.zeros(M)), np.complex128)

@@ -55,6 +55,16 @@ def test_c3_fts(tmp_path):
     assert d["molecules"] == [{"phi": 1.0, "blocks": [[50, 0], [50, 1]]}]
 
 
+def test_update_scheme_keyword(tmp_path):
+    # fts_potential_flory.cu:90-109: updateScheme EMPC selects the predictor-corrector
+    text = open(os.path.join(INPUTS, "c3_fts_diblock.in")).read().replace(
+        "potential Flory A B 20 0.01 0.01", "potential Flory A B 20 0.01 0.01 updateScheme EMPC")
+    r = dry(tmp_path, text)
+    assert r.returncode == 0, r.stdout[-1000:]
+    h, f = json.loads(r.stdout.strip().splitlines()[-1])["potentials"]
+    assert h["scheme"] == 0 and f["scheme"] == 1
+
+
 def test_flory_initialize_sets_wplus(tmp_path):
     # fts_potential_flory.cu:83-88: initialize also sets w+ = -1.5i
     text = open(os.path.join(INPUTS, "c3_fts_diblock.in")).read().replace(

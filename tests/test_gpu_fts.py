@@ -129,7 +129,8 @@ def test_closed_forms_on_device(gpu):
     s.close()
 
 
-@pytest.mark.parametrize("case", ["diblock", "diblock_square", "incompress_blend"])
+@pytest.mark.parametrize("case", ["diblock", "diblock_square", "incompress_blend", "diblock_empc",
+                                  "blend_empc_mixed"])
 def test_cl_steps_match_oracle(gpu, case):
     say("case: fts CL steps", case)
     if case == "diblock":
@@ -139,6 +140,19 @@ def test_cl_steps_match_oracle(gpu, case):
         # square x-y planes take the fused plane kernel
         spec, fields = systems.fts_diblock(grid_shape=(20, 20, 12), L=(3.5, 3.5, 2.5), Nr=20.0, NA=8, NB=12,
                                            chiN=15.0, kappaN=50.0, C=5.0, dt=0.01, sigma=0.1, seed=23)
+    elif case == "diblock_empc":
+        # predictor-corrector on both potentials (fts_updateSchemes.cu:56-81)
+        spec, fields = systems.fts_diblock(grid_shape=(16, 24, 12), L=(3.0, 4.0, 2.5), Nr=20.0, NA=8, NB=12,
+                                           chiN=15.0, kappaN=50.0, C=5.0, dt=0.01, sigma=0.1, seed=27)
+        spec["potentials"] = [dict(p, scheme="EMPC") for p in spec["potentials"]]
+    elif case == "blend_empc_mixed":
+        # EMPC Flory beside an EM incompressibility: the EM field keeps its
+        # predictor value through the corrector pass
+        spec = tiny_spec(Nx=(12, 16, 20), L=(3.0, 4.0, 5.0), pots=[dict(kind="Incompress", dt_pl=0.01),
+                                                                dict(kind="Flory", I=0, J=1, chiN=10.0, dt_pl=0.01,
+                                                                     dt_mi=0.01, scheme="EMPC")])
+        spec["molecules"] = [dict(blocks=[(4, 0), (3, 1)], phi=0.6), dict(blocks=[(5, 1)], phi=0.4)]
+        fields = random_fields(spec, 0.1, seed=9, imag=False)
     else:
         spec = tiny_spec(Nx=(12, 16, 20), L=(3.0, 4.0, 5.0), pots=[dict(kind="Incompress", dt_pl=0.01),
                                                                 dict(kind="Flory", I=0, J=1, chiN=10.0, dt_pl=0.01,
@@ -156,14 +170,17 @@ def test_cl_steps_match_oracle(gpu, case):
 
 
 def test_batch_equals_alone(gpu):
-    """Replicas of uneven grids and chain lengths advanced together give the
-    same bits as each replica alone."""
+    """Replicas of uneven grids, chain lengths and update schemes advanced
+    together give the same bits as each replica alone."""
     say("case: fts batch vs alone")
     reps = []
     for k, (g, L, NA, NB) in enumerate([((24, 24, 24), 5.0, 10, 10), ((16, 16, 16), 4.0, 6, 4),
-                                         ((24, 24, 24), 6.0, 12, 8), ((32, 32, 32), 7.0, 5, 15)]):
+                                         ((24, 24, 24), 6.0, 12, 8), ((32, 32, 32), 7.0, 5, 15),
+                                         ((24, 24, 24), 5.5, 9, 11)]):
         reps.append(systems.fts_diblock(grid_shape=g, L=L, Nr=float(NA + NB), NA=NA, NB=NB, chiN=12.0, kappaN=60.0,
                                         C=4.0, dt=0.01, sigma=0.1, seed=100 + k))
+    # the last replica uses the predictor-corrector: a group of its own
+    reps[-1][0]["potentials"] = [dict(p, scheme="EMPC") for p in reps[-1][0]["potentials"]]
     b = gpu.FtsBatch([r[0] for r in reps])
     for k, (spec, fields) in enumerate(reps):
         systems.load_fields(b.set_fields, spec, fields, rep=k)
