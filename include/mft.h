@@ -203,10 +203,13 @@ int mft_tild_autotune(mft_tild* eng, int reps, int* choice);
 int mft_fts_bond_factor(float* g2, const int Nx[3], const float L[3], double Nr, void* stream);
 
 enum { MFT_FTS_HELFAND = 0, MFT_FTS_FLORY = 1, MFT_FTS_INCOMPRESS = 2 };
-/* updateScheme (fts_potential_flory.cu:90-109): Euler-Maruyama, or the
+/* updateScheme (fts_potential_flory.cu:90-109): Euler-Maruyama; the
  * predictor-corrector of fts_updateSchemes.cu:56-81 (a box with any EMPC
- * potential recomputes its densities twice per iteration, FTS_Box.cu:53-81) */
-enum { MFT_FTS_EM = 0, MFT_FTS_EMPC = 1 };
+ * potential recomputes its densities twice per iteration, FTS_Box.cu:53-81);
+ * or the semi-implicit k-space step of fts_updateSchemes.cu:34-52 with the
+ * Debye linear coefficients of fts_molecule_linear.cu:433-560 (like the
+ * reference, it adds no noise). */
+enum { MFT_FTS_EM = 0, MFT_FTS_EMPC = 1, MFT_FTS_1S = 2 };
 
 typedef struct {
     int kind;             /* MFT_FTS_* (fts_potential.cu:25-55) */
@@ -215,7 +218,7 @@ typedef struct {
     double kappaN;        /* Helfand kappa N (fts_potential_helfand.cu:26) */
     double delt_pl;       /* time step of w+ (delt for Helfand/Incompress) */
     double delt_mi;       /* time step of w- (Flory) */
-    int scheme;           /* MFT_FTS_EM (0, the reference's default) or MFT_FTS_EMPC */
+    int scheme;           /* MFT_FTS_EM (0, the reference's default), _EMPC or _1S */
 } mft_fts_potential;
 
 typedef struct {

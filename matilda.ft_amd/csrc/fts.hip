@@ -49,6 +49,8 @@ struct FtsPotDev {
     // EMPC predictor data: fields before the predictor and its forces
     float2 *wpl_o, *wmi_o;
     double2 *fpl_o, *fmi_o;
+    // 1S: the forces, for the k-space step that follows the pass
+    float2 *f1pl, *f1mi;
 };
 
 struct FtsUpdDev {
@@ -180,7 +182,14 @@ __global__ void __launch_bounds__(kFtsThreads) k_fts_update(const FtsUpdDev* __r
             if (P.kind == MFT_FTS_FLORY) {
                 float2 wm = P.wmi[i];
                 double2 v = make_double2(wm.x, wm.y);
-                if (!D.first && (!D.corrector || P.scheme == MFT_FTS_EMPC)) {
+                if (!D.first && !D.corrector && P.scheme == MFT_FTS_1S) {
+                    // forces for the k-space step (fts_potential_flory.cu:172-193)
+                    double2 rI = pick(rho, P.I), rJ = pick(rho, P.J);
+                    P.f1pl[i] = make_float2((float)(P.cpl * w.x - (rI.y + rJ.y) / D.Nr),
+                                            (float)(P.cpl * w.y + (rI.x + rJ.x) / D.Nr));
+                    P.f1mi[i] = make_float2((float)(P.cmi * v.x + (rJ.x - rI.x) / D.Nr),
+                                            (float)(P.cmi * v.y + (rJ.y - rI.y) / D.Nr));
+                } else if (!D.first && (!D.corrector || P.scheme == MFT_FTS_EMPC)) {
                     // d_makeFloryForce (fts_potential_flory.cu:279-304)
                     double2 rI = pick(rho, P.I), rJ = pick(rho, P.J);
                     double fpx = P.cpl * w.x - (rI.y + rJ.y) / D.Nr;
@@ -240,25 +249,30 @@ __global__ void __launch_bounds__(kFtsThreads) k_fts_update(const FtsUpdDev* __r
                         fx = -rt.y / D.Nr;
                         fy = rt.x / D.Nr - D.C;
                     }
-                    const double npl = P.col_pl >= 0 ? P.nmag_pl * (double)pick(z, P.col_pl) : 0.0;
-                    if (!D.corrector) {
-                        if (P.scheme == MFT_FTS_EMPC) {
-                            P.wpl_o[i] = wp;
-                            P.fpl_o[i] = make_double2(fx, fy);
-                        }
-                        w.x -= P.dt_pl * fx;
-                        w.y -= P.dt_pl * fy;
-                        w.x += npl;
+                    if (P.scheme == MFT_FTS_1S) {
+                        // the field changes in the k-space step that follows
+                        P.f1pl[i] = make_float2((float)fx, (float)fy);
                     } else {
-                        const float2 wo = P.wpl_o[i];
-                        const double2 fo = P.fpl_o[i];
-                        const double hp = 0.5 * P.dt_pl;
-                        w.x = (double)wo.x - hp * (fx + fo.x) + npl;
-                        w.y = (double)wo.y - hp * (fy + fo.y);
+                        const double npl = P.col_pl >= 0 ? P.nmag_pl * (double)pick(z, P.col_pl) : 0.0;
+                        if (!D.corrector) {
+                            if (P.scheme == MFT_FTS_EMPC) {
+                                P.wpl_o[i] = wp;
+                                P.fpl_o[i] = make_double2(fx, fy);
+                            }
+                            w.x -= P.dt_pl * fx;
+                            w.y -= P.dt_pl * fy;
+                            w.x += npl;
+                        } else {
+                            const float2 wo = P.wpl_o[i];
+                            const double2 fo = P.fpl_o[i];
+                            const double hp = 0.5 * P.dt_pl;
+                            w.x = (double)wo.x - hp * (fx + fo.x) + npl;
+                            w.y = (double)wo.y - hp * (fy + fo.y);
+                        }
+                        float2 wf = make_float2((float)w.x, (float)w.y);
+                        P.wpl[i] = wf;
+                        w = make_double2(wf.x, wf.y);
                     }
-                    float2 wf = make_float2((float)w.x, (float)w.y);
-                    P.wpl[i] = wf;
-                    w = make_double2(wf.x, wf.y);
                 }
                 // i w+ on every species
 #pragma unroll
@@ -612,6 +626,10 @@ struct FtsRep {
     float2 *wpl_o[kFtsMaxPots] = {nullptr}, *wmi_o[kFtsMaxPots] = {nullptr};
     double2 *fpl_o[kFtsMaxPots] = {nullptr}, *fmi_o[kFtsMaxPots] = {nullptr};
     bool pc = false;              // some potential uses the predictor-corrector
+    // 1S: forces and the real linear coefficients A(k) of each field
+    float2 *f1pl[kFtsMaxPots] = {nullptr}, *f1mi[kFtsMaxPots] = {nullptr};
+    double *Akpl[kFtsMaxPots] = {nullptr}, *Akmi[kFtsMaxPots] = {nullptr};
+    bool s1 = false;              // some potential uses the one-step k-space scheme
     float2 *W = nullptr, *E = nullptr, *rhot = nullptr;
     float2 *p[kFtsMaxMol] = {nullptr}, *pd[kFtsMaxMol] = {nullptr};
     float2* tmp = nullptr; // [2*nmol][M]
@@ -627,6 +645,7 @@ struct FtsRep {
 struct FtsGroup {
     int n[3];
     bool pc = false;                 // predictor-corrector replicas (two passes per iteration)
+    bool s1 = false;                 // replicas with a one-step k-space update
     std::vector<int> reps;
     FtsUpdDev* d_upd = nullptr;
     DensDesc* d_dens = nullptr;
@@ -706,8 +725,8 @@ static int validate_desc(const mft_fts_desc& d)
         const mft_fts_potential& P = d.potentials[p];
         MFT_REQUIRE(P.kind >= 0 && P.kind <= MFT_FTS_INCOMPRESS, MFT_ERR_UNSUPPORTED, "FTS potential kind %d",
                     P.kind);
-        MFT_REQUIRE(P.scheme == MFT_FTS_EM || P.scheme == MFT_FTS_EMPC, MFT_ERR_UNSUPPORTED,
-                    "update scheme %d (EM and EMPC are built)", P.scheme);
+        MFT_REQUIRE(P.scheme == MFT_FTS_EM || P.scheme == MFT_FTS_EMPC || P.scheme == MFT_FTS_1S,
+                    MFT_ERR_UNSUPPORTED, "update scheme %d", P.scheme);
         if (P.kind == MFT_FTS_FLORY) {
             MFT_REQUIRE(P.I >= 0 && P.I < d.nspecies && P.J >= 0 && P.J < d.nspecies && P.I != P.J, MFT_ERR_INVALID,
                         "Flory species out of range");
@@ -746,6 +765,87 @@ static void axis_table(int n, float L, double Nr, std::vector<float2>& g)
     }
 }
 
+// |k|^2 of grid index i (Box::get_kD double version, Box.cu:181-203: float
+// components, squared and summed in double)
+static double k2_of(int i, const mft_fts_desc& d)
+{
+    const int ix = i % d.nx, iy = (i / d.nx) % d.ny, iz = i / (d.nx * d.ny);
+    const double kx = kcomp(ix, d.nx, d.lx), ky = kcomp(iy, d.ny, d.ly), kz = kcomp(iz, d.nz, d.lz);
+    return kx * kx + ky * ky + kz * kz;
+}
+
+// Debye functions of FTS_Box.cu:537-587 at x = k^2 alpha (f): the whole
+// chain (1 at k = 0), one block (0 at k = 0), and two blocks separated by a
+// middle block with the factor 2 (0 at k = 0)
+static double debye_chain(double k2, double a) { return 2.0 * (std::exp(-k2 * a) + k2 * a - 1.0) / (k2 * k2); }
+static double debye_block(double k2, double f, double a)
+{
+    return 2.0 * (std::exp(-k2 * a * f) + k2 * a * f - 1.0) / (k2 * k2);
+}
+static double debye_cross(double k2, double fA, double fB, double fC, double a)
+{
+    return 2.0 * (1.0 - std::exp(-k2 * a * fA)) * std::exp(-k2 * a * fB) * (1.0 - std::exp(-k2 * a * fC)) / (k2 * k2);
+}
+
+// Linear coefficients of the 1S fields: initLinearCoeffs
+// (fts_potential_helfand.cu:277-280, fts_potential_flory.cu:323-326,
+// fts_potential_incompress.cu:249-252) plus every molecule's
+// computeLinearTerms (fts_molecule_linear.cu:433-560); the Debye terms go to
+// A+ of Helfand and Flory potentials, A- of Flory stays 2C/chiN.
+static int linear_coeffs(FtsRep& r)
+{
+    const mft_fts_desc& d = r.d;
+    const size_t M = r.M;
+    std::vector<double> k2(M);
+    for (size_t i = 0; i < M; ++i) k2[i] = k2_of((int)i, d);
+    std::vector<std::vector<double>> Apl(d.npotentials), Ami(d.npotentials);
+    for (int p = 0; p < d.npotentials; ++p) {
+        const mft_fts_potential& P = d.potentials[p];
+        const double a0 = P.kind == MFT_FTS_HELFAND ? d.C / P.kappaN : (P.kind == MFT_FTS_FLORY ? 2.0 * d.C / P.chiN : 0.0);
+        Apl[p].assign(M, a0);
+        if (P.kind == MFT_FTS_FLORY) Ami[p].assign(M, a0);
+    }
+    for (int m = 0; m < d.nmolecules; ++m) {
+        const mft_fts_molecule& mol = d.molecules[m];
+        const int Nt = r.Ntot[m];
+        const double alpha = (double)Nt / d.Nr;
+        const double pref = r.nmolecs[m] * alpha * (double)Nt / r.V;
+        for (int p = 0; p < d.npotentials; ++p) {
+            const mft_fts_potential& P = d.potentials[p];
+            const bool homo_flory = P.kind == MFT_FTS_FLORY && mol.nblocks == 1;
+            if (P.kind == MFT_FTS_HELFAND || homo_flory) {
+                Apl[p][0] += pref;
+                for (size_t i = 1; i < M; ++i) Apl[p][i] += pref * debye_chain(k2[i], alpha);
+                continue;
+            }
+            if (P.kind != MFT_FTS_FLORY) continue;
+            auto acts = [&](int b) { return mol.species[b] == P.I || mol.species[b] == P.J; };
+            for (int j = 0; j < mol.nblocks - 1; ++j) {
+                if (!acts(j)) continue;
+                const double fA = (double)mol.N[j] / Nt;
+                std::vector<double> gMT(M, 0.0);
+                for (size_t i = 1; i < M; ++i) gMT[i] = debye_block(k2[i], fA, alpha);
+                int nbetween = 0;
+                for (int k = j + 1; k < mol.nblocks; ++k) {
+                    if (!acts(k)) continue;
+                    const double fC = (double)mol.N[k] / Nt, fB = (double)nbetween / Nt;
+                    for (size_t i = 1; i < M; ++i) {
+                        gMT[i] += debye_block(k2[i], fC, alpha) + debye_cross(k2[i], fA, fB, fC, alpha);
+                        Apl[p][i] += pref * gMT[i];
+                    }
+                    nbetween += mol.N[k];
+                }
+            }
+        }
+    }
+    for (int p = 0; p < d.npotentials; ++p) {
+        if (d.potentials[p].scheme != MFT_FTS_1S) continue;
+        MFT_HIP(hipMemcpy(r.Akpl[p], Apl[p].data(), sizeof(double) * M, hipMemcpyHostToDevice));
+        if (r.Akmi[p]) MFT_HIP(hipMemcpy(r.Akmi[p], Ami[p].data(), sizeof(double) * M, hipMemcpyHostToDevice));
+    }
+    return MFT_OK;
+}
+
 static int rep_setup(FtsBatch* b, FtsRep& r)
 {
     const mft_fts_desc& d = r.d;
@@ -761,6 +861,15 @@ static int rep_setup(FtsBatch* b, FtsRep& r)
         if (d.potentials[p].kind == MFT_FTS_FLORY) {
             if ((rc = dev_alloc(r.allocs, &r.wmi[p], M))) return rc;
             MFT_HIP(hipMemset(r.wmi[p], 0, sizeof(float2) * M));
+        }
+        if (d.potentials[p].scheme == MFT_FTS_1S) {
+            r.s1 = true;
+            if ((rc = dev_alloc(r.allocs, &r.f1pl[p], M))) return rc;
+            if ((rc = dev_alloc(r.allocs, &r.Akpl[p], M))) return rc;
+            if (d.potentials[p].kind == MFT_FTS_FLORY) {
+                if ((rc = dev_alloc(r.allocs, &r.f1mi[p], M))) return rc;
+                if ((rc = dev_alloc(r.allocs, &r.Akmi[p], M))) return rc;
+            }
         }
         if (d.potentials[p].scheme == MFT_FTS_EMPC) {
             r.pc = true;
@@ -794,6 +903,7 @@ static int rep_setup(FtsBatch* b, FtsRep& r)
         r.nblk += mol.nblocks;
     }
     MFT_REQUIRE(r.nblk <= kFtsMaxMol * kFtsMaxBlocks, MFT_ERR_INVALID, "too many blocks");
+    if (r.s1 && (rc = linear_coeffs(r))) return rc;
     if ((rc = dev_alloc(r.allocs, &r.rhot, M * r.nblk))) return rc;
     MFT_HIP(hipMemset(r.rhot, 0, sizeof(float2) * M * r.nblk));
     if ((rc = dev_alloc(r.allocs, &r.tmp, M * 2 * d.nmolecules))) return rc;
@@ -835,6 +945,8 @@ static void fill_upd(const FtsBatch* b, const FtsRep& r, FtsUpdDev& U, int first
         D.wmi_o = r.wmi_o[p];
         D.fpl_o = r.fpl_o[p];
         D.fmi_o = r.fmi_o[p];
+        D.f1pl = r.f1pl[p];
+        D.f1mi = r.f1mi[p];
         D.dt_pl = P.delt_pl;
         D.dt_mi = P.delt_mi;
         D.col_pl = D.col_mi = -1;
@@ -1101,12 +1213,13 @@ static FtsGroup& group_for(FtsBatch* b, const FtsRep& r)
 {
     const mft_fts_desc& d = r.d;
     for (auto& G : b->groups)
-        if (G.n[0] == d.nx && G.n[1] == d.ny && G.n[2] == d.nz && G.pc == r.pc) return G;
+        if (G.n[0] == d.nx && G.n[1] == d.ny && G.n[2] == d.nz && G.pc == r.pc && G.s1 == r.s1) return G;
     FtsGroup G;
     G.n[0] = d.nx;
     G.n[1] = d.ny;
     G.n[2] = d.nz;
     G.pc = r.pc;
+    G.s1 = r.s1;
     b->groups.push_back(G);
     return b->groups.back();
 }
@@ -1179,11 +1292,13 @@ static int batch_activate(FtsBatch* b, const std::vector<int>& reps)
     std::map<std::array<int, 4>, FtsGroup> tmp;
     for (int id : reps) {
         const mft_fts_desc& d = b->reps[id].d;
-        FtsGroup& T = tmp[std::array<int, 4>{d.nx, d.ny, d.nz, b->reps[id].pc ? 1 : 0}];
+        const FtsRep& r = b->reps[id];
+        FtsGroup& T = tmp[std::array<int, 4>{d.nx, d.ny, d.nz, (r.pc ? 1 : 0) | (r.s1 ? 2 : 0)}];
         T.n[0] = d.nx;
         T.n[1] = d.ny;
         T.n[2] = d.nz;
-        T.pc = b->reps[id].pc;
+        T.pc = r.pc;
+        T.s1 = r.s1;
         T.reps.push_back(id);
     }
     for (auto& kv : tmp) {
@@ -1291,6 +1406,67 @@ static int batch_init(FtsBatch* b)
     return MFT_OK;
 }
 
+int tild_axis(int sign, int real_in, int real_out, const void* in, void* out, AxisGeom g, int nf, long fs,
+              float scale, hipStream_t st);
+
+// d_fts_update1S (fts_updateSchemes.cu:34-52) on k-space arrays
+__global__ void k_fts_update1s(float2* __restrict__ w, const float2* __restrict__ f, const double* __restrict__ A,
+                               double dt, int M)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const double a = A[i], den = 1.0 + dt * a;
+    if (den == 0.0) return;
+    const float2 wv = w[i], fv = f[i];
+    const double x = ((double)wv.x - dt * ((double)fv.x - a * (double)wv.x)) / den;
+    const double y = ((double)wv.y - dt * ((double)fv.y - a * (double)wv.y)) / den;
+    w[i] = make_float2((float)x, (float)y);
+}
+
+// in-place 3D transform of one complex field of replica r: forward scaled by
+// 1/M (Box::cufftWrapperDouble, Box.cu:76-104), inverse unscaled
+static int fft3(float2* a, const FtsRep& r, int sign, hipStream_t st)
+{
+    const mft_fts_desc& d = r.d;
+    const float s = sign < 0 ? (float)(1.0 / (double)r.M) : 1.0f;
+    for (int ax = 0; ax < 3; ++ax) {
+        AxisGeom g{d.nx, d.ny, d.nz, ax};
+        const int rc = tild_axis(sign, 0, 0, a, a, g, 1, r.M, ax == 0 ? s : 1.0f, st);
+        if (rc) return rc;
+    }
+    return MFT_OK;
+}
+
+// the k-space step of every 1S field of the group (forces from the pass just
+// run): w <- IFFT[(w_k - dt (F_k - A w_k)) / (1 + dt A)]
+static int group_one_step(FtsBatch* b, FtsGroup& G)
+{
+    int rc;
+    hipStream_t st = b->stream;
+    for (int id : G.reps) {
+        FtsRep& r = b->reps[id];
+        for (int p = 0; p < r.d.npotentials; ++p) {
+            const mft_fts_potential& P = r.d.potentials[p];
+            if (P.scheme != MFT_FTS_1S) continue;
+            struct {
+                float2* w;
+                float2* f;
+                double* A;
+                double dt;
+            } fields[2] = {{r.wpl[p], r.f1pl[p], r.Akpl[p], P.delt_pl}, {r.wmi[p], r.f1mi[p], r.Akmi[p], P.delt_mi}};
+            for (int k = 0; k < (P.kind == MFT_FTS_FLORY ? 2 : 1); ++k) {
+                if ((rc = fft3(fields[k].f, r, -1, st))) return rc;
+                if ((rc = fft3(fields[k].w, r, -1, st))) return rc;
+                k_fts_update1s<<<(r.M + 255) / 256, 256, 0, st>>>(fields[k].w, fields[k].f, fields[k].A, fields[k].dt,
+                                                                   r.M);
+                if ((rc = fft3(fields[k].w, r, +1, st))) return rc;
+            }
+        }
+    }
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
 // one CL iteration of every group: field updates from the previous
 // densities, propagators, densities and Q; then every slot's counter
 // advances (a slot is reset when a replica takes it)
@@ -1300,6 +1476,11 @@ static int batch_iteration(FtsBatch* b)
     for (auto& G : b->groups) {
         if (G.reps.empty()) continue;
         if ((rc = group_update(b, G, 0))) return rc;
+        if (G.s1) {
+            // k-space step of the 1S fields, then the species fields again
+            if ((rc = group_one_step(b, G))) return rc;
+            if ((rc = group_update(b, G, 1))) return rc;
+        }
         if ((rc = group_density(b, G))) return rc;
         if (G.pc) {
             // FTS_Box.cu:53-81: correct, rebuild species fields, densities again

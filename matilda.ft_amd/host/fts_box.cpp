@@ -57,8 +57,8 @@ void FTS_Potential::parse_options(std::istringstream& iss, std::vector<std::comp
             }
         } else if (s1 == "updateScheme") {
             iss >> updateScheme;
-            if (updateScheme != "EM" && updateScheme != "EMPC")
-                die("updateScheme " + updateScheme + " is not part of this build (EM, EMPC)");
+            if (updateScheme != "EM" && updateScheme != "EMPC" && updateScheme != "1S")
+                die("updateScheme " + updateScheme + " is not part of this build (EM, EMPC, 1S)");
         } else if (s1 == "modify") {
             iss >> s1;
             if (s1 != "zeromean" && s1 != "zeroMean") die("Invalid modify option on " + potentialStyle + " potential");
@@ -83,7 +83,7 @@ mft_fts_potential PotentialHelfand::descriptor() const
 {
     mft_fts_potential d{};
     d.kind = MFT_FTS_HELFAND;
-    d.scheme = updateScheme == "EMPC" ? MFT_FTS_EMPC : MFT_FTS_EM;
+    d.scheme = updateScheme == "EMPC" ? MFT_FTS_EMPC : (updateScheme == "1S" ? MFT_FTS_1S : MFT_FTS_EM);
     d.kappaN = kappaN;
     d.delt_pl = delt;
     return d;
@@ -107,7 +107,7 @@ mft_fts_potential PotentialFlory::descriptor() const
 {
     mft_fts_potential d{};
     d.kind = MFT_FTS_FLORY;
-    d.scheme = updateScheme == "EMPC" ? MFT_FTS_EMPC : MFT_FTS_EM;
+    d.scheme = updateScheme == "EMPC" ? MFT_FTS_EMPC : (updateScheme == "1S" ? MFT_FTS_1S : MFT_FTS_EM);
     d.I = I;
     d.J = J;
     d.chiN = chiN;
@@ -127,7 +127,7 @@ mft_fts_potential PotentialIncompress::descriptor() const
 {
     mft_fts_potential d{};
     d.kind = MFT_FTS_INCOMPRESS;
-    d.scheme = updateScheme == "EMPC" ? MFT_FTS_EMPC : MFT_FTS_EM;
+    d.scheme = updateScheme == "EMPC" ? MFT_FTS_EMPC : (updateScheme == "1S" ? MFT_FTS_1S : MFT_FTS_EM);
     d.delt_pl = delt;
     return d;
 }

@@ -360,7 +360,7 @@ class FtsPotentialC(C.Structure):
                 ("scheme", _i)]
 
 
-FTS_SCHEME = {"EM": 0, "EMPC": 1}
+FTS_SCHEME = {"EM": 0, "EMPC": 1, "1S": 2}
 
 
 class FtsMoleculeC(C.Structure):

@@ -142,6 +142,15 @@ void orc_fts_update_empc(cplx* w, const cplx* wo, const cplx* dHdw, const cplx* 
     }
 }
 
+void orc_fts_update_1s(cplx* w, const cplx* dHdw, const double* Ak, double delt, int M)
+{
+    for (int i = 0; i < M; ++i) {
+        const real denom = (real)(1.0 + delt * Ak[i]);
+        if (denom != 0)
+            w[i] = (w[i] - (real)delt * (dHdw[i] - (real)Ak[i] * w[i])) / denom;
+    }
+}
+
 cplx orc_csum_prod(const cplx* a, const cplx* b, int M)
 {
     double _Complex s = 0;

@@ -134,6 +134,10 @@ void orc_fts_update_em(cplx* w, const cplx* dHdw, const cplx* noise, double delt
 void orc_fts_update_empc(cplx* w, const cplx* wo, const cplx* dHdw, const cplx* dHdwP, const cplx* noise,
                          double delt, int M);
 /* sum_r a(r)*b(r) (complex product, FP64) and sum_r a(r). */
+/* d_fts_update1S (fts_updateSchemes.cu:34-52) on k-space arrays (forward
+ * transforms scaled by 1/M): w = (w - delt (dHdw - A w)) / (1 + delt A)
+ * where the denominator is nonzero; A real; no noise term. */
+void orc_fts_update_1s(cplx* w, const cplx* dHdw, const double* Ak, double delt, int M);
 cplx orc_csum_prod(const cplx* a, const cplx* b, int M);
 cplx orc_csum(const cplx* a, int M);
 

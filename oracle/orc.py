@@ -65,6 +65,7 @@ def lib(bits: int = 64):
         "orc_fts_incompress_force": (None, [_p, _p, _p, _d, _d, _i]),
         "orc_fts_update_em": (None, [_p, _p, _p, _d, _i]),
         "orc_fts_update_empc": (None, [_p, _p, _p, _p, _p, _d, _i]),
+        "orc_fts_update_1s": (None, [_p, _p, _p, _d, _i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -464,8 +465,10 @@ class FtsPotential:
     dt_mi: float = 0.0
     wpl: np.ndarray = None
     wmi: np.ndarray = None
-    scheme: str = "EM"        # EM | EMPC (updateScheme keyword)
+    scheme: str = "EM"        # EM | EMPC | 1S (updateScheme keyword)
     saved: dict = None        # EMPC predictor data: fields, forces and noise
+    Akpl: np.ndarray = None   # 1S linear coefficients of w+ and w- (real, k space)
+    Akmi: np.ndarray = None
 
 
 @dataclass
@@ -613,6 +616,89 @@ def _forces(sys: FtsSystem, p: FtsPotential):
     return dHp, dHm
 
 
+def fts_k2(sys: FtsSystem):
+    """|k|^2 of every grid index (Box::get_kD double version, Box.cu:181-203)."""
+    return np.array([kvec_d(i, sys.Nx, sys.L)[0] for i in range(sys.M)])
+
+
+def debye_homopolymer(k2, alpha):
+    """FTS_Box::computeHomopolyDebye (FTS_Box.cu:537-550): g(0) = 1,
+    g = 2 (exp(-k^2 a) + k^2 a - 1) / k^4."""
+    g = np.ones_like(k2)
+    k = k2[1:]
+    g[1:] = 2.0 * (np.exp(-k * alpha) + k * alpha - 1.0) / (k * k)
+    return g
+
+
+def debye_intra_block(k2, f, alpha):
+    """FTS_Box::computeIntRABlockDebye (FTS_Box.cu:552-566): the same with
+    alpha f, and 0 at k = 0."""
+    g = np.zeros_like(k2)
+    k = k2[1:]
+    g[1:] = 2.0 * (np.exp(-k * alpha * f) + k * alpha * f - 1.0) / (k * k)
+    return g
+
+
+def debye_inter_block(k2, fA, fB, fC, alpha):
+    """FTS_Box::computeIntERBlockDebye (FTS_Box.cu:571-587): blocks A and C
+    separated by B, with the factor 2; 0 at k = 0."""
+    g = np.zeros_like(k2)
+    k = k2[1:]
+    g[1:] = 2.0 * (1.0 - np.exp(-k * alpha * fA)) * np.exp(-k * alpha * fB) * (1.0 - np.exp(-k * alpha * fC)) / (k * k)
+    return g
+
+
+def fts_linear_coeffs(sys: FtsSystem):
+    """initLinearCoeffs of every potential (fts_potential_helfand.cu:277-280,
+    fts_potential_flory.cu:323-326, fts_potential_incompress.cu:249-252) then
+    LinearMolec::computeLinearTerms of every molecule
+    (fts_molecule_linear.cu:433-560): the Debye terms go to A+ of Helfand and
+    Flory potentials only (A- of Flory stays 2C/chiN)."""
+    k2 = fts_k2(sys)
+    M = sys.M
+    for p in sys.potentials:
+        if p.kind == "Helfand":
+            p.Akpl = np.full(M, sys.C / p.kappaN)
+        elif p.kind == "Flory":
+            p.Akpl = np.full(M, 2.0 * sys.C / p.chiN)
+            p.Akmi = p.Akpl.copy()
+        else:
+            p.Akpl = np.zeros(M)
+    for m in sys.molecules:
+        N = [n for n, _ in m.blocks]
+        spec = [s for _, s in m.blocks]
+        Ntot = sum(N)
+        alpha = Ntot / sys.Nr
+        pref = m.nmolecs * alpha * Ntot / sys.V()
+        h = pref * debye_homopolymer(k2, alpha)
+        for p in sys.potentials:
+            if p.kind == "Helfand":
+                p.Akpl = p.Akpl + h
+        if len(N) == 1:
+            for p in sys.potentials:
+                if p.kind == "Flory":
+                    p.Akpl = p.Akpl + h
+            continue
+        for p in sys.potentials:
+            if p.kind != "Flory":
+                continue
+            acts = (p.I, p.J)
+            for j in range(len(N) - 1):
+                if spec[j] not in acts:
+                    continue
+                fA = N[j] / Ntot
+                gMT = debye_intra_block(k2, fA, alpha)
+                nbetween = 0
+                for k in range(j + 1, len(N)):
+                    if spec[k] not in acts:
+                        continue
+                    fC = N[k] / Ntot
+                    fB = nbetween / Ntot
+                    gMT = gMT + debye_intra_block(k2, fC, alpha) + debye_inter_block(k2, fA, fB, fC, alpha)
+                    p.Akpl = p.Akpl + pref * gMT
+                    nbetween += N[k]
+
+
 def _noises(sys: FtsSystem, p: FtsPotential, z, col):
     """The potential's CL noise fields (real, sqrt(2 dt / gvol) N(0,1)); the
     incompressible update has none (fts_potential_incompress.cu:105)."""
@@ -638,6 +724,19 @@ def fts_update_fields(sys: FtsSystem, z=None):
     for p in sys.potentials:
         dHp, dHm = _forces(sys, p)
         npl, nmi, col = _noises(sys, p, z, col)
+        if p.scheme == "1S":
+            # k-space semi-implicit step without noise (fts_potential_flory.cu:172-193)
+            if p.Akpl is None:
+                fts_linear_coeffs(sys)
+            pairs = [(p.wpl, dHp, p.Akpl, p.dt_pl)]
+            if p.kind == "Flory":
+                pairs.append((p.wmi, dHm, p.Akmi, p.dt_mi))
+            for w, dH, A, dt in pairs:
+                wk = box_fft(w, list(sys.Nx), 1)
+                fk = box_fft(dH, list(sys.Nx), 1)
+                L.orc_fts_update_1s(_ptr(wk), _ptr(fk), _ptr(np.ascontiguousarray(A, np.float64)), dt, M)
+                w[:] = box_fft(wk, list(sys.Nx), -1)
+            continue
         if p.scheme == "EMPC":
             p.saved = dict(wpl=p.wpl.copy(), wmi=None if p.wmi is None else p.wmi.copy(), dHp=dHp, dHm=dHm,
                            npl=npl, nmi=nmi)

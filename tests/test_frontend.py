@@ -63,6 +63,8 @@ def test_update_scheme_keyword(tmp_path):
     assert r.returncode == 0, r.stdout[-1000:]
     h, f = json.loads(r.stdout.strip().splitlines()[-1])["potentials"]
     assert h["scheme"] == 0 and f["scheme"] == 1
+    r = dry(tmp_path, text.replace("updateScheme EMPC", "updateScheme 1S"))
+    assert json.loads(r.stdout.strip().splitlines()[-1])["potentials"][1]["scheme"] == 2
 
 
 def test_flory_initialize_sets_wplus(tmp_path):

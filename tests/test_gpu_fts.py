@@ -130,7 +130,7 @@ def test_closed_forms_on_device(gpu):
 
 
 @pytest.mark.parametrize("case", ["diblock", "diblock_square", "incompress_blend", "diblock_empc",
-                                  "blend_empc_mixed"])
+                                  "blend_empc_mixed", "diblock_1s", "blend_schemes_mixed"])
 def test_cl_steps_match_oracle(gpu, case):
     say("case: fts CL steps", case)
     if case == "diblock":
@@ -145,6 +145,20 @@ def test_cl_steps_match_oracle(gpu, case):
         spec, fields = systems.fts_diblock(grid_shape=(16, 24, 12), L=(3.0, 4.0, 2.5), Nr=20.0, NA=8, NB=12,
                                            chiN=15.0, kappaN=50.0, C=5.0, dt=0.01, sigma=0.1, seed=27)
         spec["potentials"] = [dict(p, scheme="EMPC") for p in spec["potentials"]]
+    elif case == "diblock_1s":
+        # one-step k-space scheme with the Debye coefficients (no noise, as the
+        # reference's d_fts_update1S)
+        spec, fields = systems.fts_diblock(grid_shape=(16, 24, 12), L=(3.0, 4.0, 2.5), Nr=20.0, NA=8, NB=12,
+                                           chiN=15.0, kappaN=50.0, C=5.0, dt=0.01, sigma=0.1, seed=29)
+        spec["potentials"] = [dict(p, scheme="1S") for p in spec["potentials"]]
+    elif case == "blend_schemes_mixed":
+        # 1S incompressibility, EMPC Flory and a homopolymer (the Flory A+
+        # takes the homopolymer Debye term)
+        spec = tiny_spec(Nx=(12, 16, 20), L=(3.0, 4.0, 5.0), pots=[dict(kind="Incompress", dt_pl=0.01, scheme="1S"),
+                                                                dict(kind="Flory", I=0, J=1, chiN=10.0, dt_pl=0.01,
+                                                                     dt_mi=0.01, scheme="EMPC")])
+        spec["molecules"] = [dict(blocks=[(4, 0), (3, 1)], phi=0.6), dict(blocks=[(5, 1)], phi=0.4)]
+        fields = random_fields(spec, 0.1, seed=10, imag=False)
     elif case == "blend_empc_mixed":
         # EMPC Flory beside an EM incompressibility: the EM field keeps its
         # predictor value through the corrector pass

@@ -209,3 +209,61 @@ def test_empc_with_zero_step_is_identity():
     orc.fts_step(s, orc.fts_noise(s.M, 5, 0))
     for p, b in zip(s.potentials, before):
         assert np.array_equal(p.wpl, b)
+
+
+def test_block_debye_terms_sum_to_chain_debye():
+    """Intra-block A + intra-block C + the A-C cross term of a diblock equal
+    the Debye function of the whole chain at every k != 0 (fA + fC = 1)."""
+    k2 = np.linspace(0.0, 40.0, 201)
+    for fA in (0.25, 0.5, 0.7):
+        for alpha in (0.5, 1.0, 2.5):
+            whole = orc.debye_homopolymer(k2, alpha)
+            parts = (orc.debye_intra_block(k2, fA, alpha) + orc.debye_intra_block(k2, 1.0 - fA, alpha)
+                     + orc.debye_inter_block(k2, fA, 0.0, 1.0 - fA, alpha))
+            np.testing.assert_allclose(parts[1:], whole[1:], rtol=1e-12, atol=1e-14)
+            assert parts[0] == 0.0 and whole[0] == 1.0   # the reference's k = 0 values
+
+
+def test_debye_function_limits():
+    x = np.array([0.0, 1e-3, 2e-3, 500.0, 1000.0])
+    g = orc.debye_homopolymer(x, 1.0)
+    np.testing.assert_allclose(g[1:3], 1 - x[1:3] / 3 + x[1:3] ** 2 / 12, rtol=1e-8)
+    np.testing.assert_allclose(g[3:], 2.0 / x[3:] - 2.0 / x[3:] ** 2, rtol=1e-9)
+
+
+def test_linear_coefficients_of_a_diblock():
+    """A+ of Flory (block terms) and of Helfand (whole-chain term) carry the
+    same Debye part at k != 0; at k = 0 Helfand keeps the prefactor and Flory
+    none; A- of Flory is the constant 2C/chiN."""
+    spec = tiny_spec(blocks=((4, 0), (3, 1)))
+    s = orc.fts_system_from_spec(spec, {})
+    s.init_nmolecs()
+    orc.fts_linear_coeffs(s)
+    h, f = s.potentials
+    dh = h.Akpl - s.C / h.kappaN
+    df = f.Akpl - 2.0 * s.C / f.chiN
+    np.testing.assert_allclose(df[1:], dh[1:], rtol=1e-12)
+    m = s.molecules[0]
+    assert df[0] == 0.0 and dh[0] == pytest.approx(m.nmolecs * (7 / s.Nr) * 7 / s.V(), rel=1e-12)
+    assert np.all(f.Akmi == 2.0 * s.C / f.chiN)
+
+
+def test_one_step_scheme_is_consistent_with_em():
+    """Without noise, one 1S step and one EM step from the same state differ
+    by O(dt^2) (the semi-implicit terms cancel to first order): the
+    difference shrinks ~4x when dt halves."""
+    spec = tiny_spec(Nx=(6, 6, 6), L=(2.0, 2.0, 2.0), blocks=((3, 0), (3, 1)), Nr=6.0, C=2.0)
+    fields = random_fields(spec, 0.3, seed=4, imag=False)
+    d = []
+    for dt in (0.02, 0.01):
+        out = []
+        for scheme in ("EM", "1S"):
+            sp = dict(spec)
+            sp["potentials"] = [dict(p, dt_pl=dt, dt_mi=dt, scheme=scheme) for p in spec["potentials"]]
+            s = orc.fts_system_from_spec(sp, fields)
+            orc.fts_init(s)
+            w0 = np.concatenate([s.potentials[0].wpl, s.potentials[1].wpl, s.potentials[1].wmi])
+            orc.fts_step(s, None)
+            out.append(np.concatenate([s.potentials[0].wpl, s.potentials[1].wpl, s.potentials[1].wmi]))
+        d.append(np.max(np.abs(out[0] - out[1])) / np.max(np.abs(out[0] - w0)))
+    assert 1.7 < d[0] / d[1] < 2.3, d   # relative to the O(dt) change: O(dt)
