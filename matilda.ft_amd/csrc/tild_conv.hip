@@ -17,9 +17,10 @@ __device__ __forceinline__ float pot_shape(const KPot& P, float k2, long idx)
 // Fused z pass: forward z FFT of every density channel (already transformed
 // along x and y), k-space forces, inverse z FFT of each output component.
 template <int N, int NCH>
-__global__ void __launch_bounds__(256) k_conv_z(const float2* __restrict__ S, float2* __restrict__ Y, int nx, int ny,
-                                                float Lx, float Ly, float Lz, KParams kp, const float2* __restrict__ twg,
-                                                float invM, double* __restrict__ epart, float Mgvol)
+__device__ __forceinline__ void conv_z_body(const float2* __restrict__ S, float2* __restrict__ Y, int nx, int ny,
+                                            float Lx, float Ly, float Lz, const KParams& kp,
+                                            const float2* __restrict__ twg, float invM, double* __restrict__ epart,
+                                            float Mgvol)
 {
     using P = FftPlanLow<N>;
     constexpr int T = P::T, E = P::E, B = 256 / T;
@@ -37,17 +38,16 @@ __global__ void __launch_bounds__(256) k_conv_z(const float2* __restrict__ S, fl
     LdsMap map{lline, B, true, N};
 
     float2 spec[NCH][E];
+    // every channel's loads in flight before the first transform
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+#pragma unroll
+        for (int s = 0; s < E; ++s)
+            spec[c][s] = (valid && c < kp.nchan) ? S[c * M + line + (long)(t + T * s) * st] : make_float2(0.f, 0.f);
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-        if (c >= kp.nchan) {
-#pragma unroll
-            for (int s = 0; s < E; ++s) spec[c][s] = make_float2(0.f, 0.f);
-            continue;
-        }
-#pragma unroll
-        for (int s = 0; s < E; ++s)
-            spec[c][s] = valid ? S[c * M + line + (long)(t + T * s) * st] : make_float2(0.f, 0.f);
+        if (c >= kp.nchan) continue;
         line_fft_plan<P, -1>(spec[c], lds, tw, t, map);
 #pragma unroll
         for (int s = 0; s < E; ++s) spec[c][s] = cscale(spec[c][s], invM);
@@ -233,6 +233,25 @@ __global__ void __launch_bounds__(256) k_conv_z(const float2* __restrict__ S, fl
     }
 }
 
+// Up to three channels the pass fits 128 registers (four waves per SIMD);
+// more channels keep the register count they need.
+template <int N, int NCH>
+__global__ void __launch_bounds__(256, 4) k_conv_z(const float2* __restrict__ S, float2* __restrict__ Y, int nx, int ny,
+                                                float Lx, float Ly, float Lz, KParams kp, const float2* __restrict__ twg,
+                                                float invM, double* __restrict__ epart, float Mgvol)
+{
+    conv_z_body<N, NCH>(S, Y, nx, ny, Lx, Ly, Lz, kp, twg, invM, epart, Mgvol);
+}
+
+template <int N, int NCH>
+__global__ void __launch_bounds__(256) k_conv_z_wide(const float2* __restrict__ S, float2* __restrict__ Y, int nx,
+                                                     int ny, float Lx, float Ly, float Lz, KParams kp,
+                                                     const float2* __restrict__ twg, float invM,
+                                                     double* __restrict__ epart, float Mgvol)
+{
+    conv_z_body<N, NCH>(S, Y, nx, ny, Lx, Ly, Lz, kp, twg, invM, epart, Mgvol);
+}
+
 __global__ void k_energy_sum(double* __restrict__ out, const double* __restrict__ part, int nblocks, int npots)
 {
     int p = blockIdx.x;
@@ -255,33 +274,58 @@ int tild_conv_blocks(int nx, int ny, int nz)
     return (nx * ny + B - 1) / B;
 }
 
+// lengths whose low-register plan keeps the two- and three-channel pass in
+// 128 registers without spilling
+template <int N>
+constexpr bool conv_four_waves()
+{
+    return N == 32 || N == 64 || N == 128;
+}
+
+template <int N>
+static void launch_conv(TildEngine* e, float invM)
+{
+    const mft_tild_desc& d = e->d;
+    constexpr int B = 256 / FftPlanLow<N>::T;
+    const unsigned grid = (d.nx * d.ny + B - 1) / B;
+    const float Mg = (float)((double)e->gvol * e->M);
+    const float2* tw = fft_twiddles_device(N);
+    bool done = false;
+    if constexpr (conv_four_waves<N>()) {
+        if (e->nchan <= 2) {
+            k_conv_z<N, 2><<<grid, 256, 0, e->stream>>>(e->S, e->Y, d.nx, d.ny, d.lx, d.ly, d.lz, e->kp, tw, invM,
+                                                        e->epart, Mg);
+            done = true;
+        } else if (e->nchan == 3) {
+            k_conv_z<N, 3><<<grid, 256, 0, e->stream>>>(e->S, e->Y, d.nx, d.ny, d.lx, d.ly, d.lz, e->kp, tw, invM,
+                                                        e->epart, Mg);
+            done = true;
+        }
+    }
+    if (done)
+        ;
+    else if (e->nchan <= 2)
+        k_conv_z_wide<N, 2><<<grid, 256, 0, e->stream>>>(e->S, e->Y, d.nx, d.ny, d.lx, d.ly, d.lz, e->kp, tw, invM,
+                                                         e->epart, Mg);
+    else if (e->nchan == 3)
+        k_conv_z_wide<N, 3><<<grid, 256, 0, e->stream>>>(e->S, e->Y, d.nx, d.ny, d.lx, d.ly, d.lz, e->kp, tw, invM,
+                                                         e->epart, Mg);
+    else if (e->nchan <= 4)
+        k_conv_z_wide<N, 4><<<grid, 256, 0, e->stream>>>(e->S, e->Y, d.nx, d.ny, d.lx, d.ly, d.lz, e->kp, tw, invM,
+                                                         e->epart, Mg);
+    else
+        k_conv_z_wide<N, 8><<<grid, 256, 0, e->stream>>>(e->S, e->Y, d.nx, d.ny, d.lx, d.ly, d.lz, e->kp, tw, invM,
+                                                         e->epart, Mg);
+    if (e->kp.energy) k_energy_sum<<<d.npotentials, 256, 0, e->stream>>>(e->eout, e->epart, grid, d.npotentials);
+}
+
 int tild_conv(TildEngine* e)
 {
     const mft_tild_desc& d = e->d;
     float invM = 1.0f / (float)e->M;
     switch (d.nz) {
 #define MFT_CASE(n_)                                                                                      \
-    case n_: {                                                                                            \
-        constexpr int B = 256 / FftPlanLow<n_>::T;                                                        \
-        unsigned grid = (d.nx * d.ny + B - 1) / B;                                                        \
-        float Mg = (float)((double)e->gvol * e->M);                                                       \
-        const float2* tw = fft_twiddles_device(n_);                                                      \
-        if (e->nchan <= 2)                                                                                \
-            k_conv_z<n_, 2><<<grid, 256, 0, e->stream>>>(e->S, e->Y, d.nx, d.ny, d.lx, d.ly, d.lz, e->kp,  \
-                                                         tw, invM, e->epart, Mg);                         \
-        else if (e->nchan == 3)                                                                           \
-            k_conv_z<n_, 3><<<grid, 256, 0, e->stream>>>(e->S, e->Y, d.nx, d.ny, d.lx, d.ly, d.lz, e->kp,  \
-                                                         tw, invM, e->epart, Mg);                         \
-        else if (e->nchan <= 4)                                                                           \
-            k_conv_z<n_, 4><<<grid, 256, 0, e->stream>>>(e->S, e->Y, d.nx, d.ny, d.lx, d.ly, d.lz, e->kp,  \
-                                                         tw, invM, e->epart, Mg);                         \
-        else                                                                                              \
-            k_conv_z<n_, 8><<<grid, 256, 0, e->stream>>>(e->S, e->Y, d.nx, d.ny, d.lx, d.ly, d.lz, e->kp,  \
-                                                         tw, invM, e->epart, Mg);                         \
-        if (e->kp.energy)                                                                                 \
-            k_energy_sum<<<d.npotentials, 256, 0, e->stream>>>(e->eout, e->epart, grid, d.npotentials);   \
-        break;                                                                                            \
-    }
+    case n_: launch_conv<n_>(e, invM); break;
         MFT_FOR_EACH_FFT_SIZE(MFT_CASE)
 #undef MFT_CASE
     default: return MFT_ERR_UNSUPPORTED;

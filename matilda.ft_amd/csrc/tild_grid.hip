@@ -73,16 +73,18 @@ __global__ void __launch_bounds__(256) k_fft_x_inv_packed(const float2* __restri
     const long base = (long)line * N;
     const int g = blockIdx.y;
     float2 pk[3][E];
-    __syncthreads();
-    LdsMap map{lline, B, false, N};
+    // the three packed lines' loads in flight before the first transform
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
         const int q = 3 * g + j;
 #pragma unroll
         for (int s = 0; s < E; ++s)
             pk[j][s] = (valid && q < npk) ? Y[q * M + base + t + T * s] : make_float2(0.f, 0.f);
-        line_fft<N, +1>(pk[j], lds, tw, t, map);
     }
+    __syncthreads();
+    LdsMap map{lline, B, false, N};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) line_fft<N, +1>(pk[j], lds, tw, t, map);
     if (!valid) return;
     // fields 2g (components in pk0.re, pk0.im, pk1.re) and 2g+1 (pk1.im,
     // pk2.re, pk2.im)
