@@ -96,7 +96,8 @@ def compare(s, o, spec, what="", o32=None):
         assert rel(wp, o.potentials[k].wpl) < FIELD_TOL, what
 
 
-@pytest.mark.parametrize("blocks", [((4, 0), (3, 1)), ((2, 0), (3, 1), (2, 0)), ((6, 1),)])
+@pytest.mark.parametrize("blocks", [((4, 0), (3, 1)), ((2, 0), (3, 1), (2, 0)), ((6, 1),), ((1, 0),), ((1, 0), (1, 1)),
+                                    ((1, 0), (5, 1), (1, 0), (2, 1))])
 def test_init_matches_oracle(gpu, blocks):
     spec = tiny_spec(Nx=(16, 12, 20), L=(4.0, 3.0, 5.0), blocks=blocks)
     say("case: fts init", blocks)

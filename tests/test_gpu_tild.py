@@ -309,3 +309,62 @@ def test_large_coacervate_invariants(gpu):
     x2, _, _ = eng2.get_state()
     eng2.close()
     assert np.array_equal(x1, x2), "two identical runs differ"
+
+
+def edge_case_system(case):
+    """Small diblock melts at the edges of the engine's layouts."""
+    if case in ("p1", "p2", "p4"):
+        return systems.tild_diblock(nchains=60, NA=5, NB=5, box=6.0, grid=16, pmeorder=int(case[1]), seed=31)
+    cfg = systems.tild_diblock(nchains=60, NA=5, NB=5, box=6.0, grid=16, seed=32)
+    if case == "ragged_grid":
+        # grid lengths that are not multiples of the brick edge (short last
+        # bricks) on a box that is not a cube
+        cfg["Nx"], cfg["L"] = (20, 12, 24), (7.0, 4.5, 9.0)
+        x = cfg["x"].reshape(-1, 3) % 6.0
+        x = x * (np.asarray(cfg["L"]) / 6.0)
+        cfg["x"] = x.astype(np.float32).reshape(-1)
+    elif case == "empty_species":
+        # a third species with potentials acting on it but no particles
+        cfg["species"] = ["A", "B", "C"]
+        cfg["mass"], cfg["mobility"] = [1.0] * 3, [1.0] * 3
+        cfg["potentials"] = cfg["potentials"] + [dict(kind="gaussian", maskI=4, maskJ=1, A=1.5, sigma=0.5)]
+    elif case == "edge_positions":
+        # particles exactly on the lower faces and just below the upper faces
+        x = cfg["x"].reshape(-1, 3).copy()
+        L = np.float32(6.0)
+        x[0::7, 0] = 0.0
+        x[1::7, 1] = np.nextafter(L, np.float32(0.0))
+        x[2::7, 2] = 0.0
+        x[3::7, :] = np.nextafter(L, np.float32(0.0))
+        cfg["x"] = x.astype(np.float32).reshape(-1)
+    return cfg
+
+
+@pytest.mark.parametrize("case", ["p1", "p2", "p4", "ragged_grid", "empty_species", "edge_positions"])
+def test_engine_edge_cases(gpu, case):
+    """Forces, densities and two GJF steps against the oracle for spline
+    orders 1, 2, 4 (4 with the reference's defect), a grid that the bricks do
+    not divide, a species with no particles, and particles on the box faces."""
+    cfg = edge_case_system(case)
+    say("case: engine edge case", case, "Nx", cfg["Nx"], "p", cfg["pmeorder"])
+    eng, sys_o = _engine_vs_oracle(gpu, cfg)
+    f_g = eng.compute_forces()
+    out = orc.tild_forces(sys_o)
+    rho_g, _ = eng.densities()
+    for s, name in enumerate(cfg["species"]):
+        want = out["rho"].get(name)
+        if want is None or not np.any(want):
+            assert not np.any(rho_g[s]), name
+        else:
+            assert rel(rho_g[s], want) < TOL, name
+    assert rel(f_g, out["f"]) < TOL
+    ns = cfg["topo"]["type"].size
+    for step in range(2):
+        eng.step(1)
+        o = orc.tild_step(sys_o, orc.tild_noise(ns, cfg["seed"], step))
+        sys_o.x = o["x"].astype(np.float32)
+        sys_o.xo = o["xo"].astype(np.float32)
+        sys_o.noise_old = o["noise_old"].astype(np.float32)
+        x_g, _, _ = eng.get_state()
+        assert np.max(periodic_dist(x_g, o["x"], cfg["L"])) < 1e-6 * max(cfg["L"]), step
+    eng.close()
