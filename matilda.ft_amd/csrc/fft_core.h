@@ -193,7 +193,22 @@ struct LdsMap {
     }
 };
 
-template <int N, int T, int R, int NS, int SIGN, bool LAST>
+// Exchange ordering.  When every thread of a line sits in one wavefront
+// (T consecutive threads, T <= 64, not interleaved) the exchange only orders
+// the wave's own LDS accesses, which execute in issue order: a compiler-level
+// barrier suffices.  Otherwise the workgroup synchronises.
+template <bool WAVE>
+__device__ __forceinline__ void exchange_barrier()
+{
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    } else {
+        __syncthreads();
+    }
+}
+
+template <int N, int T, int R, int NS, int SIGN, bool LAST, bool WAVE = false>
 __device__ __forceinline__ void fft_pass(float2 (&v)[N / T], float2* lds, const float2* tw, int t,
                                          const LdsMap& map)
 {
@@ -224,7 +239,7 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[N / T], float2* lds, const 
 #pragma unroll
         for (int s = 0; s < E; ++s) v[s] = out[s];
     } else {
-        __syncthreads(); // every thread finished reading the previous layout
+        exchange_barrier<WAVE>(); // every thread finished reading the previous layout
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             const int j = t + T * b;
@@ -233,35 +248,38 @@ __device__ __forceinline__ void fft_pass(float2 (&v)[N / T], float2* lds, const 
 #pragma unroll
             for (int r = 0; r < R; ++r) lds[map(d + r * NS)] = out[b + r * NB];
         }
-        __syncthreads();
+        exchange_barrier<WAVE>();
 #pragma unroll
         for (int s = 0; s < E; ++s) v[s] = lds[map(t + T * s)];
     }
 }
 
-template <int N, int T, int SIGN, int NS, int R, int... Rest>
+template <int N, int T, int SIGN, bool WAVE, int NS, int R, int... Rest>
 __device__ __forceinline__ void fft_passes(float2 (&v)[N / T], float2* lds, const float2* tw, int t,
                                            const LdsMap& map)
 {
     constexpr bool LAST = sizeof...(Rest) == 0;
-    fft_pass<N, T, R, NS, SIGN, LAST>(v, lds, tw, t, map);
-    if constexpr (!LAST) fft_passes<N, T, SIGN, NS * R, Rest...>(v, lds, tw, t, map);
+    fft_pass<N, T, R, NS, SIGN, LAST, WAVE>(v, lds, tw, t, map);
+    if constexpr (!LAST) fft_passes<N, T, SIGN, WAVE, NS * R, Rest...>(v, lds, tw, t, map);
 }
 
-template <int N, int T, int SIGN, int... Rs>
+template <int N, int T, int SIGN, bool WAVE = false, int... Rs>
 __device__ __forceinline__ void fft_run(float2 (&v)[N / T], float2* lds, const float2* tw, int t,
                                         const LdsMap& map, Radices<Rs...>)
 {
-    fft_passes<N, T, SIGN, 1, Rs...>(v, lds, tw, t, map);
+    fft_passes<N, T, SIGN, WAVE, 1, Rs...>(v, lds, tw, t, map);
 }
 
-// Unnormalised line transform; SIGN = -1 forward, +1 inverse.
-template <int N, int SIGN>
+// Unnormalised line transform; SIGN = -1 forward, +1 inverse.  WAVE: the
+// line's T threads are consecutive threads of one wavefront and the LDS map
+// is not interleaved (exchanges need no workgroup barrier).
+template <int N, int SIGN, bool WAVE = false>
 __device__ __forceinline__ void line_fft(float2 (&v)[N / FftPlan<N>::T], float2* lds, const float2* tw,
                                          int t, const LdsMap& map)
 {
     using P = FftPlan<N>;
-    fft_run<N, P::T, SIGN>(v, lds, tw, t, map, typename P::R{});
+    static_assert(!WAVE || (P::T <= 64 && 64 % P::T == 0), "a line must fit in one wavefront");
+    fft_run<N, P::T, SIGN, WAVE>(v, lds, tw, t, map, typename P::R{});
 }
 
 // The same transform with an explicit plan.

@@ -436,7 +436,8 @@ __global__ void __launch_bounds__(256) k_fts_prop_xy(const PropDesc* __restrict_
         }
     }
     __syncthreads();
-    // rows (x): T consecutive threads per row
+    // rows (x): T consecutive threads of one wavefront per row, each row's
+    // exchanges in its own LDS row: no workgroup barrier until the columns
     {
         const int lx = threadIdx.x / T, t = threadIdx.x % T;
         for (int round = 0; round < NL; ++round) {
@@ -447,21 +448,22 @@ __global__ void __launch_bounds__(256) k_fts_prop_xy(const PropDesc* __restrict_
             for (int s = 0; s < E; ++s) v[s] = valid ? plane[row * RS + t + T * s] : make_float2(0.f, 0.f);
             float2* base = valid ? plane : scratch;
             LdsMap map{valid ? row : lx, N, false, valid ? RS : N};
-            line_fft<N, -1>(v, base, tw, t, map);
+            line_fft<N, -1, true>(v, base, tw, t, map);
 #pragma unroll
             for (int s = 0; s < E; ++s) {
                 const float2 g = gx[t + T * s];
                 v[s] = make_float2(fmaf(v[s].x, g.y, v[s].x * g.x), fmaf(v[s].y, g.y, v[s].y * g.x));
             }
-            line_fft<N, +1>(v, base, tw, t, map);
-            __syncthreads();
+            line_fft<N, +1, true>(v, base, tw, t, map);
+            exchange_barrier<true>();
             if (valid) {
 #pragma unroll
                 for (int s = 0; s < E; ++s) plane[row * RS + t + T * s] = v[s];
             }
-            __syncthreads();
+            exchange_barrier<true>();
         }
     }
+    __syncthreads();
     // columns (y): B consecutive threads cover B columns, element stride RS
     {
         const int lc = threadIdx.x % B, t = threadIdx.x / B;

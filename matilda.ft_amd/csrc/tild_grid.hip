@@ -82,9 +82,10 @@ __global__ void __launch_bounds__(256) k_fft_x_inv_packed(const float2* __restri
             pk[j][s] = (valid && q < npk) ? Y[q * M + base + t + T * s] : make_float2(0.f, 0.f);
     }
     __syncthreads();
+    // a line's T threads are consecutive threads of one wavefront
     LdsMap map{lline, B, false, N};
 #pragma unroll
-    for (int j = 0; j < 3; ++j) line_fft<N, +1>(pk[j], lds, tw, t, map);
+    for (int j = 0; j < 3; ++j) line_fft<N, +1, true>(pk[j], lds, tw, t, map);
     if (!valid) return;
     // fields 2g (components in pk0.re, pk0.im, pk1.re) and 2g+1 (pk1.im,
     // pk2.re, pk2.im)
@@ -139,7 +140,7 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd_tiles(const float* __restrict
 #pragma unroll
     for (int s = 0; s < E; ++s) v[s] = lds[lline * N + t + T * s];
     LdsMap map{lline, B, false, N};
-    line_fft<N, -1>(v, lds, tw, t, map);
+    line_fft<N, -1, true>(v, lds, tw, t, map);
     if (line >= nlines) return;
     float2* dst = out + c * (long)N * nlines + (long)line * N;
 #pragma unroll
