@@ -1,8 +1,12 @@
 // Request loop with continuous batching over the CL-FTS pool and TILD
 // engines (mft_server_* in mft.h).
 //
-// Admission is first-fit in arrival order against a device-memory budget:
-// a request is admitted when the estimate of its device buffers fits beside
+// Admission is first-fit against a device-memory budget, shortest work
+// first: queued requests are tried in order of their estimated device
+// traffic (iterations or steps times the bytes one of them moves), and a
+// request passed over by kAgeLimit admissions of others goes ahead of all
+// requests that have not waited as long, so long requests are not starved.
+// A request is admitted when the estimate of its device buffers fits beside
 // the admitted work.  Admitted CL-FTS solves join the pool after their own
 // first pass; the pool runs in passes of up to max_iters iterations, cut
 // short so that no solve runs past its requested count, and solves that
@@ -38,6 +42,8 @@ struct ServeReq {
     int kind = 0; // 0 CL-FTS, 1 TILD
     int state = MFT_REQ_QUEUED;
     size_t bytes = 0;
+    double work = 0.0;       // estimated device bytes moved by the whole request
+    int passed_over = 0;     // admissions of shorter requests while this one waited
     std::string error;
     // CL-FTS
     mft_fts_desc fd;
@@ -80,18 +86,28 @@ static size_t tild_bytes_estimate(const mft_tild_desc& d)
     return N * per_particle + M * per_point + (1 << 20);
 }
 
+constexpr int kAgeLimit = 32;
+
 static int admit(Server* s)
 {
     std::vector<int> fresh;
     std::vector<int64_t> fresh_ids;
-    for (auto it = s->queue.begin(); it != s->queue.end();) {
-        ServeReq& r = *s->reqs[*it];
+    // candidates: aged requests in arrival order, then by estimated work
+    std::vector<int64_t> order(s->queue.begin(), s->queue.end());
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        const ServeReq& ra = *s->reqs[a];
+        const ServeReq& rb = *s->reqs[b];
+        const bool aa = ra.passed_over >= kAgeLimit, ab = rb.passed_over >= kAgeLimit;
+        if (aa != ab) return aa;
+        if (aa) return false; // both aged: arrival order
+        return ra.work < rb.work;
+    });
+    std::vector<int64_t> admitted;
+    for (int64_t qid : order) {
+        ServeReq& r = *s->reqs[qid];
         const bool fits = s->in_use + (double)r.bytes <= s->budget || (s->in_use == 0.0);
         const bool slot = r.kind == 1 || s->reps_in_use < s->max_reps;
-        if (!fits || !slot) {
-            ++it;
-            continue;
-        }
+        if (!fits || !slot) continue;
         int rc = MFT_OK;
         if (r.kind == 0) {
             rc = fts_pool_add(s->pool, r.fd, &r.rep);
@@ -139,7 +155,18 @@ static int admit(Server* s)
             r.state = MFT_REQ_RUNNING;
             s->in_use += (double)r.bytes;
         }
-        it = s->queue.erase(it);
+        admitted.push_back(qid);
+    }
+    if (!admitted.empty()) {
+        for (auto it = s->queue.begin(); it != s->queue.end();) {
+            if (std::find(admitted.begin(), admitted.end(), *it) != admitted.end()) {
+                it = s->queue.erase(it);
+            } else {
+                // still waiting: passed over by the requests just admitted
+                s->reqs[*it]->passed_over += (int)admitted.size();
+                ++it;
+            }
+        }
     }
     if (!fresh.empty()) {
         int rc = fts_pool_activate(s->pool, fresh);
@@ -278,6 +305,16 @@ int mft_server_submit_fts(mft_server* s, const mft_fts_desc* desc, const float* 
         if (wmi && wmi[p]) r->wmi[p].assign(wmi[p], wmi[p] + 2 * M);
     }
     r->bytes = fts_bytes_estimate(*desc);
+    {
+        // one CL iteration moves about M * 8 * (8 Ns + 12) bytes per molecule
+        double w = 0.0;
+        for (int m = 0; m < desc->nmolecules; ++m) {
+            int ns = 0;
+            for (int b = 0; b < desc->molecules[m].nblocks; ++b) ns += desc->molecules[m].N[b];
+            w += (double)M * 8.0 * (8.0 * ns + 12.0);
+        }
+        r->work = w * (double)niter;
+    }
     r->id = s->next_id++;
     *id = r->id;
     s->queue.push_back(r->id);
@@ -318,6 +355,12 @@ int mft_server_submit_tild(mft_server* s, const mft_tild_desc* desc, const int* 
     r->x.assign(x, x + 3 * N);
     r->nsteps = nsteps;
     r->bytes = tild_bytes_estimate(*desc);
+    {
+        // one step moves about N * 112 + M * channels * 80 bytes
+        const double M = (double)desc->nx * desc->ny * desc->nz;
+        const double chan = desc->nspecies + (desc->has_charges ? 1 : 0);
+        r->work = ((double)N * 112.0 + M * chan * 80.0) * (double)nsteps;
+    }
     r->id = s->next_id++;
     *id = r->id;
     s->queue.push_back(r->id);
