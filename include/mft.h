@@ -192,8 +192,11 @@ int mft_tild_kernel_bytes(mft_tild* eng, double* bytes);
 int mft_tild_num_outputs(mft_tild* eng, int* nout);
 /* Times the kernel variants of this configuration on the device (reps
  * launches each, state unchanged) and keeps the fastest; choice receives the
- * selected gather variant (0 direct loads, 1 LDS-staged bricks). */
+ * selected gather variant (0 direct loads, 1 LDS-staged bricks).  The two
+ * can differ in the last bit of a force. */
 int mft_tild_autotune(mft_tild* eng, int reps, int* choice);
+/* Selects a gather variant directly (as above; the default is 1). */
+int mft_tild_set_gather(mft_tild* eng, int variant);
 
 /* =============================================================== FTS ==== */
 /* Per-kernel entry points (complex interleaved float fields [M]). */

@@ -720,6 +720,14 @@ int mft_tild_kernel_bytes(mft_tild* e, double* bytes)
     return MFT_OK;
 }
 
+int mft_tild_set_gather(mft_tild* e, int variant)
+{
+    MFT_REQUIRE(e, MFT_ERR_INVALID, "null argument");
+    MFT_REQUIRE(variant == 0 || variant == 1, MFT_ERR_INVALID, "gather variant %d", variant);
+    e->gather_mode = variant;
+    return MFT_OK;
+}
+
 int mft_tild_autotune(mft_tild* e, int reps, int* choice)
 {
     MFT_REQUIRE(e && e->have_state && reps >= 1, MFT_ERR_INVALID, "engine has no state");

@@ -343,6 +343,9 @@ class TildEngine:
         mask = 0xFFFFFFFF if names is None else sum(1 << TK_NAMES.index(n) for n in names)
         check(lib().mft_tild_timing_select(self.h, C.c_uint32(mask)))
 
+    def set_gather(self, variant):
+        check(lib().mft_tild_set_gather(self.h, int(variant)))
+
     def autotune(self, reps=5):
         c = _i(0)
         check(lib().mft_tild_autotune(self.h, reps, C.byref(c)))

@@ -368,3 +368,28 @@ def test_engine_edge_cases(gpu, case):
         x_g, _, _ = eng.get_state()
         assert np.max(periodic_dist(x_g, o["x"], cfg["L"])) < 1e-6 * max(cfg["L"]), step
     eng.close()
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("case", ["neutral", "charged", "mixed_charges", "ragged_grid"])
+def test_gather_variants_match_oracle(gpu, case, variant):
+    """Both gather variants (direct loads, LDS-staged bricks) against the
+    oracle's forces and a short trajectory."""
+    if case == "mixed_charges":
+        cfg = mixed_charge_system()
+    elif case == "ragged_grid":
+        cfg = edge_case_system("ragged_grid")
+    else:
+        cfg = small_system(case == "charged", seed=19)
+    say("case: gather variant", variant, case)
+    eng, sys_o = _engine_vs_oracle(gpu, cfg)
+    eng.set_gather(variant)
+    assert rel(eng.compute_forces(), orc.tild_forces(sys_o)["f"]) < TOL
+    ns = cfg["topo"]["type"].size
+    for step in range(2):
+        eng.step(1)
+        o = orc.tild_step(sys_o, orc.tild_noise(ns, cfg["seed"], step))
+        sys_o.x, sys_o.xo = o["x"].astype(np.float32), o["xo"].astype(np.float32)
+        sys_o.noise_old = o["noise_old"].astype(np.float32)
+        assert np.max(periodic_dist(eng.get_state()[0], o["x"], cfg["L"])) < 1e-6 * max(cfg["L"]), step
+    eng.close()
