@@ -387,17 +387,17 @@ __global__ void __launch_bounds__(256) k_fts_prop_axis(const PropDesc* __restric
 // arithmetic, one global read and write instead of two).  Rows are padded by
 // PAD values so the rows a half-wave touches start on different LDS banks;
 // lines past the plane use a scratch region for their exchanges.
-template <int N>
-__global__ void __launch_bounds__(256) k_fts_prop_xy(const PropDesc* __restrict__ dxs,
-                                                     const PropDesc* __restrict__ dys,
-                                                     const float2* __restrict__ twg)
+template <int N, int NT>
+__global__ void __launch_bounds__(NT) k_fts_prop_xy(const PropDesc* __restrict__ dxs,
+                                                    const PropDesc* __restrict__ dys,
+                                                    const float2* __restrict__ twg)
 {
     using P = FftPlan<N>;
-    constexpr int T = P::T, E = P::E, B = 256 / T;
+    constexpr int T = P::T, E = P::E, B = NT / T;
     constexpr int NL = (N + B - 1) / B;
     constexpr int PAD = (N % 16 == 0) ? 8 : 1, RS = N + PAD; // row stride in LDS
     constexpr int NV = N * N / 2;                            // float4 (two values) per plane
-    constexpr int PER = (NV + 255) / 256;
+    constexpr int PER = (NV + NT - 1) / NT;
     __shared__ float2 plane[N * RS];
     __shared__ float2 scratch[(N % B) ? B * N : 1];
     __shared__ float2 tw[N], gx[N], gy[N];
@@ -411,19 +411,19 @@ __global__ void __launch_bounds__(256) k_fts_prop_xy(const PropDesc* __restrict_
         float4 a[PER], e[PER];
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
-            const int v = threadIdx.x + 256 * k;
+            const int v = threadIdx.x + NT * k;
             const int vc = v < NV ? v : NV - 1;
             a[k] = src[vc];
             e[k] = pre ? pre[vc] : make_float4(1.f, 0.f, 1.f, 0.f);
         }
-        for (int i = threadIdx.x; i < N; i += 256) {
+        for (int i = threadIdx.x; i < N; i += NT) {
             tw[i] = twg[i];
             gx[i] = DX.g[i];
             gy[i] = DY.g[i];
         }
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
-            const int v = threadIdx.x + 256 * k;
+            const int v = threadIdx.x + NT * k;
             if (v >= NV) continue;
             float2 lo = make_float2(a[k].x, a[k].y), hi = make_float2(a[k].z, a[k].w);
             if (pre) {
@@ -493,7 +493,7 @@ __global__ void __launch_bounds__(256) k_fts_prop_xy(const PropDesc* __restrict_
     float4* dst = reinterpret_cast<float4*>(DY.dst + off);
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-        const int v = threadIdx.x + 256 * k;
+        const int v = threadIdx.x + NT * k;
         if (v >= NV) continue;
         const int i0 = 2 * v, r = i0 / N, c = i0 - r * N;
         const float2 lo = plane[r * RS + c], hi = plane[r * RS + c + 1];
@@ -1142,7 +1142,9 @@ template <int N>
 static void launch_prop_xy_n(const PropDesc* dx, const PropDesc* dy, int n, const FtsGroup& G, hipStream_t st)
 {
     dim3 grid(G.n[2], n);
-    k_fts_prop_xy<N><<<grid, 256, 0, st>>>(dx, dy, fft_twiddles_device(N));
+    // enough threads for every row (and column) of the plane in one round
+    constexpr int NT = N * FftPlan<N>::T > 256 ? 512 : 256;
+    k_fts_prop_xy<N, NT><<<grid, NT, 0, st>>>(dx, dy, fft_twiddles_device(N));
 }
 
 static int launch_prop_xy(const PropDesc* dx, const PropDesc* dy, int n, const FtsGroup& G, hipStream_t st)
