@@ -98,6 +98,40 @@ int mft_tild_bonds(float* f, const int* nBonds, const int* bondedTo, const int* 
 
 /* Potential kinds (ps_potential.cu:294-333). */
 enum { MFT_POT_GAUSSIAN = 0, MFT_POT_ERF2 = 1, MFT_POT_ERFG = 2, MFT_POT_CHARGES = 3 };
+
+/* initializePotential of one potential (ps_potentialGaussian.cu:28-71,
+ * ps_potentialErf2.cu:31-67, ps_potentialErfG.cu:30-66,
+ * ps_potentialCharges.cu:30-80): uk[M] = u(k) and fk[M*3] = -i k_j u(k),
+ * complex interleaved, device arrays (either may be NULL).  The erf shapes
+ * are evaluated in FP64 and rounded once.  Synchronises the stream. */
+int mft_tild_kspace_table(float* uk, float* fk, int kind, float A, float sigma, float Rp, const int Nx[3],
+                          const float L[3], void* stream);
+/* One direction of PS_Potential::CalcForces (ps_potential.cu:41-120):
+ * gridF[g*3+j] += factor * Re IFFT[fk_j FFT[rho]] with the reference's
+ * forward scaling by 1/M (Box.cu:58-127); call it again with the roles of
+ * the groups swapped, or with factor 2 when a group acts on itself.  work:
+ * device scratch of 2*M complex floats (16*M bytes). */
+int mft_tild_pair_force(float* gridF, const float* rho, const float* fk, const int Nx[3], float factor,
+                        void* work, void* stream);
+/* PS_Potential::CalcEnergy (ps_potential.cu:127-178): gvol * sum rhoI *
+ * Re IFFT[uk FFT[rhoJ]], summed in FP64 (the reference sums in float).
+ * work: device scratch of M complex floats + one double per 256 grid points.
+ * Synchronises the stream; *energy is a host pointer. */
+int mft_tild_potential_energy(double* energy, const float* rhoI, const float* rhoJ, const float* uk,
+                              const int Nx[3], double gvol, void* work, void* stream);
+/* GJF Integrate_2 on group "all" (ps_integratorGJF.cu:102-163) with the
+ * coefficients of GJF::finishInitialization (ps_integratorGJF.cu:56-82, in
+ * float): x, xo, noise_old [n*3] updated in place, f [n*3], type [n]; mass
+ * and mobility are host arrays [nspecies] (NULL = 1).  Noise: Philox
+ * normals keyed (particle, step, 'TILD', 0) by seed, the same stream the
+ * engine draws. */
+int mft_tild_gjf_step(float* x, float* xo, float* noise_old, const float* f, const int* type, int nparticles,
+                      int nspecies, const float* mass, const float* mobility, float delt, const float L[3],
+                      uint64_t seed, int64_t step, void* stream);
+/* EM Integrate_2 (ps_integratorEM.cu:39-82): x += delt f + sqrt(2 delt) N(0,1),
+ * wrapped into the box; noise as above. */
+int mft_tild_em_step(float* x, const float* f, int nparticles, float delt, const float L[3], uint64_t seed,
+                     int64_t step, void* stream);
 /* Integrators (ps_integrator.cu:51-69). */
 enum { MFT_INT_GJF = 0, MFT_INT_EM = 1, MFT_INT_NONE = 2 };
 
@@ -271,6 +305,49 @@ int mft_fts_get_propagators(mft_fts* sys, int m, float* q, float* qdag);
 /* Q[m] (complex), H (complex) and its parts: potentials then molecules
  * (FTS_Box::computeHamiltonian, FTS_Box.cu:590-602), FP64 reductions. */
 int mft_fts_observables(mft_fts* sys, double* Q, double* H, double* parts);
+
+/* Per-kernel FTS entry points on caller-owned device arrays in the
+ * reference's q-form layouts (complex interleaved float [M] per field,
+ * x-fastest).  The engine above fuses the same arithmetic; these follow the
+ * reference's data flow one step at a time. */
+
+/* FTS_Species::buildPotentialField (fts_species.cu:53-132) for every
+ * species: W[s*M+i] = (sum over potentials of i w+ and -/+ w- of Flory
+ * pairs on I/J) / Nr, accumulated in FP64.  wpl[p], wmi[p]: device fields
+ * of potential p (wmi may be NULL when no potential is Flory). */
+int mft_fts_species_fields(float* W, int nspecies, int npotentials, const mft_fts_potential* pots,
+                           const float* const* wpl, const float* const* wmi, double Nr, long M, void* stream);
+/* One potential's updateFields with Euler-Maruyama (fts_potential_helfand.cu:77-152,
+ * fts_potential_flory.cu:123-167, fts_potential_incompress.cu:84-113; force
+ * d_make*Force, step d_fts_updateEM fts_updateSchemes.cu:8-29) at the
+ * species densities rho[s] (device [M] complex each).  cl != 0 adds the real
+ * noise sqrt(2 delt / gvol) N(0,1) (fts_make_CLnoise.cu:7-24), the normal in
+ * column noise_col (w+) and noise_col+1 (Flory w-) of the 8 drawn per grid
+ * point and iteration: counters (i, iter, 'FTSN', 0) and (i, iter, 'FTSN', 1)
+ * by seed, the engine's columns for potentials in declaration order. */
+int mft_fts_update_fields(float* wpl, float* wmi, const mft_fts_potential* pot, const float* const* rho,
+                          int nspecies, double C, double Nr, const int Nx[3], const float L[3], int cl,
+                          uint64_t seed, int64_t iter, int noise_col, void* stream);
+/* LinearMolec::calcPropagators (fts_molecule_linear.cu:144-269): q[Ntot*M]
+ * and qdag[Ntot*M] (either may be NULL) from the species fields W and the
+ * bond factor g (mft_fts_bond_factor), q_s = e^{-W} IFFT[g FFT[q_{s-1}]]. */
+int mft_fts_propagate(float* q, float* qdag, const float* W, const float* g, const mft_fts_molecule* mol,
+                      const int Nx[3], void* stream);
+/* LinearMolec::calcDensity (fts_molecule_linear.cu:278-363): adds
+ * factor e^{+W_b} sum_s q_s qdag_{Ntot-1-s} of every block b to rho of its
+ * species (rho [nspecies*M]); factor = nmolecs / (Q V), complex. */
+int mft_fts_density(float* rho, const float* q, const float* qdag, const float* W, const mft_fts_molecule* mol,
+                    double factor_re, double factor_im, long M, void* stream);
+/* Q = gvol / V * sum q_{Ntot-1} (fts_molecule_linear.cu:265-266), FP64 sum.
+ * Q: host double[2]; work: device scratch of 32 bytes per 256 grid points.
+ * Synchronises the stream. */
+int mft_fts_partition_function(double* Q, const float* q_last, double gvol, double V, long M, void* work,
+                               void* stream);
+/* One potential's Hamiltonian term (fts_potential_flory.cu:252-274,
+ * fts_potential_helfand.cu:254-275, fts_potential_incompress.cu:232-247),
+ * FP64 sums.  H: host double[2]; work as above.  Synchronises the stream. */
+int mft_fts_hamiltonian_term(double* H, const mft_fts_potential* pot, const float* wpl, const float* wmi, double C,
+                             double gvol, long M, void* work, void* stream);
 
 /* Batched CL-FTS: many independent replicas of uneven sizes advanced by the
  * same launches (replicas with equal grids share every launch).  Replica r

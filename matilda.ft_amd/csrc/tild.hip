@@ -16,16 +16,16 @@ static int check_fft_size(int n, const char* what)
 
 // host-side erf shape tables in FP64 (the reference's FP32 table cancels
 // catastrophically at small k; this is the exact function, rounded once)
-static void erf_table(int kind, double sigma, double Rp, const mft_tild_desc& d, std::vector<float>& out)
+void erf_table(int kind, double sigma, double Rp, int nx, int ny, int nz, float lx, float ly, float lz,
+               std::vector<float>& out)
 {
-    const int nx = d.nx, ny = d.ny, nz = d.nz;
     out.resize((size_t)nx * ny * nz);
     const double PI4 = 12.56637061435917295384;
     double sig2 = sigma * sigma;
     for (int iz = 0; iz < nz; ++iz)
         for (int iy = 0; iy < ny; ++iy)
             for (int ix = 0; ix < nx; ++ix) {
-                float kx = kcomp(ix, nx, d.lx), ky = kcomp(iy, ny, d.ly), kz = kcomp(iz, nz, d.lz);
+                float kx = kcomp(ix, nx, lx), ky = kcomp(iy, ny, ly), kz = kcomp(iz, nz, lz);
                 double k2 = (double)(float)((float)(kx * kx + ky * ky) + kz * kz);
                 double k = std::sqrt(k2);
                 double vol = PI4 * Rp * Rp * Rp / 3.0;
@@ -103,7 +103,7 @@ static int tild_setup(TildEngine* e)
         K.table = nullptr;
         if (P.kind == MFT_POT_ERF2 || P.kind == MFT_POT_ERFG) {
             std::vector<float> tab;
-            erf_table(P.kind, P.sigma, P.Rp, d, tab);
+            erf_table(P.kind, P.sigma, P.Rp, d.nx, d.ny, d.nz, d.lx, d.ly, d.lz, tab);
             if ((rc = dalloc(&e->tables[i], tab.size()))) return rc;
             MFT_HIP(hipMemcpy(e->tables[i], tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
             K.table = e->tables[i];

@@ -270,6 +270,8 @@ int tild_density(TildEngine* e);          // tiles of the current positions
 int tild_density_reduce(TildEngine* e);   // tiles -> rho (when the densities are read)
 int tild_gather(TildEngine* e, int mode);
 int tild_integrate(TildEngine* e, bool want_f);
+void erf_table(int kind, double sigma, double Rp, int nx, int ny, int nz, float lx, float ly, float lz,
+               std::vector<float>& out);
 int tild_bond_energy(TildEngine* e);
 
 } // namespace mft

@@ -232,6 +232,68 @@ def bonds(x, topo, bond_k, bond_r0, bond_style, L, maxbonds=2):
     return f.to_array(np.float32, 3 * ns)
 
 
+def _i3(v):
+    return _ptr(np.asarray(v, np.int32))
+
+
+def _f3(v):
+    return _ptr(np.asarray(v, np.float32))
+
+
+def kspace_table(kind, A, sigma, Rp, Nx, L):
+    """mft_tild_kspace_table: (u(k) [M], f(k) [M, 3]) complex64, x fastest."""
+    M = int(np.prod(Nx))
+    u = DeviceBuffer(8 * M)
+    f = DeviceBuffer(24 * M)
+    check(lib().mft_tild_kspace_table(u.ptr, f.ptr, POT[kind], _f(A), _f(sigma), _f(Rp), _i3(Nx), _f3(L), None))
+    return u.to_array(np.complex64, M), f.to_array(np.complex64, (M, 3))
+
+
+def pair_force(gridF, rho, fk, Nx, factor=1.0):
+    """mft_tild_pair_force: gridF [M*3] + factor Re IFFT[fk_j FFT[rho]]."""
+    M = int(np.prod(Nx))
+    g = DeviceBuffer.from_array(np.asarray(gridF, np.float32))
+    r = DeviceBuffer.from_array(np.asarray(rho, np.float32))
+    fb = DeviceBuffer.from_array(np.asarray(fk, np.complex64))
+    work = DeviceBuffer(16 * M)
+    check(lib().mft_tild_pair_force(g.ptr, r.ptr, fb.ptr, _i3(Nx), _f(factor), work.ptr, None))
+    return g.to_array(np.float32, 3 * M)
+
+
+def potential_energy(rhoI, rhoJ, uk, Nx, gvol):
+    """mft_tild_potential_energy (FP64 sum)."""
+    M = int(np.prod(Nx))
+    a = DeviceBuffer.from_array(np.asarray(rhoI, np.float32))
+    b = DeviceBuffer.from_array(np.asarray(rhoJ, np.float32))
+    u = DeviceBuffer.from_array(np.asarray(uk, np.complex64))
+    work = DeviceBuffer(8 * M + 8 * ((M + 255) // 256))
+    e = _d(0.0)
+    check(lib().mft_tild_potential_energy(C.byref(e), a.ptr, b.ptr, u.ptr, _i3(Nx), _d(gvol), work.ptr, None))
+    return e.value
+
+
+def gjf_step(x, xo, noise_old, f, types, mass, mobility, delt, L, seed, step):
+    """mft_tild_gjf_step on host arrays; returns (x, xo, noise_old)."""
+    n = np.asarray(types).size
+    bufs = [DeviceBuffer.from_array(np.asarray(a, np.float32)) for a in (x, xo, noise_old, f)]
+    tb = DeviceBuffer.from_array(np.asarray(types, np.int32))
+    m = np.asarray(mass, np.float32)
+    D = np.asarray(mobility, np.float32)
+    check(lib().mft_tild_gjf_step(bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, bufs[3].ptr, tb.ptr, n, m.size, _ptr(m),
+                                  _ptr(D), _f(delt), _f3(L), C.c_uint64(seed), C.c_int64(step), None))
+    return tuple(b.to_array(np.float32, 3 * n) for b in bufs[:3])
+
+
+def em_step(x, f, delt, L, seed, step):
+    """mft_tild_em_step on host arrays; returns x."""
+    x = np.asarray(x, np.float32)
+    n = x.size // 3
+    xb = DeviceBuffer.from_array(x)
+    fb = DeviceBuffer.from_array(np.asarray(f, np.float32))
+    check(lib().mft_tild_em_step(xb.ptr, fb.ptr, n, _f(delt), _f3(L), C.c_uint64(seed), C.c_int64(step), None))
+    return xb.to_array(np.float32, 3 * n)
+
+
 def tild_desc(Nx, L, pmeorder, nspecies, nparticles, potentials, bond_k=(), bond_r0=(), bond_style=(),
               integrator="GJF", delt=0.005, mass=None, mobility=None, seed=1234, has_charges=False, maxbonds=2):
     """mft_tild_desc and the arrays it points to (keep them alive)."""
@@ -407,6 +469,112 @@ def fts_desc(spec):
 
 def _c64(a):
     return np.ascontiguousarray(a, np.complex64)
+
+
+def _fts_pot(p):
+    return FtsPotentialC(FTS_POT[p["kind"]], p.get("I", -1), p.get("J", -1), p.get("chiN", 0.0),
+                         p.get("kappaN", 0.0), p.get("dt_pl", 0.0), p.get("dt_mi", 0.0),
+                         FTS_SCHEME[p.get("scheme", "EM")])
+
+
+def _fts_mol(blocks):
+    mc = FtsMoleculeC()
+    mc.nblocks = len(blocks)
+    for j, (n, s) in enumerate(blocks):
+        mc.N[j] = n
+        mc.species[j] = s
+    mc.phi = 1.0
+    return mc
+
+
+def _ptr_array(bufs):
+    arr = (_p * max(1, len(bufs)))()
+    for k, b in enumerate(bufs):
+        arr[k] = b.ptr if b is not None else None
+    return arr
+
+
+def fts_bond_factor(Nx, L, Nr):
+    """mft_fts_bond_factor: exp(-k^2/(Nr-1)) as complex64 [M]."""
+    M = int(np.prod(Nx))
+    g = DeviceBuffer(8 * M)
+    check(lib().mft_fts_bond_factor(g.ptr, _i3(Nx), _f3(L), _d(Nr), None))
+    return g.to_array(np.complex64, M)
+
+
+def fts_species_fields(pots, wpl, wmi, nspecies, Nr):
+    """mft_fts_species_fields; pots: dicts as fts_desc, wpl/wmi: [npots] complex
+    arrays (wmi entries None for non-Flory).  Returns W [nspecies, M]."""
+    M = np.asarray(wpl[0]).size
+    pc = (FtsPotentialC * max(1, len(pots)))(*[_fts_pot(p) for p in pots])
+    wp = [DeviceBuffer.from_array(_c64(w)) for w in wpl]
+    wm = [DeviceBuffer.from_array(_c64(w)) if w is not None else None for w in wmi]
+    W = DeviceBuffer(8 * M * nspecies)
+    check(lib().mft_fts_species_fields(W.ptr, nspecies, len(pots), pc, _ptr_array(wp), _ptr_array(wm), _d(Nr),
+                                       C.c_long(M), None))
+    return W.to_array(np.complex64, (nspecies, M))
+
+
+def fts_update_fields(pot, wpl, wmi, rho, C_, Nr, Nx, L, cl, seed, it, noise_col):
+    """mft_fts_update_fields; returns (wpl, wmi)."""
+    M = int(np.prod(Nx))
+    pc = _fts_pot(pot)
+    a = DeviceBuffer.from_array(_c64(wpl))
+    b = DeviceBuffer.from_array(_c64(wmi)) if wmi is not None else None
+    rb = [DeviceBuffer.from_array(_c64(r)) for r in rho]
+    check(lib().mft_fts_update_fields(a.ptr, b.ptr if b else None, C.byref(pc), _ptr_array(rb), len(rb), _d(C_),
+                                      _d(Nr), _i3(Nx), _f3(L), int(cl), C.c_uint64(seed), C.c_int64(it), noise_col,
+                                      None))
+    return a.to_array(np.complex64, M), (b.to_array(np.complex64, M) if b else None)
+
+
+def fts_propagate(W, g, blocks, Nx):
+    """mft_fts_propagate: (q, qdag) [Ntot, M] from W [nspecies, M] and g [M]."""
+    M = int(np.prod(Nx))
+    Ntot = sum(n for n, _ in blocks)
+    mc = _fts_mol(blocks)
+    Wb = DeviceBuffer.from_array(_c64(W))
+    gb = DeviceBuffer.from_array(_c64(g))
+    q = DeviceBuffer(8 * M * Ntot)
+    qd = DeviceBuffer(8 * M * Ntot)
+    check(lib().mft_fts_propagate(q.ptr, qd.ptr, Wb.ptr, gb.ptr, C.byref(mc), _i3(Nx), None))
+    return q.to_array(np.complex64, (Ntot, M)), qd.to_array(np.complex64, (Ntot, M))
+
+
+def fts_density(rho, q, qdag, W, blocks, factor):
+    """mft_fts_density: rho [nspecies, M] + the molecule's block densities."""
+    rho = _c64(rho)
+    M = rho.shape[-1]
+    mc = _fts_mol(blocks)
+    bufs = [DeviceBuffer.from_array(_c64(a)) for a in (rho, q, qdag, W)]
+    check(lib().mft_fts_density(bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, bufs[3].ptr, C.byref(mc), _d(factor.real),
+                                _d(factor.imag), C.c_long(M), None))
+    return bufs[0].to_array(np.complex64, rho.shape)
+
+
+def fts_partition_function(q_last, gvol, V):
+    """mft_fts_partition_function: complex Q."""
+    q_last = _c64(q_last)
+    M = q_last.size
+    qb = DeviceBuffer.from_array(q_last)
+    work = DeviceBuffer(32 * ((M + 255) // 256))
+    out = (_d * 2)()
+    check(lib().mft_fts_partition_function(out, qb.ptr, _d(gvol), _d(V), C.c_long(M), work.ptr, None))
+    return complex(out[0], out[1])
+
+
+def fts_hamiltonian_term(pot, wpl, wmi, C_, gvol):
+    """mft_fts_hamiltonian_term: complex H of one potential."""
+    wpl = _c64(wpl)
+    M = wpl.size
+    pc = _fts_pot(pot)
+    a = DeviceBuffer.from_array(wpl)
+    b = DeviceBuffer.from_array(_c64(wmi)) if wmi is not None else None
+    work = DeviceBuffer(32 * ((M + 255) // 256))
+    out = (_d * 2)()
+    check(lib().mft_fts_hamiltonian_term(out, C.byref(pc), a.ptr, b.ptr if b else None, _d(C_), _d(gvol),
+                                         C.c_long(M), work.ptr, None))
+    return complex(out[0], out[1])
 
 
 class FtsSystem:
