@@ -239,7 +239,10 @@ int mft_tild_set_gather(mft_tild* eng, int variant);
  * as complex values like the reference's d_bond_fft. */
 int mft_fts_bond_factor(float* g2, const int Nx[3], const float L[3], double Nr, void* stream);
 
-enum { MFT_FTS_HELFAND = 0, MFT_FTS_FLORY = 1, MFT_FTS_INCOMPRESS = 2 };
+/* Potential styles of FTS_PotentialFactory (fts_potential.cu:25-55).  Edwards
+ * (fts_potential_edwards.cu) is Helfand's excluded volume with force
+ * w/B + i rho/Nr (no C factor, no -iC term) and H = gvol sum w^2 / (2B). */
+enum { MFT_FTS_HELFAND = 0, MFT_FTS_FLORY = 1, MFT_FTS_INCOMPRESS = 2, MFT_FTS_EDWARDS = 3 };
 /* updateScheme (fts_potential_flory.cu:90-109): Euler-Maruyama; the
  * predictor-corrector of fts_updateSchemes.cu:56-81 (a box with any EMPC
  * potential recomputes its densities twice per iteration, FTS_Box.cu:53-81);
@@ -252,7 +255,8 @@ typedef struct {
     int kind;             /* MFT_FTS_* (fts_potential.cu:25-55) */
     int I, J;             /* species indices (Flory) */
     double chiN;          /* Flory chi N (fts_potential_flory.cu:49) */
-    double kappaN;        /* Helfand kappa N (fts_potential_helfand.cu:26) */
+    double kappaN;        /* Helfand kappa N (fts_potential_helfand.cu:26); Edwards B
+                             (fts_potential_edwards.cu:18) */
     double delt_pl;       /* time step of w+ (delt for Helfand/Incompress) */
     double delt_mi;       /* time step of w- (Flory) */
     int scheme;           /* MFT_FTS_EM (0, the reference's default), _EMPC or _1S */
@@ -264,6 +268,10 @@ typedef struct {
     int species[16];      /* species index per block */
     double phi;           /* volume fraction (canonical); nmolecs derived as in
                              fts_molecule_linear.cu:433-435 */
+    double activity;      /* > 0: grand canonical (fts_molecule.cu:30-31): density
+                             factor = activity, H term -activity V Q, nSites =
+                             activity V Re Q Ntot (fts_molecule_linear.cu:278-382);
+                             phi is then ignored */
 } mft_fts_molecule;
 
 typedef struct {
@@ -288,6 +296,14 @@ int mft_fts_create(mft_fts** sys, const mft_fts_desc* desc);
 int mft_fts_destroy(mft_fts* sys);
 void* mft_fts_stream(mft_fts* sys);
 int mft_fts_set_fields(mft_fts* sys, int pot, const float* wpl, const float* wmi);
+/* LinearMolec::modifyMolecule "activity" (fts_molecule_linear.cu:386-412): the
+ * new activity of molecule m (grand canonical only) enters the observables at
+ * once and the densities from the next density pass on, so the next field
+ * update still uses the densities of the old activity, as in the reference. */
+int mft_fts_set_activity(mft_fts* sys, int m, double activity);
+/* Sites of every molecule: activity V Re Q Ntot (grand canonical,
+ * fts_molecule_linear.cu:344-346) or nmolecs Ntot (canonical). */
+int mft_fts_nsites(mft_fts* sys, double* nsites);
 int mft_fts_get_fields(mft_fts* sys, int pot, float* wpl, float* wmi);
 /* FTS_Box::finishInitialization density pass (FTS_Box.cu:193-256). */
 int mft_fts_init(mft_fts* sys);

@@ -42,6 +42,7 @@ struct FtsPotDev {
     int kind, I, J;
     int scheme;               // MFT_FTS_EM or MFT_FTS_EMPC
     double cpl, cmi;          // force coefficients of w+ and w-
+    double cterm;             // constant imaginary force: C (Helfand, Incompress), 0 (Edwards)
     double dt_pl, dt_mi;
     double nmag_pl, nmag_mi;  // sqrt(2 dt / gvol) or 0 (no noise)
     int col_pl, col_mi;       // noise normal index or -1
@@ -62,6 +63,7 @@ struct FtsUpdDev {
     float2* rhot;                  // [nblk][M] unscaled block densities
     const double2* Q;              // [nmol]
     double nmolecs[kFtsMaxMol];
+    double activity[kFtsMaxMol];   // > 0: grand canonical, density factor = activity
     double V, C, Nr;
     float2* W;                     // [nspecies][M]
     float2* E;                     // [nspecies][M]
@@ -152,6 +154,11 @@ __global__ void __launch_bounds__(kFtsThreads) k_fts_update(const FtsUpdDev* __r
                 // factor = n / (Q V)
                 double den = (Q.x * Q.x + Q.y * Q.y) * D.V;
                 double fr = D.nmolecs[m] * Q.x / den, fi = -D.nmolecs[m] * Q.y / den;
+                if (D.activity[m] > 0.0) {
+                    // fts_molecule_linear.cu:289-291
+                    fr = D.activity[m];
+                    fi = 0.0;
+                }
                 float2 r = D.rhot[(long)b * D.M + i];
                 add_at(rho, D.blk_species[b], fr * r.x - fi * r.y, fr * r.y + fi * r.x);
             }
@@ -243,7 +250,7 @@ __global__ void __launch_bounds__(kFtsThreads) k_fts_update(const FtsUpdDev* __r
                     if (P.kind == MFT_FTS_HELFAND) {
                         // d_makeHelfandForce (fts_potential_helfand.cu:223-240)
                         fx = P.cpl * w.x - rt.y / D.Nr;
-                        fy = P.cpl * w.y + rt.x / D.Nr - D.C;
+                        fy = P.cpl * w.y + rt.x / D.Nr - P.cterm;
                     } else {
                         // d_makeIncompressForce (fts_potential_incompress.cu:202-218)
                         fx = -rt.y / D.Nr;
@@ -605,6 +612,10 @@ __global__ void k_fts_species_rho(const FtsUpdDev* __restrict__ descs, float2* _
             double2 Q = D.Q[m];
             double den = (Q.x * Q.x + Q.y * Q.y) * D.V;
             double fr = D.nmolecs[m] * Q.x / den, fi = -D.nmolecs[m] * Q.y / den;
+            if (D.activity[m] > 0.0) {
+                fr = D.activity[m];
+                fi = 0.0;
+            }
             float2 r = D.rhot[(long)b * D.M + i];
             rx += fr * r.x - fi * r.y;
             ry += fr * r.y + fi * r.x;
@@ -622,6 +633,10 @@ struct FtsRep {
     int Ntot[kFtsMaxMol] = {0};
     bool sym[kFtsMaxMol] = {false};
     double nmolecs[kFtsMaxMol] = {0};
+    // activity the device descriptors use; a change takes effect after the
+    // next density pass (pending until then)
+    double act_dev[kFtsMaxMol] = {0};
+    bool act_pending = false;
     int nblk = 0;
     float2 *wpl[kFtsMaxPots] = {nullptr}, *wmi[kFtsMaxPots] = {nullptr};
     // EMPC predictor data per potential (fields before the predictor, forces)
@@ -725,7 +740,7 @@ static int validate_desc(const mft_fts_desc& d)
     int noisy = 0;
     for (int p = 0; p < d.npotentials; ++p) {
         const mft_fts_potential& P = d.potentials[p];
-        MFT_REQUIRE(P.kind >= 0 && P.kind <= MFT_FTS_INCOMPRESS, MFT_ERR_UNSUPPORTED, "FTS potential kind %d",
+        MFT_REQUIRE(P.kind >= 0 && P.kind <= MFT_FTS_EDWARDS, MFT_ERR_UNSUPPORTED, "FTS potential kind %d",
                     P.kind);
         MFT_REQUIRE(P.scheme == MFT_FTS_EM || P.scheme == MFT_FTS_EMPC || P.scheme == MFT_FTS_1S,
                     MFT_ERR_UNSUPPORTED, "update scheme %d", P.scheme);
@@ -734,8 +749,8 @@ static int validate_desc(const mft_fts_desc& d)
                         "Flory species out of range");
             MFT_REQUIRE(P.chiN != 0.0, MFT_ERR_INVALID, "Flory chiN must be nonzero");
             noisy += 2;
-        } else if (P.kind == MFT_FTS_HELFAND) {
-            MFT_REQUIRE(P.kappaN != 0.0, MFT_ERR_INVALID, "Helfand kappaN must be nonzero");
+        } else if (P.kind == MFT_FTS_HELFAND || P.kind == MFT_FTS_EDWARDS) {
+            MFT_REQUIRE(P.kappaN != 0.0, MFT_ERR_INVALID, "Helfand kappaN / Edwards B must be nonzero");
             noisy += 1;
         }
     }
@@ -749,7 +764,8 @@ static int validate_desc(const mft_fts_desc& d)
             MFT_REQUIRE(mol.species[b] >= 0 && mol.species[b] < d.nspecies, MFT_ERR_INVALID,
                         "species not found!");
         }
-        MFT_REQUIRE(mol.phi > 0.0, MFT_ERR_INVALID, "molecule phi must be positive");
+        MFT_REQUIRE(mol.phi > 0.0 || mol.activity > 0.0, MFT_ERR_INVALID,
+                    "molecule needs a positive phi or activity");
     }
     return MFT_OK;
 }
@@ -803,7 +819,11 @@ static int linear_coeffs(FtsRep& r)
     std::vector<std::vector<double>> Apl(d.npotentials), Ami(d.npotentials);
     for (int p = 0; p < d.npotentials; ++p) {
         const mft_fts_potential& P = d.potentials[p];
-        const double a0 = P.kind == MFT_FTS_HELFAND ? d.C / P.kappaN : (P.kind == MFT_FTS_FLORY ? 2.0 * d.C / P.chiN : 0.0);
+        // fts_potential_edwards.cu initLinearCoeffs: 1/B
+        const double a0 = P.kind == MFT_FTS_HELFAND
+                              ? d.C / P.kappaN
+                              : (P.kind == MFT_FTS_FLORY ? 2.0 * d.C / P.chiN
+                                                         : (P.kind == MFT_FTS_EDWARDS ? 1.0 / P.kappaN : 0.0));
         Apl[p].assign(M, a0);
         if (P.kind == MFT_FTS_FLORY) Ami[p].assign(M, a0);
     }
@@ -815,7 +835,8 @@ static int linear_coeffs(FtsRep& r)
         for (int p = 0; p < d.npotentials; ++p) {
             const mft_fts_potential& P = d.potentials[p];
             const bool homo_flory = P.kind == MFT_FTS_FLORY && mol.nblocks == 1;
-            if (P.kind == MFT_FTS_HELFAND || homo_flory) {
+            // fts_molecule_linear.cu:459: Helfand and Edwards take the chain Debye term
+            if (P.kind == MFT_FTS_HELFAND || P.kind == MFT_FTS_EDWARDS || homo_flory) {
                 Apl[p][0] += pref;
                 for (size_t i = 1; i < M; ++i) Apl[p][i] += pref * debye_chain(k2[i], alpha);
                 continue;
@@ -898,7 +919,11 @@ static int rep_setup(FtsBatch* b, FtsRep& r)
                 sym = false;
         r.sym[m] = sym;
         // fts_molecule_linear.cu:433-435 with Vfree = V
-        r.nmolecs[m] = d.C * mol.phi * r.V * d.Nr / (double)Nt;
+        // in the grand canonical case phi stays -1 (fts_molecule.cu:19), so
+        // this count is negative and only the 1S Debye prefactor uses it
+        // (fts_molecule_linear.cu:563-566)
+        r.nmolecs[m] = d.C * (mol.activity > 0.0 ? -1.0 : mol.phi) * r.V * d.Nr / (double)Nt;
+        r.act_dev[m] = mol.activity > 0.0 ? mol.activity : 0.0;
         if ((rc = dev_alloc(r.allocs, &r.p[m], M * Nt))) return rc;
         if (sym) r.pd[m] = r.p[m];
         else if ((rc = dev_alloc(r.allocs, &r.pd[m], M * Nt))) return rc;
@@ -960,8 +985,12 @@ static void fill_upd(const FtsBatch* b, const FtsRep& r, FtsUpdDev& U, int first
                 D.nmag_pl = std::sqrt(2.0 * P.delt_pl / r.gvol);
                 D.nmag_mi = std::sqrt(2.0 * P.delt_mi / r.gvol);
             }
-        } else if (P.kind == MFT_FTS_HELFAND) {
-            D.cpl = d.C / P.kappaN;
+        } else if (P.kind == MFT_FTS_HELFAND || P.kind == MFT_FTS_EDWARDS) {
+            // the device runs Edwards as Helfand with 1/B and no -iC term
+            // (d_makeEdwardsForce, fts_potential_edwards.cu:227-244)
+            D.kind = MFT_FTS_HELFAND;
+            D.cpl = P.kind == MFT_FTS_EDWARDS ? 1.0 / P.kappaN : d.C / P.kappaN;
+            D.cterm = P.kind == MFT_FTS_EDWARDS ? 0.0 : d.C;
             if (d.cl) {
                 D.col_pl = col++;
                 D.nmag_pl = std::sqrt(2.0 * P.delt_pl / r.gvol);
@@ -977,7 +1006,10 @@ static void fill_upd(const FtsBatch* b, const FtsRep& r, FtsUpdDev& U, int first
     U.nblk = k;
     U.rhot = r.rhot;
     U.Q = r.Q;
-    for (int m = 0; m < d.nmolecules; ++m) U.nmolecs[m] = r.nmolecs[m];
+    for (int m = 0; m < d.nmolecules; ++m) {
+        U.nmolecs[m] = r.nmolecs[m];
+        U.activity[m] = r.act_dev[m];
+    }
     U.V = r.V;
     U.C = d.C;
     U.Nr = d.Nr;
@@ -1538,6 +1570,27 @@ static int graph_capture(FtsBatch* b)
     return MFT_OK;
 }
 
+// Activity changes wait for one density pass (mft_fts_set_activity): once an
+// iteration has run, the descriptors take the new values.
+static int apply_pending(FtsBatch* b)
+{
+    bool any = false;
+    for (auto& r : b->reps) any = any || (r.alive && r.act_pending);
+    if (!any) return MFT_OK;
+    MFT_HIP(hipStreamSynchronize(b->stream));
+    for (auto& r : b->reps) {
+        if (!r.alive || !r.act_pending) continue;
+        for (int m = 0; m < r.d.nmolecules; ++m)
+            r.act_dev[m] = r.d.molecules[m].activity > 0.0 ? r.d.molecules[m].activity : 0.0;
+        r.act_pending = false;
+        group_for(b, r).dirty = true;
+    }
+    int rc;
+    for (auto& G : b->groups)
+        if (!G.reps.empty() && (rc = group_ready(b, G))) return rc;
+    return MFT_OK;
+}
+
 static int batch_step(FtsBatch* b, int niter)
 {
     MFT_REQUIRE(b->inited, MFT_ERR_INVALID, "call init before stepping");
@@ -1557,6 +1610,7 @@ static int batch_step(FtsBatch* b, int niter)
                 b->use_graph = false;
                 if ((rc = batch_iteration(b))) return rc;
                 b->iter++;
+                if ((rc = apply_pending(b))) return rc;
                 continue;
             }
             MFT_HIP(hipGraphLaunch(b->gexec, b->stream));
@@ -1564,6 +1618,7 @@ static int batch_step(FtsBatch* b, int niter)
             return rc;
         }
         b->iter++;
+        if ((rc = apply_pending(b))) return rc;
     }
     return MFT_OK;
 }
@@ -1618,6 +1673,8 @@ static int batch_observables(FtsBatch* b, int rep, double* Q, double* H, double*
             h = w2 * r.gvol * d.C / P.chiN + m2 * r.gvol * d.C / P.chiN;
         else if (P.kind == MFT_FTS_HELFAND)  // fts_potential_helfand.cu:254-275
             h = w2 * r.gvol * d.C / 2.0 / P.kappaN - I * d.C * (w1 * r.gvol);
+        else if (P.kind == MFT_FTS_EDWARDS)  // PotentialEdwards::calcHamiltonian
+            h = w2 * r.gvol / 2.0 / P.kappaN;
         else  // fts_potential_incompress.cu:232-247
             h = -I * d.C * (w1 * r.gvol);
         if (parts) {
@@ -1630,7 +1687,8 @@ static int batch_observables(FtsBatch* b, int rep, double* Q, double* H, double*
     for (int m = 0; m < d.nmolecules; ++m) {
         std::complex<double> Qm(q[m].x, q[m].y);
         // LinearMolec::calcHTerm (fts_molecule_linear.cu:368-382)
-        std::complex<double> h = -r.nmolecs[m] * std::log(Qm);
+        std::complex<double> h = d.molecules[m].activity > 0.0 ? -d.molecules[m].activity * r.V * Qm
+                                                               : -r.nmolecs[m] * std::log(Qm);
         if (Q) {
             Q[2 * m] = Qm.real();
             Q[2 * m + 1] = Qm.imag();
@@ -1790,6 +1848,36 @@ int mft_fts_get_propagators(mft_fts* s, int m, float* q, float* qdag)
     }
     return MFT_OK;
 }
+int mft_fts_set_activity(mft_fts* s, int m, double activity)
+{
+    MFT_REQUIRE(s, MFT_ERR_INVALID, "null system");
+    FtsRep& r = s->reps[0];
+    MFT_REQUIRE(m >= 0 && m < r.d.nmolecules, MFT_ERR_INVALID, "Invalid molecule index - out of range!");
+    MFT_REQUIRE(r.d.molecules[m].activity > 0.0, MFT_ERR_INVALID, "molecule %d is not grand canonical", m + 1);
+    MFT_REQUIRE(activity > 0.0 && std::isfinite(activity), MFT_ERR_INVALID, "activity must be positive");
+    MFT_HIP(hipStreamSynchronize(s->stream));
+    r.d.molecules[m].activity = activity;
+    if (s->inited) {
+        r.act_pending = true;
+    } else {
+        r.act_dev[m] = activity;
+        group_for(s, r).dirty = true;
+    }
+    return MFT_OK;
+}
+
+int mft_fts_nsites(mft_fts* s, double* nsites)
+{
+    MFT_REQUIRE(s && nsites, MFT_ERR_INVALID, "null argument");
+    FtsRep& r = s->reps[0];
+    std::vector<double2> q(r.d.nmolecules);
+    MFT_HIP(hipStreamSynchronize(s->stream));
+    MFT_HIP(hipMemcpy(q.data(), r.Q, sizeof(double2) * q.size(), hipMemcpyDeviceToHost));
+    for (int m = 0; m < r.d.nmolecules; ++m)
+        nsites[m] = r.act_dev[m] > 0.0 ? r.act_dev[m] * r.V * q[m].x * r.Ntot[m] : r.nmolecs[m] * r.Ntot[m];
+    return MFT_OK;
+}
+
 int mft_fts_observables(mft_fts* s, double* Q, double* H, double* parts)
 {
     MFT_REQUIRE(s, MFT_ERR_INVALID, "null system");

@@ -524,6 +524,12 @@ int mft_fts_update_fields(float* wpl, float* wmi, const mft_fts_potential* pot, 
         A.nmag_mi = std::sqrt(2.0 * pot->delt_mi / gvol);
     } else if (pot->kind == MFT_FTS_HELFAND) {
         A.cpl = C / pot->kappaN;
+    } else if (pot->kind == MFT_FTS_EDWARDS) {
+        // d_makeEdwardsForce (fts_potential_edwards.cu:227-244): Helfand's form
+        // with 1/B and no -iC term
+        A.kind = MFT_FTS_HELFAND;
+        A.cpl = 1.0 / pot->kappaN;
+        A.C = 0.0;
     }
     A.nmag_pl = std::sqrt(2.0 * pot->delt_pl / gvol);
     for (int s = 0; s < nspecies; ++s) A.rho[s] = (const float2*)rho[s];
@@ -625,6 +631,10 @@ int mft_fts_hamiltonian_term(double* H, const mft_fts_potential* pot, const floa
         // fts_potential_helfand.cu:254-275: C/(2 kappaN) gvol sum w^2 - i C gvol sum w
         H[0] = C / (2.0 * pot->kappaN) * gvol * a[2] + C * gvol * a[1];
         H[1] = C / (2.0 * pot->kappaN) * gvol * a[3] - C * gvol * a[0];
+    } else if (pot->kind == MFT_FTS_EDWARDS) {
+        // PotentialEdwards::calcHamiltonian (fts_potential_edwards.cu:256-270): gvol sum w^2 / (2B)
+        H[0] = gvol * a[2] / (2.0 * pot->kappaN);
+        H[1] = gvol * a[3] / (2.0 * pot->kappaN);
     } else {
         // fts_potential_incompress.cu:232-247: -i C gvol sum w
         H[0] = C * gvol * a[1];

@@ -28,7 +28,7 @@ POT = {"gaussian": 0, "erf2": 1, "erfG": 2, "charges": 3}
 INTEGRATOR = {"GJF": 0, "EM": 1, "none": 2}
 TK_NAMES = ["bin", "scan", "scatter", "density", "fft_x_fwd", "fft_y_fwd", "conv_z", "fft_y_inv",
             "fft_x_inv", "gather", "integrate"]
-FTS_POT = {"Helfand": 0, "Flory": 1, "Incompress": 2}
+FTS_POT = {"Helfand": 0, "Flory": 1, "Incompress": 2, "Edwards": 3}
 
 
 class MftError(RuntimeError):
@@ -429,7 +429,7 @@ FTS_SCHEME = {"EM": 0, "EMPC": 1, "1S": 2}
 
 
 class FtsMoleculeC(C.Structure):
-    _fields_ = [("nblocks", _i), ("N", _i * 16), ("species", _i * 16), ("phi", _d)]
+    _fields_ = [("nblocks", _i), ("N", _i * 16), ("species", _i * 16), ("phi", _d), ("activity", _d)]
 
 
 class FtsDescC(C.Structure):
@@ -440,8 +440,8 @@ class FtsDescC(C.Structure):
 
 def fts_desc(spec):
     """C description from a dict: Nx, L, Nr, C, nspecies, potentials
-    [dict(kind, I, J, chiN, kappaN, dt_pl, dt_mi)], molecules
-    [dict(blocks=[(N, species)], phi)], cl, seed."""
+    [dict(kind, I, J, chiN, kappaN (B for Edwards), dt_pl, dt_mi)], molecules
+    [dict(blocks=[(N, species)], phi or activity)], cl, seed."""
     d = FtsDescC()
     d.nx, d.ny, d.nz = spec["Nx"]
     d.lx, d.ly, d.lz = spec["L"]
@@ -450,9 +450,7 @@ def fts_desc(spec):
     d.nspecies = spec["nspecies"]
     d.npotentials = len(spec["potentials"])
     for k, p in enumerate(spec["potentials"]):
-        d.potentials[k] = FtsPotentialC(FTS_POT[p["kind"]], p.get("I", -1), p.get("J", -1), p.get("chiN", 0.0),
-                                        p.get("kappaN", 0.0), p.get("dt_pl", 0.0), p.get("dt_mi", 0.0),
-                                        FTS_SCHEME[p.get("scheme", "EM")])
+        d.potentials[k] = _fts_pot(p)
     d.nmolecules = len(spec["molecules"])
     for k, m in enumerate(spec["molecules"]):
         mc = FtsMoleculeC()
@@ -461,6 +459,7 @@ def fts_desc(spec):
             mc.N[j] = n
             mc.species[j] = s
         mc.phi = m.get("phi", 1.0)
+        mc.activity = m.get("activity", 0.0)
         d.molecules[k] = mc
     d.cl = int(spec.get("cl", True))
     d.seed = spec.get("seed", 1234)
@@ -473,7 +472,7 @@ def _c64(a):
 
 def _fts_pot(p):
     return FtsPotentialC(FTS_POT[p["kind"]], p.get("I", -1), p.get("J", -1), p.get("chiN", 0.0),
-                         p.get("kappaN", 0.0), p.get("dt_pl", 0.0), p.get("dt_mi", 0.0),
+                         p.get("kappaN", p.get("B", 0.0)), p.get("dt_pl", 0.0), p.get("dt_mi", 0.0),
                          FTS_SCHEME[p.get("scheme", "EM")])
 
 
@@ -644,6 +643,14 @@ class FtsSystem:
         parts = np.zeros(2 * (self.npot + self.nmol))
         check(lib().mft_fts_observables(self.h, _ptr(Q), _ptr(H), _ptr(parts)))
         return Q.view(np.complex128), complex(H[0], H[1]), parts.view(np.complex128)
+
+    def set_activity(self, m, activity):
+        check(lib().mft_fts_set_activity(self.h, m, _d(activity)))
+
+    def nsites(self):
+        n = np.zeros(self.nmol)
+        check(lib().mft_fts_nsites(self.h, _ptr(n)))
+        return n
 
 
 class FtsBatch:

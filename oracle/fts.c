@@ -124,6 +124,13 @@ void orc_fts_incompress_force(cplx* dHdw, const cplx* w, const cplx* rhoTot,
     for (int i = 0; i < M; ++i) dHdw[i] = I * rhoTot[i] / (real)Nr - I * (real)C;
 }
 
+/* d_makeEdwardsForce (fts_potential_edwards.cu:227-244): w/B + i rho/Nr */
+void orc_fts_edwards_force(cplx* dHdw, const cplx* w, const cplx* rhoTot, double B,
+                           double Nr, int M)
+{
+    for (int i = 0; i < M; ++i) dHdw[i] = w[i] / (real)B + I * rhoTot[i] / (real)Nr;
+}
+
 void orc_fts_update_em(cplx* w, const cplx* dHdw, const cplx* noise, double delt, int M)
 {
     for (int i = 0; i < M; ++i) {

@@ -127,6 +127,8 @@ void orc_fts_helfand_force(cplx* dHdw, const cplx* w, const cplx* rhoTot, double
 /* d_makeIncompressForce (fts_potential_incompress.cu:202-218). */
 void orc_fts_incompress_force(cplx* dHdw, const cplx* w, const cplx* rhoTot,
                               double C, double Nr, int M);
+void orc_fts_edwards_force(cplx* dHdw, const cplx* w, const cplx* rhoTot, double B,
+                           double Nr, int M);
 /* d_fts_updateEM (fts_updateSchemes.cu:8-29); noise may be NULL. */
 void orc_fts_update_em(cplx* w, const cplx* dHdw, const cplx* noise, double delt, int M);
 /* d_fts_updateEMPC corrector (fts_updateSchemes.cu:56-81):

@@ -63,6 +63,7 @@ def lib(bits: int = 64):
         "orc_fts_flory_force": (None, [_p] * 6 + [_d, _d, _d, _i]),
         "orc_fts_helfand_force": (None, [_p, _p, _p, _d, _d, _d, _i]),
         "orc_fts_incompress_force": (None, [_p, _p, _p, _d, _d, _i]),
+        "orc_fts_edwards_force": (None, [_p, _p, _p, _d, _d, _i]),
         "orc_fts_update_em": (None, [_p, _p, _p, _d, _i]),
         "orc_fts_update_empc": (None, [_p, _p, _p, _p, _p, _d, _i]),
         "orc_fts_update_1s": (None, [_p, _p, _p, _d, _i]),
@@ -456,7 +457,7 @@ def fts_bond_fft(Nx, L, Nr, bits=64):
 
 @dataclass
 class FtsPotential:
-    kind: str                 # Helfand | Flory | Incompress
+    kind: str                 # Helfand | Flory | Incompress | Edwards (kappaN holds B)
     I: int = -1               # species indices (Flory)
     J: int = -1
     chiN: float = 0.0
@@ -476,6 +477,7 @@ class FtsMolecule:
     blocks: list              # [(N_b, species index)]
     phi: float = -1.0
     nmolecs: float = -1.0
+    activity: float = -1.0    # > 0: grand canonical (fts_molecule.cu:30-31)
 
 
 @dataclass
@@ -509,10 +511,13 @@ class FtsSystem:
         return g
 
     def init_nmolecs(self):
-        # LinearMolec::computeLinearTerms (fts_molecule_linear.cu:433-435)
+        # LinearMolec::computeLinearTerms (fts_molecule_linear.cu:433-435); a
+        # grand canonical molecule keeps phi = -1 (fts_molecule.cu:19), so its
+        # count comes out negative and the activity branches apply
         for m in self.molecules:
             Ntot = sum(n for n, _ in m.blocks)
-            m.nmolecs = self.C * m.phi * self.V() * self.Nr / float(Ntot)
+            phi = -1.0 if m.activity > 0.0 else m.phi
+            m.nmolecs = self.C * phi * self.V() * self.Nr / float(Ntot)
 
 
 def fts_build_species_fields(sys: FtsSystem):
@@ -521,7 +526,7 @@ def fts_build_species_fields(sys: FtsSystem):
     W = np.zeros((sys.nspecies, M), np.complex128)
     for s in range(sys.nspecies):
         for p in sys.potentials:
-            if p.kind in ("Helfand", "Incompress"):
+            if p.kind in ("Helfand", "Incompress", "Edwards"):
                 W[s] += 1j * p.wpl
             elif p.kind == "Flory":
                 if s == p.I:
@@ -553,7 +558,8 @@ def fts_molecule_densities(sys: FtsSystem, bits=64, keep=False):
         qd = np.zeros(Ntot * M, _cdt(bits))
         rb = np.zeros(nb * M, _cdt(bits))
         Qc = _propagators(L, Ws, nb, N, spec, g, sys.Nx, sys.gvol(), sys.V(), q, qd, bits)
-        factor = m.nmolecs / Qc / sys.V()
+        # fts_molecule_linear.cu:286-291
+        factor = m.nmolecs / Qc / sys.V() if m.nmolecs > 0.0 else m.activity
         _density(L, Ws, nb, N, spec, q, qd, factor, M, rb, rho, bits)
         sys.Q.append(Qc)
         if keep:
@@ -608,6 +614,10 @@ def _forces(sys: FtsSystem, p: FtsPotential):
         dH = np.zeros(M, np.complex128)
         L.orc_fts_incompress_force(_ptr(dH), _ptr(p.wpl), _ptr(rho_tot), sys.C, sys.Nr, M)
         return dH, None
+    if p.kind == "Edwards":
+        dH = np.zeros(M, np.complex128)
+        L.orc_fts_edwards_force(_ptr(dH), _ptr(p.wpl), _ptr(rho_tot), p.kappaN, sys.Nr, M)
+        return dH, None
     dHp = np.zeros(M, np.complex128)
     dHm = np.zeros(M, np.complex128)
     rI = np.ascontiguousarray(sys.rho[p.I])
@@ -659,6 +669,8 @@ def fts_linear_coeffs(sys: FtsSystem):
     for p in sys.potentials:
         if p.kind == "Helfand":
             p.Akpl = np.full(M, sys.C / p.kappaN)
+        elif p.kind == "Edwards":
+            p.Akpl = np.full(M, 1.0 / p.kappaN)
         elif p.kind == "Flory":
             p.Akpl = np.full(M, 2.0 * sys.C / p.chiN)
             p.Akmi = p.Akpl.copy()
@@ -672,7 +684,7 @@ def fts_linear_coeffs(sys: FtsSystem):
         pref = m.nmolecs * alpha * Ntot / sys.V()
         h = pref * debye_homopolymer(k2, alpha)
         for p in sys.potentials:
-            if p.kind == "Helfand":
+            if p.kind in ("Helfand", "Edwards"):   # fts_molecule_linear.cu:459
                 p.Akpl = p.Akpl + h
         if len(N) == 1:
             for p in sys.potentials:
@@ -706,7 +718,7 @@ def _noises(sys: FtsSystem, p: FtsPotential, z, col):
     if z is None or p.kind == "Incompress":
         return None, None, col
     npl = (np.sqrt(2.0 * p.dt_pl / gvol) * z[:, col]).astype(np.complex128)
-    if p.kind == "Helfand":
+    if p.kind in ("Helfand", "Edwards"):
         return npl, None, col + 1
     nmi = (np.sqrt(2.0 * p.dt_mi / gvol) * z[:, col + 1]).astype(np.complex128)
     return npl, nmi, col + 2
@@ -775,12 +787,16 @@ def fts_hamiltonian(sys: FtsSystem):
             h = (np.sum(p.wpl * p.wpl) * gvol) * sys.C / p.chiN + (np.sum(p.wmi * p.wmi) * gvol) * sys.C / p.chiN
         elif p.kind == "Helfand":
             h = (np.sum(p.wpl * p.wpl) * gvol) * sys.C / 2.0 / p.kappaN - 1j * sys.C * (np.sum(p.wpl) * gvol)
+        elif p.kind == "Edwards":
+            # PotentialEdwards::calcHamiltonian (fts_potential_edwards.cu:256-270)
+            h = (np.sum(p.wpl * p.wpl) * gvol) / 2.0 / p.kappaN
         else:
             h = -1j * sys.C * (np.sum(p.wpl) * gvol)
         parts.append(h)
         H += h
     for m, Q in zip(sys.molecules, sys.Q):
-        h = -m.nmolecs * np.log(Q)
+        # LinearMolec::calcHTerm: -n log Q, or -z V Q when grand canonical
+        h = -m.nmolecs * np.log(Q) if m.nmolecs > 0.0 else -m.activity * sys.V() * Q
         parts.append(h)
         H += h
     return H, parts
@@ -813,12 +829,13 @@ def fts_system_from_spec(spec, fields):
     pots = []
     for k, p in enumerate(spec["potentials"]):
         fp = FtsPotential(kind=p["kind"], I=p.get("I", -1), J=p.get("J", -1), chiN=p.get("chiN", 0.0),
-                          kappaN=p.get("kappaN", 0.0), dt_pl=p.get("dt_pl", 0.0), dt_mi=p.get("dt_mi", 0.0),
+                          kappaN=p.get("kappaN", p.get("B", 0.0)), dt_pl=p.get("dt_pl", 0.0), dt_mi=p.get("dt_mi", 0.0),
                           scheme=p.get("scheme", "EM"))
         fp.wpl = np.array(fields.get((k, "pl"), np.zeros(M)), np.complex128)
         if p["kind"] == "Flory":
             fp.wmi = np.array(fields.get((k, "mi"), np.zeros(M)), np.complex128)
         pots.append(fp)
-    mols = [FtsMolecule(blocks=list(m["blocks"]), phi=m.get("phi", 1.0)) for m in spec["molecules"]]
+    mols = [FtsMolecule(blocks=list(m["blocks"]), phi=m.get("phi", 1.0), activity=m.get("activity", -1.0) or -1.0)
+            for m in spec["molecules"]]
     return FtsSystem(Nx=tuple(spec["Nx"]), L=tuple(spec["L"]), Nr=spec["Nr"], C=spec["C"], nspecies=spec["nspecies"],
                      potentials=pots, molecules=mols)

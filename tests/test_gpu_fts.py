@@ -83,7 +83,9 @@ def compare(s, o, spec, what="", o32=None):
         assert e < bud, (what, Qg[m], o.Q[m])
     Ho, parts_o = orc.fts_hamiltonian(o)
     budget = 1e-6 * sum(abs(h) for h in parts_o)
-    budget += sum(m.nmolecs * b for m, b in zip(o.molecules, qbud))
+    # -n log Q moves by n dlogQ; the grand canonical -z V Q by z V |Q| dlogQ
+    budget += sum((m.nmolecs if m.nmolecs > 0 else m.activity * o.V() * abs(Q)) * b
+                  for m, Q, b in zip(o.molecules, o.Q, qbud))
     assert abs(Hg - Ho) <= budget, (what, Hg, Ho, budget)
     assert rel(s.densities(), o.rho) < FIELD_TOL, what
     assert rel(s.species_fields(), o.W) < FIELD_TOL, what
@@ -131,7 +133,8 @@ def test_closed_forms_on_device(gpu):
 
 
 @pytest.mark.parametrize("case", ["diblock", "diblock_square", "incompress_blend", "diblock_empc",
-                                  "blend_empc_mixed", "diblock_1s", "blend_schemes_mixed"])
+                                  "blend_empc_mixed", "diblock_1s", "blend_schemes_mixed", "edwards_diblock",
+                                  "edwards_1s", "grand_canonical_blend", "grand_canonical_1s"])
 def test_cl_steps_match_oracle(gpu, case):
     say("case: fts CL steps", case)
     if case == "diblock":
@@ -160,6 +163,23 @@ def test_cl_steps_match_oracle(gpu, case):
                                                                      dt_mi=0.01, scheme="EMPC")])
         spec["molecules"] = [dict(blocks=[(4, 0), (3, 1)], phi=0.6), dict(blocks=[(5, 1)], phi=0.4)]
         fields = random_fields(spec, 0.1, seed=10, imag=False)
+    elif case in ("edwards_diblock", "edwards_1s"):
+        # Edwards excluded volume (fts_potential_edwards.cu) beside Flory
+        spec, fields = systems.fts_diblock(grid_shape=(16, 24, 12), L=(3.0, 4.0, 2.5), Nr=20.0, NA=8, NB=12,
+                                           chiN=15.0, kappaN=50.0, C=5.0, dt=0.01, sigma=0.1, seed=31)
+        spec["potentials"][0] = dict(kind="Edwards", B=2.0, dt_pl=0.01)
+        if case == "edwards_1s":
+            spec["potentials"] = [dict(p, scheme="1S") for p in spec["potentials"]]
+    elif case in ("grand_canonical_blend", "grand_canonical_1s"):
+        # a grand canonical homopolymer beside a canonical diblock
+        # (fts_molecule.cu:30-31, fts_molecule_linear.cu:286-291,374-376)
+        pots = [dict(kind="Helfand", kappaN=30.0, dt_pl=0.01),
+                dict(kind="Flory", I=0, J=1, chiN=10.0, dt_pl=0.01, dt_mi=0.01)]
+        if case == "grand_canonical_1s":
+            pots = [dict(p, scheme="1S") for p in pots]
+        spec = tiny_spec(Nx=(12, 16, 20), L=(3.0, 4.0, 5.0), pots=pots)
+        spec["molecules"] = [dict(blocks=[(4, 0), (3, 1)], phi=0.6), dict(blocks=[(5, 1)], activity=0.05)]
+        fields = random_fields(spec, 0.1, seed=12, imag=False)
     elif case == "blend_empc_mixed":
         # EMPC Flory beside an EM incompressibility: the EM field keeps its
         # predictor value through the corrector pass
@@ -292,3 +312,33 @@ def test_graph_replay_matches_direct_launches(gpu, monkeypatch):
         for k in range(len(reps)):
             assert d[k][0] == g[k][0] and d[k][1] == g[k][1], (i, k)
             assert np.array_equal(d[k][2], g[k][2]), (i, k)
+
+
+def test_activity_change_takes_effect_after_one_density_pass(gpu):
+    # modify molecule 1 activity scale f (fts_molecule_linear.cu:386-412): the
+    # next update still uses the densities of the old activity
+    say("case: fts activity change")
+    spec = tiny_spec(Nx=(12, 16, 20), L=(3.0, 4.0, 5.0))
+    spec["molecules"] = [dict(blocks=[(4, 0), (3, 1)], activity=0.04)]
+    fields = random_fields(spec, 0.1, seed=13, imag=False)
+    s, o = build(gpu, spec, fields)
+    M = int(np.prod(spec["Nx"]))
+    it = 0
+    for _ in range(2):
+        s.step(1)
+        orc.fts_step(o, orc.fts_noise(M, spec["seed"], it))
+        it += 1
+    s.set_activity(0, 0.04 * 0.8)
+    o.molecules[0].activity = 0.04 * 0.8
+    compare(s, o, spec, "after the change")
+    n_g = s.nsites()[0]
+    for k in range(3):
+        s.step(1)
+        orc.fts_step(o, orc.fts_noise(M, spec["seed"], it))
+        it += 1
+        compare(s, o, spec, f"step {k} after the change")
+    # nSites = z V Re Q Ntot (fts_molecule_linear.cu:344-346)
+    n_o = o.molecules[0].activity * o.V() * o.Q[0].real * 7
+    assert abs(s.nsites()[0] - n_o) < 1e-5 * abs(n_o)
+    assert n_g != s.nsites()[0]
+    s.close()

@@ -148,7 +148,8 @@ def test_em_step_matches_oracle(gpu):
 def _fts_case():
     pots = [dict(kind="Helfand", kappaN=30.0, dt_pl=0.02), dict(kind="Flory", I=0, J=1, chiN=12.0, dt_pl=0.02,
                                                                  dt_mi=0.03),
-            dict(kind="Flory", I=1, J=2, chiN=8.0, dt_pl=0.01, dt_mi=0.02), dict(kind="Incompress", dt_pl=0.01)]
+            dict(kind="Flory", I=1, J=2, chiN=8.0, dt_pl=0.01, dt_mi=0.02), dict(kind="Incompress", dt_pl=0.01),
+            dict(kind="Edwards", B=1.5, dt_pl=0.01)]
     spec = tiny_spec(Nx=(12, 10, 16), L=(3.0, 2.5, 4.0), pots=pots)
     spec["nspecies"] = 3
     spec["molecules"] = [dict(blocks=[(4, 0), (3, 1), (2, 2)], phi=0.5), dict(blocks=[(2, 1), (3, 2), (2, 1)], phi=0.3),
@@ -188,7 +189,7 @@ def test_update_fields_match_oracle(gpu, cl):
     col = 0
     for p, (wp, wm) in zip(spec["potentials"], before):
         got.append(gpu.fts_update_fields(p, wp, wm, rho, o.C, o.Nr, o.Nx, o.L, cl, spec["seed"], it, col))
-        col += {"Helfand": 1, "Flory": 2, "Incompress": 0}[p["kind"]]
+        col += {"Helfand": 1, "Flory": 2, "Incompress": 0, "Edwards": 1}[p["kind"]]
     z = np.concatenate([orc.fts_noise(M, spec["seed"], it), orc.normal4_np(np.arange(M, dtype=np.uint64), it,
                                                                             orc.TAG_FTS, 1, spec["seed"])], axis=1)
     orc.fts_update_fields(o, z if cl else None)

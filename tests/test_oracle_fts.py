@@ -267,3 +267,84 @@ def test_one_step_scheme_is_consistent_with_em():
             out.append(np.concatenate([s.potentials[0].wpl, s.potentials[1].wpl, s.potentials[1].wmi]))
         d.append(np.max(np.abs(out[0] - out[1])) / np.max(np.abs(out[0] - w0)))
     assert 1.7 < d[0] / d[1] < 2.3, d   # relative to the O(dt) change: O(dt)
+
+
+def _fd_check_forces(spec, f):
+    """Every potential's drift (orc._forces) against (1/gvol) dH/dw by central
+    differences at three sites."""
+    s = orc.fts_system_from_spec(spec, f)
+    orc.fts_init(s)
+    M, gvol = s.M, s.gvol()
+    h = 1e-5
+    for k, p in enumerate(s.potentials):
+        dHp, dHm = orc._forces(s, p)
+        for which, dH in (("pl", dHp), ("mi", dHm)):
+            if dH is None:
+                continue
+            for site in (0, 17, M - 1):
+                vals = []
+                for sgn in (1, -1):
+                    f2 = {key: v.copy() for key, v in f.items()}
+                    f2[(k, which)][site] += sgn * h
+                    t = orc.fts_system_from_spec(spec, f2)
+                    orc.fts_init(t)
+                    vals.append(orc.fts_hamiltonian(t)[0])
+                deriv = (vals[0] - vals[1]) / (2 * h) / gvol
+                assert abs(deriv - dH[site]) < 1e-5 * max(1.0, abs(dH[site])), (k, which, site, deriv, dH[site])
+
+
+def test_edwards_force_is_field_derivative_of_H():
+    # d_makeEdwardsForce (fts_potential_edwards.cu:227-244) against
+    # PotentialEdwards::calcHamiltonian and -n log Q
+    spec = tiny_spec(pots=[dict(kind="Edwards", B=0.7, dt_pl=0.02),
+                           dict(kind="Flory", I=0, J=1, chiN=12.0, dt_pl=0.02, dt_mi=0.03)])
+    _fd_check_forces(spec, random_fields(spec, 0.2, seed=7))
+
+
+def test_grand_canonical_forces_are_field_derivatives_of_H():
+    # with H term -z V Q (fts_molecule_linear.cu:374-376) the drift keeps the
+    # form i rho / Nr, rho built with the factor z
+    spec = tiny_spec()
+    spec["molecules"] = [dict(blocks=[(4, 0), (3, 1)], activity=0.9)]
+    _fd_check_forces(spec, random_fields(spec, 0.2, seed=8))
+
+
+def test_edwards_is_helfand_without_the_constant():
+    # Edwards(B) = Helfand(kappa N = C B) + i C, the same excluded-volume force
+    spec = tiny_spec(pots=[dict(kind="Edwards", B=0.7, dt_pl=0.02)])
+    f = random_fields(spec, 0.2, seed=9)
+    s = orc.fts_system_from_spec(spec, f)
+    orc.fts_init(s)
+    e, _ = orc._forces(s, s.potentials[0])
+    s.potentials[0].kind, s.potentials[0].kappaN = "Helfand", s.C * 0.7
+    hf, _ = orc._forces(s, s.potentials[0])
+    np.testing.assert_allclose(e, hf + 1j * s.C, rtol=1e-13, atol=1e-13)
+
+
+def test_grand_canonical_zero_field_closed_form():
+    # W = 0: q = 1, Q = 1, every block density z N_b (factor z,
+    # fts_molecule_linear.cu:289-291), nSites = z V Q Ntot, H term -z V
+    z = 0.35
+    spec = tiny_spec()
+    spec["molecules"] = [dict(blocks=[(4, 0), (3, 1)], activity=z)]
+    s = orc.fts_system_from_spec(spec, {})
+    orc.fts_init(s)
+    assert abs(s.Q[0] - 1.0) < 1e-12
+    np.testing.assert_allclose(s.rho[0], z * 4, rtol=1e-12)
+    np.testing.assert_allclose(s.rho[1], z * 3, rtol=1e-12)
+    H, parts = orc.fts_hamiltonian(s)
+    assert parts[-1] == pytest.approx(-z * s.V(), rel=1e-12)
+
+
+def test_grand_canonical_density_is_canonical_density_rescaled():
+    # for the same fields, factor z versus n / (Q V): the densities differ by
+    # z Q V / n exactly
+    spec = tiny_spec()
+    f = random_fields(spec, 0.2, seed=10)
+    c = orc.fts_system_from_spec(spec, f)
+    orc.fts_init(c)
+    spec_g = dict(spec, molecules=[dict(blocks=[(4, 0), (3, 1)], activity=0.6)])
+    g = orc.fts_system_from_spec(spec_g, f)
+    orc.fts_init(g)
+    scale = 0.6 * c.Q[0] * c.V() / c.molecules[0].nmolecs
+    np.testing.assert_allclose(g.rho, c.rho * scale, rtol=1e-12, atol=1e-14)
