@@ -132,6 +132,24 @@ mft_fts_potential PotentialIncompress::descriptor() const
     return d;
 }
 
+// PotentialEdwards (fts_potential_edwards.cu:12-75): B delt [options]; like
+// Helfand's, its initialize never reaches the device
+PotentialEdwards::PotentialEdwards(std::istringstream& iss, FTS_Box* box) : FTS_Potential(iss, box)
+{
+    potentialStyle = "Edwards";
+    iss >> B >> delt;
+    parse_options(iss, nullptr, false);
+}
+mft_fts_potential PotentialEdwards::descriptor() const
+{
+    mft_fts_potential d{};
+    d.kind = MFT_FTS_EDWARDS;
+    d.scheme = updateScheme == "EMPC" ? MFT_FTS_EMPC : (updateScheme == "1S" ? MFT_FTS_1S : MFT_FTS_EM);
+    d.kappaN = B;
+    d.delt_pl = delt;
+    return d;
+}
+
 FTS_Potential* FTS_PotentialFactory(std::istringstream& iss, FTS_Box* box)
 {
     std::string s1;
@@ -139,8 +157,8 @@ FTS_Potential* FTS_PotentialFactory(std::istringstream& iss, FTS_Box* box)
     if (s1 == "Helfand" || s1 == "helfand") return new PotentialHelfand(iss, box);
     if (s1 == "incompress" || s1 == "Incompress" || s1 == "incompressible") return new PotentialIncompress(iss, box);
     if (s1 == "Flory" || s1 == "flory") return new PotentialFlory(iss, box);
-    if (s1 == "Edwards" || s1 == "edwards" || s1 == "particle" || s1 == "Particle")
-        die(s1 + " is not part of this build's field-theory path");
+    if (s1 == "Edwards" || s1 == "edwards") return new PotentialEdwards(iss, box);
+    if (s1 == "particle" || s1 == "Particle") die(s1 + " is not part of this build's field-theory path");
     die(s1 + " is not a valid FTS_Potential");
 }
 
@@ -155,13 +173,39 @@ FTS_Molec::FTS_Molec(std::istringstream& iss, FTS_Box* box) : input_command(iss.
     else if (s1 == "nmolecs") iss >> nmolecs;
     else if (s1 == "activity") iss >> activity;
     else die(s1 + " is not a valid way to define the number of particles/molecules");
-    if (phi <= 0.0) die("this build's linear molecules are canonical with 'phi' given");
+    // the reference recomputes nmolecs from phi (fts_molecule_linear.cu:433-435),
+    // which leaves a molecule given by 'nmolecs' without a density factor
+    if (phi <= 0.0 && activity <= 0.0) die("this build's linear molecules take 'phi' or 'activity'");
 }
 
+// modifyMolecParam (fts_molecule.cu:77-88)
+static void modifyMolecParam(double& param, const std::string& how2Mod, double change)
+{
+    if (how2Mod == "scale") param *= change;
+    else if (how2Mod == "delta") param += change;
+    else if (how2Mod == "value") param = change;
+    else die(how2Mod + " is not a valid way to modify a molecule parameter");
+}
+
+// LinearMolec::modifyMolecule (fts_molecule_linear.cu:386-425)
 void FTS_Molec::modifyMolecule(std::istringstream& iss)
 {
-    (void)iss;
-    die("molecule modifications are not part of this build");
+    std::string property, how2Mod;
+    iss >> property;
+    if (property == "activity") {
+        double change = 0.0;
+        iss >> how2Mod >> change;
+        std::cout << "activity changed from " << activity;
+        modifyMolecParam(activity, how2Mod, change);
+        std::cout << " to " << activity << std::endl;
+        mybox->setActivity(this, activity);
+    } else if (property == "phi") {
+        die("changes in FTS:molec:phi not yet supported");
+    } else if (property == "nmolecs") {
+        die("changes in FTS:molec:nmolecs not yet supported");
+    } else {
+        die(property + " is not a valid property to change in FTS:molecule");
+    }
 }
 
 // LinearMolec (fts_molecule_linear.cu:17-140): numBlocks, (N, species) per block
@@ -192,6 +236,7 @@ mft_fts_molecule LinearMolec::descriptor() const
         d.species[j] = mybox->findSpeciesInteger(blockSpecies[j]);
     }
     d.phi = phi;
+    d.activity = activity > 0.0 ? activity : 0.0;
     return d;
 }
 
@@ -327,7 +372,8 @@ void FTS_Box::finishInitialization()
         }
         std::printf("], \"molecules\": [");
         for (int m = 0; m < d.nmolecules; ++m) {
-            std::printf("%s{\"phi\": %.9g, \"blocks\": [", m ? ", " : "", d.molecules[m].phi);
+            std::printf("%s{\"phi\": %.9g, \"activity\": %.9g, \"blocks\": [", m ? ", " : "", d.molecules[m].phi,
+                        d.molecules[m].activity);
             for (int j = 0; j < d.molecules[m].nblocks; ++j)
                 std::printf("%s[%d, %d]", j ? ", " : "", d.molecules[m].N[j], d.molecules[m].species[j]);
             std::printf("]}");
@@ -340,8 +386,9 @@ void FTS_Box::finishInitialization()
     check_mft(mft_fts_create(&sys, &d), "mft_fts_create");
     std::vector<float> a(2 * (size_t)M), b(2 * (size_t)M);
     for (size_t p = 0; p < Potentials.size(); ++p) {
-        // Helfand's initialize never reaches the device (fts_potential_helfand.cu)
-        const bool helfand = Potentials[p]->potentialStyle == "Helfand";
+        // Helfand's and Edwards' initialize never reach the device
+        // (fts_potential_helfand.cu, fts_potential_edwards.cu)
+        const bool helfand = Potentials[p]->potentialStyle == "Helfand" || Potentials[p]->potentialStyle == "Edwards";
         for (long i = 0; i < M; ++i) {
             a[2 * i] = helfand ? 0.f : (float)Potentials[p]->wpl[i].real();
             a[2 * i + 1] = helfand ? 0.f : (float)Potentials[p]->wpl[i].imag();
@@ -370,6 +417,23 @@ void FTS_Box::computeHamiltonian()
         size_t k = Potentials.size() + m;
         Molecs[m]->Hterm = {parts[2 * k], parts[2 * k + 1]};
     }
+    updateNsites();
+}
+
+// nSites of every molecule at the last density pass: activity V Re Q Ntot
+// (fts_molecule_linear.cu:344-346); canonical molecules report nmolecs Ntot
+void FTS_Box::updateNsites()
+{
+    std::vector<double> n(Molecs.size());
+    check_mft(mft_fts_nsites(sys, n.data()), "mft_fts_nsites");
+    for (size_t m = 0; m < Molecs.size(); ++m) Molecs[m]->nSites = n[m];
+}
+
+void FTS_Box::setActivity(const FTS_Molec* mol, double activity)
+{
+    if (dry_run) return;
+    for (size_t m = 0; m < Molecs.size(); ++m)
+        if (Molecs[m] == mol) check_mft(mft_fts_set_activity(sys, (int)m, activity), "mft_fts_set_activity");
 }
 
 // writeData (FTS_Box.cu:145-190): step Re(H) [Im(H) | error] Hterms -n logQ
@@ -383,7 +447,11 @@ void FTS_Box::writeData(int step)
     if (ftsStyle == "cl") line << Heff.imag() << " ";
     else line << error << " ";
     for (auto* p : Potentials) line << p->Hterm.real() << " ";
-    for (auto* m : Molecs) line << m->calcHTerm().real() << " ";
+    // -n log Q, or -z V Q followed by nSites for a grand canonical molecule
+    for (auto* m : Molecs) {
+        line << m->calcHTerm().real() << " ";
+        if (m->activity > 0.0) line << m->nSites << "  ";
+    }
     otp << line.str() << std::endl;
     std::cout << line.str() << std::endl;
 }
@@ -430,4 +498,198 @@ void FTS_Box::NVT(int nsteps)
         ++i;
     }
     check_mft(mft_fts_synchronize(sys), "mft_fts_synchronize");
+}
+
+// FTS_Box::modifyBox (FTS_Box.cu:674-708): appends Re H and (nSites,
+// activity) of every grand canonical molecule to sweep_data.dat, then applies
+// "molecule i <property> ..." to molecule i (1-indexed)
+void FTS_Box::modifyBox(std::istringstream& iss)
+{
+    std::ofstream otp("sweep_data.dat", std::ios_base::app);
+    std::string word;
+    iss >> word;
+    if (word == "potential") die("potential modifications not yet supported");
+    if (word == "molecule") {
+        if (!dry_run && sys) updateNsites();
+        otp << Heff.real() << " ";
+        for (auto* m : Molecs)
+            if (m->activity > 0.0) otp << m->nSites << " " << m->activity << " ";
+        otp << std::endl;
+        int molecInd = 0;
+        iss >> molecInd;
+        if (molecInd > (int)Molecs.size() || molecInd < 1) die("Invalid molecule index - out of range!");
+        Molecs[molecInd - 1]->modifyMolecule(iss);
+    }
+}
+
+// FTS_Box::storePotentials (FTS_BoxFindSpinodal.cu:289-329)
+void FTS_Box::storePotentials(std::vector<std::complex<double>>& mi, std::vector<std::complex<double>>& pl,
+                              std::vector<double>& z, std::vector<double>& n, int ind)
+{
+    if (ind >= 3) die("Invalid index in storePotentials");
+    const int nPot = (int)Potentials.size(), nMol = (int)Molecs.size();
+    std::vector<float> a(2 * (size_t)M), b(2 * (size_t)M);
+    for (int p = 0; p < nPot; ++p) {
+        const bool doWmi = Potentials[p]->has_wmi;
+        check_mft(mft_fts_get_fields(sys, p, a.data(), doWmi ? b.data() : nullptr), "mft_fts_get_fields");
+        const size_t off = (size_t)ind * nPot * M + (size_t)p * M;
+        for (long i = 0; i < M; ++i) {
+            pl[off + i] = {a[2 * i], a[2 * i + 1]};
+            if (doWmi) mi[off + i] = {b[2 * i], b[2 * i + 1]};
+        }
+    }
+    updateNsites();
+    for (int m = 0; m < nMol; ++m) {
+        z[ind * nMol + m] = Molecs[m]->activity;
+        n[ind * nMol + m] = Molecs[m]->nSites;
+    }
+    std::cout << "  FINISHED STORE POTENTIALS WITH nSaved = " << ind << std::endl;
+}
+
+// FTS_Box::shuffleStorage (FTS_BoxFindSpinodal.cu:245-284): slot 1 -> 0, 2 -> 1
+void FTS_Box::shuffleStorage(std::vector<std::complex<double>>& mi, std::vector<std::complex<double>>& pl,
+                             std::vector<double>& z, std::vector<double>& n)
+{
+    const size_t nPot = Potentials.size(), nMol = Molecs.size(), S = nPot * M;
+    for (size_t i = 0; i < S; ++i) {
+        mi[i] = mi[S + i];
+        pl[i] = pl[S + i];
+        mi[S + i] = mi[2 * S + i];
+        pl[S + i] = pl[2 * S + i];
+    }
+    for (size_t m = 0; m < nMol; ++m) {
+        z[m] = z[nMol + m];
+        z[nMol + m] = z[2 * nMol + m];
+        n[m] = n[nMol + m];
+        n[nMol + m] = n[2 * nMol + m];
+    }
+}
+
+// FTS_Box::restorePotentials (FTS_BoxFindSpinodal.cu:333-371)
+void FTS_Box::restorePotentials(const std::vector<std::complex<double>>& mi,
+                                const std::vector<std::complex<double>>& pl, const std::vector<double>& z, int ind)
+{
+    if (ind >= 3) die("Invalid index in storePotentials");
+    const int nPot = (int)Potentials.size(), nMol = (int)Molecs.size();
+    std::vector<float> a(2 * (size_t)M), b(2 * (size_t)M);
+    for (int p = 0; p < nPot; ++p) {
+        const bool doWmi = Potentials[p]->has_wmi;
+        const size_t off = (size_t)ind * nPot * M + (size_t)p * M;
+        for (long i = 0; i < M; ++i) {
+            a[2 * i] = (float)pl[off + i].real();
+            a[2 * i + 1] = (float)pl[off + i].imag();
+            if (doWmi) {
+                b[2 * i] = (float)mi[off + i].real();
+                b[2 * i + 1] = (float)mi[off + i].imag();
+            }
+        }
+        check_mft(mft_fts_set_fields(sys, p, a.data(), doWmi ? b.data() : nullptr), "mft_fts_set_fields");
+    }
+    for (int m = 0; m < nMol; ++m) {
+        Molecs[m]->activity = z[ind * nMol + m];
+        if (Molecs[m]->activity > 0.0) setActivity(Molecs[m], Molecs[m]->activity);
+        std::cout << "  Molec " << m << " activity restored to: " << Molecs[m]->activity << std::endl;
+    }
+    check_mft(mft_fts_init(sys), "mft_fts_init");
+}
+
+// FTS_Box::findSpinodal (FTS_BoxFindSpinodal.cu:25-240): scale the activity of
+// one molecule by f per try, re-equilibrate for nSteps, and stop at the first
+// try where |d phi / d z| falls below a third of the previous difference.
+// The reference's refinement loop ends in an unconditional break
+// (FTS_BoxFindSpinodal.cu:214), so the search ends at that first estimate
+// here as well.  maxTries (not in the reference, which has no bound) stops a
+// search that never meets the criterion.
+void FTS_Box::findSpinodal(std::istringstream& iss)
+{
+    double phiTol = 1.0E-4, f = 0.8, fScaleFactor = 0.5;
+    int molecIndex = 0, nSteps = 10000;
+    long maxTries = -1;
+    std::string fileName = "find_spinodal.dat", word;
+    while (iss >> word) {
+        if (word == "phiTolerance") iss >> phiTol;
+        else if (word == "fScale") iss >> fScaleFactor;
+        else if (word == "datFile") iss >> fileName;
+        else if (word == "molecule") {
+            iss >> molecIndex;
+            molecIndex -= 1;
+        } else if (word == "nSteps") iss >> nSteps;
+        else if (word == "initial_f") iss >> f;
+        else if (word == "maxTries") iss >> maxTries;
+    }
+    std::cout << "Searching for spinodal point using: " << std::endl;
+    std::cout << "  phiTol = " << phiTol << std::endl;
+    std::cout << "  f = " << f << std::endl;
+    std::cout << "  fScaleFactor = " << fScaleFactor << std::endl;
+    std::cout << "  molecule = " << molecIndex + 1 << std::endl;
+    std::cout << "  initalf = " << f << std::endl;
+    if (molecIndex < 0 || molecIndex >= (int)Molecs.size()) die("Invalid molecule index - out of range!");
+    if (Molecs[molecIndex]->activity <= 0.0) die("findSpinodal scales the activity of a grand canonical molecule");
+    if (dry_run) {
+        std::printf("{\"findSpinodal\": {\"phiTol\": %.9g, \"f\": %.9g, \"fScale\": %.9g, \"molecule\": %d, "
+                    "\"nSteps\": %d, \"datFile\": \"%s\", \"maxTries\": %ld}}\n",
+                    phiTol, f, fScaleFactor, molecIndex, nSteps, fileName.c_str(), maxTries);
+        std::fflush(stdout);
+        return;
+    }
+    const int nBkps = 3, nPot = (int)Potentials.size(), nMol = (int)Molecs.size();
+    int nSaved = 0;
+    std::ofstream otp(fileName);
+    std::vector<double> Zstorage((size_t)nMol * nBkps), nStorage((size_t)nMol * nBkps);
+    std::vector<std::complex<double>> wplBkps((size_t)nPot * M * nBkps), wmiBkps((size_t)nPot * M * nBkps);
+    double phi[3] = {0.0, 0.0, 0.0};
+
+    NVT(nSteps);
+    storePotentials(wmiBkps, wplBkps, Zstorage, nStorage, nSaved);
+    nSaved++;
+    int tryCt = 0;
+    const std::string command = "molecule " + std::to_string(molecIndex + 1) + " activity scale " + std::to_string(f);
+    int foundSpinodalPoint = 0, initializing = 1;
+    while (!foundSpinodalPoint) {
+        std::istringstream modss(command);
+        modifyBox(modss);
+        NVT(nSteps);
+        if (initializing) {
+            std::cout << "  nsaved < 2" << std::endl;
+            storePotentials(wmiBkps, wplBkps, Zstorage, nStorage, nSaved);
+            nSaved++;
+            if (nSaved == 3) {
+                std::cout << "INITIALIZING DONE\n\n" << std::endl;
+                initializing = 0;
+                nSaved = 2;
+            }
+        } else {
+            std::cout << "  nsaved == 2" << std::endl;
+            if (tryCt >= 2) shuffleStorage(wmiBkps, wplBkps, Zstorage, nStorage);
+            storePotentials(wmiBkps, wplBkps, Zstorage, nStorage, nSaved);
+            const int m0 = 0 * nMol + molecIndex, m1 = 1 * nMol + molecIndex, m2 = 2 * nMol + molecIndex;
+            double nTot[3] = {0.0, 0.0, 0.0};
+            for (int m = 0; m < nMol; ++m)
+                for (int k = 0; k < 3; ++k) nTot[k] += nStorage[k * nMol + m];
+            for (int k = 0; k < 3; ++k) phi[k] = nStorage[k * nMol + molecIndex] / nTot[k];
+            const double dphi01 = std::fabs((phi[1] - phi[0]) / (Zstorage[m1] - Zstorage[m0]));
+            const double dphi12 = std::fabs((phi[2] - phi[1]) / (Zstorage[m2] - Zstorage[m1]));
+            std::cout << "  nSaved: " << nSaved << " nTot: " << nTot[0] << " " << nTot[1] << " " << nTot[2]
+                      << std::endl;
+            std::cout << "  phi: " << phi[0] << " " << phi[1] << " " << phi[2] << std::endl;
+            if (dphi12 < dphi01 / 3.0) foundSpinodalPoint = 1;
+            else std::cout << "  dphi01: " << dphi01 << " dphi12: " << dphi12 << std::endl;
+        }
+        std::cout << "Try " << tryCt << ", " << nSaved << ": ";
+        for (int i = 0; i < 3 * nMol; ++i) std::cout << Zstorage[i] << " " << nStorage[i] << "   ";
+        std::cout << std::endl;
+        for (auto* m : Molecs) otp << m->activity << " " << m->nSites << " ";
+        otp << std::endl;
+        tryCt++;
+        if (maxTries > 0 && tryCt >= maxTries && !foundSpinodalPoint) {
+            std::cout << "findSpinodal: no spinodal point within " << maxTries << " tries" << std::endl;
+            return;
+        }
+    }
+    std::cout << "spinodal found!\nz = " << Zstorage[0 * nMol + molecIndex] << " " << Zstorage[1 * nMol + molecIndex]
+              << " " << Zstorage[2 * nMol + molecIndex] << std::endl;
+    std::cout << "phi = " << phi[0] << " " << phi[1] << " " << phi[2] << std::endl;
+    std::cout << "\n\n\n\n" << std::endl;
+    (void)phiTol;
+    (void)fScaleFactor;
 }

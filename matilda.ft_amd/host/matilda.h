@@ -269,6 +269,12 @@ public:
     mft_fts_potential descriptor() const override;
     double delt = 0.0;
 };
+class PotentialEdwards : public FTS_Potential {
+public:
+    PotentialEdwards(std::istringstream& iss, FTS_Box* box); // fts_potential_edwards.cu:12-75
+    mft_fts_potential descriptor() const override;
+    double B = 0.0, delt = 0.0;
+};
 FTS_Potential* FTS_PotentialFactory(std::istringstream& iss, FTS_Box* box);
 
 class FTS_Molec {
@@ -309,6 +315,10 @@ public:
     void computeHamiltonian();                          // FTS_Box.cu:590-602
     int converged(int step) override;                   // FTS_Box.cu:715-760
     int findSpeciesInteger(const std::string& s) const;
+    void findSpinodal(std::istringstream& iss) override;  // FTS_BoxFindSpinodal.cu:25-240
+    void modifyBox(std::istringstream& iss) override;     // FTS_Box.cu:674-708
+    void setActivity(const FTS_Molec* m, double activity);
+    void updateNsites();
 
     std::string ftsStyle;
     double Nr = -1.0, C = -1.0, rho0 = -1.0;
@@ -327,4 +337,12 @@ private:
     bool initialized = false;
     bool stepped = false;
     void finishInitialization();
+    // host copies of every field and the molecules' activities and site
+    // counts (FTS_BoxFindSpinodal.cu:245-371)
+    void storePotentials(std::vector<std::complex<double>>& mi, std::vector<std::complex<double>>& pl,
+                         std::vector<double>& z, std::vector<double>& n, int ind);
+    void shuffleStorage(std::vector<std::complex<double>>& mi, std::vector<std::complex<double>>& pl,
+                        std::vector<double>& z, std::vector<double>& n);
+    void restorePotentials(const std::vector<std::complex<double>>& mi, const std::vector<std::complex<double>>& pl,
+                           const std::vector<double>& z, int ind);
 };

@@ -52,7 +52,7 @@ def test_c3_fts(tmp_path):
     h, f = d["potentials"]
     assert h["kind"] == 0 and h["kappaN"] == 100 and h["delt_pl"] == pytest.approx(0.01)
     assert f["kind"] == 1 and (f["I"], f["J"]) == (0, 1) and f["chiN"] == 20
-    assert d["molecules"] == [{"phi": 1.0, "blocks": [[50, 0], [50, 1]]}]
+    assert d["molecules"] == [{"phi": 1.0, "activity": 0.0, "blocks": [[50, 0], [50, 1]]}]
 
 
 def test_update_scheme_keyword(tmp_path):
@@ -91,4 +91,54 @@ def test_ramp_and_groups(tmp_path):
                                        ("Dim 3\nintegrator all XYZ\n", "XYZ is not a valid Integrator")])
 def test_errors_die_like_the_reference(tmp_path, bad, msg):
     r = dry(tmp_path, "box ps\n" + bad + "endBox\n")
+    assert r.returncode == 1 and msg in r.stdout
+
+
+EDWARDS_GC = """box fts cl
+Dim 3
+grid 12 12 16
+boxLengths 3.0 3.0 4.0
+Nr 12
+C 4
+randSeed 5
+species A
+species B
+molecule linear phi 0.7 2 5 A 7 B
+molecule linear activity 0.02 1 6 B
+potential Edwards 3.0 0.01
+potential Flory A B 12 0.01 0.01 initialize value 0.1 0.0
+logFreq 4
+endBox
+"""
+
+
+def test_edwards_and_activity_keywords(tmp_path):
+    # fts_potential.cu:30-31 (Edwards B delt), fts_molecule.cu:30-31 (activity)
+    r = dry(tmp_path, EDWARDS_GC)
+    assert r.returncode == 0, r.stdout[-1000:]
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    e, f = d["potentials"]
+    assert e["kind"] == 3 and e["kappaN"] == pytest.approx(3.0) and e["delt_pl"] == pytest.approx(0.01)
+    assert e["wpl0"] == [0.0, 0.0]
+    m1, m2 = d["molecules"]
+    assert m1["activity"] == 0.0 and m1["phi"] == pytest.approx(0.7)
+    assert m2["activity"] == pytest.approx(0.02) and m2["blocks"] == [[6, 1]]
+
+
+def test_find_spinodal_arguments(tmp_path):
+    # FTS_BoxFindSpinodal.cu:27-67: defaults and keywords, molecule 1-indexed
+    r = dry(tmp_path, EDWARDS_GC + "run findspinodal molecule 2 nSteps 30 initial_f 0.9 fScale 0.25 datFile a.dat\n")
+    assert r.returncode == 0, r.stdout[-1000:]
+    fs = json.loads(r.stdout.strip().splitlines()[-1])["findSpinodal"]
+    assert fs["molecule"] == 1 and fs["nSteps"] == 30 and fs["f"] == pytest.approx(0.9)
+    assert fs["fScale"] == pytest.approx(0.25) and fs["datFile"] == "a.dat" and fs["phiTol"] == pytest.approx(1e-4)
+    r = dry(tmp_path, EDWARDS_GC + "run findspinodal\n")
+    assert r.returncode == 1 and "grand canonical" in r.stdout
+
+
+@pytest.mark.parametrize("cmd, msg", [("modify molecule 3 activity scale 0.5", "Invalid molecule index - out of range!"),
+                                      ("modify molecule 1 phi scale 0.5", "changes in FTS:molec:phi not yet supported"),
+                                      ("modify potential 1 chiN 3", "potential modifications not yet supported")])
+def test_modify_errors_die_like_the_reference(tmp_path, cmd, msg):
+    r = dry(tmp_path, EDWARDS_GC + cmd + "\n")
     assert r.returncode == 1 and msg in r.stdout

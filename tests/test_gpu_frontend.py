@@ -93,3 +93,64 @@ def test_fts_front_end_matches_engine(gpu, tmp_path):
         want = [H.real, H.imag] + [p.real for p in parts]
         assert close(row[1:], want), (row, want)
     s.close()
+
+
+SPINODAL_INPUT = """box fts cl
+Dim 3
+grid 12 12 16
+boxLengths 3.0 3.0 4.0
+Nr 12
+C 4
+randSeed 5
+species A
+species B
+molecule linear phi 0.7 2 5 A 7 B
+molecule linear activity 0.02 1 6 B
+potential Edwards 3.0 0.01
+potential Flory A B 12 0.01 0.01 initialize value 0.1 0.0
+logFreq 4
+endBox
+run vt 4
+modify molecule 2 activity scale 0.5
+run vt 4
+run findspinodal molecule 2 nSteps 3 initial_f 0.9 maxTries 4 datFile fs.dat
+"""
+
+
+def test_modify_and_find_spinodal_match_engine(gpu, tmp_path):
+    # modifyBox (FTS_Box.cu:674-708) and findSpinodal (FTS_BoxFindSpinodal.cu)
+    # through the front end against the same sequence of engine calls
+    say("case: front end modify + findSpinodal")
+    (tmp_path / "input").write_text(SPINODAL_INPUT)
+    r = run_mft(["-in", "input"], cwd=tmp_path)
+    assert r.returncode == 0, r.stdout[-3000:]
+    spec = dict(Nx=(12, 12, 16), L=(3.0, 3.0, 4.0), Nr=12.0, C=4.0, nspecies=2, cl=True, seed=5,
+                potentials=[dict(kind="Edwards", B=3.0, dt_pl=0.01),
+                            dict(kind="Flory", I=0, J=1, chiN=12.0, dt_pl=0.01, dt_mi=0.01)],
+                molecules=[dict(blocks=[(5, 0), (7, 1)], phi=0.7), dict(blocks=[(6, 1)], activity=0.02)])
+    M = 12 * 12 * 16
+    fields = {(1, "pl"): np.full(M, -1.5j, np.complex64), (1, "mi"): np.full(M, 0.1, np.complex64)}
+    s = systems.fts_system(spec, fields)
+    rows = read_log(tmp_path / "fts_data.dat")
+    # run vt 4 logs step 0; the grand canonical molecule adds -zVQ and nSites
+    s.step(1)
+    Q, H, parts = s.observables()
+    want = [H.real, H.imag] + [p.real for p in parts] + [s.nsites()[1]]
+    assert close(rows[0][1:], want), (rows[0], want)
+    s.step(3)
+    z = 0.02 * 0.5
+    s.set_activity(1, z)
+    s.step(4 + 3)
+    tries = np.loadtxt(tmp_path / "fs.dat", ndmin=2)
+    assert 1 <= len(tries) <= 4
+    for t in tries:
+        z *= 0.9
+        s.set_activity(1, z)
+        s.step(3)
+        n = s.nsites()
+        assert close(t, [-1.0, n[0], z, n[1]]), (t, z, n)
+    # sweep_data.dat: Re H, nSites, activity before every change
+    sweep = np.loadtxt(tmp_path / "sweep_data.dat", ndmin=2)
+    assert len(sweep) == 1 + len(tries)
+    assert close(sweep[:, 2], [0.02] + [0.01 * 0.9 ** k for k in range(len(tries))])
+    s.close()
