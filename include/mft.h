@@ -260,6 +260,10 @@ typedef struct {
     double delt_pl;       /* time step of w+ (delt for Helfand/Incompress) */
     double delt_mi;       /* time step of w- (Flory) */
     int scheme;           /* MFT_FTS_EM (0, the reference's default), _EMPC or _1S */
+    int zero_mean;        /* modify zeroMean: subtract the grid mean of w+ after every
+                             update and correction (fts_potential_helfand.cu:140-146,
+                             fts_potential_incompress.cu:125-131, fts_potential_edwards.cu:
+                             146-152); Flory parses it without effect (fts_potential_flory.cu:94-98) */
 } mft_fts_potential;
 
 typedef struct {

@@ -327,6 +327,12 @@ __global__ void __launch_bounds__(kFtsThreads) k_fts_update(const FtsUpdDev* __r
 }
 
 // One propagation slice along one axis for a list of chains.
+struct ZmDesc {
+    float2* w;
+    const double* Hpart;   // the update pass's block sums (sum w at [p*6+2], [p*6+3])
+    int nHpart, p, M;
+};
+
 struct PropDesc {
     const float2* src;
     float2* dst;
@@ -647,6 +653,7 @@ struct FtsRep {
     float2 *f1pl[kFtsMaxPots] = {nullptr}, *f1mi[kFtsMaxPots] = {nullptr};
     double *Akpl[kFtsMaxPots] = {nullptr}, *Akmi[kFtsMaxPots] = {nullptr};
     bool s1 = false;              // some potential uses the one-step k-space scheme
+    bool zm = false;              // some potential keeps a zero-mean w+ (modify zeroMean)
     float2 *W = nullptr, *E = nullptr, *rhot = nullptr;
     float2 *p[kFtsMaxMol] = {nullptr}, *pd[kFtsMaxMol] = {nullptr};
     float2* tmp = nullptr; // [2*nmol][M]
@@ -663,6 +670,9 @@ struct FtsGroup {
     int n[3];
     bool pc = false;                 // predictor-corrector replicas (two passes per iteration)
     bool s1 = false;                 // replicas with a one-step k-space update
+    bool zm = false;                 // replicas with zero-mean fields
+    ZmDesc* d_zm = nullptr;          // one per zero-mean field of the group, the EMPC ones first
+    int nzm = 0, nzm_pc = 0;
     std::vector<int> reps;
     FtsUpdDev* d_upd = nullptr;
     DensDesc* d_dens = nullptr;
@@ -885,6 +895,7 @@ static int rep_setup(FtsBatch* b, FtsRep& r)
             if ((rc = dev_alloc(r.allocs, &r.wmi[p], M))) return rc;
             MFT_HIP(hipMemset(r.wmi[p], 0, sizeof(float2) * M));
         }
+        if (d.potentials[p].zero_mean && d.potentials[p].kind != MFT_FTS_FLORY) r.zm = true;
         if (d.potentials[p].scheme == MFT_FTS_1S) {
             r.s1 = true;
             if ((rc = dev_alloc(r.allocs, &r.f1pl[p], M))) return rc;
@@ -1102,6 +1113,29 @@ static int group_setup(FtsBatch* b, FtsGroup& G)
     if ((rc = dev_alloc(G.allocs, &G.d_q, qd.size()))) return rc;
     MFT_HIP(hipMemcpy(G.d_q, qd.data(), sizeof(QDesc) * qd.size(), hipMemcpyHostToDevice));
 
+    // zero-mean fields: w+ of every Helfand / Incompress / Edwards potential
+    // with modify zeroMean
+    // (a correction re-centres only EMPC fields: correctFields returns early
+    // for the others, fts_potential_helfand.cu:155-158)
+    std::vector<ZmDesc> zm;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int k = 0; k < nr; ++k) {
+            FtsRep& r = b->reps[G.reps[k]];
+            for (int p = 0; p < r.d.npotentials; ++p) {
+                const mft_fts_potential& P = r.d.potentials[p];
+                if (!P.zero_mean || P.kind == MFT_FTS_FLORY || (P.scheme == MFT_FTS_EMPC) != (pass == 0)) continue;
+                zm.push_back(ZmDesc{r.wpl[p], r.Hpart, r.nHpart, p, r.M});
+            }
+        }
+        if (pass == 0) G.nzm_pc = (int)zm.size();
+    }
+    G.nzm = (int)zm.size();
+    G.d_zm = nullptr;
+    if (G.nzm) {
+        if ((rc = dev_alloc(G.allocs, &G.d_zm, zm.size()))) return rc;
+        MFT_HIP(hipMemcpy(G.d_zm, zm.data(), sizeof(ZmDesc) * zm.size(), hipMemcpyHostToDevice));
+    }
+
     // propagation descriptors per step and axis
     G.maxN = 0;
     for (int k = 0; k < nr; ++k)
@@ -1222,6 +1256,49 @@ static int group_density(FtsBatch* b, FtsGroup& G)
 
 // pass 0: the iteration's update (the predictor with EMPC), 1: the initial
 // pass (no update), 2: the EMPC corrector
+// w -= mean(w), the mean from the update pass's FP64 block sums, reduced in
+// the same fixed order by every workgroup (thrust::reduce / M then subtract,
+// fts_potential_helfand.cu:140-146).  256 threads.
+__global__ void __launch_bounds__(256) k_fts_zero_mean(const ZmDesc* __restrict__ descs)
+{
+    const ZmDesc& D = descs[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double sx = 0.0, sy = 0.0;
+    for (int k = threadIdx.x; k < D.nHpart; k += 256) {
+        const double* h = D.Hpart + (long)k * kFtsMaxPots * 6 + D.p * 6;
+        sx += h[2];
+        sy += h[3];
+    }
+    sx = wave_sum(sx);
+    sy = wave_sum(sy);
+    __shared__ double red[2][4];
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = sx;
+        red[1][threadIdx.x >> 6] = sy;
+    }
+    __syncthreads();
+    const double mx = (red[0][0] + red[0][1] + red[0][2] + red[0][3]) / (double)D.M;
+    const double my = (red[1][0] + red[1][1] + red[1][2] + red[1][3]) / (double)D.M;
+    if (i >= D.M) return;
+    const float2 w = D.w[i];
+    D.w[i] = make_float2((float)((double)w.x - mx), (float)((double)w.y - my));
+}
+
+static int group_update(FtsBatch* b, FtsGroup& G, int first);
+
+// zero-mean fields (after a correction only the EMPC ones), then the species
+// fields, E and the H sums again
+static int group_zero_mean(FtsBatch* b, FtsGroup& G, bool corrector)
+{
+    const int n = corrector ? G.nzm_pc : G.nzm;
+    if (!n) return MFT_OK;
+    const int M = G.n[0] * G.n[1] * G.n[2];
+    dim3 grid((M + 255) / 256, n);
+    k_fts_zero_mean<<<grid, 256, 0, b->stream>>>(G.d_zm);
+    MFT_LAUNCH_CHECK();
+    return group_update(b, G, 1);
+}
+
 static int group_update(FtsBatch* b, FtsGroup& G, int first)
 {
     const int M = G.n[0] * G.n[1] * G.n[2];
@@ -1249,13 +1326,15 @@ static FtsGroup& group_for(FtsBatch* b, const FtsRep& r)
 {
     const mft_fts_desc& d = r.d;
     for (auto& G : b->groups)
-        if (G.n[0] == d.nx && G.n[1] == d.ny && G.n[2] == d.nz && G.pc == r.pc && G.s1 == r.s1) return G;
+        if (G.n[0] == d.nx && G.n[1] == d.ny && G.n[2] == d.nz && G.pc == r.pc && G.s1 == r.s1 && G.zm == r.zm)
+            return G;
     FtsGroup G;
     G.n[0] = d.nx;
     G.n[1] = d.ny;
     G.n[2] = d.nz;
     G.pc = r.pc;
     G.s1 = r.s1;
+    G.zm = r.zm;
     b->groups.push_back(G);
     return b->groups.back();
 }
@@ -1517,10 +1596,12 @@ static int batch_iteration(FtsBatch* b)
             if ((rc = group_one_step(b, G))) return rc;
             if ((rc = group_update(b, G, 1))) return rc;
         }
+        if ((rc = group_zero_mean(b, G, false))) return rc;
         if ((rc = group_density(b, G))) return rc;
         if (G.pc) {
             // FTS_Box.cu:53-81: correct, rebuild species fields, densities again
             if ((rc = group_update(b, G, 2))) return rc;
+            if ((rc = group_zero_mean(b, G, true))) return rc;
             if ((rc = group_density(b, G))) return rc;
         }
     }

@@ -61,8 +61,12 @@ void FTS_Potential::parse_options(std::istringstream& iss, std::vector<std::comp
                 die("updateScheme " + updateScheme + " is not part of this build (EM, EMPC, 1S)");
         } else if (s1 == "modify") {
             iss >> s1;
-            if (s1 != "zeromean" && s1 != "zeroMean") die("Invalid modify option on " + potentialStyle + " potential");
-            die("modify zeromean is not part of this build");
+            // the incompressible potential reuses Helfand's message
+            // (fts_potential_incompress.cu:50-53)
+            if (s1 != "zeromean" && s1 != "zeroMean")
+                die("Invalid modify option on " + std::string(potentialStyle == "Incompress" ? "Helfand" : potentialStyle) +
+                    " potential");
+            zeroMean = true;
         } else {
             die("Invalid option " + s1 + " on " + potentialStyle + " potential");
         }
@@ -84,6 +88,7 @@ mft_fts_potential PotentialHelfand::descriptor() const
     mft_fts_potential d{};
     d.kind = MFT_FTS_HELFAND;
     d.scheme = updateScheme == "EMPC" ? MFT_FTS_EMPC : (updateScheme == "1S" ? MFT_FTS_1S : MFT_FTS_EM);
+    d.zero_mean = zeroMean ? 1 : 0;
     d.kappaN = kappaN;
     d.delt_pl = delt;
     return d;
@@ -108,6 +113,7 @@ mft_fts_potential PotentialFlory::descriptor() const
     mft_fts_potential d{};
     d.kind = MFT_FTS_FLORY;
     d.scheme = updateScheme == "EMPC" ? MFT_FTS_EMPC : (updateScheme == "1S" ? MFT_FTS_1S : MFT_FTS_EM);
+    d.zero_mean = zeroMean ? 1 : 0;
     d.I = I;
     d.J = J;
     d.chiN = chiN;
@@ -128,6 +134,7 @@ mft_fts_potential PotentialIncompress::descriptor() const
     mft_fts_potential d{};
     d.kind = MFT_FTS_INCOMPRESS;
     d.scheme = updateScheme == "EMPC" ? MFT_FTS_EMPC : (updateScheme == "1S" ? MFT_FTS_1S : MFT_FTS_EM);
+    d.zero_mean = zeroMean ? 1 : 0;
     d.delt_pl = delt;
     return d;
 }
@@ -145,6 +152,7 @@ mft_fts_potential PotentialEdwards::descriptor() const
     mft_fts_potential d{};
     d.kind = MFT_FTS_EDWARDS;
     d.scheme = updateScheme == "EMPC" ? MFT_FTS_EMPC : (updateScheme == "1S" ? MFT_FTS_1S : MFT_FTS_EM);
+    d.zero_mean = zeroMean ? 1 : 0;
     d.kappaN = B;
     d.delt_pl = delt;
     return d;
@@ -366,8 +374,8 @@ void FTS_Box::finishInitialization()
         for (int i = 0; i < d.npotentials; ++i) {
             const auto& p = d.potentials[i];
             std::printf("%s{\"kind\": %d, \"I\": %d, \"J\": %d, \"chiN\": %.9g, \"kappaN\": %.9g, \"delt_pl\": %.9g, "
-                        "\"delt_mi\": %.9g, \"scheme\": %d, \"wpl0\": [%.9g, %.9g]}",
-                        i ? ", " : "", p.kind, p.I, p.J, p.chiN, p.kappaN, p.delt_pl, p.delt_mi, p.scheme,
+                        "\"delt_mi\": %.9g, \"scheme\": %d, \"zero_mean\": %d, \"wpl0\": [%.9g, %.9g]}",
+                        i ? ", " : "", p.kind, p.I, p.J, p.chiN, p.kappaN, p.delt_pl, p.delt_mi, p.scheme, p.zero_mean,
                         Potentials[i]->wpl[0].real(), Potentials[i]->wpl[0].imag());
         }
         std::printf("], \"molecules\": [");

@@ -422,7 +422,7 @@ class TildEngine:
 # --------------------------------------------------------------- FTS -----
 class FtsPotentialC(C.Structure):
     _fields_ = [("kind", _i), ("I", _i), ("J", _i), ("chiN", _d), ("kappaN", _d), ("delt_pl", _d), ("delt_mi", _d),
-                ("scheme", _i)]
+                ("scheme", _i), ("zero_mean", _i)]
 
 
 FTS_SCHEME = {"EM": 0, "EMPC": 1, "1S": 2}
@@ -473,7 +473,7 @@ def _c64(a):
 def _fts_pot(p):
     return FtsPotentialC(FTS_POT[p["kind"]], p.get("I", -1), p.get("J", -1), p.get("chiN", 0.0),
                          p.get("kappaN", p.get("B", 0.0)), p.get("dt_pl", 0.0), p.get("dt_mi", 0.0),
-                         FTS_SCHEME[p.get("scheme", "EM")])
+                         FTS_SCHEME[p.get("scheme", "EM")], int(p.get("zero_mean", False)))
 
 
 def _fts_mol(blocks):

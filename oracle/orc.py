@@ -467,6 +467,7 @@ class FtsPotential:
     wpl: np.ndarray = None
     wmi: np.ndarray = None
     scheme: str = "EM"        # EM | EMPC | 1S (updateScheme keyword)
+    zero_mean: bool = False   # modify zeroMean (no effect on Flory, fts_potential_flory.cu:94-98)
     saved: dict = None        # EMPC predictor data: fields, forces and noise
     Akpl: np.ndarray = None   # 1S linear coefficients of w+ and w- (real, k space)
     Akmi: np.ndarray = None
@@ -755,6 +756,19 @@ def fts_update_fields(sys: FtsSystem, z=None):
         L.orc_fts_update_em(_ptr(p.wpl), _ptr(dHp), _ptr(npl), p.dt_pl, M)
         if p.kind == "Flory":
             L.orc_fts_update_em(_ptr(p.wmi), _ptr(dHm), _ptr(nmi), p.dt_mi, M)
+    _zero_means(sys)
+
+
+def _zero_means(sys: FtsSystem, only_empc=False):
+    """modify zeroMean: w+ minus its grid mean after an update or an EMPC
+    correction (fts_potential_helfand.cu:140-146,198-204,
+    fts_potential_incompress.cu:125-131,177-183, fts_potential_edwards.cu:
+    146-152,202-208); Flory only parses the keyword."""
+    for p in sys.potentials:
+        if only_empc and p.scheme != "EMPC":
+            continue
+        if p.zero_mean and p.kind != "Flory":
+            p.wpl -= p.wpl.sum() / p.wpl.size
 
 
 def fts_correct_fields(sys: FtsSystem):
@@ -772,6 +786,7 @@ def fts_correct_fields(sys: FtsSystem):
         if p.kind == "Flory":
             L.orc_fts_update_empc(_ptr(p.wmi), _ptr(sv["wmi"]), _ptr(dHm), _ptr(sv["dHm"]), _ptr(sv["nmi"]),
                                   p.dt_mi, M)
+    _zero_means(sys, only_empc=True)
 
 
 def fts_hamiltonian(sys: FtsSystem):
@@ -830,7 +845,7 @@ def fts_system_from_spec(spec, fields):
     for k, p in enumerate(spec["potentials"]):
         fp = FtsPotential(kind=p["kind"], I=p.get("I", -1), J=p.get("J", -1), chiN=p.get("chiN", 0.0),
                           kappaN=p.get("kappaN", p.get("B", 0.0)), dt_pl=p.get("dt_pl", 0.0), dt_mi=p.get("dt_mi", 0.0),
-                          scheme=p.get("scheme", "EM"))
+                          scheme=p.get("scheme", "EM"), zero_mean=bool(p.get("zero_mean", False)))
         fp.wpl = np.array(fields.get((k, "pl"), np.zeros(M)), np.complex128)
         if p["kind"] == "Flory":
             fp.wmi = np.array(fields.get((k, "mi"), np.zeros(M)), np.complex128)

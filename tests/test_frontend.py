@@ -142,3 +142,14 @@ def test_find_spinodal_arguments(tmp_path):
 def test_modify_errors_die_like_the_reference(tmp_path, cmd, msg):
     r = dry(tmp_path, EDWARDS_GC + cmd + "\n")
     assert r.returncode == 1 and msg in r.stdout
+
+
+def test_modify_zero_mean_keyword(tmp_path):
+    # fts_potential_helfand.cu:55-59 / fts_potential_flory.cu:94-98
+    text = EDWARDS_GC.replace("potential Edwards 3.0 0.01", "potential Edwards 3.0 0.01 modify zeroMean")
+    r = dry(tmp_path, text)
+    assert r.returncode == 0, r.stdout[-1000:]
+    e, f = json.loads(r.stdout.strip().splitlines()[-1])["potentials"]
+    assert e["zero_mean"] == 1 and f["zero_mean"] == 0
+    r = dry(tmp_path, text.replace("modify zeroMean", "modify sideways"))
+    assert r.returncode == 1 and "Invalid modify option on Edwards potential" in r.stdout

@@ -134,7 +134,8 @@ def test_closed_forms_on_device(gpu):
 
 @pytest.mark.parametrize("case", ["diblock", "diblock_square", "incompress_blend", "diblock_empc",
                                   "blend_empc_mixed", "diblock_1s", "blend_schemes_mixed", "edwards_diblock",
-                                  "edwards_1s", "grand_canonical_blend", "grand_canonical_1s"])
+                                  "edwards_1s", "grand_canonical_blend", "grand_canonical_1s", "zero_mean_em",
+                                  "zero_mean_empc_mixed", "zero_mean_1s"])
 def test_cl_steps_match_oracle(gpu, case):
     say("case: fts CL steps", case)
     if case == "diblock":
@@ -170,6 +171,22 @@ def test_cl_steps_match_oracle(gpu, case):
         spec["potentials"][0] = dict(kind="Edwards", B=2.0, dt_pl=0.01)
         if case == "edwards_1s":
             spec["potentials"] = [dict(p, scheme="1S") for p in spec["potentials"]]
+    elif case in ("zero_mean_em", "zero_mean_1s"):
+        # modify zeroMean on Helfand / Edwards (Flory parses it without effect)
+        spec, fields = systems.fts_diblock(grid_shape=(16, 24, 12), L=(3.0, 4.0, 2.5), Nr=20.0, NA=8, NB=12,
+                                           chiN=15.0, kappaN=50.0, C=5.0, dt=0.01, sigma=0.1, seed=33)
+        spec["potentials"] = [dict(p, zero_mean=True) for p in spec["potentials"]]
+        if case == "zero_mean_1s":
+            spec["potentials"][0] = dict(kind="Edwards", B=2.0, dt_pl=0.01, zero_mean=True)
+            spec["potentials"] = [dict(p, scheme="1S") for p in spec["potentials"]]
+    elif case == "zero_mean_empc_mixed":
+        # a zero-mean EMPC incompressibility is re-centred after the corrector too
+        spec = tiny_spec(Nx=(12, 16, 20), L=(3.0, 4.0, 5.0), pots=[
+            dict(kind="Incompress", dt_pl=0.01, scheme="EMPC", zero_mean=True),
+            dict(kind="Helfand", kappaN=40.0, dt_pl=0.01, zero_mean=True),
+            dict(kind="Flory", I=0, J=1, chiN=10.0, dt_pl=0.01, dt_mi=0.01)])
+        spec["molecules"] = [dict(blocks=[(4, 0), (3, 1)], phi=0.6), dict(blocks=[(5, 1)], phi=0.4)]
+        fields = random_fields(spec, 0.1, seed=16, imag=False)
     elif case in ("grand_canonical_blend", "grand_canonical_1s"):
         # a grand canonical homopolymer beside a canonical diblock
         # (fts_molecule.cu:30-31, fts_molecule_linear.cu:286-291,374-376)

@@ -348,3 +348,40 @@ def test_grand_canonical_density_is_canonical_density_rescaled():
     orc.fts_init(g)
     scale = 0.6 * c.Q[0] * c.V() / c.molecules[0].nmolecs
     np.testing.assert_allclose(g.rho, c.rho * scale, rtol=1e-12, atol=1e-14)
+
+
+def test_zero_mean_recentres_w_plus_after_the_update():
+    # modify zeroMean (fts_potential_helfand.cu:140-146): w+ has zero grid
+    # mean after the update; Flory only parses it (fts_potential_flory.cu:94-98)
+    pots = [dict(kind="Helfand", kappaN=30.0, dt_pl=0.02, zero_mean=True),
+            dict(kind="Flory", I=0, J=1, chiN=12.0, dt_pl=0.02, dt_mi=0.03, zero_mean=True)]
+    spec = tiny_spec(pots=pots)
+    f = random_fields(spec, 0.2, seed=14)
+    s = orc.fts_system_from_spec(spec, f)
+    orc.fts_init(s)
+    orc.fts_step(s, orc.fts_noise(s.M, 5, 0))
+    assert abs(s.potentials[0].wpl.mean()) < 1e-15
+    assert abs(s.potentials[1].wpl.mean()) > 1e-3
+
+
+def test_zero_mean_incompress_leaves_densities_unchanged():
+    # the incompressible drift i rho/Nr - iC has no w term and a constant shift
+    # of w+ only changes Q by a phase, so with zeroMean the densities and the
+    # Flory fields are those of the run without it, and w+ differs by a constant
+    pots = [dict(kind="Incompress", dt_pl=0.01), dict(kind="Flory", I=0, J=1, chiN=10.0, dt_pl=0.01, dt_mi=0.01)]
+    spec = tiny_spec(pots=pots)
+    f = random_fields(spec, 0.1, seed=15, imag=False)
+    a = orc.fts_system_from_spec(spec, f)
+    spec_z = dict(spec, potentials=[dict(pots[0], zero_mean=True), pots[1]])
+    b = orc.fts_system_from_spec(spec_z, f)
+    orc.fts_init(a)
+    orc.fts_init(b)
+    for it in range(3):
+        z = orc.fts_noise(a.M, 5, it)
+        orc.fts_step(a, z)
+        orc.fts_step(b, z)
+    np.testing.assert_allclose(b.rho, a.rho, rtol=1e-9, atol=1e-12 * np.abs(a.rho).max())
+    np.testing.assert_allclose(b.potentials[1].wmi, a.potentials[1].wmi, rtol=1e-9, atol=1e-12)
+    d = b.potentials[0].wpl - a.potentials[0].wpl
+    np.testing.assert_allclose(d, d.mean(), atol=1e-12)
+    assert abs(d.mean() + a.potentials[0].wpl.mean()) < 1e-12
