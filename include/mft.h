@@ -319,8 +319,8 @@ int mft_fts_get_densities(mft_fts* sys, float* rho);
 /* Species fields W[nspecies*M] (complex, fts_species.cu:53-132). */
 int mft_fts_get_species_fields(mft_fts* sys, float* W);
 /* Propagators of molecule m: q[Ntot*M], qdag[Ntot*M] in the reference's
- * q-form (fts_molecule_linear.cu:144-269), reconstructed from the stored
- * symmetric form; either may be NULL. */
+ * q-form (fts_molecule_linear.cu:144-269), the form the engine stores;
+ * either may be NULL. */
 int mft_fts_get_propagators(mft_fts* sys, int m, float* q, float* qdag);
 /* Q[m] (complex), H (complex) and its parts: potentials then molecules
  * (FTS_Box::computeHamiltonian, FTS_Box.cu:590-602), FP64 reductions. */

@@ -3,17 +3,18 @@
 // Per iteration and replica:
 //   update: every potential's EM step with real CL noise from the previous
 //     densities (fts_potential_*.cu updateFields), species fields
-//     W_s (fts_species.cu:53-132) and E_s = exp(-W_s/2), the first propagator
+//     W_s (fts_species.cu:53-132) and E_s = exp(-W_s), the first propagator
 //     slice of every chain, and FP64 partial sums of the Hamiltonian field
 //     terms -- one elementwise pass;
-//   propagators in the symmetric form p_s = e^{W_s/2} q_s
-//     (fts_molecule_linear.cu:144-269 uses q_s = e^{-W_s} F^-1[g F q_{s-1}]):
-//     p_s = E_{b(s)} D[E_{b(s-1)} p_{s-1}], D = F^-1 g F applied axis by axis
-//     because g(k) = exp(-k^2/(Nr-1)) factorises over kx, ky, kz; the forward
-//     and complementary chains of every replica share each launch;
-//   density: rho_b ~ sum_s p_s pdag_{N-1-s} (equal to e^{W} sum q qdag of
-//     fts_molecule_linear.cu:301-328) and Q = gvol/V sum E p_{N-1}, FP64
-//     accumulation; the n/(QV) factor is applied by the next update.
+//   propagators as the reference's q_s = e^{-W_b(s)} F^-1[g F q_{s-1}]
+//     (fts_molecule_linear.cu:144-269), D = F^-1 g F applied axis by axis
+//     because g(k) = exp(-k^2/(Nr-1)) factorises over kx, ky, kz: the x-y pass
+//     reads q_{s-1} alone and the z pass multiplies by E_b(s), so a bead costs
+//     two reads and two writes of a slice plus one read of E; the forward and
+//     complementary chains of every replica share each launch;
+//   density: rho_b ~ e^{W_b} sum_s q_s qdag_{N-1-s} (fts_molecule_linear.cu:
+//     298-327) and Q = gvol/V sum q_{N-1}, FP64 accumulation; the n/(QV)
+//     factor is applied by the next update.
 // Replicas with equal grids form one group; every launch covers a group.
 #include <cmath>
 #include <algorithm>
@@ -66,7 +67,7 @@ struct FtsUpdDev {
     double activity[kFtsMaxMol];   // > 0: grand canonical, density factor = activity
     double V, C, Nr;
     float2* W;                     // [nspecies][M]
-    float2* E;                     // [nspecies][M]
+    float2* E;                     // [nspecies][M] exp(-W)
     int nchain;
     float2* chain_p0[2 * kFtsMaxMol];
     int chain_species0[2 * kFtsMaxMol];
@@ -301,9 +302,9 @@ __global__ void __launch_bounds__(kFtsThreads) k_fts_update(const FtsUpdDev* __r
             if (s >= D.nspecies) continue;
             double wr = W[s].x * inv, wi = W[s].y * inv;
             D.W[(long)s * D.M + i] = make_float2((float)wr, (float)wi);
-            // exp(-W/2)
-            double a = exp(-0.5 * wr), sn, cs;
-            sincos(-0.5 * wi, &sn, &cs);
+            // exp(-W): the bead weight of the q-form (fts_molecule_linear.cu:153-199)
+            double a = exp(-wr), sn, cs;
+            sincos(-wi, &sn, &cs);
             Es[s] = make_float2((float)(a * cs), (float)(a * sn));
             D.E[(long)s * D.M + i] = Es[s];
         }
@@ -336,8 +337,8 @@ struct ZmDesc {
 struct PropDesc {
     const float2* src;
     float2* dst;
-    const float2* pre;   // multiply before (E_{b(s-1)}) or null
-    const float2* post;  // multiply after (E_{b(s)}) or null
+    const float2* pre;   // multiply before, or null
+    const float2* post;  // multiply after (E_b(s) on the last axis), or null
     const float2* g;     // exp(-k_a^2/(Nr-1)) / N_a along this axis as hi + lo
 };
 
@@ -393,8 +394,8 @@ __global__ void __launch_bounds__(256) k_fts_prop_axis(const PropDesc* __restric
     }
 }
 
-// x and y diffusion of one z-plane in one workgroup: the plane (times
-// E_{b(s-1)}) is staged in LDS, every row is transformed along x, multiplied
+// x and y diffusion of one z-plane in one workgroup: the plane (times pre,
+// when given) is staged in LDS, every row is transformed along x, multiplied
 // by g_x and transformed back in place, then every column along y; the x and y
 // passes of k_fts_prop_axis with the plane kept in LDS between them (the same
 // arithmetic, one global read and write instead of two).  Rows are padded by
@@ -517,14 +518,17 @@ __global__ void __launch_bounds__(NT) k_fts_prop_xy(const PropDesc* __restrict__
 struct DensDesc {
     const float2* p;       // forward chain [Ntot][M]
     const float2* pd;      // complementary chain (== p when symmetric)
-    const float2* Elast;   // E of the last bead's species (for Q)
+    const float2* W;       // species fields [nspecies][M]
     float2* rhot;          // [nblocks][M]
     double* Qpart;         // [gridDim.x]
     int M, Ntot, nblocks;
     int N[kFtsMaxBlocks];
+    int species[kFtsMaxBlocks];
 };
 
-// rho_b(r) = sum_{s in b} p_s(r) pdag_{Ntot-1-s}(r)  and  Q partial sums.
+// rho_b(r) = e^{+W_b(r)} sum_{s in b} q_s(r) qdag_{Ntot-1-s}(r)
+// (fts_molecule_linear.cu:298-327) and the partial sums of Q = sum q_{Ntot-1}
+// (fts_molecule_linear.cu:265-266).
 __global__ void __launch_bounds__(kFtsThreads) k_fts_density(const DensDesc* __restrict__ descs)
 {
     const DensDesc& D = descs[blockIdx.y];
@@ -540,12 +544,15 @@ __global__ void __launch_bounds__(kFtsThreads) k_fts_density(const DensDesc* __r
                 ar += (double)a.x * c.x - (double)a.y * c.y;
                 ai += (double)a.x * c.y + (double)a.y * c.x;
             }
-            D.rhot[(long)b * D.M + i] = make_float2((float)ar, (float)ai);
+            const float2 w = D.W[(long)D.species[b] * D.M + i];
+            double sn, cs;
+            sincos((double)w.y, &sn, &cs);
+            const double e = exp((double)w.x), er = e * cs, ei = e * sn;
+            D.rhot[(long)b * D.M + i] = make_float2((float)(ar * er - ai * ei), (float)(ar * ei + ai * er));
         }
-        float2 e = D.Elast[i];
-        float2 pl = D.p[(long)(D.Ntot - 1) * D.M + i];
-        qr = (double)e.x * pl.x - (double)e.y * pl.y;
-        qi = (double)e.x * pl.y + (double)e.y * pl.x;
+        const float2 pl = D.p[(long)(D.Ntot - 1) * D.M + i];
+        qr = pl.x;
+        qi = pl.y;
     }
     __shared__ double red[2 * (kFtsThreads / 64)];
     qr = wave_sum(qr);
@@ -1089,7 +1096,7 @@ static int group_setup(FtsBatch* b, FtsGroup& G)
             std::memset(&D, 0, sizeof(D));
             D.p = r.p[m];
             D.pd = r.pd[m];
-            D.Elast = r.E + (size_t)mol.species[mol.nblocks - 1] * r.M;
+            D.W = r.W;
             int off = 0;
             for (int mm = 0; mm < m; ++mm) off += r.d.molecules[mm].nblocks;
             D.rhot = r.rhot + (size_t)off * r.M;
@@ -1097,7 +1104,10 @@ static int group_setup(FtsBatch* b, FtsGroup& G)
             D.M = r.M;
             D.Ntot = r.Ntot[m];
             D.nblocks = mol.nblocks;
-            for (int bb = 0; bb < mol.nblocks; ++bb) D.N[bb] = mol.N[bb];
+            for (int bb = 0; bb < mol.nblocks; ++bb) {
+                D.N[bb] = mol.N[bb];
+                D.species[bb] = mol.species[bb];
+            }
             dens.push_back(D);
             QDesc q;
             q.Qpart = D.Qpart;
@@ -1153,9 +1163,9 @@ static int group_setup(FtsBatch* b, FtsGroup& G)
                     if (s >= r.Ntot[m]) continue;
                     float2* P = c == 0 ? r.p[m] : r.pd[m];
                     float2* tmp = r.tmp + (size_t)chain * r.M;
-                    const float2* Epre = r.E + (size_t)bead_species(mol, r.Ntot[m], s - 1, c) * r.M;
+                    // q_s = e^{-W_b(s)} IFFT[g FFT[q_(s-1)]] (fts_molecule_linear.cu:166-199)
                     const float2* Epost = r.E + (size_t)bead_species(mol, r.Ntot[m], s, c) * r.M;
-                    ax[0].push_back(PropDesc{P + (size_t)(s - 1) * r.M, tmp, Epre, nullptr, r.g[0]});
+                    ax[0].push_back(PropDesc{P + (size_t)(s - 1) * r.M, tmp, nullptr, nullptr, r.g[0]});
                     ax[1].push_back(PropDesc{tmp, tmp, nullptr, nullptr, r.g[1]});
                     ax[2].push_back(PropDesc{tmp, P + (size_t)s * r.M, nullptr, Epost, r.g[2]});
                 }
@@ -1911,21 +1921,10 @@ int mft_fts_get_propagators(mft_fts* s, int m, float* q, float* qdag)
     const mft_fts_molecule& mol = r.d.molecules[m];
     const int Nt = r.Ntot[m];
     const size_t M = r.M;
-    std::vector<float2> E(M * r.d.nspecies);
-    MFT_HIP(hipMemcpy(E.data(), r.E, sizeof(float2) * E.size(), hipMemcpyDeviceToHost));
+    (void)mol;
     for (int c = 0; c < 2; ++c) {
         float* out = c == 0 ? q : qdag;
-        if (!out) continue;
-        float2* o2 = (float2*)out;
-        MFT_HIP(hipMemcpy(o2, c == 0 ? r.p[m] : r.pd[m], sizeof(float2) * M * Nt, hipMemcpyDeviceToHost));
-        // q_s = E_{b(s)} p_s
-        for (int s = 0; s < Nt; ++s) {
-            int sp = bead_species(mol, Nt, s, c == 0 || r.sym[m] ? 0 : 1);
-            for (size_t i = 0; i < M; ++i) {
-                float2 a = o2[(size_t)s * M + i], e = E[(size_t)sp * M + i];
-                o2[(size_t)s * M + i] = make_float2(a.x * e.x - a.y * e.y, a.x * e.y + a.y * e.x);
-            }
-        }
+        if (out) MFT_HIP(hipMemcpy(out, c == 0 ? r.p[m] : r.pd[m], sizeof(float2) * M * Nt, hipMemcpyDeviceToHost));
     }
     return MFT_OK;
 }
