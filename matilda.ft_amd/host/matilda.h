@@ -187,6 +187,9 @@ public:
     void writeData(int step) override;                  // PS_Box.cu:339-372
     void computeThermoProps();                          // PS_Box.cu:282-317
     void writeDataConfig(const std::string& name);      // LAMMPS data, PS_Box.cu:629-720
+    void writeLammpsTraj(int step);                     // PS_Box.cu:756-801
+    int trajFreq = 0;
+    std::string trajFileName = "traj.lammpstrj";
     int findSpeciesInteger(const std::string& s) const;
     int findGroupInteger(const std::string& s) const;
     uint32_t groupMask(const std::string& s) const;

@@ -243,15 +243,19 @@ void PS_Box::readInput(std::ifstream& inp)
                 readDimension = true;
             } else if (firstWord == "doCharges") {
                 enableCharges();
+            } else if (firstWord == "trajFreq" || firstWord == "traj_freq") {
+                // PS_Box_initialization.cu:314-315
+                iss >> trajFreq;
+            } else if (firstWord == "trajFileName") {
+                iss >> trajFileName;
             } else if (firstWord == "fieldFreq" || firstWord == "field_freq" || firstWord == "gsdFreq" ||
-                       firstWord == "gsd_freq" || firstWord == "trajFreq" || firstWord == "traj_freq") {
+                       firstWord == "gsd_freq") {
                 int f;
                 iss >> f;
                 std::string rest;
                 while (iss >> rest) {}
                 if (f > 0) std::cout << "note: " << firstWord << " output is not written by this build" << std::endl;
-            } else if (firstWord == "gsdName" || firstWord == "gsd_name" || firstWord == "trajFileName" ||
-                       firstWord == "initDataFileName") {
+            } else if (firstWord == "gsdName" || firstWord == "gsd_name" || firstWord == "initDataFileName") {
                 std::string s;
                 iss >> s;
             } else if (firstWord == "grid") {
@@ -598,6 +602,33 @@ void PS_Box::doTimeStep(int step)
     check_mft(mft_tild_step(eng, 1), "mft_tild_step");
     for (auto* p : potentials) p->update_prefactor(step, maxSteps);
     if (logFreq > 0 && step % logFreq == 0) writeData(step);
+    if (trajFreq > 0 && step % trajFreq == 0) writeLammpsTraj(step);
+}
+
+// PS_Box::writeLammpsTraj (PS_Box.cu:756-801): positions wrapped once into
+// the box, "id type mol x y z" per particle; step 0 starts the file
+void PS_Box::writeLammpsTraj(int step)
+{
+    std::vector<float> x(3 * (size_t)nstot);
+    check_mft(mft_tild_get_state(eng, x.data(), nullptr, nullptr), "mft_tild_get_state");
+    for (int i = 0; i < nstot; ++i)
+        for (int j = 0; j < 3; ++j) {
+            float& v = x[(size_t)i * 3 + j];
+            if (v > L[j]) v -= L[j];
+            else if (v < 0.0f) v += L[j];
+        }
+    FILE* otp = std::fopen(trajFileName.c_str(), step == 0 ? "w" : "a");
+    if (!otp) die("Failed to open " + trajFileName);
+    std::fprintf(otp, "ITEM: TIMESTEP\n%d\nITEM: NUMBER OF ATOMS\n%d\n", step, nstot);
+    std::fprintf(otp, "ITEM: BOX BOUNDS pp pp pp\n");
+    std::fprintf(otp, "%f %f\n%f %f\n%f %f\n", 0.f, L[0], 0.f, L[1], 0.f, L[2]);
+    std::fprintf(otp, "ITEM: ATOMS id type mol x y z\n");
+    for (int i = 0; i < nstot; ++i) {
+        std::fprintf(otp, "%d %d %d  ", i + 1, intSpecies[i] + 1, mID[i] + 1);
+        for (int j = 0; j < 3; ++j) std::fprintf(otp, "%f ", x[(size_t)i * 3 + j]);
+        std::fprintf(otp, "\n");
+    }
+    std::fclose(otp);
 }
 
 void PS_Box::NVT(int nSteps)
@@ -609,8 +640,10 @@ void PS_Box::NVT(int nSteps)
     // steps between log points run as one engine call
     int i = 0;
     while (i < maxSteps) {
+        // next log or trajectory step
         int next = maxSteps;
-        if (logFreq > 0) next = std::min(maxSteps, (i / logFreq + (i % logFreq ? 1 : 0)) * logFreq);
+        if (logFreq > 0) next = std::min(next, (i / logFreq + (i % logFreq ? 1 : 0)) * logFreq);
+        if (trajFreq > 0) next = std::min(next, (i / trajFreq + (i % trajFreq ? 1 : 0)) * trajFreq);
         if (next > i) {
             check_mft(mft_tild_step(eng, next - i), "mft_tild_step");
             for (int s = i; s < next; ++s)

@@ -154,3 +154,47 @@ def test_modify_and_find_spinodal_match_engine(gpu, tmp_path):
     assert len(sweep) == 1 + len(tries)
     assert close(sweep[:, 2], [0.02] + [0.01 * 0.9 ** k for k in range(len(tries))])
     s.close()
+
+
+def read_lammpstrj(path):
+    """Frames of a LAMMPS dump (PS_Box::writeLammpsTraj): [(step, ids, types, mols, x)]."""
+    frames = []
+    lines = open(path).read().split("\n")
+    k = 0
+    while k < len(lines) and lines[k].startswith("ITEM: TIMESTEP"):
+        step = int(lines[k + 1])
+        n = int(lines[k + 3])
+        assert lines[k + 4] == "ITEM: BOX BOUNDS pp pp pp" and lines[k + 8] == "ITEM: ATOMS id type mol x y z"
+        rows = np.array([l.split() for l in lines[k + 9:k + 9 + n]], np.float64)
+        frames.append((step, rows[:, 0].astype(int), rows[:, 1].astype(int), rows[:, 2].astype(int), rows[:, 3:6]))
+        k += 9 + n
+    return frames
+
+
+def test_ps_trajectory_frames_match_engine(gpu, tmp_path):
+    # trajFreq (PS_Box.cu:190-194, 756-801): frames after steps 0, 5, 10
+    cfg = small_system(False, seed=8)
+    say("case: front end lammpstrj")
+    write_data_file(tmp_path / "init.data", cfg)
+    text = ps_input(cfg, "init.data", steps=12, log_freq=100).replace("endBox", "trajFreq 5\nendBox")
+    (tmp_path / "input").write_text(text)
+    r = run_mft(["-in", "input"], cwd=tmp_path)
+    assert r.returncode == 0, r.stdout[-3000:]
+    frames = read_lammpstrj(tmp_path / "traj.lammpstrj")
+    assert [f[0] for f in frames] == [0, 5, 10]
+    n = cfg["topo"]["type"].size
+    L = np.asarray(cfg["L"], np.float64)
+    eng = systems.tild_engine(cfg, "GJF")
+    eng.set_run(12)
+    done = 0
+    for step, ids, types, mols, x in frames:
+        assert np.array_equal(ids, np.arange(1, n + 1))
+        assert np.array_equal(types, np.asarray(cfg["topo"]["type"]) + 1)
+        eng.step(step + 1 - done)
+        done = step + 1
+        xe, _, _ = eng.get_state()
+        d = (x - xe.reshape(-1, 3)) / L
+        d -= np.round(d)
+        assert np.max(np.abs(d)) < 1e-5, step
+        assert np.all((x >= 0) & (x <= L + 1e-6))
+    eng.close()
