@@ -231,6 +231,11 @@ int mft_tild_num_outputs(mft_tild* eng, int* nout);
 int mft_tild_autotune(mft_tild* eng, int reps, int* choice);
 /* Selects a gather variant directly (as above; the default is 1). */
 int mft_tild_set_gather(mft_tild* eng, int variant);
+/* PS_ComputeSK::do_compute (ps_compute_sk.cu:18-45): sk[M] (device) = or +=
+ * |FFT[rho_group]|^2 with the forward transform scaled by 1/M, rho_group the
+ * sum of the densities of the species in species_mask at the last density
+ * pass (the start of the last step).  work: device scratch of 8*M bytes. */
+int mft_tild_structure_factor(mft_tild* eng, uint32_t species_mask, float* sk, int accumulate, void* work);
 
 /* =============================================================== FTS ==== */
 /* Per-kernel entry points (complex interleaved float fields [M]). */

@@ -277,6 +277,51 @@ int mft_tild_em_step(float* x, const float* f, int nparticles, float delt, const
 
 } // extern "C"
 
+// ----------------------------------------------- structure factor ---
+namespace mft {
+
+// density of a group of species (the sum of its species' densities)
+__global__ void k_group_density(float2* __restrict__ out, const float* __restrict__ rho, uint32_t mask, int nspecies,
+                                long M)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    float v = 0.f;
+    for (int sp = 0; sp < nspecies; ++sp)
+        if ((mask >> sp) & 1u) v += rho[(long)sp * M + i];
+    out[i] = make_float2(v, 0.f);
+}
+
+// sk = |a|^2 (or +=): rho(k) rho(-k) of a real field (d_multiplyCpxByCpxConj)
+__global__ void k_abs2(float* __restrict__ sk, const float2* __restrict__ a, long M, int accumulate)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const float2 v = a[i];
+    const float s = v.x * v.x + v.y * v.y;
+    sk[i] = accumulate ? sk[i] + s : s;
+}
+
+} // namespace mft
+
+extern "C" int mft_tild_structure_factor(mft_tild* eng, uint32_t species_mask, float* sk, int accumulate, void* work)
+{
+    // mft_tild derives from TildEngine with nothing added (tild.hip)
+    TildEngine* e = reinterpret_cast<TildEngine*>(eng);
+    MFT_REQUIRE(e && sk && work, MFT_ERR_INVALID, "null argument");
+    MFT_REQUIRE(species_mask != 0 && (e->d.nspecies >= 32 || (species_mask >> e->d.nspecies) == 0), MFT_ERR_INVALID,
+                "species mask 0x%x does not match %d species", species_mask, e->d.nspecies);
+    int rc;
+    if (!e->rho_valid && (rc = tild_density_reduce(e))) return rc;
+    const long M = e->M;
+    float2* a = (float2*)work;
+    k_group_density<<<nblk(M), 256, 0, e->stream>>>(a, e->rho, species_mask, e->d.nspecies, M);
+    if ((rc = fft3_complex(a, e->d.nx, e->d.ny, e->d.nz, -1, e->stream))) return rc;
+    k_abs2<<<nblk(M), 256, 0, e->stream>>>(sk, a, M, accumulate);
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
 // ------------------------------------------------------------- FTS ---
 namespace mft {
 

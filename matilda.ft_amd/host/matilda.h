@@ -177,6 +177,40 @@ public:
 };
 Integrator* IntegratorFactory(std::istringstream& iss, PS_Box* box);
 
+// PS_Compute (ps_compute.cu:8-43): "compute avg_sk group [wait W] [freq F]
+// [file_name name]"; PS_ComputeSK (ps_compute_sk.cu) accumulates
+// |rho_group(k)|^2 on the device every freq steps after wait and writes the
+// average at log steps.
+class PS_Compute {
+public:
+    PS_Compute(std::istringstream& iss, PS_Box* box);
+    virtual ~PS_Compute() {}
+    virtual void do_compute(int step) = 0;
+    virtual void write_output() = 0;
+    virtual void initialize_compute() = 0;
+    bool active(int step) const { return step > compute_wait && compute_freq > 0 && step % compute_freq == 0; }
+    int compute_id = 0, compute_wait = 0, compute_freq = 100, num_data_pts = 0;
+    std::string group, output_name;
+    static int total_computes;
+
+protected:
+    PS_Box* mybox;
+};
+class PS_ComputeSK : public PS_Compute {
+public:
+    PS_ComputeSK(std::istringstream& iss, PS_Box* box);
+    ~PS_ComputeSK() override;
+    void do_compute(int step) override;
+    void write_output() override;
+    void initialize_compute() override;
+
+private:
+    uint32_t mask = 0;
+    float* d_sk = nullptr;
+    void* d_work = nullptr;
+};
+PS_Compute* PSComputeFactory(std::istringstream& iss, PS_Box* box);
+
 class PS_Box : public Box {
 public:
     explicit PS_Box(std::istringstream& iss);
@@ -188,6 +222,10 @@ public:
     void computeThermoProps();                          // PS_Box.cu:282-317
     void writeDataConfig(const std::string& name);      // LAMMPS data, PS_Box.cu:629-720
     void writeLammpsTraj(int step);                     // PS_Box.cu:756-801
+    void writeKFieldFloat(const std::string& name, const float* dat); // PS_Box.cu:531-560
+    mft_tild* engine() const { return eng; }
+    long gridSize() const { return M; }
+    std::vector<PS_Compute*> computes;
     int trajFreq = 0;
     std::string trajFileName = "traj.lammpstrj";
     int findSpeciesInteger(const std::string& s) const;

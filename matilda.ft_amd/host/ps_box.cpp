@@ -161,6 +161,7 @@ PS_Box::~PS_Box()
 {
     for (auto* p : potentials) delete p;
     for (auto* i : integrators) delete i;
+    for (auto* c : computes) delete c;
     if (eng) mft_tild_destroy(eng);
 }
 
@@ -234,7 +235,7 @@ void PS_Box::readInput(std::ifstream& inp)
             } else if (firstWord == "boxLengths") {
                 read_lengths(iss);
             } else if (firstWord == "compute") {
-                die("computes are not part of this build's particle-mesh path");
+                computes.push_back(PSComputeFactory(iss, this));
             } else if (firstWord == "datFileName" || firstWord == "dat_file_name") {
                 iss >> datFileName;
             } else if (firstWord == "Dim") {
@@ -545,6 +546,11 @@ void PS_Box::finishInitialization()
         std::printf("], \"bonds\": [");
         for (size_t i = 0; i < bondK.size(); ++i)
             std::printf("%s[%d, %.9g, %.9g]", i ? ", " : "", bondStyle[i], bondK[i], bondReq[i]);
+        std::printf("], \"computes\": [");
+        for (size_t i = 0; i < computes.size(); ++i)
+            std::printf("%s{\"style\": \"avg_sk\", \"mask\": %u, \"wait\": %d, \"freq\": %d, \"file\": \"%s\"}",
+                        i ? ", " : "", groupMask(computes[i]->group), computes[i]->compute_wait,
+                        computes[i]->compute_freq, computes[i]->output_name.c_str());
         std::printf("]}\n");
         std::fflush(stdout);
         initialized = true;
@@ -559,6 +565,7 @@ void PS_Box::finishInitialization()
     check_mft(mft_tild_autotune(eng, 3, &choice), "mft_tild_autotune");
     for (auto* p : potentials) p->initializePotential();
     for (auto* i : integrators) i->finishInitialization();
+    for (auto* c : computes) c->initialize_compute();
     initialized = true;
 }
 
@@ -601,8 +608,93 @@ void PS_Box::doTimeStep(int step)
     for (auto* i : integrators) i->Integrate_1();
     check_mft(mft_tild_step(eng, 1), "mft_tild_step");
     for (auto* p : potentials) p->update_prefactor(step, maxSteps);
-    if (logFreq > 0 && step % logFreq == 0) writeData(step);
+    for (auto* c : computes) c->do_compute(step);
+    if (logFreq > 0 && step % logFreq == 0) {
+        writeData(step);
+        for (auto* c : computes) c->write_output();
+    }
     if (trajFreq > 0 && step % trajFreq == 0) writeLammpsTraj(step);
+}
+
+// PS_Box::writeKFieldFloat (PS_Box.cu:531-560): kx ky kz |v| |k| v per grid
+// point, k from Box::get_kD (float, PI2 = 6.2831853071, Nyquist negative)
+void PS_Box::writeKFieldFloat(const std::string& name, const float* dat)
+{
+    FILE* otp = std::fopen(name.c_str(), "w");
+    if (!otp) die("Failed to open output file in writeFieldFloat!");
+    const float PI2 = 6.2831853071f;
+    for (long i = 0; i < M; ++i) {
+        const int n[3] = {(int)(i % Nx[0]), (int)((i / Nx[0]) % Nx[1]), (int)(i / ((long)Nx[0] * Nx[1]))};
+        float kv[3], k2 = 0.0f;
+        for (int j = 0; j < 3; ++j) {
+            kv[j] = (2 * n[j] < Nx[j] ? PI2 * (float)n[j] : PI2 * (float)(n[j] - Nx[j])) / L[j];
+            k2 += kv[j] * kv[j];
+        }
+        for (int j = 0; j < 3; ++j) std::fprintf(otp, "%f ", kv[j]);
+        std::fprintf(otp, "%1.8e %1.8e %1.8e\n", std::fabs(dat[i]), std::sqrt(k2), dat[i]);
+    }
+    std::fclose(otp);
+}
+
+int PS_Compute::total_computes = 0;
+
+PS_Compute::PS_Compute(std::istringstream& iss, PS_Box* box) : mybox(box)
+{
+    compute_id = total_computes++;
+    output_name = "compute-" + std::to_string(compute_id) + ".dat";
+    iss >> group;
+    std::string s1;
+    while (iss >> s1) {
+        if (s1 == "wait") iss >> compute_wait;
+        else if (s1 == "freq") iss >> compute_freq;
+        else if (s1 == "file_name") iss >> output_name;
+        else die("PS_Compute: " + s1 + " invalid option for compute!");
+    }
+}
+
+PS_Compute* PSComputeFactory(std::istringstream& iss, PS_Box* box)
+{
+    std::string s1;
+    iss >> s1;
+    if (s1 == "avg_sk") return new PS_ComputeSK(iss, box);
+    die("Invalid compute option!");
+}
+
+PS_ComputeSK::PS_ComputeSK(std::istringstream& iss, PS_Box* box) : PS_Compute(iss, box)
+{
+    std::cout << "  Computing s(k) for group " << group << std::endl;
+}
+
+PS_ComputeSK::~PS_ComputeSK()
+{
+    if (d_sk) mft_free(d_sk);
+    if (d_work) mft_free(d_work);
+}
+
+void PS_ComputeSK::initialize_compute()
+{
+    mask = mybox->groupMask(group);
+    const size_t M = mybox->gridSize();
+    check_mft(mft_malloc((void**)&d_sk, sizeof(float) * M), "mft_malloc");
+    check_mft(mft_malloc(&d_work, 8 * M), "mft_malloc");
+    check_mft(mft_memset(d_sk, 0, sizeof(float) * M), "mft_memset");
+}
+
+void PS_ComputeSK::do_compute(int step)
+{
+    if (!active(step)) return;
+    check_mft(mft_tild_structure_factor(mybox->engine(), mask, d_sk, 1, d_work), "mft_tild_structure_factor");
+    num_data_pts += 1;
+}
+
+// the average over the accumulated frames (ps_compute_sk.cu:49-59)
+void PS_ComputeSK::write_output()
+{
+    const size_t M = mybox->gridSize();
+    std::vector<float> sk(M);
+    check_mft(mft_memcpy_d2h(sk.data(), d_sk, sizeof(float) * M), "mft_memcpy_d2h");
+    for (auto& v : sk) v *= 1.0 / float(num_data_pts);
+    mybox->writeKFieldFloat(output_name, sk.data());
 }
 
 // PS_Box::writeLammpsTraj (PS_Box.cu:756-801): positions wrapped once into
@@ -640,10 +732,16 @@ void PS_Box::NVT(int nSteps)
     // steps between log points run as one engine call
     int i = 0;
     while (i < maxSteps) {
-        // next log or trajectory step
+        // next log, trajectory or compute step
         int next = maxSteps;
         if (logFreq > 0) next = std::min(next, (i / logFreq + (i % logFreq ? 1 : 0)) * logFreq);
         if (trajFreq > 0) next = std::min(next, (i / trajFreq + (i % trajFreq ? 1 : 0)) * trajFreq);
+        for (auto* c : computes)
+            for (int s = i; s < next; ++s)
+                if (c->active(s)) {
+                    next = s;
+                    break;
+                }
         if (next > i) {
             check_mft(mft_tild_step(eng, next - i), "mft_tild_step");
             for (int s = i; s < next; ++s)

@@ -388,6 +388,14 @@ class TildEngine:
     def synchronize(self):
         check(lib().mft_tild_synchronize(self.h))
 
+    def structure_factor(self, species_mask):
+        """mft_tild_structure_factor: |FFT[rho_group]|^2 [M] at the last density pass."""
+        M = self.M
+        sk = DeviceBuffer(4 * M)
+        work = DeviceBuffer(8 * M)
+        check(lib().mft_tild_structure_factor(self.h, C.c_uint32(species_mask), sk.ptr, 0, work.ptr))
+        return sk.to_array(np.float32, M)
+
     def energies(self):
         pe = np.zeros(max(1, self.npot), np.float64)
         be = _d(0)

@@ -153,3 +153,16 @@ def test_modify_zero_mean_keyword(tmp_path):
     assert e["zero_mean"] == 1 and f["zero_mean"] == 0
     r = dry(tmp_path, text.replace("modify zeroMean", "modify sideways"))
     assert r.returncode == 1 and "Invalid modify option on Edwards potential" in r.stdout
+
+
+def test_compute_avg_sk_keyword(tmp_path):
+    # PS_Compute (ps_compute.cu:8-43): group, then wait / freq / file_name
+    text = open(os.path.join(INPUTS, "c1_diblock.in")).read().replace(
+        "logFreq", "compute avg_sk all freq 5 wait 10\ncompute avg_sk B file_name skB.dat\nlogFreq")
+    r = dry(tmp_path, text)
+    assert r.returncode == 0, r.stdout[-1000:]
+    c = json.loads(r.stdout.strip().splitlines()[-1])["computes"]
+    assert c[0] == {"style": "avg_sk", "mask": 3, "wait": 10, "freq": 5, "file": "compute-0.dat"}
+    assert c[1] == {"style": "avg_sk", "mask": 2, "wait": 0, "freq": 100, "file": "skB.dat"}
+    r = dry(tmp_path, text.replace("wait 10", "wiat 10"))
+    assert r.returncode == 1 and "PS_Compute: wiat invalid option for compute!" in r.stdout

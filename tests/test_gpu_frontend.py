@@ -198,3 +198,40 @@ def test_ps_trajectory_frames_match_engine(gpu, tmp_path):
         assert np.max(np.abs(d)) < 1e-5, step
         assert np.all((x >= 0) & (x <= L + 1e-6))
     eng.close()
+
+
+def read_kfield(path):
+    return np.loadtxt(path)
+
+
+def test_ps_compute_avg_sk_matches_engine(gpu, tmp_path):
+    # compute avg_sk (ps_compute.cu, ps_compute_sk.cu): |rho_B(k)|^2 after steps
+    # 2, 4, 6, 8, 10 (step > wait 1, freq 2), averaged and written at the log
+    # steps 5 and 10 in PS_Box::writeKFieldFloat's format
+    cfg = small_system(False, seed=9)
+    say("case: front end compute avg_sk")
+    write_data_file(tmp_path / "init.data", cfg)
+    text = ps_input(cfg, "init.data", steps=12, log_freq=5).replace(
+        "endBox", "compute avg_sk %s freq 2 wait 1 file_name sk.dat\nendBox" % cfg["species"][1])
+    (tmp_path / "input").write_text(text)
+    r = run_mft(["-in", "input"], cwd=tmp_path)
+    assert r.returncode == 0, r.stdout[-3000:]
+    got = read_kfield(tmp_path / "sk.dat")
+    eng = systems.tild_engine(cfg, "GJF")
+    eng.set_run(12)
+    acc, num = 0.0, 0
+    for s in range(11):
+        eng.step(1)
+        if s > 1 and s % 2 == 0:
+            acc = acc + eng.structure_factor(2).astype(np.float64)
+            num += 1
+    want = acc / num
+    M = int(np.prod(cfg["Nx"]))
+    assert got.shape == (M, 6)
+    assert close(got[:, 5], want, rel=1e-5, atol=1e-7)
+    assert close(got[:, 3], np.abs(want), rel=1e-5, atol=1e-7)
+    # k columns: Box::get_kD with the Nyquist index negative
+    nx = cfg["Nx"][0]
+    kx = np.where(2 * np.arange(nx) < nx, np.arange(nx), np.arange(nx) - nx) * 2 * np.pi / cfg["L"][0]
+    assert close(got[:nx, 0], kx, rel=1e-5, atol=1e-5)
+    eng.close()

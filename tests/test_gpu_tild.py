@@ -393,3 +393,24 @@ def test_gather_variants_match_oracle(gpu, case, variant):
         sys_o.noise_old = o["noise_old"].astype(np.float32)
         assert np.max(periodic_dist(eng.get_state()[0], o["x"], cfg["L"])) < 1e-6 * max(cfg["L"]), step
     eng.close()
+
+
+@pytest.mark.parametrize("mask", [1, 2, 3])
+def test_structure_factor_matches_oracle_fft(gpu, mask):
+    # PS_ComputeSK::do_compute (ps_compute_sk.cu:18-45): |FFT[rho_group]|^2
+    # with the 1/M forward scaling, against the oracle's FFT of the engine's
+    # own densities; S(0) = (N_group / V)^2
+    say("case: structure factor, mask", mask)
+    cfg = small_system(False)
+    eng, sys_o = _engine_vs_oracle(gpu, cfg)
+    eng.compute_forces()
+    rho, _ = eng.densities()
+    grp = sum(rho[s].astype(np.float64) for s in range(len(cfg["species"])) if (mask >> s) & 1)
+    want = np.abs(orc.box_fft(grp.astype(np.complex128), list(cfg["Nx"]), 1)) ** 2
+    got = eng.structure_factor(mask)
+    assert rel(got, want) < 1e-5
+    t = np.asarray(cfg["topo"]["type"])
+    n = sum(int((t == s).sum()) for s in range(len(cfg["species"])) if (mask >> s) & 1)
+    V = float(np.prod(np.asarray(cfg["L"], np.float64)))
+    assert abs(got[0] - (n / V) ** 2) < 1e-5 * (n / V) ** 2
+    eng.close()
