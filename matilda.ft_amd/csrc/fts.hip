@@ -2033,6 +2033,20 @@ int fts_pool_observables(FtsBatch* b, int rep, double* Q, double* H, double* par
     return batch_observables(b, rep, Q, H, parts);
 }
 int fts_pool_densities(FtsBatch* b, int rep, float* rho) { return batch_densities(b, rep, rho); }
+// whether every molecule's Q of the last density pass is finite (a runaway
+// CL trajectory overflows the propagators first); the stream must be idle
+int fts_pool_q_finite(FtsBatch* b, int rep, bool* ok)
+{
+    MFT_REQUIRE(rep >= 0 && rep < (int)b->reps.size() && b->reps[rep].alive, MFT_ERR_INVALID, "replica out of range");
+    const FtsRep& r = b->reps[rep];
+    double2 q[kFtsMaxMol];
+    MFT_HIP(hipMemcpy(q, r.Q, sizeof(double2) * r.d.nmolecules, hipMemcpyDeviceToHost));
+    *ok = true;
+    // Q = 0 (every propagator underflowed) makes -n log Q infinite as well
+    for (int m = 0; m < r.d.nmolecules; ++m)
+        *ok = *ok && std::isfinite(q[m].x) && std::isfinite(q[m].y) && (q[m].x != 0.0 || q[m].y != 0.0);
+    return MFT_OK;
+}
 int fts_pool_retire(FtsBatch* b, int rep) { return batch_retire(b, rep); }
 
 // device bytes one replica holds (rep_setup)

@@ -35,6 +35,7 @@ hipStream_t fts_pool_stream(FtsBatch* b);
 int fts_pool_observables(FtsBatch* b, int rep, double* Q, double* H, double* parts);
 int fts_pool_densities(FtsBatch* b, int rep, float* rho);
 int fts_pool_retire(FtsBatch* b, int rep);
+int fts_pool_q_finite(FtsBatch* b, int rep, bool* ok);
 size_t fts_bytes_estimate(const mft_fts_desc& d);
 
 struct ServeReq {
@@ -192,22 +193,33 @@ static int run_pool(Server* s, int max_iters)
     for (int64_t id : s->fts_running) {
         ServeReq& r = *s->reqs[id];
         if (r.remaining > 0) {
-            still.push_back(id);
-            continue;
-        }
-        const mft_fts_desc& d = r.fd;
-        r.Q.assign(2 * d.nmolecules, 0.0);
-        r.parts.assign(2 * (d.npotentials + d.nmolecules), 0.0);
-        rc = fts_pool_observables(s->pool, r.rep, r.Q.data(), r.H, r.parts.data());
-        if (!rc) {
-            r.rho.resize((size_t)2 * d.nspecies * d.nx * d.ny * d.nz);
-            rc = fts_pool_densities(s->pool, r.rep, r.rho.data());
-        }
-        if (rc) {
+            // a runaway solve ends as soon as its Q overflows, as the
+            // reference quits on a NaN Hamiltonian at its next log step
+            // (FTS_Box.cu:148-152), so it stops holding a pool slot
+            bool ok = true;
+            if ((rc = fts_pool_q_finite(s->pool, r.rep, &ok))) return rc;
+            if (ok) {
+                still.push_back(id);
+                continue;
+            }
             r.state = MFT_REQ_FAILED;
-            r.error = mft_last_error();
+            r.error = "Found Heff = NaN, quitting! (Q not finite after " +
+                      std::to_string(r.niter - r.remaining) + " iterations)";
         } else {
-            r.state = MFT_REQ_DONE;
+            const mft_fts_desc& d = r.fd;
+            r.Q.assign(2 * d.nmolecules, 0.0);
+            r.parts.assign(2 * (d.npotentials + d.nmolecules), 0.0);
+            rc = fts_pool_observables(s->pool, r.rep, r.Q.data(), r.H, r.parts.data());
+            if (!rc) {
+                r.rho.resize((size_t)2 * d.nspecies * d.nx * d.ny * d.nz);
+                rc = fts_pool_densities(s->pool, r.rep, r.rho.data());
+            }
+            if (rc) {
+                r.state = MFT_REQ_FAILED;
+                r.error = mft_last_error();
+            } else {
+                r.state = MFT_REQ_DONE;
+            }
         }
         if ((rc = fts_pool_retire(s->pool, r.rep))) return rc;
         r.rep = -1;

@@ -85,3 +85,36 @@ def test_admission_respects_budget(gpu):
     Qb = srv.fts_result(b)[0]
     assert np.array_equal(Qa, Qb)   # same request, same result, whenever it ran
     srv.close()
+
+
+def test_runaway_solve_fails_early_without_touching_the_others(gpu):
+    # a solve whose propagators overflow ends at the next pump with the
+    # reference's NaN message (FTS_Box.cu:148-152) instead of holding its slot
+    # for all its iterations; the solve beside it is unaffected
+    say("case: serve runaway solve")
+    spec, fields = fts_case((16, 16, 16), 10, 12.0, 7)
+    bad_fields = dict(fields)
+    M = 16 ** 3
+    bad_fields[(1, "mi")] = np.full(M, 1.0e4, np.complex64)   # W = -/+200: exp(-W) overflows at once
+    good, bad = fts_case((16, 16, 16), 10, 12.0, 8), (spec, bad_fields)
+    srv = mft.Server(hbm_budget_bytes=4e9, max_fts_replicas=8)
+    gid = srv.submit_fts(good[0], good[1], 60)
+    bid = srv.submit_fts(bad[0], bad[1], 3000)
+    pumps = 0
+    while srv.status(bid) not in ("done", "failed"):
+        srv.pump(10)
+        pumps += 1
+        assert pumps < 20
+    assert srv.status(bid) == "failed"
+    err = srv.error(bid)
+    assert "Found Heff = NaN" in err and "not finite after" in err
+    assert int(err.split("after ")[1].split()[0]) < 3000
+    for _ in range(50):
+        nr, nq = srv.pump(10)
+        if nr == 0 and nq == 0:
+            break
+    assert srv.status(gid) == "done"
+    Q, H, parts, rho = srv.fts_result(gid)
+    Qa, Ha, pa, rhoa = alone_fts(good[0], good[1], 60)
+    assert np.array_equal(Q, Qa) and H == Ha and np.array_equal(rho, rhoa)
+    srv.close()
