@@ -97,7 +97,13 @@ int mft_tild_bonds(float* f, const int* nBonds, const int* bondedTo, const int* 
                    const float* x, const float L[3], int ns, int maxbonds, void* stream);
 
 /* Potential kinds (ps_potential.cu:294-333). */
-enum { MFT_POT_GAUSSIAN = 0, MFT_POT_ERF2 = 1, MFT_POT_ERFG = 2, MFT_POT_CHARGES = 3 };
+enum { MFT_POT_GAUSSIAN = 0, MFT_POT_ERF2 = 1, MFT_POT_ERFG = 2, MFT_POT_CHARGES = 3, MFT_POT_BIAS = 4 };
+/* Phases of the static bias field (Box::make_bias_field, Box.cu:308-458):
+ * lamellar A sin(2 pi n x_dir / L_dir), hexagonal cylinders along dir
+ * A cos(.) cos(.), gyroid A ((sin x cos y + sin y cos z + sin z cos x)^2 - 1.44).
+ * The reference's BCC phase reads an uninitialised vector (Box.cu:359 passes
+ * the position as pbc_dr2's output), so it is rejected. */
+enum { MFT_BIAS_LAM = 0, MFT_BIAS_HEX = 1, MFT_BIAS_GYR = 2 };
 
 /* initializePotential of one potential (ps_potentialGaussian.cu:28-71,
  * ps_potentialErf2.cu:31-67, ps_potentialErfG.cu:30-66,
@@ -144,6 +150,14 @@ typedef struct {
     float Rp;          /* erf radius (ps_potentialErf2.cu:91-92) */
     int ramp;          /* PS_Potential::ramp_check_input (ps_potential.cu:253-283) */
     float A_final;
+    /* MFT_POT_BIAS (BiasField, ps_potentialBias.cu:17-200): a static external
+     * field A u(r) on the species of maskI (maskJ unused); its force -grad u
+     * is taken spectrally once, like the reference, and its energy is
+     * gvol sum rho_I A u.  No ramp (the reference's ramp scaling never reaches
+     * the precomputed bias forces). */
+    int phase;         /* MFT_BIAS_* */
+    int dir;           /* 0..2 */
+    int n_periods;     /* >= 1 */
 } mft_tild_potential;
 
 /* Everything PS_Box::readInput/finishInitialization (PS_Box_initialization.cu:

@@ -78,6 +78,11 @@ static int tild_setup(TildEngine* e)
         if (P.kind == MFT_POT_CHARGES) {
             MFT_REQUIRE(e->charge_pot < 0, MFT_ERR_UNSUPPORTED, "only one charges potential is supported");
             e->charge_pot = i;
+        } else if (P.kind == MFT_POT_BIAS) {
+            MFT_REQUIRE(P.phase >= MFT_BIAS_LAM && P.phase <= MFT_BIAS_GYR, MFT_ERR_UNSUPPORTED,
+                        "PHASE NOT FOUND in Box.cu");
+            MFT_REQUIRE(P.dir >= 0 && P.dir < 3 && P.n_periods >= 1 && P.maskI != 0, MFT_ERR_INVALID,
+                        "bias field needs a group, dir 0..2 and n_periods >= 1");
         } else {
             MFT_REQUIRE(P.kind >= 0 && P.kind <= MFT_POT_ERFG, MFT_ERR_UNSUPPORTED, "potential kind %d", P.kind);
         }
@@ -94,6 +99,15 @@ static int tild_setup(TildEngine* e)
     for (int i = 0; i < d.npotentials; ++i) {
         const mft_tild_potential& P = e->pots[i];
         KPot& K = kp.pot[i];
+        if (P.kind == MFT_POT_BIAS) {
+            // not a pair potential: a no-op entry in the k-space pass (zero
+            // amplitude, no species), the field is added to the forces after it
+            std::memset(&K, 0, sizeof(K));
+            K.kind = MFT_POT_GAUSSIAN;
+            e->pots[i].ramp = 0;
+            if ((rc = tild_bias_setup(e, i))) return rc;
+            continue;
+        }
         K.kind = P.kind;
         K.maskI = P.maskI;
         K.maskJ = P.maskJ;
@@ -209,7 +223,7 @@ static int configure_channels(TildEngine* e, const int* type, const float* charg
     uint32_t forced = 0;
     for (int i = 0; i < d.npotentials; ++i) {
         const mft_tild_potential& P = e->pots[i];
-        if (P.kind != MFT_POT_CHARGES) forced |= P.maskI | P.maskJ;
+        if (P.kind != MFT_POT_CHARGES) forced |= P.maskI | (P.kind == MFT_POT_BIAS ? 0u : P.maskJ);
     }
     if (e->charge_pot >= 0 && fold)
         for (int s = 0; s < d.nspecies; ++s)
@@ -318,6 +332,7 @@ static int grid_forces(TildEngine* e)
     {
         TimedScope ts(e, MFT_TK_FFT_X_INV);
         if ((rc = tild_x_inv_packed(e->Y, e->F4, d.nx, d.ny, d.nz, e->nout4, e->stream))) return rc;
+        if (!e->bias.empty() && (rc = tild_bias_add(e))) return rc;
     }
     {
         TimedScope ts(e, MFT_TK_GATHER);
@@ -346,6 +361,7 @@ static void build_groups(TildEngine* e)
     for (int i = 0; i < kp.npots; ++i) {
         const KPot& P = kp.pot[i];
         int g;
+        if (P.A == 0.f && P.maskI == 0u && P.maskJ == 0u) continue; // no-op (bias fields)
         if (P.kind == MFT_POT_CHARGES) {
             if (!kp.fold) continue; // separate charge field
             g = find(MFT_POT_CHARGES, P.sig2, nullptr);
@@ -465,6 +481,10 @@ int mft_tild_destroy(mft_tild* e)
         if (p) (void)hipFree(p);
     for (float* t : e->tables)
         if (t) (void)hipFree(t);
+    for (auto& b : e->bias) {
+        (void)hipFree(b.F);
+        (void)hipFree(b.u);
+    }
     for (hipEvent_t ev : e->evpool) (void)hipEventDestroy(ev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
@@ -653,6 +673,8 @@ int mft_tild_energies(mft_tild* e, double* pot_energy, double* bond_energy)
         if (rc) return rc;
         MFT_HIP(hipStreamSynchronize(e->stream));
         MFT_HIP(hipMemcpy(pot_energy, e->eout, sizeof(double) * d.npotentials, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < e->bias.size(); ++k)
+            if ((rc = tild_bias_energy(e, (int)k, &pot_energy[e->bias[k].pot]))) return rc;
     }
     if (bond_energy) {
         if ((rc = tild_bond_energy(e))) return rc;

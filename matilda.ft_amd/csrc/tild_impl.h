@@ -210,6 +210,15 @@ struct TildEngine {
     int* force_slot = nullptr; // [nspecies] -> force field slot or -1
     std::vector<int> force_slot_h;
     std::vector<float*> tables; // erf tables per potential (device)
+    // static bias fields (BiasField): potential index, species mask, the
+    // spectral force field and A u(r) on the grid (device)
+    struct Bias {
+        int pot;
+        uint32_t mask;
+        float4* F;
+        float* u;
+    };
+    std::vector<Bias> bias;
     double* epart = nullptr;
     double* eout = nullptr;   // [kMaxPots + 1]
     double* bpart = nullptr;
@@ -273,5 +282,8 @@ int tild_integrate(TildEngine* e, bool want_f);
 void erf_table(int kind, double sigma, double Rp, int nx, int ny, int nz, float lx, float ly, float lz,
                std::vector<float>& out);
 int tild_bond_energy(TildEngine* e);
+int tild_bias_setup(TildEngine* e, int pot);      // the field and its force (tild_bias.hip)
+int tild_bias_add(TildEngine* e);                 // F4 += bias forces of the biased species
+int tild_bias_energy(TildEngine* e, int k, double* energy);
 
 } // namespace mft

@@ -142,6 +142,13 @@ public:
     NBErfG(std::istringstream& iss, PS_Box* box); // ps_potentialErfG.cu
     mft_tild_potential descriptor() const override;
 };
+class BiasField : public PS_Potential {
+public:
+    BiasField(std::istringstream& iss, PS_Box* box); // ps_potentialBias.cu:17-66
+    mft_tild_potential descriptor() const override;
+    std::string phase;
+    int dir = 0, n_periods = 1;
+};
 class NBCharge : public PS_Potential {
 public:
     NBCharge(std::istringstream& iss, PS_Box* box); // ps_potentialCharges.cu:14-23

@@ -116,6 +116,40 @@ mft_tild_potential NBCharge::descriptor() const
     return d;
 }
 
+// BiasField (ps_potentialBias.cu:17-66): group Ao phase [dir d] [n_periods n];
+// Box::known_phase (Box.cu:283-303) names the phase, an unknown one stops at
+// set-up ("PHASE NOT FOUND in Box.cu")
+BiasField::BiasField(std::istringstream& iss, PS_Box* box) : PS_Potential(iss, box)
+{
+    std::string s1;
+    iss >> grpI >> Ao >> s1;
+    grpJ = grpI;
+    if (s1 == "L" || s1 == "LAM") phase = "L";
+    else if (s1 == "G" || s1 == "GYR") phase = "G";
+    else if (s1 == "BCC" || s1 == "SPH" || s1 == "S") phase = "S";
+    else if (s1 == "C" || s1 == "CYL" || s1 == "H" || s1 == "HEX") phase = "H";
+    std::string w;
+    while (iss >> w) {
+        if (w == "dir") iss >> dir;
+        else if (w == "n_periods") iss >> n_periods;
+        else die(w + " is not a valid initialize option in fts_potential");
+    }
+    if (phase.empty()) die("PHASE NOT FOUND in Box.cu");
+    // Box.cu:338-395 reads pbc_dr2's output argument before setting it
+    if (phase == "S") die("the BCC bias field of the reference reads an uninitialised vector (Box.cu:359)");
+}
+mft_tild_potential BiasField::descriptor() const
+{
+    mft_tild_potential d{};
+    d.kind = MFT_POT_BIAS;
+    d.maskI = mybox->groupMask(grpI);
+    d.A = Ao;
+    d.phase = phase == "L" ? MFT_BIAS_LAM : (phase == "H" ? MFT_BIAS_HEX : MFT_BIAS_GYR);
+    d.dir = dir;
+    d.n_periods = n_periods;
+    return d;
+}
+
 PS_Potential* PSPotentialFactory(std::istringstream& iss, PS_Box* box)
 {
     std::string s1;
@@ -124,7 +158,8 @@ PS_Potential* PSPotentialFactory(std::istringstream& iss, PS_Box* box)
     if (s1 == "erf2") return new NBErf2(iss, box);
     if (s1 == "erfG") return new NBErfG(iss, box);
     if (s1 == "gaussian") return new NBGauss(iss, box);
-    if (s1 == "bias" || s1 == "bias_lc" || s1 == "Langevin" || s1 == "langevin" || s1 == "dpd" ||
+    if (s1 == "bias") return new BiasField(iss, box);
+    if (s1 == "bias_lc" || s1 == "Langevin" || s1 == "langevin" || s1 == "dpd" ||
         s1 == "maiersaupe" || s1 == "wall")
         die("ps_potential: " + s1 + " is not part of this build's particle-mesh path");
     die("ps_potential.cu: " + s1 + " is not a supported potential");
@@ -540,9 +575,9 @@ void PS_Box::finishInitialization()
                     d.integrator, d.delt, (unsigned long long)d.seed, d.has_charges);
         for (size_t i = 0; i < pd.size(); ++i)
             std::printf("%s{\"kind\": %d, \"maskI\": %u, \"maskJ\": %u, \"A\": %.9g, \"sigma\": %.9g, \"Rp\": %.9g, "
-                        "\"ramp\": %d, \"A_final\": %.9g}",
+                        "\"ramp\": %d, \"A_final\": %.9g, \"phase\": %d, \"dir\": %d, \"n_periods\": %d}",
                         i ? ", " : "", pd[i].kind, pd[i].maskI, pd[i].maskJ, pd[i].A, pd[i].sigma, pd[i].Rp,
-                        pd[i].ramp, pd[i].A_final);
+                        pd[i].ramp, pd[i].A_final, pd[i].phase, pd[i].dir, pd[i].n_periods);
         std::printf("], \"bonds\": [");
         for (size_t i = 0; i < bondK.size(); ++i)
             std::printf("%s[%d, %.9g, %.9g]", i ? ", " : "", bondStyle[i], bondK[i], bondReq[i]);

@@ -252,12 +252,15 @@ def make_linear(nmolecs, blocks, species_index, block_bond_type=None, charges=No
 
 @dataclass
 class TildPotential:
-    kind: str            # gaussian | erf2 | erfG | charges
+    kind: str            # gaussian | erf2 | erfG | charges | bias
     I: str = "charges"   # group names (species names, "all", or custom groups)
     J: str = "charges"
-    A: float = 0.0       # amplitude (l_B for charges)
+    A: float = 0.0       # amplitude (l_B for charges, Ao for bias)
     sigma: float = 0.0
     Rp: float = 0.0
+    phase: str = "L"     # bias: L | H | G
+    dir: int = 0
+    n_periods: int = 1
 
 
 @dataclass
@@ -327,6 +330,12 @@ def tild_system_from_config(cfg, integrator="GJF"):
         if p["kind"] == "charges":
             pots.append(TildPotential("charges", A=p["A"], sigma=p["sigma"]))
             continue
+        if p["kind"] == "bias":
+            gI = [n for i, n in enumerate(names) if (p["maskI"] >> i) & 1]
+            assert len(gI) == 1
+            pots.append(TildPotential("bias", gI[0], gI[0], A=p["A"], phase=p.get("phase", "L"), dir=p.get("dir", 0),
+                                      n_periods=p.get("n_periods", 1)))
+            continue
         gI = [n for i, n in enumerate(names) if (p["maskI"] >> i) & 1]
         gJ = [n for i, n in enumerate(names) if (p["maskJ"] >> i) & 1]
         assert len(gI) == 1 and len(gJ) == 1
@@ -339,6 +348,49 @@ def tild_system_from_config(cfg, integrator="GJF"):
                       bond_k=cfg["bond_k"], bond_r0=cfg["bond_r0"], bond_style=cfg["bond_style"],
                       potentials=pots, mass=cfg["mass"], mobility=cfg["mobility"], delt=cfg["delt"],
                       integrator=integrator, xo=x.copy(), noise_old=np.zeros_like(x))
+
+
+def bias_field(pot, Nx, L):
+    """A u(r) of Box::make_bias_field (Box.cu:308-458) on the grid, x fastest,
+    r = n dx with the float dx of the box, PI = 3.141592654f (constants.h);
+    lamellar A sin(2 pi n x_dir / L_dir), cylinders along dir A cos cos,
+    gyroid A ((sin x cos y + sin y cos z + sin z cos x)^2 - 1.44)."""
+    f32 = np.float32
+    PI = f32(3.141592654)
+    n = f32(pot.n_periods)
+    Lf = [f32(l) for l in L]
+    dx = [f32(np.float64(Lf[j]) / Nx[j]) for j in range(3)]
+    iz, iy, ix = np.meshgrid(np.arange(Nx[2]), np.arange(Nx[1]), np.arange(Nx[0]), indexing="ij")
+    r = [(ix.astype(f32) * dx[0]).ravel(), (iy.astype(f32) * dx[1]).ravel(), (iz.astype(f32) * dx[2]).ravel()]
+    arg = [f32(f32(f32(2.0) * PI) * n) / Lf[j] for j in range(3)]
+    A = float(pot.A)
+    if pot.phase == "L":
+        return A * np.sin((arg[pot.dir] * r[pot.dir]).astype(np.float64))
+    if pot.phase == "H":
+        d1, d2 = 0, 1
+        if pot.dir == d1:
+            d1 = 2
+        elif pot.dir == d2:
+            d2 = 2
+        return A * np.cos((arg[d1] * r[d1]).astype(np.float64)) * np.cos((arg[d2] * r[d2]).astype(np.float64))
+    if pot.phase == "G":
+        c = [np.cos((arg[j] * r[j]).astype(np.float64)).astype(f32) for j in range(3)]
+        s = [np.sin((arg[j] * r[j]).astype(np.float64)).astype(f32) for j in range(3)]
+        e = (s[0] * c[1] + s[1] * c[2] + s[2] * c[0]).astype(f32)
+        return A * ((e * e).astype(np.float64) - 1.44)
+    raise ValueError("PHASE NOT FOUND in Box.cu")
+
+
+def bias_force(u, Nx, L):
+    """-grad u spectrally (ps_potentialBias.cu:100-185): f_j = IFFT[-i k_j FFT[u]]
+    with the float k of Box::get_kD, the Nyquist index kept; AoS [M*3]."""
+    M = int(np.prod(Nx))
+    uk = box_fft(np.asarray(u, np.complex128), list(Nx), 1)
+    k = np.array([kvec_f(i, Nx, L)[1] for i in range(M)], np.float64)
+    out = np.zeros((M, 3))
+    for j in range(3):
+        out[:, j] = box_fft(-1j * k[:, j] * uk, list(Nx), -1).real
+    return out.reshape(-1)
 
 
 def tild_forces(sys: TildSystem, bits=64, want_energy=False):
@@ -363,6 +415,13 @@ def tild_forces(sys: TildSystem, bits=64, want_energy=False):
     bonds(f, dict(sys.topo, bondReq=sys.bond_r0, bondK=sys.bond_k, bondStyle=sys.bond_style, maxbonds=2), sys.x, sys.L, bits)
     energies = []
     for pot in sys.potentials:
+        if pot.kind == "bias":
+            u = bias_field(pot, sys.Nx, sys.L)
+            gI = gridF.setdefault(pot.I, np.zeros(M * Dim, _rdt(bits)))
+            gI += bias_force(u, sys.Nx, sys.L).astype(_rdt(bits))
+            if want_energy:
+                energies.append(gvol_dev * float(np.sum(rho[pot.I].astype(np.float64) * u)))
+            continue
         uk = uk_table(pot.kind, pot.A, pot.sigma, pot.Rp, sys.Nx, sys.L, bits)
         if pot.kind == "charges":
             gf = gridF.setdefault("charges", np.zeros(M * Dim, _rdt(bits)))

@@ -166,3 +166,19 @@ def test_compute_avg_sk_keyword(tmp_path):
     assert c[1] == {"style": "avg_sk", "mask": 2, "wait": 0, "freq": 100, "file": "skB.dat"}
     r = dry(tmp_path, text.replace("wait 10", "wiat 10"))
     assert r.returncode == 1 and "PS_Compute: wiat invalid option for compute!" in r.stdout
+
+
+def test_bias_potential_keyword(tmp_path):
+    # ps_potentialBias.cu:17-66 and Box::known_phase (Box.cu:283-303)
+    base = open(os.path.join(INPUTS, "c1_diblock.in")).read()
+    text = base.replace("logFreq", "potential bias A 2.5 CYL dir 2 n_periods 2\nlogFreq")
+    r = dry(tmp_path, text)
+    assert r.returncode == 0, r.stdout[-1000:]
+    p = json.loads(r.stdout.strip().splitlines()[-1])["potentials"][-1]
+    assert (p["kind"], p["maskI"], p["phase"], p["dir"], p["n_periods"]) == (4, 1, 1, 2, 2)
+    assert p["A"] == pytest.approx(2.5)
+    for bad, msg in (("potential bias A 2.5 CUBE", "PHASE NOT FOUND in Box.cu"),
+                     ("potential bias A 2.5 L wobble 3", "wobble is not a valid initialize option"),
+                     ("potential bias A 2.5 BCC", "uninitialised")):
+        r = dry(tmp_path, base.replace("logFreq", bad + "\nlogFreq"))
+        assert r.returncode == 1 and msg in r.stdout, (bad, r.stdout[-300:])

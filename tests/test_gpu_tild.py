@@ -414,3 +414,21 @@ def test_structure_factor_matches_oracle_fft(gpu, mask):
     V = float(np.prod(np.asarray(cfg["L"], np.float64)))
     assert abs(got[0] - (n / V) ** 2) < 1e-5 * (n / V) ** 2
     eng.close()
+
+
+@pytest.mark.parametrize("phase, direction, with_pairs", [("L", 1, True), ("H", 2, True), ("G", 0, False)])
+def test_bias_field_matches_oracle(gpu, phase, direction, with_pairs):
+    # BiasField (ps_potentialBias.cu): static A u(r) on one species' particles,
+    # beside the pair potentials or alone; forces, energies and a step
+    cfg = small_system(False)
+    bias = dict(kind="bias", maskI=1, maskJ=0, A=2.5, phase=phase, dir=direction, n_periods=1)
+    cfg["potentials"] = (list(cfg["potentials"]) if with_pairs else []) + [bias]
+    say("case: bias", phase, direction, "with pairs" if with_pairs else "alone")
+    eng, sys_o = _engine_vs_oracle(gpu, cfg)
+    f_g = eng.compute_forces()
+    out = orc.tild_forces(sys_o, want_energy=True)
+    assert rel(f_g, out["f"]) < TOL
+    pe, _ = eng.energies()
+    for g, o in zip(pe, out["energies"]):
+        assert abs(g - o) <= TOL * max(abs(o), 1e-12), (g, o)
+    eng.close()

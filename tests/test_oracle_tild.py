@@ -216,3 +216,36 @@ def test_make_linear_topology_known_answer():
     mask = (np.arange(12) % 2) < np.repeat(t["nBonds"], 2)
     assert list(p["bondedTo"][mask]) == list(t["bondedTo"][mask])
     assert list(p["bondType"][mask]) == list(t["bondType"][mask])
+
+
+@pytest.mark.parametrize("phase, direction", [("L", 0), ("L", 2), ("H", 1), ("H", 2), ("G", 0)])
+def test_bias_force_is_the_analytic_gradient(phase, direction):
+    # the bias fields of Box::make_bias_field are trigonometric polynomials
+    # below the Nyquist frequency, so the spectral force of
+    # ps_potentialBias.cu:100-185 is their exact -grad (up to the float
+    # rounding of the sample positions and, for the gyroid, of its terms)
+    Nx, L = (16, 12, 20), (4.0, 3.0, 5.0)
+    pot = orc.TildPotential("bias", "A", "A", A=1.7, phase=phase, dir=direction, n_periods=1)
+    u = orc.bias_field(pot, Nx, L)
+    f = orc.bias_force(u, Nx, L).reshape(-1, 3)
+    iz, iy, ix = np.meshgrid(np.arange(Nx[2]), np.arange(Nx[1]), np.arange(Nx[0]), indexing="ij")
+    r = [ix.ravel() * L[0] / Nx[0], iy.ravel() * L[1] / Nx[1], iz.ravel() * L[2] / Nx[2]]
+    a = [2 * np.pi / l for l in L]
+    g = np.zeros((u.size, 3))
+    if phase == "L":
+        g[:, direction] = 1.7 * a[direction] * np.cos(a[direction] * r[direction])
+        assert np.allclose(u, 1.7 * np.sin(a[direction] * r[direction]), atol=1e-5)
+    elif phase == "H":
+        # Box.cu:405-411: dir 0 -> axes (2, 1), dir 1 -> (0, 2), dir 2 -> (0, 1)
+        d1, d2 = (2, 1) if direction == 0 else ((0, 2) if direction == 1 else (0, 1))
+        g[:, d1] = -1.7 * a[d1] * np.sin(a[d1] * r[d1]) * np.cos(a[d2] * r[d2])
+        g[:, d2] = -1.7 * a[d2] * np.cos(a[d1] * r[d1]) * np.sin(a[d2] * r[d2])
+    else:
+        s = [np.sin(a[j] * r[j]) for j in range(3)]
+        c = [np.cos(a[j] * r[j]) for j in range(3)]
+        e = s[0] * c[1] + s[1] * c[2] + s[2] * c[0]
+        g[:, 0] = 1.7 * 2 * e * a[0] * (c[0] * c[1] - s[2] * s[0])
+        g[:, 1] = 1.7 * 2 * e * a[1] * (c[1] * c[2] - s[0] * s[1])
+        g[:, 2] = 1.7 * 2 * e * a[2] * (c[2] * c[0] - s[1] * s[2])
+    # force = -grad u
+    assert np.max(np.abs(f + g)) < 2e-5 * max(1.0, np.abs(g).max())
