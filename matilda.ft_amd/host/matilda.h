@@ -24,6 +24,8 @@
 #include <complex>
 #include <cstdint>
 #include <fstream>
+#include <map>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -34,6 +36,35 @@
 // status codes other than MFT_OK are reported through it with mft_last_error().
 [[noreturn]] void die(const std::string& msg);
 void check_mft(int rc, const std::string& what);
+
+// GSD 2.0 file writer for the trajectory frames of PS_Box::writeGSDtraj
+// (gsd_writer.cpp): chunks of the current frame, then end_frame().
+class GsdWriter {
+public:
+    enum Type { UINT8 = 1, UINT32 = 3, UINT64 = 4, INT8 = 5, INT32 = 7, FLOAT = 9 };
+    GsdWriter(const std::string& path, const std::string& application, const std::string& schema,
+              uint32_t schema_version);
+    ~GsdWriter();
+    void chunk(const std::string& name, Type type, uint64_t N, uint32_t M, const void* data);
+    void end_frame();
+
+private:
+    struct Entry {
+        uint64_t frame, N;
+        int64_t location;
+        uint32_t M;
+        uint16_t id;
+        uint8_t type;
+    };
+    void flush();
+    std::string path_, application_, schema_;
+    uint32_t schema_version_;
+    int fd_ = -1;
+    uint64_t data_end_ = 0, frame_ = 0;
+    std::vector<std::string> names_;
+    std::map<std::string, uint16_t> ids_;
+    std::vector<Entry> entries_, pending_;
+};
 
 // Uniform deviates for the molecule makers and random field initialisation.
 // A 64-bit xorshift* stream seeded by randSeed (the reference uses its own
@@ -235,6 +266,11 @@ public:
     std::vector<PS_Compute*> computes;
     int trajFreq = 0;
     std::string trajFileName = "traj.lammpstrj";
+    void writeGSDtraj();                                // PS_Box_gsdUtils.cu:14-160
+    int gsdFreq = 0, logSpaceGSDflag = 0;
+    float logScaleGSD = 1.f;
+    std::string gsd_name = "traj.gsd";
+    std::unique_ptr<GsdWriter> gsd;
     int findSpeciesInteger(const std::string& s) const;
     int findGroupInteger(const std::string& s) const;
     uint32_t groupMask(const std::string& s) const;

@@ -2,6 +2,7 @@
 // keywords, molecule makers and log output over the mft_tild engine.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <cstdio>
 #include <iostream>
 
@@ -284,14 +285,25 @@ void PS_Box::readInput(std::ifstream& inp)
                 iss >> trajFreq;
             } else if (firstWord == "trajFileName") {
                 iss >> trajFileName;
-            } else if (firstWord == "fieldFreq" || firstWord == "field_freq" || firstWord == "gsdFreq" ||
-                       firstWord == "gsd_freq") {
+            } else if (firstWord == "gsdFreq" || firstWord == "gsd_freq") {
+                // PS_Box_initialization.cu:199-211
+                iss >> gsdFreq;
+                std::string tps;
+                if (iss >> tps) {
+                    if (tps != "logspace" && tps != "log_space" && tps != "logSpace")
+                        die("PS_Box_init::readInput: invalid option to gsdFreq");
+                    logSpaceGSDflag = 1;
+                    iss >> logScaleGSD;
+                }
+            } else if (firstWord == "gsdName" || firstWord == "gsd_name") {
+                iss >> gsd_name;
+            } else if (firstWord == "fieldFreq" || firstWord == "field_freq") {
                 int f;
                 iss >> f;
                 std::string rest;
                 while (iss >> rest) {}
                 if (f > 0) std::cout << "note: " << firstWord << " output is not written by this build" << std::endl;
-            } else if (firstWord == "gsdName" || firstWord == "gsd_name" || firstWord == "initDataFileName") {
+            } else if (firstWord == "initDataFileName") {
                 std::string s;
                 iss >> s;
             } else if (firstWord == "grid") {
@@ -648,7 +660,77 @@ void PS_Box::doTimeStep(int step)
         writeData(step);
         for (auto* c : computes) c->write_output();
     }
+    if (gsdFreq > 0 && step % gsdFreq == 0) {
+        writeGSDtraj();
+        // PS_Box.cu:181-188: log-spaced frames
+        if (step > 1 && logSpaceGSDflag == 1) gsdFreq = int(float(gsdFreq) * logScaleGSD);
+    }
     if (trajFreq > 0 && step % trajFreq == 0) writeLammpsTraj(step);
+}
+
+// PS_Box::writeGSDtraj (PS_Box_gsdUtils.cu:14-160): the first frame (at
+// totSteps 0) creates the file and carries the static data -- box, N, masses,
+// type ids (species + 1), molecule ids (+ 1), type names, bonds (i < j pairs,
+// GSDinit, PS_Box_gsdUtils.cu:586-605) and zero angles; every frame carries the
+// step and positions shifted by -L/2 (and charges when they are on)
+void PS_Box::writeGSDtraj()
+{
+    const int n = nstot;
+    if (totSteps == 0 || !gsd) {
+        gsd.reset(new GsdWriter(gsd_name, "gpu-tild", "hoomd", (1u << 16) | 4u));
+        const uint8_t dims = 3;
+        const uint64_t step = (uint64_t)totSteps;
+        gsd->chunk("configuration/step", GsdWriter::UINT64, 1, 1, &step);
+        gsd->chunk("configuration/dimensions", GsdWriter::UINT8, 1, 1, &dims);
+        const float box[6] = {L[0], L[1], L[2], 0.f, 0.f, 0.f};
+        gsd->chunk("configuration/box", GsdWriter::FLOAT, 6, 1, box);
+        const uint32_t N = (uint32_t)n;
+        gsd->chunk("particles/N", GsdWriter::UINT32, 1, 1, &N);
+        std::vector<float> masses(n);
+        std::vector<uint32_t> types(n), mols(n);
+        for (int i = 0; i < n; ++i) {
+            masses[i] = species[intSpecies[i]].mass;
+            types[i] = (uint32_t)intSpecies[i] + 1;
+            mols[i] = (uint32_t)mID[i] + 1;
+        }
+        gsd->chunk("particles/mass", GsdWriter::FLOAT, n, 1, masses.data());
+        gsd->chunk("particles/typeid", GsdWriter::UINT32, n, 1, types.data());
+        gsd->chunk("log/particles/moleculeid", GsdWriter::UINT32, n, 1, mols.data());
+        const int ntypes = (int)species.size() + 1, max_len = 10;
+        std::vector<char> names((size_t)ntypes * max_len, 0);
+        for (int i = 0; i < ntypes; ++i) {
+            const std::string t = "type" + std::to_string(i);
+            std::memcpy(&names[(size_t)i * max_len], t.c_str(), std::min<size_t>(t.size() + 1, max_len));
+        }
+        gsd->chunk("particles/types", GsdWriter::INT8, ntypes, max_len, names.data());
+        std::vector<uint32_t> btype, bpair;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < nBonds[i]; ++j) {
+                const int k = bondedTo[(size_t)i * MAXBONDS + j];
+                if (k > i) {
+                    bpair.push_back((uint32_t)i);
+                    bpair.push_back((uint32_t)k);
+                    btype.push_back((uint32_t)bondType[(size_t)i * MAXBONDS + j]);
+                }
+            }
+        const uint32_t nb = (uint32_t)btype.size(), na = 0;
+        gsd->chunk("bonds/N", GsdWriter::UINT32, 1, 1, &nb);
+        gsd->chunk("bonds/typeid", GsdWriter::UINT32, nb, 1, btype.data());
+        gsd->chunk("bonds/group", GsdWriter::UINT32, nb, 2, bpair.data());
+        gsd->chunk("angles/N", GsdWriter::UINT32, 1, 1, &na);
+        gsd->chunk("angles/typeid", GsdWriter::UINT32, 0, 1, nullptr);
+        gsd->chunk("angles/group", GsdWriter::UINT32, 0, 3, nullptr);
+    } else {
+        const uint64_t step = (uint64_t)totSteps;
+        gsd->chunk("configuration/step", GsdWriter::UINT64, 1, 1, &step);
+    }
+    std::vector<float> xc(3 * (size_t)n);
+    check_mft(mft_tild_get_state(eng, xc.data(), nullptr, nullptr), "mft_tild_get_state");
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < 3; ++j) xc[(size_t)i * 3 + j] -= 0.5f * L[j];
+    gsd->chunk("particles/position", GsdWriter::FLOAT, n, 3, xc.data());
+    if (doCharges) gsd->chunk("particles/charge", GsdWriter::FLOAT, n, 1, charges.data());
+    gsd->end_frame();
 }
 
 // PS_Box::writeKFieldFloat (PS_Box.cu:531-560): kx ky kz |v| |k| v per grid
@@ -771,6 +853,7 @@ void PS_Box::NVT(int nSteps)
         int next = maxSteps;
         if (logFreq > 0) next = std::min(next, (i / logFreq + (i % logFreq ? 1 : 0)) * logFreq);
         if (trajFreq > 0) next = std::min(next, (i / trajFreq + (i % trajFreq ? 1 : 0)) * trajFreq);
+        if (gsdFreq > 0) next = std::min(next, (i / gsdFreq + (i % gsdFreq ? 1 : 0)) * gsdFreq);
         for (auto* c : computes)
             for (int s = i; s < next; ++s)
                 if (c->active(s)) {

@@ -3,6 +3,9 @@ syntax input run through the `mft` executable gives the trajectory and the
 logged energies / Hamiltonian of the engine called directly on the same
 system (positions within 1e-5 of the box length, logged values within 1e-5
 relative; the log prints 6 significant digits)."""
+import ctypes as C
+import os
+
 import numpy as np
 import pytest
 
@@ -234,4 +237,81 @@ def test_ps_compute_avg_sk_matches_engine(gpu, tmp_path):
     nx = cfg["Nx"][0]
     kx = np.where(2 * np.arange(nx) < nx, np.arange(nx), np.arange(nx) - nx) * 2 * np.pi / cfg["L"][0]
     assert close(got[:nx, 0], kx, rel=1e-5, atol=1e-5)
+    eng.close()
+
+
+GSD_REF = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref",
+                       "libgsd_ref.so")
+
+
+class _GsdEntry(C.Structure):
+    _fields_ = [("frame", C.c_uint64), ("N", C.c_uint64), ("location", C.c_int64), ("M", C.c_uint32),
+                ("id", C.c_uint16), ("type", C.c_uint8), ("flags", C.c_uint8)]
+
+
+def read_gsd(path):
+    """Every chunk of every frame, read with the reference's own GSD library
+    (gsd.c compiled from its sources into oracle/_ref)."""
+    lib = C.CDLL(GSD_REF)
+    lib.gsd_find_chunk.restype = C.POINTER(_GsdEntry)
+    lib.gsd_get_nframes.restype = C.c_uint64
+    h = C.create_string_buffer(1 << 16)      # room for struct gsd_handle
+    assert lib.gsd_open(h, str(path).encode(), 2) == 0   # GSD_OPEN_READONLY
+    dtypes = {1: np.uint8, 3: np.uint32, 4: np.uint64, 5: np.int8, 9: np.float32}
+    names = ["configuration/step", "configuration/dimensions", "configuration/box", "particles/N",
+             "particles/mass", "particles/typeid", "log/particles/moleculeid", "particles/types", "bonds/N",
+             "bonds/typeid", "bonds/group", "angles/N", "particles/position", "particles/charge"]
+    frames = []
+    for f in range(lib.gsd_get_nframes(h)):
+        fr = {}
+        for name in names:
+            e = lib.gsd_find_chunk(h, C.c_uint64(f), name.encode())
+            if not e:
+                continue
+            ent = e.contents
+            out = np.zeros(max(1, ent.N * ent.M), dtypes[ent.type])
+            assert lib.gsd_read_chunk(h, out.ctypes.data_as(C.c_void_p), e) == 0
+            fr[name] = out[: ent.N * ent.M].reshape(ent.N, ent.M) if ent.M > 1 else out[: ent.N]
+        frames.append(fr)
+    lib.gsd_close(h)
+    return frames
+
+
+def test_ps_gsd_frames_read_by_the_reference_library(gpu, tmp_path):
+    # gsdFreq (PS_Box_gsdUtils.cu:14-160): frames after steps 0, 5, 10 that the
+    # reference's gsd.c opens and reads back
+    if not os.path.exists(GSD_REF):
+        pytest.skip("oracle/_ref/libgsd_ref.so not built (reference sources absent where the oracle was built)")
+    cfg = small_system(True, seed=10)
+    say("case: front end gsd")
+    write_data_file(tmp_path / "init.data", cfg)
+    text = ps_input(cfg, "init.data", steps=12, log_freq=100).replace("endBox", "gsdFreq 5\nendBox")
+    (tmp_path / "input").write_text(text)
+    r = run_mft(["-in", "input"], cwd=tmp_path)
+    assert r.returncode == 0, r.stdout[-3000:]
+    frames = read_gsd(tmp_path / "traj.gsd")
+    assert len(frames) == 3
+    topo = cfg["topo"]
+    n = topo["type"].size
+    L = np.asarray(cfg["L"], np.float32)
+    f0 = frames[0]
+    assert f0["configuration/dimensions"][0] == 3 and f0["particles/N"][0] == n
+    assert np.array_equal(f0["configuration/box"], np.r_[L, 0, 0, 0].astype(np.float32))
+    assert np.array_equal(f0["particles/typeid"], topo["type"] + 1)
+    nb, mb = topo["nBonds"], topo["bondedTo"].reshape(n, -1)
+    pairs = sorted((i, int(mb[i, j])) for i in range(n) for j in range(nb[i]) if mb[i, j] > i)
+    assert f0["bonds/N"][0] == len(pairs)
+    assert [tuple(p) for p in f0["bonds/group"].tolist()] == pairs
+    assert f0["angles/N"][0] == 0
+    assert "bonds/N" not in frames[1]
+    eng = systems.tild_engine(cfg, "GJF")
+    eng.set_run(12)
+    done = 0
+    for k, fr in enumerate(frames):
+        assert fr["configuration/step"][0] == 5 * k
+        eng.step(5 * k + 1 - done)
+        done = 5 * k + 1
+        xe, _, _ = eng.get_state()
+        assert np.array_equal(fr["particles/position"], xe.reshape(-1, 3).astype(np.float32) - 0.5 * L)
+        assert np.array_equal(fr["particles/charge"], topo["charge"].astype(np.float32))
     eng.close()
