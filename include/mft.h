@@ -97,7 +97,8 @@ int mft_tild_bonds(float* f, const int* nBonds, const int* bondedTo, const int* 
                    const float* x, const float L[3], int ns, int maxbonds, void* stream);
 
 /* Potential kinds (ps_potential.cu:294-333). */
-enum { MFT_POT_GAUSSIAN = 0, MFT_POT_ERF2 = 1, MFT_POT_ERFG = 2, MFT_POT_CHARGES = 3, MFT_POT_BIAS = 4 };
+enum { MFT_POT_GAUSSIAN = 0, MFT_POT_ERF2 = 1, MFT_POT_ERFG = 2, MFT_POT_CHARGES = 3, MFT_POT_BIAS = 4,
+       MFT_POT_WALL = 5 };
 /* Phases of the static bias field (Box::make_bias_field, Box.cu:308-458):
  * lamellar A sin(2 pi n x_dir / L_dir), hexagonal cylinders along dir
  * A cos(.) cos(.), gyroid A ((sin x cos y + sin y cos z + sin z cos x)^2 - 1.44).
@@ -158,6 +159,11 @@ typedef struct {
     int phase;         /* MFT_BIAS_* */
     int dir;           /* 0..2 */
     int n_periods;     /* >= 1 */
+    /* MFT_POT_WALL (Wall, ps_potentialWall.cu:10-147): a harmonic wall on
+     * axis dir for the species of maskI: f_dir += A (wall_pos - x_dir) and
+     * energy A/2 (wall_pos - x_dir)^2 where wall_sign (wall_pos - x_dir) > 0. */
+    float wall_pos;
+    int wall_sign;     /* +1 or -1 */
 } mft_tild_potential;
 
 /* Everything PS_Box::readInput/finishInitialization (PS_Box_initialization.cu:

@@ -78,6 +78,17 @@ static int tild_setup(TildEngine* e)
         if (P.kind == MFT_POT_CHARGES) {
             MFT_REQUIRE(e->charge_pot < 0, MFT_ERR_UNSUPPORTED, "only one charges potential is supported");
             e->charge_pot = i;
+        } else if (P.kind == MFT_POT_WALL) {
+            MFT_REQUIRE(P.dir >= 0 && P.dir < 3 && (P.wall_sign == 1 || P.wall_sign == -1) && P.maskI != 0,
+                        MFT_ERR_INVALID, "wall needs a group, an axis 0..2 and dirSign +1 or -1");
+            MFT_REQUIRE(e->walls.n < kMaxWalls, MFT_ERR_UNSUPPORTED, "at most %d walls", kMaxWalls);
+            const int w = e->walls.n++;
+            e->walls.mask[w] = P.maskI;
+            e->walls.axis[w] = P.dir;
+            e->walls.sign[w] = P.wall_sign;
+            e->walls.pos[w] = P.wall_pos;
+            e->walls.k[w] = P.A;
+            e->wall_pot.push_back(i);
         } else if (P.kind == MFT_POT_BIAS) {
             MFT_REQUIRE(P.phase >= MFT_BIAS_LAM && P.phase <= MFT_BIAS_GYR, MFT_ERR_UNSUPPORTED,
                         "PHASE NOT FOUND in Box.cu");
@@ -99,13 +110,14 @@ static int tild_setup(TildEngine* e)
     for (int i = 0; i < d.npotentials; ++i) {
         const mft_tild_potential& P = e->pots[i];
         KPot& K = kp.pot[i];
-        if (P.kind == MFT_POT_BIAS) {
+        if (P.kind == MFT_POT_BIAS || P.kind == MFT_POT_WALL) {
             // not a pair potential: a no-op entry in the k-space pass (zero
-            // amplitude, no species), the field is added to the forces after it
+            // amplitude, no species); a bias field is added to the force
+            // fields after it, a wall to the particle forces in the integrate pass
             std::memset(&K, 0, sizeof(K));
             K.kind = MFT_POT_GAUSSIAN;
             e->pots[i].ramp = 0;
-            if ((rc = tild_bias_setup(e, i))) return rc;
+            if (P.kind == MFT_POT_BIAS && (rc = tild_bias_setup(e, i))) return rc;
             continue;
         }
         K.kind = P.kind;
@@ -223,6 +235,7 @@ static int configure_channels(TildEngine* e, const int* type, const float* charg
     uint32_t forced = 0;
     for (int i = 0; i < d.npotentials; ++i) {
         const mft_tild_potential& P = e->pots[i];
+        if (P.kind == MFT_POT_WALL) continue; // per-particle force
         if (P.kind != MFT_POT_CHARGES) forced |= P.maskI | (P.kind == MFT_POT_BIAS ? 0u : P.maskJ);
     }
     if (e->charge_pot >= 0 && fold)
@@ -675,6 +688,8 @@ int mft_tild_energies(mft_tild* e, double* pot_energy, double* bond_energy)
         MFT_HIP(hipMemcpy(pot_energy, e->eout, sizeof(double) * d.npotentials, hipMemcpyDeviceToHost));
         for (size_t k = 0; k < e->bias.size(); ++k)
             if ((rc = tild_bias_energy(e, (int)k, &pot_energy[e->bias[k].pot]))) return rc;
+        for (size_t w = 0; w < e->wall_pot.size(); ++w)
+            if ((rc = tild_wall_energy(e, (int)w, &pot_energy[e->wall_pot[w]]))) return rc;
     }
     if (bond_energy) {
         if ((rc = tild_bond_energy(e))) return rc;

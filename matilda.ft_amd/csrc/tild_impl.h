@@ -81,6 +81,16 @@ struct BondParams {
     int style[kMaxBondTypes];
 };
 
+// harmonic walls (Wall, ps_potentialWall.cu): per-particle forces added in
+// the integrate pass
+constexpr int kMaxWalls = 4;
+struct WallParams {
+    int n;
+    uint32_t mask[kMaxWalls];
+    int axis[kMaxWalls], sign[kMaxWalls];
+    float pos[kMaxWalls], k[kMaxWalls];
+};
+
 struct GjfParams {
     float a[kMaxSpecies], b[kMaxSpecies], nmag[kMaxSpecies], cf[kMaxSpecies], cn[kMaxSpecies];
 };
@@ -219,6 +229,8 @@ struct TildEngine {
         float* u;
     };
     std::vector<Bias> bias;
+    WallParams walls{};
+    std::vector<int> wall_pot;  // potential index of each wall
     double* epart = nullptr;
     double* eout = nullptr;   // [kMaxPots + 1]
     double* bpart = nullptr;
@@ -285,5 +297,6 @@ int tild_bond_energy(TildEngine* e);
 int tild_bias_setup(TildEngine* e, int pot);      // the field and its force (tild_bias.hip)
 int tild_bias_add(TildEngine* e);                 // F4 += bias forces of the biased species
 int tild_bias_energy(TildEngine* e, int k, double* energy);
+int tild_wall_energy(TildEngine* e, int w, double* energy); // tild_particles.hip
 
 } // namespace mft

@@ -769,7 +769,19 @@ struct StepArgs {
     uint32_t step;
     float dx[3];
     int p, nx, ny, nz;
+    WallParams walls;
 };
+
+// Wall forces of ps_potentialWall.cu:92-108 on a particle of species ty
+__device__ __forceinline__ void wall_forces(const WallParams& W, int ty, float4 xc, float* fo)
+{
+    for (int w = 0; w < W.n; ++w) {
+        if (!((W.mask[w] >> ty) & 1u)) continue;
+        const int a = W.axis[w];
+        const float disp = W.pos[w] - (a == 0 ? xc.x : (a == 1 ? xc.y : xc.z));
+        if ((float)W.sign[w] * disp > 0.0f) fo[a] += W.k[w] * disp;
+    }
+}
 
 // GJF (ps_integratorGJF.cu:102-163) or EM (ps_integratorEM.cu:39-82) update
 // of slot s from its total force fo: writes the next position and noise and
@@ -829,6 +841,7 @@ __global__ void __launch_bounds__(256) k_integrate(
         const float4 fg = fgrid[s];
         float fo[3] = {fg.x, fg.y, fg.z};
         bond_force_slots(s, ns, X, NB, BS, BT, maxbonds, bp, A.L[0], A.L[1], A.L[2], xc.x, xc.y, xc.z, fo);
+        if (A.walls.n) wall_forces(A.walls, META[s].y, xc, fo);
         if (ftot) ftot[s] = make_float4(fo[0], fo[1], fo[2], 0.f);
         if (A.integ != MFT_INT_NONE) k = integrate_slot(s, xc, META[s], fo, A, tg);
     }
@@ -1131,6 +1144,7 @@ static StepArgs step_args(const TildEngine* e)
     A.nx = d.nx;
     A.ny = d.ny;
     A.nz = d.nz;
+    A.walls = e->walls;
     return A;
 }
 
@@ -1142,6 +1156,44 @@ int tild_integrate(TildEngine* e, bool want_f)
                                                        want_f ? e->ftot : nullptr, std::max(1, d.maxbonds), e->bp,
                                                        step_args(e), e->N, e->count, e->key, e->rank, e->tg);
     MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
+// d_wallHarmonicEnergy (ps_potentialWall.cu:129-147), FP64 block sums
+__global__ void k_wall_energy(double* __restrict__ partial, const float4* __restrict__ X,
+                              const int2* __restrict__ META, WallParams W, int w, int ns)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (s < ns && ((W.mask[w] >> META[s].y) & 1u)) {
+        const float4 xc = X[s];
+        const int a = W.axis[w];
+        const float disp = W.pos[w] - (a == 0 ? xc.x : (a == 1 ? xc.y : xc.z));
+        if ((float)W.sign[w] * disp > 0.0f) v = 0.5f * W.k[w] * disp * disp;
+    }
+    v = wave_sum(v);
+    __shared__ double red[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < kBlock / 64; ++k) t += red[k];
+        partial[blockIdx.x] = t;
+    }
+}
+
+int tild_wall_energy(TildEngine* e, int w, double* energy)
+{
+    const int c = e->cur;
+    const unsigned nb = nblk(e->N);
+    k_wall_energy<<<nb, kBlock, 0, e->stream>>>(e->bpart, e->X[c], e->META[c], e->walls, w, e->N);
+    MFT_LAUNCH_CHECK();
+    std::vector<double> h(nb);
+    MFT_HIP(hipMemcpyAsync(h.data(), e->bpart, sizeof(double) * nb, hipMemcpyDeviceToHost, e->stream));
+    MFT_HIP(hipStreamSynchronize(e->stream));
+    double t = 0.0;
+    for (double v : h) t += v;
+    *energy = t;
     return MFT_OK;
 }
 

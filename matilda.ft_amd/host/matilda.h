@@ -173,6 +173,13 @@ public:
     NBErfG(std::istringstream& iss, PS_Box* box); // ps_potentialErfG.cu
     mft_tild_potential descriptor() const override;
 };
+class Wall : public PS_Potential {
+public:
+    Wall(std::istringstream& iss, PS_Box* box); // ps_potentialWall.cu:10-48
+    mft_tild_potential descriptor() const override;
+    int normalDim = 0, dirSign = 1;
+    float wallPos = 0.f, k = 0.f;
+};
 class BiasField : public PS_Potential {
 public:
     BiasField(std::istringstream& iss, PS_Box* box); // ps_potentialBias.cu:17-66

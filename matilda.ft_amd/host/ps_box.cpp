@@ -151,6 +151,38 @@ mft_tild_potential BiasField::descriptor() const
     return d;
 }
 
+// Wall (ps_potentialWall.cu:10-48): group axis wallPos dirSign k; no ramp
+Wall::Wall(std::istringstream& iss, PS_Box* box) : PS_Potential(iss, box)
+{
+    std::string axis;
+    iss >> grpI >> axis;
+    grpJ = grpI;
+    if (axis == "x" || axis == "X") normalDim = 0;
+    else if (axis == "y" || axis == "Y") normalDim = 1;
+    else if (axis == "z" || axis == "Z") normalDim = 2;
+    else die("ps_potentialWall.cu: axis must be x, y, or z (got '" + axis + "')");
+    iss >> wallPos >> dirSign;
+    if (dirSign != 1 && dirSign != -1)
+        die("ps_potentialWall.cu: dirSign must be +1 or -1 (got " + std::to_string(dirSign) + ")");
+    iss >> k;
+    std::string s1;
+    if (iss >> s1) {
+        if (s1 == "ramp") die("ramping applied on an unsupported potential!");
+        die("Invalid option " + s1 + " on wall potential");
+    }
+}
+mft_tild_potential Wall::descriptor() const
+{
+    mft_tild_potential d{};
+    d.kind = MFT_POT_WALL;
+    d.maskI = mybox->groupMask(grpI);
+    d.A = k;
+    d.dir = normalDim;
+    d.wall_pos = wallPos;
+    d.wall_sign = dirSign;
+    return d;
+}
+
 PS_Potential* PSPotentialFactory(std::istringstream& iss, PS_Box* box)
 {
     std::string s1;
@@ -160,8 +192,9 @@ PS_Potential* PSPotentialFactory(std::istringstream& iss, PS_Box* box)
     if (s1 == "erfG") return new NBErfG(iss, box);
     if (s1 == "gaussian") return new NBGauss(iss, box);
     if (s1 == "bias") return new BiasField(iss, box);
+    if (s1 == "wall") return new Wall(iss, box);
     if (s1 == "bias_lc" || s1 == "Langevin" || s1 == "langevin" || s1 == "dpd" ||
-        s1 == "maiersaupe" || s1 == "wall")
+        s1 == "maiersaupe")
         die("ps_potential: " + s1 + " is not part of this build's particle-mesh path");
     die("ps_potential.cu: " + s1 + " is not a supported potential");
 }
@@ -587,9 +620,11 @@ void PS_Box::finishInitialization()
                     d.integrator, d.delt, (unsigned long long)d.seed, d.has_charges);
         for (size_t i = 0; i < pd.size(); ++i)
             std::printf("%s{\"kind\": %d, \"maskI\": %u, \"maskJ\": %u, \"A\": %.9g, \"sigma\": %.9g, \"Rp\": %.9g, "
-                        "\"ramp\": %d, \"A_final\": %.9g, \"phase\": %d, \"dir\": %d, \"n_periods\": %d}",
+                        "\"ramp\": %d, \"A_final\": %.9g, \"phase\": %d, \"dir\": %d, \"n_periods\": %d, "
+                        "\"wall_pos\": %.9g, \"wall_sign\": %d}",
                         i ? ", " : "", pd[i].kind, pd[i].maskI, pd[i].maskJ, pd[i].A, pd[i].sigma, pd[i].Rp,
-                        pd[i].ramp, pd[i].A_final, pd[i].phase, pd[i].dir, pd[i].n_periods);
+                        pd[i].ramp, pd[i].A_final, pd[i].phase, pd[i].dir, pd[i].n_periods, pd[i].wall_pos,
+                        pd[i].wall_sign);
         std::printf("], \"bonds\": [");
         for (size_t i = 0; i < bondK.size(); ++i)
             std::printf("%s[%d, %.9g, %.9g]", i ? ", " : "", bondStyle[i], bondK[i], bondReq[i]);

@@ -24,7 +24,7 @@ _d = C.c_double
 _f = C.c_float
 
 MFT_OK, MFT_ERR_INVALID, MFT_ERR_OOM, MFT_ERR_NONFINITE, MFT_ERR_DEVICE, MFT_ERR_UNSUPPORTED = range(6)
-POT = {"gaussian": 0, "erf2": 1, "erfG": 2, "charges": 3, "bias": 4}
+POT = {"gaussian": 0, "erf2": 1, "erfG": 2, "charges": 3, "bias": 4, "wall": 5}
 BIAS_PHASE = {"L": 0, "H": 1, "G": 2}
 INTEGRATOR = {"GJF": 0, "EM": 1, "none": 2}
 TK_NAMES = ["bin", "scan", "scatter", "density", "fft_x_fwd", "fft_y_fwd", "conv_z", "fft_y_inv",
@@ -158,7 +158,8 @@ def fft_supported(n):
 # -------------------------------------------------------------- TILD -----
 class TildPotentialC(C.Structure):
     _fields_ = [("kind", _i), ("maskI", C.c_uint32), ("maskJ", C.c_uint32), ("A", _f), ("sigma", _f),
-                ("Rp", _f), ("ramp", _i), ("A_final", _f), ("phase", _i), ("dir", _i), ("n_periods", _i)]
+                ("Rp", _f), ("ramp", _i), ("A_final", _f), ("phase", _i), ("dir", _i), ("n_periods", _i),
+                ("wall_pos", _f), ("wall_sign", _i)]
 
 
 class TildDescC(C.Structure):
@@ -303,7 +304,7 @@ def tild_desc(Nx, L, pmeorder, nspecies, nparticles, potentials, bond_k=(), bond
         pots[i] = TildPotentialC(POT[p["kind"]], p.get("maskI", 0), p.get("maskJ", 0), p["A"],
                                  p.get("sigma", 0.0), p.get("Rp", 0.0), int(p.get("ramp", False)),
                                  p.get("A_final", 0.0), BIAS_PHASE.get(p.get("phase", "L"), -1), p.get("dir", 0),
-                                 p.get("n_periods", 1))
+                                 p.get("n_periods", 1), p.get("wall_pos", 0.0), p.get("wall_sign", 1))
     bk = np.asarray(bond_k, np.float32)
     br = np.asarray(bond_r0, np.float32)
     bs = np.asarray(bond_style, np.int32)

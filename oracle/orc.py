@@ -259,8 +259,10 @@ class TildPotential:
     sigma: float = 0.0
     Rp: float = 0.0
     phase: str = "L"     # bias: L | H | G
-    dir: int = 0
+    dir: int = 0         # bias direction, wall normal axis
     n_periods: int = 1
+    wall_pos: float = 0.0
+    wall_sign: int = 1
 
 
 @dataclass
@@ -329,6 +331,12 @@ def tild_system_from_config(cfg, integrator="GJF"):
     for p in cfg["potentials"]:
         if p["kind"] == "charges":
             pots.append(TildPotential("charges", A=p["A"], sigma=p["sigma"]))
+            continue
+        if p["kind"] == "wall":
+            gI = [n for i, n in enumerate(names) if (p["maskI"] >> i) & 1]
+            assert len(gI) == 1
+            pots.append(TildPotential("wall", gI[0], gI[0], A=p["A"], dir=p.get("dir", 0), wall_pos=p["wall_pos"],
+                                      wall_sign=p.get("wall_sign", 1)))
             continue
         if p["kind"] == "bias":
             gI = [n for i, n in enumerate(names) if (p["maskI"] >> i) & 1]
@@ -415,6 +423,17 @@ def tild_forces(sys: TildSystem, bits=64, want_energy=False):
     bonds(f, dict(sys.topo, bondReq=sys.bond_r0, bondK=sys.bond_k, bondStyle=sys.bond_style, maxbonds=2), sys.x, sys.L, bits)
     energies = []
     for pot in sys.potentials:
+        if pot.kind == "wall":
+            # Wall (ps_potentialWall.cu:92-147): k (pos - x_a) where sign (pos - x_a) > 0
+            sites = np.asarray(sys.group_sites(pot.I))
+            xa = sys.x.reshape(-1, Dim)[sites, pot.dir].astype(np.float64)
+            disp = np.float64(np.float32(pot.wall_pos)) - xa
+            on = pot.wall_sign * disp > 0
+            fr = f.reshape(-1, Dim)
+            fr[sites[on], pot.dir] += float(np.float32(pot.A)) * disp[on]
+            if want_energy:
+                energies.append(float(np.sum(0.5 * float(np.float32(pot.A)) * disp[on] ** 2)))
+            continue
         if pot.kind == "bias":
             u = bias_field(pot, sys.Nx, sys.L)
             gI = gridF.setdefault(pot.I, np.zeros(M * Dim, _rdt(bits)))

@@ -182,3 +182,18 @@ def test_bias_potential_keyword(tmp_path):
                      ("potential bias A 2.5 BCC", "uninitialised")):
         r = dry(tmp_path, base.replace("logFreq", bad + "\nlogFreq"))
         assert r.returncode == 1 and msg in r.stdout, (bad, r.stdout[-300:])
+
+
+def test_wall_potential_keyword(tmp_path):
+    # ps_potentialWall.cu:10-48: group axis wallPos dirSign k, with its die texts
+    base = open(os.path.join(INPUTS, "c1_diblock.in")).read()
+    r = dry(tmp_path, base.replace("logFreq", "potential wall B y 7.5 -1 4.0\nlogFreq"))
+    assert r.returncode == 0, r.stdout[-1000:]
+    p = json.loads(r.stdout.strip().splitlines()[-1])["potentials"][-1]
+    assert (p["kind"], p["maskI"], p["dir"], p["wall_sign"]) == (5, 2, 1, -1)
+    assert p["wall_pos"] == pytest.approx(7.5) and p["A"] == pytest.approx(4.0)
+    for bad, msg in (("potential wall B w 7.5 1 4.0", "axis must be x, y, or z (got 'w')"),
+                     ("potential wall B y 7.5 2 4.0", "dirSign must be +1 or -1 (got 2)"),
+                     ("potential wall B y 7.5 1 4.0 ramp 5", "ramping applied on an unsupported potential!")):
+        r = dry(tmp_path, base.replace("logFreq", bad + "\nlogFreq"))
+        assert r.returncode == 1 and msg in r.stdout, (bad, r.stdout[-300:])

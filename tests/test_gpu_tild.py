@@ -432,3 +432,38 @@ def test_bias_field_matches_oracle(gpu, phase, direction, with_pairs):
     for g, o in zip(pe, out["energies"]):
         assert abs(g - o) <= TOL * max(abs(o), 1e-12), (g, o)
     eng.close()
+
+
+def test_walls_match_oracle(gpu):
+    # Wall (ps_potentialWall.cu): harmonic walls on one species along z at the
+    # mid-plane (pushing up) and on the other along x (pushing down), beside
+    # the pair potentials; forces, energies and three GJF steps
+    cfg = small_system(False, seed=12)
+    L = cfg["L"]
+    cfg["potentials"] = list(cfg["potentials"]) + [
+        dict(kind="wall", maskI=1, maskJ=0, A=3.0, dir=2, wall_pos=0.5 * L[2], wall_sign=1),
+        dict(kind="wall", maskI=2, maskJ=0, A=1.5, dir=0, wall_pos=0.3 * L[0], wall_sign=-1)]
+    say("case: walls")
+    eng, sys_o = _engine_vs_oracle(gpu, cfg)
+    f_g = eng.compute_forces()
+    out = orc.tild_forces(sys_o, want_energy=True)
+    assert rel(f_g, out["f"]) < TOL
+    pe, _ = eng.energies()
+    for g, o in zip(pe, out["energies"]):
+        assert abs(g - o) <= TOL * max(abs(o), 1e-12), (g, o)
+    # closed form of the first wall's energy: sum over A particles below the plane
+    x = cfg["x"].reshape(-1, 3).astype(np.float64)
+    a = np.asarray(cfg["topo"]["type"]) == 0
+    d = np.float32(0.5 * L[2]) - x[a, 2]
+    assert pe[-2] == pytest.approx(np.sum(0.5 * 3.0 * d[d > 0] ** 2), rel=1e-5)
+    ns = cfg["topo"]["type"].size
+    for step in range(3):
+        eng.step(1)
+        xi = orc.tild_noise(ns, cfg["seed"], step)
+        o = orc.tild_step(sys_o, xi)
+        sys_o.x = o["x"].astype(np.float32)
+        sys_o.xo = o["xo"].astype(np.float32)
+        sys_o.noise_old = o["noise_old"].astype(np.float32)
+        x_g, _, _ = eng.get_state()
+        assert np.max(periodic_dist(x_g, o["x"], cfg["L"])) < 1e-6 * max(cfg["L"]), step
+    eng.close()
