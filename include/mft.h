@@ -187,6 +187,14 @@ typedef struct {
     const float* mobility;
     uint64_t seed;
     int has_charges;
+    /* angles (ps_deviceAngles.cu, PS_Box_initialization.cu:83-106): up to
+     * maxangles per particle (0 = none), per type k, theta_eq (radians) and
+     * style 0 = WLC k (1 + cos theta), 1 = harmonic k (theta - theta_eq)^2 */
+    int maxangles;
+    int nangletypes;
+    const float* angle_k;
+    const float* angle_theta;
+    const int* angle_style;
 } mft_tild_desc;
 
 typedef struct mft_tild mft_tild;
@@ -200,6 +208,13 @@ void* mft_tild_stream(mft_tild* eng);
  * type[N], charge[N] (may be NULL), nBonds[N], bondedTo/bondType[N*maxbonds]. */
 int mft_tild_set_topology(mft_tild* eng, const int* type, const float* charge,
                           const int* nBonds, const int* bondedTo, const int* bondType);
+/* Angles (PS_Box_moleculeMakers.cu:242-281 layout): nAngles[N], angleType[N*maxangles],
+ * angleGroup[N*maxangles*3] (particle ids i, j, k of each angle the particle
+ * is in); host arrays, before mft_tild_set_state. */
+int mft_tild_set_angles(mft_tild* eng, const int* nAngles, const int* angleType, const int* angleGroup);
+/* Angle energy at the current state (d_anglesStressEnergy / 3,
+ * PS_Box.cu:296-305), FP64. */
+int mft_tild_angle_energy(mft_tild* eng, double* energy);
 /* Positions, previous positions and previous noise (GJF state,
  * ps_integratorGJF.cu:43-51); xo/noise_old may be NULL (x_old = x, 0). */
 int mft_tild_set_state(mft_tild* eng, const float* x, const float* xo, const float* noise_old);

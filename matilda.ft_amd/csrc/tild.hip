@@ -181,6 +181,34 @@ static int tild_setup(TildEngine* e)
     if ((rc = dalloc(&e->nBonds, (size_t)N))) return rc;
     if ((rc = dalloc(&e->bondedTo, (size_t)N * mb))) return rc;
     if ((rc = dalloc(&e->bondType, (size_t)N * mb))) return rc;
+    // angles (PS_Box_initialization.cu:83-106, MAXANGLES :229-231)
+    std::memset(&e->ap, 0, sizeof(e->ap));
+    if (d.maxangles > 0) {
+        MFT_REQUIRE(d.maxangles <= kMaxAngles, MFT_ERR_UNSUPPORTED, "MAXANGLES must be 1..%d", kMaxAngles);
+        MFT_REQUIRE(d.nangletypes >= 1 && d.nangletypes <= kMaxAngleTypes && d.angle_k && d.angle_theta &&
+                        d.angle_style,
+                    MFT_ERR_INVALID, "angles need 1..%d angle types", kMaxAngleTypes);
+        e->ap.ntypes = d.nangletypes;
+        e->ap.maxangles = d.maxangles;
+        for (int t = 0; t < d.nangletypes; ++t) {
+            MFT_REQUIRE(d.angle_style[t] == 0 || d.angle_style[t] == 1, MFT_ERR_INVALID, "Angle style not supported!");
+            e->ap.k[t] = d.angle_k[t];
+            e->ap.theta[t] = d.angle_theta[t];
+            e->ap.style[t] = d.angle_style[t];
+        }
+        const size_t ma = d.maxangles;
+        for (int b = 0; b < 2; ++b) {
+            if ((rc = dalloc(&e->NA[b], (size_t)N))) return rc;
+            if ((rc = dalloc(&e->AT[b], (size_t)N * ma))) return rc;
+            if ((rc = dalloc(&e->AG[b], (size_t)N * ma * 3))) return rc;
+        }
+        if ((rc = dalloc(&e->nAngles, (size_t)N))) return rc;
+        if ((rc = dalloc(&e->angleType, (size_t)N * ma))) return rc;
+        if ((rc = dalloc(&e->angleGroup, (size_t)N * ma * 3))) return rc;
+        MFT_HIP(hipMemset(e->nAngles, 0, sizeof(int) * N));
+        MFT_HIP(hipMemset(e->angleType, 0, sizeof(int) * N * ma));
+        MFT_HIP(hipMemset(e->angleGroup, 0, sizeof(int) * N * ma * 3));
+    }
     if ((rc = dalloc(&e->pio, 3 * (size_t)N))) return rc;
     if ((rc = dalloc(&e->pio2, 3 * (size_t)N))) return rc;
     if ((rc = dalloc(&e->pio3, 3 * (size_t)N))) return rc;
@@ -489,7 +517,8 @@ int mft_tild_destroy(mft_tild* e)
                     e->NB[1], e->BS[0], e->BS[1], e->BT[0], e->BT[1], e->XN, e->NN, e->fgrid, e->ftot, e->key,
                     e->rank, e->map, e->slot_of, e->type, e->charge, e->nBonds, e->bondedTo, e->bondType, e->pio,
                     e->pio2, e->pio3, e->count, e->start, e->partial, e->rho, e->S, e->Y, e->F4, e->force_slot,
-                    e->epart, e->eout, e->bpart, e->tiles, e->tile_tab};
+                    e->epart, e->eout, e->bpart, e->tiles, e->tile_tab, e->NA[0], e->NA[1], e->AT[0], e->AT[1],
+                    e->AG[0], e->AG[1], e->nAngles, e->angleType, e->angleGroup};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (float* t : e->tables)
@@ -505,6 +534,43 @@ int mft_tild_destroy(mft_tild* e)
 }
 
 void* mft_tild_stream(mft_tild* e) { return e ? (void*)e->stream : nullptr; }
+
+int mft_tild_set_angles(mft_tild* e, const int* nAngles, const int* angleType, const int* angleGroup)
+{
+    MFT_REQUIRE(e && nAngles && angleType && angleGroup, MFT_ERR_INVALID, "null argument");
+    MFT_REQUIRE(e->d.maxangles > 0, MFT_ERR_INVALID, "the engine was created without angles (maxangles 0)");
+    MFT_REQUIRE(!e->have_state, MFT_ERR_INVALID, "set the angles before the state");
+    const size_t N = e->N, ma = e->d.maxangles;
+    for (size_t i = 0; i < N; ++i) {
+        MFT_REQUIRE(nAngles[i] >= 0 && nAngles[i] <= (int)ma, MFT_ERR_INVALID, "particle %zu has %d angles", i,
+                    nAngles[i]);
+        for (int a = 0; a < nAngles[i]; ++a) {
+            const int t = angleType[i * ma + a];
+            MFT_REQUIRE(t >= 0 && t < e->ap.ntypes, MFT_ERR_INVALID, "angle type %d out of range", t);
+            bool member = false;
+            for (int c = 0; c < 3; ++c) {
+                const int p = angleGroup[(i * ma + a) * 3 + c];
+                MFT_REQUIRE(p >= 0 && (size_t)p < N, MFT_ERR_INVALID, "angle member %d out of range", p);
+                member = member || (size_t)p == i;
+            }
+            MFT_REQUIRE(member, MFT_ERR_INVALID, "particle %zu is not a member of its angle %d", i, a);
+        }
+    }
+    MFT_HIP(hipMemcpy(e->nAngles, nAngles, sizeof(int) * N, hipMemcpyHostToDevice));
+    MFT_HIP(hipMemcpy(e->angleType, angleType, sizeof(int) * N * ma, hipMemcpyHostToDevice));
+    MFT_HIP(hipMemcpy(e->angleGroup, angleGroup, sizeof(int) * N * ma * 3, hipMemcpyHostToDevice));
+    return MFT_OK;
+}
+
+int mft_tild_angle_energy(mft_tild* e, double* energy)
+{
+    MFT_REQUIRE(e && energy && e->have_state, MFT_ERR_INVALID, "engine has no state");
+    if (e->d.maxangles <= 0) {
+        *energy = 0.0;
+        return MFT_OK;
+    }
+    return tild_angle_energy(e, energy);
+}
 
 int mft_tild_set_topology(mft_tild* e, const int* type, const float* charge, const int* nBonds, const int* bondedTo,
                           const int* bondType)

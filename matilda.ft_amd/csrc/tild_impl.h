@@ -34,6 +34,14 @@ const float2* fft_twiddles_device(int n); // fft.hip
 constexpr int kMaxSpecies = 16;
 constexpr int kMaxPots = 32;
 constexpr int kMaxBondTypes = 16;
+constexpr int kMaxAngleTypes = 8;
+constexpr int kMaxAngles = 8;
+
+struct AngleParams {
+    int ntypes, maxangles;
+    float k[kMaxAngleTypes], theta[kMaxAngleTypes];
+    int style[kMaxAngleTypes];
+};
 constexpr int kMaxChan = 8;
 
 struct KPot {
@@ -191,6 +199,11 @@ struct TildEngine {
     int* NB[2] = {nullptr, nullptr};     // bond count
     int* BS[2] = {nullptr, nullptr};     // bond partners as slots [N][maxbonds]
     int* BT[2] = {nullptr, nullptr};     // bond types [N][maxbonds]
+    int* NA[2] = {nullptr, nullptr};     // angle count (angles only)
+    int* AT[2] = {nullptr, nullptr};     // angle types [N][maxangles]
+    int* AG[2] = {nullptr, nullptr};     // angle members as slots [N][maxangles][3]
+    int *nAngles = nullptr, *angleType = nullptr, *angleGroup = nullptr; // particle-id order
+    AngleParams ap{};
     int cur = 0;
     // step temporaries in slot order
     float4 *XN = nullptr, *NN = nullptr; // next positions and noise
@@ -298,5 +311,6 @@ int tild_bias_setup(TildEngine* e, int pot);      // the field and its force (ti
 int tild_bias_add(TildEngine* e);                 // F4 += bias forces of the biased species
 int tild_bias_energy(TildEngine* e, int k, double* energy);
 int tild_wall_energy(TildEngine* e, int w, double* energy); // tild_particles.hip
+int tild_angle_energy(TildEngine* e, double* energy);         // tild_particles.hip
 
 } // namespace mft

@@ -252,6 +252,22 @@ __global__ void k_sort_bonds(const int* __restrict__ bondedTo, const int* __rest
     for (int b = 0; b < maxbonds; ++b) BS[s * maxbonds + b] = b < nBonds[i] ? slot_of[bondedTo[i * maxbonds + b]] : 0;
 }
 
+// Initial sort of the angles: counts, types and members as slots.
+__global__ void k_sort_angles(const int* __restrict__ nAngles, const int* __restrict__ angleType,
+                              const int* __restrict__ angleGroup, int ma, int ns, const int* __restrict__ slot_of,
+                              int* __restrict__ NA, int* __restrict__ AT, int* __restrict__ AG)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    const int s = slot_of[i], na = nAngles[i];
+    NA[s] = na;
+    for (int a = 0; a < ma; ++a) {
+        AT[s * ma + a] = angleType[i * ma + a];
+        for (int c = 0; c < 3; ++c)
+            AG[(s * ma + a) * 3 + c] = a < na ? slot_of[angleGroup[(i * ma + a) * 3 + c]] : 0;
+    }
+}
+
 // key order -> particle-id order: out[3 id + j] = src[slot].j
 __global__ void k_unsort(float* __restrict__ out, const float4* __restrict__ src, const int2* __restrict__ META,
                          int ns)
@@ -342,6 +358,21 @@ __global__ void k_remap_bonds(int ns, int maxbonds, const int* __restrict__ map,
     if (s >= ns) return;
     const int n = map[s], nb = NB[s];
     for (int b = 0; b < maxbonds; ++b) BS2[n * maxbonds + b] = b < nb ? map[BS[s * maxbonds + b]] : 0;
+}
+
+// angles into the new order (after every map entry is written)
+__global__ void k_permute_angles(int ns, int ma, const int* __restrict__ map, const int* __restrict__ NA,
+                                 const int* __restrict__ AT, const int* __restrict__ AG, int* __restrict__ NA2,
+                                 int* __restrict__ AT2, int* __restrict__ AG2)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const int n = map[s], na = NA[s];
+    NA2[n] = na;
+    for (int a = 0; a < ma; ++a) {
+        AT2[n * ma + a] = AT[s * ma + a];
+        for (int c = 0; c < 3; ++c) AG2[(n * ma + a) * 3 + c] = a < na ? map[AG[(s * ma + a) * 3 + c]] : 0;
+    }
 }
 
 // --------------------------------------------------------- tile density ---
@@ -756,6 +787,83 @@ __device__ __forceinline__ void bond_force_slots(int s, int ns, const float4* __
     }
 }
 
+// One angle of slot s (d_angles, ps_deviceAngles.cu:8-133): the force on
+// whichever member s is, and the angle's energy (d_anglesStressEnergy,
+// ps_deviceAngles.cu:138-290); false when a bond vector is too short.
+__device__ __forceinline__ bool angle_terms(int s, const float4* __restrict__ X, int ai, int aj, int ak, int t,
+                                            const AngleParams& ap, float L0, float L1, float L2, float* fs,
+                                            float* energy)
+{
+    const float Ls[3] = {L0, L1, L2};
+    const float4 xi = X[ai], xj = X[aj], xk = X[ak];
+    const float pi[3] = {xi.x, xi.y, xi.z}, pj[3] = {xj.x, xj.y, xj.z}, pk[3] = {xk.x, xk.y, xk.z};
+    float rij[3], rkj[3], mrij2 = 0.0f, mrkj2 = 0.0f;
+    for (int j = 0; j < 3; ++j) {
+        rij[j] = min_image(pi[j] - pj[j], Ls[j], 0.5f * Ls[j]);
+        mrij2 += rij[j] * rij[j];
+    }
+    if (mrij2 < 1.0E-4f) return false;
+    const float mrij = sqrtf(mrij2);
+    for (int j = 0; j < 3; ++j) {
+        rkj[j] = min_image(pk[j] - pj[j], Ls[j], 0.5f * Ls[j]);
+        mrkj2 += rkj[j] * rkj[j];
+    }
+    if (mrkj2 < 1.0E-4f) return false;
+    const float mrkj = sqrtf(mrkj2);
+    float dot = 0.0f;
+    for (int j = 0; j < 3; ++j) dot += rij[j] * rkj[j];
+    const float cos_th = dot / mrij / mrkj;
+    float fi[3], fk[3];
+    if (ap.style[t] == 0) {
+        // WLC: k (1 + cos theta)
+        const float kang = ap.k[t];
+        for (int j = 0; j < 3; ++j) {
+            fi[j] = -kang * (rkj[j] / mrij / mrkj - cos_th * rij[j] / mrij2);
+            fk[j] = -kang * (rij[j] / mrkj / mrij - cos_th * rkj[j] / mrkj2);
+        }
+        *energy = (float)((double)kang * (1.0 + (double)cos_th));
+    } else {
+        // harmonic: k (theta - theta_eq)^2
+        if (cos_th < -1.f || cos_th > 1.f) return false;
+        const float theta = acosf(cos_th);
+        const float denom = sqrtf(1.f - cos_th * cos_th);
+        const float iSinTheta = denom > 1.0E-4f ? (float)(1.0 / (double)denom) : 0.f;
+        const float dtheta = theta - ap.theta[t];
+        const float du_mag = 2.f * ap.k[t] * dtheta;
+        for (int j = 0; j < 3; ++j) {
+            fi[j] = du_mag * iSinTheta * (rkj[j] / mrij / mrkj - cos_th * rij[j] / mrij2);
+            fk[j] = du_mag * iSinTheta * (rij[j] / mrij / mrkj - cos_th * rkj[j] / mrkj2);
+        }
+        *energy = ap.k[t] * dtheta * dtheta;
+    }
+    for (int j = 0; j < 3; ++j) {
+        if (s == ai) fs[j] = fi[j];
+        else if (s == aj) fs[j] = -(fi[j] + fk[j]);
+        else if (s == ak) fs[j] = fk[j];
+        else fs[j] = 0.f;
+    }
+    return true;
+}
+
+__device__ __forceinline__ void angle_force_slots(int s, const float4* __restrict__ X, const int* __restrict__ NA,
+                                                  const int* __restrict__ AT, const int* __restrict__ AG,
+                                                  const AngleParams& ap, float L0, float L1, float L2, float* fo)
+{
+    const int ma = ap.maxangles, na = min(NA[s], ma);
+    float lf[3] = {0.f, 0.f, 0.f};
+    for (int a = 0; a < na; ++a) {
+        const int* g = AG + (s * ma + a) * 3;
+        float fs[3], en;
+        if (!angle_terms(s, X, g[0], g[1], g[2], AT[s * ma + a], ap, L0, L1, L2, fs, &en)) continue;
+        lf[0] += fs[0];
+        lf[1] += fs[1];
+        lf[2] += fs[2];
+    }
+    fo[0] += lf[0];
+    fo[1] += lf[1];
+    fo[2] += lf[2];
+}
+
 struct StepArgs {
     const float4* XO;
     const float4* NZ;
@@ -770,6 +878,8 @@ struct StepArgs {
     float dx[3];
     int p, nx, ny, nz;
     WallParams walls;
+    const int *NA, *AT, *AG;   // angles (null without)
+    AngleParams ap;
 };
 
 // Wall forces of ps_potentialWall.cu:92-108 on a particle of species ty
@@ -841,6 +951,7 @@ __global__ void __launch_bounds__(256) k_integrate(
         const float4 fg = fgrid[s];
         float fo[3] = {fg.x, fg.y, fg.z};
         bond_force_slots(s, ns, X, NB, BS, BT, maxbonds, bp, A.L[0], A.L[1], A.L[2], xc.x, xc.y, xc.z, fo);
+        if (A.NA) angle_force_slots(s, X, A.NA, A.AT, A.AG, A.ap, A.L[0], A.L[1], A.L[2], fo);
         if (A.walls.n) wall_forces(A.walls, META[s].y, xc, fo);
         if (ftot) ftot[s] = make_float4(fo[0], fo[1], fo[2], 0.f);
         if (A.integ != MFT_INT_NONE) k = integrate_slot(s, xc, META[s], fo, A, tg);
@@ -998,6 +1109,9 @@ int tild_sort_state(TildEngine* e)
                                                      e->bondType, mb, N, e->key, e->rank, e->start, e->slot_of,
                                                      e->X[c], e->XO[c], e->NZ[c], e->META[c], e->NB[c], e->BT[c]);
     k_sort_bonds<<<nblk(N), kBlock, 0, e->stream>>>(e->bondedTo, e->nBonds, mb, N, e->slot_of, e->BS[c]);
+    if (d.maxangles > 0)
+        k_sort_angles<<<nblk(N), kBlock, 0, e->stream>>>(e->nAngles, e->angleType, e->angleGroup, d.maxangles, N,
+                                                         e->slot_of, e->NA[c], e->AT[c], e->AG[c]);
     MFT_LAUNCH_CHECK();
     return MFT_OK;
 }
@@ -1029,6 +1143,9 @@ int tild_permute(TildEngine* e)
                                                   e->META[c], e->NB[c], e->BT[c], e->X[n], e->XO[n], e->NZ[n],
                                                   e->META[n], e->NB[n], e->BT[n]);
     k_remap_bonds<<<nblk(N), kBlock, 0, e->stream>>>(N, mb, e->map, e->NB[c], e->BS[c], e->BS[n]);
+    if (e->d.maxangles > 0)
+        k_permute_angles<<<nblk(N), kBlock, 0, e->stream>>>(N, e->d.maxangles, e->map, e->NA[c], e->AT[c], e->AG[c],
+                                                            e->NA[n], e->AT[n], e->AG[n]);
     MFT_LAUNCH_CHECK();
     e->cur = n;
     return MFT_OK;
@@ -1145,6 +1262,11 @@ static StepArgs step_args(const TildEngine* e)
     A.ny = d.ny;
     A.nz = d.nz;
     A.walls = e->walls;
+    const bool ang = d.maxangles > 0;
+    A.NA = ang ? e->NA[c] : nullptr;
+    A.AT = ang ? e->AT[c] : nullptr;
+    A.AG = ang ? e->AG[c] : nullptr;
+    A.ap = e->ap;
     return A;
 }
 
@@ -1194,6 +1316,51 @@ int tild_wall_energy(TildEngine* e, int w, double* energy)
     double t = 0.0;
     for (double v : h) t += v;
     *energy = t;
+    return MFT_OK;
+}
+
+// d_anglesStressEnergy summed over particles / 3 (PS_Box.cu:296-305), FP64
+__global__ void k_angle_energy_slots(double* __restrict__ partial, const float4* __restrict__ X,
+                                     const int* __restrict__ NA, const int* __restrict__ AT,
+                                     const int* __restrict__ AG, AngleParams ap, float L0, float L1, float L2, int ns)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (s < ns) {
+        const int ma = ap.maxangles, na = min(NA[s], ma);
+        float es = 0.0f;
+        for (int a = 0; a < na; ++a) {
+            const int* g = AG + (s * ma + a) * 3;
+            float fs[3], en;
+            if (angle_terms(s, X, g[0], g[1], g[2], AT[s * ma + a], ap, L0, L1, L2, fs, &en)) es += en;
+        }
+        v = es;
+    }
+    v = wave_sum(v);
+    __shared__ double red[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < kBlock / 64; ++k) t += red[k];
+        partial[blockIdx.x] = t;
+    }
+}
+
+int tild_angle_energy(TildEngine* e, double* energy)
+{
+    const mft_tild_desc& d = e->d;
+    const int c = e->cur;
+    const unsigned nb = nblk(e->N);
+    k_angle_energy_slots<<<nb, kBlock, 0, e->stream>>>(e->bpart, e->X[c], e->NA[c], e->AT[c], e->AG[c], e->ap, d.lx,
+                                                        d.ly, d.lz, e->N);
+    MFT_LAUNCH_CHECK();
+    std::vector<double> h(nb);
+    MFT_HIP(hipMemcpyAsync(h.data(), e->bpart, sizeof(double) * nb, hipMemcpyDeviceToHost, e->stream));
+    MFT_HIP(hipStreamSynchronize(e->stream));
+    double t = 0.0;
+    for (double v : h) t += v;
+    *energy = t / 3.0;
     return MFT_OK;
 }
 

@@ -293,9 +293,14 @@ public:
     // particles (reference layouts: x AoS [nstot*3], bonds [nstot*MAXBONDS])
     std::vector<float> x, charges;
     std::vector<int> intSpecies, mID, nBonds, bondedTo, bondType;
+    // angles (PS_Box_initialization.cu:83-106, 229-231): per type k, theta_eq
+    // (radians), style 0 WLC / 1 harmonic; per particle [nstot*MAXANGLES(*3)]
+    int MAXANGLES = 3, nAnglesTot = 0;
+    std::vector<float> angleK, angleTheq;
+    std::vector<int> angleStyle, nAngles, angleType, angleGroup;
     std::string datFileName = "ps_data.dat";
     int maxSteps = 0, totSteps = 0;
-    float Upe = 0.f, Ubond = 0.f;
+    float Upe = 0.f, Ubond = 0.f, Uangle = 0.f;
 
 private:
     HostRng rng;
@@ -305,6 +310,7 @@ private:
     void readDataConfig(const std::string& name);       // PS_Box_initialization.cu:787-990
     void enableCharges();
     void grow(int n);
+    void addAngle(int i1, int i2, int i3, int type);
     void finishInitialization();                        // PS_Box_initialization.cu:352-426
 };
 

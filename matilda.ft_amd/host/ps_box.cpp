@@ -269,6 +269,25 @@ void PS_Box::grow(int n)
     nBonds.resize(n, 0);
     bondedTo.resize((size_t)n * MAXBONDS, 0);
     bondType.resize((size_t)n * MAXBONDS, 0);
+    nAngles.resize(n, 0);
+    angleType.resize((size_t)n * MAXANGLES, 0);
+    angleGroup.resize((size_t)n * MAXANGLES * 3, 0);
+}
+
+// one angle (i1, i2 vertex, i3) entered on the lists of its three members
+// (PS_Box_moleculeMakers.cu:242-288, PS_Box_initialization.cu:962-985)
+void PS_Box::addAngle(int i1, int i2, int i3, int type)
+{
+    for (int p : {i1, i2, i3}) {
+        const int na = nAngles[p];
+        if (na >= MAXANGLES) die("MAXANGLES too small for the angles of particle " + std::to_string(p + 1));
+        int* g = &angleGroup[((size_t)p * MAXANGLES + na) * 3];
+        g[0] = i1;
+        g[1] = i2;
+        g[2] = i3;
+        angleType[(size_t)p * MAXANGLES + na] = type;
+        nAngles[p] = na + 1;
+    }
 }
 
 void PS_Box::enableCharges() { doCharges = 1; }
@@ -285,7 +304,28 @@ void PS_Box::readInput(std::ifstream& inp)
                 ended = true;
                 break;
             }
-            if (firstWord == "angle") die("angles are not part of this build's particle-mesh path");
+            if (firstWord == "angle") {
+                int atype;
+                std::string style;
+                iss >> atype >> style;
+                if (atype < 1) die("angle type must be >= 1");
+                if (atype > (int)angleK.size()) {
+                    angleK.resize(atype, 0.f);
+                    angleTheq.resize(atype, 0.f);
+                    angleStyle.resize(atype, 0);
+                }
+                iss >> angleK[atype - 1];
+                if (style == "wlc") {
+                    angleStyle[atype - 1] = 0;
+                } else if (style == "harmonic") {
+                    angleStyle[atype - 1] = 1;
+                    float th;
+                    iss >> th;
+                    angleTheq[atype - 1] = (float)(th * (3.141592654f / 180.0));
+                } else {
+                    die("Angle style not supported!");
+                }
+            }
             else if (firstWord == "blocksize" || firstWord == "blockSize") { int b; iss >> b; }
             else if (firstWord == "bond") {
                 int btype;
@@ -348,8 +388,9 @@ void PS_Box::readInput(std::ifstream& inp)
             } else if (firstWord == "logFreq" || firstWord == "logfreq") {
                 iss >> logFreq;
             } else if (firstWord == "MAXANGLES") {
-                int a;
-                iss >> a;
+                if (nstot > 0) die("MAXANGLES must be set before defining any molecules");
+                iss >> MAXANGLES;
+                if (MAXANGLES < 1) die("MAXANGLES must be >= 1");
             } else if (firstWord == "MAXBONDS") {
                 if (nstot > 0) die("MAXBONDS must be set before defining any molecules");
                 iss >> MAXBONDS;
@@ -415,7 +456,8 @@ void PS_Box::makeLinear(std::istringstream& iss)
     else die("invalid keyword after molecule linear. must be 'phi' or 'nmolecs'");
     iss >> numBlocks;
     if (numBlocks < 1) die("molecule linear needs at least one block");
-    std::vector<int> Nb(numBlocks), bType(numBlocks, 1);
+    std::vector<int> Nb(numBlocks), bType(numBlocks, 1), aType(numBlocks, 0);
+    bool angleFlag = false;
     std::vector<std::string> sp(numBlocks);
     std::vector<float> bq(numBlocks, 0.f);
     bool molCharges = false;
@@ -432,7 +474,8 @@ void PS_Box::makeLinear(std::istringstream& iss)
         } else if (s1 == "bondType" || s1 == "bondtype") {
             for (int j = 0; j < numBlocks; ++j) iss >> bType[j];
         } else if (s1 == "angleType" || s1 == "angletype") {
-            die("angles are not part of this build's particle-mesh path");
+            for (int j = 0; j < numBlocks; ++j) iss >> aType[j];
+            angleFlag = true;
         } else if (s1 == "xrange") {
             iss >> Rmin[0] >> Rmax[0];
         } else if (s1 == "yrange") {
@@ -454,15 +497,20 @@ void PS_Box::makeLinear(std::istringstream& iss)
     grow(nstot);
     std::vector<int> spi(numBlocks);
     for (int j = 0; j < numBlocks; ++j) spi[j] = findSpeciesInteger(sp[j]);
+    for (int i = nstot - nmolecs * Ntot; i < nstot; ++i) nAngles[i] = 0;
     for (int i = 0; i < nmolecs; ++i) {
-        double ru[3];
+        // with angles the chain is laid out straight: one bond direction per
+        // molecule, drawn at its second bead (PS_Box_moleculeMakers.cu:186-190)
+        double ru[3] = {0.0, 0.0, 0.0};
+        bool have_ru = false;
         for (int j = 0; j < numBlocks; ++j) {
             for (int s = 0; s < Nb[j]; ++s) {
                 intSpecies[ind] = spi[j];
                 if (j == 0 && s == 0) {
                     for (int a = 0; a < 3; ++a) x[ind * 3 + a] = (float)(rng.uniform() * (Rmax[a] - Rmin[a]) + Rmin[a]);
                 } else {
-                    rng.unit_vector(ru);
+                    if ((s == 1 && j == 0) || !angleFlag || !have_ru) rng.unit_vector(ru);
+                    have_ru = true;
                     for (int a = 0; a < 3; ++a) {
                         float v = x[(ind - 1) * 3 + a] + (float)ru[a];
                         if (v > L[a]) v -= L[a];
@@ -483,6 +531,9 @@ void PS_Box::makeLinear(std::istringstream& iss)
                     bondType[ind * MAXBONDS + nBonds[ind]] = bType[j] - 1;
                     nBonds[ind]++;
                 }
+                // an angle at every interior bead of a block with an angle type
+                if (aType[j] != 0 && (j > 0 || s > 0) && (j < numBlocks - 1 || s < Nb[j] - 1))
+                    addAngle(ind - 1, ind, ind + 1, aType[j] - 1);
                 mID[ind] = molec0 + i;
                 ++ind;
             }
@@ -564,7 +615,19 @@ void PS_Box::readDataConfig(const std::string& name)
             bondType[i2 * MAXBONDS + nBonds[i2]++] = bt;
         }
     }
-    if (nang > 0) die("angles are not part of this build's particle-mesh path");
+    if (nang > 0) {
+        next_nonblank(line); // Angles
+        for (int i = 0; i < nang; ++i) {
+            if (!next_nonblank(line)) die("Premature end of " + name);
+            std::istringstream ls(line);
+            int cnt, at, i1, i2, i3;
+            ls >> cnt >> at >> i1 >> i2 >> i3;
+            if (i3 < i1) std::swap(i1, i3);
+            if (std::min(i1, std::min(i2, i3)) < 1 || std::max(i1, std::max(i2, i3)) > nat)
+                die("readData: angle member out of range");
+            addAngle(i1 - 1, i2 - 1, i3 - 1, at - 1);
+        }
+    }
     nBondsTot = nb;
 }
 
@@ -581,6 +644,16 @@ void PS_Box::finishInitialization()
     nBondsTot = 0;
     for (int i = 0; i < nstot; ++i) nBondsTot += nBonds[i];
     nBondsTot /= 2;
+    // angles counted once, at their vertex (PS_Box_initialization.cu:429-444)
+    nAnglesTot = 0;
+    for (int i = 0; i < nstot; ++i)
+        for (int j = 0; j < nAngles[i]; ++j)
+            if (angleGroup[((size_t)i * MAXANGLES + j) * 3 + 1] == i) ++nAnglesTot;
+    for (int i = 0; i < nstot; ++i)
+        for (int j = 0; j < nAngles[i]; ++j) {
+            const int t = angleType[(size_t)i * MAXANGLES + j];
+            if (t < 0 || t >= (int)angleK.size()) die("angle type " + std::to_string(t + 1) + " is not defined");
+        }
     // the reference checks neutrality when charges are on (PS_Box_initialization.cu:398-412)
     std::vector<mft_tild_potential> pd;
     for (auto* p : potentials) pd.push_back(p->descriptor());
@@ -612,6 +685,13 @@ void PS_Box::finishInitialization()
     d.mobility = mob.data();
     d.seed = RANDSEED;
     d.has_charges = doCharges;
+    if (nAnglesTot > 0) {
+        d.maxangles = MAXANGLES;
+        d.nangletypes = (int)angleK.size();
+        d.angle_k = angleK.data();
+        d.angle_theta = angleTheq.data();
+        d.angle_style = angleStyle.data();
+    }
     if (dry_run) {
         std::printf("{\"box\": \"ps\", \"grid\": [%d, %d, %d], \"L\": [%.9g, %.9g, %.9g], \"pmeorder\": %d, "
                     "\"nparticles\": %d, \"nspecies\": %d, \"maxbonds\": %d, \"nbonds\": %d, \"integrator\": %d, "
@@ -628,6 +708,9 @@ void PS_Box::finishInitialization()
         std::printf("], \"bonds\": [");
         for (size_t i = 0; i < bondK.size(); ++i)
             std::printf("%s[%d, %.9g, %.9g]", i ? ", " : "", bondStyle[i], bondK[i], bondReq[i]);
+        std::printf("], \"nangles\": %d, \"maxangles\": %d, \"angles\": [", nAnglesTot, MAXANGLES);
+        for (size_t i = 0; i < angleK.size(); ++i)
+            std::printf("%s[%d, %.9g, %.9g]", i ? ", " : "", angleStyle[i], angleK[i], angleTheq[i]);
         std::printf("], \"computes\": [");
         for (size_t i = 0; i < computes.size(); ++i)
             std::printf("%s{\"style\": \"avg_sk\", \"mask\": %u, \"wait\": %d, \"freq\": %d, \"file\": \"%s\"}",
@@ -642,6 +725,8 @@ void PS_Box::finishInitialization()
     check_mft(mft_tild_set_topology(eng, intSpecies.data(), doCharges ? charges.data() : nullptr, nBonds.data(),
                                     bondedTo.data(), bondType.data()),
               "mft_tild_set_topology");
+    if (nAnglesTot > 0)
+        check_mft(mft_tild_set_angles(eng, nAngles.data(), angleType.data(), angleGroup.data()), "mft_tild_set_angles");
     check_mft(mft_tild_set_state(eng, x.data(), nullptr, nullptr), "mft_tild_set_state");
     int choice = 0;
     check_mft(mft_tild_autotune(eng, 3, &choice), "mft_tild_autotune");
@@ -658,7 +743,10 @@ void PS_Box::computeThermoProps()
     double be = 0.0;
     check_mft(mft_tild_energies(eng, pe.data(), &be), "mft_tild_energies");
     Ubond = (float)be;
-    Upe = Ubond;
+    double ae = 0.0;
+    if (nAnglesTot > 0) check_mft(mft_tild_angle_energy(eng, &ae), "mft_tild_angle_energy");
+    Uangle = (float)ae;
+    Upe = Ubond + Uangle;
     for (size_t i = 0; i < potentials.size(); ++i) {
         potentials[i]->energy = (float)pe[i];
         Upe += (float)pe[i];
@@ -674,6 +762,10 @@ void PS_Box::writeData(int step)
     if (nBondsTot > 0) {
         otp << " " << Ubond;
         std::cout << " Ubond: " << Ubond;
+    }
+    if (nAnglesTot > 0) {
+        otp << " " << Uangle;
+        std::cout << " Uangle: " << Uangle;
     }
     for (size_t i = 0; i < potentials.size(); ++i) {
         otp << " " << potentials[i]->energy;
@@ -748,13 +840,21 @@ void PS_Box::writeGSDtraj()
                     btype.push_back((uint32_t)bondType[(size_t)i * MAXBONDS + j]);
                 }
             }
-        const uint32_t nb = (uint32_t)btype.size(), na = 0;
+        std::vector<uint32_t> atype, agroup;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < nAngles[i]; ++j) {
+                const int* g = &angleGroup[((size_t)i * MAXANGLES + j) * 3];
+                if (g[1] != i) continue; // each angle once, at its vertex
+                atype.push_back((uint32_t)angleType[(size_t)i * MAXANGLES + j]);
+                for (int c = 0; c < 3; ++c) agroup.push_back((uint32_t)g[c]);
+            }
+        const uint32_t nb = (uint32_t)btype.size(), na = (uint32_t)atype.size();
         gsd->chunk("bonds/N", GsdWriter::UINT32, 1, 1, &nb);
         gsd->chunk("bonds/typeid", GsdWriter::UINT32, nb, 1, btype.data());
         gsd->chunk("bonds/group", GsdWriter::UINT32, nb, 2, bpair.data());
         gsd->chunk("angles/N", GsdWriter::UINT32, 1, 1, &na);
-        gsd->chunk("angles/typeid", GsdWriter::UINT32, 0, 1, nullptr);
-        gsd->chunk("angles/group", GsdWriter::UINT32, 0, 3, nullptr);
+        gsd->chunk("angles/typeid", GsdWriter::UINT32, na, 1, atype.data());
+        gsd->chunk("angles/group", GsdWriter::UINT32, na, 3, agroup.data());
     } else {
         const uint64_t step = (uint64_t)totSteps;
         gsd->chunk("configuration/step", GsdWriter::UINT64, 1, 1, &step);
@@ -920,8 +1020,10 @@ void PS_Box::writeDataConfig(const std::string& name)
     check_mft(mft_tild_get_state(eng, xc.data(), nullptr, nullptr), "mft_tild_get_state");
     FILE* otp = std::fopen(name.c_str(), "w");
     if (!otp) die("Failed to open " + name);
-    std::fprintf(otp, "Generated by the MI355X front end\n\n%d atoms\n%d bonds\n0 angles\n\n", nstot, nBondsTot);
-    std::fprintf(otp, "%d atom types\n%d bond types\n0 angle types\n\n", (int)species.size(), (int)bondK.size());
+    std::fprintf(otp, "Generated by the MI355X front end\n\n%d atoms\n%d bonds\n%d angles\n\n", nstot, nBondsTot,
+                 nAnglesTot);
+    std::fprintf(otp, "%d atom types\n%d bond types\n%d angle types\n\n", (int)species.size(), (int)bondK.size(),
+                 (int)angleK.size());
     std::fprintf(otp, "0 %f xlo xhi\n0 %f ylo yhi\n0 %f zlo zhi\n\nMasses\n\n", L[0], L[1], L[2]);
     for (size_t i = 0; i < species.size(); ++i) std::fprintf(otp, "%zu %f\n", i + 1, species[i].mass);
     std::fprintf(otp, "\nAtoms\n\n");
@@ -937,5 +1039,17 @@ void PS_Box::writeDataConfig(const std::string& name)
             int j = bondedTo[i * MAXBONDS + k];
             if (j > i) std::fprintf(otp, "%d %d %d %d\n", ++b, bondType[i * MAXBONDS + k] + 1, i + 1, j + 1);
         }
+    if (nAnglesTot > 0) {
+        // PS_Box.cu:728-745: each angle once, from its first member
+        std::fprintf(otp, "\nAngles\n\n");
+        int a = 0;
+        for (int i = 0; i < nstot; ++i)
+            for (int k = 0; k < nAngles[i]; ++k) {
+                const int* g = &angleGroup[((size_t)i * MAXANGLES + k) * 3];
+                if (g[0] == i)
+                    std::fprintf(otp, "%d %d %d %d %d\n", ++a, angleType[(size_t)i * MAXANGLES + k] + 1, g[0] + 1,
+                                 g[1] + 1, g[2] + 1);
+            }
+    }
     std::fclose(otp);
 }

@@ -167,7 +167,8 @@ class TildDescC(C.Structure):
                 ("nparticles", _i), ("nspecies", _i), ("maxbonds", _i), ("nbondtypes", _i),
                 ("bond_k", _p), ("bond_r0", _p), ("bond_style", _p), ("npotentials", _i),
                 ("potentials", _p), ("integrator", _i), ("delt", _f), ("mass", _p), ("mobility", _p),
-                ("seed", C.c_uint64), ("has_charges", _i)]
+                ("seed", C.c_uint64), ("has_charges", _i), ("maxangles", _i), ("nangletypes", _i),
+                ("angle_k", _p), ("angle_theta", _p), ("angle_style", _p)]
 
 
 def grid_weights(x, Nx, dx, pmeorder):
@@ -297,7 +298,8 @@ def em_step(x, f, delt, L, seed, step):
 
 
 def tild_desc(Nx, L, pmeorder, nspecies, nparticles, potentials, bond_k=(), bond_r0=(), bond_style=(),
-              integrator="GJF", delt=0.005, mass=None, mobility=None, seed=1234, has_charges=False, maxbonds=2):
+              integrator="GJF", delt=0.005, mass=None, mobility=None, seed=1234, has_charges=False, maxbonds=2,
+              angle_k=(), angle_theta=(), angle_style=(), maxangles=0):
     """mft_tild_desc and the arrays it points to (keep them alive)."""
     pots = (TildPotentialC * max(1, len(potentials)))()
     for i, p in enumerate(potentials):
@@ -313,7 +315,13 @@ def tild_desc(Nx, L, pmeorder, nspecies, nparticles, potentials, bond_k=(), bond
     d = TildDescC(Nx[0], Nx[1], Nx[2], L[0], L[1], L[2], pmeorder, nparticles, nspecies, maxbonds,
                   bk.size, _ptr(bk), _ptr(br), _ptr(bs), len(potentials), C.cast(pots, _p),
                   INTEGRATOR[integrator], delt, _ptr(ma), _ptr(mo), seed, int(has_charges))
-    return d, [pots, bk, br, bs, ma, mo]
+    ak = np.asarray(angle_k, np.float32)
+    at = np.asarray(angle_theta, np.float32)
+    ast = np.asarray(angle_style, np.int32)
+    d.maxangles = maxangles
+    d.nangletypes = ak.size
+    d.angle_k, d.angle_theta, d.angle_style = _ptr(ak), _ptr(at), _ptr(ast)
+    return d, [pots, bk, br, bs, ma, mo, ak, at, ast]
 
 
 class TildEngine:
@@ -321,9 +329,10 @@ class TildEngine:
 
     def __init__(self, Nx, L, pmeorder, nspecies, nparticles, potentials, bond_k=(), bond_r0=(),
                  bond_style=(), integrator="GJF", delt=0.005, mass=None, mobility=None, seed=1234,
-                 has_charges=False, maxbonds=2):
+                 has_charges=False, maxbonds=2, angle_k=(), angle_theta=(), angle_style=(), maxangles=0):
         d, self._keep = tild_desc(Nx, L, pmeorder, nspecies, nparticles, potentials, bond_k, bond_r0, bond_style,
-                                  integrator, delt, mass, mobility, seed, has_charges, maxbonds)
+                                  integrator, delt, mass, mobility, seed, has_charges, maxbonds, angle_k,
+                                  angle_theta, angle_style, maxangles)
         self.h = _p()
         check(lib().mft_tild_create(C.byref(self.h), C.byref(d)))
         self.N = nparticles
@@ -350,6 +359,19 @@ class TildEngine:
         bt = np.ascontiguousarray(bondedTo, np.int32) if bondedTo is not None else None
         ty = np.ascontiguousarray(bondType, np.int32) if bondType is not None else None
         check(lib().mft_tild_set_topology(self.h, _ptr(t), _ptr(q), _ptr(nb), _ptr(bt), _ptr(ty)))
+
+    def set_angles(self, nAngles, angleType, angleGroup):
+        """Per-particle angle lists (before set_state): nAngles[N], angleType[N, maxangles],
+        angleGroup[N, maxangles, 3] particle ids i, j (vertex), k."""
+        na = np.ascontiguousarray(nAngles, np.int32)
+        at = np.ascontiguousarray(angleType, np.int32)
+        ag = np.ascontiguousarray(angleGroup, np.int32)
+        check(lib().mft_tild_set_angles(self.h, _ptr(na), _ptr(at), _ptr(ag)))
+
+    def angle_energy(self):
+        e = _d(0)
+        check(lib().mft_tild_angle_energy(self.h, C.byref(e)))
+        return e.value
 
     def set_state(self, x, xo=None, noise_old=None):
         x = np.ascontiguousarray(x, np.float32)

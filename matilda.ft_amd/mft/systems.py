@@ -123,16 +123,46 @@ def tild_coacervate(nchains=40000, Nchain=20, nsolvent=200000, box=60.0, grid=12
                 delt=delt, mass=[1.0] * 3, mobility=[1.0] * 3, has_charges=True, seed=seed)
 
 
+def add_chain_angles(cfg, chain_len, bead_type, k, theta, style, maxangles=3):
+    """Angles on every interior bead of the linear chains of cfg (chains of
+    chain_len consecutive ids), the layout of the linear molecule maker
+    (PS_Box_moleculeMakers.cu:242-288): each angle (i-1, i, i+1) is entered on
+    the lists of its three members.  bead_type(i) gives the 0-based angle type
+    of the vertex bead, or -1 for none.  k, theta (radians), style per type."""
+    ns = cfg["topo"]["type"].size
+    nA = np.zeros(ns, np.int32)
+    aT = np.zeros((ns, maxangles), np.int32)
+    aG = np.zeros((ns, maxangles, 3), np.int32)
+    for i in range(ns):
+        pos = i % chain_len
+        t = bead_type(i)
+        if t < 0 or pos == 0 or pos == chain_len - 1:
+            continue
+        for p in (i - 1, i, i + 1):
+            aG[p, nA[p]] = (i - 1, i, i + 1)
+            aT[p, nA[p]] = t
+            nA[p] += 1
+    cfg["angles"] = dict(nAngles=nA, angleType=aT, angleGroup=aG, maxangles=maxangles,
+                         k=list(k), theta=list(theta), style=list(style))
+    return cfg
+
+
 def tild_engine(cfg, integrator="GJF"):
     """Build a TildEngine for a config dict from this module."""
     from . import TildEngine
     topo = cfg["topo"]
+    ang = cfg.get("angles")
+    akw = {}
+    if ang is not None:
+        akw = dict(angle_k=ang["k"], angle_theta=ang["theta"], angle_style=ang["style"], maxangles=ang["maxangles"])
     eng = TildEngine(cfg["Nx"], cfg["L"], cfg["pmeorder"], len(cfg["species"]), topo["type"].size,
                      cfg["potentials"], cfg["bond_k"], cfg["bond_r0"], cfg["bond_style"], integrator=integrator,
                      delt=cfg["delt"], mass=cfg["mass"], mobility=cfg["mobility"], seed=cfg["seed"],
-                     has_charges=cfg["has_charges"])
+                     has_charges=cfg["has_charges"], **akw)
     eng.set_topology(topo["type"], topo["charge"] if cfg["has_charges"] else None, topo["nBonds"],
                      topo["bondedTo"], topo["bondType"])
+    if ang is not None:
+        eng.set_angles(ang["nAngles"], ang["angleType"], ang["angleGroup"])
     eng.set_state(cfg["x"])
     return eng
 

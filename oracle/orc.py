@@ -185,6 +185,72 @@ def map_forces(f, gridF, sites, inds, W, G, gvol, Dim, q=None, bits=64):
     lib(bits).orc_map_forces(_ptr(f), _ptr(gridF), _ptr(_fa(q) if q is not None else None), _ptr(sites), sites.size, _ptr(inds), _ptr(W), G, gvol, Dim)
 
 
+def _angle_terms(ang, x, L):
+    """Per (particle, angle) entries of d_angles / d_anglesStressEnergy
+    (ps_deviceAngles.cu:8-290) in FP64: the owner particle, the force on it
+    and the angle's energy, with the reference's skips (a bond vector shorter
+    than 1e-2, |cos| > 1 for the harmonic style)."""
+    Dim = len(L)
+    Lf = np.asarray(_fa(L), np.float64)
+    Lh = np.asarray(_fa(0.5 * _fa(L)), np.float64)
+    xr = np.asarray(x).reshape(-1, Dim).astype(np.float64)
+    na = np.asarray(ang["nAngles"])
+    ma = int(ang["maxangles"])
+    at = np.asarray(ang["angleType"]).reshape(-1, ma)
+    ag = np.asarray(ang["angleGroup"]).reshape(-1, ma, 3)
+    owner, slot = np.nonzero(np.arange(ma)[None, :] < na[:, None])
+    t = at[owner, slot]
+    ai, aj, ak = ag[owner, slot, 0], ag[owner, slot, 1], ag[owner, slot, 2]
+
+    def mi(d):
+        d = np.where(d > Lh, d - Lf, d)
+        return np.where(d < -Lh, d + Lf, d)
+
+    rij = mi(xr[ai] - xr[aj])
+    rkj = mi(xr[ak] - xr[aj])
+    mrij2 = np.sum(rij * rij, axis=1)
+    mrkj2 = np.sum(rkj * rkj, axis=1)
+    ok = (mrij2 >= 1e-4) & (mrkj2 >= 1e-4)
+    mrij = np.sqrt(np.maximum(mrij2, 1e-300))
+    mrkj = np.sqrt(np.maximum(mrkj2, 1e-300))
+    cos = np.sum(rij * rkj, axis=1) / mrij / mrkj
+    k = np.asarray(ang["k"], np.float32).astype(np.float64)[t]
+    th0 = np.asarray(ang["theta"], np.float32).astype(np.float64)[t]
+    style = np.asarray(ang["style"])[t]
+    dci = rkj / (mrij * mrkj)[:, None] - (cos / mrij2)[:, None] * rij
+    dck = rij / (mrij * mrkj)[:, None] - (cos / mrkj2)[:, None] * rkj
+    harm = style == 1
+    ok &= ~(harm & ((cos < -1.0) | (cos > 1.0)))
+    cc = np.clip(cos, -1.0, 1.0)
+    theta = np.arccos(cc)
+    denom = np.sqrt(1.0 - cc * cc)
+    isin = np.where(denom > 1e-4, 1.0 / np.maximum(denom, 1e-300), 0.0)
+    dth = theta - th0
+    pre = np.where(harm, 2.0 * k * dth * isin, -k)
+    fi = pre[:, None] * dci
+    fk = pre[:, None] * dck
+    en = np.where(harm, k * dth * dth, k * (1.0 + cos))
+    fo = np.where((owner == ai)[:, None], fi,
+                  np.where((owner == aj)[:, None], -(fi + fk), np.where((owner == ak)[:, None], fk, 0.0)))
+    fo[~ok] = 0.0
+    en = np.where(ok, en, 0.0)
+    return owner, fo, en
+
+
+def angle_forces(f, ang, x, L):
+    """Adds the angle forces of every particle to f (d_angles, ps_deviceAngles.cu:8-145)."""
+    Dim = len(L)
+    owner, fo, _ = _angle_terms(ang, x, L)
+    fr = f.reshape(-1, Dim)
+    np.add.at(fr, owner, fo.astype(fr.dtype))
+
+
+def angle_energy(ang, x, L):
+    """Uangle = sum over particles of their angles' energies / 3 (PS_Box.cu:296-305)."""
+    _, _, en = _angle_terms(ang, x, L)
+    return float(np.sum(en)) / 3.0
+
+
 def bonds(f, topo, x, L, bits=64):
     Lf = _fa(L)
     Lh = _fa(0.5 * Lf)
@@ -285,6 +351,9 @@ class TildSystem:
     groups: dict = field(default_factory=dict)   # extra named groups -> list of species names
     xo: np.ndarray = None
     noise_old: np.ndarray = None
+    # nAngles [ns], angleType [ns, maxangles], angleGroup [ns, maxangles, 3],
+    # k, theta (radians), style per angle type (None: no angles)
+    angles: dict = None
 
     @property
     def Dim(self):
@@ -355,7 +424,7 @@ def tild_system_from_config(cfg, integrator="GJF"):
     return TildSystem(Nx=cfg["Nx"], L=cfg["L"], pmeorder=cfg["pmeorder"], species=names, x=x, topo=t,
                       bond_k=cfg["bond_k"], bond_r0=cfg["bond_r0"], bond_style=cfg["bond_style"],
                       potentials=pots, mass=cfg["mass"], mobility=cfg["mobility"], delt=cfg["delt"],
-                      integrator=integrator, xo=x.copy(), noise_old=np.zeros_like(x))
+                      integrator=integrator, xo=x.copy(), noise_old=np.zeros_like(x), angles=cfg.get("angles"))
 
 
 def bias_field(pot, Nx, L):
@@ -421,6 +490,8 @@ def tild_forces(sys: TildSystem, bits=64, want_energy=False):
             rho[g], _ = fill_density(M, sites, inds, W, G, bits=bits)
     f = np.zeros(sys.ns * Dim, _rdt(bits))
     bonds(f, dict(sys.topo, bondReq=sys.bond_r0, bondK=sys.bond_k, bondStyle=sys.bond_style, maxbonds=2), sys.x, sys.L, bits)
+    if sys.angles is not None:
+        angle_forces(f, sys.angles, sys.x, sys.L)
     energies = []
     for pot in sys.potentials:
         if pot.kind == "wall":

@@ -25,9 +25,16 @@ def write_data_file(path, cfg):
     btype = topo["bondType"].reshape(n, -1)
     bonds = [(int(btype[i, k]) + 1, i + 1, int(bt[i, k]) + 1) for i in range(n) for k in range(nb[i]) if bt[i, k] > i]
     charged = cfg["has_charges"]
+    angles = []
+    ang = cfg.get("angles")
+    if ang is not None:
+        na, ag, at = ang["nAngles"], ang["angleGroup"], ang["angleType"]
+        angles = [(int(at[i, a]) + 1, *(int(v) + 1 for v in ag[i, a])) for i in range(n) for a in range(na[i])
+                  if ag[i, a, 1] == i]
+    nat = len(ang["k"]) if ang is not None else 0
     with open(path, "w") as f:
-        f.write("data file\n\n%d atoms\n%d bonds\n0 angles\n\n" % (n, len(bonds)))
-        f.write("%d atom types\n%d bond types\n0 angle types\n\n" % (len(cfg["species"]), len(cfg["bond_k"])))
+        f.write("data file\n\n%d atoms\n%d bonds\n%d angles\n\n" % (n, len(bonds), len(angles)))
+        f.write("%d atom types\n%d bond types\n%d angle types\n\n" % (len(cfg["species"]), len(cfg["bond_k"]), nat))
         for a, l in zip("xyz", cfg["L"]):
             f.write("0 %r %slo %shi\n" % (float(l), a, a))
         f.write("\nMasses\n\n")
@@ -42,6 +49,10 @@ def write_data_file(path, cfg):
         f.write("\nBonds\n\n")
         for k, (t, i, j) in enumerate(bonds):
             f.write("%d %d %d %d\n" % (k + 1, t, i, j))
+        if angles:
+            f.write("\nAngles\n\n")
+            for k, a in enumerate(angles):
+                f.write("%d %d %d %d %d\n" % ((k + 1,) + a))
 
 
 def ps_input(cfg, data_name, steps, log_freq, integrator="GJF"):
@@ -52,6 +63,12 @@ def ps_input(cfg, data_name, steps, log_freq, integrator="GJF"):
     lines += ["species %s" % s for s in cfg["species"]]
     for k, (kk, r0, st) in enumerate(zip(cfg["bond_k"], cfg["bond_r0"], cfg["bond_style"])):
         lines.append("bond %d %s %r %r" % (k + 1, ["harmonic", "fene"][st], float(kk), float(r0)))
+    ang = cfg.get("angles")
+    if ang is not None:
+        lines.append("MAXANGLES %d" % ang["maxangles"])
+        for t, (k, th, st) in enumerate(zip(ang["k"], ang["theta"], ang["style"])):
+            deg = " %r" % round(float(th) * 180.0 / float(np.float32(3.141592654)), 4) if st == 1 else ""
+            lines.append("angle %d %s %r%s" % (t + 1, ["wlc", "harmonic"][st], float(k), deg))
     lines.append("readData %s" % data_name)
     names = cfg["species"]
     for p in cfg["potentials"]:

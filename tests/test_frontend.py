@@ -197,3 +197,29 @@ def test_wall_potential_keyword(tmp_path):
                      ("potential wall B y 7.5 1 4.0 ramp 5", "ramping applied on an unsupported potential!")):
         r = dry(tmp_path, base.replace("logFreq", bad + "\nlogFreq"))
         assert r.returncode == 1 and msg in r.stdout, (bad, r.stdout[-300:])
+
+
+def test_angle_keywords(tmp_path):
+    # angle type style k [theta] (PS_Box_initialization.cu:83-106), MAXANGLES,
+    # and angleType on linear molecules (PS_Box_moleculeMakers.cu:242-288):
+    # every interior bead of the chains is the vertex of one angle
+    base = open(os.path.join(INPUTS, "c1_diblock.in")).read()
+    text = base.replace("bond 1 harmonic 1.5 0.0", "bond 1 harmonic 1.5 0.0\nangle 1 wlc 2.0\nangle 2 harmonic 3.0 120")
+    text = text.replace("molecule linear nmolecs 2000 2 10 A 10 B",
+                        "molecule linear nmolecs 2000 2 10 A 10 B angleType 1 2")
+    r = dry(tmp_path, text)
+    assert r.returncode == 0, r.stdout[-1000:]
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    assert d["nangles"] == 2000 * 18 and d["maxangles"] == 3
+    assert d["angles"][0] == [0, 2.0, 0.0]
+    assert d["angles"][1][0] == 1 and d["angles"][1][2] == pytest.approx(120 * 3.141592654 / 180, rel=1e-6)
+    for bad, msg in (("angle 1 cosine 2.0", "Angle style not supported!"),):
+        r = dry(tmp_path, base.replace("logFreq", bad + "\nlogFreq"))
+        assert r.returncode == 1 and msg in r.stdout, (bad, r.stdout[-300:])
+    late = base.replace("logFreq", "MAXANGLES 4\nlogFreq")
+    r = dry(tmp_path, late)
+    assert r.returncode == 1 and "MAXANGLES must be set before defining any molecules" in r.stdout
+    undefined = base.replace("molecule linear nmolecs 2000 2 10 A 10 B",
+                             "molecule linear nmolecs 2000 2 10 A 10 B angleType 1 1")
+    r = dry(tmp_path, undefined)
+    assert r.returncode == 1 and "angle type 1 is not defined" in r.stdout

@@ -467,3 +467,32 @@ def test_walls_match_oracle(gpu):
         x_g, _, _ = eng.get_state()
         assert np.max(periodic_dist(x_g, o["x"], cfg["L"])) < 1e-6 * max(cfg["L"]), step
     eng.close()
+
+
+def test_angles_match_oracle(gpu):
+    # d_angles / d_anglesStressEnergy (ps_deviceAngles.cu): WLC and harmonic
+    # angles on the diblock chains beside the pair potentials; forces, the
+    # angle energy and five GJF steps (each re-sorts the slots, so the angle
+    # lists are permuted and remapped every step)
+    from test_oracle_tild import angle_system
+    cfg = angle_system()
+    say("case: angles")
+    eng, sys_o = _engine_vs_oracle(gpu, cfg)
+    f_g = eng.compute_forces()
+    out = orc.tild_forces(sys_o)
+    assert rel(f_g, out["f"]) < TOL
+    ea = eng.angle_energy()
+    eo = orc.angle_energy(cfg["angles"], sys_o.x, cfg["L"])
+    assert ea == pytest.approx(eo, rel=1e-5)
+    ns = cfg["topo"]["type"].size
+    for step in range(5):
+        eng.step(1)
+        xi = orc.tild_noise(ns, cfg["seed"], step)
+        o = orc.tild_step(sys_o, xi)
+        sys_o.x = o["x"].astype(np.float32)
+        sys_o.xo = o["xo"].astype(np.float32)
+        sys_o.noise_old = o["noise_old"].astype(np.float32)
+        x_g, _, _ = eng.get_state()
+        assert np.max(periodic_dist(x_g, o["x"], cfg["L"])) < 1e-6 * max(cfg["L"]), step
+    assert eng.angle_energy() == pytest.approx(orc.angle_energy(cfg["angles"], sys_o.x, cfg["L"]), rel=1e-5)
+    eng.close()

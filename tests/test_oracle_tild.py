@@ -249,3 +249,61 @@ def test_bias_force_is_the_analytic_gradient(phase, direction):
         g[:, 2] = 1.7 * 2 * e * a[2] * (c[2] * c[0] - s[1] * s[2])
     # force = -grad u
     assert np.max(np.abs(f + g)) < 2e-5 * max(1.0, np.abs(g).max())
+
+
+def angle_system(seed=7):
+    """Diblock chains of 10 with an angle at every interior bead: WLC
+    (k = 2) at A vertices, harmonic (k = 3, theta_eq = 100 degrees, converted
+    as the input reader does: float degrees times PI / 180 in double) at B."""
+    cfg = small_system(False, seed=seed)
+    t = cfg["topo"]["type"]
+    th = float(np.float32(100.0 * (float(np.float32(3.141592654)) / 180.0)))
+    return systems.add_chain_angles(cfg, 10, lambda i: int(t[i]), k=[2.0, 3.0], theta=[0.0, th], style=[0, 1])
+
+
+def test_angle_forces_are_minus_energy_gradient():
+    # d_angles (ps_deviceAngles.cu:8-145) against central differences of the
+    # energy d_anglesStressEnergy / 3 (PS_Box.cu:296-305) in FP64: a wrong
+    # sign, factor or member role fails here
+    cfg = angle_system()
+    ang, L = cfg["angles"], cfg["L"]
+    x = cfg["x"].astype(np.float64).reshape(-1)
+    f = np.zeros_like(x)
+    orc.angle_forces(f, ang, x, L)
+    rng = np.random.default_rng(3)
+    h = 1e-6
+    for c in rng.choice(x.size, 60, replace=False):
+        xp, xm = x.copy(), x.copy()
+        xp[c] += h
+        xm[c] -= h
+        g = (orc.angle_energy(ang, xp, L) - orc.angle_energy(ang, xm, L)) / (2 * h)
+        assert f[c] == pytest.approx(-g, rel=1e-5, abs=1e-6), c
+    # each angle's forces sum to zero
+    assert np.max(np.abs(f.reshape(-1, 3).sum(axis=0))) < 1e-9
+
+
+def test_angle_closed_forms():
+    # a right angle: WLC k (1 + cos) = k; harmonic at its theta_eq: no energy
+    # and no force; WLC on a straight line: no energy and no force
+    L = (10.0, 10.0, 10.0)
+
+    def one(x, style, k, th):
+        ang = dict(nAngles=np.array([1, 1, 1], np.int32), angleType=np.zeros((3, 1), np.int32),
+                   angleGroup=np.tile(np.array([0, 1, 2], np.int32), (3, 1, 1)), maxangles=1,
+                   k=[k], theta=[th], style=[style])
+        f = np.zeros(9)
+        orc.angle_forces(f, ang, np.asarray(x, np.float64).reshape(-1), L)
+        return orc.angle_energy(ang, np.asarray(x, np.float64).reshape(-1), L), f
+
+    right = [[6.0, 5.0, 5.0], [5.0, 5.0, 5.0], [5.0, 6.5, 5.0]]
+    e, f = one(right, 0, 2.5, 0.0)
+    assert e == pytest.approx(2.5)
+    # WLC force on i: -k (rkj / |rij||rkj| - cos rij / |rij|^2) = -k (0, 1, 0)
+    np.testing.assert_allclose(f[0:3], [0.0, -2.5, 0.0], atol=1e-12)
+    e, f = one(right, 1, 4.0, np.pi / 2)
+    assert abs(e) < 1e-12 and np.max(np.abs(f)) < 1e-6
+    e, f = one([[4.0, 5.0, 5.0], [5.0, 5.0, 5.0], [6.2, 5.0, 5.0]], 0, 2.0, 0.0)
+    assert abs(e) < 1e-12 and np.max(np.abs(f)) < 1e-12
+    # across the periodic boundary: the minimum image is used
+    e2, _ = one([[9.5, 5.0, 5.0], [0.5, 5.0, 5.0], [0.5, 6.0, 5.0]], 0, 2.5, 0.0)
+    assert e2 == pytest.approx(2.5)
