@@ -140,7 +140,10 @@ int mft_tild_gjf_step(float* x, float* xo, float* noise_old, const float* f, con
 int mft_tild_em_step(float* x, const float* f, int nparticles, float delt, const float L[3], uint64_t seed,
                      int64_t step, void* stream);
 /* Integrators (ps_integrator.cu:51-69). */
-enum { MFT_INT_GJF = 0, MFT_INT_EM = 1, MFT_INT_NONE = 2 };
+/* MFT_INT_VV: velocity Verlet (ps_integratorVV.cu); the velocities travel in
+ * the noise_old slot of mft_tild_set_state / mft_tild_get_state, start at zero
+ * and, between steps, hold the half kick of the last step */
+enum { MFT_INT_GJF = 0, MFT_INT_EM = 1, MFT_INT_NONE = 2, MFT_INT_VV = 3 };
 
 typedef struct {
     int kind;          /* MFT_POT_* */

@@ -145,6 +145,9 @@ static int tild_setup(TildEngine* e)
         e->bp.r0[i] = d.bond_r0[i];
         e->bp.style[i] = d.bond_style[i];
     }
+    MFT_REQUIRE(d.integrator == MFT_INT_GJF || d.integrator == MFT_INT_EM || d.integrator == MFT_INT_NONE ||
+                    d.integrator == MFT_INT_VV,
+                MFT_ERR_INVALID, "unknown integrator %d", d.integrator);
     std::memset(&e->gp, 0, sizeof(e->gp));
     for (int s = 0; s < d.nspecies; ++s) {
         float m = e->mass[s], D = e->mob[s], dt = d.delt;
@@ -155,6 +158,7 @@ static int tild_setup(TildEngine* e)
         e->gp.nmag[s] = sqrtf(2.0f * dt) / sqrtf(D);
         e->gp.cf[s] = b * (dt * dt) / m;
         e->gp.cn[s] = b * dt / (2.0f * m);
+        e->gp.inv_m[s] = 1.0f / m;
     }
 
     const int N = e->N, M = e->M;
@@ -470,6 +474,7 @@ static int do_steps(TildEngine* e, int nsteps, bool want_f, bool advance = true)
             if ((rc = tild_permute(e))) return rc;
         }
         if (!advance) continue;
+        if (d.integrator == MFT_INT_VV) e->vv_state = e->vv_state == 0 ? 1 : 2;
         if (e->run_max > 0) update_ramps(e, e->run_pos++, e->run_max);
         else update_ramps(e, i, nsteps);
         e->step++;
@@ -622,6 +627,11 @@ int mft_tild_set_state(mft_tild* e, const float* x, const float* xo, const float
     e->cur = 0;
     MFT_HIP(hipMemcpy(e->pio, x, sizeof(float) * n3, hipMemcpyHostToDevice));
     MFT_HIP(hipMemcpy(e->pio2, xo ? xo : x, sizeof(float) * n3, hipMemcpyHostToDevice));
+    // velocity Verlet starts from rest with no stored force (the reference's
+    // calloc'd velocities and freshly allocated force array)
+    MFT_REQUIRE(!(noise_old && e->d.integrator == MFT_INT_VV), MFT_ERR_INVALID,
+                "velocity Verlet starts from zero velocities: pass no noise_old");
+    e->vv_state = 0;
     if (noise_old) MFT_HIP(hipMemcpy(e->pio3, noise_old, sizeof(float) * n3, hipMemcpyHostToDevice));
     else MFT_HIP(hipMemset(e->pio3, 0, sizeof(float) * n3));
     int rc = tild_sort_state(e);

@@ -101,6 +101,7 @@ struct WallParams {
 
 struct GjfParams {
     float a[kMaxSpecies], b[kMaxSpecies], nmag[kMaxSpecies], cf[kMaxSpecies], cn[kMaxSpecies];
+    float inv_m[kMaxSpecies]; // VV::finishInitialization (ps_integratorVV.cu:15-26)
 };
 
 // Brick decomposition of the grid.  A particle belongs to the brick holding
@@ -205,6 +206,7 @@ struct TildEngine {
     int *nAngles = nullptr, *angleType = nullptr, *angleGroup = nullptr; // particle-id order
     AngleParams ap{};
     int cur = 0;
+    int vv_state = 0; // velocity Verlet: 0 before the first step, 1 after it, 2 afterwards
     // step temporaries in slot order
     float4 *XN = nullptr, *NN = nullptr; // next positions and noise
     float4* fgrid = nullptr;             // grid force per slot (x, y, z, 0)

@@ -880,6 +880,7 @@ struct StepArgs {
     WallParams walls;
     const int *NA, *AT, *AG;   // angles (null without)
     AngleParams ap;
+    int vv;                    // velocity Verlet phase (TildEngine::vv_state)
 };
 
 // Wall forces of ps_potentialWall.cu:92-108 on a particle of species ty
@@ -918,6 +919,31 @@ __device__ __forceinline__ int integrate_slot(int s, float4 xc, int2 meta, const
             else if (v < 0.0f) v += A.L[j];
             xn[j] = v;
             nn[j] = nnj;
+        }
+    } else if (A.integ == MFT_INT_VV) {
+        // The reference's step is Integrate_1 (half kick with the stored
+        // force, drift), forces, Integrate_2 (half kick with the new force;
+        // ps_integratorVV.cu:31-94).  This pass has the forces at the current
+        // positions, so it applies the previous step's Integrate_2, then this
+        // step's Integrate_1; the velocity kept between steps is the half-kicked
+        // one.  Before the first step the stored force is zero and the
+        // velocities are zero, so the first step does not move and only kicks.
+        const float inv_m = A.gp.inv_m[meta.y];
+        const float hd = A.delt * 0.5f;
+        const float4 v4 = A.NZ[s];
+        const float vv0[3] = {v4.x, v4.y, v4.z};
+        for (int j = 0; j < 3; ++j) {
+            float v = vv0[j];
+            if (A.vv == 2) v += hd * inv_m * fo[j];
+            v += hd * inv_m * fo[j];
+            float x = xcv[j];
+            if (A.vv != 0) {
+                x = xcv[j] + (hd * 2.0f) * v;
+                if (x >= A.L[j]) x -= A.L[j];
+                else if (x < 0.0f) x += A.L[j];
+            }
+            xn[j] = x;
+            nn[j] = v;
         }
     } else {
         const float nm = sqrtf(2.0f * A.delt);
@@ -1267,6 +1293,7 @@ static StepArgs step_args(const TildEngine* e)
     A.AT = ang ? e->AT[c] : nullptr;
     A.AG = ang ? e->AG[c] : nullptr;
     A.ap = e->ap;
+    A.vv = e->vv_state;
     return A;
 }
 

@@ -220,6 +220,12 @@ public:
     EM(std::istringstream& iss, PS_Box* box) : Integrator(iss, box) {}
     int engine_kind() const override { return MFT_INT_EM; }
 };
+// velocity Verlet (ps_integratorVV.cu): both half steps in the engine's step
+class VV : public Integrator {
+public:
+    VV(std::istringstream& iss, PS_Box* box) : Integrator(iss, box) {}
+    int engine_kind() const override { return MFT_INT_VV; }
+};
 Integrator* IntegratorFactory(std::istringstream& iss, PS_Box* box);
 
 // PS_Compute (ps_compute.cu:8-43): "compute avg_sk group [wait W] [freq F]

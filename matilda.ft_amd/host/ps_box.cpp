@@ -219,7 +219,7 @@ Integrator* IntegratorFactory(std::istringstream& iss, PS_Box* box)
     iss.seekg(pos);
     if (s1 == "EM") return new EM(iss, box);
     if (s1 == "GJF") return new GJF(iss, box);
-    if (s1 == "VV") die("VV is not part of this build's particle-mesh path");
+    if (s1 == "VV") return new VV(iss, box);
     die(s1 + " is not a valid Integrator");
 }
 

@@ -26,7 +26,7 @@ _f = C.c_float
 MFT_OK, MFT_ERR_INVALID, MFT_ERR_OOM, MFT_ERR_NONFINITE, MFT_ERR_DEVICE, MFT_ERR_UNSUPPORTED = range(6)
 POT = {"gaussian": 0, "erf2": 1, "erfG": 2, "charges": 3, "bias": 4, "wall": 5}
 BIAS_PHASE = {"L": 0, "H": 1, "G": 2}
-INTEGRATOR = {"GJF": 0, "EM": 1, "none": 2}
+INTEGRATOR = {"GJF": 0, "EM": 1, "none": 2, "VV": 3}
 TK_NAMES = ["bin", "scan", "scatter", "density", "fft_x_fwd", "fft_y_fwd", "conv_z", "fft_y_inv",
             "fft_x_inv", "gather", "integrate"]
 FTS_POT = {"Helfand": 0, "Flory": 1, "Incompress": 2, "Edwards": 3}

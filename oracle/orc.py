@@ -354,6 +354,10 @@ class TildSystem:
     # nAngles [ns], angleType [ns, maxangles], angleGroup [ns, maxangles, 3],
     # k, theta (radians), style per angle type (None: no angles)
     angles: dict = None
+    # velocity Verlet state: velocities and the force array of the last step
+    # [ns*Dim] (None: zero, as the reference allocates them)
+    v: np.ndarray = None
+    f_prev: np.ndarray = None
 
     @property
     def Dim(self):
@@ -578,8 +582,10 @@ def tild_noise(ns, seed, step, Dim=3):
     return np.ascontiguousarray(z[:, :Dim]).reshape(-1)
 
 
-def tild_step(sys: TildSystem, xi, bits=64):
-    """One PS_Box::doTimeStep with the GJF or EM integrator on group 'all'."""
+def tild_step(sys: TildSystem, xi=None, bits=64):
+    """One PS_Box::doTimeStep with the GJF, EM or VV integrator on group 'all'."""
+    if sys.integrator == "VV":
+        return _vv_step(sys, bits)
     out = tild_forces(sys, bits)
     f = out["f"]
     sites = np.arange(sys.ns, dtype=np.int32)
@@ -594,6 +600,31 @@ def tild_step(sys: TildSystem, xi, bits=64):
     else:
         lib(bits).orc_em_step(_ptr(x), _ptr(f), sys.delt, _ptr(Lf), _ptr(sites), sys.ns, sys.Dim, _ptr(np.ascontiguousarray(xi, np.float64)))
         out.update(x=x)
+    return out
+
+
+def _vv_step(sys: TildSystem, bits=64):
+    """PS_Box::doTimeStep (PS_Box.cu:73-207) with VV (ps_integratorVV.cu:15-94):
+    Integrate_1 -- v += (dt/2) f_prev / m, x += dt v, wrapped into [0, L) --
+    then the forces at the new positions, then Integrate_2 -- v += (dt/2) f / m.
+    Positions are rounded to FP32 before the forces, as the device stores them."""
+    D = sys.Dim
+    t = sys.topo["intSpecies"]
+    inv_m = np.array([float(np.float32(1.0) / np.float32(m)) for m in sys.mass], np.float64)[t][:, None]
+    hd = float(np.float32(sys.delt) * np.float32(0.5))
+    L = np.asarray(_fa(sys.L), np.float64)
+    v = np.zeros((sys.ns, D)) if sys.v is None else np.asarray(sys.v, np.float64).reshape(-1, D).copy()
+    fp = np.zeros((sys.ns, D)) if sys.f_prev is None else np.asarray(sys.f_prev, np.float64).reshape(-1, D)
+    v += hd * inv_m * fp
+    x = np.asarray(sys.x, np.float64).reshape(-1, D) + (2.0 * hd) * v
+    x = np.where(x >= L, x - L, np.where(x < 0.0, x + L, x))
+    x0 = sys.x
+    sys.x = x.astype(np.float32).reshape(-1)
+    out = tild_forces(sys, bits)
+    sys.x = x0
+    f = out["f"].reshape(-1, D).astype(np.float64)
+    v += hd * inv_m * f
+    out.update(x=x.reshape(-1), v=v.reshape(-1), f_prev=f.reshape(-1))
     return out
 
 

@@ -223,3 +223,14 @@ def test_angle_keywords(tmp_path):
                              "molecule linear nmolecs 2000 2 10 A 10 B angleType 1 1")
     r = dry(tmp_path, undefined)
     assert r.returncode == 1 and "angle type 1 is not defined" in r.stdout
+
+
+def test_vv_integrator_keyword(tmp_path):
+    # ps_integrator.cu:52-68: VV is one of the three integrators
+    base = open(os.path.join(INPUTS, "c1_diblock.in")).read()
+    r = dry(tmp_path, base.replace("integrator all GJF delt 0.005", "integrator all VV delt 0.002"))
+    assert r.returncode == 0, r.stdout[-1000:]
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    assert d["integrator"] == 3 and d["delt"] == pytest.approx(0.002)
+    r = dry(tmp_path, base.replace("integrator all GJF delt 0.005", "integrator all Langevin"))
+    assert r.returncode == 1 and "Langevin is not a valid Integrator" in r.stdout

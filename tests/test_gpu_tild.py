@@ -496,3 +496,33 @@ def test_angles_match_oracle(gpu):
         assert np.max(periodic_dist(x_g, o["x"], cfg["L"])) < 1e-6 * max(cfg["L"]), step
     assert eng.angle_energy() == pytest.approx(orc.angle_energy(cfg["angles"], sys_o.x, cfg["L"]), rel=1e-5)
     eng.close()
+
+
+def test_velocity_verlet_matches_oracle(gpu):
+    # VV (ps_integratorVV.cu) from rest: the engine fuses the previous step's
+    # second half kick into the next step, so its velocities between steps are
+    # the oracle's less (dt/2) f / m; positions agree step by step
+    cfg = small_system(False, seed=14)
+    say("case: velocity Verlet")
+    eng = systems.tild_engine(cfg, "VV")
+    sys_o = oracle_system(cfg, "VV")
+    t = cfg["topo"]["type"]
+    inv_m = np.array([float(np.float32(1.0) / np.float32(m)) for m in cfg["mass"]])[t][:, None]
+    hd = float(np.float32(cfg["delt"]) * np.float32(0.5))
+    x_first = None
+    for step in range(6):
+        eng.step(1)
+        o = orc.tild_step(sys_o)
+        sys_o.x = o["x"].astype(np.float32)
+        sys_o.v, sys_o.f_prev = o["v"], o["f_prev"]
+        x_g, _, v_g = eng.get_state()
+        assert np.max(periodic_dist(x_g, o["x"], cfg["L"])) < 1e-6 * max(cfg["L"]), step
+        v_o = o["v"].reshape(-1, 3)
+        if step > 0:
+            v_o = v_o - hd * inv_m * o["f_prev"].reshape(-1, 3)
+        assert rel(v_g.reshape(-1, 3), v_o) < 1e-4, step
+        if step == 0:
+            x_first = x_g
+    # the first step from rest does not move the particles
+    assert np.array_equal(x_first, cfg["x"].astype(np.float32).reshape(x_first.shape))
+    eng.close()

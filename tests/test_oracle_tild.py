@@ -307,3 +307,49 @@ def test_angle_closed_forms():
     # across the periodic boundary: the minimum image is used
     e2, _ = one([[9.5, 5.0, 5.0], [0.5, 5.0, 5.0], [0.5, 6.0, 5.0]], 0, 2.5, 0.0)
     assert e2 == pytest.approx(2.5)
+
+
+def _dimer(v0=None, k=1.5):
+    """Two particles joined by a harmonic bond (k (r - 0)^2) and no grid
+    potentials, in a 10^3 box."""
+    x = np.array([4.0, 5.0, 5.0, 5.2, 5.3, 4.9], np.float32)
+    topo = dict(intSpecies=np.array([0, 0], np.int32), nBonds=np.array([1, 1], np.int32),
+                bondedTo=np.array([1, 0, 0, 0], np.int32), bondType=np.zeros(4, np.int32),
+                charge=np.zeros(2, np.float32))
+    return orc.TildSystem(Nx=(8, 8, 8), L=(10.0, 10.0, 10.0), pmeorder=1, species=["A"], x=x, topo=topo,
+                          bond_k=[k], bond_r0=[0.0], bond_style=[0], potentials=[], mass=[2.0], mobility=[1.0],
+                          delt=0.01, integrator="VV", v=v0)
+
+
+def test_vv_free_flight_and_first_step():
+    # no forces: x_n = x_0 + n dt v_0, wrapped into [0, L); from rest with no
+    # stored force the first step does not move the particles
+    sys = _dimer(v0=np.array([30.0, -1.0, 0.5, -2.0, 40.0, 0.0]), k=0.0)
+    x0 = sys.x.astype(np.float64).copy()
+    for n in range(1, 4):
+        o = orc.tild_step(sys)
+        sys.x, sys.v, sys.f_prev = o["x"].astype(np.float32), o["v"], o["f_prev"]
+        want = (x0 + n * 0.01 * np.array([30.0, -1.0, 0.5, -2.0, 40.0, 0.0])) % 10.0
+        np.testing.assert_allclose(sys.x, want, atol=2e-6)
+    rest = _dimer()
+    o = orc.tild_step(rest)
+    np.testing.assert_array_equal(o["x"], rest.x.astype(np.float64))
+    assert np.all(o["v"] != 0.0)
+
+
+def test_vv_is_time_reversible():
+    # velocity Verlet is reversible: N steps, flip the velocities, N steps
+    # back to the start (a kick in the wrong place breaks this); the stored
+    # force starts consistent with the positions
+    sys = _dimer(v0=np.array([0.3, 0.0, -0.2, -0.3, 0.1, 0.2]))
+    sys.f_prev = orc.tild_forces(sys)["f"]
+    x0 = sys.x.astype(np.float64).copy()
+    xs = sys.x.astype(np.float64)
+    for sign in (1, -1):
+        if sign < 0:
+            sys.v = -sys.v
+        for _ in range(25):
+            o = orc.tild_step(sys)
+            xs, sys.v, sys.f_prev = o["x"], o["v"], o["f_prev"]
+            sys.x = xs        # FP64 positions for this property
+    np.testing.assert_allclose(xs, x0, atol=1e-9)
