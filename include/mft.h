@@ -97,8 +97,13 @@ int mft_tild_bonds(float* f, const int* nBonds, const int* bondedTo, const int* 
                    const float* x, const float L[3], int ns, int maxbonds, void* stream);
 
 /* Potential kinds (ps_potential.cu:294-333). */
+/* MFT_POT_LANGEVIN (Langevin, ps_potentialLangevin.cu): drag and random
+ * forces on group maskI, f += -v A + sqrtf(2 A / sigma) N(0, 1) with A the
+ * drag and sigma the time step of the noise; v is the velocity under
+ * MFT_INT_VV and zero otherwise (the reference's never-updated velocities);
+ * its energy is zero */
 enum { MFT_POT_GAUSSIAN = 0, MFT_POT_ERF2 = 1, MFT_POT_ERFG = 2, MFT_POT_CHARGES = 3, MFT_POT_BIAS = 4,
-       MFT_POT_WALL = 5 };
+       MFT_POT_WALL = 5, MFT_POT_LANGEVIN = 6 };
 /* Phases of the static bias field (Box::make_bias_field, Box.cu:308-458):
  * lamellar A sin(2 pi n x_dir / L_dir), hexagonal cylinders along dir
  * A cos(.) cos(.), gyroid A ((sin x cos y + sin y cos z + sin z cos x)^2 - 1.44).

@@ -89,6 +89,15 @@ static int tild_setup(TildEngine* e)
             e->walls.pos[w] = P.wall_pos;
             e->walls.k[w] = P.A;
             e->wall_pot.push_back(i);
+        } else if (P.kind == MFT_POT_LANGEVIN) {
+            MFT_REQUIRE(P.maskI != 0 && P.sigma > 0.0f, MFT_ERR_INVALID,
+                        "ps-potentialLangevin.cu: did not find appropriate delt for the Langevin noise");
+            MFT_REQUIRE(e->lang.n < kMaxWalls, MFT_ERR_UNSUPPORTED, "at most %d Langevin potentials", kMaxWalls);
+            const int l = e->lang.n++;
+            e->lang.mask[l] = P.maskI;
+            e->lang.drag[l] = P.A;
+            // noise_mag = sqrtf(2.0 * drag / delt) (ps_potentialLangevin.cu:64)
+            e->lang.nmag[l] = sqrtf((float)(2.0 * (double)P.A / (double)P.sigma));
         } else if (P.kind == MFT_POT_BIAS) {
             MFT_REQUIRE(P.phase >= MFT_BIAS_LAM && P.phase <= MFT_BIAS_GYR, MFT_ERR_UNSUPPORTED,
                         "PHASE NOT FOUND in Box.cu");
@@ -110,7 +119,7 @@ static int tild_setup(TildEngine* e)
     for (int i = 0; i < d.npotentials; ++i) {
         const mft_tild_potential& P = e->pots[i];
         KPot& K = kp.pot[i];
-        if (P.kind == MFT_POT_BIAS || P.kind == MFT_POT_WALL) {
+        if (P.kind == MFT_POT_BIAS || P.kind == MFT_POT_WALL || P.kind == MFT_POT_LANGEVIN) {
             // not a pair potential: a no-op entry in the k-space pass (zero
             // amplitude, no species); a bias field is added to the force
             // fields after it, a wall to the particle forces in the integrate pass
@@ -148,6 +157,7 @@ static int tild_setup(TildEngine* e)
     MFT_REQUIRE(d.integrator == MFT_INT_GJF || d.integrator == MFT_INT_EM || d.integrator == MFT_INT_NONE ||
                     d.integrator == MFT_INT_VV,
                 MFT_ERR_INVALID, "unknown integrator %d", d.integrator);
+    e->is_vv = d.integrator == MFT_INT_VV;
     std::memset(&e->gp, 0, sizeof(e->gp));
     for (int s = 0; s < d.nspecies; ++s) {
         float m = e->mass[s], D = e->mob[s], dt = d.delt;
@@ -267,7 +277,7 @@ static int configure_channels(TildEngine* e, const int* type, const float* charg
     uint32_t forced = 0;
     for (int i = 0; i < d.npotentials; ++i) {
         const mft_tild_potential& P = e->pots[i];
-        if (P.kind == MFT_POT_WALL) continue; // per-particle force
+        if (P.kind == MFT_POT_WALL || P.kind == MFT_POT_LANGEVIN) continue; // per-particle force
         if (P.kind != MFT_POT_CHARGES) forced |= P.maskI | (P.kind == MFT_POT_BIAS ? 0u : P.maskJ);
     }
     if (e->charge_pot >= 0 && fold)
@@ -474,7 +484,7 @@ static int do_steps(TildEngine* e, int nsteps, bool want_f, bool advance = true)
             if ((rc = tild_permute(e))) return rc;
         }
         if (!advance) continue;
-        if (d.integrator == MFT_INT_VV) e->vv_state = e->vv_state == 0 ? 1 : 2;
+        if (d.integrator == MFT_INT_VV) e->vv_state = 1;
         if (e->run_max > 0) update_ramps(e, e->run_pos++, e->run_max);
         else update_ramps(e, i, nsteps);
         e->step++;

@@ -99,6 +99,13 @@ struct WallParams {
     float pos[kMaxWalls], k[kMaxWalls];
 };
 
+// Langevin potentials (ps_potentialLangevin.cu:60-133), up to kMaxWalls
+struct LangevinParams {
+    int n;
+    uint32_t mask[kMaxWalls];
+    float drag[kMaxWalls], nmag[kMaxWalls];
+};
+
 struct GjfParams {
     float a[kMaxSpecies], b[kMaxSpecies], nmag[kMaxSpecies], cf[kMaxSpecies], cn[kMaxSpecies];
     float inv_m[kMaxSpecies]; // VV::finishInitialization (ps_integratorVV.cu:15-26)
@@ -206,7 +213,7 @@ struct TildEngine {
     int *nAngles = nullptr, *angleType = nullptr, *angleGroup = nullptr; // particle-id order
     AngleParams ap{};
     int cur = 0;
-    int vv_state = 0; // velocity Verlet: 0 before the first step, 1 after it, 2 afterwards
+    int vv_state = 0; // velocity Verlet: 0 before the first step, 1 afterwards
     // step temporaries in slot order
     float4 *XN = nullptr, *NN = nullptr; // next positions and noise
     float4* fgrid = nullptr;             // grid force per slot (x, y, z, 0)
@@ -245,6 +252,8 @@ struct TildEngine {
     };
     std::vector<Bias> bias;
     WallParams walls{};
+    LangevinParams lang{};
+    bool is_vv = false; // created with MFT_INT_VV (velocities live in the noise slot)
     std::vector<int> wall_pot;  // potential index of each wall
     double* epart = nullptr;
     double* eout = nullptr;   // [kMaxPots + 1]

@@ -880,7 +880,9 @@ struct StepArgs {
     WallParams walls;
     const int *NA, *AT, *AG;   // angles (null without)
     AngleParams ap;
-    int vv;                    // velocity Verlet phase (TildEngine::vv_state)
+    int vv;                    // velocity Verlet: 0 before the first step (TildEngine::vv_state)
+    int use_v;                 // the noise slot holds velocities (VV)
+    LangevinParams lang;
 };
 
 // Wall forces of ps_potentialWall.cu:92-108 on a particle of species ty
@@ -891,6 +893,20 @@ __device__ __forceinline__ void wall_forces(const WallParams& W, int ty, float4 
         const int a = W.axis[w];
         const float disp = W.pos[w] - (a == 0 ? xc.x : (a == 1 ? xc.y : xc.z));
         if ((float)W.sign[w] * disp > 0.0f) fo[a] += W.k[w] * disp;
+    }
+}
+
+// d_add_Langevin_forces (ps_potentialLangevin.cu:101-133) on particle id of
+// species ty with velocity v: one normal per axis, keyed by (particle, step,
+// 'LANG', potential + 1)
+__device__ __forceinline__ void langevin_forces(const LangevinParams& P, int id, int ty, const float* v,
+                                                uint64_t seed, uint32_t step, float* fo)
+{
+    for (int l = 0; l < P.n; ++l) {
+        if (!((P.mask[l] >> ty) & 1u)) continue;
+        float z[4];
+        normal4((uint32_t)id, step, TAG_LANGEVIN, (uint32_t)(l + 1), seed, z);
+        for (int j = 0; j < 3; ++j) fo[j] += -v[j] * P.drag[l] + P.nmag[l] * z[j];
     }
 }
 
@@ -921,23 +937,23 @@ __device__ __forceinline__ int integrate_slot(int s, float4 xc, int2 meta, const
             nn[j] = nnj;
         }
     } else if (A.integ == MFT_INT_VV) {
-        // The reference's step is Integrate_1 (half kick with the stored
-        // force, drift), forces, Integrate_2 (half kick with the new force;
-        // ps_integratorVV.cu:31-94).  This pass has the forces at the current
-        // positions, so it applies the previous step's Integrate_2, then this
-        // step's Integrate_1; the velocity kept between steps is the half-kicked
-        // one.  Before the first step the stored force is zero and the
-        // velocities are zero, so the first step does not move and only kicks.
+        // The reference's step k is Integrate_1 (half kick with the stored
+        // force, drift), the forces F_k at the new positions, Integrate_2
+        // (half kick with F_k; ps_integratorVV.cu:31-94).  Step k of this pass
+        // has the forces at its current positions, which are F_(k-1): it
+        // applies step k-1's Integrate_2, then step k's Integrate_1, and keeps
+        // the half-kicked velocity.  Step 0 starts from rest with the zero force
+        // array of a fresh allocation: neither kick nor drift changes anything.
         const float inv_m = A.gp.inv_m[meta.y];
         const float hd = A.delt * 0.5f;
         const float4 v4 = A.NZ[s];
         const float vv0[3] = {v4.x, v4.y, v4.z};
         for (int j = 0; j < 3; ++j) {
             float v = vv0[j];
-            if (A.vv == 2) v += hd * inv_m * fo[j];
-            v += hd * inv_m * fo[j];
             float x = xcv[j];
             if (A.vv != 0) {
+                v += hd * inv_m * fo[j];
+                v += hd * inv_m * fo[j];
                 x = xcv[j] + (hd * 2.0f) * v;
                 if (x >= A.L[j]) x -= A.L[j];
                 else if (x < 0.0f) x += A.L[j];
@@ -979,6 +995,20 @@ __global__ void __launch_bounds__(256) k_integrate(
         bond_force_slots(s, ns, X, NB, BS, BT, maxbonds, bp, A.L[0], A.L[1], A.L[2], xc.x, xc.y, xc.z, fo);
         if (A.NA) angle_force_slots(s, X, A.NA, A.AT, A.AG, A.ap, A.L[0], A.L[1], A.L[2], fo);
         if (A.walls.n) wall_forces(A.walls, META[s].y, xc, fo);
+        if (A.lang.n) {
+            // the velocities the forces see: VV's (the half-kicked ones the
+            // reference holds at force time), zero under GJF and EM
+            float v[3] = {0.f, 0.f, 0.f};
+            if (A.use_v) {
+                const float4 v4 = A.NZ[s];
+                v[0] = v4.x;
+                v[1] = v4.y;
+                v[2] = v4.z;
+            }
+            // under VV these are the forces of the previous step (its noise)
+            const uint32_t ls = (A.use_v && A.vv != 0) ? A.step - 1u : A.step;
+            langevin_forces(A.lang, META[s].x, META[s].y, v, A.seed, ls, fo);
+        }
         if (ftot) ftot[s] = make_float4(fo[0], fo[1], fo[2], 0.f);
         if (A.integ != MFT_INT_NONE) k = integrate_slot(s, xc, META[s], fo, A, tg);
     }
@@ -1294,6 +1324,8 @@ static StepArgs step_args(const TildEngine* e)
     A.AG = ang ? e->AG[c] : nullptr;
     A.ap = e->ap;
     A.vv = e->vv_state;
+    A.lang = e->lang;
+    A.use_v = e->is_vv ? 1 : 0;
     return A;
 }
 

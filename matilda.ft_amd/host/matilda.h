@@ -180,6 +180,12 @@ public:
     int normalDim = 0, dirSign = 1;
     float wallPos = 0.f, k = 0.f;
 };
+class Langevin : public PS_Potential {
+public:
+    Langevin(std::istringstream& iss, PS_Box* box); // ps_potentialLangevin.cu:16-57
+    mft_tild_potential descriptor() const override;
+    float drag = 1.0f, delt = -5.0f;
+};
 class BiasField : public PS_Potential {
 public:
     BiasField(std::istringstream& iss, PS_Box* box); // ps_potentialBias.cu:17-66

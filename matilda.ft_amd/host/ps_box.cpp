@@ -183,6 +183,30 @@ mft_tild_potential Wall::descriptor() const
     return d;
 }
 
+Langevin::Langevin(std::istringstream& iss, PS_Box* box) : PS_Potential(iss, box)
+{
+    iss >> grpI;
+    grpJ = grpI;
+    // the noise's time step defaults to the first integrator's
+    if (!mybox->integrators.empty()) delt = mybox->integrators[0]->delt;
+    std::string s1;
+    while (iss >> s1) {
+        if (s1 == "drag") iss >> drag;
+        else if (s1 == "delt") iss >> delt;
+        else die(s1 + " is not a valid initialize option in fts_potential");
+    }
+    if (delt < 0.0f) die("ps-potentialLangevin.cu: did not find appropriate delt for the Langevin noise");
+}
+mft_tild_potential Langevin::descriptor() const
+{
+    mft_tild_potential d{};
+    d.kind = MFT_POT_LANGEVIN;
+    d.maskI = mybox->groupMask(grpI);
+    d.A = drag;
+    d.sigma = delt;
+    return d;
+}
+
 PS_Potential* PSPotentialFactory(std::istringstream& iss, PS_Box* box)
 {
     std::string s1;
@@ -193,7 +217,8 @@ PS_Potential* PSPotentialFactory(std::istringstream& iss, PS_Box* box)
     if (s1 == "gaussian") return new NBGauss(iss, box);
     if (s1 == "bias") return new BiasField(iss, box);
     if (s1 == "wall") return new Wall(iss, box);
-    if (s1 == "bias_lc" || s1 == "Langevin" || s1 == "langevin" || s1 == "dpd" ||
+    if (s1 == "Langevin" || s1 == "langevin") return new Langevin(iss, box);
+    if (s1 == "bias_lc" || s1 == "dpd" ||
         s1 == "maiersaupe")
         die("ps_potential: " + s1 + " is not part of this build's particle-mesh path");
     die("ps_potential.cu: " + s1 + " is not a supported potential");

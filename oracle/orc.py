@@ -358,6 +358,8 @@ class TildSystem:
     # [ns*Dim] (None: zero, as the reference allocates them)
     v: np.ndarray = None
     f_prev: np.ndarray = None
+    seed: int = 0        # Langevin noise key, with the step below
+    step: int = 0
 
     @property
     def Dim(self):
@@ -405,6 +407,11 @@ def tild_system_from_config(cfg, integrator="GJF"):
         if p["kind"] == "charges":
             pots.append(TildPotential("charges", A=p["A"], sigma=p["sigma"]))
             continue
+        if p["kind"] == "langevin":
+            gI = [n for i, n in enumerate(names) if (p["maskI"] >> i) & 1]
+            assert len(gI) == 1
+            pots.append(TildPotential("langevin", gI[0], gI[0], A=p["A"], sigma=p["sigma"]))
+            continue
         if p["kind"] == "wall":
             gI = [n for i, n in enumerate(names) if (p["maskI"] >> i) & 1]
             assert len(gI) == 1
@@ -428,7 +435,8 @@ def tild_system_from_config(cfg, integrator="GJF"):
     return TildSystem(Nx=cfg["Nx"], L=cfg["L"], pmeorder=cfg["pmeorder"], species=names, x=x, topo=t,
                       bond_k=cfg["bond_k"], bond_r0=cfg["bond_r0"], bond_style=cfg["bond_style"],
                       potentials=pots, mass=cfg["mass"], mobility=cfg["mobility"], delt=cfg["delt"],
-                      integrator=integrator, xo=x.copy(), noise_old=np.zeros_like(x), angles=cfg.get("angles"))
+                      integrator=integrator, xo=x.copy(), noise_old=np.zeros_like(x), angles=cfg.get("angles"),
+                      seed=cfg["seed"])
 
 
 def bias_field(pot, Nx, L):
@@ -497,7 +505,25 @@ def tild_forces(sys: TildSystem, bits=64, want_energy=False):
     if sys.angles is not None:
         angle_forces(f, sys.angles, sys.x, sys.L)
     energies = []
+    nlang = 0
     for pot in sys.potentials:
+        if pot.kind == "langevin":
+            # d_add_Langevin_forces (ps_potentialLangevin.cu:60-133): -v drag +
+            # sqrtf(2 drag / delt) N(0, 1), normals keyed (particle, step,
+            # 'LANG', potential + 1); v is VV's velocity, zero otherwise
+            nlang += 1
+            sites = np.asarray(sys.group_sites(pot.I))
+            drag = float(np.float32(pot.A))
+            nmag = float(np.sqrt(np.float32(2.0 * drag / float(np.float32(pot.sigma)))).astype(np.float32))
+            z = normal4_np(sites.astype(np.uint64), sys.step, TAG_LANGEVIN, nlang, sys.seed)[:, :Dim]
+            v = np.zeros((sites.size, Dim))
+            if sys.integrator == "VV" and sys.v is not None:
+                v = np.asarray(sys.v, np.float64).reshape(-1, Dim)[sites]
+            fr = f.reshape(-1, Dim)
+            fr[sites] += -v * drag + nmag * z
+            if want_energy:
+                energies.append(0.0)
+            continue
         if pot.kind == "wall":
             # Wall (ps_potentialWall.cu:92-147): k (pos - x_a) where sign (pos - x_a) > 0
             sites = np.asarray(sys.group_sites(pot.I))
@@ -572,6 +598,7 @@ def normal4_np(c0, c1, c2, c3, seed):
 
 
 TAG_TILD = 0x54494C44  # 'TILD'
+TAG_LANGEVIN = 0x4C414E47  # 'LANG'
 TAG_FTS = 0x4654534E   # 'FTSN'
 
 
@@ -618,10 +645,11 @@ def _vv_step(sys: TildSystem, bits=64):
     v += hd * inv_m * fp
     x = np.asarray(sys.x, np.float64).reshape(-1, D) + (2.0 * hd) * v
     x = np.where(x >= L, x - L, np.where(x < 0.0, x + L, x))
-    x0 = sys.x
+    x0, v0 = sys.x, sys.v
     sys.x = x.astype(np.float32).reshape(-1)
+    sys.v = v.reshape(-1)          # the velocities the forces see (Langevin drag)
     out = tild_forces(sys, bits)
-    sys.x = x0
+    sys.x, sys.v = x0, v0
     f = out["f"].reshape(-1, D).astype(np.float64)
     v += hd * inv_m * f
     out.update(x=x.reshape(-1), v=v.reshape(-1), f_prev=f.reshape(-1))

@@ -234,3 +234,24 @@ def test_vv_integrator_keyword(tmp_path):
     assert d["integrator"] == 3 and d["delt"] == pytest.approx(0.002)
     r = dry(tmp_path, base.replace("integrator all GJF delt 0.005", "integrator all Langevin"))
     assert r.returncode == 1 and "Langevin is not a valid Integrator" in r.stdout
+
+
+def test_langevin_potential_keyword(tmp_path):
+    # ps_potentialLangevin.cu:16-57: group [drag d] [delt dt], delt defaulting
+    # to the integrator's, with its die texts
+    base = open(os.path.join(INPUTS, "c1_diblock.in")).read()
+    text = base.replace("integrator all GJF delt 0.005", "integrator all VV delt 0.004")
+    r = dry(tmp_path, text.replace("logFreq", "potential Langevin A drag 0.5\nlogFreq"))
+    assert r.returncode == 0, r.stdout[-1000:]
+    p = json.loads(r.stdout.strip().splitlines()[-1])["potentials"][-1]
+    assert (p["kind"], p["maskI"]) == (6, 1)
+    assert p["A"] == pytest.approx(0.5) and p["sigma"] == pytest.approx(0.004)
+    r = dry(tmp_path, text.replace("logFreq", "potential langevin B delt 0.01\nlogFreq"))
+    p = json.loads(r.stdout.strip().splitlines()[-1])["potentials"][-1]
+    assert p["A"] == pytest.approx(1.0) and p["sigma"] == pytest.approx(0.01) and p["maskI"] == 2
+    r = dry(tmp_path, text.replace("logFreq", "potential Langevin A friction 2\nlogFreq"))
+    assert r.returncode == 1 and "friction is not a valid initialize option in fts_potential" in r.stdout
+    # before any integrator: no delt to default to
+    early = base.replace("potential gaussian A A", "potential Langevin A\npotential gaussian A A")
+    r = dry(tmp_path, early)
+    assert r.returncode == 1 and "did not find appropriate delt for the Langevin noise" in r.stdout

@@ -499,9 +499,9 @@ def test_angles_match_oracle(gpu):
 
 
 def test_velocity_verlet_matches_oracle(gpu):
-    # VV (ps_integratorVV.cu) from rest: the engine fuses the previous step's
-    # second half kick into the next step, so its velocities between steps are
-    # the oracle's less (dt/2) f / m; positions agree step by step
+    # VV (ps_integratorVV.cu) from rest: the engine fuses each step's second
+    # half kick into the next step, so its velocities between steps are the
+    # oracle's less (dt/2) f / m; positions agree step by step
     cfg = small_system(False, seed=14)
     say("case: velocity Verlet")
     eng = systems.tild_engine(cfg, "VV")
@@ -517,12 +517,44 @@ def test_velocity_verlet_matches_oracle(gpu):
         sys_o.v, sys_o.f_prev = o["v"], o["f_prev"]
         x_g, _, v_g = eng.get_state()
         assert np.max(periodic_dist(x_g, o["x"], cfg["L"])) < 1e-6 * max(cfg["L"]), step
-        v_o = o["v"].reshape(-1, 3)
-        if step > 0:
-            v_o = v_o - hd * inv_m * o["f_prev"].reshape(-1, 3)
-        assert rel(v_g.reshape(-1, 3), v_o) < 1e-4, step
+        v_o = o["v"].reshape(-1, 3) - hd * inv_m * o["f_prev"].reshape(-1, 3)
+        if step == 0:
+            assert np.all(v_g == 0.0)
+        else:
+            assert rel(v_g.reshape(-1, 3), v_o) < 1e-4, step
         if step == 0:
             x_first = x_g
     # the first step from rest does not move the particles
     assert np.array_equal(x_first, cfg["x"].astype(np.float32).reshape(x_first.shape))
+    eng.close()
+
+
+@pytest.mark.parametrize("integrator", ["VV", "GJF"])
+def test_langevin_matches_oracle(gpu, integrator):
+    # Langevin (ps_potentialLangevin.cu) on species A beside the pair
+    # potentials: under VV the drag acts on the velocities, under GJF the
+    # reference's velocities stay zero and only the noise acts
+    cfg = small_system(False, seed=15)
+    cfg["potentials"] = list(cfg["potentials"]) + [dict(kind="langevin", maskI=1, maskJ=0, A=0.8, sigma=0.005)]
+    say("case: langevin", integrator)
+    eng = systems.tild_engine(cfg, integrator)
+    sys_o = oracle_system(cfg, integrator)
+    f_g = eng.compute_forces()
+    assert rel(f_g, orc.tild_forces(sys_o)["f"]) < TOL
+    ns = cfg["topo"]["type"].size
+    for step in range(4):
+        eng.step(1)
+        sys_o.step = step
+        if integrator == "VV":
+            o = orc.tild_step(sys_o)
+            sys_o.v, sys_o.f_prev = o["v"], o["f_prev"]
+        else:
+            o = orc.tild_step(sys_o, orc.tild_noise(ns, cfg["seed"], step))
+            sys_o.xo = o["xo"].astype(np.float32)
+            sys_o.noise_old = o["noise_old"].astype(np.float32)
+        sys_o.x = o["x"].astype(np.float32)
+        x_g, _, _ = eng.get_state()
+        assert np.max(periodic_dist(x_g, o["x"], cfg["L"])) < 1e-6 * max(cfg["L"]), step
+    pe, _ = eng.energies()
+    assert pe[-1] == 0.0
     eng.close()

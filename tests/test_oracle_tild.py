@@ -353,3 +353,26 @@ def test_vv_is_time_reversible():
             xs, sys.v, sys.f_prev = o["x"], o["v"], o["f_prev"]
             sys.x = xs        # FP64 positions for this property
     np.testing.assert_allclose(xs, x0, atol=1e-9)
+
+
+def test_langevin_with_vv_thermalizes_to_unit_temperature():
+    # Langevin drag + noise (ps_potentialLangevin.cu) with velocity Verlet on
+    # free particles of mass 2: equipartition m <v^2> = kT = 1 per degree of
+    # freedom (a wrong drag sign, noise magnitude or mass factor fails)
+    rng = np.random.default_rng(4)
+    n = 1500
+    x = (rng.random(3 * n) * 10.0).astype(np.float32)
+    topo = dict(intSpecies=np.zeros(n, np.int32), nBonds=np.zeros(n, np.int32), bondedTo=np.zeros(2 * n, np.int32),
+                bondType=np.zeros(2 * n, np.int32), charge=np.zeros(n, np.float32))
+    sys = orc.TildSystem(Nx=(8, 8, 8), L=(10.0, 10.0, 10.0), pmeorder=1, species=["A"], x=x, topo=topo,
+                         bond_k=[0.0], bond_r0=[0.0], bond_style=[0], mass=[2.0], mobility=[1.0], delt=0.01,
+                         integrator="VV", seed=11,
+                         potentials=[orc.TildPotential("langevin", "A", "A", A=1.0, sigma=0.01)])
+    acc = []
+    for step in range(500):
+        sys.step = step
+        o = orc.tild_step(sys)
+        sys.x, sys.v, sys.f_prev = o["x"].astype(np.float32), o["v"], o["f_prev"]
+        if step >= 250:
+            acc.append(np.mean(2.0 * sys.v ** 2))
+    assert np.mean(acc) == pytest.approx(1.0, rel=0.06)
