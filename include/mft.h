@@ -103,7 +103,14 @@ int mft_tild_bonds(float* f, const int* nBonds, const int* bondedTo, const int* 
  * MFT_INT_VV and zero otherwise (the reference's never-updated velocities);
  * its energy is zero */
 enum { MFT_POT_GAUSSIAN = 0, MFT_POT_ERF2 = 1, MFT_POT_ERFG = 2, MFT_POT_CHARGES = 3, MFT_POT_BIAS = 4,
-       MFT_POT_WALL = 5, MFT_POT_LANGEVIN = 6 };
+       MFT_POT_WALL = 5, MFT_POT_LANGEVIN = 6, MFT_POT_DPD = 7 };
+/* MFT_POT_DPD (DPD, ps_potentialDPD.cu, with the group's neighbor_list,
+ * ps_neighborList.cu): pairwise dissipative and random forces between the
+ * group's particles closer than Rp (the list's rcut), F = (-A w^2 (v_ij . r^)
+ * + sqrtf(2 A sigma) w xi / sqrtf(wall_pos)) r^ with w = 1 - r/Rp, A = gamma,
+ * sigma = kT, wall_pos = the noise's time step; xi is one normal per pair and
+ * step; v as for MFT_POT_LANGEVIN; at most one; Rp at most a third of every box
+ * length; energy zero */
 /* Phases of the static bias field (Box::make_bias_field, Box.cu:308-458):
  * lamellar A sin(2 pi n x_dir / L_dir), hexagonal cylinders along dir
  * A cos(.) cos(.), gyroid A ((sin x cos y + sin y cos z + sin z cos x)^2 - 1.44).

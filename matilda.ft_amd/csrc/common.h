@@ -47,6 +47,7 @@ int last_code();
 // the CPU oracle: u1 = ((r0 >> 8) + 1) 2^-24, u2 = (r1 >> 8) 2^-24.
 constexpr uint32_t TAG_TILD = 0x54494C44u; // 'TILD'
 constexpr uint32_t TAG_LANGEVIN = 0x4C414E47u; // 'LANG'
+constexpr uint32_t TAG_DPD = 0x44504450u;      // 'DPDP'
 constexpr uint32_t TAG_FTS = 0x4654534Eu;  // 'FTSN'
 
 __host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)

@@ -89,6 +89,9 @@ static int tild_setup(TildEngine* e)
             e->walls.pos[w] = P.wall_pos;
             e->walls.k[w] = P.A;
             e->wall_pot.push_back(i);
+        } else if (P.kind == MFT_POT_DPD) {
+            const int rcd = tild_dpd_setup(e, i);
+            if (rcd) return rcd;
         } else if (P.kind == MFT_POT_LANGEVIN) {
             MFT_REQUIRE(P.maskI != 0 && P.sigma > 0.0f, MFT_ERR_INVALID,
                         "ps-potentialLangevin.cu: did not find appropriate delt for the Langevin noise");
@@ -119,7 +122,7 @@ static int tild_setup(TildEngine* e)
     for (int i = 0; i < d.npotentials; ++i) {
         const mft_tild_potential& P = e->pots[i];
         KPot& K = kp.pot[i];
-        if (P.kind == MFT_POT_BIAS || P.kind == MFT_POT_WALL || P.kind == MFT_POT_LANGEVIN) {
+        if (P.kind == MFT_POT_BIAS || P.kind == MFT_POT_WALL || P.kind == MFT_POT_LANGEVIN || P.kind == MFT_POT_DPD) {
             // not a pair potential: a no-op entry in the k-space pass (zero
             // amplitude, no species); a bias field is added to the force
             // fields after it, a wall to the particle forces in the integrate pass
@@ -277,7 +280,8 @@ static int configure_channels(TildEngine* e, const int* type, const float* charg
     uint32_t forced = 0;
     for (int i = 0; i < d.npotentials; ++i) {
         const mft_tild_potential& P = e->pots[i];
-        if (P.kind == MFT_POT_WALL || P.kind == MFT_POT_LANGEVIN) continue; // per-particle force
+        if (P.kind == MFT_POT_WALL || P.kind == MFT_POT_LANGEVIN || P.kind == MFT_POT_DPD)
+            continue; // per-particle force
         if (P.kind != MFT_POT_CHARGES) forced |= P.maskI | (P.kind == MFT_POT_BIAS ? 0u : P.maskJ);
     }
     if (e->charge_pot >= 0 && fold)
@@ -469,6 +473,7 @@ static int do_steps(TildEngine* e, int nsteps, bool want_f, bool advance = true)
     for (int i = 0; i < nsteps; ++i) {
         if (e->timing && 2 * (e->evused + MFT_TK_COUNT) > e->evpool.size()) resolve_timing(e);
         if ((rc = grid_forces(e))) return rc;
+        if (e->dpd.on && (rc = tild_dpd_forces(e))) return rc;
         const bool last = i == nsteps - 1;
         {
             TimedScope ts(e, MFT_TK_INTEGRATE);
@@ -533,7 +538,8 @@ int mft_tild_destroy(mft_tild* e)
                     e->rank, e->map, e->slot_of, e->type, e->charge, e->nBonds, e->bondedTo, e->bondType, e->pio,
                     e->pio2, e->pio3, e->count, e->start, e->partial, e->rho, e->S, e->Y, e->F4, e->force_slot,
                     e->epart, e->eout, e->bpart, e->tiles, e->tile_tab, e->NA[0], e->NA[1], e->AT[0], e->AT[1],
-                    e->AG[0], e->AG[1], e->nAngles, e->angleType, e->angleGroup};
+                    e->AG[0], e->AG[1], e->nAngles, e->angleType, e->angleGroup, e->dcount, e->dstart,
+                    e->dcursor, e->dcell, e->dslots, e->fdpd};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (float* t : e->tables)

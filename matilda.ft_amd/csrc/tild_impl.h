@@ -106,6 +106,16 @@ struct LangevinParams {
     float drag[kMaxWalls], nmag[kMaxWalls];
 };
 
+// DPD pair forces through a cell search of width >= rcut (ps_potentialDPD.cu,
+// ps_neighborList.cu)
+struct DpdParams {
+    int on;
+    uint32_t mask;
+    float gamma, sigma, rcut, rcut2, inv_sqrt_delt;
+    int nc[3];
+    float cw[3];
+};
+
 struct GjfParams {
     float a[kMaxSpecies], b[kMaxSpecies], nmag[kMaxSpecies], cf[kMaxSpecies], cn[kMaxSpecies];
     float inv_m[kMaxSpecies]; // VV::finishInitialization (ps_integratorVV.cu:15-26)
@@ -254,6 +264,9 @@ struct TildEngine {
     WallParams walls{};
     LangevinParams lang{};
     bool is_vv = false; // created with MFT_INT_VV (velocities live in the noise slot)
+    DpdParams dpd{};
+    int *dcount = nullptr, *dstart = nullptr, *dcursor = nullptr, *dcell = nullptr, *dslots = nullptr;
+    float4* fdpd = nullptr;
     std::vector<int> wall_pot;  // potential index of each wall
     double* epart = nullptr;
     double* eout = nullptr;   // [kMaxPots + 1]
@@ -322,6 +335,8 @@ int tild_bias_setup(TildEngine* e, int pot);      // the field and its force (ti
 int tild_bias_add(TildEngine* e);                 // F4 += bias forces of the biased species
 int tild_bias_energy(TildEngine* e, int k, double* energy);
 int tild_wall_energy(TildEngine* e, int w, double* energy); // tild_particles.hip
+int tild_dpd_setup(TildEngine* e, int pot);                  // tild_dpd.hip
+int tild_dpd_forces(TildEngine* e);
 int tild_angle_energy(TildEngine* e, double* energy);         // tild_particles.hip
 
 } // namespace mft

@@ -882,6 +882,7 @@ struct StepArgs {
     AngleParams ap;
     int vv;                    // velocity Verlet: 0 before the first step (TildEngine::vv_state)
     int use_v;                 // the noise slot holds velocities (VV)
+    const float4* FD;          // DPD pair forces per slot (null without)
     LangevinParams lang;
 };
 
@@ -995,6 +996,12 @@ __global__ void __launch_bounds__(256) k_integrate(
         bond_force_slots(s, ns, X, NB, BS, BT, maxbonds, bp, A.L[0], A.L[1], A.L[2], xc.x, xc.y, xc.z, fo);
         if (A.NA) angle_force_slots(s, X, A.NA, A.AT, A.AG, A.ap, A.L[0], A.L[1], A.L[2], fo);
         if (A.walls.n) wall_forces(A.walls, META[s].y, xc, fo);
+        if (A.FD) {
+            const float4 fd = A.FD[s];
+            fo[0] += fd.x;
+            fo[1] += fd.y;
+            fo[2] += fd.z;
+        }
         if (A.lang.n) {
             // the velocities the forces see: VV's (the half-kicked ones the
             // reference holds at force time), zero under GJF and EM
@@ -1326,6 +1333,7 @@ static StepArgs step_args(const TildEngine* e)
     A.vv = e->vv_state;
     A.lang = e->lang;
     A.use_v = e->is_vv ? 1 : 0;
+    A.FD = e->dpd.on ? e->fdpd : nullptr;
     return A;
 }
 

@@ -180,6 +180,12 @@ public:
     int normalDim = 0, dirSign = 1;
     float wallPos = 0.f, k = 0.f;
 };
+class DPD : public PS_Potential {
+public:
+    DPD(std::istringstream& iss, PS_Box* box); // ps_potentialDPD.cu:6-29
+    mft_tild_potential descriptor() const override;
+    float gamma = 0.f, kT = 0.f, delt = -1.0f;
+};
 class Langevin : public PS_Potential {
 public:
     Langevin(std::istringstream& iss, PS_Box* box); // ps_potentialLangevin.cu:16-57
@@ -308,6 +314,12 @@ public:
     // angles (PS_Box_initialization.cu:83-106, 229-231): per type k, theta_eq
     // (radians), style 0 WLC / 1 harmonic; per particle [nstot*MAXANGLES(*3)]
     int MAXANGLES = 3, nAnglesTot = 0;
+    struct NeighborList {
+        std::string group;
+        float rcut = 0.f;
+        int maxNeighbors = 64;
+    };
+    std::vector<NeighborList> neighborLists;
     std::vector<float> angleK, angleTheq;
     std::vector<int> angleStyle, nAngles, angleType, angleGroup;
     std::string datFileName = "ps_data.dat";

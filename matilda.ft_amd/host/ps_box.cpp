@@ -183,6 +183,37 @@ mft_tild_potential Wall::descriptor() const
     return d;
 }
 
+DPD::DPD(std::istringstream& iss, PS_Box* box) : PS_Potential(iss, box)
+{
+    iss >> grpI >> gamma >> kT;
+    grpJ = grpI;
+    if (!mybox->integrators.empty()) delt = mybox->integrators[0]->delt;
+    std::string word;
+    while (iss >> word) {
+        if (word == "delt") iss >> delt;
+        else die("Invalid keyword in dpd potential: " + word);
+    }
+    if (delt <= 0.0f) die("ps_potentialDPD: delt not set or invalid");
+}
+mft_tild_potential DPD::descriptor() const
+{
+    // DPD::initializePotential (ps_potentialDPD.cu:33-43): the group's neighbour list
+    const PS_Box::NeighborList* nl = nullptr;
+    for (const auto& n : mybox->neighborLists)
+        if (n.group == grpI) nl = &n;
+    if (!nl)
+        die("dpd: no neighbor_list found for group " + grpI + " \xe2\x80\x94 add 'neighbor_list " + grpI +
+            " <rcut>' before this potential");
+    mft_tild_potential d{};
+    d.kind = MFT_POT_DPD;
+    d.maskI = mybox->groupMask(grpI);
+    d.A = gamma;
+    d.sigma = kT;
+    d.Rp = nl->rcut;
+    d.wall_pos = delt;
+    return d;
+}
+
 Langevin::Langevin(std::istringstream& iss, PS_Box* box) : PS_Potential(iss, box)
 {
     iss >> grpI;
@@ -218,7 +249,8 @@ PS_Potential* PSPotentialFactory(std::istringstream& iss, PS_Box* box)
     if (s1 == "bias") return new BiasField(iss, box);
     if (s1 == "wall") return new Wall(iss, box);
     if (s1 == "Langevin" || s1 == "langevin") return new Langevin(iss, box);
-    if (s1 == "bias_lc" || s1 == "dpd" ||
+    if (s1 == "dpd") return new DPD(iss, box);
+    if (s1 == "bias_lc" ||
         s1 == "maiersaupe")
         die("ps_potential: " + s1 + " is not part of this build's particle-mesh path");
     die("ps_potential.cu: " + s1 + " is not a supported potential");
@@ -441,7 +473,13 @@ void PS_Box::readInput(std::ifstream& inp)
             } else if (firstWord == "potential") {
                 potentials.push_back(PSPotentialFactory(iss, this));
             } else if (firstWord == "neighbor_list") {
-                die("neighbor lists are not part of this build's particle-mesh path");
+                // PS_NeighborList (ps_neighborList.cu:132-141): group rcut [maxNeighbors];
+                // the engine searches cells of width >= rcut every step, with no cap
+                NeighborList nl;
+                int maxn;
+                iss >> nl.group >> nl.rcut;
+                if (iss >> maxn) nl.maxNeighbors = maxn;
+                neighborLists.push_back(nl);
             } else if (firstWord == "randSeed" || firstWord == "RAND_SEED" || firstWord == "RANDSEED") {
                 long long sd;
                 iss >> sd;

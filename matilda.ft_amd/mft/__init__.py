@@ -24,7 +24,7 @@ _d = C.c_double
 _f = C.c_float
 
 MFT_OK, MFT_ERR_INVALID, MFT_ERR_OOM, MFT_ERR_NONFINITE, MFT_ERR_DEVICE, MFT_ERR_UNSUPPORTED = range(6)
-POT = {"gaussian": 0, "erf2": 1, "erfG": 2, "charges": 3, "bias": 4, "wall": 5, "langevin": 6}
+POT = {"gaussian": 0, "erf2": 1, "erfG": 2, "charges": 3, "bias": 4, "wall": 5, "langevin": 6, "dpd": 7}
 BIAS_PHASE = {"L": 0, "H": 1, "G": 2}
 INTEGRATOR = {"GJF": 0, "EM": 1, "none": 2, "VV": 3}
 TK_NAMES = ["bin", "scan", "scatter", "density", "fft_x_fwd", "fft_y_fwd", "conv_z", "fft_y_inv",

@@ -407,6 +407,12 @@ def tild_system_from_config(cfg, integrator="GJF"):
         if p["kind"] == "charges":
             pots.append(TildPotential("charges", A=p["A"], sigma=p["sigma"]))
             continue
+        if p["kind"] == "dpd":
+            gI = [n for i, n in enumerate(names) if (p["maskI"] >> i) & 1]
+            assert len(gI) == 1
+            pots.append(TildPotential("dpd", gI[0], gI[0], A=p["A"], sigma=p["sigma"], Rp=p["Rp"],
+                                      wall_pos=p["wall_pos"]))
+            continue
         if p["kind"] == "langevin":
             gI = [n for i, n in enumerate(names) if (p["maskI"] >> i) & 1]
             assert len(gI) == 1
@@ -507,6 +513,13 @@ def tild_forces(sys: TildSystem, bits=64, want_energy=False):
     energies = []
     nlang = 0
     for pot in sys.potentials:
+        if pot.kind == "dpd":
+            sites = np.asarray(sys.group_sites(pot.I))
+            fr = f.reshape(-1, Dim)
+            fr[sites] += dpd_pair_forces(sys, pot, sites)
+            if want_energy:
+                energies.append(0.0)
+            continue
         if pot.kind == "langevin":
             # d_add_Langevin_forces (ps_potentialLangevin.cu:60-133): -v drag +
             # sqrtf(2 drag / delt) N(0, 1), normals keyed (particle, step,
@@ -599,6 +612,7 @@ def normal4_np(c0, c1, c2, c3, seed):
 
 TAG_TILD = 0x54494C44  # 'TILD'
 TAG_LANGEVIN = 0x4C414E47  # 'LANG'
+TAG_DPD = 0x44504450       # 'DPDP'
 TAG_FTS = 0x4654534E   # 'FTSN'
 
 
@@ -628,6 +642,47 @@ def tild_step(sys: TildSystem, xi=None, bits=64):
         lib(bits).orc_em_step(_ptr(x), _ptr(f), sys.delt, _ptr(Lf), _ptr(sites), sys.ns, sys.Dim, _ptr(np.ascontiguousarray(xi, np.float64)))
         out.update(x=x)
     return out
+
+
+def dpd_pair_forces(sys: TildSystem, pot: TildPotential, sites):
+    """d_DPD_forces (ps_potentialDPD.cu:63-137) over every pair of the group
+    closer than rcut (PS_NeighborList has no skin and is rebuilt each step,
+    ps_neighborList.cu:200-272): F_i += Fmag r^, F_j -= Fmag r^ with r = x_i - x_j,
+    Fmag = -gamma w^2 (v_i - v_j).r^ + sqrtf(2 gamma kT) w xi / sqrtf(delt),
+    w = 1 - r / rcut.  pot.A = gamma, pot.sigma = kT, pot.Rp = rcut,
+    pot.wall_pos = delt; xi is one normal per pair keyed (lower id, higher
+    id, 'DPDP', step); v is VV's velocity, zero otherwise."""
+    D = sys.Dim
+    x = np.asarray(sys.x, np.float64).reshape(-1, D)[sites]
+    v = np.zeros_like(x)
+    if sys.integrator == "VV" and sys.v is not None:
+        v = np.asarray(sys.v, np.float64).reshape(-1, D)[sites]
+    L = np.asarray(_fa(sys.L), np.float64)
+    Lh = np.asarray(_fa(0.5 * _fa(sys.L)), np.float64)
+    g32, kT32 = np.float32(pot.A), np.float32(pot.sigma)
+    gamma = float(g32)
+    sig = float(np.sqrt(np.float32(np.float32(2.0) * g32 * kT32)))
+    rcut = float(np.float32(pot.Rp))
+    rc2 = float(np.float32(rcut) * np.float32(rcut))
+    isd = float(np.float32(1.0) / np.sqrt(np.float32(pot.wall_pos)))
+    n = x.shape[0]
+    i, j = np.triu_indices(n, 1)
+    dr = x[i] - x[j]
+    dr = np.where(dr > Lh, dr - L, np.where(dr < -Lh, dr + L, dr))
+    r2 = np.sum(dr * dr, axis=1)
+    keep = (r2 < rc2) & (r2 > 0.0)
+    i, j, dr, r2 = i[keep], j[keep], dr[keep], r2[keep]
+    r = np.sqrt(r2)
+    w = 1.0 - r / rcut
+    rhat = dr / r[:, None]
+    vdot = np.sum((v[i] - v[j]) * rhat, axis=1)
+    a, b = sites[i].astype(np.uint64), sites[j].astype(np.uint64)
+    xi = normal4_np(np.minimum(a, b), np.maximum(a, b), TAG_DPD, sys.step, sys.seed)[:, 0]
+    fmag = -gamma * w * w * vdot + sig * w * xi * isd
+    F = np.zeros((n, D))
+    np.add.at(F, i, fmag[:, None] * rhat)
+    np.add.at(F, j, -fmag[:, None] * rhat)
+    return F
 
 
 def _vv_step(sys: TildSystem, bits=64):

@@ -255,3 +255,19 @@ def test_langevin_potential_keyword(tmp_path):
     early = base.replace("potential gaussian A A", "potential Langevin A\npotential gaussian A A")
     r = dry(tmp_path, early)
     assert r.returncode == 1 and "did not find appropriate delt for the Langevin noise" in r.stdout
+
+
+def test_dpd_potential_keyword(tmp_path):
+    # ps_potentialDPD.cu:6-43 and ps_neighborList.cu:132-141: dpd needs a
+    # neighbor_list on its group; delt defaults to the integrator's
+    base = open(os.path.join(INPUTS, "c1_diblock.in")).read()
+    text = base.replace("logFreq", "neighbor_list A 1.2\npotential dpd A 4.5 1.0\nlogFreq")
+    r = dry(tmp_path, text)
+    assert r.returncode == 0, r.stdout[-1000:]
+    p = json.loads(r.stdout.strip().splitlines()[-1])["potentials"][-1]
+    assert (p["kind"], p["maskI"]) == (7, 1)
+    assert (p["A"], p["sigma"], p["Rp"]) == pytest.approx((4.5, 1.0, 1.2)) and p["wall_pos"] == pytest.approx(0.005)
+    r = dry(tmp_path, base.replace("logFreq", "potential dpd A 4.5 1.0\nlogFreq"))
+    assert r.returncode == 1 and "dpd: no neighbor_list found for group A" in r.stdout
+    r = dry(tmp_path, base.replace("logFreq", "neighbor_list A 1.2\npotential dpd A 4.5 1.0 rc 2\nlogFreq"))
+    assert r.returncode == 1 and "Invalid keyword in dpd potential: rc" in r.stdout

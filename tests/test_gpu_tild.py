@@ -558,3 +558,33 @@ def test_langevin_matches_oracle(gpu, integrator):
     pe, _ = eng.energies()
     assert pe[-1] == 0.0
     eng.close()
+
+
+@pytest.mark.parametrize("integrator", ["VV", "GJF"])
+def test_dpd_matches_oracle(gpu, integrator):
+    # DPD pair forces (ps_potentialDPD.cu) on species A through the cell
+    # search (rcut 1.0 in a 6^3 box: 6 cells per axis) beside the pair
+    # potentials; forces and four steps against the oracle's all-pairs sum
+    cfg = small_system(False, seed=16)
+    cfg["potentials"] = list(cfg["potentials"]) + [
+        dict(kind="dpd", maskI=1, maskJ=0, A=4.5, sigma=1.0, Rp=1.0, wall_pos=cfg["delt"])]
+    say("case: dpd", integrator)
+    eng = systems.tild_engine(cfg, integrator)
+    sys_o = oracle_system(cfg, integrator)
+    f_g = eng.compute_forces()
+    assert rel(f_g, orc.tild_forces(sys_o)["f"]) < TOL
+    ns = cfg["topo"]["type"].size
+    for step in range(4):
+        eng.step(1)
+        sys_o.step = step
+        if integrator == "VV":
+            o = orc.tild_step(sys_o)
+            sys_o.v, sys_o.f_prev = o["v"], o["f_prev"]
+        else:
+            o = orc.tild_step(sys_o, orc.tild_noise(ns, cfg["seed"], step))
+            sys_o.xo = o["xo"].astype(np.float32)
+            sys_o.noise_old = o["noise_old"].astype(np.float32)
+        sys_o.x = o["x"].astype(np.float32)
+        x_g, _, _ = eng.get_state()
+        assert np.max(periodic_dist(x_g, o["x"], cfg["L"])) < 1e-6 * max(cfg["L"]), step
+    eng.close()

@@ -376,3 +376,33 @@ def test_langevin_with_vv_thermalizes_to_unit_temperature():
         if step >= 250:
             acc.append(np.mean(2.0 * sys.v ** 2))
     assert np.mean(acc) == pytest.approx(1.0, rel=0.06)
+
+
+def _dpd_fluid(n=800, box=8.0, kT=1.5, gamma=4.5, seed=21):
+    rng = np.random.default_rng(seed)
+    x = (rng.random(3 * n) * box).astype(np.float32)
+    topo = dict(intSpecies=np.zeros(n, np.int32), nBonds=np.zeros(n, np.int32), bondedTo=np.zeros(2 * n, np.int32),
+                bondType=np.zeros(2 * n, np.int32), charge=np.zeros(n, np.float32))
+    return orc.TildSystem(Nx=(8, 8, 8), L=(box, box, box), pmeorder=1, species=["A"], x=x, topo=topo,
+                          bond_k=[0.0], bond_r0=[0.0], bond_style=[0], mass=[1.0], mobility=[1.0], delt=0.02,
+                          integrator="VV", seed=seed,
+                          potentials=[orc.TildPotential("dpd", "A", "A", A=gamma, sigma=kT, Rp=1.0, wall_pos=0.02)])
+
+
+def test_dpd_conserves_momentum_and_thermalizes():
+    # DPD pair forces (ps_potentialDPD.cu) obey Newton's third law, so the
+    # total force vanishes; with velocity Verlet the fluid reaches the set
+    # temperature, <v^2> = kT (fluctuation-dissipation: sigma^2 = 2 gamma kT)
+    sys = _dpd_fluid()
+    sys.v = np.random.default_rng(1).normal(size=3 * 800) * 0.2
+    f = orc.tild_forces(sys)["f"].reshape(-1, 3)
+    assert np.max(np.abs(f.sum(axis=0))) < 1e-9 * np.max(np.abs(f)) * f.shape[0]
+    sys.v = None
+    acc = []
+    for step in range(300):
+        sys.step = step
+        o = orc.tild_step(sys)
+        sys.x, sys.v, sys.f_prev = o["x"].astype(np.float32), o["v"], o["f_prev"]
+        if step >= 150:
+            acc.append(np.mean(sys.v.reshape(-1, 3) ** 2))
+    assert np.mean(acc) == pytest.approx(1.5, rel=0.08)
