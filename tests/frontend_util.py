@@ -76,6 +76,14 @@ def ps_input(cfg, data_name, steps, log_freq, integrator="GJF"):
             lines.append("potential charges %r %r" % (float(p["A"]), float(p["sigma"])))
             continue
         gi = names[int(np.log2(p["maskI"]))]
+        if p["kind"] == "langevin":
+            lines.append("potential Langevin %s drag %r delt %r" % (gi, float(p["A"]), float(p["sigma"])))
+            continue
+        if p["kind"] == "dpd":
+            lines.append("neighbor_list %s %r" % (gi, float(p["Rp"])))
+            lines.append("potential dpd %s %r %r delt %r" % (gi, float(p["A"]), float(p["sigma"]),
+                                                             float(p["wall_pos"])))
+            continue
         gj = names[int(np.log2(p["maskJ"]))]
         ramp = (" ramp %r" % float(p["A_final"])) if p.get("ramp") else ""
         lines.append("potential %s %s %s %r %r%s" % (p["kind"], gi, gj, float(p["A"]), float(p["sigma"]), ramp))

@@ -359,3 +359,36 @@ def test_ps_front_end_angles_match_engine(gpu, tmp_path):
                  if ag[i, a, 1] == i]
         assert f0["angles/N"][0] == len(verts) == 60 * 8
         assert [tuple(g) for g in f0["angles/group"].tolist()] == verts
+
+
+def test_ps_front_end_vv_dpd_langevin_match_engine(gpu, tmp_path):
+    # integrator VV with a dpd potential on A (through its neighbor_list) and
+    # a Langevin potential on B: the front end's log rows and final positions
+    # agree with the engine driven directly
+    cfg = small_system(False, seed=11)
+    cfg["potentials"] = list(cfg["potentials"]) + [
+        dict(kind="dpd", maskI=1, maskJ=0, A=4.5, sigma=1.0, Rp=1.0, wall_pos=cfg["delt"]),
+        dict(kind="langevin", maskI=2, maskJ=0, A=0.7, sigma=cfg["delt"])]
+    say("case: front end VV + dpd + Langevin")
+    write_data_file(tmp_path / "init.data", cfg)
+    (tmp_path / "input").write_text(ps_input(cfg, "init.data", steps=12, log_freq=5, integrator="VV"))
+    r = run_mft(["-in", "input"], cwd=tmp_path)
+    assert r.returncode == 0, r.stdout[-3000:]
+    n = cfg["topo"]["type"].size
+    eng = systems.tild_engine(cfg, "VV")
+    eng.set_run(12)
+    done = 0
+    for row in read_log(tmp_path / "ps_data.dat"):
+        k = {-1: 0, 0: 1, 5: 6, 10: 11, 12: 12}[int(row[0])]
+        eng.step(k - done)
+        done = k
+        pe, be = eng.energies()
+        assert close(row[1:], [be] + list(pe)), (row, be, pe)
+    assert done == 12
+    x_fe = read_data_positions(tmp_path / "final.input.data", n)
+    x_api, _, _ = eng.get_state()
+    d = (x_fe.astype(np.float64) - x_api).reshape(-1, 3)
+    L = np.asarray(cfg["L"])
+    d -= L * np.round(d / L)
+    assert np.max(np.abs(d)) < 1e-5 * max(cfg["L"])
+    eng.close()
