@@ -602,9 +602,15 @@ __global__ void __launch_bounds__(256) k_gather(float4* __restrict__ fgrid, cons
 
 // Tile variant.  The force field window of a brick ((b+p)^3 points from
 // origin - p/2, periodic) is staged in LDS by 16 lanes per window row, every
-// load of a lane issued before its first LDS store.
+// load of a lane issued before its first LDS store.  The window keeps the
+// three force components only (12 bytes a point): the stencil sums read a
+// quarter fewer LDS bytes than with the padded float4.
+struct Win3 {
+    float x, y, z;
+};
+
 template <int P, int BM>
-__device__ __forceinline__ void stage_window(float4* ft, int* xg, int* yo, int* zo, unsigned char (*rows)[2],
+__device__ __forceinline__ void stage_window(Win3* ft, int* xg, int* yo, int* zo, unsigned char (*rows)[2],
                                              const float4* __restrict__ F, int o0, int o1, int o2, int e0, int e1,
                                              int e2, int nx, int ny, int nz)
 {
@@ -625,7 +631,7 @@ __device__ __forceinline__ void stage_window(float4* ft, int* xg, int* yo, int* 
     const int lane16 = threadIdx.x & 15, row16 = threadIdx.x >> 4;
     if (lane16 < e0) {
         const float4* src = F + xg[lane16];
-        float4* dst = ft + lane16;
+        Win3* dst = ft + lane16;
         // rows past the window re-load the last row (in bounds) and skip the
         // store, so the arrays stay in registers
         float4 v[RMAX];
@@ -639,18 +645,18 @@ __device__ __forceinline__ void stage_window(float4* ft, int* xg, int* yo, int* 
         }
 #pragma unroll
         for (int j = 0; j < RMAX; ++j)
-            if (row16 + 16 * j < nrows) dst[dofs[j]] = v[j];
+            if (row16 + 16 * j < nrows) dst[dofs[j]] = Win3{v[j].x, v[j].y, v[j].z};
     }
     __syncthreads();
 }
 
 // sum_w w F over the particle's stencil, from the staged window
 template <int P, int BM>
-__device__ __forceinline__ float3 stencil_sum(const float4* ft, const int* l, const float* wx, const float* wy,
+__device__ __forceinline__ float3 stencil_sum(const Win3* ft, const int* l, const float* wx, const float* wy,
                                               const float* wz)
 {
     constexpr int TP = BM + P;
-    const float4* t = ft + l[0] + TP * (l[1] + TP * l[2]);
+    const Win3* t = ft + l[0] + TP * (l[1] + TP * l[2]);
     float a0 = 0.f, a1 = 0.f, a2 = 0.f;
 #pragma unroll
     for (int iz = 0; iz <= P; ++iz) {
@@ -660,7 +666,7 @@ __device__ __forceinline__ float3 stencil_sum(const float4* ft, const int* l, co
 #pragma unroll
             for (int ix = 0; ix <= P; ++ix) {
                 const float w = wx[ix] * wyz;
-                const float4 v = t[ix + TP * (iy + TP * iz)];
+                const Win3 v = t[ix + TP * (iy + TP * iz)];
                 a0 += v.x * w;
                 a1 += v.y * w;
                 a2 += v.z * w;
@@ -685,7 +691,7 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_species(
     float dx1, float dx2)
 {
     constexpr int TP = BM + P, T3 = TP * TP * TP;
-    __shared__ float4 ft[T3];
+    __shared__ Win3 ft[T3];
     __shared__ int xg[TP], yo[TP], zo[TP];
     __shared__ unsigned char rows[TP * TP][2];
     const int brick = brick_of_block(blockIdx.x, tg.nbricks);
@@ -726,7 +732,7 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_charge(
     float dx1, float dx2)
 {
     constexpr int TP = BM + P, T3 = TP * TP * TP;
-    __shared__ float4 ft[T3];
+    __shared__ Win3 ft[T3];
     __shared__ int xg[TP], yo[TP], zo[TP];
     __shared__ unsigned char rows[TP * TP][2];
     const int brick = brick_of_block(blockIdx.x, tg.nbricks);
