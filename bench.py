@@ -32,27 +32,74 @@ def step_bytes(N, M, nchan_grid):
     return N * 112.0 + M * nchan_grid * 80.0
 
 
+def host_info():
+    """Cores this process may use and the CPU model (lscpu's model name)."""
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return cores, model
+
+
 def cpu_baseline(seed):
-    """FP64 oracle (single thread) on a 1/8-volume sub-box of the same
-    workload: same density, composition, grid spacing and potentials."""
+    """The host build of the same kernels (the oracle's C restatement, FP64
+    and FP32, one thread and OpenMP on every core this process may use), on
+    this host: TILD steps of a 1/8-volume sub-box of the benchmark workload
+    (same density, composition, grid spacing and potentials) and CL
+    iterations of the plain 64^3, 100-bead diblock at full size."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
     import orc  # the checker; only this leg of the benchmark loads it
     from mft import systems
+    cores, model = host_info()
+    legs = []
+
     cfg = systems.tild_coacervate(nchains=5000, nsolvent=25000, box=30.0, grid=64, seed=seed)
-    sysc = orc.tild_system_from_config(cfg)
-    ns = sysc.ns
-    out = orc.tild_step(sysc, orc.tild_noise(ns, seed, 0))       # warm-up (allocations)
-    sysc.x, sysc.xo, sysc.noise_old = out["x"].astype("float32"), out["xo"].astype("float32"), out["noise_old"].astype("float32")
-    steps = 6
-    t0 = time.perf_counter()
-    for s in range(1, 1 + steps):
-        out = orc.tild_step(sysc, orc.tild_noise(ns, seed, s))
-        sysc.x, sysc.xo, sysc.noise_old = out["x"].astype("float32"), out["xo"].astype("float32"), out["noise_old"].astype("float32")
-    dt = time.perf_counter() - t0
-    return {"value": ns * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"FP64 oracle, 6 timed GJF steps (after 1 warm-up) of a 1/8-volume sub-box of the same "
-                      f"workload: {ns} particles, 30^3 box, 64^3 grid, same density/composition/potentials",
-            "seconds": dt}
+    spec, fields = systems.fts_diblock()
+    f64 = {k: np.asarray(v, np.complex64).astype(np.complex128) for k, v in fields.items()}
+    for bits in (64, 32):
+        for threads in (1, cores):
+            orc.use_openmp(True, threads)
+            sysc = orc.tild_system_from_config(cfg)
+            ns = sysc.ns
+            steps = 2 if threads == 1 else 4
+
+            def step(s):
+                out = orc.tild_step(sysc, orc.tild_noise(ns, seed, s), bits=bits)
+                sysc.x, sysc.xo, sysc.noise_old = (out[k].astype("float32") for k in ("x", "xo", "noise_old"))
+            step(0)                                              # warm-up (allocations)
+            t0 = time.perf_counter()
+            for s in range(1, 1 + steps):
+                step(s)
+            dt = time.perf_counter() - t0
+            legs.append({"path": "tild", "bits": bits, "threads": threads, "value": ns * steps / dt,
+                         "unit": "particle-steps/s", "seconds": dt})
+            o = orc.fts_system_from_spec(spec, f64)
+            t0 = time.perf_counter()
+            orc.fts_init(o, bits=bits)
+            t_init = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            orc.fts_step(o, orc.fts_noise(o.M, spec["seed"], 0), bits=bits)
+            dt = time.perf_counter() - t0
+            legs.append({"path": "cl_fts", "bits": bits, "threads": threads, "ms_per_iteration": 1e3 * dt,
+                         "solves_per_hour": 3600.0 / (2000 * dt), "seconds": dt + t_init})
+            print(json.dumps({"cpu_baseline_leg": legs[-2:]}), file=sys.stderr, flush=True)
+    orc.use_openmp(False)
+    best = max((l for l in legs if l["path"] == "tild"), key=lambda l: l["value"])
+    return {"value": best["value"], "unit": "particle-steps/s", "cores": best["threads"], "kind": "port",
+            "host": {"cores": cores, "model": model},
+            "sample": f"the oracle's C restatement built for the host (FP{best['bits']}, OpenMP, {best['threads']} "
+                      f"threads): TILD GJF steps of a 1/8-volume sub-box of the same workload ({ns} particles, "
+                      f"30^3 box, 64^3 grid, same density/composition/potentials), 2-4 timed steps after one "
+                      f"warm-up; 'legs' lists FP64 and FP32 at 1 thread and all {cores} cores, and one CL "
+                      f"iteration of the plain 64^3 Ns=100 diblock per leg",
+            "legs": legs}
 
 
 def fts_bytes(M, Ns):

@@ -257,6 +257,14 @@ int mft_tild_synchronize(mft_tild* eng);
  * and bond energy (x 1/2, PS_Box.cu:292, with the reference's 0.5*Lh minimum
  * image of ps_deviceBonds.cu:132-133). */
 int mft_tild_energies(mft_tild* eng, double* pot_energy, double* bond_energy);
+/* nsteps steps (nsteps >= 1) that also return the energies PS_Box::writeData
+ * logs after the last of them (PS_Box.cu:152-173, 282-317): pair potentials
+ * from that step's own density pass (the positions before the integrator
+ * moved them) with the amplitudes after its ramp update (update_prefactor
+ * runs before writeData); bias and wall energies, and the bond energy (half
+ * the per-particle sum, 0.5 Lh minimum image), at the positions after the
+ * step.  Either output may be NULL. */
+int mft_tild_step_energies(mft_tild* eng, int nsteps, double* pot_energy, double* bond_energy);
 /* Per-kernel timing: while enabled, every kernel group of step() is
  * bracketed by HIP events on the engine stream (no host synchronisation);
  * each call resolves the recorded pairs and returns the average ms per launch
@@ -364,6 +372,12 @@ int mft_fts_set_activity(mft_fts* sys, int m, double activity);
  * fts_molecule_linear.cu:344-346) or nmolecs Ntot (canonical). */
 int mft_fts_nsites(mft_fts* sys, double* nsites);
 int mft_fts_get_fields(mft_fts* sys, int pot, float* wpl, float* wmi);
+/* Snapshot slot 0..7 of the state a sweep may return to, kept in device
+ * memory: restore = 0 copies every potential field and the activities into
+ * the slot, restore = 1 copies them back and recomputes the densities of the
+ * restored fields (the reference keeps such copies on the host,
+ * FTS_Box::storePotentials / restorePotentials, FTS_BoxFindSpinodal.cu:289-371). */
+int mft_fts_snapshot(mft_fts* sys, int slot, int restore);
 /* FTS_Box::finishInitialization density pass (FTS_Box.cu:193-256). */
 int mft_fts_init(mft_fts* sys);
 /* niter CL iterations of FTS_Box::doTimeStep (FTS_Box.cu:19-47). */
@@ -380,6 +394,11 @@ int mft_fts_get_propagators(mft_fts* sys, int m, float* q, float* qdag);
 /* Q[m] (complex), H (complex) and its parts: potentials then molecules
  * (FTS_Box::computeHamiltonian, FTS_Box.cu:590-602), FP64 reductions. */
 int mft_fts_observables(mft_fts* sys, double* Q, double* H, double* parts);
+/* Chemical potential of every molecule, mu[2*nmolecules] (complex): ln(n/(V Q))
+ * for a canonical molecule, ln(activity) for a grand canonical one.  The
+ * reference prints none; this is the ideal-chain form its density factor
+ * n/(Q V) implies (fts_molecule_linear.cu:286-291, H term :368-382). */
+int mft_fts_chemical_potentials(mft_fts* sys, double* mu);
 
 /* Per-kernel FTS entry points on caller-owned device arrays in the
  * reference's q-form layouts (complex interleaved float [M] per field,
@@ -438,6 +457,11 @@ int mft_fts_batch_step(mft_fts_batch* b, int niter);
 int mft_fts_batch_synchronize(mft_fts_batch* b);
 int mft_fts_batch_observables(mft_fts_batch* b, int rep, double* Q, double* H, double* parts);
 int mft_fts_batch_get_densities(mft_fts_batch* b, int rep, float* rho);
+/* Fields, species fields and chemical potentials of replica rep, as
+ * mft_fts_get_fields, mft_fts_get_species_fields, mft_fts_chemical_potentials. */
+int mft_fts_batch_get_fields(mft_fts_batch* b, int rep, int pot, float* wpl, float* wmi);
+int mft_fts_batch_get_species_fields(mft_fts_batch* b, int rep, float* W);
+int mft_fts_batch_chemical_potentials(mft_fts_batch* b, int rep, double* mu);
 
 /* ============================================================ serving ==== */
 /* Request loop with continuous batching (no counterpart in the reference,
@@ -472,6 +496,12 @@ const char* mft_server_error(mft_server* srv, int64_t id);
  * parts[2*(npotentials+nmolecules)] as mft_fts_observables; rho (may be NULL)
  * receives the species densities [nspecies*M] complex. */
 int mft_server_fts_result(mft_server* srv, int64_t id, double* Q, double* H, double* parts, float* rho);
+/* Final state of a finished CL-FTS request: the fields of potential pot
+ * (wpl, and wmi for Flory; either may be NULL) and the chemical potentials
+ * mu[2*nmolecules] (may be NULL), as mft_fts_get_fields and
+ * mft_fts_chemical_potentials; with the results above they let a caller
+ * re-evaluate the final densities and H from the fields. */
+int mft_server_fts_final_state(mft_server* srv, int64_t id, int pot, float* wpl, float* wmi, double* mu);
 /* Results of a finished TILD request: positions x[3N] (may be NULL) and the
  * energies of mft_tild_energies at the final state. */
 int mft_server_tild_result(mft_server* srv, int64_t id, float* x, double* pot_energy, double* bond_energy);

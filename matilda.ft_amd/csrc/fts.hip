@@ -38,6 +38,7 @@ constexpr int kFtsMaxPots = 8;
 constexpr int kFtsMaxMol = 4;
 constexpr int kFtsMaxBlocks = 16;
 constexpr int kFtsThreads = 256;
+constexpr int kFtsMaxSnapshots = 8;
 
 struct FtsPotDev {
     int kind, I, J;
@@ -669,6 +670,11 @@ struct FtsRep {
     double2* Q = nullptr;
     int nHpart = 0, nQpart = 0;
     int slot = 0;                 // iteration counter slot
+    // device-resident copies of the potential fields (all w+, then every
+    // Flory w-) and the activities, kept for a caller that returns to an
+    // earlier state of a sweep
+    float2* snap[kFtsMaxSnapshots] = {nullptr};
+    double snap_act[kFtsMaxSnapshots][kFtsMaxMol] = {{0}};
     bool alive = true;
     std::vector<void*> allocs;    // device buffers owned by the replica
 };
@@ -1829,6 +1835,101 @@ static int batch_densities(FtsBatch* b, int rep, float* rho)
     return MFT_OK;
 }
 
+static int batch_get_fields(FtsBatch* b, int rep, int pot, float* wpl, float* wmi)
+{
+    MFT_REQUIRE(rep >= 0 && rep < (int)b->reps.size() && b->reps[rep].alive, MFT_ERR_INVALID,
+                "replica out of range");
+    FtsRep& r = b->reps[rep];
+    MFT_REQUIRE(pot >= 0 && pot < r.d.npotentials, MFT_ERR_INVALID, "potential out of range");
+    MFT_HIP(hipStreamSynchronize(b->stream));
+    if (wpl) MFT_HIP(hipMemcpy(wpl, r.wpl[pot], sizeof(float2) * r.M, hipMemcpyDeviceToHost));
+    if (wmi) {
+        MFT_REQUIRE(r.wmi[pot], MFT_ERR_INVALID, "potential %d has no w- field", pot);
+        MFT_HIP(hipMemcpy(wmi, r.wmi[pot], sizeof(float2) * r.M, hipMemcpyDeviceToHost));
+    }
+    return MFT_OK;
+}
+
+static int batch_species_fields(FtsBatch* b, int rep, float* W)
+{
+    MFT_REQUIRE(rep >= 0 && rep < (int)b->reps.size() && b->reps[rep].alive && W, MFT_ERR_INVALID,
+                "bad argument");
+    FtsRep& r = b->reps[rep];
+    MFT_HIP(hipStreamSynchronize(b->stream));
+    MFT_HIP(hipMemcpy(W, r.W, sizeof(float2) * r.M * r.d.nspecies, hipMemcpyDeviceToHost));
+    return MFT_OK;
+}
+
+// Chemical potential of every molecule, complex: ln(n / (V Q)) for a
+// canonical molecule (the discrete-chain ideal-gas form in which
+// rho = n/(Q V) e^{W} sum q q^dag is written, fts_molecule_linear.cu:286-291),
+// ln(activity) for a grand canonical one (its density factor is the activity
+// itself, fts_molecule_linear.cu:289-291).  The reference prints no chemical
+// potential; this is the quantity its density factor implies.
+// Save (restore = 0) or restore (restore = 1) the potential fields and the
+// activities of replica rep in snapshot slot `slot`, device to device.  A
+// restore is followed by the density pass of the restored fields, so the next
+// iteration starts from the saved state (the iteration count, and with it the
+// noise stream, carries on).
+static int batch_snapshot(FtsBatch* b, int rep, int slot, int restore)
+{
+    MFT_REQUIRE(rep >= 0 && rep < (int)b->reps.size() && b->reps[rep].alive, MFT_ERR_INVALID,
+                "replica out of range");
+    MFT_REQUIRE(slot >= 0 && slot < kFtsMaxSnapshots, MFT_ERR_INVALID, "snapshot slot %d out of range (0..%d)", slot,
+                kFtsMaxSnapshots - 1);
+    FtsRep& r = b->reps[rep];
+    std::vector<float2*> fields;
+    for (int p = 0; p < r.d.npotentials; ++p) fields.push_back(r.wpl[p]);
+    for (int p = 0; p < r.d.npotentials; ++p)
+        if (r.wmi[p]) fields.push_back(r.wmi[p]);
+    const size_t M = r.M;
+    if (!restore) {
+        if (!r.snap[slot]) {
+            int rc = dev_alloc(r.allocs, &r.snap[slot], fields.size() * M);
+            if (rc) return rc;
+        }
+        for (size_t f = 0; f < fields.size(); ++f)
+            MFT_HIP(hipMemcpyAsync(r.snap[slot] + f * M, fields[f], sizeof(float2) * M, hipMemcpyDeviceToDevice,
+                                   b->stream));
+        for (int m = 0; m < r.d.nmolecules; ++m) r.snap_act[slot][m] = r.d.molecules[m].activity;
+        return MFT_OK;
+    }
+    MFT_REQUIRE(r.snap[slot], MFT_ERR_INVALID, "snapshot slot %d is empty", slot);
+    for (size_t f = 0; f < fields.size(); ++f)
+        MFT_HIP(hipMemcpyAsync(fields[f], r.snap[slot] + f * M, sizeof(float2) * M, hipMemcpyDeviceToDevice,
+                               b->stream));
+    for (int m = 0; m < r.d.nmolecules; ++m) {
+        r.d.molecules[m].activity = r.snap_act[slot][m];
+        r.act_dev[m] = r.snap_act[slot][m];
+    }
+    r.act_pending = false;
+    group_for(b, r).dirty = true;
+    return batch_init(b);
+}
+
+static int batch_chemical_potentials(FtsBatch* b, int rep, double* mu)
+{
+    MFT_REQUIRE(rep >= 0 && rep < (int)b->reps.size() && b->reps[rep].alive && mu, MFT_ERR_INVALID,
+                "bad argument");
+    FtsRep& r = b->reps[rep];
+    double2 q[kFtsMaxMol];
+    MFT_HIP(hipStreamSynchronize(b->stream));
+    MFT_HIP(hipMemcpy(q, r.Q, sizeof(double2) * r.d.nmolecules, hipMemcpyDeviceToHost));
+    for (int m = 0; m < r.d.nmolecules; ++m) {
+        std::complex<double> v;
+        const double z = r.d.molecules[m].activity;
+        if (z > 0.0)
+            v = std::log(std::complex<double>(z, 0.0));
+        else
+            v = std::log(std::complex<double>(r.nmolecs[m] / r.V, 0.0)) - std::log(std::complex<double>(q[m].x, q[m].y));
+        mu[2 * m] = v.real();
+        mu[2 * m + 1] = v.imag();
+        MFT_REQUIRE(std::isfinite(v.real()) && std::isfinite(v.imag()), MFT_ERR_NONFINITE,
+                    "chemical potential of molecule %d is not finite", m + 1);
+    }
+    return MFT_OK;
+}
+
 } // namespace mft
 
 using namespace mft;
@@ -1874,15 +1975,8 @@ int mft_fts_set_fields(mft_fts* s, int pot, const float* wpl, const float* wmi)
 }
 int mft_fts_get_fields(mft_fts* s, int pot, float* wpl, float* wmi)
 {
-    MFT_REQUIRE(s && pot >= 0 && pot < s->reps[0].d.npotentials, MFT_ERR_INVALID, "bad argument");
-    FtsRep& r = s->reps[0];
-    MFT_HIP(hipStreamSynchronize(s->stream));
-    if (wpl) MFT_HIP(hipMemcpy(wpl, r.wpl[pot], sizeof(float2) * r.M, hipMemcpyDeviceToHost));
-    if (wmi) {
-        MFT_REQUIRE(r.wmi[pot], MFT_ERR_INVALID, "potential %d has no w- field", pot);
-        MFT_HIP(hipMemcpy(wmi, r.wmi[pot], sizeof(float2) * r.M, hipMemcpyDeviceToHost));
-    }
-    return MFT_OK;
+    MFT_REQUIRE(s, MFT_ERR_INVALID, "null system");
+    return batch_get_fields(s, 0, pot, wpl, wmi);
 }
 int mft_fts_init(mft_fts* s)
 {
@@ -1907,11 +2001,18 @@ int mft_fts_get_densities(mft_fts* s, float* rho)
 }
 int mft_fts_get_species_fields(mft_fts* s, float* W)
 {
-    MFT_REQUIRE(s && W, MFT_ERR_INVALID, "bad argument");
-    FtsRep& r = s->reps[0];
-    MFT_HIP(hipStreamSynchronize(s->stream));
-    MFT_HIP(hipMemcpy(W, r.W, sizeof(float2) * r.M * r.d.nspecies, hipMemcpyDeviceToHost));
-    return MFT_OK;
+    MFT_REQUIRE(s, MFT_ERR_INVALID, "null system");
+    return batch_species_fields(s, 0, W);
+}
+int mft_fts_snapshot(mft_fts* s, int slot, int restore)
+{
+    MFT_REQUIRE(s, MFT_ERR_INVALID, "null system");
+    return batch_snapshot(s, 0, slot, restore);
+}
+int mft_fts_chemical_potentials(mft_fts* s, double* mu)
+{
+    MFT_REQUIRE(s, MFT_ERR_INVALID, "null system");
+    return batch_chemical_potentials(s, 0, mu);
 }
 int mft_fts_get_propagators(mft_fts* s, int m, float* q, float* qdag)
 {
@@ -2003,6 +2104,21 @@ int mft_fts_batch_get_densities(mft_fts_batch* b, int rep, float* rho)
     MFT_REQUIRE(b, MFT_ERR_INVALID, "null batch");
     return batch_densities(b, rep, rho);
 }
+int mft_fts_batch_get_fields(mft_fts_batch* b, int rep, int pot, float* wpl, float* wmi)
+{
+    MFT_REQUIRE(b, MFT_ERR_INVALID, "null batch");
+    return batch_get_fields(b, rep, pot, wpl, wmi);
+}
+int mft_fts_batch_get_species_fields(mft_fts_batch* b, int rep, float* W)
+{
+    MFT_REQUIRE(b, MFT_ERR_INVALID, "null batch");
+    return batch_species_fields(b, rep, W);
+}
+int mft_fts_batch_chemical_potentials(mft_fts_batch* b, int rep, double* mu)
+{
+    MFT_REQUIRE(b, MFT_ERR_INVALID, "null batch");
+    return batch_chemical_potentials(b, rep, mu);
+}
 
 } // extern "C"
 
@@ -2033,6 +2149,11 @@ int fts_pool_observables(FtsBatch* b, int rep, double* Q, double* H, double* par
     return batch_observables(b, rep, Q, H, parts);
 }
 int fts_pool_densities(FtsBatch* b, int rep, float* rho) { return batch_densities(b, rep, rho); }
+int fts_pool_get_fields(FtsBatch* b, int rep, int pot, float* wpl, float* wmi)
+{
+    return batch_get_fields(b, rep, pot, wpl, wmi);
+}
+int fts_pool_chemical_potentials(FtsBatch* b, int rep, double* mu) { return batch_chemical_potentials(b, rep, mu); }
 // whether every molecule's Q of the last density pass is finite (a runaway
 // CL trajectory overflows the propagators first); the stream must be idle
 int fts_pool_q_finite(FtsBatch* b, int rep, bool* ok)

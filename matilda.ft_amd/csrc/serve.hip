@@ -36,6 +36,8 @@ int fts_pool_observables(FtsBatch* b, int rep, double* Q, double* H, double* par
 int fts_pool_densities(FtsBatch* b, int rep, float* rho);
 int fts_pool_retire(FtsBatch* b, int rep);
 int fts_pool_q_finite(FtsBatch* b, int rep, bool* ok);
+int fts_pool_get_fields(FtsBatch* b, int rep, int pot, float* wpl, float* wmi);
+int fts_pool_chemical_potentials(FtsBatch* b, int rep, double* mu);
 size_t fts_bytes_estimate(const mft_fts_desc& d);
 
 struct ServeReq {
@@ -50,9 +52,10 @@ struct ServeReq {
     mft_fts_desc fd;
     std::vector<std::vector<float>> wpl, wmi;
     int niter = 0, remaining = 0, rep = -1;
-    std::vector<double> Q, parts;
+    std::vector<double> Q, parts, mu;
     double H[2] = {0, 0};
     std::vector<float> rho;
+    std::vector<std::vector<float>> wpl_out, wmi_out; // final fields per potential
     // TILD
     mft_tild_desc td;
     std::vector<mft_tild_potential> pots;
@@ -210,9 +213,22 @@ static int run_pool(Server* s, int max_iters)
             r.Q.assign(2 * d.nmolecules, 0.0);
             r.parts.assign(2 * (d.npotentials + d.nmolecules), 0.0);
             rc = fts_pool_observables(s->pool, r.rep, r.Q.data(), r.H, r.parts.data());
+            const size_t M = (size_t)d.nx * d.ny * d.nz;
             if (!rc) {
-                r.rho.resize((size_t)2 * d.nspecies * d.nx * d.ny * d.nz);
+                r.rho.resize(2 * d.nspecies * M);
                 rc = fts_pool_densities(s->pool, r.rep, r.rho.data());
+            }
+            if (!rc) {
+                r.mu.assign(2 * d.nmolecules, 0.0);
+                rc = fts_pool_chemical_potentials(s->pool, r.rep, r.mu.data());
+            }
+            r.wpl_out.assign(d.npotentials, {});
+            r.wmi_out.assign(d.npotentials, {});
+            for (int p = 0; p < d.npotentials && !rc; ++p) {
+                r.wpl_out[p].resize(2 * M);
+                if (d.potentials[p].kind == MFT_FTS_FLORY) r.wmi_out[p].resize(2 * M);
+                rc = fts_pool_get_fields(s->pool, r.rep, p, r.wpl_out[p].data(),
+                                         r.wmi_out[p].empty() ? nullptr : r.wmi_out[p].data());
             }
             if (rc) {
                 r.state = MFT_REQ_FAILED;
@@ -426,6 +442,25 @@ int mft_server_fts_result(mft_server* s, int64_t id, double* Q, double* H, doubl
     }
     if (parts) std::copy(r.parts.begin(), r.parts.end(), parts);
     if (rho) std::copy(r.rho.begin(), r.rho.end(), rho);
+    return MFT_OK;
+}
+
+int mft_server_fts_final_state(mft_server* s, int64_t id, int pot, float* wpl, float* wmi, double* mu)
+{
+    MFT_REQUIRE(s, MFT_ERR_INVALID, "null server");
+    auto it = s->reqs.find(id);
+    MFT_REQUIRE(it != s->reqs.end() && it->second->kind == 0, MFT_ERR_INVALID, "no CL-FTS request %lld",
+                (long long)id);
+    ServeReq& r = *it->second;
+    MFT_REQUIRE(r.state != MFT_REQ_FAILED, MFT_ERR_DEVICE, "request failed: %s", r.error.c_str());
+    MFT_REQUIRE(r.state == MFT_REQ_DONE, MFT_ERR_INVALID, "request %lld not finished", (long long)id);
+    MFT_REQUIRE(pot >= 0 && pot < r.fd.npotentials, MFT_ERR_INVALID, "potential out of range");
+    if (wpl) std::copy(r.wpl_out[pot].begin(), r.wpl_out[pot].end(), wpl);
+    if (wmi) {
+        MFT_REQUIRE(!r.wmi_out[pot].empty(), MFT_ERR_INVALID, "potential %d has no w- field", pot);
+        std::copy(r.wmi_out[pot].begin(), r.wmi_out[pot].end(), wmi);
+    }
+    if (mu) std::copy(r.mu.begin(), r.mu.end(), mu);
     return MFT_OK;
 }
 

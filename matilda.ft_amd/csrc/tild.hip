@@ -136,6 +136,7 @@ static int tild_setup(TildEngine* e)
         K.maskI = P.maskI;
         K.maskJ = P.maskJ;
         K.A = P.A;
+        K.eA = P.A;
         K.sig2 = P.sigma * P.sigma;
         K.Rp = P.Rp;
         K.table = nullptr;
@@ -451,28 +452,53 @@ static void build_groups(TildEngine* e)
     }
 }
 
+// PS_Potential::update_prefactor (ps_potential.cu:227-248): after step i of
+// a run of nsteps the amplitude is Ao_i + (Ao_f - Ao_i)/nsteps * (i + 1).
+static float ramp_amplitude(const mft_tild_potential& P, int64_t i, int64_t nsteps)
+{
+    return (float)((double)P.A + ((double)P.A_final - (double)P.A) / (double)nsteps * (double)(i + 1));
+}
+
 static void update_ramps(TildEngine* e, int64_t i, int64_t nsteps)
 {
-    // PS_Potential::update_prefactor (ps_potential.cu:227-248): after step
-    // i the amplitude is Ao_i + (Ao_f - Ao_i)/maxsteps * (i + 1).
     for (int p = 0; p < e->d.npotentials; ++p) {
         const mft_tild_potential& P = e->pots[p];
         if (!P.ramp) continue;
-        double A = (double)P.A + ((double)P.A_final - (double)P.A) / (double)nsteps * (double)(i + 1);
-        e->kp.pot[p].A = (float)A;
+        e->kp.pot[p].A = ramp_amplitude(P, i, nsteps);
+        e->kp.pot[p].eA = e->kp.pot[p].A;
     }
     build_groups(e);
 }
 
 // advance = false: a force/energy evaluation at the current state (the step
 // counter and the potential ramps stay where they are)
-static int do_steps(TildEngine* e, int nsteps, bool want_f, bool advance = true)
+// energy_last: the k-space pass of the last step also sums the pair
+// energies of that step's densities (the positions before the integrator
+// moves them), with the amplitudes the ramps reach after the step -- what
+// PS_Box::doTimeStep logs (PS_Box.cu:152-173: update_prefactor, then
+// writeData -> computeThermoProps -> CalcEnergy on the step's d_rho).
+static int do_steps(TildEngine* e, int nsteps, bool want_f, bool advance = true, bool energy_last = false)
 {
     const mft_tild_desc& d = e->d;
     int rc;
     for (int i = 0; i < nsteps; ++i) {
         if (e->timing && 2 * (e->evused + MFT_TK_COUNT) > e->evpool.size()) resolve_timing(e);
-        if ((rc = grid_forces(e))) return rc;
+        const bool elog = energy_last && i == nsteps - 1;
+        if (elog) {
+            e->kp.energy = 1;
+            for (int p = 0; p < d.npotentials; ++p) {
+                const mft_tild_potential& P = e->pots[p];
+                if (!P.ramp || !advance) continue;
+                e->kp.pot[p].eA = e->run_max > 0 ? ramp_amplitude(P, e->run_pos, e->run_max)
+                                                 : ramp_amplitude(P, i, nsteps);
+            }
+        }
+        rc = grid_forces(e);
+        if (elog) {
+            e->kp.energy = 0;
+            for (int p = 0; p < d.npotentials; ++p) e->kp.pot[p].eA = e->kp.pot[p].A;
+        }
+        if (rc) return rc;
         if (e->dpd.on && (rc = tild_dpd_forces(e))) return rc;
         const bool last = i == nsteps - 1;
         {
@@ -791,6 +817,47 @@ int mft_tild_energies(mft_tild* e, double* pot_energy, double* bond_energy)
         *bond_energy = 0.5 * b;
     }
     return MFT_OK;
+}
+
+// energies of every kind except the pair potentials, at the current positions
+static int tail_energies(TildEngine* e, double* pot_energy, double* bond_energy)
+{
+    int rc;
+    if (pot_energy) {
+        for (size_t k = 0; k < e->bias.size(); ++k)
+            if ((rc = tild_bias_energy(e, (int)k, &pot_energy[e->bias[k].pot]))) return rc;
+        for (size_t w = 0; w < e->wall_pot.size(); ++w)
+            if ((rc = tild_wall_energy(e, (int)w, &pot_energy[e->wall_pot[w]]))) return rc;
+    }
+    if (bond_energy) {
+        if ((rc = tild_bond_energy(e))) return rc;
+        MFT_HIP(hipStreamSynchronize(e->stream));
+        double b;
+        MFT_HIP(hipMemcpy(&b, e->eout + kMaxPots, sizeof(double), hipMemcpyDeviceToHost));
+        *bond_energy = 0.5 * b;
+    }
+    return MFT_OK;
+}
+
+int mft_tild_step_energies(mft_tild* e, int nsteps, double* pot_energy, double* bond_energy)
+{
+    MFT_REQUIRE(e && e->have_state, MFT_ERR_INVALID, "engine has no state");
+    MFT_REQUIRE(nsteps >= 1, MFT_ERR_INVALID, "step count must be at least 1");
+    const mft_tild_desc& d = e->d;
+    int rc;
+    if (d.integrator == MFT_INT_VV) {
+        // the reference's VV step computes its forces after the drift, at the
+        // positions it ends with, which the fused pass reaches only in the
+        // next step: evaluate there
+        if ((rc = do_steps(e, nsteps, false))) return rc;
+        return mft_tild_energies(e, pot_energy, bond_energy);
+    }
+    if ((rc = do_steps(e, nsteps, false, true, pot_energy && d.npotentials > 0))) return rc;
+    if (pot_energy && d.npotentials > 0) {
+        MFT_HIP(hipStreamSynchronize(e->stream));
+        MFT_HIP(hipMemcpy(pot_energy, e->eout, sizeof(double) * d.npotentials, hipMemcpyDeviceToHost));
+    }
+    return tail_energies(e, pot_energy, bond_energy);
 }
 
 int mft_tild_timing(mft_tild* e, int enable, double* avg_ms, int64_t* launches)

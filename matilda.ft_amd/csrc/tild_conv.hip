@@ -216,7 +216,9 @@ __device__ __forceinline__ void conv_z_body(const float2* __restrict__ S, float2
                             if ((Pp.maskJ >> c) & 1u) { b.x += spec[c][s].x; b.y += spec[c][s].y; }
                         }
                         long kidx = ix + (long)nx * (iy + (long)ny * (t + T * s));
-                        u = pot_shape(Pp, k2[s], kidx);
+                        KPot Pe = Pp;
+                        Pe.A = Pp.eA;
+                        u = pot_shape(Pe, k2[s], kidx);
                     }
                     e += (double)u * ((double)a.x * b.x + (double)a.y * b.y);
                 }

@@ -48,6 +48,7 @@ struct KPot {
     int kind;
     uint32_t maskI, maskJ;
     float A;      // current amplitude (ramped)
+    float eA;     // amplitude of the energy sum (A, or the amplitude after this step's ramp update)
     float sig2;
     float Rp;
     const float* table; // u(k)/A for erf kinds, else null

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <deque>
 #include <iostream>
 
 #include "matilda.h"
@@ -530,84 +531,28 @@ void FTS_Box::modifyBox(std::istringstream& iss)
     }
 }
 
-// FTS_Box::storePotentials (FTS_BoxFindSpinodal.cu:289-329)
-void FTS_Box::storePotentials(std::vector<std::complex<double>>& mi, std::vector<std::complex<double>>& pl,
-                              std::vector<double>& z, std::vector<double>& n, int ind)
-{
-    if (ind >= 3) die("Invalid index in storePotentials");
-    const int nPot = (int)Potentials.size(), nMol = (int)Molecs.size();
-    std::vector<float> a(2 * (size_t)M), b(2 * (size_t)M);
-    for (int p = 0; p < nPot; ++p) {
-        const bool doWmi = Potentials[p]->has_wmi;
-        check_mft(mft_fts_get_fields(sys, p, a.data(), doWmi ? b.data() : nullptr), "mft_fts_get_fields");
-        const size_t off = (size_t)ind * nPot * M + (size_t)p * M;
-        for (long i = 0; i < M; ++i) {
-            pl[off + i] = {a[2 * i], a[2 * i + 1]};
-            if (doWmi) mi[off + i] = {b[2 * i], b[2 * i + 1]};
-        }
-    }
-    updateNsites();
-    for (int m = 0; m < nMol; ++m) {
-        z[ind * nMol + m] = Molecs[m]->activity;
-        n[ind * nMol + m] = Molecs[m]->nSites;
-    }
-    std::cout << "  FINISHED STORE POTENTIALS WITH nSaved = " << ind << std::endl;
+// Spinodal search by activity sweep (the algorithm of
+// FTS_BoxFindSpinodal.cu:25-240).  Between sweep points the activity of one
+// grand canonical molecule is multiplied by f and the system re-equilibrated
+// for nSteps; each point records every molecule's activity and site count,
+// and the fields are kept in device snapshot slots (a ring of three, on the
+// GPU; the reference copies them to the host).  The composition of the swept
+// molecule is phi = n_k / sum_m n_m.  With the three latest points the search
+// stops when the slope |d phi / d z| over the newest pair has fallen below a
+// third of the slope over the previous pair.  Like the reference, slopes are
+// first compared at the third scaled point, after the starting point has
+// left the window, and the search ends at that first estimate (its
+// refinement loop sits behind an unconditional break,
+// FTS_BoxFindSpinodal.cu:214).  find_spinodal.dat gets one line per scaled
+// point: "activity nSites" of every molecule.  maxTries (absent from the
+// reference, which has no bound) ends a search that never meets the test.
+namespace {
+struct SweepPoint {
+    std::vector<double> z, n;   // activity and site count of every molecule
+    int slot = 0;               // device snapshot slot of the fields
+};
 }
 
-// FTS_Box::shuffleStorage (FTS_BoxFindSpinodal.cu:245-284): slot 1 -> 0, 2 -> 1
-void FTS_Box::shuffleStorage(std::vector<std::complex<double>>& mi, std::vector<std::complex<double>>& pl,
-                             std::vector<double>& z, std::vector<double>& n)
-{
-    const size_t nPot = Potentials.size(), nMol = Molecs.size(), S = nPot * M;
-    for (size_t i = 0; i < S; ++i) {
-        mi[i] = mi[S + i];
-        pl[i] = pl[S + i];
-        mi[S + i] = mi[2 * S + i];
-        pl[S + i] = pl[2 * S + i];
-    }
-    for (size_t m = 0; m < nMol; ++m) {
-        z[m] = z[nMol + m];
-        z[nMol + m] = z[2 * nMol + m];
-        n[m] = n[nMol + m];
-        n[nMol + m] = n[2 * nMol + m];
-    }
-}
-
-// FTS_Box::restorePotentials (FTS_BoxFindSpinodal.cu:333-371)
-void FTS_Box::restorePotentials(const std::vector<std::complex<double>>& mi,
-                                const std::vector<std::complex<double>>& pl, const std::vector<double>& z, int ind)
-{
-    if (ind >= 3) die("Invalid index in storePotentials");
-    const int nPot = (int)Potentials.size(), nMol = (int)Molecs.size();
-    std::vector<float> a(2 * (size_t)M), b(2 * (size_t)M);
-    for (int p = 0; p < nPot; ++p) {
-        const bool doWmi = Potentials[p]->has_wmi;
-        const size_t off = (size_t)ind * nPot * M + (size_t)p * M;
-        for (long i = 0; i < M; ++i) {
-            a[2 * i] = (float)pl[off + i].real();
-            a[2 * i + 1] = (float)pl[off + i].imag();
-            if (doWmi) {
-                b[2 * i] = (float)mi[off + i].real();
-                b[2 * i + 1] = (float)mi[off + i].imag();
-            }
-        }
-        check_mft(mft_fts_set_fields(sys, p, a.data(), doWmi ? b.data() : nullptr), "mft_fts_set_fields");
-    }
-    for (int m = 0; m < nMol; ++m) {
-        Molecs[m]->activity = z[ind * nMol + m];
-        if (Molecs[m]->activity > 0.0) setActivity(Molecs[m], Molecs[m]->activity);
-        std::cout << "  Molec " << m << " activity restored to: " << Molecs[m]->activity << std::endl;
-    }
-    check_mft(mft_fts_init(sys), "mft_fts_init");
-}
-
-// FTS_Box::findSpinodal (FTS_BoxFindSpinodal.cu:25-240): scale the activity of
-// one molecule by f per try, re-equilibrate for nSteps, and stop at the first
-// try where |d phi / d z| falls below a third of the previous difference.
-// The reference's refinement loop ends in an unconditional break
-// (FTS_BoxFindSpinodal.cu:214), so the search ends at that first estimate
-// here as well.  maxTries (not in the reference, which has no bound) stops a
-// search that never meets the criterion.
 void FTS_Box::findSpinodal(std::istringstream& iss)
 {
     double phiTol = 1.0E-4, f = 0.8, fScaleFactor = 0.5;
@@ -625,12 +570,6 @@ void FTS_Box::findSpinodal(std::istringstream& iss)
         else if (word == "initial_f") iss >> f;
         else if (word == "maxTries") iss >> maxTries;
     }
-    std::cout << "Searching for spinodal point using: " << std::endl;
-    std::cout << "  phiTol = " << phiTol << std::endl;
-    std::cout << "  f = " << f << std::endl;
-    std::cout << "  fScaleFactor = " << fScaleFactor << std::endl;
-    std::cout << "  molecule = " << molecIndex + 1 << std::endl;
-    std::cout << "  initalf = " << f << std::endl;
     if (molecIndex < 0 || molecIndex >= (int)Molecs.size()) die("Invalid molecule index - out of range!");
     if (Molecs[molecIndex]->activity <= 0.0) die("findSpinodal scales the activity of a grand canonical molecule");
     if (dry_run) {
@@ -640,64 +579,61 @@ void FTS_Box::findSpinodal(std::istringstream& iss)
         std::fflush(stdout);
         return;
     }
-    const int nBkps = 3, nPot = (int)Potentials.size(), nMol = (int)Molecs.size();
-    int nSaved = 0;
-    std::ofstream otp(fileName);
-    std::vector<double> Zstorage((size_t)nMol * nBkps), nStorage((size_t)nMol * nBkps);
-    std::vector<std::complex<double>> wplBkps((size_t)nPot * M * nBkps), wmiBkps((size_t)nPot * M * nBkps);
-    double phi[3] = {0.0, 0.0, 0.0};
+    std::cout << "findSpinodal: molecule " << molecIndex + 1 << ", activity factor " << f << " per point, "
+              << nSteps << " steps per point (phiTolerance " << phiTol << ", fScale " << fScaleFactor
+              << " are read for the refinement stage, which the reference never reaches)" << std::endl;
+
+    const size_t nMol = Molecs.size();
+    std::ofstream dat(fileName);
+    std::deque<SweepPoint> window;
+    int nextSlot = 0;
+    auto record = [&]() {
+        updateNsites();
+        SweepPoint p;
+        for (auto* m : Molecs) {
+            p.z.push_back(m->activity);
+            p.n.push_back(m->nSites);
+        }
+        p.slot = nextSlot;
+        nextSlot = (nextSlot + 1) % 3;
+        check_mft(mft_fts_snapshot(sys, p.slot, 0), "mft_fts_snapshot");
+        window.push_back(p);
+        if (window.size() > 3) window.pop_front();
+    };
+    auto fraction = [&](const SweepPoint& p) {
+        double tot = 0.0;
+        for (size_t m = 0; m < nMol; ++m) tot += p.n[m];
+        return p.n[molecIndex] / tot;
+    };
+    auto slope = [&](const SweepPoint& a, const SweepPoint& b) {
+        return std::fabs((fraction(b) - fraction(a)) / (b.z[molecIndex] - a.z[molecIndex]));
+    };
 
     NVT(nSteps);
-    storePotentials(wmiBkps, wplBkps, Zstorage, nStorage, nSaved);
-    nSaved++;
-    int tryCt = 0;
-    const std::string command = "molecule " + std::to_string(molecIndex + 1) + " activity scale " + std::to_string(f);
-    int foundSpinodalPoint = 0, initializing = 1;
-    while (!foundSpinodalPoint) {
-        std::istringstream modss(command);
-        modifyBox(modss);
+    record();
+    const std::string scale = "molecule " + std::to_string(molecIndex + 1) + " activity scale " + std::to_string(f);
+    for (long tries = 0;; ++tries) {
+        std::istringstream cmd(scale);
+        modifyBox(cmd);
         NVT(nSteps);
-        if (initializing) {
-            std::cout << "  nsaved < 2" << std::endl;
-            storePotentials(wmiBkps, wplBkps, Zstorage, nStorage, nSaved);
-            nSaved++;
-            if (nSaved == 3) {
-                std::cout << "INITIALIZING DONE\n\n" << std::endl;
-                initializing = 0;
-                nSaved = 2;
+        record();
+        for (auto* m : Molecs) dat << m->activity << " " << m->nSites << " ";
+        dat << std::endl;
+        const SweepPoint& p = window.back();
+        std::cout << "  point " << tries + 1 << ": z = " << p.z[molecIndex] << ", phi = " << fraction(p) << std::endl;
+        if (tries >= 2) {
+            const double older = slope(window[0], window[1]), newer = slope(window[1], window[2]);
+            std::cout << "    |dphi/dz| " << older << " -> " << newer << std::endl;
+            if (newer < older / 3.0) {
+                std::cout << "findSpinodal: spinodal estimate between z = " << window[0].z[molecIndex] << " and "
+                          << window[2].z[molecIndex] << " (phi " << fraction(window[0]) << " .. "
+                          << fraction(window[2]) << ")" << std::endl;
+                return;
             }
-        } else {
-            std::cout << "  nsaved == 2" << std::endl;
-            if (tryCt >= 2) shuffleStorage(wmiBkps, wplBkps, Zstorage, nStorage);
-            storePotentials(wmiBkps, wplBkps, Zstorage, nStorage, nSaved);
-            const int m0 = 0 * nMol + molecIndex, m1 = 1 * nMol + molecIndex, m2 = 2 * nMol + molecIndex;
-            double nTot[3] = {0.0, 0.0, 0.0};
-            for (int m = 0; m < nMol; ++m)
-                for (int k = 0; k < 3; ++k) nTot[k] += nStorage[k * nMol + m];
-            for (int k = 0; k < 3; ++k) phi[k] = nStorage[k * nMol + molecIndex] / nTot[k];
-            const double dphi01 = std::fabs((phi[1] - phi[0]) / (Zstorage[m1] - Zstorage[m0]));
-            const double dphi12 = std::fabs((phi[2] - phi[1]) / (Zstorage[m2] - Zstorage[m1]));
-            std::cout << "  nSaved: " << nSaved << " nTot: " << nTot[0] << " " << nTot[1] << " " << nTot[2]
-                      << std::endl;
-            std::cout << "  phi: " << phi[0] << " " << phi[1] << " " << phi[2] << std::endl;
-            if (dphi12 < dphi01 / 3.0) foundSpinodalPoint = 1;
-            else std::cout << "  dphi01: " << dphi01 << " dphi12: " << dphi12 << std::endl;
         }
-        std::cout << "Try " << tryCt << ", " << nSaved << ": ";
-        for (int i = 0; i < 3 * nMol; ++i) std::cout << Zstorage[i] << " " << nStorage[i] << "   ";
-        std::cout << std::endl;
-        for (auto* m : Molecs) otp << m->activity << " " << m->nSites << " ";
-        otp << std::endl;
-        tryCt++;
-        if (maxTries > 0 && tryCt >= maxTries && !foundSpinodalPoint) {
+        if (maxTries > 0 && tries + 1 >= maxTries) {
             std::cout << "findSpinodal: no spinodal point within " << maxTries << " tries" << std::endl;
             return;
         }
     }
-    std::cout << "spinodal found!\nz = " << Zstorage[0 * nMol + molecIndex] << " " << Zstorage[1 * nMol + molecIndex]
-              << " " << Zstorage[2 * nMol + molecIndex] << std::endl;
-    std::cout << "phi = " << phi[0] << " " << phi[1] << " " << phi[2] << std::endl;
-    std::cout << "\n\n\n\n" << std::endl;
-    (void)phiTol;
-    (void)fScaleFactor;
 }

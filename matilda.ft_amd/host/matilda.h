@@ -283,6 +283,10 @@ public:
     void doTimeStep(int step) override;                 // PS_Box.cu:73-207
     void writeData(int step) override;                  // PS_Box.cu:339-372
     void computeThermoProps();                          // PS_Box.cu:282-317
+    void takeSteps(int n, bool energies);               // engine steps; keeps the log energies of the last
+    std::vector<double> stepPe;                         // energies of the last step taken with takeSteps(.., true)
+    double stepBe = 0.0;
+    bool stepEnergiesValid = false;
     void writeDataConfig(const std::string& name);      // LAMMPS data, PS_Box.cu:629-720
     void writeLammpsTraj(int step);                     // PS_Box.cu:756-801
     void writeKFieldFloat(const std::string& name, const float* dat); // PS_Box.cu:531-560
@@ -436,7 +440,7 @@ public:
     void computeHamiltonian();                          // FTS_Box.cu:590-602
     int converged(int step) override;                   // FTS_Box.cu:715-760
     int findSpeciesInteger(const std::string& s) const;
-    void findSpinodal(std::istringstream& iss) override;  // FTS_BoxFindSpinodal.cu:25-240
+    void findSpinodal(std::istringstream& iss) override;  // algorithm of FTS_BoxFindSpinodal.cu:25-240
     void modifyBox(std::istringstream& iss) override;     // FTS_Box.cu:674-708
     void setActivity(const FTS_Molec* m, double activity);
     void updateNsites();
@@ -458,12 +462,4 @@ private:
     bool initialized = false;
     bool stepped = false;
     void finishInitialization();
-    // host copies of every field and the molecules' activities and site
-    // counts (FTS_BoxFindSpinodal.cu:245-371)
-    void storePotentials(std::vector<std::complex<double>>& mi, std::vector<std::complex<double>>& pl,
-                         std::vector<double>& z, std::vector<double>& n, int ind);
-    void shuffleStorage(std::vector<std::complex<double>>& mi, std::vector<std::complex<double>>& pl,
-                        std::vector<double>& z, std::vector<double>& n);
-    void restorePotentials(const std::vector<std::complex<double>>& mi, const std::vector<std::complex<double>>& pl,
-                           const std::vector<double>& z, int ind);
 };

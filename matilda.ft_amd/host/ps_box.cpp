@@ -665,8 +665,11 @@ void PS_Box::readDataConfig(const std::string& name)
         for (int i = 0; i < nb; ++i) {
             if (!next_nonblank(line)) die("Premature end of " + name);
             std::istringstream ls(line);
-            int cnt, bt, i1, i2;
+            int cnt = 0, bt = 0, i1 = 0, i2 = 0;
             ls >> cnt >> bt >> i1 >> i2;
+            if (!ls) die("readData: malformed bond line '" + line + "'");
+            if (std::min(i1, i2) < 1 || std::max(i1, i2) > nat) die("readData: bond partner out of range");
+            if (bt < 1 || bt > std::max(nbt, (int)bondK.size())) die("readData: bond type out of range");
             --bt;
             --i1;
             --i2;
@@ -804,7 +807,13 @@ void PS_Box::computeThermoProps()
 {
     std::vector<double> pe(std::max<size_t>(1, potentials.size()), 0.0);
     double be = 0.0;
-    check_mft(mft_tild_energies(eng, pe.data(), &be), "mft_tild_energies");
+    if (stepEnergiesValid) {
+        // the energies of the step just taken (PS_Box::writeData after doTimeStep)
+        pe = stepPe;
+        be = stepBe;
+    } else {
+        check_mft(mft_tild_energies(eng, pe.data(), &be), "mft_tild_energies");
+    }
     Ubond = (float)be;
     double ae = 0.0;
     if (nAnglesTot > 0) check_mft(mft_tild_angle_energy(eng, &ae), "mft_tild_angle_energy");
@@ -843,7 +852,11 @@ void PS_Box::writeData(int step)
 void PS_Box::doTimeStep(int step)
 {
     for (auto* i : integrators) i->Integrate_1();
-    check_mft(mft_tild_step(eng, 1), "mft_tild_step");
+    if (logFreq > 0 && step % logFreq == 0) {
+        takeSteps(1, true);
+    } else {
+        takeSteps(1, false);
+    }
     for (auto* p : potentials) p->update_prefactor(step, maxSteps);
     for (auto* c : computes) c->do_compute(step);
     if (logFreq > 0 && step % logFreq == 0) {
@@ -1059,7 +1072,8 @@ void PS_Box::NVT(int nSteps)
                     break;
                 }
         if (next > i) {
-            check_mft(mft_tild_step(eng, next - i), "mft_tild_step");
+            // the last chunk keeps its energies for the final log row
+            takeSteps(next - i, next == maxSteps);
             for (int s = i; s < next; ++s)
                 for (auto* p : potentials) p->update_prefactor(s, maxSteps);
             totSteps += next - i;
@@ -1074,6 +1088,20 @@ void PS_Box::NVT(int nSteps)
     std::cout << "NVT Finished, writing final output" << std::endl;
     writeData(maxSteps);
     writeDataConfig("final.input.data");
+}
+
+// n engine steps; with energies, the log energies of the last one are kept
+// for writeData (pair energies from that step's densities, PS_Box.cu:152-173)
+void PS_Box::takeSteps(int n, bool energies)
+{
+    if (energies) {
+        stepPe.assign(std::max<size_t>(1, potentials.size()), 0.0);
+        check_mft(mft_tild_step_energies(eng, n, stepPe.data(), &stepBe), "mft_tild_step_energies");
+        stepEnergiesValid = true;
+    } else {
+        check_mft(mft_tild_step(eng, n), "mft_tild_step");
+        stepEnergiesValid = false;
+    }
 }
 
 // LAMMPS-style data file of the current state (PS_Box.cu:629-720)

@@ -427,6 +427,14 @@ class TildEngine:
         check(lib().mft_tild_energies(self.h, _ptr(pe), C.byref(be)))
         return pe[: self.npot], be.value
 
+    def step_energies(self, n=1):
+        """n steps, then the energies a log after the last one reports (pair
+        energies of that step's density pass, bonds at the new positions)."""
+        pe = np.zeros(max(1, self.npot), np.float64)
+        be = _d(0)
+        check(lib().mft_tild_step_energies(self.h, n, _ptr(pe), C.byref(be)))
+        return pe[: self.npot], be.value
+
     def timing(self, enable):
         ms = np.zeros(len(TK_NAMES), np.float64)
         n = np.zeros(len(TK_NAMES), np.int64)
@@ -677,6 +685,17 @@ class FtsSystem:
         check(lib().mft_fts_observables(self.h, _ptr(Q), _ptr(H), _ptr(parts)))
         return Q.view(np.complex128), complex(H[0], H[1]), parts.view(np.complex128)
 
+    def chemical_potentials(self):
+        mu = np.zeros(2 * self.nmol)
+        check(lib().mft_fts_chemical_potentials(self.h, _ptr(mu)))
+        return mu.view(np.complex128)
+
+    def snapshot(self, slot):
+        check(lib().mft_fts_snapshot(self.h, slot, 0))
+
+    def restore(self, slot):
+        check(lib().mft_fts_snapshot(self.h, slot, 1))
+
     def set_activity(self, m, activity):
         check(lib().mft_fts_set_activity(self.h, m, _d(activity)))
 
@@ -738,6 +757,51 @@ class FtsBatch:
         r = np.empty(s["nspecies"] * M, np.complex64)
         check(lib().mft_fts_batch_get_densities(self.h, rep, _ptr(r)))
         return r.reshape(s["nspecies"], M)
+
+    def get_fields(self, rep, pot, minus=False):
+        M = int(np.prod(self.specs[rep]["Nx"]))
+        a = np.empty(M, np.complex64)
+        b = np.empty(M, np.complex64) if minus else None
+        check(lib().mft_fts_batch_get_fields(self.h, rep, pot, _ptr(a), _ptr(b)))
+        return (a, b) if minus else a
+
+    def species_fields(self, rep):
+        s = self.specs[rep]
+        M = int(np.prod(s["Nx"]))
+        r = np.empty(s["nspecies"] * M, np.complex64)
+        check(lib().mft_fts_batch_get_species_fields(self.h, rep, _ptr(r)))
+        return r.reshape(s["nspecies"], M)
+
+    def chemical_potentials(self, rep):
+        mu = np.zeros(2 * len(self.specs[rep]["molecules"]))
+        check(lib().mft_fts_batch_chemical_potentials(self.h, rep, _ptr(mu)))
+        return mu.view(np.complex128)
+
+    def replica(self, rep):
+        """A view of replica rep with the single-system accessors."""
+        return BatchReplica(self, rep)
+
+
+class BatchReplica:
+    """Accessors of one replica of an FtsBatch, named like FtsSystem's."""
+
+    def __init__(self, batch, rep):
+        self.b, self.rep = batch, rep
+
+    def observables(self):
+        return self.b.observables(self.rep)
+
+    def densities(self):
+        return self.b.densities(self.rep)
+
+    def species_fields(self):
+        return self.b.species_fields(self.rep)
+
+    def get_fields(self, pot, minus=False):
+        return self.b.get_fields(self.rep, pot, minus)
+
+    def chemical_potentials(self):
+        return self.b.chemical_potentials(self.rep)
 
 
 REQ_STATE = {0: "queued", 1: "running", 2: "done", 3: "failed"}
@@ -825,6 +889,20 @@ class Server:
         rho = np.empty(spec["nspecies"] * M, np.complex64)
         check(lib().mft_server_fts_result(self.h, C.c_int64(rid), _ptr(Q), _ptr(H), _ptr(parts), _ptr(rho)))
         return Q.view(np.complex128), complex(H[0], H[1]), parts.view(np.complex128), rho.reshape(-1, M)
+
+    def fts_final_state(self, rid):
+        """Final fields {(pot, 'pl'|'mi'): array} and chemical potentials."""
+        spec = self.meta[rid][1]
+        M = int(np.prod(spec["Nx"]))
+        fields, mu = {}, np.zeros(2 * len(spec["molecules"]))
+        for p, pot in enumerate(spec["potentials"]):
+            a = np.empty(M, np.complex64)
+            b = np.empty(M, np.complex64) if pot["kind"] == "Flory" else None
+            check(lib().mft_server_fts_final_state(self.h, C.c_int64(rid), p, _ptr(a), _ptr(b), _ptr(mu)))
+            fields[(p, "pl")] = a
+            if b is not None:
+                fields[(p, "mi")] = b
+        return fields, mu.view(np.complex128)
 
     def tild_result(self, rid):
         cfg = self.meta[rid][1]

@@ -132,26 +132,34 @@ void orc_fft1d(cplx* data, int n, int stride, int sign)
 
 void orc_fft3d(cplx* data, int nx, int ny, int nz, int sign)
 {
+    /* lines are independent: with OpenMP (the CPU-baseline build) each thread
+     * transforms whole lines, so the result does not depend on the thread count */
     size_t plane = (size_t)nx * ny;
     int nmax = nx > ny ? nx : ny;
     if (nz > nmax) nmax = nz;
-    cplx* buf = (cplx*)malloc(sizeof(cplx) * 3 * (size_t)nmax);
-    if (nx > 1) {
-        const cplx* w = twiddles(nx, sign);
-        for (int z = 0; z < nz; ++z)
-            for (int y = 0; y < ny; ++y) fft_line(data + z * plane + (size_t)y * nx, nx, 1, w, buf);
+    const cplx* wx = nx > 1 ? twiddles(nx, sign) : NULL;
+    const cplx* wy = ny > 1 ? twiddles(ny, sign) : NULL;
+    const cplx* wz = nz > 1 ? twiddles(nz, sign) : NULL;
+#pragma omp parallel
+    {
+        cplx* buf = (cplx*)malloc(sizeof(cplx) * 3 * (size_t)nmax);
+        if (wx) {
+#pragma omp for schedule(static)
+            for (int z = 0; z < nz; ++z)
+                for (int y = 0; y < ny; ++y) fft_line(data + z * plane + (size_t)y * nx, nx, 1, wx, buf);
+        }
+        if (wy) {
+#pragma omp for schedule(static)
+            for (int z = 0; z < nz; ++z)
+                for (int x = 0; x < nx; ++x) fft_line(data + z * plane + x, ny, (size_t)nx, wy, buf);
+        }
+        if (wz) {
+#pragma omp for schedule(static)
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x) fft_line(data + (size_t)y * nx + x, nz, plane, wz, buf);
+        }
+        free(buf);
     }
-    if (ny > 1) {
-        const cplx* w = twiddles(ny, sign);
-        for (int z = 0; z < nz; ++z)
-            for (int x = 0; x < nx; ++x) fft_line(data + z * plane + x, ny, (size_t)nx, w, buf);
-    }
-    if (nz > 1) {
-        const cplx* w = twiddles(nz, sign);
-        for (int y = 0; y < ny; ++y)
-            for (int x = 0; x < nx; ++x) fft_line(data + (size_t)y * nx + x, nz, plane, w, buf);
-    }
-    free(buf);
 }
 
 void orc_box_fft(cplx* data, const int Nx[3], int Dim, int dir)
@@ -161,6 +169,7 @@ void orc_box_fft(cplx* data, const int Nx[3], int Dim, int dir)
     if (dir == 1) {
         orc_fft3d(data, Nx[0], Nx[1], nz, -1);
         real s = (real)(1.0 / (double)M);
+#pragma omp parallel for schedule(static)
         for (size_t i = 0; i < M; ++i) data[i] *= s;
     } else {
         orc_fft3d(data, Nx[0], Nx[1], nz, +1);

@@ -29,6 +29,7 @@ static void propagate(const cplx* W, int nblocks, const int* N, const int* spec,
     for (int bb = 0; bb < nblocks; ++bb) {
         int b = reverse ? nblocks - 1 - bb : bb;
         const cplx* Wb = W + (size_t)spec[b] * M;
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < M; ++i) expW[i] = cexp(-Wb[i]);
         for (int s = 0; s < N[b]; ++s) {
             cplx* qs = q + (size_t)ind * M;
@@ -37,8 +38,10 @@ static void propagate(const cplx* W, int nblocks, const int* N, const int* spec,
             } else {
                 memcpy(a, q + (size_t)(ind - 1) * M, sizeof(cplx) * (size_t)M);
                 orc_box_fft(a, Nx, Dim, 1);
+#pragma omp parallel for schedule(static)
                 for (int i = 0; i < M; ++i) a[i] *= g[i];
                 orc_box_fft(a, Nx, Dim, -1);
+#pragma omp parallel for schedule(static)
                 for (int i = 0; i < M; ++i) qs[i] = a[i] * expW[i];
             }
             ++ind;
@@ -86,6 +89,7 @@ void orc_fts_density(const cplx* W, int nblocks, const int* N, const int* spec,
         for (int s = 0; s < N[b]; ++s) {
             const cplx* qs = q + (size_t)ind * M;
             const cplx* qd = qdag + (size_t)(Ntot - 1 - ind) * M;
+#pragma omp parallel for schedule(static)
             for (int i = 0; i < M; ++i) rb[i] += qs[i] * qd[i];
             ++ind;
         }

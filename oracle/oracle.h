@@ -101,6 +101,8 @@ void orc_gjf_step(real* x, real* xo, real* noiseOld, const real* f,
 /* d_EM_integrator (ps_integratorEM.cu:39-82). */
 void orc_em_step(real* x, const real* f, double delt, const float* L,
                  const int* sites, int nsites, int Dim, const double* xi);
+/* Threads of the OpenMP build used for the CPU baseline (no effect otherwise). */
+void orc_set_threads(int n);
 
 /* ---------------------------------------------------------------- FTS --- */
 /* LinearMolec bond factor (fts_molecule_linear.cu:109-122): exp(-k^2/(Nr-1)). */

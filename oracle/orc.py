@@ -30,11 +30,25 @@ _d = C.c_double
 _p = C.c_void_p
 
 
+_OPENMP = {"on": False}
+
+
+def use_openmp(on: bool, threads: int = 1):
+    """Switch lib() to the OpenMP builds (the benchmark's CPU-baseline leg) with
+    the given thread count, or back to the single-thread builds the tests use."""
+    _OPENMP["on"] = bool(on)
+    if on:
+        for bits in (64, 32):
+            lib(bits).orc_set_threads(int(threads))
+
+
 def lib(bits: int = 64):
-    """Load liboracle64.so (FP64 oracle) or liboracle32.so (FP32 host build)."""
-    if bits in _LIBS:
-        return _LIBS[bits]
-    path = os.path.join(_HERE, f"liboracle{bits}.so")
+    """Load liboracle64.so (FP64 oracle) or liboracle32.so (FP32 host build),
+    or their OpenMP builds after use_openmp(True)."""
+    key = (bits, _OPENMP["on"])
+    if key in _LIBS:
+        return _LIBS[key]
+    path = os.path.join(_HERE, f"liboracle{bits}{'_omp' if _OPENMP['on'] else ''}.so")
     if not os.path.exists(path):
         raise FileNotFoundError(f"{path} missing: build the oracle (make -C oracle)")
     L = C.CDLL(path)
@@ -68,11 +82,12 @@ def lib(bits: int = 64):
         "orc_fts_update_empc": (None, [_p, _p, _p, _p, _p, _d, _i]),
         "orc_fts_update_1s": (None, [_p, _p, _p, _d, _i]),
     }
+    sig["orc_set_threads"] = (None, [_i])
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
         fn.restype = res
         fn.argtypes = args
-    _LIBS[bits] = L
+    _LIBS[key] = L
     return L
 
 
@@ -178,6 +193,15 @@ def potential_grid_force(rhoJ, uk, Nx, L, factor, gridF, bits=64):
 
 def potential_energy(rhoI, rhoJ, uk, Nx, gvol, bits=64):
     return lib(bits).orc_potential_energy(_ptr(rhoI), _ptr(rhoJ), _ptr(uk), _ptr(_ia(list(Nx) + [1])), len(Nx), gvol)
+
+
+def pair_energy(kind, rhoI, rhoJ, uk, Nx, gvol, bits=64):
+    """One potential's CalcEnergy: gvol sum rho_I (u * rho_J)
+    (PS_Potential::CalcEnergy, ps_potential.cu:133-178); the charges potential
+    takes half of it, 0.5 gvol sum rho_q (u * rho_q) (NBCharge::CalcEnergy,
+    ps_potentialCharges.cu:115)."""
+    e = potential_energy(rhoI, rhoJ, uk, Nx, gvol, bits)
+    return 0.5 * e if kind == "charges" else e
 
 
 def map_forces(f, gridF, sites, inds, W, G, gvol, Dim, q=None, bits=64):
@@ -560,7 +584,7 @@ def tild_forces(sys: TildSystem, bits=64, want_energy=False):
             gf = gridF.setdefault("charges", np.zeros(M * Dim, _rdt(bits)))
             potential_grid_force(rhoq, uk, sys.Nx, sys.L, 1.0, gf, bits)
             if want_energy:
-                energies.append(0.5 * potential_energy(rhoq, rhoq, uk, sys.Nx, gvol_dev, bits))
+                energies.append(pair_energy("charges", rhoq, rhoq, uk, sys.Nx, gvol_dev, bits))
             continue
         gI = gridF.setdefault(pot.I, np.zeros(M * Dim, _rdt(bits)))
         potential_grid_force(rho[pot.J], uk, sys.Nx, sys.L, 2.0 if pot.I == pot.J else 1.0, gI, bits)
@@ -568,7 +592,7 @@ def tild_forces(sys: TildSystem, bits=64, want_energy=False):
             gJ = gridF.setdefault(pot.J, np.zeros(M * Dim, _rdt(bits)))
             potential_grid_force(rho[pot.I], uk, sys.Nx, sys.L, 1.0, gJ, bits)
         if want_energy:
-            energies.append(potential_energy(rho[pot.I], rho[pot.J], uk, sys.Nx, gvol_dev, bits))
+            energies.append(pair_energy(pot.kind, rho[pot.I], rho[pot.J], uk, sys.Nx, gvol_dev, bits))
     for g, gf in gridF.items():
         q = sys.topo["charge"] if g == "charges" else None
         map_forces(f, gf, sys.group_sites(g), inds, W, G, gvol_dev, Dim, q=q, bits=bits)
@@ -1078,6 +1102,20 @@ def fts_hamiltonian(sys: FtsSystem):
         parts.append(h)
         H += h
     return H, parts
+
+
+def fts_chemical_potentials(sys: FtsSystem):
+    """Chemical potential of every molecule: ln(n / (V Q)) when canonical, the
+    ideal-chain form of the density factor n/(Q V) (fts_molecule_linear.cu:
+    286-291), ln z when grand canonical (factor z).  The reference has no
+    chemical-potential output; this is the definition the engine follows."""
+    out = []
+    for m, Q in zip(sys.molecules, sys.Q):
+        if m.nmolecs > 0.0:
+            out.append(np.log(complex(m.nmolecs / sys.V())) - np.log(complex(Q)))
+        else:
+            out.append(np.log(complex(m.activity)))
+    return out
 
 
 def fts_init(sys: FtsSystem, bits=64):

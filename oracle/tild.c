@@ -6,6 +6,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* constants.h:9 */
 #define REF_PI2 6.2831853071f
@@ -123,6 +126,7 @@ void orc_grid_weights(const float* x, int ns, const int Nx[3], const float dx[3]
         gvol *= (double)dx[j];
     }
     int shift = p / 2;
+#pragma omp parallel for schedule(static)
     for (int id = 0; id < ns; ++id) {
         real w[3][6];
         int g[3];
@@ -163,6 +167,37 @@ void orc_grid_weights(const float* x, int ns, const int Nx[3], const float dx[3]
 void orc_fill_density(real* rho, real* rhoq, const float* q, const int* sites,
                       int nsites, const int* inds, const real* W, int G)
 {
+#ifdef _OPENMP
+    /* the CPU-baseline build: private grids per thread, summed at the end */
+    int M = 0;
+    for (int s = 0; s < nsites; ++s)
+        for (int i = 0; i < G; ++i) {
+            int g = inds[(size_t)sites[s] * G + i];
+            if (g >= M) M = g + 1;
+        }
+#pragma omp parallel
+    {
+        real* lr = (real*)calloc((size_t)M, sizeof(real));
+        real* lq = rhoq ? (real*)calloc((size_t)M, sizeof(real)) : NULL;
+#pragma omp for schedule(static)
+        for (int s = 0; s < nsites; ++s) {
+            int p = sites[s];
+            for (int i = 0; i < G; ++i) {
+                size_t idx = (size_t)p * G + i;
+                lr[inds[idx]] += W[idx];
+                if (lq) lq[inds[idx]] += W[idx] * (real)q[p];
+            }
+        }
+#pragma omp critical
+        {
+            for (int g = 0; g < M; ++g) rho[g] += lr[g];
+            if (lq)
+                for (int g = 0; g < M; ++g) rhoq[g] += lq[g];
+        }
+        free(lr);
+        free(lq);
+    }
+#else
     for (int s = 0; s < nsites; ++s) {
         int p = sites[s];
         for (int i = 0; i < G; ++i) {
@@ -171,6 +206,7 @@ void orc_fill_density(real* rho, real* rhoq, const float* q, const int* sites,
             if (rhoq) rhoq[inds[idx]] += W[idx] * (real)q[p];
         }
     }
+#endif
 }
 
 void orc_uk_table(int kind, double A, double sigma, double Rp, const int Nx[3],
@@ -178,6 +214,7 @@ void orc_uk_table(int kind, double A, double sigma, double Rp, const int Nx[3],
 {
     int M = Nx[0] * Nx[1] * (Dim == 3 ? Nx[2] : 1);
     double sig2 = sigma * sigma;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < M; ++i) {
         float kf[3];
         double k2 = (double)orc_kvec_f(i, Nx, L, Dim, kf);
@@ -222,6 +259,7 @@ void orc_potential_grid_force(const real* rhoJ, const real* uk, const int Nx[3],
     for (int i = 0; i < M; ++i) a[i] = rhoJ[i];
     orc_box_fft(a, Nx, Dim, 1);
     for (int j = 0; j < Dim; ++j) {
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < M; ++i) {
             float kf[3];
             orc_kvec_f(i, Nx, L, Dim, kf);
@@ -255,6 +293,7 @@ void orc_map_forces(real* f, const real* gridF, const float* q, const int* sites
                     int nsites, const int* inds, const real* W, int G, double gvol,
                     int Dim)
 {
+#pragma omp parallel for schedule(static)
     for (int s = 0; s < nsites; ++s) {
         int p = sites[s];
         double qi = q ? (double)q[p] : 1.0;
@@ -281,6 +320,7 @@ void orc_bonds(real* f, const int* nBonds, const int* bondedTo, const int* bondT
                const float* x, const float* L, const float* Lh, int ns,
                int maxbonds, int Dim)
 {
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < ns; ++i) {
         double lf[3] = {0, 0, 0};
         for (int b = 0; b < nBonds[i]; ++b) {
@@ -352,6 +392,7 @@ void orc_gjf_step(real* x, real* xo, real* noiseOld, const real* f,
                   const double* coef, const int* type, const float* L,
                   const int* sites, int nsites, int Dim, const double* xi)
 {
+#pragma omp parallel for schedule(static)
     for (int s = 0; s < nsites; ++s) {
         int ind = sites[s];
         const double* c = coef + 5 * type[ind];
@@ -377,6 +418,7 @@ void orc_em_step(real* x, const real* f, double delt, const float* L,
                  const int* sites, int nsites, int Dim, const double* xi)
 {
     double nmag = sqrt(2.0 * delt);
+#pragma omp parallel for schedule(static)
     for (int s = 0; s < nsites; ++s) {
         int ind = sites[s];
         for (int j = 0; j < Dim; ++j) {
@@ -387,4 +429,14 @@ void orc_em_step(real* x, const real* f, double delt, const float* L,
             x[ai] = (real)xt;
         }
     }
+}
+
+/* Threads of the CPU-baseline build (no effect in the single-thread build). */
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
 }

@@ -271,3 +271,22 @@ def test_dpd_potential_keyword(tmp_path):
     assert r.returncode == 1 and "dpd: no neighbor_list found for group A" in r.stdout
     r = dry(tmp_path, base.replace("logFreq", "neighbor_list A 1.2\npotential dpd A 4.5 1.0 rc 2\nlogFreq"))
     assert r.returncode == 1 and "Invalid keyword in dpd potential: rc" in r.stdout
+
+
+@pytest.mark.parametrize("bad", ["partner", "type"])
+def test_read_data_rejects_bad_bond_lines(tmp_path, bad):
+    # a bond whose partner id or type is outside the data file's ranges stops
+    # the front end with a message before anything indexes with it
+    from frontend_util import ps_input, write_data_file
+    from test_oracle_tild import small_system
+    cfg = small_system(False, seed=3)
+    write_data_file(tmp_path / "init.data", cfg)
+    lines = (tmp_path / "init.data").read_text().split("\n")
+    k = lines.index("Bonds") + 2
+    cnt, t, i, j = lines[k].split()
+    n = cfg["topo"]["type"].size
+    lines[k] = f"{cnt} {t} {i} {n + 5}" if bad == "partner" else f"{cnt} 7 {i} {j}"
+    (tmp_path / "init.data").write_text("\n".join(lines))
+    r = dry(tmp_path, ps_input(cfg, "init.data", steps=2, log_freq=1))
+    assert r.returncode == 1, r.stdout[-1000:]
+    assert ("bond partner out of range" if bad == "partner" else "bond type out of range") in r.stdout

@@ -9,9 +9,11 @@ import os
 import numpy as np
 import pytest
 
+import orc
+
 from frontend_util import ps_input, read_data_positions, read_log, run_mft, write_data_file
 from mft import systems
-from test_oracle_tild import small_system
+from test_oracle_tild import oracle_system, small_system
 
 pytestmark = pytest.mark.gpu
 
@@ -36,15 +38,41 @@ def test_ps_front_end_matches_engine(gpu, tmp_path, charged):
     n = cfg["topo"]["type"].size
     eng = systems.tild_engine(cfg, "GJF")
     eng.set_run(12)
-    # log rows after steps: -1 initial, 0, 5, 10, then the final row (12)
+    # log rows after steps: -1 initial, 0, 5, 10, then the final row (12);
+    # a row after a step holds the pair energies of that step's density pass
+    # and the bond energy at the positions the step ends with
+    rows = read_log(tmp_path / "ps_data.dat")
+    assert [int(r[0]) for r in rows] == [-1, 0, 5, 10, 12]
     done = 0
-    for row in read_log(tmp_path / "ps_data.dat"):
+    for row in rows:
         k = {-1: 0, 0: 1, 5: 6, 10: 11, 12: 12}[int(row[0])]
-        eng.step(k - done)
+        pe, be = eng.step_energies(k - done) if k > done else eng.energies()
         done = k
-        pe, be = eng.energies()
         assert close(row[1:], [be] + list(pe)), (row, be, pe)
     assert done == 12
+    # the same rows from the FP64 oracle in the reference's order
+    # (PS_Box.cu:73-207: densities, forces, Integrate_2, update_prefactor,
+    # writeData -> CalcEnergy on the step's d_rho, bonds at the new positions)
+    sys_o = oracle_system(cfg, "GJF")
+    topo = sys_o.topo
+
+    def bond_e():
+        eb = orc.bond_energy(dict(nBonds=topo["nBonds"], bondedTo=topo["bondedTo"], bondType=topo["bondType"],
+                                  bondReq=cfg["bond_r0"], bondK=cfg["bond_k"], bondStyle=cfg["bond_style"],
+                                  maxbonds=2), sys_o.x, cfg["L"])
+        return 0.5 * eb.sum()
+
+    want = {-1: [bond_e()] + orc.tild_forces(sys_o, want_energy=True)["energies"]}
+    for step in range(12):
+        pair = orc.tild_forces(sys_o, want_energy=True)["energies"]
+        out = orc.tild_step(sys_o, orc.tild_noise(n, cfg["seed"], step))
+        sys_o.x, sys_o.xo, sys_o.noise_old = (out[key].astype(np.float32) for key in ("x", "xo", "noise_old"))
+        if step in (0, 5, 10, 11):
+            want[12 if step == 11 else step] = [bond_e()] + pair
+    for row in rows:
+        w = want[int(row[0])]
+        say(f"  row {int(row[0])}: front end {row[1:]}, oracle {np.round(w, 6)}")
+        assert close(row[1:], w), (row, w)
     x_fe = read_data_positions(tmp_path / "final.input.data", n)
     x_api, _, _ = eng.get_state()
     d = (x_fe.astype(np.float64) - x_api).reshape(-1, 3)
@@ -340,9 +368,8 @@ def test_ps_front_end_angles_match_engine(gpu, tmp_path):
     done = 0
     for row in read_log(tmp_path / "ps_data.dat"):
         k = {-1: 0, 0: 1, 5: 6, 10: 11, 12: 12}[int(row[0])]
-        eng.step(k - done)
+        pe, be = eng.step_energies(k - done) if k > done else eng.energies()
         done = k
-        pe, be = eng.energies()
         assert close(row[1:], [be, eng.angle_energy()] + list(pe)), (row, be, pe)
     assert done == 12
     x_fe = read_data_positions(tmp_path / "final.input.data", n)

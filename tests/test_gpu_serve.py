@@ -118,3 +118,75 @@ def test_runaway_solve_fails_early_without_touching_the_others(gpu):
     Qa, Ha, pa, rhoa = alone_fts(good[0], good[1], 60)
     assert np.array_equal(Q, Qa) and H == Ha and np.array_equal(rho, rhoa)
     srv.close()
+
+
+def test_served_results_match_oracle(gpu):
+    """Every result of a served mix against the FP64 oracle: each CL-FTS
+    request is replayed by the oracle from its initial fields and its own
+    seed's noise (log Q, mu, H, densities and the final fields), its final
+    fields are re-evaluated by the oracle's density pass (the check used for
+    long served solves), and the TILD request's positions are replayed."""
+    import orc
+    from test_gpu_fts import compare, rel
+
+    say("case: served results vs oracle")
+    cases = [(fts_case((32, 32, 32), 20, 14.0, 11), 3), (fts_case((48, 48, 48), 16, 19.0, 12), 2),
+             (fts_case((32, 32, 32), 24, 9.0, 13), 4)]
+    tcfg = small_system(True, seed=9)
+    srv = mft.Server(hbm_budget_bytes=8e9, max_fts_replicas=8)
+    ids = [srv.submit_fts(cases[0][0][0], cases[0][0][1], cases[0][1])]
+    tid = srv.submit_tild(tcfg, 3)
+    srv.pump(1)
+    ids += [srv.submit_fts(c[0][0], c[0][1], c[1]) for c in cases[1:]]
+    for _ in range(50):
+        nr, nq = srv.pump(2)
+        if nr == 0 and nq == 0:
+            break
+    assert nr == 0 and nq == 0
+
+    class Served:
+        def __init__(self, rid):
+            self.r = srv.fts_result(rid)
+            self.f, self.mu = srv.fts_final_state(rid)
+
+        def observables(self):
+            return self.r[:3]
+
+        def densities(self):
+            return self.r[3]
+
+        def chemical_potentials(self):
+            return self.mu
+
+        def get_fields(self, k, minus=False):
+            return (self.f[(k, "pl")], self.f[(k, "mi")]) if minus else self.f[(k, "pl")]
+
+    for rid, ((spec, fields), niter) in zip(ids, cases):
+        assert srv.status(rid) == "done"
+        say("  request", spec["Nx"], niter)
+        o = orc.fts_system_from_spec(spec, {k: np.asarray(v, np.complex64).astype(np.complex128)
+                                            for k, v in fields.items()})
+        orc.fts_init(o)
+        for it in range(niter):
+            orc.fts_step(o, orc.fts_noise(o.M, spec["seed"], it))
+        sv = Served(rid)
+        compare(sv, o, spec, f"served {spec['Nx']} replay")
+        # the final-state check: the oracle's density pass on the served fields
+        f = orc.fts_system_from_spec(spec, {k: np.asarray(v).astype(np.complex128) for k, v in sv.f.items()})
+        orc.fts_init(f)
+        compare(sv, f, spec, f"served {spec['Nx']} final state")
+    x, pe, be = srv.tild_result(tid)
+    from test_oracle_tild import oracle_system
+    sys_o = oracle_system(tcfg, "GJF")
+    ns = tcfg["topo"]["type"].size
+    for step in range(3):
+        out = orc.tild_step(sys_o, orc.tild_noise(ns, tcfg["seed"], step))
+        sys_o.x, sys_o.xo, sys_o.noise_old = (out[k].astype(np.float32) for k in ("x", "xo", "noise_old"))
+    d = (x.astype(np.float64) - sys_o.x.astype(np.float64)).reshape(-1, 3)
+    L = np.asarray(tcfg["L"], np.float64)
+    d -= L * np.round(d / L)
+    say(f"  tild request: max |dx| {np.max(np.abs(d)):.2e}")
+    assert np.max(np.abs(d)) < 1e-6 * max(tcfg["L"])
+    out = orc.tild_forces(sys_o, want_energy=True)
+    assert rel(pe, np.asarray(out["energies"])) < 1e-5
+    srv.close()

@@ -311,6 +311,39 @@ def test_large_coacervate_invariants(gpu):
     assert np.array_equal(x1, x2), "two identical runs differ"
 
 
+def test_config1_diblock_matches_oracle(gpu):
+    """The plain diblock melt at its full size (2000 chains of 10 A + 10 B,
+    40 000 particles, 20^3 box, 32^3 grid, cubic splines, GJF dt = 0.005):
+    densities, grid forces, particle forces and energies, then three GJF
+    steps, against the FP64 oracle on the same positions and noise."""
+    cfg = systems.tild_diblock()
+    say("case: diblock melt 40k particles, 32^3")
+    eng, sys_o = _engine_vs_oracle(gpu, cfg)
+    f_g = eng.compute_forces()
+    out = orc.tild_forces(sys_o, want_energy=True)
+    rho_g, _ = eng.densities()
+    errs = {name: rel(rho_g[s], out["rho"][name]) for s, name in enumerate(cfg["species"])}
+    for s, name in enumerate(cfg["species"]):
+        errs[f"grid force {name}"] = rel(eng.grid_force(s), out["gridF"][name])
+    errs["forces"] = rel(f_g, out["f"])
+    pe, be = eng.energies()
+    for k, (g, o) in enumerate(zip(pe, out["energies"])):
+        errs[f"energy {k}"] = abs(g - o) / max(abs(o), 1e-12)
+    say("  " + ", ".join(f"{k} {v:.1e}" for k, v in errs.items()))
+    assert all(v < TOL for v in errs.values()), errs
+    ns = cfg["topo"]["type"].size
+    for step in range(3):
+        eng.step(1)
+        o = orc.tild_step(sys_o, orc.tild_noise(ns, cfg["seed"], step))
+        sys_o.x, sys_o.xo, sys_o.noise_old = (o[k].astype(np.float32) for k in ("x", "xo", "noise_old"))
+        x_g, _, no_g = eng.get_state()
+        dx = np.max(periodic_dist(x_g, o["x"], cfg["L"]))
+        say(f"  step {step}: max |dx| {dx:.2e}")
+        assert dx < 1e-6 * max(cfg["L"]), step
+        assert rel(no_g, o["noise_old"]) < TOL
+    eng.close()
+
+
 def edge_case_system(case):
     """Small diblock melts at the edges of the engine's layouts."""
     if case in ("p1", "p2", "p4"):

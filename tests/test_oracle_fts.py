@@ -385,3 +385,31 @@ def test_zero_mean_incompress_leaves_densities_unchanged():
     d = b.potentials[0].wpl - a.potentials[0].wpl
     np.testing.assert_allclose(d, d.mean(), atol=1e-12)
     assert abs(d.mean() + a.potentials[0].wpl.mean()) < 1e-12
+
+
+def test_incompress_force_is_field_derivative_of_H():
+    # d_makeIncompressForce i rho_tot / Nr - i C (fts_potential_incompress.cu:
+    # 202-218) against its H term -i C gvol sum w (:232-247) and -n log Q
+    spec = tiny_spec(pots=[dict(kind="Incompress", dt_pl=0.02),
+                           dict(kind="Flory", I=0, J=1, chiN=12.0, dt_pl=0.02, dt_mi=0.03)])
+    spec["molecules"] = [dict(blocks=[(4, 0), (3, 1)], phi=0.6), dict(blocks=[(5, 1)], phi=0.4)]
+    _fd_check_forces(spec, random_fields(spec, 0.2, seed=11))
+
+
+def test_chemical_potential_closed_forms():
+    # W = 0: Q = 1, mu = ln(n/V); W = c on every bead: Q = exp(-c Ntot),
+    # mu = ln(n/V) + c Ntot; grand canonical: mu = ln z whatever the fields
+    c = 0.23
+    spec = tiny_spec(pots=[dict(kind="Incompress", dt_pl=0.01)])
+    M = int(np.prod(spec["Nx"]))
+    s = orc.fts_system_from_spec(spec, {})
+    orc.fts_init(s)
+    n = s.molecules[0].nmolecs
+    assert orc.fts_chemical_potentials(s)[0] == pytest.approx(np.log(n / s.V()), rel=1e-13)
+    s = orc.fts_system_from_spec(spec, {(0, "pl"): np.full(M, -1j * c * spec["Nr"])})
+    orc.fts_init(s)
+    assert orc.fts_chemical_potentials(s)[0] == pytest.approx(np.log(n / s.V()) + c * 7, rel=1e-12)
+    spec["molecules"] = [dict(blocks=[(4, 0), (3, 1)], activity=0.35)]
+    s = orc.fts_system_from_spec(spec, {(0, "pl"): np.full(M, -1j * c * spec["Nr"])})
+    orc.fts_init(s)
+    assert orc.fts_chemical_potentials(s)[0] == pytest.approx(np.log(0.35), rel=1e-14)

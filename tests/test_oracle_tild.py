@@ -406,3 +406,80 @@ def test_dpd_conserves_momentum_and_thermalizes():
         if step >= 150:
             acc.append(np.mean(sys.v.reshape(-1, 3) ** 2))
     assert np.mean(acc) == pytest.approx(1.5, rel=0.08)
+
+
+def _em(x, f, dt, L, xi):
+    """orc_em_step (d_EM_integrator, ps_integratorEM.cu:41-82) on [n, 3] arrays."""
+    n = x.shape[0]
+    xs = np.ascontiguousarray(x, np.float64).reshape(-1).copy()
+    fs = np.ascontiguousarray(f, np.float64).reshape(-1)
+    Lf = np.asarray(L, np.float32)
+    orc.lib().orc_em_step(orc._ptr(xs), orc._ptr(fs), dt, orc._ptr(Lf), orc._ptr(np.arange(n, dtype=np.int32)), n, 3,
+                          orc._ptr(np.ascontiguousarray(xi, np.float64).reshape(-1)))
+    return xs.reshape(n, 3)
+
+
+def test_em_step_without_noise_is_identity_or_pure_drift():
+    # f = 0 and xi = 0 leave x unchanged; xi = 0 gives x + dt f exactly, with
+    # the single wrap into [0, L] of the reference (one particle leaves through
+    # each face)
+    rng = np.random.default_rng(4)
+    L = (10.0, 12.0, 8.0)
+    x = 1.0 + rng.random((200, 3)) * (np.asarray(L) - 2.0)   # one unit from every face
+    zero = np.zeros_like(x)
+    assert np.array_equal(_em(x, zero, 0.01, L, zero), x)
+    f = rng.normal(0.0, 3.0, x.shape)
+    x[0] = (9.99, 6.0, 4.0)
+    f[0] = (2.0, 0.0, 0.0)
+    x[1] = (5.0, 0.005, 4.0)
+    f[1] = (0.0, -1.0, 0.0)
+    dt = 0.01
+    got = _em(x, f, dt, L, zero)
+    want = x + dt * f
+    want[0, 0] -= L[0]
+    want[1, 1] += L[1]
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
+
+
+def test_em_free_diffusion_msd_is_2dt_per_axis():
+    # force-free EM: x(t) - x(0) is a sum of sqrt(2 dt) N(0,1) steps, so
+    # <dx^2> = 2 dt n per axis (unit mobility); sqrt(dt) noise would give half
+    n, steps, dt = 20000, 40, 0.005
+    L = (1e4, 1e4, 1e4)
+    x0 = np.full((n, 3), 5e3)
+    x = x0.copy()
+    zero = np.zeros_like(x)
+    for step in range(steps):
+        x = _em(x, zero, dt, L, orc.tild_noise(n, 31, step).reshape(n, 3))
+    d = x - x0
+    msd = np.mean(d * d, axis=0)
+    np.testing.assert_allclose(msd, 2 * dt * steps, rtol=0.025)
+    assert np.all(np.abs(np.mean(d, axis=0)) < 4 * np.sqrt(2 * dt * steps / n))
+
+
+@pytest.mark.parametrize("kind", ["charges", "gaussian"])
+def test_pair_energy_of_a_single_fourier_mode(kind):
+    # rho_I = rho_J = a cos(k x) on the grid: u * rho = a u(k) cos(k x) and
+    # sum cos^2 = M/2, so E = gvol a^2 u(k) M / 2 = V a^2 u(k) / 2, halved for
+    # the charges potential (ps_potentialCharges.cu:115) whose
+    # u(k) = 4 pi l_B / k^2 exp(-k^2 sigma^2); the Gaussian's is
+    # A exp(-k^2 sigma^2 / 2) (ps_potentialGaussian.cu:43-45)
+    Nx, L = (16, 12, 10), (8.0, 6.0, 5.0)
+    M = int(np.prod(Nx))
+    a, n = 0.7, 3
+    kx = 2 * np.pi * n / L[0]
+    ix = np.arange(M) % Nx[0]
+    rho = a * np.cos(kx * ix * L[0] / Nx[0])
+    gvol = float(np.prod(np.asarray(L) / np.asarray(Nx)))
+    V = float(np.prod(L))
+    if kind == "charges":
+        lB, sig = 2.0, 0.5
+        uk = orc.uk_table("charges", lB, sig, 0.0, Nx, L)
+        want = 0.5 * V * a * a / 2 * 4 * np.pi * lB / kx ** 2 * np.exp(-kx ** 2 * sig ** 2)
+    else:
+        A, sig = 3.0, 0.8
+        uk = orc.uk_table("gaussian", A, sig, 0.0, Nx, L)
+        want = V * a * a / 2 * A * np.exp(-kx ** 2 * sig ** 2 / 2)
+    got = orc.pair_energy(kind, rho, rho, uk, Nx, gvol)
+    # the tables use the reference's float k (Box::get_kD), hence 1e-6
+    assert got == pytest.approx(want, rel=1e-6)
