@@ -117,6 +117,13 @@ enum { MFT_POT_GAUSSIAN = 0, MFT_POT_ERF2 = 1, MFT_POT_ERFG = 2, MFT_POT_CHARGES
  * The reference's BCC phase reads an uninitialised vector (Box.cu:359 passes
  * the position as pbc_dr2's output), so it is rejected. */
 enum { MFT_BIAS_LAM = 0, MFT_BIAS_HEX = 1, MFT_BIAS_GYR = 2 };
+/* Box::make_bias_field (Box.cu:308-458) on the host: u[M] = A times the
+ * lamellar (sin along dir), cylinder (cos cos across dir) or gyroid shape
+ * with n_periods periods, at r = n dx in float like the reference; the
+ * ordered-phase initialize of an FTS field (fts_potential.cu:102-138) and
+ * the TILD bias potential use it.  The BCC phase is not offered (the
+ * reference evaluates it on an uninitialised vector, Box.cu:359). */
+int mft_bias_field(double* u, const int Nx[3], const float L[3], int phase, double A, int dir, int n_periods);
 
 /* initializePotential of one potential (ps_potentialGaussian.cu:28-71,
  * ps_potentialErf2.cu:31-67, ps_potentialErfG.cu:30-66,

@@ -197,3 +197,27 @@ int tild_bias_energy(TildEngine* e, int k, double* energy)
 }
 
 } // namespace mft
+
+extern "C" int mft_bias_field(double* u, const int Nx[3], const float L[3], int phase, double A, int dir,
+                              int n_periods)
+{
+    MFT_REQUIRE(u && Nx && L, MFT_ERR_INVALID, "null argument");
+    MFT_REQUIRE(phase >= MFT_BIAS_LAM && phase <= MFT_BIAS_GYR, MFT_ERR_INVALID,
+                "bias phase %d is not lamellar, cylinder or gyroid", phase);
+    MFT_REQUIRE(dir >= 0 && dir < 3 && n_periods > 0 && Nx[0] > 0 && Nx[1] > 0 && Nx[2] > 0, MFT_ERR_INVALID,
+                "bad bias field arguments");
+    mft_tild_potential P{};
+    P.phase = phase;
+    P.A = 1.0f;
+    P.dir = dir;
+    P.n_periods = n_periods;
+    const float dx[3] = {L[0] / (float)Nx[0], L[1] / (float)Nx[1], L[2] / (float)Nx[2]};
+    const long M = (long)Nx[0] * Nx[1] * Nx[2];
+    for (long i = 0; i < M; ++i) {
+        const int n[3] = {(int)(i % Nx[0]), (int)((i / Nx[0]) % Nx[1]), (int)(i / ((long)Nx[0] * Nx[1]))};
+        const float r[3] = {(float)n[0] * dx[0], (float)n[1] * dx[1], (float)n[2] * dx[2]};
+        // the prefactor enters in double, as Box::make_bias_field's Ao
+        u[i] = A * mft::bias_value(P, r, L);
+    }
+    return MFT_OK;
+}

@@ -5,6 +5,8 @@
 #include <cmath>
 #include <cstdio>
 #include <deque>
+#include <sstream>
+#include <fstream>
 #include <iostream>
 
 #include "matilda.h"
@@ -40,8 +42,66 @@ void FTS_Potential::initializeField(std::istringstream& iss, std::vector<std::co
             double im = ia * mybox->rng.uniform();
             w[i] = {r, im};
         }
+    } else if (s1 == "sin" || s1 == "sine") {
+        // (real_amp + i imag_amp) sin(2 PI r_dir period / L_dir), r = n dx
+        // (fts_potential.cu:88-100, Box::get_r Box.cu:219-232, PI of constants.h)
+        int dir = 0;
+        double ra = 0.0, ia = 0.0, period = 1.0;
+        iss >> dir >> ra >> ia >> period;
+        if (dir < 0 || dir > 2) die("initialize sin: direction must be 0, 1 or 2");
+        const double PIf = (double)3.141592654f;
+        const int* Nx = mybox->gridNx();
+        const float* L = mybox->boxL();
+        const float dxd = L[dir] / (float)Nx[dir];
+        for (long i = 0; i < M; ++i) {
+            const long n[3] = {i % Nx[0], (i / Nx[0]) % Nx[1], i / ((long)Nx[0] * Nx[1])};
+            const double r = (double)n[dir] * (double)dxd;
+            const double v = std::sin(2.0 * PIf * r * period / (double)L[dir]);
+            w[i] = {ra * v, ia * v};
+        }
+    } else if (s1 == "L" || s1 == "LAM" || s1 == "C" || s1 == "CYL" || s1 == "H" || s1 == "HEX" || s1 == "G" ||
+               s1 == "GYR") {
+        // an ordered phase: Ao [dir d] [n_periods n] (fts_potential.cu:102-138)
+        const int phase = (s1 == "L" || s1 == "LAM") ? MFT_BIAS_LAM : (s1 == "G" || s1 == "GYR") ? MFT_BIAS_GYR
+                                                                                                  : MFT_BIAS_HEX;
+        double Ao = 0.0;
+        int dir = 0, nper = 1;
+        iss >> Ao;
+        std::string opt;
+        while (iss >> opt) {
+            if (opt == "dir") iss >> dir;
+            else if (opt == "n_periods") iss >> nper;
+            else die(opt + " is not a valid initialize option in fts_potential");
+        }
+        std::cout << "phase read as " << s1 << " Ao: " << Ao << std::endl;
+        std::vector<double> u((size_t)M);
+        check_mft(mft_bias_field(u.data(), mybox->gridNx(), mybox->boxL(), phase, Ao, dir, nper), "mft_bias_field");
+        for (long i = 0; i < M; ++i) w[i] = {u[i], 0.0};
+    } else if (s1 == "BCC" || s1 == "SPH" || s1 == "S") {
+        die("initialize " + s1 + ": the BCC phase is not offered (the reference builds it from an uninitialised "
+            "vector, Box.cu:359)");
+    } else if (s1 == "readDatFile") {
+        // Box::readDatFile (Box.cu:622-656): per line the coordinates, then
+        // Re w and Im w.  Like the reference (fts_potential.cu:142-146), the
+        // file always fills this potential's w+, whichever field the
+        // initialize names (so a Flory potential's w+ is then set to -1.5i
+        // anyway, and Helfand's and Edwards' never reach the device)
+        std::vector<std::complex<double>>& dst = (&w == &wmi) ? wpl : w;
+        std::string name;
+        iss >> name;
+        std::ifstream inp(name);
+        if (!inp.is_open()) die("Failed to open " + name);
+        std::string line;
+        for (long i = 0; i < M; ++i) {
+            if (!std::getline(inp, line)) die("readDatFile: " + name + " ends before the grid does");
+            std::istringstream ls(line);
+            double xd, wr = 0.0, wi = 0.0;
+            for (int j = 0; j < 3; ++j) ls >> xd;
+            ls >> wr >> wi;
+            dst[i] = {wr, wi};
+        }
     } else {
-        die(s1 + " is not a valid initialize option in this build (value, random)");
+        die("Invalid initialize option on potential " + potentialStyle);
     }
 }
 
@@ -74,7 +134,22 @@ void FTS_Potential::parse_options(std::istringstream& iss, std::vector<std::comp
     }
 }
 
-void FTS_Potential::writeFields(int potInd) { (void)potInd; }
+// writeFields of each potential (fts_potential_flory.cu:307-321,
+// fts_potential_helfand.cu:243-250, fts_potential_incompress.cu:221-228,
+// fts_potential_edwards.cu:245-252): the current fields as text grids
+void FTS_Potential::writeFields(int potInd, mft_fts* sys)
+{
+    const long M = mybox->gridSize();
+    std::vector<float> a(2 * (size_t)M), b(has_wmi ? 2 * (size_t)M : 0);
+    check_mft(mft_fts_get_fields(sys, potInd, a.data(), has_wmi ? b.data() : nullptr), "mft_fts_get_fields");
+    std::vector<std::complex<double>> w((size_t)M);
+    for (long i = 0; i < M; ++i) w[i] = {a[2 * i], a[2 * i + 1]};
+    mybox->writeTComplexGridData("wpl_" + potentialStyle + std::to_string(potInd) + ".dat", w.data());
+    if (has_wmi) {
+        for (long i = 0; i < M; ++i) w[i] = {b[2 * i], b[2 * i + 1]};
+        mybox->writeTComplexGridData("wmi_" + potentialStyle + std::to_string(potInd) + ".dat", w.data());
+    }
+}
 
 // PotentialHelfand (fts_potential_helfand.cu:15-70): kappaN delt [options];
 // its initialize never reaches the device (the field stays zero)
@@ -297,12 +372,9 @@ void FTS_Box::readInput(std::ifstream& inp)
             }
             if (firstWord == "boxLengths") read_lengths(iss);
             else if (firstWord == "C") iss >> C;
-            else if (firstWord == "chemFieldFreq" || firstWord == "chemfieldfreq" || firstWord == "clSampleFreq" ||
-                     firstWord == "clsamplefreq" || firstWord == "densityFieldFreq" ||
-                     firstWord == "densityfieldfreq") {
-                int f;
-                iss >> f;
-            } else if (firstWord == "Dim") {
+            else if (firstWord == "chemFieldFreq" || firstWord == "chemfieldfreq") iss >> chemFieldFreq;
+            else if (firstWord == "clSampleFreq" || firstWord == "clsamplefreq") iss >> clSampleFreq;
+            else if (firstWord == "densityFieldFreq" || firstWord == "densityfieldfreq") iss >> densityFieldFreq; else if (firstWord == "Dim") {
                 iss >> Dim;
                 if (Dim != 3) die("this build runs 3D boxes (Dim 3)");
                 readDimension = true;
@@ -479,11 +551,76 @@ int FTS_Box::converged(int step)
     return 0;
 }
 
+// the I/O of FTS_Box::doTimeStep (FTS_Box.cu:84-119) after iteration step
+bool FTS_Box::ioStep(int step) const
+{
+    return (logFreq > 0 && step % logFreq == 0) || (densityFieldFreq > 0 && step % densityFieldFreq == 0) ||
+           (chemFieldFreq > 0 && step % chemFieldFreq == 0) ||
+           (ftsStyle == "cl" && clSampleFreq > 0 && densityFieldFreq > 0 && step % clSampleFreq == 0);
+}
+
+// FTS_Box::writeTComplexGridData (FTS_Box.cu:284-306): per grid point the
+// coordinates r = n dx ("%lf "), then "%1.4e %1.4e" of Re and Im
+void FTS_Box::writeTComplexGridData(const std::string& name, const std::complex<double>* field) const
+{
+    FILE* otp = std::fopen(name.c_str(), "w");
+    if (!otp) die("Failed to open " + name);
+    const float dx[3] = {L[0] / (float)Nx[0], L[1] / (float)Nx[1], L[2] / (float)Nx[2]};
+    for (long i = 0; i < M; ++i) {
+        const long n[3] = {i % Nx[0], (i / Nx[0]) % Nx[1], i / ((long)Nx[0] * Nx[1])};
+        for (int j = 0; j < 3; ++j) std::fprintf(otp, "%lf ", (double)n[j] * (double)dx[j]);
+        std::fprintf(otp, "%1.4e %1.4e\n", field[i].real(), field[i].imag());
+    }
+    std::fclose(otp);
+}
+
+// CL density samples every clSampleFreq steps (their running average is
+// written with the densities), rhoSpecies<i>.dat [rhoSpeciesAvg<i>.dat] every
+// densityFieldFreq steps, the potentials' fields and fieldSpecies<i>.dat every
+// chemFieldFreq steps (fts_species.cu:136-172)
+void FTS_Box::writeFieldFiles(int step)
+{
+    const size_t ns = Species.size(), m = (size_t)M;
+    const bool sample = ftsStyle == "cl" && clSampleFreq > 0 && densityFieldFreq > 0 && step % clSampleFreq == 0;
+    const bool dens = densityFieldFreq > 0 && step % densityFieldFreq == 0;
+    const bool chem = chemFieldFreq > 0 && step % chemFieldFreq == 0;
+    if (!sample && !dens && !chem) return;
+    std::vector<float> buf(2 * ns * m);
+    std::vector<std::complex<double>> f(m);
+    if (sample || dens) {
+        check_mft(mft_fts_get_densities(sys, buf.data()), "mft_fts_get_densities");
+        if (sample) {
+            if (densityAvg.empty()) densityAvg.assign(ns * m, {0.0, 0.0});
+            for (size_t i = 0; i < ns * m; ++i) densityAvg[i] += std::complex<double>(buf[2 * i], buf[2 * i + 1]);
+            ++nCLSamples;
+        }
+        if (dens) {
+            for (size_t s = 0; s < ns; ++s) {
+                for (size_t i = 0; i < m; ++i) f[i] = {buf[2 * (s * m + i)], buf[2 * (s * m + i) + 1]};
+                writeTComplexGridData("rhoSpecies" + std::to_string(s) + ".dat", f.data());
+                if (ftsStyle == "cl" && nCLSamples > 0) {
+                    for (size_t i = 0; i < m; ++i) f[i] = densityAvg[s * m + i] / (double)nCLSamples;
+                    writeTComplexGridData("rhoSpeciesAvg" + std::to_string(s) + ".dat", f.data());
+                }
+            }
+        }
+    }
+    if (chem) {
+        for (size_t p = 0; p < Potentials.size(); ++p) Potentials[p]->writeFields((int)p, sys);
+        check_mft(mft_fts_get_species_fields(sys, buf.data()), "mft_fts_get_species_fields");
+        for (size_t s = 0; s < ns; ++s) {
+            for (size_t i = 0; i < m; ++i) f[i] = {buf[2 * (s * m + i)], buf[2 * (s * m + i) + 1]};
+            writeTComplexGridData("fieldSpecies" + std::to_string(s) + ".dat", f.data());
+        }
+    }
+}
+
 void FTS_Box::doTimeStep(int step)
 {
     for (auto* p : Potentials) p->updateFields();
     for (auto* m : Molecs) m->calcDensity();
     check_mft(mft_fts_step(sys, 1), "mft_fts_step");
+    writeFieldFiles(step);
     if (logFreq > 0 && step % logFreq == 0) writeData(step);
 }
 
@@ -493,8 +630,9 @@ void FTS_Box::NVT(int nsteps)
     if (dry_run) return;
     int i = 0;
     while (i < nsteps) {
-        int next = nsteps;
-        if (logFreq > 0) next = std::min(nsteps, (i / logFreq + (i % logFreq ? 1 : 0)) * logFreq);
+        // the next iteration with I/O (log, field files, CL samples)
+        int next = i;
+        while (next < nsteps && !ioStep(next)) ++next;
         if (next > i) {
             check_mft(mft_fts_step(sys, next - i), "mft_fts_step");
             totSteps += next - i;

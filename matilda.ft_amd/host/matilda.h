@@ -92,6 +92,8 @@ public:
     virtual int converged(int step) { (void)step; return 0; }
     std::string returnBoxStyle() const { return boxStyle; }
     int returnDimension() const { return Dim; }
+    const int* gridNx() const { return Nx; }
+    const float* boxL() const { return L; }
     // print the engine descriptor instead of creating the engine
     static bool dry_run;
 
@@ -298,6 +300,10 @@ public:
     void writeGSDtraj();                                // PS_Box_gsdUtils.cu:14-160
     int gsdFreq = 0, logSpaceGSDflag = 0;
     float logScaleGSD = 1.f;
+    int fieldFreq = 0, logSpaceFieldflag = 0;            // PS_Box_initialization.cu:171-182
+    float logScaleFields = 1.f;
+    void initFieldFiles();                              // binary density headers, PS_Box_initialization.cu:528-535
+    void writeFields();                                 // one binary density frame per group, PS_Box.cu:376-388
     std::string gsd_name = "traj.gsd";
     std::unique_ptr<GsdWriter> gsd;
     int findSpeciesInteger(const std::string& s) const;
@@ -361,7 +367,7 @@ public:
     virtual std::complex<double> calcHamiltonian() { return Hterm; }
     virtual void initLinearCoeffs() {}
     virtual void storePredictorData() {}
-    virtual void writeFields(int potInd);
+    virtual void writeFields(int potInd, mft_fts* sys); // wpl_<style><i>.dat [, wmi_Flory<i>.dat]
     virtual mft_fts_potential descriptor() const = 0;
     std::string input_command, potentialStyle, updateScheme = "EM";
     std::vector<std::string> actsOn;
@@ -452,6 +458,12 @@ public:
     std::vector<FTS_Molec*> Molecs;
     std::complex<double> Heff{0.0, 0.0};
     double Hold = 1.0e8, tolerance = 1.0e-5, error = 0.0;
+    // field files (FTS_Box.cu:84-113, 318-320, 373-377)
+    int chemFieldFreq = 0, densityFieldFreq = 0, clSampleFreq = 100, nCLSamples = 0;
+    std::vector<std::complex<double>> densityAvg;       // running sum of CL density samples [nspecies*M]
+    void writeTComplexGridData(const std::string& name, const std::complex<double>* field) const; // FTS_Box.cu:284-306
+    void writeFieldFiles(int step);
+    bool ioStep(int step) const;
     std::string tolMetric = "Heff";
     int totSteps = 0;
     HostRng rng;

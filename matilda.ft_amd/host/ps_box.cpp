@@ -428,11 +428,14 @@ void PS_Box::readInput(std::ifstream& inp)
             } else if (firstWord == "gsdName" || firstWord == "gsd_name") {
                 iss >> gsd_name;
             } else if (firstWord == "fieldFreq" || firstWord == "field_freq") {
-                int f;
-                iss >> f;
-                std::string rest;
-                while (iss >> rest) {}
-                if (f > 0) std::cout << "note: " << firstWord << " output is not written by this build" << std::endl;
+                iss >> fieldFreq;
+                std::string tps;
+                if (iss >> tps) {
+                    if (tps != "logspace" && tps != "log_space" && tps != "logSpace")
+                        die("PS_Box_init::readInput: invalid option to gsdFreq");
+                    logSpaceFieldflag = 1;
+                    iss >> logScaleFields;
+                }
             } else if (firstWord == "initDataFileName") {
                 std::string s;
                 iss >> s;
@@ -799,6 +802,7 @@ void PS_Box::finishInitialization()
     for (auto* p : potentials) p->initializePotential();
     for (auto* i : integrators) i->finishInitialization();
     for (auto* c : computes) c->initialize_compute();
+    initFieldFiles();
     initialized = true;
 }
 
@@ -869,6 +873,11 @@ void PS_Box::doTimeStep(int step)
         if (step > 1 && logSpaceGSDflag == 1) gsdFreq = int(float(gsdFreq) * logScaleGSD);
     }
     if (trajFreq > 0 && step % trajFreq == 0) writeLammpsTraj(step);
+    if (fieldFreq > 0 && step % fieldFreq == 0) {
+        writeFields();
+        // PS_Box.cu:198-203: log-spaced frames
+        if (step > 1 && logSpaceFieldflag == 1) fieldFreq = int(float(fieldFreq) * logScaleFields);
+    }
 }
 
 // PS_Box::writeGSDtraj (PS_Box_gsdUtils.cu:14-160): the first frame (at
@@ -1065,6 +1074,7 @@ void PS_Box::NVT(int nSteps)
         if (logFreq > 0) next = std::min(next, (i / logFreq + (i % logFreq ? 1 : 0)) * logFreq);
         if (trajFreq > 0) next = std::min(next, (i / trajFreq + (i % trajFreq ? 1 : 0)) * trajFreq);
         if (gsdFreq > 0) next = std::min(next, (i / gsdFreq + (i % gsdFreq ? 1 : 0)) * gsdFreq);
+        if (fieldFreq > 0) next = std::min(next, (i / fieldFreq + (i % fieldFreq ? 1 : 0)) * fieldFreq);
         for (auto* c : computes)
             for (int s = i; s < next; ++s)
                 if (c->active(s)) {
@@ -1087,7 +1097,103 @@ void PS_Box::NVT(int nSteps)
     check_mft(mft_tild_synchronize(eng), "mft_tild_synchronize");
     std::cout << "NVT Finished, writing final output" << std::endl;
     writeData(maxSteps);
+    writeFields();
     writeDataConfig("final.input.data");
+}
+
+// Binary density files of the reference (Box::initBinaryDataFile and
+// writeBinaryData, Box.cu:553-582; PS_Group::writeDensityFieldBinary,
+// ps_groups.cu:257-262): "density-<group>.bin" for "all", every species, the
+// "charges" group and the user groups, starting with int Dim, int Nx[Dim],
+// float L[Dim], then one frame of M floats (the group's number density, x
+// fastest) per write.
+namespace {
+struct DensityGroup {
+    std::string name;
+    uint32_t mask;     // species whose densities sum to the group's
+};
+}
+
+static std::vector<DensityGroup> density_groups(const PS_Box& b, const std::vector<PS_Species>& species,
+                                                const std::vector<PS_Group>& groups, bool charges,
+                                                uint32_t charged_mask)
+{
+    (void)b;
+    std::vector<DensityGroup> out;
+    const uint32_t all = species.size() >= 32 ? 0xffffffffu : (1u << species.size()) - 1u;
+    out.push_back({"all", all});
+    for (size_t i = 0; i < species.size(); ++i) out.push_back({species[i].speciesLabel, 1u << i});
+    if (charges) out.push_back({"charges", charged_mask});
+    for (const auto& g : groups) out.push_back({g.name, g.speciesMask});
+    return out;
+}
+
+// species whose every particle carries a charge (their union is the charges
+// group); a species with charged and neutral particles makes it unavailable
+static bool charged_species(const std::vector<int>& type, const std::vector<float>& q, int ns, int nspecies,
+                            uint32_t* mask)
+{
+    std::vector<int> nq(nspecies, 0), n(nspecies, 0);
+    for (int i = 0; i < ns; ++i) {
+        n[type[i]]++;
+        if (q[i] != 0.f) nq[type[i]]++;
+    }
+    *mask = 0;
+    for (int s = 0; s < nspecies; ++s) {
+        if (nq[s] != 0 && nq[s] != n[s]) return false;
+        if (nq[s] > 0) *mask |= 1u << s;
+    }
+    return true;
+}
+
+void PS_Box::initFieldFiles()
+{
+    uint32_t qmask = 0;
+    const bool uniform = charged_species(intSpecies, charges, nstot, (int)species.size(), &qmask);
+    for (const auto& g : density_groups(*this, species, psGroup, doCharges, qmask)) {
+        if (g.name == "charges" && !uniform) {
+            std::cout << "note: density-charges.bin is not written (a species holds charged and neutral "
+                         "particles)" << std::endl;
+            continue;
+        }
+        std::ofstream otp("density-" + g.name + ".bin", std::ios::binary);
+        if (!otp.is_open()) die("Failed to initialize output binary file!");
+        const int dim = 3;
+        const int nx[3] = {Nx[0], Nx[1], Nx[2]};
+        otp.write(reinterpret_cast<const char*>(&dim), sizeof(int));
+        otp.write(reinterpret_cast<const char*>(nx), 3 * sizeof(int));
+        otp.write(reinterpret_cast<const char*>(L), 3 * sizeof(float));
+    }
+}
+
+void PS_Box::writeFields()
+{
+    if (dry_run) return;
+    // the densities of the last density pass (the step's, before the
+    // integrator moved the particles); the velocity-Verlet step takes its
+    // densities after the drift, at the positions it ends with
+    if (integrators.size() == 1 && integrators[0]->name == "VV") {
+        std::vector<double> pe(std::max<size_t>(1, potentials.size()));
+        check_mft(mft_tild_energies(eng, pe.data(), nullptr), "mft_tild_energies");
+    }
+    const size_t ns = species.size();
+    std::vector<float> rho(ns * (size_t)M);
+    check_mft(mft_tild_get_densities(eng, rho.data(), nullptr), "mft_tild_get_densities");
+    uint32_t qmask = 0;
+    const bool uniform = charged_species(intSpecies, charges, nstot, (int)ns, &qmask);
+    std::vector<float> frame((size_t)M);
+    for (const auto& g : density_groups(*this, species, psGroup, doCharges, qmask)) {
+        if (g.name == "charges" && !uniform) continue;
+        for (long i = 0; i < M; ++i) {
+            float v = 0.f;
+            for (size_t sp = 0; sp < ns; ++sp)
+                if ((g.mask >> sp) & 1u) v += rho[sp * (size_t)M + i];
+            frame[i] = v;
+        }
+        std::ofstream otp("density-" + g.name + ".bin", std::ios::app | std::ios::binary);
+        if (!otp.is_open()) die("Failed to open output binary file!");
+        otp.write(reinterpret_cast<const char*>(frame.data()), sizeof(float) * (size_t)M);
+    }
 }
 
 // n engine steps; with energies, the log energies of the last one are kept

@@ -419,3 +419,143 @@ def test_ps_front_end_vv_dpd_langevin_match_engine(gpu, tmp_path):
     d -= L * np.round(d / L)
     assert np.max(np.abs(d)) < 1e-5 * max(cfg["L"])
     eng.close()
+
+
+def read_bin_frames(path, M):
+    """Binary density file of Box::initBinaryDataFile / writeBinaryData
+    (Box.cu:553-582): int Dim, int Nx[Dim], float L[Dim], then M floats a frame."""
+    raw = open(path, "rb").read()
+    dim = np.frombuffer(raw[:4], np.int32)[0]
+    nx = np.frombuffer(raw[4:4 + 4 * dim], np.int32)
+    L = np.frombuffer(raw[4 + 4 * dim:4 + 8 * dim], np.float32)
+    body = np.frombuffer(raw[4 + 8 * dim:], np.float32)
+    assert body.size % M == 0
+    return dim, nx, L, body.reshape(-1, M)
+
+
+@pytest.mark.parametrize("charged", [False, True])
+def test_ps_binary_density_frames_match_engine(gpu, tmp_path, charged):
+    # fieldFreq 5 over 12 steps: frames after steps 0, 5, 10 and the final one
+    # (PS_Box.cu:198-203, 228), each group's density from that step's density pass
+    cfg = small_system(charged, seed=12)
+    say("case: front end binary density frames, charged =", charged)
+    write_data_file(tmp_path / "init.data", cfg)
+    text = ps_input(cfg, "init.data", steps=12, log_freq=100).replace("endBox", "fieldFreq 5\nendBox")
+    (tmp_path / "input").write_text(text)
+    r = run_mft(["-in", "input"], cwd=tmp_path)
+    assert r.returncode == 0, r.stdout[-3000:]
+    M = int(np.prod(cfg["Nx"]))
+    names = ["all"] + list(cfg["species"]) + (["charges"] if charged else [])
+    frames = {}
+    for nm in names:
+        dim, nx, L, fr = read_bin_frames(tmp_path / f"density-{nm}.bin", M)
+        assert dim == 3 and list(nx) == list(cfg["Nx"]) and np.allclose(L, cfg["L"])
+        assert fr.shape[0] == 4, (nm, fr.shape)
+        frames[nm] = fr
+    eng = systems.tild_engine(cfg, "GJF")
+    eng.set_run(12)
+    q = cfg["topo"]["charge"]
+    t = cfg["topo"]["type"]
+    charged_species = [s for s in range(len(cfg["species"])) if np.any(q[t == s] != 0)]
+    done = 0
+    for k, step in enumerate([1, 6, 11, 12]):
+        eng.step(step - done)
+        done = step
+        rho, _ = eng.densities()
+        want = {"all": rho.sum(axis=0)}
+        for s, nm in enumerate(cfg["species"]):
+            want[nm] = rho[s]
+        if charged:
+            want["charges"] = rho[charged_species].sum(axis=0)
+        for nm in names:
+            assert np.allclose(frames[nm][k], want[nm], rtol=1e-6, atol=1e-6 * np.abs(want[nm]).max()), (nm, k)
+    eng.close()
+
+
+FTS_FIELDS_INPUT = """box fts cl
+Dim 3
+grid 16 12 20
+boxLengths 3.0 2.5 4.0
+Nr 12
+C 4
+randSeed 9
+species A
+species B
+molecule linear phi 1.0 2 5 A 7 B
+potential Incompress 0.01 initialize sin 2 0.3 0.1 2
+potential Flory A B 12 0.01 0.01 initialize L 0.2 dir 1 n_periods 2
+chemFieldFreq 2
+densityFieldFreq 2
+clSampleFreq 1
+logFreq 100
+endBox
+run vt 5
+"""
+
+
+def read_grid_text(path):
+    a = np.loadtxt(path)
+    return a[:, :3], a[:, 3] + 1j * a[:, 4]
+
+
+def test_fts_field_files_and_initialize_options(gpu, tmp_path):
+    # initialize sin / ordered phase (fts_potential.cu:88-138), then the field
+    # files of FTS_Box::doTimeStep (FTS_Box.cu:84-113) after iterations 0, 2, 4
+    say("case: front end fts field files")
+    (tmp_path / "input").write_text(FTS_FIELDS_INPUT)
+    r = run_mft(["-in", "input"], cwd=tmp_path)
+    assert r.returncode == 0, r.stdout[-3000:]
+    Nx, L = (16, 12, 20), (3.0, 2.5, 4.0)
+    M = int(np.prod(Nx))
+    n = np.stack([np.arange(M) % Nx[0], (np.arange(M) // Nx[0]) % Nx[1], np.arange(M) // (Nx[0] * Nx[1])], axis=1)
+    dx = np.float32(np.asarray(L, np.float32) / np.asarray(Nx, np.float32)).astype(np.float64)
+    # initial fields: sin along z with real and imaginary amplitudes; a
+    # lamellar w- along y (the Flory initialize also sets w+ = -1.5i)
+    PI = float(np.float32(3.141592654))
+    wpl0 = (0.3 + 0.1j) * np.sin(2 * PI * n[:, 2] * dx[2] * 2 / float(np.float32(L[2])))
+    arg = np.float32(2.0 * np.float32(PI) * np.float32(2) / np.float32(L[1]))
+    wmi0 = 0.2 * np.sin((arg * (n[:, 1].astype(np.float32) * np.float32(dx[1]))).astype(np.float32).astype(np.float64))
+    spec = dict(Nx=Nx, L=L, Nr=12.0, C=4.0, nspecies=2, cl=True, seed=9,
+                potentials=[dict(kind="Incompress", dt_pl=0.01),
+                            dict(kind="Flory", I=0, J=1, chiN=12.0, dt_pl=0.01, dt_mi=0.01)],
+                molecules=[dict(blocks=[(5, 0), (7, 1)], phi=1.0)])
+    fields = {(0, "pl"): wpl0.astype(np.complex64), (1, "pl"): np.full(M, -1.5j, np.complex64),
+              (1, "mi"): wmi0.astype(np.complex64)}
+    s = systems.fts_system(spec, fields)
+    avg = np.zeros((2, M), np.complex128)
+    for it in range(5):
+        s.step(1)
+        avg += s.densities()
+        if it % 2:
+            continue
+    # after the run: the files hold iteration 4's state
+    coords, w = read_grid_text(tmp_path / "wpl_Incompress0.dat")
+    assert np.allclose(coords, n * dx, atol=1e-6)
+    tol = dict(rtol=2e-4, atol=2e-4)   # %1.4e: five significant digits
+    assert np.allclose(w, s.get_fields(0), **tol)
+    wp, wm = s.get_fields(1, minus=True)
+    assert np.allclose(read_grid_text(tmp_path / "wpl_Flory1.dat")[1], wp, **tol)
+    assert np.allclose(read_grid_text(tmp_path / "wmi_Flory1.dat")[1], wm, **tol)
+    rho = s.densities()
+    W = s.species_fields()
+    for sp in range(2):
+        got = read_grid_text(tmp_path / f"rhoSpecies{sp}.dat")[1]
+        assert np.allclose(got, rho[sp], rtol=2e-4, atol=2e-4 * np.abs(rho[sp]).max())
+        got = read_grid_text(tmp_path / f"rhoSpeciesAvg{sp}.dat")[1]
+        assert np.allclose(got, avg[sp] / 5, rtol=2e-4, atol=2e-4 * np.abs(avg[sp]).max())
+        got = read_grid_text(tmp_path / f"fieldSpecies{sp}.dat")[1]
+        assert np.allclose(got, W[sp], rtol=2e-4, atol=2e-4 * np.abs(W[sp]).max())
+    s.close()
+    # readDatFile reads the written w+ back as the initial field of a new run
+    text = FTS_FIELDS_INPUT.replace("initialize sin 2 0.3 0.1 2", "initialize readDatFile wpl_Incompress0.dat")
+    text = text.replace("run vt 5", "run vt 1").replace("chemFieldFreq 2", "chemFieldFreq 0")
+    (tmp_path / "input2").write_text(text)
+    r = run_mft(["-in", "input2"], cwd=tmp_path)
+    assert r.returncode == 0, r.stdout[-3000:]
+    w_read = read_grid_text(tmp_path / "wpl_Incompress0.dat")[1]
+    s = systems.fts_system(spec, dict(fields, **{(0, "pl"): w_read.astype(np.complex64)}))
+    s.step(1)
+    rows = read_log(tmp_path / "fts_data.dat")
+    Q, H, parts = s.observables()
+    assert close(rows[-1][1:3], [H.real, H.imag]), (rows[-1], H)
+    s.close()
