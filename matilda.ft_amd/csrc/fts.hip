@@ -132,6 +132,17 @@ __device__ __forceinline__ void add_at(double2 (&a)[N], int k, double x, double 
     }
 }
 
+// v times the per-axis diffusion factor g = hi + lo (exact in double) with a
+// single rounding to float.  Rounding v*hi first and adding v*lo after rounds
+// twice, and when the slice mean sits near a power of two (weak fields keep
+// it near 1) the first rounding pulls the mean the same way every bead: log Q
+// drifted by ~1e-6 over 20 beads that way.
+__device__ __forceinline__ float2 scale_hilo(float2 v, float2 g)
+{
+    const double gd = (double)g.x + (double)g.y;
+    return make_float2((float)((double)v.x * gd), (float)((double)v.y * gd));
+}
+
 // Elementwise update of one replica (blockIdx.y selects the replica).  NS
 // and NP bound the species and potentials of every replica of the launch, so
 // the per-thread arrays stay in registers.
@@ -380,9 +391,9 @@ __global__ void __launch_bounds__(256) k_fts_prop_axis(const PropDesc* __restric
 #pragma unroll
     for (int s = 0; s < E; ++s) {
         // two-float factor: the per-axis 1/N of a non-power-of-two length is not
-        // a float, and rounding it once would bias the mean every slice
+        // a float; applied with one rounding (scale_hilo)
         const float2 g = gk[t + T * s];
-        v[s] = make_float2(fmaf(v[s].x, g.y, v[s].x * g.x), fmaf(v[s].y, g.y, v[s].y * g.x));
+        v[s] = scale_hilo(v[s], g);
     }
     line_fft<N, +1>(v, lds, tw, t, map);
     if (!valid) return;
@@ -467,7 +478,7 @@ __global__ void __launch_bounds__(NT) k_fts_prop_xy(const PropDesc* __restrict__
 #pragma unroll
             for (int s = 0; s < E; ++s) {
                 const float2 g = gx[t + T * s];
-                v[s] = make_float2(fmaf(v[s].x, g.y, v[s].x * g.x), fmaf(v[s].y, g.y, v[s].y * g.x));
+                v[s] = scale_hilo(v[s], g);
             }
             line_fft<N, +1, true>(v, base, tw, t, map);
             exchange_barrier<true>();
@@ -494,7 +505,7 @@ __global__ void __launch_bounds__(NT) k_fts_prop_xy(const PropDesc* __restrict__
 #pragma unroll
             for (int s = 0; s < E; ++s) {
                 const float2 g = gy[t + T * s];
-                v[s] = make_float2(fmaf(v[s].x, g.y, v[s].x * g.x), fmaf(v[s].y, g.y, v[s].y * g.x));
+                v[s] = scale_hilo(v[s], g);
             }
             line_fft<N, +1>(v, base, tw, t, map);
             __syncthreads();
