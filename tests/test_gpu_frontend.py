@@ -550,10 +550,12 @@ def test_fts_field_files_and_initialize_options(gpu, tmp_path):
     text = FTS_FIELDS_INPUT.replace("initialize sin 2 0.3 0.1 2", "initialize readDatFile wpl_Incompress0.dat")
     text = text.replace("run vt 5", "run vt 1").replace("chemFieldFreq 2", "chemFieldFreq 0")
     (tmp_path / "input2").write_text(text)
+    w_read = read_grid_text(tmp_path / "wpl_Incompress0.dat")[1]
     r = run_mft(["-in", "input2"], cwd=tmp_path)
     assert r.returncode == 0, r.stdout[-3000:]
-    w_read = read_grid_text(tmp_path / "wpl_Incompress0.dat")[1]
-    s = systems.fts_system(spec, dict(fields, **{(0, "pl"): w_read.astype(np.complex64)}))
+    fields2 = dict(fields)
+    fields2[(0, "pl")] = w_read.astype(np.complex64)
+    s = systems.fts_system(spec, fields2)
     s.step(1)
     rows = read_log(tmp_path / "fts_data.dat")
     Q, H, parts = s.observables()
