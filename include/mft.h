@@ -444,6 +444,17 @@ int mft_fts_density(float* rho, const float* q, const float* qdag, const float* 
  * Synchronises the stream. */
 int mft_fts_partition_function(double* Q, const float* q_last, double gvol, double V, long M, void* work,
                                void* stream);
+/* Allocation-free variant of mft_fts_partition_function, with no counterpart
+ * in the reference: for callers that cannot allocate device memory (code
+ * around HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same result, bit for bit, without the work argument: the FP64
+ * partial sums live in a library-owned static device array (2 MiB per
+ * device, held until the process ends), so M is limited to 2^24 points
+ * (256^3); larger grids return MFT_ERR_INVALID.  Calls no hipMalloc/hipFree.
+ * Calls that overlap on one device share that array and race.  Synchronises
+ * the stream. */
+int mft_fts_partition_function_noalloc(double* Q, const float* q_last, double gvol, double V, long M,
+                                       void* stream);
 /* One potential's Hamiltonian term (fts_potential_flory.cu:252-274,
  * fts_potential_helfand.cu:254-275, fts_potential_incompress.cu:232-247),
  * FP64 sums.  H: host double[2]; work as above.  Synchronises the stream. */

@@ -8,6 +8,7 @@
 // entry points trade speed for the reference's data flow.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "tild_impl.h"
@@ -505,6 +506,55 @@ static int csum(const float2* a, long M, bool squares, double out[4], double* dp
     return sum_partials(dpart, (int)nblk(M), 4, out, st);
 }
 
+// ------------------------------------------- allocation-free Q variant ---
+// Device-resident scratch of mft_fts_partition_function_noalloc: the four
+// FP64 block partial sums of k_csum for every 256-point block of the largest
+// accepted grid, followed by the four totals.  Every call writes all the
+// partials it later reads, so nothing needs clearing between calls; calls that
+// overlap on one device share this array and race.
+constexpr long kNoAllocMaxPoints = 1L << 24;                     // 256^3 grid points
+constexpr int kNoAllocMaxBlocks = (int)(kNoAllocMaxPoints / 256);
+constexpr int kMaxDevicesNoAlloc = 64;
+static __device__ __align__(256) double scratch[4 * kNoAllocMaxBlocks + 4];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[kMaxDevicesNoAlloc];   // device address of `scratch` per device
+
+// Address of this device's copy of `scratch`, looked up once per device with
+// that device current (the array's host-side name is not a device address).
+static int scratch_address(double** out)
+{
+    int dev = 0;
+    MFT_HIP(hipGetDevice(&dev));
+    MFT_REQUIRE(dev >= 0 && dev < kMaxDevicesNoAlloc, MFT_ERR_UNSUPPORTED, "device %d beyond the %d supported",
+                dev, kMaxDevicesNoAlloc);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(MFT_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d failed: %s", dev,
+                      hipGetErrorString(e));
+            return MFT_ERR_DEVICE;
+        }
+        g_scratch_dev[dev] = (double*)p;
+    }
+    *out = g_scratch_dev[dev];
+    return MFT_OK;
+}
+
+// out[c] = sum_b part[b*ncomp + c], one thread per component adding the
+// partials in block order: the same order and rounding as sum_partials on
+// the host
+__global__ void k_sum_partials_ordered(double* __restrict__ out, const double* __restrict__ part, int n, int ncomp)
+{
+    const int c = threadIdx.x;
+    if (c >= ncomp) return;
+    double s = 0.0;
+    for (int b = 0; b < n; ++b) s += part[(long)b * ncomp + c];
+    out[c] = s;
+}
+
 } // namespace mft
 
 extern "C" {
@@ -652,6 +702,29 @@ int mft_fts_partition_function(double* Q, const float* q_last, double gvol, doub
     double s[4];
     int rc = csum((const float2*)q_last, M, false, s, (double*)work, (hipStream_t)stream);
     if (rc) return rc;
+    // fts_molecule_linear.cu:265-266
+    Q[0] = gvol / V * s[0];
+    Q[1] = gvol / V * s[1];
+    return MFT_OK;
+}
+
+int mft_fts_partition_function_noalloc(double* Q, const float* q_last, double gvol, double V, long M, void* stream)
+{
+    MFT_REQUIRE(Q && q_last && M > 0 && V > 0.0, MFT_ERR_INVALID, "bad argument");
+    MFT_REQUIRE(M <= kNoAllocMaxPoints, MFT_ERR_INVALID,
+                "grid of %ld points exceeds the allocation-free limit of %ld points (256^3)", M, kNoAllocMaxPoints);
+    double* part = nullptr;
+    int rc = scratch_address(&part);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const int nb = (int)nblk(M);
+    double* tot = part + 4L * kNoAllocMaxBlocks;
+    k_csum<<<nb, 256, 0, st>>>(part, (const float2*)q_last, M, 0);
+    k_sum_partials_ordered<<<1, 64, 0, st>>>(tot, part, nb, 4);
+    MFT_LAUNCH_CHECK();
+    double s[4];
+    MFT_HIP(hipMemcpyAsync(s, tot, sizeof(s), hipMemcpyDeviceToHost, st));
+    MFT_HIP(hipStreamSynchronize(st));
     // fts_molecule_linear.cu:265-266
     Q[0] = gvol / V * s[0];
     Q[1] = gvol / V * s[1];

@@ -603,6 +603,20 @@ def fts_partition_function(q_last, gvol, V):
     return complex(out[0], out[1])
 
 
+def fts_partition_function_noalloc(q_last, gvol, V, M=None):
+    """mft_fts_partition_function_noalloc: complex Q of the first M points of
+    q_last (a DeviceBuffer or an array), without a work buffer."""
+    if isinstance(q_last, DeviceBuffer):
+        qb = q_last
+    else:
+        q_last = _c64(q_last)
+        M = q_last.size if M is None else M
+        qb = DeviceBuffer.from_array(q_last)
+    out = (_d * 2)()
+    check(lib().mft_fts_partition_function_noalloc(out, qb.ptr, _d(gvol), _d(V), C.c_long(M), None))
+    return complex(out[0], out[1])
+
+
 def fts_hamiltonian_term(pot, wpl, wmi, C_, gvol):
     """mft_fts_hamiltonian_term: complex H of one potential."""
     wpl = _c64(wpl)
