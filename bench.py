@@ -235,7 +235,7 @@ def serve_measure(seed=11, window=1.0, rate=50.0, tild_frac=0.2, max_iters=25, b
             reqs.append((t, "fts", (spec, fields), int(rng.uniform(500, 5000))))
     free = mft.mem_info()[0]
     srv = mft.Server(hbm_budget_bytes=budget_frac * free, max_fts_replicas=max_pool)
-    arrival, finish, kind = {}, {}, {}
+    arrival, finish, kind, ok = {}, {}, {}, {}
     errors = []
     pending = list(reqs)
     running = set()
@@ -255,23 +255,29 @@ def serve_measure(seed=11, window=1.0, rate=50.0, tild_frac=0.2, max_iters=25, b
         for rid in list(running):
             st = srv.status(rid)
             if st in ("done", "failed"):
-                finish[rid] = now if st == "done" else float("nan")
+                finish[rid] = now
+                ok[rid] = st == "done"
                 if st == "failed":
                     errors.append(srv.error(rid))
                 running.discard(rid)
                 srv.release(rid)
     total = time.perf_counter() - t0
     srv.close()
+    # latency = arrival -> result, a failed request's result being its error;
+    # failed requests stay in the percentiles and are also counted apart
     lat = {k: np.array([finish[r] - arrival[r] for r in finish if kind[r] == k]) for k in ("fts", "tild")}
+    good = {k: np.array([finish[r] - arrival[r] for r in finish if kind[r] == k and ok[r]]) for k in ("fts", "tild")}
     allv = np.concatenate([lat["fts"], lat["tild"]])
-    nf = int(np.sum(np.isfinite(lat["fts"])))
+    allg = np.concatenate([good["fts"], good["tild"]])
+    pct = lambda a, q: float(np.percentile(a, q)) if a.size else None
+    nf = int(good["fts"].size)
     return {"requests": len(reqs), "fts_requests": int(lat["fts"].size), "tild_requests": int(lat["tild"].size),
             "arrival_window_s": window, "rate_per_s": rate, "wall_s": total,
-            "p50_latency_s": float(np.nanpercentile(allv, 50)), "p99_latency_s": float(np.nanpercentile(allv, 99)),
-            "fts_p50_s": float(np.nanpercentile(lat["fts"], 50)) if lat["fts"].size else None,
-            "fts_p99_s": float(np.nanpercentile(lat["fts"], 99)) if lat["fts"].size else None,
-            "solves_per_hour": 3600.0 * nf / total,
-            "failed": int(np.sum(~np.isfinite(allv))), "errors": sorted(set(errors))[:5]}
+            "p50_latency_s": pct(allv, 50), "p99_latency_s": pct(allv, 99),
+            "p50_completed_s": pct(allg, 50), "p99_completed_s": pct(allg, 99),
+            "fts_p50_s": pct(lat["fts"], 50), "fts_p99_s": pct(lat["fts"], 99),
+            "solves_per_hour": 3600.0 * nf / total, "completed_fts_solves": nf,
+            "failed": int(sum(1 for r in ok if not ok[r])), "errors": sorted(set(errors))[:5]}
 
 
 def main():

@@ -2,10 +2,12 @@
 // input file line by line; `box` creates a box and hands it the following
 // lines up to endBox; `run nvt|vt N` runs it; `modify` edits it.
 //
-//   mft [-in input] [-device k] [--dry-run]
+//   mft [-in input] [-device k] [--dry-run] [--dump-topology file]
 //
 // --dry-run parses the input and prints the engine description as JSON
-// instead of creating the engine (no GPU needed).
+// instead of creating the engine (no GPU needed).  --dump-topology writes the
+// particle box's topology ("species charge molecule nBonds bondedTo[MAXBONDS]
+// bondType[MAXBONDS]" per particle) to file.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -15,6 +17,7 @@
 #include "matilda.h"
 
 bool Box::dry_run = false;
+std::string Box::topology_dump;
 
 void die(const std::string& msg)
 {
@@ -106,6 +109,8 @@ int main(int argc, char** argv)
             if (dev != 0 || n < 1) die("Device " + std::to_string(dev) + " not found; this build drives device 0");
         } else if (s == "--dry-run") {
             Box::dry_run = true;
+        } else if (s == "--dump-topology" && a + 1 < argc) {
+            Box::topology_dump = argv[++a];
         } else {
             die(s + " is not a valid argument");
         }

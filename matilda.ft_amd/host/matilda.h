@@ -96,6 +96,9 @@ public:
     const float* boxL() const { return L; }
     // print the engine descriptor instead of creating the engine
     static bool dry_run;
+    // when not empty: the particle box writes its topology here once its
+    // molecules and data files are read (one text line per particle)
+    static std::string topology_dump;
 
 protected:
     std::string boxStyle;

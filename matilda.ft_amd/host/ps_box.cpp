@@ -558,7 +558,11 @@ void PS_Box::makeLinear(std::istringstream& iss)
     if (nmolecs < 0) die("molecule linear: negative molecule count");
     std::cout << "Generating " << nmolecs << " molecules each with " << Ntot << " sites" << std::endl;
     int ind = nstot;
-    const int molec0 = nstot > 0 ? *std::max_element(mID.begin(), mID.end()) + 1 : 0;
+    // the first new molecule takes the largest molecule id already in the box
+    // (0 in an empty box), so a second molecule command starts at the id of
+    // the previous command's last molecule, as the reference's search over
+    // mID does (PS_Box_moleculeMakers.cu:140-145, 296, 306)
+    const int molec0 = nstot > 0 ? std::max(0, *std::max_element(mID.begin(), mID.begin() + nstot)) : 0;
     nstot += nmolecs * Ntot;
     grow(nstot);
     std::vector<int> spi(numBlocks);
@@ -708,6 +712,17 @@ void PS_Box::finishInitialization()
     if (L[0] <= 0.f) die("boxLengths must be defined");
     if (species.empty()) die("no species defined");
     if (nstot == 0) die("Box created with no particles");
+    if (!topology_dump.empty()) {
+        FILE* fp = std::fopen(topology_dump.c_str(), "w");
+        if (!fp) die("Failed to open " + topology_dump);
+        for (int i = 0; i < nstot; ++i) {
+            std::fprintf(fp, "%d %.9g %d %d", intSpecies[i], doCharges ? charges[i] : 0.f, mID[i], nBonds[i]);
+            for (int b = 0; b < MAXBONDS; ++b) std::fprintf(fp, " %d", b < nBonds[i] ? bondedTo[i * MAXBONDS + b] : -1);
+            for (int b = 0; b < MAXBONDS; ++b) std::fprintf(fp, " %d", b < nBonds[i] ? bondType[i * MAXBONDS + b] : -1);
+            std::fprintf(fp, "\n");
+        }
+        std::fclose(fp);
+    }
     if (integrators.size() != 1) die("this build runs exactly one integrator on group all");
     if (integrators[0]->groupName != "all") die("integrator group must be 'all' in this build");
     nBondsTot = 0;

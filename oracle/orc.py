@@ -301,12 +301,16 @@ def gjf_coeffs(delt, mass, mobility):
     return out
 
 
-def make_linear(nmolecs, blocks, species_index, block_bond_type=None, charges=None, maxbonds=2, start=0):
+def make_linear(nmolecs, blocks, species_index, block_bond_type=None, charges=None, maxbonds=2, start=0,
+                mid0=0):
     """Topology of PS_Box::makeLinear (PS_Box_moleculeMakers.cu:15-311).
 
     blocks = [(N_b, species_name)], species_index maps names to integer types.
     Returns intSpecies, nBonds, bondedTo, bondType, charge, mID arrays for the new
-    molecules; bead order is molecule-major then block-major.
+    molecules; bead order is molecule-major then block-major.  start is the
+    index of the first new bead; mid0 the id of the first new molecule, which
+    the reference takes as the largest id already in the box, 0 when empty
+    (PS_Box_moleculeMakers.cu:140-145; ids then grow by one per molecule, :306).
     """
     nb = len(blocks)
     if block_bond_type is None:
@@ -335,7 +339,7 @@ def make_linear(nmolecs, blocks, species_index, block_bond_type=None, charges=No
                     bondedTo[ind * maxbonds + nBonds[ind]] = gi + 1
                     bondType[ind * maxbonds + nBonds[ind]] = block_bond_type[j] - 1
                     nBonds[ind] += 1
-                mID[ind] = m
+                mID[ind] = mid0 + m
                 ind += 1
     return dict(intSpecies=intSpecies, nBonds=nBonds, bondedTo=bondedTo, bondType=bondType, charge=q, mID=mID)
 

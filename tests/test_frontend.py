@@ -290,3 +290,62 @@ def test_read_data_rejects_bad_bond_lines(tmp_path, bad):
     r = dry(tmp_path, ps_input(cfg, "init.data", steps=2, log_freq=1))
     assert r.returncode == 1, r.stdout[-1000:]
     assert ("bond partner out of range" if bad == "partner" else "bond type out of range") in r.stdout
+
+
+MAKELINEAR_INPUT = """box ps
+Dim 3
+boxLengths 12 10 8
+grid 16 16 16
+pmeorder 3
+randSeed 77
+rho0 3.0
+species A
+species B
+species C
+bond 1 harmonic 1.5 0.0
+bond 2 harmonic 2.0 0.5
+molecule linear nmolecs 7 2 4 A 3 B bondType 1 2 charge 1.0 -1.0
+molecule linear nmolecs 5 3 1 C 2 A 1 B bondType 2 1 1
+molecule linear nmolecs 9 1 1 C
+molecule linear phi 0.05 1 6 B
+potential gaussian A B 1.0 0.5
+integrator all GJF delt 0.005
+logFreq 100
+endBox
+run nvt 1
+"""
+
+
+def test_make_linear_topology_of_the_front_end_is_the_oracles(tmp_path):
+    # the C++ PS_Box::makeLinear (the drop-in's topology source) against the
+    # oracle's make_linear, bit for bit, over four molecule commands: block
+    # bond types, charges, a one-bead molecule, phi-derived counts
+    # (PS_Box_moleculeMakers.cu:15-311) and molecule ids continuing from the
+    # largest id already present (:140-145)
+    import numpy as np
+    import orc
+    (tmp_path / "input").write_text(MAKELINEAR_INPUT)
+    r = run_mft(["-in", "input", "--dry-run", "--dump-topology", "topo.txt"], cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:]
+    got = np.loadtxt(tmp_path / "topo.txt")
+    sp = {"A": 0, "B": 1, "C": 2}
+    nphi = int(np.float32(3.0 * 12 * 10 * 8 * 0.05) / np.float32(6))   # int(rho0 V phi / Ntot)
+    cmds = [(7, [(4, "A"), (3, "B")], [1, 2], [1.0, -1.0]), (5, [(1, "C"), (2, "A"), (1, "B")], [2, 1, 1], None),
+            (9, [(1, "C")], None, None), (nphi, [(6, "B")], None, None)]
+    start, mids, rows = 0, [], []
+    for nm, blocks, bt, q in cmds:
+        mid0 = max([0] + mids)
+        t = orc.make_linear(nm, blocks, sp, block_bond_type=bt, charges=q, start=start, mid0=mid0)
+        n = len(t["intSpecies"])
+        for i in range(n):
+            nb = int(t["nBonds"][i])
+            bto = [int(t["bondedTo"][2 * i + b]) if b < nb else -1 for b in range(2)]
+            bty = [int(t["bondType"][2 * i + b]) if b < nb else -1 for b in range(2)]
+            rows.append([t["intSpecies"][i], t["charge"][i], t["mID"][i], nb] + bto + bty)
+        mids += list(t["mID"])
+        start += n
+    want = np.array(rows, np.float64)
+    assert got.shape == want.shape, (got.shape, want.shape)
+    assert np.array_equal(got, want)
+    # the second command's first molecule shares the id of the first command's last one
+    assert got[7 * 7, 2] == got[7 * 7 - 1, 2] == 6
