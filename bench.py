@@ -381,7 +381,7 @@ def main():
         "hbm_frac_step": hbm_step / HBM_PEAK_GBS,
         "step_algorithmic_gbs": hbm_step,
         "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step_ms.items()},
-        "autotune": {"gather": ["direct", "lds_bricks"][tuned]},
+        "autotune": {"gather": ["direct", "lds_bricks"][tuned & 1], "xy_passes": ["per_axis", "per_plane"][tuned >> 1]},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes": kbytes[dom],
                      "avg_launch_ms": avg_ms[dom], "traffic": traffic},

@@ -291,11 +291,16 @@ int mft_tild_kernel_bytes(mft_tild* eng, double* bytes);
 int mft_tild_num_outputs(mft_tild* eng, int* nout);
 /* Times the kernel variants of this configuration on the device (reps
  * launches each, state unchanged) and keeps the fastest; choice receives the
- * selected gather variant (0 direct loads, 1 LDS-staged bricks).  The two
- * can differ in the last bit of a force. */
+ * selected variants: bit 0 the gather (0 direct loads, 1 LDS-staged bricks;
+ * the two can differ in the last bit of a force), bit 1 the x-y transform
+ * passes (0 one pass per axis, 1 one pass per z-plane; equal to rounding). */
 int mft_tild_autotune(mft_tild* eng, int reps, int* choice);
 /* Selects a gather variant directly (as above; the default is 1). */
 int mft_tild_set_gather(mft_tild* eng, int variant);
+/* Selects the x-y transform passes directly: 0 one pass per axis (default),
+ * 1 both axes of a z-plane in one workgroup (square 64, 96 or 128 point
+ * planes; MFT_ERR_UNSUPPORTED otherwise). */
+int mft_tild_set_grid_passes(mft_tild* eng, int variant);
 /* PS_ComputeSK::do_compute (ps_compute_sk.cu:18-45): sk[M] (device) = or +=
  * |FFT[rho_group]|^2 with the forward transform scaled by 1/M, rho_group the
  * sum of the densities of the species in species_mask at the last density

@@ -370,14 +370,21 @@ static int grid_forces(TildEngine* e)
     }
     const long M = e->M;
     AxisGeom gx{d.nx, d.ny, d.nz, 0}, gy{d.nx, d.ny, d.nz, 1};
-    {
+    const TileGeom& tg = e->tg;
+    const long cstride = (long)tg.ex[0] * tg.ex[1] * tg.ex[2];
+    const bool plane = e->plane_passes && tild_plane_supported(d.nx, d.ny);
+    if (plane) {
+        // x and y of each z-plane in one pass (timed as the x pass)
         TimedScope ts(e, MFT_TK_FFT_X_FWD);
-        const TileGeom& tg = e->tg;
-        if ((rc = tild_x_fwd_tiles(e->tiles, e->tile_tab, e->S, d.nx, d.ny, d.nz, e->nchan,
-                                   (long)tg.ex[0] * tg.ex[1] * tg.ex[2], e->stream)))
+        if ((rc = tild_xy_fwd_tiles_plane(e->tiles, e->tile_tab, e->S, d.nx, d.ny, d.nz, e->nchan, cstride,
+                                          e->stream)))
             return rc;
-    }
-    {
+    } else {
+        {
+            TimedScope ts(e, MFT_TK_FFT_X_FWD);
+            if ((rc = tild_x_fwd_tiles(e->tiles, e->tile_tab, e->S, d.nx, d.ny, d.nz, e->nchan, cstride, e->stream)))
+                return rc;
+        }
         TimedScope ts(e, MFT_TK_FFT_Y_FWD);
         if ((rc = tild_axis(-1, 0, 0, e->S, e->S, gy, e->nchan, M, 1.0f, e->stream))) return rc;
     }
@@ -385,13 +392,15 @@ static int grid_forces(TildEngine* e)
         TimedScope ts(e, MFT_TK_CONV_Z);
         if ((rc = tild_conv(e))) return rc;
     }
-    {
+    if (!plane) {
         TimedScope ts(e, MFT_TK_FFT_Y_INV);
         if ((rc = tild_axis(+1, 0, 0, e->Y, e->Y, gy, e->npk, M, 1.0f, e->stream))) return rc;
     }
     {
         TimedScope ts(e, MFT_TK_FFT_X_INV);
-        if ((rc = tild_x_inv_packed(e->Y, e->F4, d.nx, d.ny, d.nz, e->nout4, e->stream))) return rc;
+        if (plane) rc = tild_yx_inv_packed_plane(e->Y, e->F4, d.nx, d.ny, d.nz, e->nout4, e->stream);
+        else rc = tild_x_inv_packed(e->Y, e->F4, d.nx, d.ny, d.nz, e->nout4, e->stream);
+        if (rc) return rc;
         if (!e->bias.empty() && (rc = tild_bias_add(e))) return rc;
     }
     {
@@ -516,6 +525,7 @@ static int do_steps(TildEngine* e, int nsteps, bool want_f, bool advance = true,
         }
         if (!advance) continue;
         if (d.integrator == MFT_INT_VV) e->vv_state = 1;
+        if (d.integrator == MFT_INT_GJF) e->nz_keyed = true;
         if (e->run_max > 0) update_ramps(e, e->run_pos++, e->run_max);
         else update_ramps(e, i, nsteps);
         e->step++;
@@ -903,16 +913,25 @@ int mft_tild_kernel_bytes(mft_tild* e, double* bytes)
     bytes[MFT_TK_SCAN] = (double)e->nkeys * (4 + 4 + 4);      // count read, start write, count clear
     // permutation: key, rank, start, next x and noise, x, id/species, bonds
     // in; x, previous x, noise, id/species, bonds, map out; bond slots remapped
-    bytes[MFT_TK_SCATTER] = N * ((4 + 4 + 4 + 16 + 16 + 16 + 8 + 4 + 4 * mb) + (16 + 16 + 16 + 8 + 4 + 4 * mb + 4) +
+    // under GJF the previous noise is regenerated, not carried (16 bytes in
+    // and out of the permutation and of the integrator fewer)
+    const double nz = e->d.integrator == MFT_INT_GJF ? 0.0 : 16.0;
+    bytes[MFT_TK_SCATTER] = N * ((4 + 4 + 4 + 16 + nz + 16 + 8 + 4 + 4 * mb) + (16 + 16 + nz + 8 + 4 + 4 * mb + 4) +
                                  (4 + 4 + 8 * mb + 4 * mb));
     bytes[MFT_TK_DENSITY] = N * (16 + 8) + M * 4 * C;         // key-ordered records, rho
-    bytes[MFT_TK_FFT_X_FWD] = M * C * (4 + 8);              // densities in, spectra out (tiles summed on the fly)
-    bytes[MFT_TK_FFT_Y_FWD] = M * C * 16;
+    const bool plane = e->plane_passes && tild_plane_supported(e->d.nx, e->d.ny);
+    if (plane) {
+        bytes[MFT_TK_FFT_X_FWD] = M * C * (4 + 8);          // densities in, x-y spectra out
+        bytes[MFT_TK_FFT_X_INV] = M * O * 8 + M * O4 * 16;  // packed spectra in, force fields out
+    } else {
+        bytes[MFT_TK_FFT_X_FWD] = M * C * (4 + 8);          // densities in, spectra out (tiles summed on the fly)
+        bytes[MFT_TK_FFT_Y_FWD] = M * C * 16;
+        bytes[MFT_TK_FFT_Y_INV] = M * O * 16;
+        bytes[MFT_TK_FFT_X_INV] = M * O * 8 + M * O4 * 16;
+    }
     bytes[MFT_TK_CONV_Z] = M * 8 * (C + O);
-    bytes[MFT_TK_FFT_Y_INV] = M * O * 16;
-    bytes[MFT_TK_FFT_X_INV] = M * O * 8 + M * O4 * 16;
     bytes[MFT_TK_GATHER] = N * (16 + 8 + 16) + M * 16 * O4;
-    bytes[MFT_TK_INTEGRATE] = N * ((16 + 16 + 16 + 8 + 4 + 8 * mb + 16) + (16 + 16 + 4 + 4));
+    bytes[MFT_TK_INTEGRATE] = N * ((16 + 16 + nz + 8 + 4 + 8 * mb + 16) + (16 + nz + 4 + 4));
     return MFT_OK;
 }
 
@@ -921,6 +940,16 @@ int mft_tild_set_gather(mft_tild* e, int variant)
     MFT_REQUIRE(e, MFT_ERR_INVALID, "null argument");
     MFT_REQUIRE(variant == 0 || variant == 1, MFT_ERR_INVALID, "gather variant %d", variant);
     e->gather_mode = variant;
+    return MFT_OK;
+}
+
+int mft_tild_set_grid_passes(mft_tild* e, int variant)
+{
+    MFT_REQUIRE(e, MFT_ERR_INVALID, "null argument");
+    MFT_REQUIRE(variant == 0 || variant == 1, MFT_ERR_INVALID, "grid pass variant %d", variant);
+    MFT_REQUIRE(variant == 0 || tild_plane_supported(e->d.nx, e->d.ny), MFT_ERR_UNSUPPORTED,
+                "plane passes need square x-y planes of 64, 96 or 128 points (grid %d x %d)", e->d.nx, e->d.ny);
+    e->plane_passes = variant;
     return MFT_OK;
 }
 
@@ -956,10 +985,32 @@ int mft_tild_autotune(mft_tild* e, int reps, int* choice)
             best_mode = mode;
         }
     }
+    e->gather_mode = best_mode;
+    // line or plane x-y passes: the whole grid-force evaluation timed with
+    // each (densities to per-particle grid forces; the state is unchanged)
+    int best_plane = 0;
+    if (tild_plane_supported(e->d.nx, e->d.ny)) {
+        float bestg = 1e30f;
+        for (int v = 0; v < 2; ++v) {
+            e->plane_passes = v;
+            if ((rc = grid_forces(e))) return rc;   // warm
+            MFT_HIP(hipEventRecord(a, e->stream));
+            for (int r = 0; r < reps; ++r)
+                if ((rc = grid_forces(e))) return rc;
+            MFT_HIP(hipEventRecord(b, e->stream));
+            MFT_HIP(hipEventSynchronize(b));
+            float ms = 0.f;
+            MFT_HIP(hipEventElapsedTime(&ms, a, b));
+            if (ms < bestg) {
+                bestg = ms;
+                best_plane = v;
+            }
+        }
+    }
+    e->plane_passes = best_plane;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    e->gather_mode = best_mode;
-    if (choice) *choice = best_mode;
+    if (choice) *choice = best_mode | (best_plane << 1);
     return MFT_OK;
 }
 

@@ -147,6 +147,229 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd_tiles(const float* __restrict
     for (int s = 0; s < E; ++s) dst[t + T * s] = v[s];
 }
 
+// ------------------------------------------------------- plane passes ---
+// The x and y transforms of one z-plane of an N x N grid in one workgroup:
+// the plane sits in LDS with a row pitch of N + 1 (a column's elements fall in
+// different banks) and its lines are transformed in place, T consecutive
+// threads per line (so a line's exchanges stay inside one wavefront), LINES
+// lines at a time.  The arithmetic is that of the line passes (same plans,
+// twiddles and summation order; the compiler may still contract different
+// multiply-adds in the two kernels, so results can differ by rounding); the
+// plane passes save one HBM round trip of the spectra per direction.
+template <int N>
+struct PlaneShape {
+    using P = FftPlan<N>;
+    static constexpr int T = P::T, E = P::E;
+    static constexpr int NT = T * N >= 1024 ? 1024 : T * N;
+    static constexpr int LINES = NT / T;
+    static constexpr int ROUNDS = N / LINES;
+    static constexpr int PITCH = N + 1;
+    static constexpr size_t LDS = sizeof(float2) * N * PITCH;
+    static_assert(N % LINES == 0, "whole rounds of lines");
+};
+
+// Forward x then y transforms of channel blockIdx.y, z-plane blockIdx.x, from
+// the density tiles (each grid value summed from the covering tile entries in
+// k_tile_reduce's order), written as spectra x-fastest.
+template <int N>
+__global__ void __launch_bounds__(PlaneShape<N>::NT) k_fft_xy_fwd_tiles_plane(
+    const float* __restrict__ tiles, const int* __restrict__ tab, float2* __restrict__ out, int nz, long cstride,
+    const float2* __restrict__ twg)
+{
+    using S = PlaneShape<N>;
+    constexpr int T = S::T, E = S::E, NT = S::NT, LINES = S::LINES, PITCH = S::PITCH;
+    extern __shared__ float2 plane[];
+    __shared__ float2 tw[N];
+    __shared__ int tabxy[5 * 2 * N];
+    for (int i = threadIdx.x; i < N; i += NT) tw[i] = twg[i];
+    for (int i = threadIdx.x; i < 5 * 2 * N; i += NT) tabxy[i] = tab[i];
+    __syncthreads();
+    const int iz = blockIdx.x, c = blockIdx.y;
+    const float* tc = tiles + c * cstride;
+    const int* tz = tab + 5 * (2 * N + iz);
+    for (int i = threadIdx.x; i < N * N; i += NT) {
+        const int y = i / N, x = i - y * N;
+        plane[y * PITCH + x] = make_float2(tile_sum(tc, tz, tabxy + 5 * (N + y), tabxy + 5 * x), 0.f);
+    }
+    __syncthreads();
+    const int lt = threadIdx.x / T, t = threadIdx.x % T;
+#pragma unroll
+    for (int r = 0; r < S::ROUNDS; ++r) {
+        LdsMap map{r * LINES + lt, 0, false, PITCH};   // row y
+        float2 v[E];
+#pragma unroll
+        for (int s = 0; s < E; ++s) v[s] = plane[map(t + T * s)];
+        line_fft<N, -1, true>(v, plane, tw, t, map);
+#pragma unroll
+        for (int s = 0; s < E; ++s) plane[map(t + T * s)] = v[s];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < S::ROUNDS; ++r) {
+        LdsMap map{r * LINES + lt, PITCH, true, N};    // column x
+        float2 v[E];
+#pragma unroll
+        for (int s = 0; s < E; ++s) v[s] = plane[map(t + T * s)];
+        line_fft<N, -1, true>(v, plane, tw, t, map);
+#pragma unroll
+        for (int s = 0; s < E; ++s) plane[map(t + T * s)] = v[s];
+    }
+    __syncthreads();
+    float2* dst = out + (long)c * N * N * nz + (long)iz * N * N;
+    for (int i = threadIdx.x; i < N * N; i += NT) {
+        const int y = i / N, x = i - y * N;
+        dst[i] = plane[y * PITCH + x];
+    }
+}
+
+// Inverse y then x transforms of the packed force spectra of z-plane
+// blockIdx.x, written as the float4 force fields 2g and 2g + 1 (g =
+// blockIdx.y) like k_fft_x_inv_packed: the three packed planes 3g .. 3g + 2
+// are transformed one after the other, the first one's values (and the
+// second one's imaginary parts) kept in registers until the fields they
+// complete are written.
+template <int N>
+__global__ void __launch_bounds__(PlaneShape<N>::NT) k_fft_yx_inv_packed_plane(
+    const float2* __restrict__ Y, float4* __restrict__ F4, int nz, int nfield, int npk,
+    const float2* __restrict__ twg)
+{
+    using S = PlaneShape<N>;
+    constexpr int T = S::T, E = S::E, NT = S::NT, LINES = S::LINES, PITCH = S::PITCH, R = S::ROUNDS;
+    extern __shared__ float2 plane[];
+    __shared__ float2 tw[N];
+    for (int i = threadIdx.x; i < N; i += NT) tw[i] = twg[i];
+    const int iz = blockIdx.x, g = blockIdx.y;
+    const long M = (long)N * N * nz, pbase = (long)iz * N * N;
+    const int lt = threadIdx.x / T, t = threadIdx.x % T;
+    float2 keep[R][E];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int s = 0; s < E; ++s) keep[r][s] = make_float2(0.f, 0.f);
+    float4* d0 = F4 + (long)(2 * g) * M + pbase;
+    float4* d1 = F4 + (long)(2 * g + 1) * M + pbase;
+    const bool second = 2 * g + 1 < nfield;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int q = 3 * g + j;
+        const bool have = q < npk;   // a missing packed plane is zero
+        if (have) {
+            const float2* src = Y + (long)q * M + pbase;
+            for (int i = threadIdx.x; i < N * N; i += NT) {
+                const int y = i / N, x = i - y * N;
+                plane[y * PITCH + x] = src[i];
+            }
+        }
+        __syncthreads();
+        if (have) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                LdsMap map{r * LINES + lt, PITCH, true, N};    // column x
+                float2 v[E];
+#pragma unroll
+                for (int s = 0; s < E; ++s) v[s] = plane[map(t + T * s)];
+                line_fft<N, +1, true>(v, plane, tw, t, map);
+#pragma unroll
+                for (int s = 0; s < E; ++s) plane[map(t + T * s)] = v[s];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int row = r * LINES + lt;
+            float2 v[E];
+            if (have) {
+                LdsMap map{row, 0, false, PITCH};          // row y
+#pragma unroll
+                for (int s = 0; s < E; ++s) v[s] = plane[map(t + T * s)];
+                line_fft<N, +1, true>(v, plane, tw, t, map);
+            } else {
+#pragma unroll
+                for (int s = 0; s < E; ++s) v[s] = make_float2(0.f, 0.f);
+            }
+            const long o = (long)row * N + t;
+            if (j == 1) {
+                // field 2g: packed 3g (re, im) and 3g + 1 (re)
+#pragma unroll
+                for (int s = 0; s < E; ++s) d0[o + T * s] = make_float4(keep[r][s].x, keep[r][s].y, v[s].x, 0.f);
+            } else if (j == 2 && second) {
+                // field 2g + 1: packed 3g + 1 (im) and 3g + 2 (re, im)
+#pragma unroll
+                for (int s = 0; s < E; ++s) d1[o + T * s] = make_float4(keep[r][s].y, v[s].x, v[s].y, 0.f);
+            }
+#pragma unroll
+            for (int s = 0; s < E; ++s) keep[r][s] = v[s];
+        }
+        __syncthreads();   // the plane is reloaded next
+    }
+}
+
+// grids whose z-planes the plane passes handle (square planes that fit LDS)
+bool tild_plane_supported(int nx, int ny)
+{
+    return nx == ny && (nx == 64 || nx == 96 || nx == 128);
+}
+
+// the plane kernels use more dynamic LDS than the default limit: raise it
+// once per kernel (setting it again is harmless)
+static int allow_plane_lds(const void* fn, size_t bytes, bool* done)
+{
+    if (*done) return MFT_OK;
+    MFT_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    *done = true;
+    return MFT_OK;
+}
+
+int tild_xy_fwd_tiles_plane(const float* tiles, const int* tab, float2* S, int nx, int ny, int nz, int nchan,
+                            long cstride, hipStream_t st)
+{
+    if (!tild_plane_supported(nx, ny)) return MFT_ERR_UNSUPPORTED;
+    switch (nx) {
+#define MFT_CASE(n_)                                                                                      \
+    case n_: {                                                                                            \
+        using PS = PlaneShape<n_>;                                                                        \
+        static bool attr = false;                                                                         \
+        int rc = allow_plane_lds((const void*)k_fft_xy_fwd_tiles_plane<n_>, PS::LDS, &attr);              \
+        if (rc) return rc;                                                                                \
+        k_fft_xy_fwd_tiles_plane<n_><<<dim3(nz, nchan), PS::NT, PS::LDS, st>>>(tiles, tab, S, nz, cstride, \
+                                                                            fft_twiddles_device(n_));    \
+        break;                                                                                            \
+    }
+        MFT_CASE(64)
+        MFT_CASE(96)
+        MFT_CASE(128)
+#undef MFT_CASE
+    default: return MFT_ERR_UNSUPPORTED;
+    }
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
+int tild_yx_inv_packed_plane(const float2* Y, float4* F4, int nx, int ny, int nz, int nfield, hipStream_t st)
+{
+    if (!tild_plane_supported(nx, ny)) return MFT_ERR_UNSUPPORTED;
+    const int npk = (3 * nfield + 1) / 2;
+    switch (nx) {
+#define MFT_CASE(n_)                                                                                      \
+    case n_: {                                                                                            \
+        using PS = PlaneShape<n_>;                                                                        \
+        static bool attr = false;                                                                         \
+        int rc = allow_plane_lds((const void*)k_fft_yx_inv_packed_plane<n_>, PS::LDS, &attr);             \
+        if (rc) return rc;                                                                                \
+        k_fft_yx_inv_packed_plane<n_><<<dim3(nz, (nfield + 1) / 2), PS::NT, PS::LDS, st>>>(              \
+            Y, F4, nz, nfield, npk, fft_twiddles_device(n_));                                             \
+        break;                                                                                            \
+    }
+        MFT_CASE(64)
+        MFT_CASE(96)
+        MFT_CASE(128)
+#undef MFT_CASE
+    default: return MFT_ERR_UNSUPPORTED;
+    }
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
 int tild_x_fwd_tiles(const float* tiles, const int* tab, float2* S, int nx, int ny, int nz, int nchan, long cstride,
                      hipStream_t st)
 {

@@ -225,6 +225,11 @@ struct TildEngine {
     AngleParams ap{};
     int cur = 0;
     int vv_state = 0; // velocity Verlet: 0 before the first step, 1 afterwards
+    // GJF: the previous noise of every particle is nmag(type) times the
+    // Philox normals keyed by (particle, step - 1), because the last step was
+    // a GJF step of this engine; the integrator then regenerates it instead of
+    // reading NZ, and NZ is not carried through the permutation
+    bool nz_keyed = false;
     // step temporaries in slot order
     float4 *XN = nullptr, *NN = nullptr; // next positions and noise
     float4* fgrid = nullptr;             // grid force per slot (x, y, z, 0)
@@ -278,6 +283,7 @@ struct TildEngine {
     size_t dens_lds = 0;
     uint32_t tk_mask = ~0u;      // kernel groups bracketed by timing events
     int gather_mode = 1;    // 0 direct, 1 LDS windows (mft_tild_autotune picks)
+    int plane_passes = 0;   // 1: x and y transforms fused per z-plane (mft_tild_autotune picks)
     int64_t step = 0;
     int64_t run_max = 0, run_pos = 0; // declared run (mft_tild_set_run)
     int have_state = 0;
@@ -316,6 +322,10 @@ int tild_axis(int sign, int real_in, int real_out, const void* in, void* out, Ax
 int tild_x_fwd_tiles(const float* tiles, const int* tab, float2* S, int nx, int ny, int nz, int nchan, long cstride,
                      hipStream_t st);
 int tild_x_inv_packed(const float2* Y, float4* F4, int nx, int ny, int nz, int nfield, hipStream_t st);
+bool tild_plane_supported(int nx, int ny);
+int tild_xy_fwd_tiles_plane(const float* tiles, const int* tab, float2* S, int nx, int ny, int nz, int nchan,
+                            long cstride, hipStream_t st);
+int tild_yx_inv_packed_plane(const float2* Y, float4* F4, int nx, int ny, int nz, int nfield, hipStream_t st);
 int tild_conv_blocks(int nx, int ny, int nz);
 int tild_conv(TildEngine* e);
 

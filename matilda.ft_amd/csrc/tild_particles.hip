@@ -344,7 +344,7 @@ __global__ void k_permute(int ns, int maxbonds, const int* __restrict__ key, con
     const float4 xc = X[s], xn = XN[s];
     X2[n] = make_float4(xn.x, xn.y, xn.z, xc.w);
     XO2[n] = make_float4(xc.x, xc.y, xc.z, 0.f);
-    NZ2[n] = NN[s];
+    if (NN) NZ2[n] = NN[s];
     META2[n] = META[s];
     NB2[n] = NB[s];
     for (int b = 0; b < maxbonds; ++b) BT2[n * maxbonds + b] = BT[s * maxbonds + b];
@@ -890,6 +890,8 @@ struct StepArgs {
     int use_v;                 // the noise slot holds velocities (VV)
     const float4* FD;          // DPD pair forces per slot (null without)
     LangevinParams lang;
+    int nz_keyed;              // GJF: previous noise regenerated from (particle, step - 1), NZ not read
+    int write_nn;              // store the new noise in NN (off under GJF: the next step regenerates it)
 };
 
 // Wall forces of ps_potentialWall.cu:92-108 on a particle of species ty
@@ -930,8 +932,20 @@ __device__ __forceinline__ int integrate_slot(int s, float4 xc, int2 meta, const
     if (A.integ == MFT_INT_GJF) {
         const int ty = meta.y;
         const float a = A.gp.a[ty], b = A.gp.b[ty], nm = A.gp.nmag[ty], cf = A.gp.cf[ty], cn = A.gp.cn[ty];
-        const float4 xo4 = A.XO[s], no4 = A.NZ[s];
-        const float xov[3] = {xo4.x, xo4.y, xo4.z}, nov[3] = {no4.x, no4.y, no4.z};
+        const float4 xo4 = A.XO[s];
+        float nov[3];
+        if (A.nz_keyed) {
+            // the previous step's nm * normal, the same bits as when it was drawn
+            float zo[4];
+            normal4((uint32_t)meta.x, A.step - 1u, TAG_TILD, 0u, A.seed, zo);
+            for (int j = 0; j < 3; ++j) nov[j] = nm * zo[j];
+        } else {
+            const float4 no4 = A.NZ[s];
+            nov[0] = no4.x;
+            nov[1] = no4.y;
+            nov[2] = no4.z;
+        }
+        const float xov[3] = {xo4.x, xo4.y, xo4.z};
         for (int j = 0; j < 3; ++j) {
             const float nnj = nm * z[j];
             float xo = xov[j];
@@ -978,7 +992,7 @@ __device__ __forceinline__ int integrate_slot(int s, float4 xc, int2 meta, const
         }
     }
     A.XN[s] = make_float4(xn[0], xn[1], xn[2], 0.f);
-    A.NN[s] = make_float4(nn[0], nn[1], nn[2], 0.f);
+    if (A.write_nn) A.NN[s] = make_float4(nn[0], nn[1], nn[2], 0.f);
     return key_of(xn[0], xn[1], xn[2], meta.y, A.dx[0], A.dx[1], A.dx[2], A.p, A.nx, A.ny, A.nz, tg);
 }
 
@@ -1168,6 +1182,7 @@ int tild_sort_state(TildEngine* e)
 {
     const mft_tild_desc& d = e->d;
     const int N = e->N, mb = std::max(1, d.maxbonds), c = e->cur;
+    e->nz_keyed = false; // NZ holds the given previous noise
     MFT_HIP(hipMemsetAsync(e->count, 0, sizeof(int) * (size_t)e->nkeys, e->stream));
     k_sort_count<<<nblk(N), kBlock, 0, e->stream>>>(e->pio, e->type, N, e->count, e->key, e->rank, e->dx[0],
                                                      e->dx[1], e->dx[2], d.pmeorder, d.nx, d.ny, d.nz, e->tg);
@@ -1185,11 +1200,33 @@ int tild_sort_state(TildEngine* e)
     return MFT_OK;
 }
 
+// previous GJF noise in particle-id order, regenerated from (particle,
+// step - 1): out[3 id + j] = nmag(type) * normal_j
+__global__ void k_unsort_keyed_noise(float* __restrict__ out, const int2* __restrict__ META, GjfParams gp, int ns,
+                                     uint64_t seed, uint32_t prev_step)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const int2 m = META[s];
+    float z[4];
+    normal4((uint32_t)m.x, prev_step, TAG_TILD, 0u, seed, z);
+    const float nm = gp.nmag[m.y];
+    out[m.x * 3 + 0] = nm * z[0];
+    out[m.x * 3 + 1] = nm * z[1];
+    out[m.x * 3 + 2] = nm * z[2];
+}
+
 // key order -> particle-id order into pio: what 0 positions, 1 previous
 // positions, 2 previous noise, 3 total force of the last evaluation
 int tild_unsort(TildEngine* e, int what)
 {
     const int c = e->cur;
+    if (what == 2 && e->nz_keyed) {
+        k_unsort_keyed_noise<<<nblk(e->N), kBlock, 0, e->stream>>>(e->pio, e->META[c], e->gp, e->N, e->d.seed,
+                                                                   (uint32_t)e->step - 1u);
+        MFT_LAUNCH_CHECK();
+        return MFT_OK;
+    }
     const float4* src = what == 0 ? e->X[c] : what == 1 ? e->XO[c] : what == 2 ? e->NZ[c] : e->ftot;
     k_unsort<<<nblk(e->N), kBlock, 0, e->stream>>>(e->pio, src, e->META[c], e->N);
     MFT_LAUNCH_CHECK();
@@ -1208,7 +1245,9 @@ int tild_scan(TildEngine* e)
 int tild_permute(TildEngine* e)
 {
     const int N = e->N, mb = std::max(1, e->d.maxbonds), c = e->cur, n = 1 - c;
-    k_permute<<<nblk(N), kBlock, 0, e->stream>>>(N, mb, e->key, e->rank, e->start, e->map, e->XN, e->NN, e->X[c],
+    // under GJF the next step regenerates the previous noise: NZ stays behind
+    const float4* nn = e->d.integrator == MFT_INT_GJF ? nullptr : e->NN;
+    k_permute<<<nblk(N), kBlock, 0, e->stream>>>(N, mb, e->key, e->rank, e->start, e->map, e->XN, nn, e->X[c],
                                                   e->META[c], e->NB[c], e->BT[c], e->X[n], e->XO[n], e->NZ[n],
                                                   e->META[n], e->NB[n], e->BT[n]);
     k_remap_bonds<<<nblk(N), kBlock, 0, e->stream>>>(N, mb, e->map, e->NB[c], e->BS[c], e->BS[n]);
@@ -1340,6 +1379,8 @@ static StepArgs step_args(const TildEngine* e)
     A.lang = e->lang;
     A.use_v = e->is_vv ? 1 : 0;
     A.FD = e->dpd.on ? e->fdpd : nullptr;
+    A.nz_keyed = (d.integrator == MFT_INT_GJF && e->nz_keyed) ? 1 : 0;
+    A.write_nn = d.integrator == MFT_INT_GJF ? 0 : 1;
     return A;
 }
 

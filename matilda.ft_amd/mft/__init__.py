@@ -449,6 +449,9 @@ class TildEngine:
     def set_gather(self, variant):
         check(lib().mft_tild_set_gather(self.h, int(variant)))
 
+    def set_grid_passes(self, variant):
+        check(lib().mft_tild_set_grid_passes(self.h, int(variant)))
+
     def autotune(self, reps=5):
         c = _i(0)
         check(lib().mft_tild_autotune(self.h, reps, C.byref(c)))

@@ -621,3 +621,64 @@ def test_dpd_matches_oracle(gpu, integrator):
         x_g, _, _ = eng.get_state()
         assert np.max(periodic_dist(x_g, o["x"], cfg["L"])) < 1e-6 * max(cfg["L"]), step
     eng.close()
+
+
+@pytest.mark.parametrize("grid, nchains, nsolvent, box", [(64, 300, 3000, 14.0), (96, 400, 4000, 16.0),
+                                                         (128, 500, 5000, 18.0)])
+def test_plane_passes_equal_line_passes(gpu, grid, nchains, nsolvent, box):
+    """The x-y transforms of each z-plane in one workgroup run the arithmetic
+    of one pass per axis (same plans, twiddles and tile sums): the densities
+    are the same bits, the forces agree to float rounding (the compiler may
+    contract different multiply-adds in the two kernels; at 64 and 128 points
+    they are the same bits) and both match the oracle's forces; so do three
+    GJF steps (whose previous noise is regenerated from Philox after the
+    first)."""
+    say("case: plane passes, grid", grid)
+    cfg = systems.tild_coacervate(nchains=nchains, Nchain=10, nsolvent=nsolvent, box=box, grid=grid, seed=23)
+    engs = []
+    for variant in (0, 1):
+        eng = systems.tild_engine(cfg, "GJF")
+        eng.set_grid_passes(variant)
+        engs.append(eng)
+    f0, f1 = engs[0].compute_forces(), engs[1].compute_forces()
+    r0, r1 = engs[0].densities()[0], engs[1].densities()[0]
+    assert np.array_equal(r0, r1)
+    assert rel(f1, f0) < 1e-6
+    if grid in (64, 128):
+        assert np.array_equal(f0, f1)
+    sys_o = oracle_system(cfg, "GJF")
+    assert rel(f1, orc.tild_forces(sys_o)["f"]) < TOL
+    for eng in engs:
+        eng.step(3)
+    s0, s1 = engs[0].get_state(), engs[1].get_state()
+    assert np.max(periodic_dist(s0[0], s1[0], cfg["L"])) < 1e-6 * max(cfg["L"])
+    assert rel(s1[2], s0[2]) == 0.0          # regenerated noise: the same normals
+    for eng in engs:
+        eng.close()
+
+
+def test_gjf_previous_noise_is_regenerated_bit_for_bit(gpu):
+    """After GJF steps the engine regenerates the previous noise from
+    (particle, step - 1) instead of carrying it: get_state's noise_old and
+    every position equal the oracle's trajectory that carries it."""
+    say("case: gjf regenerated noise")
+    cfg = small_system(True, seed=31)
+    eng, sys_o = _engine_vs_oracle(gpu, cfg)
+    ns = cfg["topo"]["type"].size
+    for step in range(4):
+        eng.step(1)
+        o = orc.tild_step(sys_o, orc.tild_noise(ns, cfg["seed"], step))
+        sys_o.x, sys_o.xo = o["x"].astype(np.float32), o["xo"].astype(np.float32)
+        sys_o.noise_old = o["noise_old"].astype(np.float32)
+        x, xo, no = eng.get_state()
+        assert np.max(periodic_dist(x, o["x"], cfg["L"])) < 1e-6 * max(cfg["L"]), step
+        assert rel(no, o["noise_old"]) < 1e-6, step
+    # a new state restarts from the given previous noise (the oracle from the
+    # same state, so one step's error is compared)
+    x, xo, no = eng.get_state()
+    eng.set_state(x, xo, no)
+    sys_o.x, sys_o.xo, sys_o.noise_old = x.copy(), xo.copy(), no.copy()
+    eng.step(1)
+    o = orc.tild_step(sys_o, orc.tild_noise(ns, cfg["seed"], 4))
+    assert np.max(periodic_dist(eng.get_state()[0], o["x"], cfg["L"])) < 1e-6 * max(cfg["L"])
+    eng.close()
