@@ -61,7 +61,7 @@ def cpu_baseline(seed):
     legs = []
 
     cfg = systems.tild_coacervate(nchains=5000, nsolvent=25000, box=30.0, grid=64, seed=seed)
-    spec, fields = systems.fts_diblock()
+    spec, fields = systems.fts_diblock(dt=0.001)
     f64 = {k: np.asarray(v, np.complex64).astype(np.complex128) for k, v in fields.items()}
     for bits in (64, 32):
         for threads in (1, cores):
@@ -108,29 +108,39 @@ def fts_bytes(M, Ns):
     return M * 8.0 * (8 * Ns + 12)
 
 
-def fts_measure(iters_plain=20, iters_batch=5, nrep=256, seed=7):
-    """Plain CL-FTS (64^3, Ns = 100, 2000-iteration solves) and the batched
-    sweep of nrep uneven replicas (1000-iteration solves) against the same
-    replicas run one at a time."""
+def fts_measure(iters_plain=2000, iters_probe=20, iters_batch=1000, nrep=256, seed=7):
+    """Plain CL-FTS: one whole 64^3, Ns = 100 solve (init + 2000 iterations,
+    CL step 0.001: at 0.01 the explicit scheme runs away within 200
+    iterations, in the FP64 oracle as well).  Batched sweep: all nrep uneven
+    replicas through their 1000 iterations (CL step from the grid spacing,
+    mft.systems.cl_step), against the same replicas run one at a time on the
+    plain path (one replica per (grid, Ns) class timed over iters_probe
+    iterations).  Only solves that end finite count."""
     import numpy as np
     import mft
     from mft import systems
 
     out = {}
-    spec, fields = systems.fts_diblock()
-    s = systems.fts_system(spec, fields)
-    s.step(2)
-    s.synchronize()
+    spec, fields = systems.fts_diblock(dt=0.001)
+    M = int(np.prod(spec["Nx"]))
+    mft.synchronize()
     t0 = time.perf_counter()
+    s = systems.fts_system(spec, fields)
     s.step(iters_plain)
     s.synchronize()
-    t_it = (time.perf_counter() - t0) / iters_plain
-    Q, H, _ = s.observables()
+    t_solve = time.perf_counter() - t0
+    ok = True
+    try:
+        Q, H, _ = s.observables()
+        ok = bool(np.isfinite(H) and np.all(np.isfinite(Q)))
+    except mft.MftError:
+        ok = False
     s.close()
-    M = int(np.prod(spec["Nx"]))
-    out["plain_64cubed_Ns100"] = {"ms_per_iteration": 1e3 * t_it, "solves_per_hour": 3600.0 / (2000 * t_it),
-                                  "hbm_frac": fts_bytes(M, 100) / t_it / 1e9 / HBM_PEAK_GBS,
-                                  "H_finite": bool(np.isfinite(H))}
+    t_it = t_solve / iters_plain
+    out["plain_64cubed_Ns100"] = {"iterations": iters_plain, "dt": 0.001, "seconds_per_solve": t_solve,
+                                  "ms_per_iteration": 1e3 * t_it, "completed": ok,
+                                  "solves_per_hour": 3600.0 / t_solve if ok else 0.0,
+                                  "hbm_frac": fts_bytes(M, 100) / t_it / 1e9 / HBM_PEAK_GBS}
 
     reps = systems.fts_sweep_replicas(nrep, seed=seed)
     # one replica alone per (grid, Ns) class, timed on the plain path
@@ -143,33 +153,42 @@ def fts_measure(iters_plain=20, iters_batch=5, nrep=256, seed=7):
         a.step(2)
         a.synchronize()
         t0 = time.perf_counter()
-        a.step(iters_plain)
+        a.step(iters_probe)
         a.synchronize()
-        alone[key] = (time.perf_counter() - t0) / iters_plain
+        alone[key] = (time.perf_counter() - t0) / iters_probe
         a.close()
     t_alone_total = sum(alone[(sp["Nx"][0], sum(n for n, _ in sp["molecules"][0]["blocks"]))] for sp, _ in reps)
 
     b = mft.FtsBatch([r[0] for r in reps])
     for k, (spec_r, fields_r) in enumerate(reps):
         systems.load_fields(b.set_fields, spec_r, fields_r, rep=k)
-    b.init()
-    b.step(1)
     b.synchronize()
     t0 = time.perf_counter()
-    b.step(iters_batch)
-    b.synchronize()
+    b.init()
+    done = 0
+    while done < iters_batch:
+        n = min(100, iters_batch - done)
+        b.step(n)
+        b.synchronize()
+        done += n
+        print(json.dumps({"sweep_iterations": done, "seconds": time.perf_counter() - t0}), file=sys.stderr, flush=True)
     t_batch = (time.perf_counter() - t0) / iters_batch
-    finite = all(np.isfinite(b.observables(k)[1]) for k in range(0, nrep, 17))
+    completed = 0
+    for k in range(nrep):
+        try:
+            Q, H, _ = b.observables(k)
+            completed += bool(np.isfinite(H) and np.all(np.isfinite(Q)))
+        except mft.MftError:
+            pass
     b.close()
     bytes_it = sum(fts_bytes(int(np.prod(sp["Nx"])), sum(n for n, _ in sp["molecules"][0]["blocks"])) for sp, _ in reps)
     out["batched_sweep"] = {
-        "replicas": nrep, "iterations_per_solve": 1000,
+        "replicas": nrep, "iterations_per_solve": iters_batch, "completed": completed,
         "ms_per_iteration_all_replicas": 1e3 * t_batch,
-        "solves_per_hour": nrep * 3600.0 / (1000 * t_batch),
-        "plain_one_at_a_time_solves_per_hour": nrep * 3600.0 / (1000 * t_alone_total),
+        "solves_per_hour": completed * 3600.0 / (iters_batch * t_batch),
+        "plain_one_at_a_time_solves_per_hour": completed * 3600.0 / (iters_batch * t_alone_total),
         "speedup_vs_plain": t_alone_total / t_batch,
         "hbm_frac": bytes_it / t_batch / 1e9 / HBM_PEAK_GBS,
-        "H_finite_sampled": bool(finite),
     }
     return out
 
@@ -180,7 +199,7 @@ def fts_profile(which, iters=20):
     import mft
     from mft import systems
     if which == "plain":
-        spec, fields = systems.fts_diblock()
+        spec, fields = systems.fts_diblock(dt=0.001)
         s = systems.fts_system(spec, fields)
     else:
         reps = systems.fts_sweep_replicas(256, seed=7)
@@ -202,7 +221,7 @@ def fts_profile(which, iters=20):
 def serve_measure(seed=11, window=1.0, rate=50.0, tild_frac=0.2, max_iters=25, budget_frac=0.7, max_pool=16):
     """Served mixed workload: a Poisson stream (rate per s) of CL-FTS diblock
     solves (grids 32^3-96^3, chi N in [8, 25], 500-5000 iterations, Ns = 50,
-    dt = 0.005;
+    CL step from the grid spacing;
     at most max_pool solves share the pool at a time)
     and TILD diblock runs (5k-500k particles at rho0 = 5, grids 32^3-128^3,
     200 steps) arriving over `window` seconds, served with continuous batching
@@ -226,12 +245,15 @@ def serve_measure(seed=11, window=1.0, rate=50.0, tild_frac=0.2, max_iters=25, b
             reqs.append((t, "tild", cfg, 200))
         else:
             g = int(rng.choice([32, 48, 64, 96]))
-            # box 0.10-0.16 Rg per grid point; explicit CL step dt = 0.005 (at
-            # chi N above ~19 a CL trajectory can still run away: such a
-            # request ends as failed, with the engine's message)
-            spec, fields = systems.fts_diblock(grid_shape=(g, g, g), L=float(g * rng.uniform(0.10, 0.16)), Nr=50.0, NA=25,
+            # box 0.10-0.16 Rg per grid point; explicit CL step from the grid
+            # spacing with a cap of 0.00025 (mft.systems.cl_step): with a cap
+            # of 0.001, 3 of these 39 solves (96^3, chi N > 20) run away
+            # after 1250-2250 iterations
+            h = float(rng.uniform(0.10, 0.16))
+            spec, fields = systems.fts_diblock(grid_shape=(g, g, g), L=float(g * h), Nr=50.0, NA=25,
                                                NB=25, chiN=float(rng.uniform(8, 25)), kappaN=100.0, C=50.0,
-                                               dt=0.005, sigma=0.1, seed=int(rng.integers(1 << 30)))
+                                               dt=systems.cl_step(h, dt_max=0.00025), sigma=0.1,
+                                               seed=int(rng.integers(1 << 30)))
             reqs.append((t, "fts", (spec, fields), int(rng.uniform(500, 5000))))
     free = mft.mem_info()[0]
     srv = mft.Server(hbm_budget_bytes=budget_frac * free, max_fts_replicas=max_pool)

@@ -264,6 +264,11 @@ int mft_tild_synchronize(mft_tild* eng);
  * and bond energy (x 1/2, PS_Box.cu:292, with the reference's 0.5*Lh minimum
  * image of ps_deviceBonds.cu:132-133). */
 int mft_tild_energies(mft_tild* eng, double* pot_energy, double* bond_energy);
+/* Virial part of d_bondStressEnergy (ps_deviceBonds.cu:85-171) at the current
+ * state: vir[6N] (host, particle-id order, the layout of the reference's
+ * d_bondVirScratch), per particle the terms -mf dr_a dr_b over its bonds as
+ * xx, yy, zz, xy, xz, yz, with the reference's 0.5*Lh minimum image. */
+int mft_tild_bond_virial(mft_tild* eng, float* vir);
 /* nsteps steps (nsteps >= 1) that also return the energies PS_Box::writeData
  * logs after the last of them (PS_Box.cu:152-173, 282-317): pair potentials
  * from that step's own density pass (the positions before the integrator

@@ -1235,8 +1235,10 @@ template <int N>
 static void launch_prop_xy_n(const PropDesc* dx, const PropDesc* dy, int n, const FtsGroup& G, hipStream_t st)
 {
     dim3 grid(G.n[2], n);
-    // enough threads for every row (and column) of the plane in one round
-    constexpr int NT = N * FftPlan<N>::T > 256 ? 512 : 256;
+    // enough threads for every row (and column) of the plane in one round;
+    // 48-point planes take exactly 48 lines (192 threads), so no line needs
+    // the scratch region and more workgroups fit in LDS
+    constexpr int NT = N == 48 ? 192 : (N * FftPlan<N>::T > 256 ? 512 : 256);
     k_fts_prop_xy<N, NT><<<grid, NT, 0, st>>>(dx, dy, fft_twiddles_device(N));
 }
 

@@ -796,6 +796,26 @@ int mft_tild_synchronize(mft_tild* e)
     return MFT_OK;
 }
 
+int mft_tild_bond_virial(mft_tild* e, float* vir)
+{
+    MFT_REQUIRE(e && vir, MFT_ERR_INVALID, "null argument");
+    MFT_REQUIRE(e->have_state, MFT_ERR_INVALID, "engine has no state");
+    float* dv = nullptr;
+    int rc = dalloc(&dv, (size_t)6 * e->N);
+    if (rc) return rc;
+    rc = tild_bond_virial(e, dv);
+    if (!rc) {
+        hipError_t err = hipStreamSynchronize(e->stream);
+        if (err == hipSuccess) err = hipMemcpy(vir, dv, sizeof(float) * 6 * (size_t)e->N, hipMemcpyDeviceToHost);
+        if (err != hipSuccess) {
+            set_error(MFT_ERR_DEVICE, "bond virial readback: %s", hipGetErrorString(err));
+            rc = MFT_ERR_DEVICE;
+        }
+    }
+    (void)hipFree(dv);
+    return rc;
+}
+
 int mft_tild_energies(mft_tild* e, double* pot_energy, double* bond_energy)
 {
     MFT_REQUIRE(e && e->have_state, MFT_ERR_INVALID, "engine has no state");

@@ -30,7 +30,10 @@ __device__ __forceinline__ void conv_z_body(const float2* __restrict__ S, float2
     for (int i = threadIdx.x; i < N; i += 256) tw[i] = twg[i];
     const int lline = threadIdx.x % B;
     const int t = threadIdx.x / B;
-    const int line = blockIdx.x * B + lline;
+    // consecutive line groups on one XCD: a group's B lines are half of a
+    // 128-byte row of x, and the group beside it reads the other half, so both
+    // halves are fetched into the same L2 (brick_of_block's mapping)
+    const int line = brick_of_block(blockIdx.x, gridDim.x) * B + lline;
     const long M = (long)nx * ny * N;
     const bool valid = line < nx * ny;
     const int ix = line % nx, iy = line / nx;

@@ -122,15 +122,26 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd_tiles(const float* __restrict
     const int c = blockIdx.y;
     const int nlines = ny * nz;
     const float* tc = tiles + c * cstride;
-    for (int i = threadIdx.x; i < B * N; i += 256) {
+    // every point's tile loads in flight together (the loop unrolled), then
+    // the sums into LDS
+    constexpr int PER = (B * N + 255) / 256;
+    float sums[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int i = threadIdx.x + 256 * k;
         const int l = i / N, x = i - l * N;
         const int line = blockIdx.x * B + l;
         float sum = 0.f;
-        if (line < nlines) {
+        if (i < B * N && line < nlines) {
             const int gy = line % ny, gz = line / ny;
             sum = tile_sum(tc, tab + 5 * (N + ny + gz), tab + 5 * (N + gy), tabx + 5 * x);
         }
-        lds[i] = make_float2(sum, 0.f);
+        sums[k] = sum;
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int i = threadIdx.x + 256 * k;
+        if (i < B * N) lds[i] = make_float2(sums[k], 0.f);
     }
     __syncthreads();
     const int lline = threadIdx.x / T;

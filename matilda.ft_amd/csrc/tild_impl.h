@@ -342,6 +342,7 @@ int tild_integrate(TildEngine* e, bool want_f);
 void erf_table(int kind, double sigma, double Rp, int nx, int ny, int nz, float lx, float ly, float lz,
                std::vector<float>& out);
 int tild_bond_energy(TildEngine* e);
+int tild_bond_virial(TildEngine* e, float* vir);   // [6N] device, particle-id order
 int tild_bias_setup(TildEngine* e, int pot);      // the field and its force (tild_bias.hip)
 int tild_bias_add(TildEngine* e);                 // F4 += bias forces of the biased species
 int tild_bias_energy(TildEngine* e, int k, double* energy);

@@ -1047,6 +1047,62 @@ __global__ void __launch_bounds__(256) k_integrate(
     }
 }
 
+// d_bondStressEnergy virial part (ps_deviceBonds.cu:85-171): per particle the
+// six terms -mf dr_a dr_b (xx, yy, zz, xy, xz, yz) summed over its bonds in
+// float, written in particle-id order (the layout of the reference's
+// d_bondVirScratch); dr with the 0.5*Lh minimum image (:132-133); no term for
+// r^2 <= 1e-4 or for FENE at or beyond r0, where the reference leaves mf unset.
+__global__ void k_bond_virial_slots(float* __restrict__ vir, const float4* __restrict__ X,
+                                    const int2* __restrict__ META, const int* __restrict__ NB,
+                                    const int* __restrict__ BS, const int* __restrict__ BT, BondParams bp, float L0,
+                                    float L1, float L2, int ns, int maxbonds)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const float Ls[3] = {L0, L1, L2};
+    const float4 x = X[s];
+    const float xv[3] = {x.x, x.y, x.z};
+    float v[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < NB[s]; ++b) {
+        const int s2 = BS[s * maxbonds + b];
+        const int bt = BT[s * maxbonds + b];
+        const float4 y = X[s2];
+        const float yv[3] = {y.x, y.y, y.z};
+        float d[3], r2 = 0.f;
+        for (int j = 0; j < 3; ++j) {
+            d[j] = min_image(xv[j] - yv[j], Ls[j], 0.25f * Ls[j]);
+            r2 += d[j] * d[j];
+        }
+        float mf = 0.f;
+        if (r2 > 1.0e-4f) {
+            const float r = sqrtf(r2);
+            if (bp.style[bt] == 0) mf = 2.0f * bp.k[bt] * (r - bp.r0[bt]) / r;
+            else if (bp.style[bt] == 1 && r / bp.r0[bt] < 1.f) {
+                const float u = r / bp.r0[bt];
+                mf = 2.0f * bp.k[bt] * (1.f / (1.f - u * u));
+            }
+        }
+        v[0] += -mf * d[0] * d[0];
+        v[1] += -mf * d[1] * d[1];
+        v[2] += -mf * d[2] * d[2];
+        v[3] += -mf * d[0] * d[1];
+        v[4] += -mf * d[0] * d[2];
+        v[5] += -mf * d[1] * d[2];
+    }
+    const long id = META[s].x;
+    for (int c = 0; c < 6; ++c) vir[id * 6 + c] = v[c];
+}
+
+int tild_bond_virial(TildEngine* e, float* vir)
+{
+    const int c = e->cur;
+    k_bond_virial_slots<<<nblk(e->N), kBlock, 0, e->stream>>>(vir, e->X[c], e->META[c], e->NB[c], e->BS[c],
+                                                               e->BT[c], e->bp, e->d.lx, e->d.ly, e->d.lz, e->N,
+                                                               std::max(1, e->d.maxbonds));
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
 // d_bondStressEnergy energy part (ps_deviceBonds.cu:85-171) over slots, FP64
 // block sums, with the reference's 0.5*Lh minimum image (:132-133).
 __global__ void k_bond_energy_slots(double* __restrict__ partial, const float4* __restrict__ X,

@@ -446,6 +446,12 @@ class TildEngine:
         mask = 0xFFFFFFFF if names is None else sum(1 << TK_NAMES.index(n) for n in names)
         check(lib().mft_tild_timing_select(self.h, C.c_uint32(mask)))
 
+    def bond_virial(self):
+        """mft_tild_bond_virial: [N, 6] per-particle bond virial terms."""
+        v = np.empty(6 * self.N, np.float32)
+        check(lib().mft_tild_bond_virial(self.h, _ptr(v)))
+        return v.reshape(-1, 6)
+
     def set_gather(self, variant):
         check(lib().mft_tild_set_gather(self.h, int(variant)))
 

@@ -188,10 +188,22 @@ def fts_diblock(grid=64, L=8.0, Nr=100.0, NA=50, NB=50, chiN=20.0, kappaN=100.0,
     return spec, fields
 
 
-def fts_sweep_replicas(n=256, seed=7, iterations=1000):
+def cl_step(spacing, dt_max=0.001, h_ref=0.125):
+    """Explicit Euler-Maruyama CL step for a grid spacing h (in Rg):
+    dt_max (h / h_ref)^3, at most dt_max.  At C = 50 the explicit step runs
+    away (in the FP64 oracle as in the engine) once dt exceeds what the
+    finest grid modes tolerate: the 64^3, 8 Rg diblock at dt = 0.01 does so
+    within 200 iterations and stays finite over 2000 at dt = 0.001; the
+    noise variance per site, 2 dt / gvol, grows as h^-3, so the step shrinks
+    with h^3 below h_ref."""
+    return min(dt_max, dt_max * (spacing / h_ref) ** 3)
+
+
+def fts_sweep_replicas(n=256, seed=7, iterations=1000, dt=None):
     """Batched sweep (config 4 shape): grids from {24,32,48,64}^3, box
     4-10 Rg, chiN in [10,20], f_A in [0.3,0.5], Ns in {50,100}, kappaN = 100,
-    C = 50; one seed per replica."""
+    C = 50; one seed per replica; CL step dt, or cl_step(L / grid) per
+    replica when dt is None."""
     rng = np.random.default_rng(seed)
     out = []
     for r in range(n):
@@ -201,7 +213,9 @@ def fts_sweep_replicas(n=256, seed=7, iterations=1000):
         fA = float(rng.uniform(0.3, 0.5))
         Ns = int(rng.choice([50, 100]))
         NA = int(round(fA * Ns))
-        spec, fields = fts_diblock(grid=g, L=Lr, Nr=float(Ns), NA=NA, NB=Ns - NA, chiN=chiN, seed=seed * 100003 + r)
+        dtr = cl_step(Lr / g) if dt is None else dt
+        spec, fields = fts_diblock(grid=g, L=Lr, Nr=float(Ns), NA=NA, NB=Ns - NA, chiN=chiN, dt=dtr,
+                                   seed=seed * 100003 + r)
         out.append((spec, fields))
     return out
 

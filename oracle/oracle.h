@@ -91,6 +91,11 @@ void orc_bond_energy(real* e, const int* nBonds, const int* bondedTo,
                      const int* bondType, const float* bondReq, const float* bondK,
                      const int* bondStyle, const float* x, const float* L,
                      const float* Lh, int ns, int maxbonds, int Dim);
+/* ps_deviceBonds.cu:85-171, virial terms (xx, yy, zz, xy, xz, yz) per particle */
+void orc_bond_virial(real* vir, const int* nBonds, const int* bondedTo,
+                     const int* bondType, const float* bondReq, const float* bondK,
+                     const int* bondStyle, const float* x, const float* L,
+                     const float* Lh, int ns, int maxbonds, int Dim);
 /* GJF::finishInitialization coefficients (ps_integratorGJF.cu:56-82). */
 void orc_gjf_coeffs(double delt, double mass, double mobility, double out[5]);
 /* d_GJF_integrator (ps_integratorGJF.cu:102-163).  coef[type*5 + {a,b,noiseMag,

@@ -70,6 +70,7 @@ def lib(bits: int = 64):
         "orc_map_forces": (None, [_p, _p, _p, _p, _i, _p, _p, _i, _d, _i]),
         "orc_bonds": (None, [_p] * 10 + [_i, _i, _i]),
         "orc_bond_energy": (None, [_p] * 10 + [_i, _i, _i]),
+        "orc_bond_virial": (None, [_p] * 10 + [_i, _i, _i]),
         "orc_gjf_coeffs": (None, [_d, _d, _d, _p]),
         "orc_gjf_step": (None, [_p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _p]),
         "orc_em_step": (None, [_p, _p, _d, _p, _p, _i, _i, _p]),
@@ -293,6 +294,19 @@ def bond_energy(topo, x, L, bits=64):
                               _ptr(_fa(topo["bondReq"])), _ptr(_fa(topo["bondK"])), _ptr(_ia(topo["bondStyle"])),
                               _ptr(_fa(x)), _ptr(Lf), _ptr(Lh), ns, topo["maxbonds"], len(L))
     return e
+
+
+def bond_virial(topo, x, L, bits=64):
+    """Per-particle bond virial terms of d_bondStressEnergy
+    (ps_deviceBonds.cu:85-171): [ns, 6] as xx, yy, zz, xy, xz, yz."""
+    Lf = _fa(L)
+    Lh = _fa(0.5 * Lf)
+    ns = topo["nBonds"].size
+    v = np.zeros(ns * 6, _rdt(bits))
+    lib(bits).orc_bond_virial(_ptr(v), _ptr(topo["nBonds"]), _ptr(topo["bondedTo"]), _ptr(topo["bondType"]),
+                              _ptr(_fa(topo["bondReq"])), _ptr(_fa(topo["bondK"])), _ptr(_ia(topo["bondStyle"])),
+                              _ptr(_fa(x)), _ptr(Lf), _ptr(Lh), ns, topo["maxbonds"], len(L))
+    return v.reshape(ns, 6)
 
 
 def gjf_coeffs(delt, mass, mobility):

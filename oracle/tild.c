@@ -377,6 +377,45 @@ void orc_bond_energy(real* e, const int* nBonds, const int* bondedTo,
     }
 }
 
+/* d_bondStressEnergy virial part (ps_deviceBonds.cu:85-171): per particle
+ * the six pressure tensor terms -mf dr_a dr_b (xx, yy, zz, xy, xz, yz) summed
+ * over its bonds, dr with the 0.5*Lh minimum image (:132-133); mf = 2k(r-r0)/r
+ * (harmonic), 2k/(1-(r/r0)^2) (FENE below r0), 0 for r^2 <= 1e-4 and for
+ * FENE at or beyond r0 (where the reference leaves mf unset). */
+void orc_bond_virial(real* vir, const int* nBonds, const int* bondedTo,
+                     const int* bondType, const float* bondReq, const float* bondK,
+                     const int* bondStyle, const float* x, const float* L,
+                     const float* Lh, int ns, int maxbonds, int Dim)
+{
+    for (int i = 0; i < ns; ++i) {
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        for (int b = 0; b < nBonds[i]; ++b) {
+            int id2 = bondedTo[i * maxbonds + b];
+            int bt = bondType[i * maxbonds + b];
+            double dr[3] = {0, 0, 0}, mdr2 = 0.0;
+            for (int j = 0; j < Dim; ++j) {
+                dr[j] = min_image((double)x[i * Dim + j] - (double)x[id2 * Dim + j], L[j],
+                                  0.5 * (double)Lh[j]);
+                mdr2 += dr[j] * dr[j];
+            }
+            double mf = 0.0;
+            if (mdr2 > 1.0e-4) {
+                double mdr = sqrt(mdr2);
+                double k = bondK[bt], r0 = bondReq[bt];
+                if (bondStyle[bt] == 0) mf = 2.0 * k * (mdr - r0) / mdr;
+                else if (bondStyle[bt] == 1 && mdr / r0 < 1.0) mf = 2.0 * k / (1.0 - (mdr / r0) * (mdr / r0));
+            }
+            acc[0] += -mf * dr[0] * dr[0];
+            acc[1] += -mf * dr[1] * dr[1];
+            acc[2] += -mf * dr[2] * dr[2];
+            acc[3] += -mf * dr[0] * dr[1];
+            acc[4] += -mf * dr[0] * dr[2];
+            acc[5] += -mf * dr[1] * dr[2];
+        }
+        for (int c = 0; c < 6; ++c) vir[i * 6 + c] = (real)acc[c];
+    }
+}
+
 void orc_gjf_coeffs(double delt, double m, double D, double out[5])
 {
     double b = m / (m + delt / (2.0 * D));

@@ -682,3 +682,25 @@ def test_gjf_previous_noise_is_regenerated_bit_for_bit(gpu):
     o = orc.tild_step(sys_o, orc.tild_noise(ns, cfg["seed"], 4))
     assert np.max(periodic_dist(eng.get_state()[0], o["x"], cfg["L"])) < 1e-6 * max(cfg["L"])
     eng.close()
+
+
+def test_bond_virial_matches_oracle(gpu):
+    """Per-particle bond virial terms of d_bondStressEnergy
+    (ps_deviceBonds.cu:85-171) after two GJF steps (particles re-keyed), in
+    particle-id order, against the oracle at the engine's positions; bonds
+    longer than L/4 wrap against 0.5*Lh as in the reference."""
+    cfg = small_system(True, seed=41)
+    say("case: bond virial")
+    eng, sys_o = _engine_vs_oracle(gpu, cfg)
+    eng.step(2)
+    x, _, _ = eng.get_state()
+    v = eng.bond_virial()
+    topo = sys_o.topo
+    vo = orc.bond_virial(dict(nBonds=topo["nBonds"], bondedTo=topo["bondedTo"], bondType=topo["bondType"],
+                              bondReq=cfg["bond_r0"], bondK=cfg["bond_k"], bondStyle=cfg["bond_style"], maxbonds=2),
+                         x, cfg["L"])
+    assert v.shape == vo.shape
+    assert rel(v, vo) < TOL
+    # summed over particles: twice the pair virial, trace = -sum mf r^2 < 0 for stretched harmonic bonds
+    assert np.sum(v[:, :3], dtype=np.float64) == pytest.approx(np.sum(vo[:, :3]), rel=TOL)
+    eng.close()

@@ -174,6 +174,37 @@ def test_bond_force_and_energy_analytic():
     assert e2[0] == pytest.approx(1.5 * (7.0 - 0.5) ** 2, rel=1e-6)
 
 
+def test_bond_virial_closed_form():
+    # d_bondStressEnergy's virial terms (ps_deviceBonds.cu:85-171):
+    # -mf dr_a dr_b with mf = 2k(r - r0)/r, i.e. for a harmonic bond the tensor
+    # -2k(r - r0) r n n^T (n the unit bond vector), the same for both members;
+    # its trace is -2k(r - r0) r, the pair's contribution to -sum r.f
+    L = (10.0, 10.0, 10.0)
+    d = np.array([0.6, -0.8, 1.2])
+    x = np.array([[2.0, 3.0, 4.0], [2.0, 3.0, 4.0] + d], np.float32)
+    topo = dict(nBonds=np.array([1, 1], np.int32), bondedTo=np.array([1, 0, 0, 0], np.int32),
+                bondType=np.zeros(4, np.int32), bondReq=[0.5], bondK=[1.5], bondStyle=[0], maxbonds=2)
+    v = orc.bond_virial(topo, x.reshape(-1), L)
+    dr = (x[0] - x[1]).astype(np.float64)
+    r = np.linalg.norm(dr)
+    mf = 2 * 1.5 * (r - 0.5) / r
+    want = -mf * np.array([dr[0] ** 2, dr[1] ** 2, dr[2] ** 2, dr[0] * dr[1], dr[0] * dr[2], dr[1] * dr[2]])
+    np.testing.assert_allclose(v[0], want, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(v[1], want, rtol=1e-12, atol=1e-12)
+    assert v[0, :3].sum() == pytest.approx(-2 * 1.5 * (r - 0.5) * r, rel=1e-12)
+    # the trace is r_ij . f_i of the bond force the oracle applies to i
+    f = np.zeros(6)
+    orc.bonds(f, topo, x.reshape(-1), L)
+    assert v[0, :3].sum() == pytest.approx(float(dr @ f.reshape(2, 3)[0]), rel=1e-6)
+    # FENE below r0: mf = 2k / (1 - (r/r0)^2); at r >= r0 no term
+    topo_f = dict(topo, bondReq=[2.5], bondStyle=[1])
+    v = orc.bond_virial(topo_f, x.reshape(-1), L)
+    u = r / 2.5
+    np.testing.assert_allclose(v[0, 0], -2 * 1.5 / (1 - u * u) * dr[0] ** 2, rtol=1e-12)
+    topo_f = dict(topo, bondReq=[1.0], bondStyle=[1])
+    assert np.all(orc.bond_virial(topo_f, x.reshape(-1), L) == 0.0)
+
+
 def test_gjf_harmonic_configurational_temperature():
     # GJF samples <x^2> = kT/kappa exactly for a harmonic well at any stable dt
     # (Gronbech-Jensen & Farago 2013), here with dt = 0.2, kappa = 4
