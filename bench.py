@@ -193,6 +193,45 @@ def fts_measure(iters_plain=2000, iters_probe=20, iters_batch=1000, nrep=256, se
     return out
 
 
+def tild_c1_measure(steps=200, replicas=16, seed=1234):
+    """The plain TILD diblock melt (2000 chains of 10 A + 10 B, 40 000
+    particles, 20^3 box, 32^3 grid, GJF): stepped alone, as a one-replica
+    graph group (two steps per graph launch), and as a group of `replicas`
+    such systems (different seeds) advanced by one graph; particle-steps/s
+    over all replicas."""
+    import mft
+    from mft import systems
+    out = {}
+    cfgs = [systems.tild_diblock(seed=seed + r) for r in range(replicas)]
+    N = cfgs[0]["topo"]["type"].size
+
+    def timed(run, engines, nrep):
+        run(4)
+        for e in engines:
+            e.synchronize()
+        t0 = time.perf_counter()
+        run(steps)
+        for e in engines:
+            e.synchronize()
+        dt = time.perf_counter() - t0
+        return {"us_per_step": 1e6 * dt / steps, "particle_steps_per_s": N * nrep * steps / dt}
+
+    eng = systems.tild_engine(cfgs[0], "GJF")
+    out["alone"] = timed(eng.step, [eng], 1)
+    g1 = mft.TildGroup([eng])
+    out["graph_1"] = timed(g1.step, [eng], 1)
+    g1.close()
+    eng.close()
+    engs = [systems.tild_engine(c, "GJF") for c in cfgs]
+    g = mft.TildGroup(engs)
+    out[f"graph_{replicas}"] = timed(g.step, engs, replicas)
+    g.close()
+    for e in engs:
+        e.close()
+    out["particles_per_replica"] = N
+    return out
+
+
 def fts_profile(which, iters=20):
     """A few CL iterations of one FTS workload (2 untimed first), for a
     profiler: the plain 64^3, Ns = 100 solve or the 256-replica sweep."""
@@ -410,6 +449,7 @@ def main():
         "cpu_baseline": None,
     }
     eng.close()
+    out["tild_c1"] = tild_c1_measure()
     if not args.no_fts:
         out["fts"] = fts_measure()
     if not args.no_serve:

@@ -306,6 +306,19 @@ int mft_tild_set_gather(mft_tild* eng, int variant);
  * 1 both axes of a z-plane in one workgroup (square 64, 96 or 128 point
  * planes; MFT_ERR_UNSUPPORTED otherwise). */
 int mft_tild_set_grid_passes(mft_tild* eng, int variant);
+/* Replica groups (no counterpart in the reference, which steps one box):
+ * independent TILD engines advanced together by one HIP graph holding eight
+ * steps of every engine (each engine's kernels on its own stream, so the
+ * replicas overlap on the device; remainders take ordinary steps).  Each engine gives the same bits as when
+ * stepped alone.  Engines with potential ramps, DPD, Langevin potentials,
+ * velocity Verlet or timing enabled, and a GJF engine's first step after its
+ * state is set, take ordinary steps.  The engines stay owned by the caller;
+ * destroy the group before them. */
+typedef struct mft_tild_group mft_tild_group;
+int mft_tild_group_create(mft_tild_group** grp, mft_tild* const* engines, int n);
+int mft_tild_group_destroy(mft_tild_group* grp);
+int mft_tild_group_step(mft_tild_group* grp, int nsteps);
+int mft_tild_group_synchronize(mft_tild_group* grp);
 /* PS_ComputeSK::do_compute (ps_compute_sk.cu:18-45): sk[M] (device) = or +=
  * |FFT[rho_group]|^2 with the forward transform scaled by 1/M, rho_group the
  * sum of the densities of the species in species_mask at the last density

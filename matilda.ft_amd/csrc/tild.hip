@@ -234,6 +234,7 @@ static int tild_setup(TildEngine* e)
     const int nconv_blocks = tild_conv_blocks(d.nx, d.ny, d.nz);
     if ((rc = dalloc(&e->epart, (size_t)nconv_blocks * kMaxPots))) return rc;
     if ((rc = dalloc(&e->eout, (size_t)kMaxPots + 1))) return rc;
+    if ((rc = dalloc(&e->d_step, (size_t)1))) return rc;
     if ((rc = dalloc(&e->bpart, (size_t)nblk(N)))) return rc;
     MFT_HIP(hipMemset(e->nBonds, 0, sizeof(int) * (size_t)N));
     MFT_HIP(hipMemset(e->bondedTo, 0, sizeof(int) * (size_t)N * mb));
@@ -518,7 +519,7 @@ static int do_steps(TildEngine* e, int nsteps, bool want_f, bool advance = true,
             // the next positions into their key order
             {
                 TimedScope ts(e, MFT_TK_SCAN);
-                if ((rc = tild_scan(e))) return rc;
+                if ((rc = tild_scan(e, true))) return rc;
             }
             TimedScope ts(e, MFT_TK_SCATTER);
             if ((rc = tild_permute(e))) return rc;
@@ -575,7 +576,7 @@ int mft_tild_destroy(mft_tild* e)
                     e->pio2, e->pio3, e->count, e->start, e->partial, e->rho, e->S, e->Y, e->F4, e->force_slot,
                     e->epart, e->eout, e->bpart, e->tiles, e->tile_tab, e->NA[0], e->NA[1], e->AT[0], e->AT[1],
                     e->AG[0], e->AG[1], e->nAngles, e->angleType, e->angleGroup, e->dcount, e->dstart,
-                    e->dcursor, e->dcell, e->dslots, e->fdpd};
+                    e->dcursor, e->dcell, e->dslots, e->fdpd, e->d_step};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (float* t : e->tables)
@@ -1038,6 +1039,238 @@ int mft_tild_num_outputs(mft_tild* e, int* nout)
 {
     MFT_REQUIRE(e && nout, MFT_ERR_INVALID, "null argument");
     *nout = e->nout;
+    return MFT_OK;
+}
+
+} // extern "C"
+
+// ------------------------------------------------- replica groups ---
+// Independent TILD systems advanced by one HIP graph: kGroupSteps steps of
+// every engine are captured once, each engine's launches on its own stream
+// between a fork from and a join back to the group's stream, so one graph
+// launch runs those steps of all replicas with their kernels overlapping on
+// the device.  The integrator reads the step index from device memory and the
+// key scan advances it, so a replay keys its noise by the steps it runs; an
+// even step count brings every engine's double buffers back to the captured
+// parity.  Engines
+// whose configuration a graph cannot replay (ramps, DPD, Langevin, velocity
+// Verlet, timing) and GJF engines whose previous noise is not yet keyed (the
+// first step after a state is set) take ordinary steps.
+namespace mft {
+
+constexpr int kGroupSteps = 8;   // steps per graph replay (even)
+
+__global__ void k_set_step(uint32_t* __restrict__ d, uint32_t v) { *d = v; }
+
+static bool graph_ready(const TildEngine* e)
+{
+    const mft_tild_desc& d = e->d;
+    if (d.integrator != MFT_INT_GJF && d.integrator != MFT_INT_EM) return false;
+    if (e->dpd.on || e->lang.n || e->timing || e->nout == 0) return false;
+    for (int p = 0; p < d.npotentials; ++p)
+        if (e->pots[p].ramp) return false;
+    if (d.integrator == MFT_INT_GJF && !e->nz_keyed) return false;
+    return true;
+}
+
+} // namespace mft
+
+struct mft_tild_group {
+    std::vector<mft_tild*> eng;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<int> cur0, gather0, plane0;   // what the graph was captured with
+    std::vector<int64_t> dstep_at;            // host step the device counter holds (-1: stale)
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join, pre, post;
+};
+
+static void group_drop_graph(mft_tild_group* g)
+{
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    g->exec = nullptr;
+    g->graph = nullptr;
+}
+
+// capture kGroupSteps steps of every engine into one graph (nothing runs;
+// the host bookkeeping the capture advanced is restored)
+static int group_capture(mft_tild_group* g)
+{
+    group_drop_graph(g);
+    const size_t n = g->eng.size();
+    std::vector<int64_t> step(n), run_pos(n);
+    std::vector<int> cur(n);
+    std::vector<bool> rho_valid(n), keyed(n);
+    for (size_t i = 0; i < n; ++i) {
+        TildEngine* e = g->eng[i];
+        step[i] = e->step;
+        run_pos[i] = e->run_pos;
+        cur[i] = e->cur;
+        rho_valid[i] = e->rho_valid;
+        keyed[i] = e->nz_keyed;
+        e->use_dstep = true;
+    }
+    int rc = MFT_OK;
+    hipError_t err = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal);
+    if (err == hipSuccess) err = hipEventRecord(g->fork, g->stream);
+    for (size_t i = 0; i < n && err == hipSuccess && rc == MFT_OK; ++i) {
+        TildEngine* e = g->eng[i];
+        err = hipStreamWaitEvent(e->stream, g->fork, 0);
+        if (err != hipSuccess) break;
+        rc = do_steps(e, kGroupSteps, false);
+        if (rc) break;
+        err = hipEventRecord(g->join[i], e->stream);
+        if (err == hipSuccess) err = hipStreamWaitEvent(g->stream, g->join[i], 0);
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t end = hipStreamEndCapture(g->stream, &graph);
+    for (size_t i = 0; i < n; ++i) {
+        TildEngine* e = g->eng[i];
+        e->use_dstep = false;
+        e->step = step[i];
+        e->run_pos = run_pos[i];
+        e->cur = cur[i];
+        e->rho_valid = rho_valid[i];
+        e->nz_keyed = keyed[i];
+    }
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (err == hipSuccess) err = end;
+    if (err == hipSuccess) err = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    if (err != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        g->exec = nullptr;
+        set_error(MFT_ERR_DEVICE, "capturing the replica group's steps: %s", hipGetErrorString(err));
+        return MFT_ERR_DEVICE;
+    }
+    g->graph = graph;
+    for (size_t i = 0; i < n; ++i) {
+        g->cur0[i] = g->eng[i]->cur;
+        g->gather0[i] = g->eng[i]->gather_mode;
+        g->plane0[i] = g->eng[i]->plane_passes;
+    }
+    return MFT_OK;
+}
+
+extern "C" {
+
+int mft_tild_group_create(mft_tild_group** out, mft_tild* const* engines, int n)
+{
+    MFT_REQUIRE(out && engines && n >= 1, MFT_ERR_INVALID, "bad argument");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        MFT_REQUIRE(engines[i], MFT_ERR_INVALID, "null engine %d", i);
+        for (int j = 0; j < i; ++j) MFT_REQUIRE(engines[j] != engines[i], MFT_ERR_INVALID, "engine %d twice", i);
+    }
+    mft_tild_group* g = new mft_tild_group();
+    g->eng.assign(engines, engines + n);
+    g->cur0.assign(n, -1);
+    g->gather0.assign(n, -1);
+    g->plane0.assign(n, -1);
+    g->dstep_at.assign(n, -1);
+    g->join.assign(n, nullptr);
+    g->pre.assign(n, nullptr);
+    g->post.assign(n, nullptr);
+    hipError_t err = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&g->fork, hipEventDisableTiming);
+    for (int i = 0; i < n && err == hipSuccess; ++i) {
+        err = hipEventCreateWithFlags(&g->join[i], hipEventDisableTiming);
+        if (err == hipSuccess) err = hipEventCreateWithFlags(&g->pre[i], hipEventDisableTiming);
+        if (err == hipSuccess) err = hipEventCreateWithFlags(&g->post[i], hipEventDisableTiming);
+    }
+    if (err != hipSuccess) {
+        mft_tild_group_destroy(g);
+        set_error(MFT_ERR_DEVICE, "replica group set-up: %s", hipGetErrorString(err));
+        return MFT_ERR_DEVICE;
+    }
+    *out = g;
+    return MFT_OK;
+}
+
+int mft_tild_group_destroy(mft_tild_group* g)
+{
+    if (!g) return MFT_OK;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    group_drop_graph(g);
+    if (g->fork) (void)hipEventDestroy(g->fork);
+    for (auto* v : {&g->join, &g->pre, &g->post})
+        for (hipEvent_t ev : *v)
+            if (ev) (void)hipEventDestroy(ev);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+    return MFT_OK;
+}
+
+int mft_tild_group_step(mft_tild_group* g, int nsteps)
+{
+    MFT_REQUIRE(g && nsteps >= 0, MFT_ERR_INVALID, "bad argument");
+    const size_t n = g->eng.size();
+    for (size_t i = 0; i < n; ++i) MFT_REQUIRE(g->eng[i]->have_state, MFT_ERR_INVALID, "engine %zu has no state", i);
+    int rc;
+    int done = 0;
+    while (done < nsteps) {
+        bool ready = nsteps - done >= kGroupSteps;
+        for (size_t i = 0; i < n && ready; ++i) ready = graph_ready(g->eng[i]);
+        if (!ready) {
+            // one ordinary step of every engine, each on its own stream
+            for (size_t i = 0; i < n; ++i) {
+                if ((rc = do_steps(g->eng[i], 1, false))) return rc;
+                g->dstep_at[i] = -1;
+            }
+            ++done;
+            continue;
+        }
+        bool same = g->exec != nullptr;
+        for (size_t i = 0; i < n && same; ++i) {
+            const TildEngine* e = g->eng[i];
+            same = g->gather0[i] == e->gather_mode && g->plane0[i] == e->plane_passes;
+        }
+        if (!same && (rc = group_capture(g))) return rc;
+        bool parity = true;
+        for (size_t i = 0; i < n; ++i) parity = parity && g->eng[i]->cur == g->cur0[i];
+        if (!parity) {
+            // an odd number of ordinary steps since the capture: one more
+            for (size_t i = 0; i < n; ++i) {
+                if ((rc = do_steps(g->eng[i], 1, false))) return rc;
+                g->dstep_at[i] = -1;
+            }
+            ++done;
+            continue;
+        }
+        // each engine's earlier work and device step index before the replay
+        for (size_t i = 0; i < n; ++i) {
+            TildEngine* e = g->eng[i];
+            if (g->dstep_at[i] != e->step) {
+                k_set_step<<<1, 1, 0, e->stream>>>(e->d_step, (uint32_t)e->step);
+                MFT_LAUNCH_CHECK();
+            }
+            MFT_HIP(hipEventRecord(g->pre[i], e->stream));
+            MFT_HIP(hipStreamWaitEvent(g->stream, g->pre[i], 0));
+        }
+        MFT_HIP(hipGraphLaunch(g->exec, g->stream));
+        for (size_t i = 0; i < n; ++i) {
+            TildEngine* e = g->eng[i];
+            MFT_HIP(hipEventRecord(g->post[i], g->stream));
+            MFT_HIP(hipStreamWaitEvent(e->stream, g->post[i], 0));
+            e->step += kGroupSteps;
+            if (e->run_max > 0) e->run_pos += kGroupSteps;
+            e->rho_valid = false;
+            g->dstep_at[i] = e->step;
+        }
+        done += kGroupSteps;
+    }
+    return MFT_OK;
+}
+
+int mft_tild_group_synchronize(mft_tild_group* g)
+{
+    MFT_REQUIRE(g, MFT_ERR_INVALID, "null argument");
+    MFT_HIP(hipStreamSynchronize(g->stream));
+    for (mft_tild* e : g->eng) MFT_HIP(hipStreamSynchronize(e->stream));
     return MFT_OK;
 }
 

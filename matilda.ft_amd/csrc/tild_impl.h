@@ -230,6 +230,11 @@ struct TildEngine {
     // a GJF step of this engine; the integrator then regenerates it instead of
     // reading NZ, and NZ is not carried through the permutation
     bool nz_keyed = false;
+    // step index on the device for graph-captured steps: the integrator
+    // reads it and the key scan advances it, so replays key their noise by
+    // the step they run (null kernel argument on the ordinary path)
+    uint32_t* d_step = nullptr;
+    bool use_dstep = false;
     // step temporaries in slot order
     float4 *XN = nullptr, *NN = nullptr; // next positions and noise
     float4* fgrid = nullptr;             // grid force per slot (x, y, z, 0)
@@ -333,7 +338,7 @@ int tild_conv(TildEngine* e);
 int tild_tiles_configure(TildEngine* e); // brick geometry, LDS sizes, tile tables
 int tild_sort_state(TildEngine* e);     // particle-id-order staging -> key order
 int tild_unsort(TildEngine* e, int what); // key order -> particle-id-order staging
-int tild_scan(TildEngine* e);
+int tild_scan(TildEngine* e, bool advance_step = false);
 int tild_permute(TildEngine* e);         // next positions into the next key order
 int tild_density(TildEngine* e);          // tiles of the current positions
 int tild_density_reduce(TildEngine* e);   // tiles -> rho (when the densities are read)

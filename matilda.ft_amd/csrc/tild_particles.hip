@@ -299,7 +299,7 @@ __global__ void k_scan_reduce(const int* __restrict__ count, int M, int* __restr
 // Each block adds the totals of the blocks before it (at most a few thousand
 // values, read from L2), so the scan takes two launches.
 __global__ void k_scan_apply(int* __restrict__ count, int M, const int* __restrict__ partial,
-                             int* __restrict__ start, int ns)
+                             int* __restrict__ start, int ns, uint32_t* __restrict__ dstep)
 {
     __shared__ int sh[16];
     int c = 0;
@@ -324,7 +324,12 @@ __global__ void k_scan_apply(int* __restrict__ count, int M, const int* __restri
         }
         ex += v[k];
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) start[M] = ns;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        start[M] = ns;
+        // the step's last reader of the device step index ran before this
+        // scan (the integrator): advance it for the next step
+        if (dstep) *dstep += 1u;
+    }
 }
 
 // Particle state into the next step's key order: the slot's next position
@@ -892,6 +897,7 @@ struct StepArgs {
     LangevinParams lang;
     int nz_keyed;              // GJF: previous noise regenerated from (particle, step - 1), NZ not read
     int write_nn;              // store the new noise in NN (off under GJF: the next step regenerates it)
+    const uint32_t* dstep;     // step index on the device (graph replays), else null: use step
 };
 
 // Wall forces of ps_potentialWall.cu:92-108 on a particle of species ty
@@ -923,10 +929,10 @@ __device__ __forceinline__ void langevin_forces(const LangevinParams& P, int id,
 // of slot s from its total force fo: writes the next position and noise and
 // returns the next key.
 __device__ __forceinline__ int integrate_slot(int s, float4 xc, int2 meta, const float* fo, const StepArgs& A,
-                                              const TileGeom& tg)
+                                              uint32_t step, const TileGeom& tg)
 {
     float z[4];
-    normal4((uint32_t)meta.x, A.step, TAG_TILD, 0u, A.seed, z);
+    normal4((uint32_t)meta.x, step, TAG_TILD, 0u, A.seed, z);
     const float xcv[3] = {xc.x, xc.y, xc.z};
     float xn[3], nn[3] = {0.f, 0.f, 0.f};
     if (A.integ == MFT_INT_GJF) {
@@ -937,7 +943,7 @@ __device__ __forceinline__ int integrate_slot(int s, float4 xc, int2 meta, const
         if (A.nz_keyed) {
             // the previous step's nm * normal, the same bits as when it was drawn
             float zo[4];
-            normal4((uint32_t)meta.x, A.step - 1u, TAG_TILD, 0u, A.seed, zo);
+            normal4((uint32_t)meta.x, step - 1u, TAG_TILD, 0u, A.seed, zo);
             for (int j = 0; j < 3; ++j) nov[j] = nm * zo[j];
         } else {
             const float4 no4 = A.NZ[s];
@@ -1008,6 +1014,7 @@ __global__ void __launch_bounds__(256) k_integrate(
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = s < ns;
+    const uint32_t step = A.dstep ? *A.dstep : A.step;
     int k = 0;
     if (active) {
         const float4 xc = X[s];
@@ -1033,11 +1040,11 @@ __global__ void __launch_bounds__(256) k_integrate(
                 v[2] = v4.z;
             }
             // under VV these are the forces of the previous step (its noise)
-            const uint32_t ls = (A.use_v && A.vv != 0) ? A.step - 1u : A.step;
+            const uint32_t ls = (A.use_v && A.vv != 0) ? step - 1u : step;
             langevin_forces(A.lang, META[s].x, META[s].y, v, A.seed, ls, fo);
         }
         if (ftot) ftot[s] = make_float4(fo[0], fo[1], fo[2], 0.f);
-        if (A.integ != MFT_INT_NONE) k = integrate_slot(s, xc, META[s], fo, A, tg);
+        if (A.integ != MFT_INT_NONE) k = integrate_slot(s, xc, META[s], fo, A, step, tg);
     }
     if (A.integ == MFT_INT_NONE) return; // uniform over the grid
     const int r = wave_rank(count, k, active);
@@ -1289,10 +1296,12 @@ int tild_unsort(TildEngine* e, int what)
     return MFT_OK;
 }
 
-int tild_scan(TildEngine* e)
+int tild_scan(TildEngine* e, bool advance_step)
 {
+    uint32_t* ds = (advance_step && e->use_dstep) ? e->d_step : nullptr;
     k_scan_reduce<<<e->nscan_blocks, kScanThreads, 0, e->stream>>>(e->count, e->nkeys, e->partial);
-    k_scan_apply<<<e->nscan_blocks, kScanThreads, 0, e->stream>>>(e->count, e->nkeys, e->partial, e->start, e->N);
+    k_scan_apply<<<e->nscan_blocks, kScanThreads, 0, e->stream>>>(e->count, e->nkeys, e->partial, e->start, e->N,
+                                                                    ds);
     MFT_LAUNCH_CHECK();
     return MFT_OK;
 }
@@ -1437,6 +1446,7 @@ static StepArgs step_args(const TildEngine* e)
     A.FD = e->dpd.on ? e->fdpd : nullptr;
     A.nz_keyed = (d.integrator == MFT_INT_GJF && e->nz_keyed) ? 1 : 0;
     A.write_nn = d.integrator == MFT_INT_GJF ? 0 : 1;
+    A.dstep = e->use_dstep ? e->d_step : nullptr;
     return A;
 }
 

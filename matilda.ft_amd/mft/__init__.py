@@ -728,6 +728,34 @@ class FtsSystem:
         return n
 
 
+class TildGroup:
+    """Replica group (mft_tild_group_*): engines advanced by one HIP graph of
+    eight steps each; the engines stay owned by the caller."""
+
+    def __init__(self, engines):
+        self.engines = list(engines)
+        arr = (_p * len(self.engines))(*[e.h for e in self.engines])
+        self.h = _p()
+        check(lib().mft_tild_group_create(C.byref(self.h), arr, len(self.engines)))
+
+    def step(self, n=1):
+        check(lib().mft_tild_group_step(self.h, int(n)))
+
+    def synchronize(self):
+        check(lib().mft_tild_group_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            lib().mft_tild_group_destroy(self.h)
+            self.h = _p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class FtsBatch:
     """Many CL-FTS replicas advanced together (mft_fts_batch_*)."""
 
