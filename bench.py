@@ -350,6 +350,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fts", action="store_true", help="skip the CL-FTS plain/batched measurements")
     ap.add_argument("--no-serve", action="store_true", help="skip the served mixed workload")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 40k-particle diblock (alone, graph, replicas)")
     ap.add_argument("--serve-window", type=float, default=1.0, help="seconds of Poisson arrivals to serve")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run 5 device-to-device copies of BYTES (the known-size kernel that scales the "
@@ -442,14 +443,16 @@ def main():
         "hbm_frac_step": hbm_step / HBM_PEAK_GBS,
         "step_algorithmic_gbs": hbm_step,
         "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step_ms.items()},
-        "autotune": {"gather": ["direct", "lds_bricks"][tuned & 1], "xy_passes": ["per_axis", "per_plane"][tuned >> 1]},
+        "autotune": {"gather": ["direct", "lds_brick_species", "lds_brick"][tuned & 3],
+                     "xy_passes": ["per_axis", "per_plane"][tuned >> 2]},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes": kbytes[dom],
                      "avg_launch_ms": avg_ms[dom], "traffic": traffic},
         "cpu_baseline": None,
     }
     eng.close()
-    out["tild_c1"] = tild_c1_measure()
+    if not args.no_c1:
+        out["tild_c1"] = tild_c1_measure()
     if not args.no_fts:
         out["fts"] = fts_measure()
     if not args.no_serve:

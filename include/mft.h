@@ -296,9 +296,11 @@ int mft_tild_kernel_bytes(mft_tild* eng, double* bytes);
 int mft_tild_num_outputs(mft_tild* eng, int* nout);
 /* Times the kernel variants of this configuration on the device (reps
  * launches each, state unchanged) and keeps the fastest; choice receives the
- * selected variants: bit 0 the gather (0 direct loads, 1 LDS-staged bricks;
- * the two can differ in the last bit of a force), bit 1 the x-y transform
- * passes (0 one pass per axis, 1 one pass per z-plane; equal to rounding). */
+ * selected variants: bits 0-1 the gather (0 direct loads, 1 LDS windows per
+ * brick and species, 2 LDS windows of every species per brick; the direct
+ * and LDS variants can differ in the last bit of a force), bit 2 the x-y
+ * transform passes (0 one pass per axis, 1 one pass per z-plane; equal to
+ * rounding). */
 int mft_tild_autotune(mft_tild* eng, int reps, int* choice);
 /* Selects a gather variant directly (as above; the default is 1). */
 int mft_tild_set_gather(mft_tild* eng, int variant);

@@ -959,7 +959,7 @@ int mft_tild_kernel_bytes(mft_tild* e, double* bytes)
 int mft_tild_set_gather(mft_tild* e, int variant)
 {
     MFT_REQUIRE(e, MFT_ERR_INVALID, "null argument");
-    MFT_REQUIRE(variant == 0 || variant == 1, MFT_ERR_INVALID, "gather variant %d", variant);
+    MFT_REQUIRE(variant >= 0 && variant <= 2, MFT_ERR_INVALID, "gather variant %d", variant);
     e->gather_mode = variant;
     return MFT_OK;
 }
@@ -992,7 +992,8 @@ int mft_tild_autotune(mft_tild* e, int reps, int* choice)
     MFT_HIP(hipEventCreate(&b));
     float best = 1e30f;
     int best_mode = 0;
-    for (int mode = 0; mode < 2; ++mode) {
+    const int nmodes = tild_gather_brick_fits(e) ? 3 : 2;
+    for (int mode = 0; mode < nmodes; ++mode) {
         if ((rc = tild_gather(e, mode))) return rc;  // warm
         MFT_HIP(hipEventRecord(a, e->stream));
         for (int r = 0; r < reps; ++r)
@@ -1031,7 +1032,7 @@ int mft_tild_autotune(mft_tild* e, int reps, int* choice)
     e->plane_passes = best_plane;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    if (choice) *choice = best_mode | (best_plane << 1);
+    if (choice) *choice = best_mode | (best_plane << 2);
     return MFT_OK;
 }
 

@@ -287,7 +287,7 @@ struct TildEngine {
     KParams kp;
     size_t dens_lds = 0;
     uint32_t tk_mask = ~0u;      // kernel groups bracketed by timing events
-    int gather_mode = 1;    // 0 direct, 1 LDS windows (mft_tild_autotune picks)
+    int gather_mode = 1;    // 0 direct, 1 LDS windows per (brick, species), 2 per brick (mft_tild_autotune picks)
     int plane_passes = 0;   // 1: x and y transforms fused per z-plane (mft_tild_autotune picks)
     int64_t step = 0;
     int64_t run_max = 0, run_pos = 0; // declared run (mft_tild_set_run)
@@ -343,6 +343,7 @@ int tild_permute(TildEngine* e);         // next positions into the next key ord
 int tild_density(TildEngine* e);          // tiles of the current positions
 int tild_density_reduce(TildEngine* e);   // tiles -> rho (when the densities are read)
 int tild_gather(TildEngine* e, int mode);
+bool tild_gather_brick_fits(const TildEngine* e);   // gather variant 2's windows fit in LDS
 int tild_integrate(TildEngine* e, bool want_f);
 void erf_table(int kind, double sigma, double Rp, int nx, int ny, int nz, float lx, float ly, float lz,
                std::vector<float>& out);

@@ -727,6 +727,84 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_species(
     }
 }
 
+// Brick variant: one workgroup per brick stages the force windows of every
+// species with a force field (the window tables built once) and writes the
+// grid forces of all the brick's particles, whatever their species; fewer,
+// fuller workgroups than one per (brick, species).  LDS: nwin windows of
+// (b+p)^3 12-byte points (dynamic).
+template <int P, int BM>
+__global__ void __launch_bounds__(kTileThreads) k_gather_brick(
+    float4* __restrict__ fgrid, const float4* __restrict__ F4, GatherMap gm, const float4* __restrict__ spos,
+    const int2* __restrict__ smeta, const int* __restrict__ start, TileGeom tg, int nx, int ny, int nz, float dx0,
+    float dx1, float dx2)
+{
+    constexpr int TP = BM + P, T3 = TP * TP * TP, shift = P / 2;
+    extern __shared__ Win3 ftw[];   // [window][T3]
+    __shared__ int xg[TP], yo[TP], zo[TP];
+    __shared__ unsigned char rows[TP * TP][2];
+    const int brick = brick_of_block(blockIdx.x, tg.nbricks);
+    const int pb = start[(brick * tg.nspec) << 6], pe = start[((brick + 1) * tg.nspec) << 6];
+    if (pb == pe) return;
+    const int k0 = brick % tg.nb[0], k1 = (brick / tg.nb[0]) % tg.nb[1], k2 = brick / (tg.nb[0] * tg.nb[1]);
+    const int o0 = k0 << tg.sb[0], o1 = k1 << tg.sb[1], o2 = k2 << tg.sb[2];
+    const int e0 = min(tg.b[0], nx - o0) + P, e1 = min(tg.b[1], ny - o1) + P, e2 = min(tg.b[2], nz - o2) + P;
+    const long M = (long)nx * ny * nz;
+    if ((int)threadIdx.x < TP) {
+        const int t = threadIdx.x;
+        xg[t] = wrap(o0 - shift + t, nx);
+        yo[t] = nx * wrap(o1 - shift + t, ny);
+        zo[t] = nx * ny * wrap(o2 - shift + t, nz);
+    }
+    const int nrows = e1 * e2;
+    for (int r = threadIdx.x; r < nrows; r += kTileThreads) {
+        rows[r][0] = (unsigned char)(r % e1);
+        rows[r][1] = (unsigned char)(r / e1);
+    }
+    __syncthreads();
+    // windows of the species that have force fields, in species order
+    constexpr int RMAX = (TP * TP + 15) / 16;
+    const int lane16 = threadIdx.x & 15, row16 = threadIdx.x >> 4;
+    int nwin = 0;
+    for (int sp = 0; sp < tg.nspec; ++sp) {
+        const int f = gm.slot[sp];
+        if (f < 0) continue;
+        Win3* ft = ftw + nwin * T3;
+        ++nwin;
+        if (lane16 < e0) {
+            const float4* src = F4 + f * M + xg[lane16];
+            Win3* dst = ft + lane16;
+            float4 v[RMAX];
+            int dofs[RMAX];
+#pragma unroll
+            for (int j = 0; j < RMAX; ++j) {
+                const int r = min(row16 + 16 * j, nrows - 1);
+                const int t1 = rows[r][0], t2 = rows[r][1];
+                v[j] = src[yo[t1] + zo[t2]];
+                dofs[j] = TP * (t1 + TP * t2);
+            }
+#pragma unroll
+            for (int j = 0; j < RMAX; ++j)
+                if (row16 + 16 * j < nrows) dst[dofs[j]] = Win3{v[j].x, v[j].y, v[j].z};
+        }
+    }
+    __syncthreads();
+    const float i0 = 1.0f / dx0, i1 = 1.0f / dx1, i2 = 1.0f / dx2;
+    for (int it = pb + (int)threadIdx.x; it < pe; it += kTileThreads) {
+        const float4 pp = spos[it];
+        const int sp = smeta[it].y;
+        float3 a = make_float3(0.f, 0.f, 0.f);
+        if (gm.slot[sp] >= 0) {
+            int w = 0;
+            for (int q = 0; q < sp; ++q) w += gm.slot[q] >= 0 ? 1 : 0;
+            float wx[P + 1], wy[P + 1], wz[P + 1];
+            int l[3];
+            if (stencil_in_brick<P>(pp, dx0, dx1, dx2, i0, i1, i2, nx, ny, nz, o0, o1, o2, tg, wx, wy, wz, l))
+                a = stencil_sum<P, BM>(ftw + w * T3, l, wx, wy, wz);
+        }
+        fgrid[it] = make_float4(a.x, a.y, a.z, 0.f);
+    }
+}
+
 // Separate charge field (charges not uniform per species): one workgroup per
 // brick adds q times the field to every charged particle's grid force, after
 // the species pass.
@@ -1363,10 +1441,56 @@ int tild_density_reduce(TildEngine* e)
     return MFT_OK;
 }
 
-// gather variants: 0 = direct loads in key order, 1 = LDS tiles
+// LDS bytes of the brick gather's windows (every species with a force field)
+static size_t gather_brick_lds(const TildEngine* e)
+{
+    const TileGeom& tg = e->tg;
+    const int TP = (1 << tg.lbmax) + e->d.pmeorder;
+    int nwin = 0;
+    for (int sp = 0; sp < e->d.nspecies; ++sp) nwin += e->force_slot_h[sp] >= 0 ? 1 : 0;
+    return sizeof(Win3) * (size_t)nwin * TP * TP * TP;
+}
+
+bool tild_gather_brick_fits(const TildEngine* e) { return gather_brick_lds(e) <= 64 * 1024; }
+
+// gather variants: 0 = direct loads in key order, 1 = LDS windows per (brick,
+// species), 2 = LDS windows of every species per brick
 int tild_gather(TildEngine* e, int mode)
 {
     const mft_tild_desc& d = e->d;
+    if (mode == 2 && !tild_gather_brick_fits(e)) mode = 1;
+    if (mode == 2) {
+        const TileGeom& tg = e->tg;
+        const int BM = 1 << tg.lbmax;
+        GatherMap gm;
+        for (int sp = 0; sp < kMaxSpecies; ++sp) gm.slot[sp] = sp < d.nspecies ? e->force_slot_h[sp] : -1;
+        const size_t lds = gather_brick_lds(e);
+        const float4* Fq = e->F4 + (size_t)(e->qslot >= 0 ? e->qslot : 0) * e->M;
+#define MFT_G2(P_, B_)                                                                                    \
+    do {                                                                                                  \
+        k_gather_brick<P_, B_><<<tg.nbricks, kTileThreads, lds, e->stream>>>(e->fgrid, e->F4, gm, e->X[e->cur], \
+                                                                             e->META[e->cur], e->start, tg, d.nx, \
+                                                                             d.ny, d.nz, e->dx[0], e->dx[1], \
+                                                                             e->dx[2]);                   \
+        if (e->qslot >= 0)                                                                                \
+            k_gather_charge<P_, B_><<<tg.nbricks, kTileThreads, 0, e->stream>>>(                          \
+                e->fgrid, Fq, e->X[e->cur], e->META[e->cur], e->start, tg, d.nx, d.ny, d.nz, e->dx[0], e->dx[1], e->dx[2]); \
+    } while (0)
+#define MFT_G2_B(P_)                                                                                      \
+    if (BM == 8) MFT_G2(P_, 8);                                                                           \
+    else MFT_G2(P_, 4);
+        switch (d.pmeorder) {
+        case 0: MFT_G2_B(0); break;
+        case 1: MFT_G2_B(1); break;
+        case 2: MFT_G2_B(2); break;
+        case 3: MFT_G2_B(3); break;
+        default: MFT_G2_B(4); break;
+        }
+#undef MFT_G2_B
+#undef MFT_G2
+        MFT_LAUNCH_CHECK();
+        return MFT_OK;
+    }
     if (mode == 0) {
 #define MFT_G0(P_)                                                                                        \
     k_gather<P_><<<nblk(e->N), kBlock, 0, e->stream>>>(e->fgrid, e->F4, e->force_slot, e->qslot, e->X[e->cur], e->META[e->cur], \

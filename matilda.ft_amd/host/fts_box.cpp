@@ -535,6 +535,16 @@ void FTS_Box::writeData(int step)
     }
     otp << line.str() << std::endl;
     std::cout << line.str() << std::endl;
+    // chemical potentials mu = ln(n / (V Q)) of every molecule (ln z for a
+    // grand canonical one) in a file of their own, so fts_data.dat keeps the
+    // reference's columns: step Re(mu) Im(mu) per molecule
+    std::vector<double> mu(2 * Molecs.size());
+    check_mft(mft_fts_chemical_potentials(sys, mu.data()), "mft_fts_chemical_potentials");
+    std::ofstream omu("fts_mu.dat", std::ios_base::app);
+    omu << step;
+    omu.precision(12);
+    for (size_t m = 0; m < Molecs.size(); ++m) omu << " " << mu[2 * m] << " " << mu[2 * m + 1];
+    omu << std::endl;
 }
 
 // converged (FTS_Box.cu:715-760): SCFT only, Heff change at log steps

@@ -115,14 +115,17 @@ def test_fts_front_end_matches_engine(gpu, tmp_path):
     s = systems.fts_system(spec, fields)
     done = 0
     rows = read_log(tmp_path / "fts_data.dat")
-    assert [int(r[0]) for r in rows] == [0, 2, 4]
-    for row in rows:
+    mus = read_log(tmp_path / "fts_mu.dat")
+    assert [int(r[0]) for r in rows] == [0, 2, 4] and [int(r[0]) for r in mus] == [0, 2, 4]
+    for row, mrow in zip(rows, mus):
         k = int(row[0]) + 1
         s.step(k - done)
         done = k
         Q, H, parts = s.observables()
         want = [H.real, H.imag] + [p.real for p in parts]
         assert close(row[1:], want), (row, want)
+        mu = s.chemical_potentials()
+        assert np.allclose(mrow[1:], [mu[0].real, mu[0].imag], rtol=1e-9, atol=1e-9), (mrow, mu)
     s.close()
 
 
