@@ -165,6 +165,9 @@ def fts_measure(iters_plain=2000, iters_probe=20, iters_batch=1000, nrep=256, se
     b.synchronize()
     t0 = time.perf_counter()
     b.init()
+    # propagation chunk size per grid group, timed on density passes of the
+    # initial fields (inside the solve's wall time)
+    b.autotune(2)
     done = 0
     while done < iters_batch:
         n = min(100, iters_batch - done)
@@ -246,6 +249,7 @@ def fts_profile(which, iters=20):
         for k, (spec_r, fields_r) in enumerate(reps):
             systems.load_fields(s.set_fields, spec_r, fields_r, rep=k)
         s.init()
+        s.autotune(2)
         iters = 5
     s.step(2)
     s.synchronize()
@@ -444,7 +448,8 @@ def main():
         "step_algorithmic_gbs": hbm_step,
         "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step_ms.items()},
         "autotune": {"gather": ["direct", "lds_brick_species", "lds_brick"][tuned & 3],
-                     "xy_passes": ["per_axis", "per_plane"][tuned >> 2]},
+                     "xy_passes": ["per_axis", "per_plane"][(tuned >> 2) & 1],
+                     "x_fwd": ["direct_tiles", "staged_tile_rows"][(tuned >> 3) & 1]},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes": kbytes[dom],
                      "avg_launch_ms": avg_ms[dom], "traffic": traffic},

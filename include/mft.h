@@ -300,13 +300,15 @@ int mft_tild_num_outputs(mft_tild* eng, int* nout);
  * brick and species, 2 LDS windows of every species per brick; the direct
  * and LDS variants can differ in the last bit of a force), bit 2 the x-y
  * transform passes (0 one pass per axis, 1 one pass per z-plane; equal to
- * rounding). */
+ * rounding), bit 3 the forward x pass (0 tile entries read directly, 1 tile
+ * rows staged in LDS first; the same bits). */
 int mft_tild_autotune(mft_tild* eng, int reps, int* choice);
 /* Selects a gather variant directly (as above; the default is 1). */
 int mft_tild_set_gather(mft_tild* eng, int variant);
 /* Selects the x-y transform passes directly: 0 one pass per axis (default),
  * 1 both axes of a z-plane in one workgroup (square 64, 96 or 128 point
- * planes; MFT_ERR_UNSUPPORTED otherwise). */
+ * planes), 2 one pass per axis with the forward x pass staging its tile rows
+ * in LDS; MFT_ERR_UNSUPPORTED where the grid does not allow the variant. */
 int mft_tild_set_grid_passes(mft_tild* eng, int variant);
 /* Replica groups (no counterpart in the reference, which steps one box):
  * independent TILD engines advanced together by one HIP graph holding eight
@@ -503,6 +505,19 @@ int mft_fts_batch_get_densities(mft_fts_batch* b, int rep, float* rho);
 /* Fields, species fields and chemical potentials of replica rep, as
  * mft_fts_get_fields, mft_fts_get_species_fields, mft_fts_chemical_potentials. */
 int mft_fts_batch_get_fields(mft_fts_batch* b, int rep, int pot, float* wpl, float* wmi);
+/* Propagation in chunks of replicas (no counterpart in the reference): each
+ * group of equal grids runs its replicas' propagators chunk by chunk, all
+ * beads of one chunk before the next, so a chunk's slices are read back while
+ * they are still cached.  bytes bounds a chunk's per-bead working set (four
+ * slices per chain); 0 keeps one chunk per group.  Results are the same bits
+ * whatever the chunking. */
+int mft_fts_batch_set_chunk_bytes(mft_fts_batch* b, double bytes);
+/* Times each group's density pass (propagators, densities, Q from the current
+ * fields, which it leaves unchanged) with one chunk and with chunks of 512,
+ * 256, 128 and 64 MB, reps passes each, and keeps the fastest per group;
+ * ms_per_pass (may be NULL) receives the chosen passes' summed time.  Call
+ * after mft_fts_batch_init. */
+int mft_fts_batch_autotune(mft_fts_batch* b, int reps, double* ms_per_pass);
 int mft_fts_batch_get_species_fields(mft_fts_batch* b, int rep, float* W);
 int mft_fts_batch_chemical_potentials(mft_fts_batch* b, int rep, double* mu);
 

@@ -702,9 +702,14 @@ struct FtsGroup {
     DensDesc* d_dens = nullptr;
     int ndens = 0;
     QDesc* d_q = nullptr;
-    // propagation: per step s (1..maxN-1) and axis, a descriptor list
-    std::vector<PropDesc*> d_prop;   // [step*3 + axis]
-    std::vector<int> nprop;          // chains per step
+    // propagation: the group's replicas in chunks (consecutive replicas whose
+    // per-bead working set fits chunk_bytes; one chunk when it is 0), each
+    // running all its beads before the next chunk starts, so a chunk's
+    // slices are read back while they are still cached; per chunk, step s
+    // (1..maxN-1) and axis a descriptor list
+    std::vector<std::vector<PropDesc*>> d_prop;   // [chunk][step*3 + axis]
+    std::vector<std::vector<int>> nprop;          // [chunk][step] chains
+    size_t chunk_bytes = 0;
     int maxN = 0;
     bool dirty = false;              // membership changed: rebuild descriptors
     std::vector<void*> allocs;       // descriptor buffers
@@ -1163,15 +1168,33 @@ static int group_setup(FtsBatch* b, FtsGroup& G)
         MFT_HIP(hipMemcpy(G.d_zm, zm.data(), sizeof(ZmDesc) * zm.size(), hipMemcpyHostToDevice));
     }
 
-    // propagation descriptors per step and axis
+    // propagation descriptors per chunk, step and axis
     G.maxN = 0;
     for (int k = 0; k < nr; ++k)
         for (int m = 0; m < b->reps[G.reps[k]].d.nmolecules; ++m) G.maxN = std::max(G.maxN, b->reps[G.reps[k]].Ntot[m]);
-    G.d_prop.assign((size_t)3 * G.maxN, nullptr);
-    G.nprop.assign(G.maxN, 0);
+    // chunks: a replica's per-bead working set is four slices per chain
+    // (q_(s-1) read, the intermediate written and read, e^-W read, q_s written)
+    std::vector<int> chunk_end;
+    size_t acc = 0;
+    for (int k = 0; k < nr; ++k) {
+        const FtsRep& r = b->reps[G.reps[k]];
+        int chains = 0;
+        for (int m = 0; m < r.d.nmolecules; ++m) chains += r.sym[m] ? 1 : 2;
+        const size_t ws = (size_t)chains * 4 * sizeof(float2) * (size_t)r.M;
+        if (G.chunk_bytes > 0 && k > 0 && acc + ws > G.chunk_bytes) {
+            chunk_end.push_back(k);
+            acc = 0;
+        }
+        acc += ws;
+    }
+    chunk_end.push_back(nr);
+    const int nchunks = (int)chunk_end.size();
+    G.d_prop.assign(nchunks, std::vector<PropDesc*>((size_t)3 * G.maxN, nullptr));
+    G.nprop.assign(nchunks, std::vector<int>(G.maxN, 0));
+    for (int ch = 0; ch < nchunks; ++ch)
     for (int s = 1; s < G.maxN; ++s) {
         std::vector<PropDesc> ax[3];
-        for (int k = 0; k < nr; ++k) {
+        for (int k = ch == 0 ? 0 : chunk_end[ch - 1]; k < chunk_end[ch]; ++k) {
             FtsRep& r = b->reps[G.reps[k]];
             int chain = 0;
             for (int m = 0; m < r.d.nmolecules; ++m) {
@@ -1188,12 +1211,13 @@ static int group_setup(FtsBatch* b, FtsGroup& G)
                 }
             }
         }
-        G.nprop[s] = (int)ax[0].size();
+        G.nprop[ch][s] = (int)ax[0].size();
+        if (ax[0].empty()) continue;
         for (int a = 0; a < 3; ++a) {
             PropDesc* dp = nullptr;
             if ((rc = dev_alloc(G.allocs, &dp, ax[a].size()))) return rc;
             MFT_HIP(hipMemcpy(dp, ax[a].data(), sizeof(PropDesc) * ax[a].size(), hipMemcpyHostToDevice));
-            G.d_prop[(size_t)3 * s + a] = dp;
+            G.d_prop[ch][(size_t)3 * s + a] = dp;
         }
     }
     return MFT_OK;
@@ -1264,16 +1288,19 @@ static int group_density(FtsBatch* b, FtsGroup& G)
     int rc;
     hipStream_t st = b->stream;
     const bool fused = plane_fused(G) && b->plane_xy;
-    for (int s = 1; s < G.maxN; ++s) {
-        if (G.nprop[s] == 0) continue;
-        if (fused) {
-            if ((rc = launch_prop_xy(G.d_prop[(size_t)3 * s], G.d_prop[(size_t)3 * s + 1], G.nprop[s], G, st)))
-                return rc;
-        } else {
-            for (int a = 0; a < 2; ++a)
-                if ((rc = launch_prop(G.d_prop[(size_t)3 * s + a], G.nprop[s], G, a, st))) return rc;
+    for (size_t ch = 0; ch < G.d_prop.size(); ++ch) {
+        const std::vector<PropDesc*>& dp = G.d_prop[ch];
+        for (int s = 1; s < G.maxN; ++s) {
+            const int n = G.nprop[ch][s];
+            if (n == 0) continue;
+            if (fused) {
+                if ((rc = launch_prop_xy(dp[(size_t)3 * s], dp[(size_t)3 * s + 1], n, G, st))) return rc;
+            } else {
+                for (int a = 0; a < 2; ++a)
+                    if ((rc = launch_prop(dp[(size_t)3 * s + a], n, G, a, st))) return rc;
+            }
+            if ((rc = launch_prop(dp[(size_t)3 * s + 2], n, G, 2, st))) return rc;
         }
-        if ((rc = launch_prop(G.d_prop[(size_t)3 * s + 2], G.nprop[s], G, 2, st))) return rc;
     }
     const int M = G.n[0] * G.n[1] * G.n[2];
     dim3 grid((M + kFtsThreads - 1) / kFtsThreads, G.ndens);
@@ -2096,6 +2123,74 @@ int mft_fts_batch_init(mft_fts_batch* b)
     MFT_REQUIRE(b, MFT_ERR_INVALID, "null batch");
     return batch_init(b);
 }
+// Chunk size of every group's propagation (0: one chunk per group).
+int mft_fts_batch_set_chunk_bytes(mft_fts_batch* b, double bytes)
+{
+    MFT_REQUIRE(b && bytes >= 0.0, MFT_ERR_INVALID, "bad argument");
+    MFT_HIP(hipStreamSynchronize(b->stream));
+    int rc;
+    for (auto& G : b->groups) {
+        if (G.reps.empty()) continue;
+        G.chunk_bytes = (size_t)bytes;
+        if ((rc = group_setup(b, G))) return rc;
+    }
+    return MFT_OK;
+}
+
+// Per group, the propagation chunk size whose density pass (propagators,
+// densities and Q from the current fields, which it leaves unchanged) runs
+// fastest: one chunk, or chunks of 512, 256, 128 or 64 MB of per-bead
+// working set.
+int mft_fts_batch_autotune(mft_fts_batch* b, int reps, double* ms_per_pass)
+{
+    MFT_REQUIRE(b && reps >= 1, MFT_ERR_INVALID, "bad argument");
+    MFT_REQUIRE(b->inited, MFT_ERR_INVALID, "call init before the autotuner");
+    MFT_HIP(hipStreamSynchronize(b->stream));
+    hipEvent_t e0, e1;
+    MFT_HIP(hipEventCreate(&e0));
+    MFT_HIP(hipEventCreate(&e1));
+    const size_t MB = (size_t)1 << 20;
+    const size_t cand[] = {0, 512 * MB, 256 * MB, 128 * MB, 64 * MB};
+    double total = 0.0;
+    int rc = MFT_OK;
+    for (auto& G : b->groups) {
+        if (G.reps.empty()) continue;
+        size_t ws = 0;
+        for (int k : G.reps) {
+            const FtsRep& r = b->reps[k];
+            int chains = 0;
+            for (int m = 0; m < r.d.nmolecules; ++m) chains += r.sym[m] ? 1 : 2;
+            ws += (size_t)chains * 4 * sizeof(float2) * (size_t)r.M;
+        }
+        float best = 1e30f;
+        size_t best_c = 0;
+        for (size_t c : cand) {
+            if (c != 0 && c >= ws) continue;   // one chunk anyway
+            G.chunk_bytes = c;
+            if ((rc = group_setup(b, G)) || (rc = group_density(b, G))) break;   // warm
+            MFT_HIP(hipEventRecord(e0, b->stream));
+            for (int r = 0; r < reps && !rc; ++r) rc = group_density(b, G);
+            if (rc) break;
+            MFT_HIP(hipEventRecord(e1, b->stream));
+            MFT_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            MFT_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                best_c = c;
+            }
+        }
+        if (rc) break;
+        G.chunk_bytes = best_c;
+        if ((rc = group_setup(b, G))) break;
+        total += best / reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!rc && ms_per_pass) *ms_per_pass = total;
+    return rc;
+}
+
 int mft_fts_batch_step(mft_fts_batch* b, int niter)
 {
     MFT_REQUIRE(b && niter >= 0, MFT_ERR_INVALID, "bad argument");

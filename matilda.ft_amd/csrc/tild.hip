@@ -383,7 +383,9 @@ static int grid_forces(TildEngine* e)
     } else {
         {
             TimedScope ts(e, MFT_TK_FFT_X_FWD);
-            if ((rc = tild_x_fwd_tiles(e->tiles, e->tile_tab, e->S, d.nx, d.ny, d.nz, e->nchan, cstride, e->stream)))
+            const bool staged = e->xfwd_staged && tild_x_fwd_staged_ok(d.nx, tg.ex[0], e->tile_maxcover);
+            if ((rc = tild_x_fwd_tiles(e->tiles, e->tile_tab, e->S, d.nx, d.ny, d.nz, e->nchan, cstride, e->stream,
+                                       tg.ex[0], staged)))
                 return rc;
         }
         TimedScope ts(e, MFT_TK_FFT_Y_FWD);
@@ -967,10 +969,13 @@ int mft_tild_set_gather(mft_tild* e, int variant)
 int mft_tild_set_grid_passes(mft_tild* e, int variant)
 {
     MFT_REQUIRE(e, MFT_ERR_INVALID, "null argument");
-    MFT_REQUIRE(variant == 0 || variant == 1, MFT_ERR_INVALID, "grid pass variant %d", variant);
-    MFT_REQUIRE(variant == 0 || tild_plane_supported(e->d.nx, e->d.ny), MFT_ERR_UNSUPPORTED,
+    MFT_REQUIRE(variant >= 0 && variant <= 2, MFT_ERR_INVALID, "grid pass variant %d", variant);
+    MFT_REQUIRE(variant != 1 || tild_plane_supported(e->d.nx, e->d.ny), MFT_ERR_UNSUPPORTED,
                 "plane passes need square x-y planes of 64, 96 or 128 points (grid %d x %d)", e->d.nx, e->d.ny);
-    e->plane_passes = variant;
+    MFT_REQUIRE(variant != 2 || tild_x_fwd_staged_ok(e->d.nx, e->tg.ex[0], e->tile_maxcover), MFT_ERR_UNSUPPORTED,
+                "the staged x pass does not fit this grid");
+    e->plane_passes = variant == 1;
+    e->xfwd_staged = variant == 2;
     return MFT_OK;
 }
 
@@ -1030,9 +1035,30 @@ int mft_tild_autotune(mft_tild* e, int reps, int* choice)
         }
     }
     e->plane_passes = best_plane;
+    // forward x pass reading the tiles directly or through LDS-staged rows
+    int best_staged = 0;
+    if (!best_plane && tild_x_fwd_staged_ok(e->d.nx, e->tg.ex[0], e->tile_maxcover)) {
+        float bests = 1e30f;
+        for (int v = 0; v < 2; ++v) {
+            e->xfwd_staged = v;
+            if ((rc = grid_forces(e))) return rc;   // warm
+            MFT_HIP(hipEventRecord(a, e->stream));
+            for (int r = 0; r < reps; ++r)
+                if ((rc = grid_forces(e))) return rc;
+            MFT_HIP(hipEventRecord(b, e->stream));
+            MFT_HIP(hipEventSynchronize(b));
+            float ms = 0.f;
+            MFT_HIP(hipEventElapsedTime(&ms, a, b));
+            if (ms < bests) {
+                bests = ms;
+                best_staged = v;
+            }
+        }
+    }
+    e->xfwd_staged = best_staged;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    if (choice) *choice = best_mode | (best_plane << 2);
+    if (choice) *choice = best_mode | (best_plane << 2) | (best_staged << 3);
     return MFT_OK;
 }
 
@@ -1152,7 +1178,7 @@ static int group_capture(mft_tild_group* g)
     for (size_t i = 0; i < n; ++i) {
         g->cur0[i] = g->eng[i]->cur;
         g->gather0[i] = g->eng[i]->gather_mode;
-        g->plane0[i] = g->eng[i]->plane_passes;
+        g->plane0[i] = g->eng[i]->plane_passes | (g->eng[i]->xfwd_staged << 1);
     }
     return MFT_OK;
 }
@@ -1228,7 +1254,7 @@ int mft_tild_group_step(mft_tild_group* g, int nsteps)
         bool same = g->exec != nullptr;
         for (size_t i = 0; i < n && same; ++i) {
             const TildEngine* e = g->eng[i];
-            same = g->gather0[i] == e->gather_mode && g->plane0[i] == e->plane_passes;
+            same = g->gather0[i] == e->gather_mode && g->plane0[i] == (e->plane_passes | (e->xfwd_staged << 1));
         }
         if (!same && (rc = group_capture(g))) return rc;
         bool parity = true;

@@ -381,9 +381,130 @@ int tild_yx_inv_packed_plane(const float2* Y, float4* F4, int nx, int ny, int nz
     return MFT_OK;
 }
 
-int tild_x_fwd_tiles(const float* tiles, const int* tab, float2* S, int nx, int ny, int nz, int nchan, long cstride,
-                     hipStream_t st)
+// The same pass with the tile rows staged in LDS first: the (z cover, line,
+// y cover) rows of the halo-expanded tile grid that the block's B lines read
+// (at most two covers per axis) are copied with float4 loads, then each point
+// sums its entries from LDS in k_tile_reduce's order (the same bits), and the
+// staging buffer is reused for the transform's exchanges.  Needs the
+// halo-expanded row length ex0 to be a multiple of 4.
+template <int N>
+__global__ void __launch_bounds__(256) k_fft_x_fwd_tiles_staged(const float* __restrict__ tiles,
+                                                                const int* __restrict__ tab,
+                                                                float2* __restrict__ out, int ny, int nz,
+                                                                long cstride, int ex0,
+                                                                const float2* __restrict__ twg)
 {
+    using P = FftPlan<N>;
+    constexpr int T = P::T, E = P::E, B = 256 / T;
+    extern __shared__ float4 stage4[];   // [2 z covers][B lines][2 y covers][ex0 / 4]
+    float* stage = reinterpret_cast<float*>(stage4);
+    float2* lds = reinterpret_cast<float2*>(stage4);
+    __shared__ float2 tw[N];
+    __shared__ int tabx[5 * N];
+    __shared__ int rowoff[2 * B * 2];
+    __shared__ int ncz[B], ncy[B];
+    for (int i = threadIdx.x; i < N; i += 256) tw[i] = twg[i];
+    for (int i = threadIdx.x; i < 5 * N; i += 256) tabx[i] = tab[i];
+    const int c = blockIdx.y;
+    const int nlines = ny * nz;
+    if ((int)threadIdx.x < B) {
+        const int l = threadIdx.x, line = blockIdx.x * B + l;
+        int cz = 0, cy = 0;
+        const int* tz = nullptr;
+        const int* ty = nullptr;
+        if (line < nlines) {
+            tz = tab + 5 * (N + ny + line / ny);
+            ty = tab + 5 * (N + line % ny);
+            cz = tz[0];
+            cy = ty[0];
+        }
+        ncz[l] = cz;
+        ncy[l] = cy;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                rowoff[(a * B + l) * 2 + b] = (a < cz && b < cy) ? tz[1 + a] + ty[1 + b] : -1;
+    }
+    __syncthreads();
+    const int q = ex0 >> 2;
+    const float4* tc4 = reinterpret_cast<const float4*>(tiles + c * cstride);
+    for (int i = threadIdx.x; i < 4 * B * q; i += 256) {
+        const int r = i / q, j = i - r * q;
+        const int off = rowoff[r];
+        if (off >= 0) stage4[(long)r * q + j] = tc4[(off >> 2) + j];
+    }
+    __syncthreads();
+    constexpr int PER = (B * N + 255) / 256;
+    float sums[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int i = threadIdx.x + 256 * k;
+        float s = 0.f;
+        if (i < B * N) {
+            const int l = i / N, x = i - l * N;
+            const int* tx = tabx + 5 * x;
+            const int cz = ncz[l], cy = ncy[l], cx = tx[0];
+            for (int a = 0; a < cz; ++a)
+                for (int b = 0; b < cy; ++b) {
+                    const float* row = stage + (long)((a * B + l) * 2 + b) * ex0;
+                    for (int d = 0; d < cx; ++d) s += row[tx[1 + d]];
+                }
+        }
+        sums[k] = s;
+    }
+    __syncthreads();   // the staged rows are read; the buffer takes the transform
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int i = threadIdx.x + 256 * k;
+        if (i < B * N) lds[i] = make_float2(sums[k], 0.f);
+    }
+    __syncthreads();
+    const int lline = threadIdx.x / T;
+    const int t = threadIdx.x % T;
+    const int line = blockIdx.x * B + lline;
+    float2 v[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) v[s] = lds[lline * N + t + T * s];
+    LdsMap map{lline, B, false, N};
+    line_fft<N, -1, true>(v, lds, tw, t, map);
+    if (line >= nlines) return;
+    float2* dst = out + c * (long)N * nlines + (long)line * N;
+#pragma unroll
+    for (int s = 0; s < E; ++s) dst[t + T * s] = v[s];
+}
+
+// staged x pass possible: row length a multiple of 4 floats, at most two
+// covering tiles per coordinate (every axis), the staging buffer in 64 KB
+bool tild_x_fwd_staged_ok(int nx, int ex0, int maxcover)
+{
+    int T = 0;
+#define MFT_T(n_) if (nx == n_) T = FftPlan<n_>::T;
+    MFT_FOR_EACH_FFT_SIZE(MFT_T)
+#undef MFT_T
+    if (T == 0 || ex0 % 4 != 0 || maxcover > 2) return false;
+    const size_t B = 256 / T;
+    return 16 * B * (size_t)ex0 <= 64 * 1024 && B * (size_t)nx * 8 <= 16 * B * (size_t)ex0;
+}
+
+int tild_x_fwd_tiles(const float* tiles, const int* tab, float2* S, int nx, int ny, int nz, int nchan, long cstride,
+                     hipStream_t st, int ex0, bool staged)
+{
+    if (staged) {
+        switch (nx) {
+#define MFT_CASE(n_)                                                                                      \
+    case n_: {                                                                                            \
+        constexpr int B = 256 / FftPlan<n_>::T;                                                           \
+        dim3 grid((ny * nz + B - 1) / B, nchan);                                                          \
+        k_fft_x_fwd_tiles_staged<n_><<<grid, 256, 16 * B * (size_t)ex0, st>>>(tiles, tab, S, ny, nz, cstride, ex0, \
+                                                                            fft_twiddles_device(n_));    \
+        break;                                                                                            \
+    }
+            MFT_FOR_EACH_FFT_SIZE(MFT_CASE)
+#undef MFT_CASE
+        default: return MFT_ERR_UNSUPPORTED;
+        }
+        MFT_LAUNCH_CHECK();
+        return MFT_OK;
+    }
     switch (nx) {
 #define MFT_CASE(n_)                                                                                      \
     case n_: {                                                                                            \

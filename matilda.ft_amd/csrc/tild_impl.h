@@ -289,6 +289,8 @@ struct TildEngine {
     uint32_t tk_mask = ~0u;      // kernel groups bracketed by timing events
     int gather_mode = 1;    // 0 direct, 1 LDS windows per (brick, species), 2 per brick (mft_tild_autotune picks)
     int plane_passes = 0;   // 1: x and y transforms fused per z-plane (mft_tild_autotune picks)
+    int xfwd_staged = 0;    // 1: forward x pass stages its tile rows in LDS (mft_tild_autotune picks)
+    int tile_maxcover = 4;  // most tiles covering one coordinate along any axis
     int64_t step = 0;
     int64_t run_max = 0, run_pos = 0; // declared run (mft_tild_set_run)
     int have_state = 0;
@@ -325,7 +327,8 @@ static int dalloc(T** p, size_t n)
 int tild_axis(int sign, int real_in, int real_out, const void* in, void* out, AxisGeom g, int nf, long fs,
               float scale, hipStream_t st);
 int tild_x_fwd_tiles(const float* tiles, const int* tab, float2* S, int nx, int ny, int nz, int nchan, long cstride,
-                     hipStream_t st);
+                     hipStream_t st, int ex0 = 0, bool staged = false);
+bool tild_x_fwd_staged_ok(int nx, int ex0, int maxcover);
 int tild_x_inv_packed(const float2* Y, float4* F4, int nx, int ny, int nz, int nfield, hipStream_t st);
 bool tild_plane_supported(int nx, int ny);
 int tild_xy_fwd_tiles_plane(const float* tiles, const int* tab, float2* S, int nx, int ny, int nz, int nchan,

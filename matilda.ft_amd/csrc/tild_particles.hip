@@ -1279,7 +1279,7 @@ int tild_tiles_configure(TildEngine* e)
     // the halo-expanded grid along that axis
     std::vector<int> tab(5 * (size_t)(n[0] + n[1] + n[2]), 0);
     const long estride[3] = {1, tg.ex[0], (long)tg.ex[0] * tg.ex[1]};
-    int base = 0;
+    int base = 0, maxcover = 0;
     for (int a = 0; a < 3; ++a) {
         for (int X = 0; X < n[a]; ++X) {
             int* row = &tab[5 * (size_t)(base + X)];
@@ -1293,9 +1293,11 @@ int tild_tiles_configure(TildEngine* e)
                 }
             }
             row[0] = cnt;
+            maxcover = std::max(maxcover, cnt);
         }
         base += n[a];
     }
+    e->tile_maxcover = maxcover;
     if (e->tile_tab) (void)hipFree(e->tile_tab);
     e->tile_tab = nullptr;
     int rc;

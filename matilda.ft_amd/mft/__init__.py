@@ -787,6 +787,14 @@ class FtsBatch:
     def init(self):
         check(lib().mft_fts_batch_init(self.h))
 
+    def set_chunk_bytes(self, nbytes):
+        check(lib().mft_fts_batch_set_chunk_bytes(self.h, _d(nbytes)))
+
+    def autotune(self, reps=2):
+        ms = _d(0.0)
+        check(lib().mft_fts_batch_autotune(self.h, int(reps), C.byref(ms)))
+        return ms.value
+
     def step(self, n=1):
         check(lib().mft_fts_batch_step(self.h, n))
 
