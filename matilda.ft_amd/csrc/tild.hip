@@ -385,7 +385,7 @@ static int grid_forces(TildEngine* e)
             TimedScope ts(e, MFT_TK_FFT_X_FWD);
             const bool staged = e->xfwd_staged && tild_x_fwd_staged_ok(d.nx, tg.ex[0], e->tile_maxcover);
             if ((rc = tild_x_fwd_tiles(e->tiles, e->tile_tab, e->S, d.nx, d.ny, d.nz, e->nchan, cstride, e->stream,
-                                       tg.ex[0], staged)))
+                                       tg.ex[0], staged, e->tile_maxcover <= 2)))
                 return rc;
         }
         TimedScope ts(e, MFT_TK_FFT_Y_FWD);

@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(256) k_fft_x_inv_packed(const float2* __restri
 template <int N>
 __global__ void __launch_bounds__(256) k_fft_x_fwd_tiles(const float* __restrict__ tiles, const int* __restrict__ tab,
                                                          float2* __restrict__ out, int ny, int nz, long cstride,
-                                                         const float2* __restrict__ twg)
+                                                         const float2* __restrict__ twg, int cover2)
 {
     using P = FftPlan<N>;
     constexpr int T = P::T, E = P::E, B = 256 / T;
@@ -122,21 +122,70 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd_tiles(const float* __restrict
     const int c = blockIdx.y;
     const int nlines = ny * nz;
     const float* tc = tiles + c * cstride;
-    // every point's tile loads in flight together (the loop unrolled), then
-    // the sums into LDS
     constexpr int PER = (B * N + 255) / 256;
     float sums[PER];
+    if (cover2) {
+        // at most two covering tiles per axis: every point's cover rows are
+        // read first, then all its (up to eight) tile entries are loaded
+        // together, branch-free, and summed in k_tile_reduce's order
+        int cnt[PER][3], oz[PER][2], oy[PER][2], ox[PER][2];
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const int i = threadIdx.x + 256 * k;
-        const int l = i / N, x = i - l * N;
-        const int line = blockIdx.x * B + l;
-        float sum = 0.f;
-        if (i < B * N && line < nlines) {
-            const int gy = line % ny, gz = line / ny;
-            sum = tile_sum(tc, tab + 5 * (N + ny + gz), tab + 5 * (N + gy), tabx + 5 * x);
+        for (int k = 0; k < PER; ++k) {
+            const int i = threadIdx.x + 256 * k;
+            const int l = i / N, x = i - l * N;
+            const int line = blockIdx.x * B + l;
+            const bool valid = i < B * N && line < nlines;
+            const int gy = valid ? line % ny : 0, gz = valid ? line / ny : 0;
+            const int* tz = tab + 5 * (N + ny + gz);
+            const int* ty = tab + 5 * (N + gy);
+            const int* tx = tabx + 5 * (valid ? x : 0);
+            cnt[k][0] = valid ? tz[0] : 0;
+            cnt[k][1] = ty[0];
+            cnt[k][2] = tx[0];
+            oz[k][0] = tz[1];
+            oz[k][1] = tz[2];
+            oy[k][0] = ty[1];
+            oy[k][1] = ty[2];
+            ox[k][0] = tx[1];
+            ox[k][1] = tx[2];
         }
-        sums[k] = sum;
+        float v[PER][8];
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int d = 0; d < 2; ++d) {
+                        const bool in = a < cnt[k][0] && b < cnt[k][1] && d < cnt[k][2];
+                        v[k][4 * a + 2 * b + d] = in ? tc[oz[k][a] + oy[k][b] + ox[k][d]] : 0.f;
+                    }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            float sum = 0.f;
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int d = 0; d < 2; ++d)
+                        if (a < cnt[k][0] && b < cnt[k][1] && d < cnt[k][2]) sum += v[k][4 * a + 2 * b + d];
+            sums[k] = sum;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = threadIdx.x + 256 * k;
+            const int l = i / N, x = i - l * N;
+            const int line = blockIdx.x * B + l;
+            float sum = 0.f;
+            if (i < B * N && line < nlines) {
+                const int gy = line % ny, gz = line / ny;
+                sum = tile_sum(tc, tab + 5 * (N + ny + gz), tab + 5 * (N + gy), tabx + 5 * x);
+            }
+            sums[k] = sum;
+        }
     }
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
@@ -486,7 +535,7 @@ bool tild_x_fwd_staged_ok(int nx, int ex0, int maxcover)
 }
 
 int tild_x_fwd_tiles(const float* tiles, const int* tab, float2* S, int nx, int ny, int nz, int nchan, long cstride,
-                     hipStream_t st, int ex0, bool staged)
+                     hipStream_t st, int ex0, bool staged, bool cover2)
 {
     if (staged) {
         switch (nx) {
@@ -510,7 +559,8 @@ int tild_x_fwd_tiles(const float* tiles, const int* tab, float2* S, int nx, int 
     case n_: {                                                                                            \
         constexpr int B = 256 / FftPlan<n_>::T;                                                           \
         dim3 grid((ny * nz + B - 1) / B, nchan);                                                          \
-        k_fft_x_fwd_tiles<n_><<<grid, 256, 0, st>>>(tiles, tab, S, ny, nz, cstride, fft_twiddles_device(n_)); \
+        k_fft_x_fwd_tiles<n_><<<grid, 256, 0, st>>>(tiles, tab, S, ny, nz, cstride, fft_twiddles_device(n_), \
+                                                    cover2 ? 1 : 0);                                      \
         break;                                                                                            \
     }
         MFT_FOR_EACH_FFT_SIZE(MFT_CASE)

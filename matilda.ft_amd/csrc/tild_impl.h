@@ -327,7 +327,7 @@ static int dalloc(T** p, size_t n)
 int tild_axis(int sign, int real_in, int real_out, const void* in, void* out, AxisGeom g, int nf, long fs,
               float scale, hipStream_t st);
 int tild_x_fwd_tiles(const float* tiles, const int* tab, float2* S, int nx, int ny, int nz, int nchan, long cstride,
-                     hipStream_t st, int ex0 = 0, bool staged = false);
+                     hipStream_t st, int ex0 = 0, bool staged = false, bool cover2 = false);
 bool tild_x_fwd_staged_ok(int nx, int ex0, int maxcover);
 int tild_x_inv_packed(const float2* Y, float4* F4, int nx, int ny, int nz, int nfield, hipStream_t st);
 bool tild_plane_supported(int nx, int ny);

@@ -362,6 +362,14 @@ __global__ void k_remap_bonds(int ns, int maxbonds, const int* __restrict__ map,
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
     const int n = map[s], nb = NB[s];
+    if (maxbonds == 2) {
+        // both partners' map entries loaded together
+        const int2 b = reinterpret_cast<const int2*>(BS)[s];
+        const int m0 = nb > 0 ? map[b.x] : 0;
+        const int m1 = nb > 1 ? map[b.y] : 0;
+        reinterpret_cast<int2*>(BS2)[n] = make_int2(m0, m1);
+        return;
+    }
     for (int b = 0; b < maxbonds; ++b) BS2[n * maxbonds + b] = b < nb ? map[BS[s * maxbonds + b]] : 0;
 }
 
@@ -876,6 +884,28 @@ __device__ __forceinline__ void bond_force_slots(int s, int ns, const float4* __
     }
 }
 
+// One bond of bond_force_slots with the partner's position given
+__device__ __forceinline__ void bond_force_pair(const BondParams& bp, int bt, float L0, float L1, float L2,
+                                                float4 x, float4 y, float* fo)
+{
+    const float d0 = min_image(x.x - y.x, L0, 0.5f * L0);
+    const float d1 = min_image(x.y - y.y, L1, 0.5f * L1);
+    const float d2 = min_image(x.z - y.z, L2, 0.5f * L2);
+    const float r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    float mf = 0.f;
+    if (r2 > 1.0e-4f) {
+        const float r = sqrtf(r2);
+        if (bp.style[bt] == 0) mf = 2.0f * bp.k[bt] * (r - bp.r0[bt]) / r;
+        else if (bp.style[bt] == 1) {
+            const float u = r / bp.r0[bt];
+            mf = 2.0f * bp.k[bt] * r / (1.0f - u * u) / r;
+        }
+    }
+    fo[0] -= mf * d0;
+    fo[1] -= mf * d1;
+    fo[2] -= mf * d2;
+}
+
 // One angle of slot s (d_angles, ps_deviceAngles.cu:8-133): the force on
 // whichever member s is, and the angle's energy (d_anglesStressEnergy,
 // ps_deviceAngles.cu:138-290); false when a bond vector is too short.
@@ -1007,7 +1037,7 @@ __device__ __forceinline__ void langevin_forces(const LangevinParams& P, int id,
 // of slot s from its total force fo: writes the next position and noise and
 // returns the next key.
 __device__ __forceinline__ int integrate_slot(int s, float4 xc, int2 meta, const float* fo, const StepArgs& A,
-                                              uint32_t step, const TileGeom& tg)
+                                              uint32_t step, const TileGeom& tg, float4 xo4, float4 nz4)
 {
     float z[4];
     normal4((uint32_t)meta.x, step, TAG_TILD, 0u, A.seed, z);
@@ -1016,7 +1046,6 @@ __device__ __forceinline__ int integrate_slot(int s, float4 xc, int2 meta, const
     if (A.integ == MFT_INT_GJF) {
         const int ty = meta.y;
         const float a = A.gp.a[ty], b = A.gp.b[ty], nm = A.gp.nmag[ty], cf = A.gp.cf[ty], cn = A.gp.cn[ty];
-        const float4 xo4 = A.XO[s];
         float nov[3];
         if (A.nz_keyed) {
             // the previous step's nm * normal, the same bits as when it was drawn
@@ -1024,10 +1053,9 @@ __device__ __forceinline__ int integrate_slot(int s, float4 xc, int2 meta, const
             normal4((uint32_t)meta.x, step - 1u, TAG_TILD, 0u, A.seed, zo);
             for (int j = 0; j < 3; ++j) nov[j] = nm * zo[j];
         } else {
-            const float4 no4 = A.NZ[s];
-            nov[0] = no4.x;
-            nov[1] = no4.y;
-            nov[2] = no4.z;
+            nov[0] = nz4.x;
+            nov[1] = nz4.y;
+            nov[2] = nz4.z;
         }
         const float xov[3] = {xo4.x, xo4.y, xo4.z};
         for (int j = 0; j < 3; ++j) {
@@ -1051,8 +1079,7 @@ __device__ __forceinline__ int integrate_slot(int s, float4 xc, int2 meta, const
         // array of a fresh allocation: neither kick nor drift changes anything.
         const float inv_m = A.gp.inv_m[meta.y];
         const float hd = A.delt * 0.5f;
-        const float4 v4 = A.NZ[s];
-        const float vv0[3] = {v4.x, v4.y, v4.z};
+        const float vv0[3] = {nz4.x, nz4.y, nz4.z};
         for (int j = 0; j < 3; ++j) {
             float v = vv0[j];
             float x = xcv[j];
@@ -1095,12 +1122,31 @@ __global__ void __launch_bounds__(256) k_integrate(
     const uint32_t step = A.dstep ? *A.dstep : A.step;
     int k = 0;
     if (active) {
+        // every load that does not depend on another issued first, the bond
+        // partners' positions next: two rounds of memory latency per slot
         const float4 xc = X[s];
         const float4 fg = fgrid[s];
+        const int2 meta = META[s];
+        const bool gjf = A.integ == MFT_INT_GJF;
+        const bool need_nz = (gjf && !A.nz_keyed) || A.integ == MFT_INT_VV || A.use_v;
+        const float4 xo4 = gjf ? A.XO[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 nz4 = need_nz ? A.NZ[s] : make_float4(0.f, 0.f, 0.f, 0.f);
         float fo[3] = {fg.x, fg.y, fg.z};
-        bond_force_slots(s, ns, X, NB, BS, BT, maxbonds, bp, A.L[0], A.L[1], A.L[2], xc.x, xc.y, xc.z, fo);
+        if (maxbonds == 2) {
+            const int nb = NB[s];
+            const int2 bs = reinterpret_cast<const int2*>(BS)[s];
+            const int2 bt = reinterpret_cast<const int2*>(BT)[s];
+            const bool v0 = nb > 0 && bs.x >= 0 && bs.x < ns && bt.x >= 0 && bt.x < bp.ntypes;
+            const bool v1 = nb > 1 && bs.y >= 0 && bs.y < ns && bt.y >= 0 && bt.y < bp.ntypes;
+            const float4 y0 = X[v0 ? bs.x : s];
+            const float4 y1 = X[v1 ? bs.y : s];
+            if (v0) bond_force_pair(bp, bt.x, A.L[0], A.L[1], A.L[2], xc, y0, fo);
+            if (v1) bond_force_pair(bp, bt.y, A.L[0], A.L[1], A.L[2], xc, y1, fo);
+        } else {
+            bond_force_slots(s, ns, X, NB, BS, BT, maxbonds, bp, A.L[0], A.L[1], A.L[2], xc.x, xc.y, xc.z, fo);
+        }
         if (A.NA) angle_force_slots(s, X, A.NA, A.AT, A.AG, A.ap, A.L[0], A.L[1], A.L[2], fo);
-        if (A.walls.n) wall_forces(A.walls, META[s].y, xc, fo);
+        if (A.walls.n) wall_forces(A.walls, meta.y, xc, fo);
         if (A.FD) {
             const float4 fd = A.FD[s];
             fo[0] += fd.x;
@@ -1112,17 +1158,16 @@ __global__ void __launch_bounds__(256) k_integrate(
             // reference holds at force time), zero under GJF and EM
             float v[3] = {0.f, 0.f, 0.f};
             if (A.use_v) {
-                const float4 v4 = A.NZ[s];
-                v[0] = v4.x;
-                v[1] = v4.y;
-                v[2] = v4.z;
+                v[0] = nz4.x;
+                v[1] = nz4.y;
+                v[2] = nz4.z;
             }
             // under VV these are the forces of the previous step (its noise)
             const uint32_t ls = (A.use_v && A.vv != 0) ? step - 1u : step;
-            langevin_forces(A.lang, META[s].x, META[s].y, v, A.seed, ls, fo);
+            langevin_forces(A.lang, meta.x, meta.y, v, A.seed, ls, fo);
         }
         if (ftot) ftot[s] = make_float4(fo[0], fo[1], fo[2], 0.f);
-        if (A.integ != MFT_INT_NONE) k = integrate_slot(s, xc, META[s], fo, A, step, tg);
+        if (A.integ != MFT_INT_NONE) k = integrate_slot(s, xc, meta, fo, A, step, tg, xo4, nz4);
     }
     if (A.integ == MFT_INT_NONE) return; // uniform over the grid
     const int r = wave_rank(count, k, active);
