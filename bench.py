@@ -447,7 +447,7 @@ def main():
         "hbm_frac_step": hbm_step / HBM_PEAK_GBS,
         "step_algorithmic_gbs": hbm_step,
         "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step_ms.items()},
-        "autotune": {"gather": ["direct", "lds_brick_species", "lds_brick"][tuned & 3],
+        "autotune": {"gather": ["direct", "lds_brick_species", "lds_brick", "lds_pipelined"][tuned & 3],
                      "xy_passes": ["per_axis", "per_plane"][(tuned >> 2) & 1],
                      "x_fwd": ["direct_tiles", "staged_tile_rows"][(tuned >> 3) & 1]},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

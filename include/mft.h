@@ -297,8 +297,9 @@ int mft_tild_num_outputs(mft_tild* eng, int* nout);
 /* Times the kernel variants of this configuration on the device (reps
  * launches each, state unchanged) and keeps the fastest; choice receives the
  * selected variants: bits 0-1 the gather (0 direct loads, 1 LDS windows per
- * brick and species, 2 LDS windows of every species per brick; the direct
- * and LDS variants can differ in the last bit of a force), bit 2 the x-y
+ * brick and species, 2 LDS windows of every species per brick, 3 as 1 with
+ * the next window's loads overlapping the current sums; the direct and LDS
+ * variants can differ in the last bit of a force), bit 2 the x-y
  * transform passes (0 one pass per axis, 1 one pass per z-plane; equal to
  * rounding), bit 3 the forward x pass (0 tile entries read directly, 1 tile
  * rows staged in LDS first; the same bits). */

@@ -961,7 +961,7 @@ int mft_tild_kernel_bytes(mft_tild* e, double* bytes)
 int mft_tild_set_gather(mft_tild* e, int variant)
 {
     MFT_REQUIRE(e, MFT_ERR_INVALID, "null argument");
-    MFT_REQUIRE(variant >= 0 && variant <= 2, MFT_ERR_INVALID, "gather variant %d", variant);
+    MFT_REQUIRE(variant >= 0 && variant <= 3, MFT_ERR_INVALID, "gather variant %d", variant);
     e->gather_mode = variant;
     return MFT_OK;
 }
@@ -997,8 +997,8 @@ int mft_tild_autotune(mft_tild* e, int reps, int* choice)
     MFT_HIP(hipEventCreate(&b));
     float best = 1e30f;
     int best_mode = 0;
-    const int nmodes = tild_gather_brick_fits(e) ? 3 : 2;
-    for (int mode = 0; mode < nmodes; ++mode) {
+    for (int mode = 0; mode < 4; ++mode) {
+        if (mode == 2 && !tild_gather_brick_fits(e)) continue;
         if ((rc = tild_gather(e, mode))) return rc;  // warm
         MFT_HIP(hipEventRecord(a, e->stream));
         for (int r = 0; r < reps; ++r)

@@ -735,6 +735,112 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_species(
     }
 }
 
+// Pipelined variant: each workgroup walks a contiguous run of (brick,
+// species) tasks and loads the next task's window into registers while it
+// sums the current one from LDS, so the window loads overlap the stencil
+// sums.  A lane owns window column lane16 and rows row16 + 16 j; their
+// periodic grid offsets are computed in registers (no LDS tables).
+template <int P, int BM>
+__device__ __forceinline__ void window_loads(float4 (&v)[((BM + P) * (BM + P) + 15) / 16], const float4* __restrict__ F,
+                                             int o0, int o1, int o2, int e0, int e1, int e2, int nx, int ny, int nz)
+{
+    constexpr int TP = BM + P, shift = P / 2, RMAX = (TP * TP + 15) / 16;
+    const int lane16 = threadIdx.x & 15, row16 = threadIdx.x >> 4;
+    const int nrows = e1 * e2;
+    const int gx = wrap(o0 - shift + min(lane16, e0 - 1), nx);
+#pragma unroll
+    for (int j = 0; j < RMAX; ++j) {
+        const int r = min(row16 + 16 * j, nrows - 1);
+        const int t1 = r % e1, t2 = r / e1;
+        const long off = gx + (long)nx * (wrap(o1 - shift + t1, ny) + (long)ny * wrap(o2 - shift + t2, nz));
+        v[j] = F[off];
+    }
+}
+
+template <int P, int BM>
+__global__ void __launch_bounds__(kTileThreads) k_gather_species_pipe(
+    float4* __restrict__ fgrid, const float4* __restrict__ F4, GatherMap gm, const float4* __restrict__ spos,
+    const int2* __restrict__ smeta, const int* __restrict__ start, TileGeom tg, int nx, int ny, int nz, float dx0,
+    float dx1, float dx2, int ntasks, int per)
+{
+    constexpr int TP = BM + P, T3 = TP * TP * TP, RMAX = (TP * TP + 15) / 16;
+    __shared__ Win3 ft[T3];
+    const int chunk = brick_of_block(blockIdx.x, gridDim.x);
+    const int t0 = chunk * per, t1 = min(t0 + per, ntasks);
+    const long M = (long)nx * ny * nz;
+    const int lane16 = threadIdx.x & 15, row16 = threadIdx.x >> 4;
+    const float i0 = 1.0f / dx0, i1 = 1.0f / dx1, i2 = 1.0f / dx2;
+    // the next task that needs a window (has particles and a force field)
+    auto next_window = [&](int t) {
+        for (; t < t1; ++t) {
+            const int brick = t / tg.nspec, sp = t - brick * tg.nspec;
+            const int key0 = (brick * tg.nspec + sp) << 6;
+            if (start[key0] != start[key0 + 64] && gm.slot[sp] >= 0) return t;
+        }
+        return t1;
+    };
+    float4 v[RMAX];
+    int tw = next_window(t0);
+    auto geom = [&](int t, int* o, int* e) {
+        const int brick = t / tg.nspec;
+        const int k0 = brick % tg.nb[0], k1 = (brick / tg.nb[0]) % tg.nb[1], k2 = brick / (tg.nb[0] * tg.nb[1]);
+        o[0] = k0 << tg.sb[0];
+        o[1] = k1 << tg.sb[1];
+        o[2] = k2 << tg.sb[2];
+        e[0] = min(tg.b[0], nx - o[0]) + P;
+        e[1] = min(tg.b[1], ny - o[1]) + P;
+        e[2] = min(tg.b[2], nz - o[2]) + P;
+    };
+    if (tw < t1) {
+        int o[3], e[3];
+        geom(tw, o, e);
+        window_loads<P, BM>(v, F4 + gm.slot[tw % tg.nspec] * M, o[0], o[1], o[2], e[0], e[1], e[2], nx, ny, nz);
+    }
+    for (int t = t0; t < t1; ++t) {
+        const int brick = t / tg.nspec, sp = t - brick * tg.nspec;
+        const int key0 = (brick * tg.nspec + sp) << 6;
+        const int pb = start[key0], pe = start[key0 + 64];
+        if (pb == pe) continue;
+        if (gm.slot[sp] < 0) {
+            for (int it = pb + (int)threadIdx.x; it < pe; it += kTileThreads)
+                fgrid[it] = make_float4(0.f, 0.f, 0.f, 0.f);
+            continue;
+        }
+        // t == tw: its window is in v
+        int o[3], e[3];
+        geom(t, o, e);
+        __syncthreads();   // the previous window's sums are done
+        if (lane16 < e[0]) {
+            const int nrows = e[1] * e[2];
+#pragma unroll
+            for (int j = 0; j < RMAX; ++j) {
+                const int r = row16 + 16 * j;
+                if (r < nrows) {
+                    const int t1r = r % e[1], t2r = r / e[1];
+                    ft[lane16 + TP * (t1r + TP * t2r)] = Win3{v[j].x, v[j].y, v[j].z};
+                }
+            }
+        }
+        __syncthreads();
+        tw = next_window(t + 1);
+        if (tw < t1) {
+            int on[3], en[3];
+            geom(tw, on, en);
+            window_loads<P, BM>(v, F4 + gm.slot[tw % tg.nspec] * M, on[0], on[1], on[2], en[0], en[1], en[2], nx,
+                                ny, nz);
+        }
+        for (int it = pb + (int)threadIdx.x; it < pe; it += kTileThreads) {
+            const float4 pp = spos[it];
+            float wx[P + 1], wy[P + 1], wz[P + 1];
+            int l[3];
+            float3 a = make_float3(0.f, 0.f, 0.f);
+            if (stencil_in_brick<P>(pp, dx0, dx1, dx2, i0, i1, i2, nx, ny, nz, o[0], o[1], o[2], tg, wx, wy, wz, l))
+                a = stencil_sum<P, BM>(ft, l, wx, wy, wz);
+            fgrid[it] = make_float4(a.x, a.y, a.z, 0.f);
+        }
+    }
+}
+
 // Brick variant: one workgroup per brick stages the force windows of every
 // species with a force field (the window tables built once) and writes the
 // grid forces of all the brick's particles, whatever their species; fewer,
@@ -1501,11 +1607,47 @@ static size_t gather_brick_lds(const TildEngine* e)
 bool tild_gather_brick_fits(const TildEngine* e) { return gather_brick_lds(e) <= 64 * 1024; }
 
 // gather variants: 0 = direct loads in key order, 1 = LDS windows per (brick,
-// species), 2 = LDS windows of every species per brick
+// species), 2 = LDS windows of every species per brick, 3 = as 1 with each
+// workgroup walking a run of (brick, species) tasks, the next window's loads
+// in flight during the current sums
 int tild_gather(TildEngine* e, int mode)
 {
     const mft_tild_desc& d = e->d;
     if (mode == 2 && !tild_gather_brick_fits(e)) mode = 1;
+    if (mode == 3) {
+        const TileGeom& tg = e->tg;
+        const int BM = 1 << tg.lbmax;
+        GatherMap gm;
+        for (int sp = 0; sp < kMaxSpecies; ++sp) gm.slot[sp] = sp < d.nspecies ? e->force_slot_h[sp] : -1;
+        const int ntasks = tg.nbricks * tg.nspec;
+        const int nwg = std::min(ntasks, 256 * 8);
+        const int per = (ntasks + nwg - 1) / nwg;
+        const int grid = (ntasks + per - 1) / per;
+        const float4* Fq = e->F4 + (size_t)(e->qslot >= 0 ? e->qslot : 0) * e->M;
+#define MFT_G3(P_, B_)                                                                                    \
+    do {                                                                                                  \
+        k_gather_species_pipe<P_, B_><<<grid, kTileThreads, 0, e->stream>>>(                              \
+            e->fgrid, e->F4, gm, e->X[e->cur], e->META[e->cur], e->start, tg, d.nx, d.ny, d.nz, e->dx[0], e->dx[1], \
+            e->dx[2], ntasks, per);                                                                       \
+        if (e->qslot >= 0)                                                                                \
+            k_gather_charge<P_, B_><<<tg.nbricks, kTileThreads, 0, e->stream>>>(                          \
+                e->fgrid, Fq, e->X[e->cur], e->META[e->cur], e->start, tg, d.nx, d.ny, d.nz, e->dx[0], e->dx[1], e->dx[2]); \
+    } while (0)
+#define MFT_G3_B(P_)                                                                                      \
+    if (BM == 8) MFT_G3(P_, 8);                                                                           \
+    else MFT_G3(P_, 4);
+        switch (d.pmeorder) {
+        case 0: MFT_G3_B(0); break;
+        case 1: MFT_G3_B(1); break;
+        case 2: MFT_G3_B(2); break;
+        case 3: MFT_G3_B(3); break;
+        default: MFT_G3_B(4); break;
+        }
+#undef MFT_G3_B
+#undef MFT_G3
+        MFT_LAUNCH_CHECK();
+        return MFT_OK;
+    }
     if (mode == 2) {
         const TileGeom& tg = e->tg;
         const int BM = 1 << tg.lbmax;

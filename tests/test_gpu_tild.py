@@ -403,12 +403,12 @@ def test_engine_edge_cases(gpu, case):
     eng.close()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("case", ["neutral", "charged", "mixed_charges", "ragged_grid"])
 def test_gather_variants_match_oracle(gpu, case, variant):
     """The gather variants (direct loads, LDS windows per brick and species,
-    LDS windows of every species per brick) against the oracle's forces and a
-    short trajectory."""
+    LDS windows of every species per brick, pipelined windows per brick and
+    species) against the oracle's forces and a short trajectory."""
     if case == "mixed_charges":
         cfg = mixed_charge_system()
     elif case == "ragged_grid":
