@@ -390,6 +390,7 @@ def main():
     M = int(cfg["Nx"][0] * cfg["Nx"][1] * cfg["Nx"][2])
     eng = systems.tild_engine(cfg, "GJF")
     tuned = eng.autotune()
+    tune_ms = eng.autotune_timings()
     eng.step(args.warmup)
     eng.synchronize()
 
@@ -449,7 +450,8 @@ def main():
         "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step_ms.items()},
         "autotune": {"gather": ["direct", "lds_brick_species", "lds_brick", "lds_pipelined"][tuned & 3],
                      "xy_passes": ["per_axis", "per_plane"][(tuned >> 2) & 1],
-                     "x_fwd": ["direct_tiles", "staged_tile_rows"][(tuned >> 3) & 1]},
+                     "x_fwd": ["direct_tiles", "staged_tile_rows"][(tuned >> 3) & 1],
+                     "timings_ms": tune_ms},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes": kbytes[dom],
                      "avg_launch_ms": avg_ms[dom], "traffic": traffic},

@@ -264,6 +264,12 @@ int mft_tild_synchronize(mft_tild* eng);
  * and bond energy (x 1/2, PS_Box.cu:292, with the reference's 0.5*Lh minimum
  * image of ps_deviceBonds.cu:132-133). */
 int mft_tild_energies(mft_tild* eng, double* pot_energy, double* bond_energy);
+/* mft_tild_energies into device memory, without synchronising: d_out
+ * [npotentials + 1] receives, on the engine stream, the pair-potential
+ * energies (FP64; zero for bias, wall, DPD and Langevin entries) and, last,
+ * the bond energy (PS_Box::computeThermoProps, PS_Box.cu:282-292).  The
+ * caller reads it back when it needs it (mft_tild_stream orders the copy). */
+int mft_tild_energies_device(mft_tild* eng, double* d_out);
 /* Virial part of d_bondStressEnergy (ps_deviceBonds.cu:85-171) at the current
  * state: vir[6N] (host, particle-id order, the layout of the reference's
  * d_bondVirScratch), per particle the terms -mf dr_a dr_b over its bonds as
@@ -304,6 +310,10 @@ int mft_tild_num_outputs(mft_tild* eng, int* nout);
  * rounding), bit 3 the forward x pass (0 tile entries read directly, 1 tile
  * rows staged in LDS first; the same bits). */
 int mft_tild_autotune(mft_tild* eng, int reps, int* choice);
+/* The last autotune's timings, ms[n] (up to 8; -1 where not timed): gather
+ * variants 0-3 (ms per launch), then x-y passes per axis / per plane and the
+ * forward x pass direct / staged (ms per grid-force evaluation). */
+int mft_tild_autotune_timings(mft_tild* eng, double* ms, int n);
 /* Selects a gather variant directly (as above; the default is 1). */
 int mft_tild_set_gather(mft_tild* eng, int variant);
 /* Selects the x-y transform passes directly: 0 one pass per axis (default),

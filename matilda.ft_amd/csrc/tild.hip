@@ -852,6 +852,51 @@ int mft_tild_energies(mft_tild* e, double* pot_energy, double* bond_energy)
     return MFT_OK;
 }
 
+} // extern "C"
+
+namespace mft {
+// device-buffer energies: the pair-potential entries (zero for bias, wall,
+// DPD and Langevin entries, which these do not cover) and the bond energy
+// with the 1/2 of PS_Box::computeThermoProps (PS_Box.cu:292)
+__global__ void k_finish_energies(double* __restrict__ out, const double* __restrict__ eout, int npot,
+                                  unsigned pair_mask)
+{
+    const int p = threadIdx.x;
+    if (p < npot) out[p] = ((pair_mask >> p) & 1u) ? eout[p] : 0.0;
+    if (p == 0) out[npot] = 0.5 * eout[kMaxPots];
+}
+} // namespace mft
+
+extern "C" {
+
+int mft_tild_energies_device(mft_tild* e, double* d_out)
+{
+    MFT_REQUIRE(e && e->have_state && d_out, MFT_ERR_INVALID, "bad argument");
+    const mft_tild_desc& d = e->d;
+    int rc;
+    unsigned pair_mask = 0;
+    for (int p = 0; p < d.npotentials; ++p) {
+        const int k = e->pots[p].kind;
+        if (k == MFT_POT_GAUSSIAN || k == MFT_POT_ERF2 || k == MFT_POT_ERFG || k == MFT_POT_CHARGES)
+            pair_mask |= 1u << p;
+    }
+    if (d.npotentials > 0) {
+        int saved = d.integrator;
+        e->d.integrator = MFT_INT_NONE;
+        e->kp.energy = 1;
+        int64_t st = e->step;
+        rc = do_steps(e, 1, false, false);
+        e->kp.energy = 0;
+        e->d.integrator = saved;
+        e->step = st;
+        if (rc) return rc;
+    }
+    if ((rc = tild_bond_energy(e))) return rc;
+    k_finish_energies<<<1, 64, 0, e->stream>>>(d_out, e->eout, d.npotentials, pair_mask);
+    MFT_LAUNCH_CHECK();
+    return MFT_OK;
+}
+
 // energies of every kind except the pair potentials, at the current positions
 static int tail_energies(TildEngine* e, double* pot_energy, double* bond_energy)
 {
@@ -966,6 +1011,13 @@ int mft_tild_set_gather(mft_tild* e, int variant)
     return MFT_OK;
 }
 
+int mft_tild_autotune_timings(mft_tild* e, double* ms, int n)
+{
+    MFT_REQUIRE(e && ms && n >= 0, MFT_ERR_INVALID, "bad argument");
+    for (int i = 0; i < n; ++i) ms[i] = i < 8 ? e->tune_ms[i] : -1.0;
+    return MFT_OK;
+}
+
 int mft_tild_set_grid_passes(mft_tild* e, int variant)
 {
     MFT_REQUIRE(e, MFT_ERR_INVALID, "null argument");
@@ -1007,6 +1059,7 @@ int mft_tild_autotune(mft_tild* e, int reps, int* choice)
         MFT_HIP(hipEventSynchronize(b));
         float ms = 0.f;
         MFT_HIP(hipEventElapsedTime(&ms, a, b));
+        e->tune_ms[mode] = ms / reps;
         if (ms < best) {
             best = ms;
             best_mode = mode;
@@ -1028,6 +1081,7 @@ int mft_tild_autotune(mft_tild* e, int reps, int* choice)
             MFT_HIP(hipEventSynchronize(b));
             float ms = 0.f;
             MFT_HIP(hipEventElapsedTime(&ms, a, b));
+            e->tune_ms[4 + v] = ms / reps;
             if (ms < bestg) {
                 bestg = ms;
                 best_plane = v;
@@ -1049,6 +1103,7 @@ int mft_tild_autotune(mft_tild* e, int reps, int* choice)
             MFT_HIP(hipEventSynchronize(b));
             float ms = 0.f;
             MFT_HIP(hipEventElapsedTime(&ms, a, b));
+            e->tune_ms[6 + v] = ms / reps;
             if (ms < bests) {
                 bests = ms;
                 best_staged = v;

@@ -291,6 +291,10 @@ struct TildEngine {
     int plane_passes = 0;   // 1: x and y transforms fused per z-plane (mft_tild_autotune picks)
     int xfwd_staged = 0;    // 1: forward x pass stages its tile rows in LDS (mft_tild_autotune picks)
     int tile_maxcover = 4;  // most tiles covering one coordinate along any axis
+    // the autotuner's timings (ms per launch or per grid-force pass; -1 where
+    // not timed): gather variants 0-3, x-y per axis / per plane, x pass
+    // direct / staged
+    double tune_ms[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
     int64_t step = 0;
     int64_t run_max = 0, run_pos = 0; // declared run (mft_tild_set_run)
     int have_state = 0;

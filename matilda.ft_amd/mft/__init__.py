@@ -446,6 +446,15 @@ class TildEngine:
         mask = 0xFFFFFFFF if names is None else sum(1 << TK_NAMES.index(n) for n in names)
         check(lib().mft_tild_timing_select(self.h, C.c_uint32(mask)))
 
+    def energies_device(self):
+        """mft_tild_energies_device into a device buffer, read back afterwards:
+        (pair energies [npotentials], bond energy)."""
+        buf = DeviceBuffer(8 * (self.npot + 1))
+        check(lib().mft_tild_energies_device(self.h, buf.ptr))
+        v = buf.to_array(np.float64, (self.npot + 1,))
+        buf.free()
+        return v[:self.npot], float(v[self.npot])
+
     def bond_virial(self):
         """mft_tild_bond_virial: [N, 6] per-particle bond virial terms."""
         v = np.empty(6 * self.N, np.float32)
@@ -462,6 +471,14 @@ class TildEngine:
         c = _i(0)
         check(lib().mft_tild_autotune(self.h, reps, C.byref(c)))
         return c.value
+
+    def autotune_timings(self):
+        """mft_tild_autotune_timings: {variant: ms} of the last autotune."""
+        ms = np.zeros(8)
+        check(lib().mft_tild_autotune_timings(self.h, _ptr(ms), 8))
+        names = ["gather_direct", "gather_lds_brick_species", "gather_lds_brick", "gather_lds_pipelined",
+                 "xy_per_axis_pass", "xy_per_plane_pass", "x_fwd_direct_pass", "x_fwd_staged_pass"]
+        return {k: float(v) for k, v in zip(names, ms) if v >= 0}
 
     def kernel_bytes(self):
         b = np.zeros(len(TK_NAMES), np.float64)

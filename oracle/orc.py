@@ -48,7 +48,10 @@ def lib(bits: int = 64):
     key = (bits, _OPENMP["on"])
     if key in _LIBS:
         return _LIBS[key]
-    path = os.path.join(_HERE, f"liboracle{bits}{'_omp' if _OPENMP['on'] else ''}.so")
+    # ORC_SANITIZE=_asan selects the AddressSanitizer/UBSan build (the CPU
+    # sanitizer run loads it with libasan preloaded)
+    san = os.environ.get("ORC_SANITIZE", "")
+    path = os.path.join(_HERE, f"liboracle{bits}{'_omp' if _OPENMP['on'] else ''}{san}.so")
     if not os.path.exists(path):
         raise FileNotFoundError(f"{path} missing: build the oracle (make -C oracle)")
     L = C.CDLL(path)

@@ -235,6 +235,9 @@ def test_energies_match_oracle(gpu, case):
                               bondReq=cfg["bond_r0"], bondK=cfg["bond_k"], bondStyle=cfg["bond_style"], maxbonds=2),
                          sys_o.x, cfg["L"])
     assert be == pytest.approx(0.5 * eb.sum(), rel=TOL)
+    # the same energies through a device buffer, without a host round trip
+    pd, bd = eng.energies_device()
+    assert np.array_equal(pd, np.asarray(pe)) and bd == be
     eng.close()
 
 
