@@ -182,16 +182,25 @@ MFT_PLAN_LOW(256, 64, 4, 4, 4, 4)
 
 // LDS addressing of element e of local line l: lines contiguous (x axis) or
 // interleaved so that neighbouring lanes hit neighbouring words (y, z axes).
+// swz (contiguous lines): element e sits at e + e/8 within a line pitch of at
+// least swz_pitch(N).  A Stockham exchange writes element R j + r from thread
+// j, so without the skew the lanes of one write land R elements (2 R words)
+// apart and all lines start on the same bank: a few banks take every word.
+// One skipped word every eight elements spreads those lanes over the banks.
 struct LdsMap {
     int line;
     int nlines; // lines per workgroup (B)
     bool interleaved;
     int n;
+    bool swz = false;
     __device__ __forceinline__ int operator()(int e) const
     {
-        return interleaved ? e * nlines + line : line * n + e;
+        return interleaved ? e * nlines + line : line * n + (swz ? e + (e >> 3) : e);
     }
 };
+
+// line pitch that holds a skewed line of N elements
+__host__ __device__ constexpr int swz_pitch(int N) { return N + (N + 7) / 8; }
 
 // Exchange ordering.  When every thread of a line sits in one wavefront
 // (T consecutive threads, T <= 64, not interleaved) the exchange only orders

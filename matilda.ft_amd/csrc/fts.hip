@@ -377,13 +377,20 @@ __global__ void __launch_bounds__(256) k_fts_prop_axis(const PropDesc* __restric
     int ta, tb;
     const long base = valid ? geom.base(line, &ta, &tb) : 0;
     const long st = geom.stride();
-    float2 v[E];
+    float2 v[E], pv[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) {
         long o = base + (long)(t + T * s) * st;
         float2 a = valid ? D.src[o] : make_float2(0.f, 0.f);
         if (D.pre && valid) a = cmul(a, D.pre[o]);
         v[s] = a;
+    }
+    // the post factors (E = exp(-W) on the last axis) are loaded with the
+    // line, so their latency overlaps the transforms instead of following them
+    if (D.post) {
+#pragma unroll
+        for (int s = 0; s < E; ++s)
+            pv[s] = valid ? D.post[base + (long)(t + T * s) * st] : make_float2(1.f, 0.f);
     }
     __syncthreads();
     LdsMap map{lline, B, inter, N};
@@ -401,7 +408,7 @@ __global__ void __launch_bounds__(256) k_fts_prop_axis(const PropDesc* __restric
     for (int s = 0; s < E; ++s) {
         long o = base + (long)(t + T * s) * st;
         float2 a = v[s];
-        if (D.post) a = cmul(a, D.post[o]);
+        if (D.post) a = cmul(a, pv[s]);
         D.dst[o] = a;
     }
 }
@@ -473,7 +480,9 @@ __global__ void __launch_bounds__(NT) k_fts_prop_xy(const PropDesc* __restrict__
 #pragma unroll
             for (int s = 0; s < E; ++s) v[s] = valid ? plane[row * RS + t + T * s] : make_float2(0.f, 0.f);
             float2* base = valid ? plane : scratch;
-            LdsMap map{valid ? row : lx, N, false, valid ? RS : N};
+            // skewed exchanges within the padded row (RS >= swz_pitch(N) when
+            // PAD is 8, N <= 64)
+            LdsMap map{valid ? row : lx, N, false, valid ? RS : N, valid && RS >= swz_pitch(N)};
             line_fft<N, -1, true>(v, base, tw, t, map);
 #pragma unroll
             for (int s = 0; s < E; ++s) {

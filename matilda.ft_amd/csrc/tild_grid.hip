@@ -62,7 +62,8 @@ __global__ void __launch_bounds__(256) k_fft_x_inv_packed(const float2* __restri
 {
     using P = FftPlan<N>;
     constexpr int T = P::T, E = P::E, B = 256 / T;
-    __shared__ float2 lds[B * N];
+    constexpr int NP = swz_pitch(N);
+    __shared__ float2 lds[B * NP];
     __shared__ float2 tw[N];
     for (int i = threadIdx.x; i < N; i += 256) tw[i] = twg[i];
     const int lline = threadIdx.x / T;
@@ -83,7 +84,7 @@ __global__ void __launch_bounds__(256) k_fft_x_inv_packed(const float2* __restri
     }
     __syncthreads();
     // a line's T threads are consecutive threads of one wavefront
-    LdsMap map{lline, B, false, N};
+    LdsMap map{lline, B, false, NP, true};
 #pragma unroll
     for (int j = 0; j < 3; ++j) line_fft<N, +1, true>(pk[j], lds, tw, t, map);
     if (!valid) return;
@@ -113,7 +114,8 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd_tiles(const float* __restrict
 {
     using P = FftPlan<N>;
     constexpr int T = P::T, E = P::E, B = 256 / T;
-    __shared__ float2 lds[B * N];
+    constexpr int NP = swz_pitch(N);
+    __shared__ float2 lds[B * NP];
     __shared__ float2 tw[N];
     __shared__ int tabx[5 * N];
     for (int i = threadIdx.x; i < N; i += 256) tw[i] = twg[i];
@@ -190,7 +192,7 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd_tiles(const float* __restrict
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const int i = threadIdx.x + 256 * k;
-        if (i < B * N) lds[i] = make_float2(sums[k], 0.f);
+        if (i < B * N) lds[(i / N) * NP + i % N] = make_float2(sums[k], 0.f);
     }
     __syncthreads();
     const int lline = threadIdx.x / T;
@@ -198,8 +200,9 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd_tiles(const float* __restrict
     const int line = blockIdx.x * B + lline;
     float2 v[E];
 #pragma unroll
-    for (int s = 0; s < E; ++s) v[s] = lds[lline * N + t + T * s];
-    LdsMap map{lline, B, false, N};
+    for (int s = 0; s < E; ++s) v[s] = lds[lline * NP + t + T * s];
+    // lines NP apart: each line's exchanges stay inside its own region
+    LdsMap map{lline, B, false, NP, true};
     line_fft<N, -1, true>(v, lds, tw, t, map);
     if (line >= nlines) return;
     float2* dst = out + c * (long)N * nlines + (long)line * N;

@@ -306,23 +306,32 @@ __global__ void k_scan_apply(int* __restrict__ count, int M, const int* __restri
     for (int j = threadIdx.x; j < (int)blockIdx.x; j += blockDim.x) c += partial[j];
     int carry;
     block_exclusive_scan(c, sh, &carry);
-    long base = (long)blockIdx.x * kScanChunk + (long)threadIdx.x * kScanItems;
-    int v[kScanItems];
+    // M is a multiple of 64 (64 sub-cell keys per brick and species), so a
+    // thread's 16 keys are all in range or all past it: int4 loads and stores
+    static_assert(kScanItems == 16, "four int4 per thread");
+    const long base = (long)blockIdx.x * kScanChunk + (long)threadIdx.x * kScanItems;
+    const bool in = base < M;
+    int4 v4[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        v4[k] = in ? reinterpret_cast<const int4*>(count + base)[k] : make_int4(0, 0, 0, 0);
     int s = 0;
-    for (int k = 0; k < kScanItems; ++k) {
-        long i = base + k;
-        v[k] = i < M ? count[i] : 0;
-        s += v[k];
-    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s += v4[k].x + v4[k].y + v4[k].z + v4[k].w;
     int tot;
     int ex = block_exclusive_scan(s, sh, &tot) + carry;
-    for (int k = 0; k < kScanItems; ++k) {
-        long i = base + k;
-        if (i < M) {
-            start[i] = ex;
-            count[i] = 0;
+    if (in) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            int4 o;
+            o.x = ex;
+            o.y = o.x + v4[k].x;
+            o.z = o.y + v4[k].y;
+            o.w = o.z + v4[k].z;
+            ex = o.w + v4[k].w;
+            reinterpret_cast<int4*>(start + base)[k] = o;
+            reinterpret_cast<int4*>(count + base)[k] = make_int4(0, 0, 0, 0);
         }
-        ex += v[k];
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         start[M] = ns;
@@ -423,7 +432,7 @@ template <int P, int BM, typename ACC>
 __device__ __forceinline__ void deposit_tile(ACC* acc, int qchan, const float4* __restrict__ spos,
                                              const int2* __restrict__ smeta, int pb, int n, float sc, float dx0,
                                              float dx1, float dx2, int nx, int ny, int nz, int o0, int o1, int o2,
-                                             const TileGeom& tg)
+                                             const TileGeom& tg, float4 pp0, int typ0)
 {
     constexpr int TP = BM + P, T3 = TP * TP * TP;
     const float i0 = 1.0f / dx0, i1 = 1.0f / dx1, i2 = 1.0f / dx2;
@@ -435,8 +444,9 @@ __device__ __forceinline__ void deposit_tile(ACC* acc, int qchan, const float4* 
         const int r = (j & 63) * S + (j >> 6);
         if (r >= n) continue;
         const int it = pb + r;
-        const float4 pp = spos[it];
-        const int typ = smeta[it].y;
+        const bool first = j == (int)threadIdx.x; // loaded by the caller
+        const float4 pp = first ? pp0 : spos[it];
+        const int typ = first ? typ0 : smeta[it].y;
         float wx[P + 1], wy[P + 1], wz[P + 1];
         int l[3];
         if (!stencil_in_brick<P>(pp, dx0, dx1, dx2, i0, i1, i2, nx, ny, nz, o0, o1, o2, tg, wx, wy, wz, l)) continue;
@@ -486,6 +496,13 @@ __global__ void __launch_bounds__(kTileThreads) k_density_tile(
     const int n = pe - pb;
     // largest power-of-two scale 2^sc with n (w_max 2^sc + 1/2) < 2^32, the
     // bound of one point's sum of rounded deposits (checked for sc >= 16)
+    // the first pass's particle is loaded before the tile is cleared, so its
+    // latency overlaps the clearing and the barrier (which waits for LDS only)
+    const int S0 = (n + 63) >> 6;
+    const int r0 = ((int)threadIdx.x & 63) * S0 + ((int)threadIdx.x >> 6);
+    const bool has0 = (int)threadIdx.x < 64 * S0 && r0 < n;
+    const float4 pp0 = has0 ? spos[pb + r0] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int typ0 = has0 ? smeta[pb + r0].y : 0;
     const float bound = (float)n * (stencil_wmax(P) + 1.0f / 65536.0f) + 1.0f;
     int need = 1;
     while (need < 31 && (float)(1u << need) <= bound) ++need;
@@ -503,7 +520,7 @@ __global__ void __launch_bounds__(kTileThreads) k_density_tile(
         for (int i = threadIdx.x; i < nchan * T3; i += kTileThreads) acc32[i] = 0u;
         __syncthreads();
         deposit_tile<P, BM, unsigned int>(acc32, qchan, spos, smeta, pb, n, (float)(1u << sc), dx0, dx1, dx2, nx,
-                                          ny, nz, o0, o1, o2, tg);
+                                          ny, nz, o0, o1, o2, tg, pp0, typ0);
         __syncthreads();
         const double scale = inv_gvol / (double)(1u << sc);
         for (int i = threadIdx.x; i < nchan * T3; i += kTileThreads) *out(i) = (float)((double)acc32[i] * scale);
@@ -511,7 +528,7 @@ __global__ void __launch_bounds__(kTileThreads) k_density_tile(
         for (int i = threadIdx.x; i < nchan * T3; i += kTileThreads) acc64[i] = 0ull;
         __syncthreads();
         deposit_tile<P, BM, unsigned long long>(acc64, qchan, spos, smeta, pb, n, 2147483648.0f, dx0, dx1, dx2, nx,
-                                                ny, nz, o0, o1, o2, tg);
+                                                ny, nz, o0, o1, o2, tg, pp0, typ0);
         __syncthreads();
         const double scale = inv_gvol / 2147483648.0;
         for (int i = threadIdx.x; i < nchan * T3; i += kTileThreads)
@@ -722,10 +739,14 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_species(
     const int o0 = k0 << tg.sb[0], o1 = k1 << tg.sb[1], o2 = k2 << tg.sb[2];
     const int e0 = min(tg.b[0], nx - o0) + P, e1 = min(tg.b[1], ny - o1) + P, e2 = min(tg.b[2], nz - o2) + P;
     const long M = (long)nx * ny * nz;
+    // the first pass's particle is loaded before the window is staged, so its
+    // latency overlaps the window loads (the barriers wait for LDS only)
+    const int it0 = pb + (int)threadIdx.x;
+    const float4 pp0 = it0 < pe ? spos[it0] : make_float4(0.f, 0.f, 0.f, 0.f);
     stage_window<P, BM>(ft, xg, yo, zo, rows, F4 + f * M, o0, o1, o2, e0, e1, e2, nx, ny, nz);
     const float i0 = 1.0f / dx0, i1 = 1.0f / dx1, i2 = 1.0f / dx2;
-    for (int it = pb + (int)threadIdx.x; it < pe; it += kTileThreads) {
-        const float4 pp = spos[it];
+    for (int it = it0; it < pe; it += kTileThreads) {
+        const float4 pp = it == it0 ? pp0 : spos[it];
         float wx[P + 1], wy[P + 1], wz[P + 1];
         int l[3];
         float3 a = make_float3(0.f, 0.f, 0.f);
@@ -1223,7 +1244,10 @@ __global__ void __launch_bounds__(256) k_integrate(
     float4* __restrict__ ftot, int maxbonds, BondParams bp, StepArgs A, int ns, int* __restrict__ count,
     int* __restrict__ key, int* __restrict__ rank, TileGeom tg)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    // contiguous slot ranges per XCD (brick_of_block): bond partners, mostly
+    // a few slots away in key order, are read from the L2 that holds the
+    // block's own slots rather than from another XCD's lines
+    const int s = brick_of_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool active = s < ns;
     const uint32_t step = A.dstep ? *A.dstep : A.step;
     int k = 0;
