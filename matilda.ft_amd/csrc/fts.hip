@@ -1268,10 +1268,12 @@ template <int N>
 static void launch_prop_xy_n(const PropDesc* dx, const PropDesc* dy, int n, const FtsGroup& G, hipStream_t st)
 {
     dim3 grid(G.n[2], n);
-    // enough threads for every row (and column) of the plane in one round;
-    // 48-point planes take exactly 48 lines (192 threads), so no line needs
-    // the scratch region and more workgroups fit in LDS
-    constexpr int NT = N == 48 ? 192 : (N * FftPlan<N>::T > 256 ? 512 : 256);
+    // enough threads for every row (and column) of the plane in one round,
+    // in whole wavefronts: 48-point planes take exactly 48 lines (192
+    // threads), so no line needs the scratch region and more workgroups fit in
+    // LDS; small planes take one or two wavefronts rather than 256 threads of
+    // which most would transform scratch lines
+    constexpr int NT = N == 48 ? 192 : (N * FftPlan<N>::T > 256 ? 512 : std::max(64, (N * FftPlan<N>::T + 63) / 64 * 64));
     k_fts_prop_xy<N, NT><<<grid, NT, 0, st>>>(dx, dy, fft_twiddles_device(N));
 }
 
