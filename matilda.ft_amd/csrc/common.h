@@ -10,6 +10,18 @@
 
 namespace mft {
 
+// Block index -> work item such that each of the 8 XCDs (blocks are
+// dispatched to them round-robin) takes one contiguous slab of the n items:
+// neighbouring items share that XCD's L2 and address translations.
+__device__ __forceinline__ int xcd_slab(int blk, int n)
+{
+    const int per = (n + 7) >> 3;
+    const int x = blk & 7, j = blk >> 3;
+    const int full = n - 8 * (per - 1); // XCDs with `per` items
+    const int base = x < full ? x * per : full * per + (x - full) * (per - 1);
+    return base + j;
+}
+
 // ----------------------------------------------------------- status ------
 void set_error(int code, const char* fmt, ...);
 int last_code();

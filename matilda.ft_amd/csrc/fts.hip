@@ -547,33 +547,87 @@ struct DensDesc {
     int species[kFtsMaxBlocks];
 };
 
+// Points per thread of k_fts_density: two (float4 loads of the propagators)
+// when the grid has an even number of points, so every pair is aligned.
+__host__ __device__ constexpr int fts_density_points(long M) { return (M & 1) == 0 ? 2 : 1; }
+
+template <int PTS>
+__device__ __forceinline__ void load_pts(const float2* __restrict__ p, float2 (&v)[PTS])
+{
+    if constexpr (PTS == 2) {
+        const float4 q = *reinterpret_cast<const float4*>(p);
+        v[0] = make_float2(q.x, q.y);
+        v[1] = make_float2(q.z, q.w);
+    } else {
+        v[0] = *p;
+    }
+}
+
 // rho_b(r) = e^{+W_b(r)} sum_{s in b} q_s(r) qdag_{Ntot-1-s}(r)
 // (fts_molecule_linear.cu:298-327) and the partial sums of Q = sum q_{Ntot-1}
 // (fts_molecule_linear.cu:265-266).
+// PTS consecutive points per thread; the beads of a block are taken four at
+// a time with all eight propagator loads issued before the sums, which run in
+// bead order for each point as before.
+template <int PTS>
 __global__ void __launch_bounds__(kFtsThreads) k_fts_density(const DensDesc* __restrict__ descs)
 {
+    const int bx = blockIdx.x;
     const DensDesc& D = descs[blockIdx.y];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (bx * blockDim.x + threadIdx.x) * PTS;
     double qr = 0.0, qi = 0.0;
     if (i < D.M) {
+        const long M = D.M;
         int s = 0;
         for (int b = 0; b < D.nblocks; ++b) {
-            double ar = 0.0, ai = 0.0;
-            for (int k = 0; k < D.N[b]; ++k, ++s) {
-                float2 a = D.p[(long)s * D.M + i];
-                float2 c = D.pd[(long)(D.Ntot - 1 - s) * D.M + i];
-                ar += (double)a.x * c.x - (double)a.y * c.y;
-                ai += (double)a.x * c.y + (double)a.y * c.x;
+            double ar[PTS], ai[PTS];
+#pragma unroll
+            for (int j = 0; j < PTS; ++j) ar[j] = ai[j] = 0.0;
+            const int nb = D.N[b];
+            int k = 0;
+            for (; k + 4 <= nb; k += 4, s += 4) {
+                float2 a[4][PTS], c[4][PTS];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    load_pts<PTS>(D.p + (long)(s + u) * M + i, a[u]);
+                    load_pts<PTS>(D.pd + (long)(D.Ntot - 1 - s - u) * M + i, c[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int j = 0; j < PTS; ++j) {
+                        ar[j] += (double)a[u][j].x * c[u][j].x - (double)a[u][j].y * c[u][j].y;
+                        ai[j] += (double)a[u][j].x * c[u][j].y + (double)a[u][j].y * c[u][j].x;
+                    }
             }
-            const float2 w = D.W[(long)D.species[b] * D.M + i];
-            double sn, cs;
-            sincos((double)w.y, &sn, &cs);
-            const double e = exp((double)w.x), er = e * cs, ei = e * sn;
-            D.rhot[(long)b * D.M + i] = make_float2((float)(ar * er - ai * ei), (float)(ar * ei + ai * er));
+            for (; k < nb; ++k, ++s) {
+                float2 a[PTS], c[PTS];
+                load_pts<PTS>(D.p + (long)s * M + i, a);
+                load_pts<PTS>(D.pd + (long)(D.Ntot - 1 - s) * M + i, c);
+#pragma unroll
+                for (int j = 0; j < PTS; ++j) {
+                    ar[j] += (double)a[j].x * c[j].x - (double)a[j].y * c[j].y;
+                    ai[j] += (double)a[j].x * c[j].y + (double)a[j].y * c[j].x;
+                }
+            }
+            float2 w[PTS];
+            load_pts<PTS>(D.W + (long)D.species[b] * M + i, w);
+#pragma unroll
+            for (int j = 0; j < PTS; ++j) {
+                double sn, cs;
+                sincos((double)w[j].y, &sn, &cs);
+                const double e = exp((double)w[j].x), er = e * cs, ei = e * sn;
+                D.rhot[(long)b * M + i + j] =
+                    make_float2((float)(ar[j] * er - ai[j] * ei), (float)(ar[j] * ei + ai[j] * er));
+            }
         }
-        const float2 pl = D.p[(long)(D.Ntot - 1) * D.M + i];
-        qr = pl.x;
-        qi = pl.y;
+        float2 pl[PTS];
+        load_pts<PTS>(D.p + (long)(D.Ntot - 1) * M + i, pl);
+#pragma unroll
+        for (int j = 0; j < PTS; ++j) {
+            qr += pl[j].x;
+            qi += pl[j].y;
+        }
     }
     __shared__ double red[2 * (kFtsThreads / 64)];
     qr = wave_sum(qr);
@@ -589,8 +643,8 @@ __global__ void __launch_bounds__(kFtsThreads) k_fts_density(const DensDesc* __r
             a += red[2 * w];
             b += red[2 * w + 1];
         }
-        D.Qpart[2 * blockIdx.x] = a;
-        D.Qpart[2 * blockIdx.x + 1] = b;
+        D.Qpart[2 * bx] = a;
+        D.Qpart[2 * bx + 1] = b;
     }
 }
 
@@ -1142,7 +1196,9 @@ static int group_setup(FtsBatch* b, FtsGroup& G)
             dens.push_back(D);
             QDesc q;
             q.Qpart = D.Qpart;
-            q.nparts = r.nQpart;
+            // the density pass's blocks (its partial sums; nQpart covers one
+            // point per thread)
+            q.nparts = (int)((r.M / fts_density_points(r.M) + kFtsThreads - 1) / kFtsThreads);
             q.scale = r.gvol / r.V;
             q.Q = r.Q + m;
             qd.push_back(q);
@@ -1314,8 +1370,10 @@ static int group_density(FtsBatch* b, FtsGroup& G)
         }
     }
     const int M = G.n[0] * G.n[1] * G.n[2];
-    dim3 grid((M + kFtsThreads - 1) / kFtsThreads, G.ndens);
-    k_fts_density<<<grid, kFtsThreads, 0, st>>>(G.d_dens);
+    const int pts = fts_density_points(M);
+    dim3 grid((M / pts + kFtsThreads - 1) / kFtsThreads, G.ndens);
+    if (pts == 2) k_fts_density<2><<<grid, kFtsThreads, 0, st>>>(G.d_dens);
+    else k_fts_density<1><<<grid, kFtsThreads, 0, st>>>(G.d_dens);
     k_fts_qreduce<<<G.ndens, 256, 0, st>>>(G.d_q);
     MFT_LAUNCH_CHECK();
     return MFT_OK;

@@ -190,15 +190,7 @@ __device__ __forceinline__ float tile_sum(const float* __restrict__ tc, const in
     return s;
 }
 
-__device__ __forceinline__ int brick_of_block(int blk, int nbricks)
-{
-    const int per = (nbricks + 7) >> 3;
-    const int x = blk & 7, j = blk >> 3;
-    const int full = nbricks - 8 * (per - 1); // XCDs with `per` bricks
-    // blocks are numbered so XCD x handles bricks [base(x), base(x) + cnt(x))
-    const int base = x < full ? x * per : full * per + (x - full) * (per - 1);
-    return base + j;
-}
+__device__ __forceinline__ int brick_of_block(int blk, int nbricks) { return xcd_slab(blk, nbricks); }
 
 struct TildEngine {
     mft_tild_desc d;
