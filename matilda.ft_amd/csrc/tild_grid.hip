@@ -160,8 +160,13 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd_tiles(const float* __restrict
                 for (int b = 0; b < 2; ++b)
 #pragma unroll
                     for (int d = 0; d < 2; ++d) {
-                        const bool in = a < cnt[k][0] && b < cnt[k][1] && d < cnt[k][2];
-                        v[k][4 * a + 2 * b + d] = in ? tc[oz[k][a] + oy[k][b] + ox[k][d]] : 0.f;
+                        // entries a point lacks read its first entry instead (an
+                        // offset in range, on a line the wave reads anyway), so
+                        // every load is unconditional and none waits on a branch
+                        const bool in = (a < cnt[k][0]) & (b < cnt[k][1]) & (d < cnt[k][2]);
+                        const int off = in ? oz[k][a] + oy[k][b] + ox[k][d] : oz[k][0] + oy[k][0] + ox[k][0];
+                        const float t = tc[off];
+                        v[k][4 * a + 2 * b + d] = in ? t : 0.f;
                     }
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
@@ -172,7 +177,10 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd_tiles(const float* __restrict
                 for (int b = 0; b < 2; ++b)
 #pragma unroll
                     for (int d = 0; d < 2; ++d)
-                        if (a < cnt[k][0] && b < cnt[k][1] && d < cnt[k][2]) sum += v[k][4 * a + 2 * b + d];
+                    {
+                        const bool in = (a < cnt[k][0]) & (b < cnt[k][1]) & (d < cnt[k][2]);
+                        sum = in ? sum + v[k][4 * a + 2 * b + d] : sum;
+                    }
             sums[k] = sum;
         }
     } else {
