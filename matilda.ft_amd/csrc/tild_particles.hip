@@ -1012,8 +1012,9 @@ __device__ __forceinline__ void bond_force_slots(int s, int ns, const float4* __
 }
 
 // One bond of bond_force_slots with the partner's position given
-__device__ __forceinline__ void bond_force_pair(const BondParams& bp, int bt, float L0, float L1, float L2,
-                                                float4 x, float4 y, float* fo)
+// one bond's force with its type's parameters (style, k, r0) already loaded
+__device__ __forceinline__ void bond_force_params(int style, float kb, float r0, float L0, float L1, float L2,
+                                                  float4 x, float4 y, float* fo)
 {
     const float d0 = min_image(x.x - y.x, L0, 0.5f * L0);
     const float d1 = min_image(x.y - y.y, L1, 0.5f * L1);
@@ -1022,15 +1023,21 @@ __device__ __forceinline__ void bond_force_pair(const BondParams& bp, int bt, fl
     float mf = 0.f;
     if (r2 > 1.0e-4f) {
         const float r = sqrtf(r2);
-        if (bp.style[bt] == 0) mf = 2.0f * bp.k[bt] * (r - bp.r0[bt]) / r;
-        else if (bp.style[bt] == 1) {
-            const float u = r / bp.r0[bt];
-            mf = 2.0f * bp.k[bt] * r / (1.0f - u * u) / r;
+        if (style == 0) mf = 2.0f * kb * (r - r0) / r;
+        else if (style == 1) {
+            const float u = r / r0;
+            mf = 2.0f * kb * r / (1.0f - u * u) / r;
         }
     }
     fo[0] -= mf * d0;
     fo[1] -= mf * d1;
     fo[2] -= mf * d2;
+}
+
+__device__ __forceinline__ void bond_force_pair(const BondParams& bp, int bt, float L0, float L1, float L2,
+                                                float4 x, float4 y, float* fo)
+{
+    bond_force_params(bp.style[bt], bp.k[bt], bp.r0[bt], L0, L1, L2, x, y, fo);
 }
 
 // One angle of slot s (d_angles, ps_deviceAngles.cu:8-133): the force on
@@ -1132,7 +1139,6 @@ struct StepArgs {
     LangevinParams lang;
     int nz_keyed;              // GJF: previous noise regenerated from (particle, step - 1), NZ not read
     int write_nn;              // store the new noise in NN (off under GJF: the next step regenerates it)
-    const uint32_t* dstep;     // step index on the device (graph replays), else null: use step
 };
 
 // Wall forces of ps_potentialWall.cu:92-108 on a particle of species ty
@@ -1163,16 +1169,33 @@ __device__ __forceinline__ void langevin_forces(const LangevinParams& P, int id,
 // GJF (ps_integratorGJF.cu:102-163) or EM (ps_integratorEM.cu:39-82) update
 // of slot s from its total force fo: writes the next position and noise and
 // returns the next key.
+// Per-species integrator coefficients of one particle, read (per lane, so
+// from memory) before the integrator needs them: GJF a, b, noise magnitude,
+// force and noise factors; VV's 1/m in a.
+struct SpeciesCoef {
+    float a, b, nm, cf, cn;
+};
+
+__device__ __forceinline__ SpeciesCoef species_coef(const StepArgs& A, int ty)
+{
+    // every table read unconditionally (no branch between the loads and
+    // their use), the integrator's own picked after
+    const float a = A.gp.a[ty], b = A.gp.b[ty], nm = A.gp.nmag[ty], cf = A.gp.cf[ty], cn = A.gp.cn[ty];
+    const float im = A.gp.inv_m[ty];
+    const bool vv = A.integ == MFT_INT_VV;
+    return SpeciesCoef{vv ? im : a, b, nm, cf, cn};
+}
+
 __device__ __forceinline__ int integrate_slot(int s, float4 xc, int2 meta, const float* fo, const StepArgs& A,
-                                              uint32_t step, const TileGeom& tg, float4 xo4, float4 nz4)
+                                              uint32_t step, const TileGeom& tg, float4 xo4, float4 nz4,
+                                              const SpeciesCoef& sc)
 {
     float z[4];
     normal4((uint32_t)meta.x, step, TAG_TILD, 0u, A.seed, z);
     const float xcv[3] = {xc.x, xc.y, xc.z};
     float xn[3], nn[3] = {0.f, 0.f, 0.f};
     if (A.integ == MFT_INT_GJF) {
-        const int ty = meta.y;
-        const float a = A.gp.a[ty], b = A.gp.b[ty], nm = A.gp.nmag[ty], cf = A.gp.cf[ty], cn = A.gp.cn[ty];
+        const float a = sc.a, b = sc.b, nm = sc.nm, cf = sc.cf, cn = sc.cn;
         float nov[3];
         if (A.nz_keyed) {
             // the previous step's nm * normal, the same bits as when it was drawn
@@ -1204,7 +1227,7 @@ __device__ __forceinline__ int integrate_slot(int s, float4 xc, int2 meta, const
         // applies step k-1's Integrate_2, then step k's Integrate_1, and keeps
         // the half-kicked velocity.  Step 0 starts from rest with the zero force
         // array of a fresh allocation: neither kick nor drift changes anything.
-        const float inv_m = A.gp.inv_m[meta.y];
+        const float inv_m = sc.a;
         const float hd = A.delt * 0.5f;
         const float vv0[3] = {nz4.x, nz4.y, nz4.z};
         for (int j = 0; j < 3; ++j) {
@@ -1237,19 +1260,22 @@ __device__ __forceinline__ int integrate_slot(int s, float4 xc, int2 meta, const
 // Bonds (ps_deviceBonds.cu:7-79) plus the integrator on group "all", one
 // thread per slot of the key order.  Writes the next position and noise of
 // the slot and counts the next key; integ = MFT_INT_NONE only writes the
-// total force.
+// total force.  TWO: at most two bonds per particle (the slots' bond lists
+// are loaded with the slot's own records, so a slot waits for two rounds of
+// memory latency: its records, then its partners' positions).  dstep: the
+// step index on the device (graph replays), else null.
+template <bool TWO>
 __global__ void __launch_bounds__(256) k_integrate(
     const float4* __restrict__ X, const int2* __restrict__ META, const int* __restrict__ NB,
     const int* __restrict__ BS, const int* __restrict__ BT, const float4* __restrict__ fgrid,
     float4* __restrict__ ftot, int maxbonds, BondParams bp, StepArgs A, int ns, int* __restrict__ count,
-    int* __restrict__ key, int* __restrict__ rank, TileGeom tg)
+    int* __restrict__ key, int* __restrict__ rank, TileGeom tg, const uint32_t* __restrict__ dstep)
 {
     // contiguous slot ranges per XCD (brick_of_block): bond partners, mostly
     // a few slots away in key order, are read from the L2 that holds the
     // block's own slots rather than from another XCD's lines
     const int s = brick_of_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool active = s < ns;
-    const uint32_t step = A.dstep ? *A.dstep : A.step;
     int k = 0;
     if (active) {
         // every load that does not depend on another issued first, the bond
@@ -1261,18 +1287,32 @@ __global__ void __launch_bounds__(256) k_integrate(
         const bool need_nz = (gjf && !A.nz_keyed) || A.integ == MFT_INT_VV || A.use_v;
         const float4 xo4 = gjf ? A.XO[s] : make_float4(0.f, 0.f, 0.f, 0.f);
         const float4 nz4 = need_nz ? A.NZ[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+        int nb = 0;
+        int2 bs = make_int2(0, 0), bt = make_int2(0, 0);
+        if constexpr (TWO) {
+            nb = NB[s];
+            bs = reinterpret_cast<const int2*>(BS)[s];
+            bt = reinterpret_cast<const int2*>(BT)[s];
+        }
+        const uint32_t step = dstep ? *dstep : A.step;
         float fo[3] = {fg.x, fg.y, fg.z};
-        if (maxbonds == 2) {
-            const int nb = NB[s];
-            const int2 bs = reinterpret_cast<const int2*>(BS)[s];
-            const int2 bt = reinterpret_cast<const int2*>(BT)[s];
-            const bool v0 = nb > 0 && bs.x >= 0 && bs.x < ns && bt.x >= 0 && bt.x < bp.ntypes;
-            const bool v1 = nb > 1 && bs.y >= 0 && bs.y < ns && bt.y >= 0 && bt.y < bp.ntypes;
+        SpeciesCoef sco;
+        if constexpr (TWO) {
+            const bool v0 = (nb > 0) & (bs.x >= 0) & (bs.x < ns) & (bt.x >= 0) & (bt.x < bp.ntypes);
+            const bool v1 = (nb > 1) & (bs.y >= 0) & (bs.y < ns) & (bt.y >= 0) & (bt.y < bp.ntypes);
+            // the bond types' parameters (indexed per lane, so read from
+            // memory) are loaded with the partners' positions, not after them
+            const int t0 = v0 ? bt.x : 0, t1 = v1 ? bt.y : 0;
+            const int st0 = bp.style[t0], st1 = bp.style[t1];
+            const float k0 = bp.k[t0], k1 = bp.k[t1], q0 = bp.r0[t0], q1 = bp.r0[t1];
             const float4 y0 = X[v0 ? bs.x : s];
             const float4 y1 = X[v1 ? bs.y : s];
-            if (v0) bond_force_pair(bp, bt.x, A.L[0], A.L[1], A.L[2], xc, y0, fo);
-            if (v1) bond_force_pair(bp, bt.y, A.L[0], A.L[1], A.L[2], xc, y1, fo);
+            // the species' coefficients in the same round of loads
+            sco = species_coef(A, meta.y);
+            if (v0) bond_force_params(st0, k0, q0, A.L[0], A.L[1], A.L[2], xc, y0, fo);
+            if (v1) bond_force_params(st1, k1, q1, A.L[0], A.L[1], A.L[2], xc, y1, fo);
         } else {
+            sco = species_coef(A, meta.y);
             bond_force_slots(s, ns, X, NB, BS, BT, maxbonds, bp, A.L[0], A.L[1], A.L[2], xc.x, xc.y, xc.z, fo);
         }
         if (A.NA) angle_force_slots(s, X, A.NA, A.AT, A.AG, A.ap, A.L[0], A.L[1], A.L[2], fo);
@@ -1297,7 +1337,7 @@ __global__ void __launch_bounds__(256) k_integrate(
             langevin_forces(A.lang, meta.x, meta.y, v, A.seed, ls, fo);
         }
         if (ftot) ftot[s] = make_float4(fo[0], fo[1], fo[2], 0.f);
-        if (A.integ != MFT_INT_NONE) k = integrate_slot(s, xc, meta, fo, A, step, tg, xo4, nz4);
+        if (A.integ != MFT_INT_NONE) k = integrate_slot(s, xc, meta, fo, A, step, tg, xo4, nz4, sco);
     }
     if (A.integ == MFT_INT_NONE) return; // uniform over the grid
     const int r = wave_rank(count, k, active);
@@ -1783,7 +1823,6 @@ static StepArgs step_args(const TildEngine* e)
     A.FD = e->dpd.on ? e->fdpd : nullptr;
     A.nz_keyed = (d.integrator == MFT_INT_GJF && e->nz_keyed) ? 1 : 0;
     A.write_nn = d.integrator == MFT_INT_GJF ? 0 : 1;
-    A.dstep = e->use_dstep ? e->d_step : nullptr;
     return A;
 }
 
@@ -1791,9 +1830,17 @@ int tild_integrate(TildEngine* e, bool want_f)
 {
     const mft_tild_desc& d = e->d;
     const int c = e->cur;
-    k_integrate<<<nblk(e->N), kBlock, 0, e->stream>>>(e->X[c], e->META[c], e->NB[c], e->BS[c], e->BT[c], e->fgrid,
-                                                       want_f ? e->ftot : nullptr, std::max(1, d.maxbonds), e->bp,
-                                                       step_args(e), e->N, e->count, e->key, e->rank, e->tg);
+    const uint32_t* ds = e->use_dstep ? e->d_step : nullptr;
+    if (d.maxbonds == 2)
+        k_integrate<true><<<nblk(e->N), kBlock, 0, e->stream>>>(e->X[c], e->META[c], e->NB[c], e->BS[c], e->BT[c],
+                                                              e->fgrid, want_f ? e->ftot : nullptr, 2, e->bp,
+                                                              step_args(e), e->N, e->count, e->key, e->rank, e->tg,
+                                                              ds);
+    else
+        k_integrate<false><<<nblk(e->N), kBlock, 0, e->stream>>>(e->X[c], e->META[c], e->NB[c], e->BS[c], e->BT[c],
+                                                               e->fgrid, want_f ? e->ftot : nullptr,
+                                                               std::max(1, d.maxbonds), e->bp, step_args(e), e->N,
+                                                               e->count, e->key, e->rank, e->tg, ds);
     MFT_LAUNCH_CHECK();
     return MFT_OK;
 }
