@@ -424,56 +424,80 @@ __host__ __device__ constexpr float stencil_wmax(int p)
     return p <= 1 ? 1.0f : (p == 2 ? 0.43f : (p == 3 ? 0.30f : 0.22f));
 }
 
-// Deposit the brick's particles into the LDS tile.  ACC is the fixed-point
-// accumulator word: 32-bit with 2^sc scaling when n w_max 2^sc < 2^32 bounds
-// every sum, else 64-bit with 2^31 (and signed charges).  Integer sums do not
-// depend on the order of the atomics.
+// Passes of the deposit whose particles the caller loads up front (before the
+// tile is cleared), so their loads overlap the clearing and each other.
+constexpr int kDepositPre = 2;
+
 template <int P, int BM, typename ACC>
-__device__ __forceinline__ void deposit_tile(ACC* acc, int qchan, const float4* __restrict__ spos,
-                                             const int2* __restrict__ smeta, int pb, int n, float sc, float dx0,
-                                             float dx1, float dx2, int nx, int ny, int nz, int o0, int o1, int o2,
-                                             const TileGeom& tg, float4 pp0, int typ0)
+__device__ __forceinline__ void deposit_one(ACC* acc, int qchan, const float4& pp, int typ, float sc, float dx0,
+                                            float dx1, float dx2, float i0, float i1, float i2, int nx, int ny,
+                                            int nz, int o0, int o1, int o2, const TileGeom& tg)
 {
     constexpr int TP = BM + P, T3 = TP * TP * TP;
-    const float i0 = 1.0f / dx0, i1 = 1.0f / dx1, i2 = 1.0f / dx2;
-    // particles are in cell order, so neighbouring ones share stencil points:
-    // the 64 lanes of a wave take particles n/64 apart (a 64 x S transpose)
-    // so one atomic instruction rarely hits the same LDS word twice
-    const int S = (n + 63) >> 6;
-    for (int j = threadIdx.x; j < 64 * S; j += kTileThreads) {
-        const int r = (j & 63) * S + (j >> 6);
-        if (r >= n) continue;
-        const int it = pb + r;
-        const bool first = j == (int)threadIdx.x; // loaded by the caller
-        const float4 pp = first ? pp0 : spos[it];
-        const int typ = first ? typ0 : smeta[it].y;
-        float wx[P + 1], wy[P + 1], wz[P + 1];
-        int l[3];
-        if (!stencil_in_brick<P>(pp, dx0, dx1, dx2, i0, i1, i2, nx, ny, nz, o0, o1, o2, tg, wx, wy, wz, l)) continue;
-        ACC* a = acc + typ * T3 + l[0] + TP * (l[1] + TP * l[2]);
+    float wx[P + 1], wy[P + 1], wz[P + 1];
+    int l[3];
+    if (!stencil_in_brick<P>(pp, dx0, dx1, dx2, i0, i1, i2, nx, ny, nz, o0, o1, o2, tg, wx, wy, wz, l)) return;
+    ACC* a = acc + typ * T3 + l[0] + TP * (l[1] + TP * l[2]);
+#pragma unroll
+    for (int iz = 0; iz <= P; ++iz) {
+#pragma unroll
+        for (int iy = 0; iy <= P; ++iy) {
+            const float wyz = wy[iy] * wz[iz] * sc;
+#pragma unroll
+            for (int ix = 0; ix <= P; ++ix)
+                atomicAdd(a + ix + TP * (iy + TP * iz), (ACC)__float2uint_rz(fmaf(wx[ix], wyz, 0.5f)));
+        }
+    }
+    if (sizeof(ACC) == 8 && qchan >= 0 && pp.w != 0.f) {
+        ACC* aq = acc + qchan * T3 + l[0] + TP * (l[1] + TP * l[2]);
 #pragma unroll
         for (int iz = 0; iz <= P; ++iz) {
 #pragma unroll
             for (int iy = 0; iy <= P; ++iy) {
-                const float wyz = wy[iy] * wz[iz] * sc;
+                const float wyz = wy[iy] * wz[iz] * pp.w * sc;
 #pragma unroll
                 for (int ix = 0; ix <= P; ++ix)
-                    atomicAdd(a + ix + TP * (iy + TP * iz), (ACC)__float2uint_rz(fmaf(wx[ix], wyz, 0.5f)));
+                    atomicAdd(aq + ix + TP * (iy + TP * iz), (ACC)__float2ll_rn(wx[ix] * wyz));
             }
         }
-        if (sizeof(ACC) == 8 && qchan >= 0 && pp.w != 0.f) {
-            ACC* aq = acc + qchan * T3 + l[0] + TP * (l[1] + TP * l[2]);
+    }
+}
+
+// Particle r of the brick taken by pass-j thread: particles are in cell
+// order, so neighbouring ones share stencil points; the 64 lanes of a wave
+// take particles n/64 apart (a 64 x S transpose) so one atomic instruction
+// rarely hits the same LDS word twice.  -1: none.
+__device__ __forceinline__ int deposit_rank(int j, int n)
+{
+    const int S = (n + 63) >> 6;
+    const int r = (j & 63) * S + (j >> 6);
+    return (j < 64 * S && r < n) ? r : -1;
+}
+
+// Deposit the brick's particles into the LDS tile.  ACC is the fixed-point
+// accumulator word: 32-bit with 2^sc scaling when n w_max 2^sc < 2^32 bounds
+// every sum, else 64-bit with 2^31 (and signed charges).  Integer sums do not
+// depend on the order of the atomics.  pp/typ: this thread's particles of the
+// first kDepositPre passes, loaded by the caller.
+template <int P, int BM, typename ACC>
+__device__ __forceinline__ void deposit_tile(ACC* acc, int qchan, const float4* __restrict__ spos,
+                                             const int2* __restrict__ smeta, int pb, int n, float sc, float dx0,
+                                             float dx1, float dx2, int nx, int ny, int nz, int o0, int o1, int o2,
+                                             const TileGeom& tg, const float4 (&pp)[kDepositPre],
+                                             const int (&typ)[kDepositPre])
+{
+    const float i0 = 1.0f / dx0, i1 = 1.0f / dx1, i2 = 1.0f / dx2;
 #pragma unroll
-            for (int iz = 0; iz <= P; ++iz) {
-#pragma unroll
-                for (int iy = 0; iy <= P; ++iy) {
-                    const float wyz = wy[iy] * wz[iz] * pp.w * sc;
-#pragma unroll
-                    for (int ix = 0; ix <= P; ++ix)
-                        atomicAdd(aq + ix + TP * (iy + TP * iz), (ACC)__float2ll_rn(wx[ix] * wyz));
-                }
-            }
-        }
+    for (int q = 0; q < kDepositPre; ++q)
+        if (deposit_rank(threadIdx.x + q * kTileThreads, n) >= 0)
+            deposit_one<P, BM, ACC>(acc, qchan, pp[q], typ[q], sc, dx0, dx1, dx2, i0, i1, i2, nx, ny, nz, o0, o1,
+                                    o2, tg);
+    const int S = (n + 63) >> 6;
+    for (int j = threadIdx.x + kDepositPre * kTileThreads; j < 64 * S; j += kTileThreads) {
+        const int r = deposit_rank(j, n);
+        if (r < 0) continue;
+        deposit_one<P, BM, ACC>(acc, qchan, spos[pb + r], smeta[pb + r].y, sc, dx0, dx1, dx2, i0, i1, i2, nx, ny,
+                                nz, o0, o1, o2, tg);
     }
 }
 
@@ -496,13 +520,17 @@ __global__ void __launch_bounds__(kTileThreads) k_density_tile(
     const int n = pe - pb;
     // largest power-of-two scale 2^sc with n (w_max 2^sc + 1/2) < 2^32, the
     // bound of one point's sum of rounded deposits (checked for sc >= 16)
-    // the first pass's particle is loaded before the tile is cleared, so its
-    // latency overlaps the clearing and the barrier (which waits for LDS only)
-    const int S0 = (n + 63) >> 6;
-    const int r0 = ((int)threadIdx.x & 63) * S0 + ((int)threadIdx.x >> 6);
-    const bool has0 = (int)threadIdx.x < 64 * S0 && r0 < n;
-    const float4 pp0 = has0 ? spos[pb + r0] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const int typ0 = has0 ? smeta[pb + r0].y : 0;
+    // the first passes' particles are loaded before the tile is cleared, so
+    // their latency overlaps the clearing and the barrier (which waits for LDS
+    // only)
+    float4 pp0[kDepositPre];
+    int typ0[kDepositPre];
+#pragma unroll
+    for (int q = 0; q < kDepositPre; ++q) {
+        const int r = deposit_rank(threadIdx.x + q * kTileThreads, n);
+        pp0[q] = r >= 0 ? spos[pb + r] : make_float4(0.f, 0.f, 0.f, 0.f);
+        typ0[q] = r >= 0 ? smeta[pb + r].y : 0;
+    }
     const float bound = (float)n * (stencil_wmax(P) + 1.0f / 65536.0f) + 1.0f;
     int need = 1;
     while (need < 31 && (float)(1u << need) <= bound) ++need;
