@@ -261,7 +261,7 @@ def fts_profile(which, iters=20):
     s.close()
 
 
-def serve_measure(seed=11, window=1.0, rate=50.0, tild_frac=0.2, max_iters=25, budget_frac=0.7, max_pool=16):
+def serve_measure(seed=11, window=1.0, rate=50.0, tild_frac=0.2, max_iters=25, budget_frac=0.7, max_pool=32):
     """Served mixed workload: a Poisson stream (rate per s) of CL-FTS diblock
     solves (grids 32^3-96^3, chi N in [8, 25], 500-5000 iterations, Ns = 50,
     CL step from the grid spacing;
