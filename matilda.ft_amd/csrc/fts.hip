@@ -428,7 +428,9 @@ __global__ void __launch_bounds__(NT) k_fts_prop_xy(const PropDesc* __restrict__
     using P = FftPlan<N>;
     constexpr int T = P::T, E = P::E, B = NT / T;
     constexpr int NL = (N + B - 1) / B;
-    constexpr int PAD = (N % 16 == 0) ? 8 : 1, RS = N + PAD; // row stride in LDS
+    // row stride in LDS: room for the skewed row exchanges (swz_pitch) when N
+    // is a multiple of 16
+    constexpr int PAD = (N % 16 == 0) ? (N / 8 > 8 ? N / 8 : 8) : 1, RS = N + PAD;
     constexpr int NV = N * N / 2;                            // float4 (two values) per plane
     constexpr int PER = (NV + NT - 1) / NT;
     __shared__ float2 plane[N * RS];
@@ -1315,7 +1317,7 @@ static bool plane_fused(const FtsGroup& G)
 {
     if (G.n[0] != G.n[1]) return false;
     switch (G.n[0]) {
-    case 16: case 20: case 24: case 32: case 40: case 48: case 64: return true;
+    case 16: case 20: case 24: case 32: case 40: case 48: case 64: case 96: return true;
     default: return false;
     }
 }
@@ -1329,7 +1331,8 @@ static void launch_prop_xy_n(const PropDesc* dx, const PropDesc* dy, int n, cons
     // threads), so no line needs the scratch region and more workgroups fit in
     // LDS; small planes take one or two wavefronts rather than 256 threads of
     // which most would transform scratch lines
-    constexpr int NT = N == 48 ? 192 : (N * FftPlan<N>::T > 256 ? 512 : std::max(64, (N * FftPlan<N>::T + 63) / 64 * 64));
+    // (96-point planes: 96 lines of 8 threads, 768 threads, an 81 KB plane)
+    constexpr int NT = N == 48 ? 192 : N == 96 ? 768 : (N * FftPlan<N>::T > 256 ? 512 : std::max(64, (N * FftPlan<N>::T + 63) / 64 * 64));
     k_fts_prop_xy<N, NT><<<grid, NT, 0, st>>>(dx, dy, fft_twiddles_device(N));
 }
 
@@ -1343,6 +1346,7 @@ static int launch_prop_xy(const PropDesc* dx, const PropDesc* dy, int n, const F
     case 40: launch_prop_xy_n<40>(dx, dy, n, G, st); break;
     case 48: launch_prop_xy_n<48>(dx, dy, n, G, st); break;
     case 64: launch_prop_xy_n<64>(dx, dy, n, G, st); break;
+    case 96: launch_prop_xy_n<96>(dx, dy, n, G, st); break;
     default: set_error(MFT_ERR_UNSUPPORTED, "no plane kernel for %d", G.n[0]); return MFT_ERR_UNSUPPORTED;
     }
     MFT_LAUNCH_CHECK();

@@ -311,7 +311,7 @@ def test_config3_log_q_against_oracle(gpu):
 
 
 @pytest.mark.parametrize("grid", [(16, 16, 12), (20, 20, 10), (24, 24, 16), (32, 32, 12), (40, 40, 8),
-                                  (48, 48, 8), (64, 64, 10)])
+                                  (48, 48, 8), (64, 64, 10), (96, 96, 6)])
 def test_plane_kernel_matches_line_kernels(gpu, grid, monkeypatch):
     """The fused x-y plane kernel against the separate x and y line kernels
     (same arithmetic) over a few CL steps."""
