@@ -354,11 +354,31 @@ struct PropDesc {
     const float2* g;     // exp(-k_a^2/(Nr-1)) / N_a along this axis as hi + lo
 };
 
+// Line plans of the single-axis pass.  The default 24- and 96-point plans
+// hold twelve elements per thread, which with the line's exp(-W) factors and
+// addresses takes ~200 VGPRs (two waves per SIMD); six elements per thread
+// (radix-2 and radix-3 passes) take 120-141 (24-point z pass 13.4 -> 9.5 us
+// in the sweep, 96^3 iteration 2.33 -> 2.24 ms).  At 48 points the five-pass
+// six-element plan measured slower (27.8 -> 30.1 us) and the default stays.
+template <int N>
+struct FtsAxisPlan : FftPlan<N> {};
+#define MFT_AXIS_PLAN(N_, T_, ...)                                              \
+    template <>                                                                 \
+    struct FtsAxisPlan<N_> {                                                    \
+        static constexpr int N = N_;                                            \
+        static constexpr int T = T_;                                            \
+        static constexpr int E = N_ / T_;                                       \
+        using R = Radices<__VA_ARGS__>;                                         \
+    };
+MFT_AXIS_PLAN(24, 4, 2, 2, 2, 3)
+MFT_AXIS_PLAN(96, 16, 2, 2, 2, 2, 2, 3)
+#undef MFT_AXIS_PLAN
+
 template <int N>
 __global__ void __launch_bounds__(256) k_fts_prop_axis(const PropDesc* __restrict__ descs, int nx, int ny, int nz,
                                                        int axis, const float2* __restrict__ twg)
 {
-    using P = FftPlan<N>;
+    using P = FtsAxisPlan<N>;
     constexpr int T = P::T, E = P::E, B = 256 / T;
     __shared__ float2 lds[B * N];
     __shared__ float2 tw[N];
@@ -394,7 +414,7 @@ __global__ void __launch_bounds__(256) k_fts_prop_axis(const PropDesc* __restric
     }
     __syncthreads();
     LdsMap map{lline, B, inter, N};
-    line_fft<N, -1>(v, lds, tw, t, map);
+    line_fft_plan<P, -1>(v, lds, tw, t, map);
 #pragma unroll
     for (int s = 0; s < E; ++s) {
         // two-float factor: the per-axis 1/N of a non-power-of-two length is not
@@ -402,7 +422,7 @@ __global__ void __launch_bounds__(256) k_fts_prop_axis(const PropDesc* __restric
         const float2 g = gk[t + T * s];
         v[s] = scale_hilo(v[s], g);
     }
-    line_fft<N, +1>(v, lds, tw, t, map);
+    line_fft_plan<P, +1>(v, lds, tw, t, map);
     if (!valid) return;
 #pragma unroll
     for (int s = 0; s < E; ++s) {
@@ -1293,7 +1313,7 @@ static int group_setup(FtsBatch* b, FtsGroup& G)
 template <int N>
 static void launch_prop_n(const PropDesc* d, int n, const FtsGroup& G, int axis, hipStream_t st)
 {
-    constexpr int B = 256 / FftPlan<N>::T;
+    constexpr int B = 256 / FtsAxisPlan<N>::T;
     AxisGeom geom{G.n[0], G.n[1], G.n[2], axis};
     dim3 grid((geom.nlines() + B - 1) / B, n);
     k_fts_prop_axis<N><<<grid, 256, 0, st>>>(d, G.n[0], G.n[1], G.n[2], axis, fft_twiddles_device(N));
