@@ -685,26 +685,28 @@ __device__ __forceinline__ void stage_window(Win3* ft, int* xg, int* yo, int* zo
         rows[r][1] = (unsigned char)(r / e1);
     }
     __syncthreads();
-    constexpr int RMAX = (TP * TP + 15) / 16;
-    const int lane16 = threadIdx.x & 15, row16 = threadIdx.x >> 4;
-    if (lane16 < e0) {
-        const float4* src = F + xg[lane16];
-        Win3* dst = ft + lane16;
-        // rows past the window re-load the last row (in bounds) and skip the
-        // store, so the arrays stay in registers
-        float4 v[RMAX];
-        int dofs[RMAX];
+    // the window's points dealt to every lane in turn (x fastest, so a wave
+    // reads a few whole rows); points past the window re-load the last point
+    // (in bounds) and skip the store, so the arrays stay in registers.  Row r
+    // and column x of point q: r = floor((q + 1/2) / e0) in float, exact for
+    // q < 2^20 with e0 <= TP
+    constexpr int KMAX = (TP * TP * TP + kTileThreads - 1) / kTileThreads;
+    const int npts = nrows * e0;
+    const float ie0 = 1.0f / (float)e0;
+    float4 v[KMAX];
+    int dofs[KMAX];
 #pragma unroll
-        for (int j = 0; j < RMAX; ++j) {
-            const int r = min(row16 + 16 * j, nrows - 1);
-            const int t1 = rows[r][0], t2 = rows[r][1];
-            v[j] = src[yo[t1] + zo[t2]];
-            dofs[j] = TP * (t1 + TP * t2);
-        }
-#pragma unroll
-        for (int j = 0; j < RMAX; ++j)
-            if (row16 + 16 * j < nrows) dst[dofs[j]] = Win3{v[j].x, v[j].y, v[j].z};
+    for (int k = 0; k < KMAX; ++k) {
+        const int q = min((int)threadIdx.x + kTileThreads * k, npts - 1);
+        const int r = (int)(((float)q + 0.5f) * ie0);
+        const int x = q - r * e0;
+        const int t1 = rows[r][0], t2 = rows[r][1];
+        v[k] = F[xg[x] + yo[t1] + zo[t2]];
+        dofs[k] = x + TP * (t1 + TP * t2);
     }
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k)
+        if ((int)threadIdx.x + kTileThreads * k < npts) ft[dofs[k]] = Win3{v[k].x, v[k].y, v[k].z};
     __syncthreads();
 }
 
