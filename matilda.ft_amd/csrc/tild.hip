@@ -766,14 +766,9 @@ int mft_tild_get_grid_force(mft_tild* e, int s, float* F)
         std::memset(F, 0, sizeof(float) * 3 * M);
         return MFT_OK;
     }
-    std::vector<float4> tmp(M);
+    // the field's first 3 M floats are its points' (fx, fy, fz)
     MFT_HIP(hipStreamSynchronize(e->stream));
-    MFT_HIP(hipMemcpy(tmp.data(), e->F4 + (size_t)slot * M, sizeof(float4) * M, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < M; ++i) {
-        F[3 * i + 0] = tmp[i].x;
-        F[3 * i + 1] = tmp[i].y;
-        F[3 * i + 2] = tmp[i].z;
-    }
+    MFT_HIP(hipMemcpy(F, e->F4 + (size_t)slot * M, sizeof(float) * 3 * M, hipMemcpyDeviceToHost));
     return MFT_OK;
 }
 
@@ -990,15 +985,15 @@ int mft_tild_kernel_bytes(mft_tild* e, double* bytes)
     const bool plane = e->plane_passes && tild_plane_supported(e->d.nx, e->d.ny);
     if (plane) {
         bytes[MFT_TK_FFT_X_FWD] = M * C * (4 + 8);          // densities in, x-y spectra out
-        bytes[MFT_TK_FFT_X_INV] = M * O * 8 + M * O4 * 16;  // packed spectra in, force fields out
+        bytes[MFT_TK_FFT_X_INV] = M * O * 8 + M * O4 * 12;  // packed spectra in, force fields out
     } else {
         bytes[MFT_TK_FFT_X_FWD] = M * C * (4 + 8);          // densities in, spectra out (tiles summed on the fly)
         bytes[MFT_TK_FFT_Y_FWD] = M * C * 16;
         bytes[MFT_TK_FFT_Y_INV] = M * O * 16;
-        bytes[MFT_TK_FFT_X_INV] = M * O * 8 + M * O4 * 16;
+        bytes[MFT_TK_FFT_X_INV] = M * O * 8 + M * O4 * 12;   // force fields of 12-byte points
     }
     bytes[MFT_TK_CONV_Z] = M * 8 * (C + O);
-    bytes[MFT_TK_GATHER] = N * (16 + 8 + 16) + M * 16 * O4;
+    bytes[MFT_TK_GATHER] = N * (16 + 8 + 16) + M * 12 * O4;
     bytes[MFT_TK_INTEGRATE] = N * ((16 + 16 + nz + 8 + 4 + 8 * mb + 16) + (16 + nz + 4 + 4));
     return MFT_OK;
 }

@@ -124,16 +124,14 @@ int tild_bias_setup(TildEngine* e, int pot)
     return MFT_OK;
 }
 
+// F: an engine force field (12 bytes a point); B: the bias force (float4)
 __global__ void k_bias_add(float4* __restrict__ F, const float4* __restrict__ B, long M)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
-    float4 f = F[i];
+    const float3 f = ff_load(F, i);
     const float4 b = B[i];
-    f.x += b.x;
-    f.y += b.y;
-    f.z += b.z;
-    F[i] = f;
+    ff_store(F, i, f.x + b.x, f.y + b.y, f.z + b.z);
 }
 
 // accumulateGridForceComp of the bias on its group (ps_potentialBias.cu:200-211)

@@ -90,13 +90,13 @@ __global__ void __launch_bounds__(256) k_fft_x_inv_packed(const float2* __restri
     if (!valid) return;
     // fields 2g (components in pk0.re, pk0.im, pk1.re) and 2g+1 (pk1.im,
     // pk2.re, pk2.im)
-    float4* d0 = F4 + (2 * g) * M + base;
+    float4* d0 = F4 + (2 * g) * M;
 #pragma unroll
-    for (int s = 0; s < E; ++s) d0[t + T * s] = make_float4(pk[0][s].x, pk[0][s].y, pk[1][s].x, 0.f);
+    for (int s = 0; s < E; ++s) ff_store(d0, base + t + T * s, pk[0][s].x, pk[0][s].y, pk[1][s].x);
     if (2 * g + 1 < nfield) {
-        float4* d1 = F4 + (2 * g + 1) * M + base;
+        float4* d1 = F4 + (2 * g + 1) * M;
 #pragma unroll
-        for (int s = 0; s < E; ++s) d1[t + T * s] = make_float4(pk[1][s].y, pk[2][s].x, pk[2][s].y, 0.f);
+        for (int s = 0; s < E; ++s) ff_store(d1, base + t + T * s, pk[1][s].y, pk[2][s].x, pk[2][s].y);
     }
 }
 
@@ -317,8 +317,8 @@ __global__ void __launch_bounds__(PlaneShape<N>::NT) k_fft_yx_inv_packed_plane(
     for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int s = 0; s < E; ++s) keep[r][s] = make_float2(0.f, 0.f);
-    float4* d0 = F4 + (long)(2 * g) * M + pbase;
-    float4* d1 = F4 + (long)(2 * g + 1) * M + pbase;
+    float4* d0 = F4 + (long)(2 * g) * M;
+    float4* d1 = F4 + (long)(2 * g + 1) * M;
     const bool second = 2 * g + 1 < nfield;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
@@ -362,11 +362,11 @@ __global__ void __launch_bounds__(PlaneShape<N>::NT) k_fft_yx_inv_packed_plane(
             if (j == 1) {
                 // field 2g: packed 3g (re, im) and 3g + 1 (re)
 #pragma unroll
-                for (int s = 0; s < E; ++s) d0[o + T * s] = make_float4(keep[r][s].x, keep[r][s].y, v[s].x, 0.f);
+                for (int s = 0; s < E; ++s) ff_store(d0, pbase + o + T * s, keep[r][s].x, keep[r][s].y, v[s].x);
             } else if (j == 2 && second) {
                 // field 2g + 1: packed 3g + 1 (im) and 3g + 2 (re, im)
 #pragma unroll
-                for (int s = 0; s < E; ++s) d1[o + T * s] = make_float4(keep[r][s].y, v[s].x, v[s].y, 0.f);
+                for (int s = 0; s < E; ++s) ff_store(d1, pbase + o + T * s, keep[r][s].y, v[s].x, v[s].y);
             }
 #pragma unroll
             for (int s = 0; s < E; ++s) keep[r][s] = v[s];

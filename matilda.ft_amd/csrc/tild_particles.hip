@@ -635,13 +635,13 @@ __global__ void __launch_bounds__(256) k_gather(float4* __restrict__ fgrid, cons
                     long g = row + xs[ix];
                     float w = wx[ix] * wyz;
                     if (fs >= 0) {
-                        float4 v = Fs[g];
+                        const float3 v = ff_load(Fs, g);
                         f[0] += v.x * w;
                         f[1] += v.y * w;
                         f[2] += v.z * w;
                     }
                     if (charged) {
-                        float4 v = Fq[g];
+                        const float3 v = ff_load(Fq, g);
                         fq[0] += v.x * w;
                         fq[1] += v.y * w;
                         fq[2] += v.z * w;
@@ -693,7 +693,7 @@ __device__ __forceinline__ void stage_window(Win3* ft, int* xg, int* yo, int* zo
     constexpr int KMAX = (TP * TP * TP + kTileThreads - 1) / kTileThreads;
     const int npts = nrows * e0;
     const float ie0 = 1.0f / (float)e0;
-    float4 v[KMAX];
+    float3 v[KMAX];
     int dofs[KMAX];
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
@@ -701,7 +701,7 @@ __device__ __forceinline__ void stage_window(Win3* ft, int* xg, int* yo, int* zo
         const int r = (int)(((float)q + 0.5f) * ie0);
         const int x = q - r * e0;
         const int t1 = rows[r][0], t2 = rows[r][1];
-        v[k] = F[xg[x] + yo[t1] + zo[t2]];
+        v[k] = ff_load(F, xg[x] + yo[t1] + zo[t2]);
         dofs[k] = x + TP * (t1 + TP * t2);
     }
 #pragma unroll
@@ -792,7 +792,7 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_species(
 // sums.  A lane owns window column lane16 and rows row16 + 16 j; their
 // periodic grid offsets are computed in registers (no LDS tables).
 template <int P, int BM>
-__device__ __forceinline__ void window_loads(float4 (&v)[((BM + P) * (BM + P) + 15) / 16], const float4* __restrict__ F,
+__device__ __forceinline__ void window_loads(float3 (&v)[((BM + P) * (BM + P) + 15) / 16], const float4* __restrict__ F,
                                              int o0, int o1, int o2, int e0, int e1, int e2, int nx, int ny, int nz)
 {
     constexpr int TP = BM + P, shift = P / 2, RMAX = (TP * TP + 15) / 16;
@@ -804,7 +804,7 @@ __device__ __forceinline__ void window_loads(float4 (&v)[((BM + P) * (BM + P) + 
         const int r = min(row16 + 16 * j, nrows - 1);
         const int t1 = r % e1, t2 = r / e1;
         const long off = gx + (long)nx * (wrap(o1 - shift + t1, ny) + (long)ny * wrap(o2 - shift + t2, nz));
-        v[j] = F[off];
+        v[j] = ff_load(F, off);
     }
 }
 
@@ -830,7 +830,7 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_species_pipe(
         }
         return t1;
     };
-    float4 v[RMAX];
+    float3 v[RMAX];
     int tw = next_window(t0);
     auto geom = [&](int t, int* o, int* e) {
         const int brick = t / tg.nspec;
@@ -936,15 +936,16 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_brick(
         Win3* ft = ftw + nwin * T3;
         ++nwin;
         if (lane16 < e0) {
-            const float4* src = F4 + f * M + xg[lane16];
+            const float4* src = F4 + f * M;
+            const int x0 = xg[lane16];
             Win3* dst = ft + lane16;
-            float4 v[RMAX];
+            float3 v[RMAX];
             int dofs[RMAX];
 #pragma unroll
             for (int j = 0; j < RMAX; ++j) {
                 const int r = min(row16 + 16 * j, nrows - 1);
                 const int t1 = rows[r][0], t2 = rows[r][1];
-                v[j] = src[yo[t1] + zo[t2]];
+                v[j] = ff_load(src, x0 + yo[t1] + zo[t2]);
                 dofs[j] = TP * (t1 + TP * t2);
             }
 #pragma unroll
