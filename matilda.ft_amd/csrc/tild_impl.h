@@ -245,8 +245,8 @@ struct TildEngine {
     uint32_t* d_step = nullptr;
     bool use_dstep = false;
     // step temporaries in slot order
-    float4 *XN = nullptr, *NN = nullptr; // next positions and noise
-    float4* fgrid = nullptr;             // grid force per slot (x, y, z, 0)
+    float4 *XN = nullptr, *NN = nullptr; // next positions (12 bytes a slot: ff_load / ff_store) and noise
+    float4* fgrid = nullptr;             // grid force per slot, 12 bytes a slot (ff_load / ff_store)
     float4* ftot = nullptr;              // total force per slot (force evaluations)
     int *key = nullptr, *rank = nullptr, *map = nullptr;
     // topology and state staging in particle-id order

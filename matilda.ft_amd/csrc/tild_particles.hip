@@ -355,7 +355,8 @@ __global__ void k_permute(int ns, int maxbonds, const int* __restrict__ key, con
     if (s >= ns) return;
     const int n = start[key[s]] + rank[s];
     map[s] = n;
-    const float4 xc = X[s], xn = XN[s];
+    const float4 xc = X[s];
+    const float3 xn = ff_load(XN, s);
     X2[n] = make_float4(xn.x, xn.y, xn.z, xc.w);
     XO2[n] = make_float4(xc.x, xc.y, xc.z, 0.f);
     if (NN) NZ2[n] = NN[s];
@@ -655,7 +656,7 @@ __global__ void __launch_bounds__(256) k_gather(float4* __restrict__ fgrid, cons
             f[2] += fq[2] * pp.w;
         }
     }
-    fgrid[i] = make_float4(f[0], f[1], f[2], 0.f);
+    ff_store(fgrid, i, f[0], f[1], f[2]);
 }
 
 // Tile variant.  The force field window of a brick ((b+p)^3 points from
@@ -762,7 +763,7 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_species(
     const int f = gm.slot[s];
     if (f < 0) {
         for (int it = pb + (int)threadIdx.x; it < pe; it += kTileThreads)
-            fgrid[it] = make_float4(0.f, 0.f, 0.f, 0.f);
+            ff_store(fgrid, it, 0.f, 0.f, 0.f);
         return;
     }
     const int k0 = brick % tg.nb[0], k1 = (brick / tg.nb[0]) % tg.nb[1], k2 = brick / (tg.nb[0] * tg.nb[1]);
@@ -782,7 +783,7 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_species(
         float3 a = make_float3(0.f, 0.f, 0.f);
         if (stencil_in_brick<P>(pp, dx0, dx1, dx2, i0, i1, i2, nx, ny, nz, o0, o1, o2, tg, wx, wy, wz, l))
             a = stencil_sum<P, BM>(ft, l, wx, wy, wz);
-        fgrid[it] = make_float4(a.x, a.y, a.z, 0.f);
+        ff_store(fgrid, it, a.x, a.y, a.z);
     }
 }
 
@@ -854,7 +855,7 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_species_pipe(
         if (pb == pe) continue;
         if (gm.slot[sp] < 0) {
             for (int it = pb + (int)threadIdx.x; it < pe; it += kTileThreads)
-                fgrid[it] = make_float4(0.f, 0.f, 0.f, 0.f);
+                ff_store(fgrid, it, 0.f, 0.f, 0.f);
             continue;
         }
         // t == tw: its window is in v
@@ -887,7 +888,7 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_species_pipe(
             float3 a = make_float3(0.f, 0.f, 0.f);
             if (stencil_in_brick<P>(pp, dx0, dx1, dx2, i0, i1, i2, nx, ny, nz, o[0], o[1], o[2], tg, wx, wy, wz, l))
                 a = stencil_sum<P, BM>(ft, l, wx, wy, wz);
-            fgrid[it] = make_float4(a.x, a.y, a.z, 0.f);
+            ff_store(fgrid, it, a.x, a.y, a.z);
         }
     }
 }
@@ -967,7 +968,7 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_brick(
             if (stencil_in_brick<P>(pp, dx0, dx1, dx2, i0, i1, i2, nx, ny, nz, o0, o1, o2, tg, wx, wy, wz, l))
                 a = stencil_sum<P, BM>(ftw + w * T3, l, wx, wy, wz);
         }
-        fgrid[it] = make_float4(a.x, a.y, a.z, 0.f);
+        ff_store(fgrid, it, a.x, a.y, a.z);
     }
 }
 
@@ -1000,8 +1001,8 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_charge(
         int l[3];
         if (!stencil_in_brick<P>(pp, dx0, dx1, dx2, i0, i1, i2, nx, ny, nz, o0, o1, o2, tg, wx, wy, wz, l)) continue;
         const float3 a = stencil_sum<P, BM>(ft, l, wx, wy, wz);
-        const float4 o = fgrid[pid];
-        fgrid[pid] = make_float4(o.x + pp.w * a.x, o.y + pp.w * a.y, o.z + pp.w * a.z, 0.f);
+        const float3 o = ff_load(fgrid, pid);
+        ff_store(fgrid, pid, o.x + pp.w * a.x, o.y + pp.w * a.y, o.z + pp.w * a.z);
     }
 }
 
@@ -1283,7 +1284,7 @@ __device__ __forceinline__ int integrate_slot(int s, float4 xc, int2 meta, const
             xn[j] = v;
         }
     }
-    A.XN[s] = make_float4(xn[0], xn[1], xn[2], 0.f);
+    ff_store(A.XN, s, xn[0], xn[1], xn[2]);
     if (A.write_nn) A.NN[s] = make_float4(nn[0], nn[1], nn[2], 0.f);
     return key_of(xn[0], xn[1], xn[2], meta.y, A.dx[0], A.dx[1], A.dx[2], A.p, A.nx, A.ny, A.nz, tg);
 }
@@ -1312,7 +1313,7 @@ __global__ void __launch_bounds__(256) k_integrate(
         // every load that does not depend on another issued first, the bond
         // partners' positions next: two rounds of memory latency per slot
         const float4 xc = X[s];
-        const float4 fg = fgrid[s];
+        const float3 fg = ff_load(fgrid, s);
         const int2 meta = META[s];
         const bool gjf = A.integ == MFT_INT_GJF;
         const bool need_nz = (gjf && !A.nz_keyed) || A.integ == MFT_INT_VV || A.use_v;
