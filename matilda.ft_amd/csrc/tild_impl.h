@@ -221,7 +221,7 @@ struct TildEngine {
     // particle state in key order (brick, species, sub-cell), double-buffered
     // for the per-step permutation into the next step's order
     float4* X[2] = {nullptr, nullptr};   // x, y, z, charge
-    float4* XO[2] = {nullptr, nullptr};  // previous positions (GJF)
+    float4* XO[2] = {nullptr, nullptr};  // previous positions (GJF), 12 bytes a slot (ff_load / ff_store)
     float4* NZ[2] = {nullptr, nullptr};  // previous noise (GJF)
     int2* META[2] = {nullptr, nullptr};  // particle id, species
     int* NB[2] = {nullptr, nullptr};     // bond count

@@ -235,7 +235,7 @@ __global__ void k_sort_place(const float* __restrict__ x, const float* __restric
     const int s = start[key[i]] + rank[i];
     slot_of[i] = s;
     X[s] = make_float4(x[i * 3], x[i * 3 + 1], x[i * 3 + 2], q ? q[i] : 0.f);
-    XO[s] = make_float4(xo[i * 3], xo[i * 3 + 1], xo[i * 3 + 2], 0.f);
+    ff_store(XO, s, xo[i * 3], xo[i * 3 + 1], xo[i * 3 + 2]);
     NZ[s] = make_float4(noise[i * 3], noise[i * 3 + 1], noise[i * 3 + 2], 0.f);
     META[s] = make_int2(i, type[i]);
     NB[s] = nBonds[i];
@@ -268,13 +268,20 @@ __global__ void k_sort_angles(const int* __restrict__ nAngles, const int* __rest
     }
 }
 
-// key order -> particle-id order: out[3 id + j] = src[slot].j
+// key order -> particle-id order: out[3 id + j] = src[slot].j (packed: src
+// holds 12-byte records, ff_load)
 __global__ void k_unsort(float* __restrict__ out, const float4* __restrict__ src, const int2* __restrict__ META,
-                         int ns)
+                         int ns, int packed)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
-    const float4 v = src[s];
+    float4 v;
+    if (packed) {
+        const float3 p = ff_load(src, s);
+        v = make_float4(p.x, p.y, p.z, 0.f);
+    } else {
+        v = src[s];
+    }
     const int id = META[s].x;
     out[id * 3 + 0] = v.x;
     out[id * 3 + 1] = v.y;
@@ -358,7 +365,7 @@ __global__ void k_permute(int ns, int maxbonds, const int* __restrict__ key, con
     const float4 xc = X[s];
     const float3 xn = ff_load(XN, s);
     X2[n] = make_float4(xn.x, xn.y, xn.z, xc.w);
-    XO2[n] = make_float4(xc.x, xc.y, xc.z, 0.f);
+    ff_store(XO2, n, xc.x, xc.y, xc.z);
     if (NN) NZ2[n] = NN[s];
     META2[n] = META[s];
     NB2[n] = NB[s];
@@ -1317,7 +1324,8 @@ __global__ void __launch_bounds__(256) k_integrate(
         const int2 meta = META[s];
         const bool gjf = A.integ == MFT_INT_GJF;
         const bool need_nz = (gjf && !A.nz_keyed) || A.integ == MFT_INT_VV || A.use_v;
-        const float4 xo4 = gjf ? A.XO[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float3 xo3 = gjf ? ff_load(A.XO, s) : make_float3(0.f, 0.f, 0.f);
+        const float4 xo4 = make_float4(xo3.x, xo3.y, xo3.z, 0.f);
         const float4 nz4 = need_nz ? A.NZ[s] : make_float4(0.f, 0.f, 0.f, 0.f);
         int nb = 0;
         int2 bs = make_int2(0, 0), bt = make_int2(0, 0);
@@ -1618,7 +1626,7 @@ int tild_unsort(TildEngine* e, int what)
         return MFT_OK;
     }
     const float4* src = what == 0 ? e->X[c] : what == 1 ? e->XO[c] : what == 2 ? e->NZ[c] : e->ftot;
-    k_unsort<<<nblk(e->N), kBlock, 0, e->stream>>>(e->pio, src, e->META[c], e->N);
+    k_unsort<<<nblk(e->N), kBlock, 0, e->stream>>>(e->pio, src, e->META[c], e->N, what == 1 ? 1 : 0);
     MFT_LAUNCH_CHECK();
     return MFT_OK;
 }
