@@ -981,7 +981,7 @@ int mft_tild_kernel_bytes(mft_tild* e, double* bytes)
     const double nz = e->d.integrator == MFT_INT_GJF ? 0.0 : 16.0;
     bytes[MFT_TK_SCATTER] = N * ((4 + 4 + 4 + 12 + nz + 16 + 8 + 4 + 4 * mb) + (16 + 12 + nz + 8 + 4 + 4 * mb + 4) +
                                  (4 + 4 + 8 * mb + 4 * mb));
-    bytes[MFT_TK_DENSITY] = N * (16 + 8) + M * 4 * C;         // key-ordered records, rho
+    bytes[MFT_TK_DENSITY] = N * 16 + M * 4 * C;               // key-ordered positions, rho
     const bool plane = e->plane_passes && tild_plane_supported(e->d.nx, e->d.ny);
     if (plane) {
         bytes[MFT_TK_FFT_X_FWD] = M * C * (4 + 8);          // densities in, x-y spectra out

@@ -482,6 +482,17 @@ __device__ __forceinline__ int deposit_rank(int j, int n)
     return (j < 64 * S && r < n) ? r : -1;
 }
 
+// Species of the brick's particle at slot it: the key order puts a brick's
+// particles species by species, so the species follows from the segment
+// starts (uniform over the workgroup: scalar loads), with no per-particle
+// record read.  key0 = the brick's first key.
+__device__ __forceinline__ int species_in_brick(const int* __restrict__ start, int key0, int nspec, int it)
+{
+    int t = 0;
+    for (int sp = 1; sp < nspec; ++sp) t += it >= start[key0 + (sp << 6)] ? 1 : 0;
+    return t;
+}
+
 // Deposit the brick's particles into the LDS tile.  ACC is the fixed-point
 // accumulator word: 32-bit with 2^sc scaling when n w_max 2^sc < 2^32 bounds
 // every sum, else 64-bit with 2^31 (and signed charges).  Integer sums do not
@@ -489,9 +500,9 @@ __device__ __forceinline__ int deposit_rank(int j, int n)
 // first kDepositPre passes, loaded by the caller.
 template <int P, int BM, typename ACC>
 __device__ __forceinline__ void deposit_tile(ACC* acc, int qchan, const float4* __restrict__ spos,
-                                             const int2* __restrict__ smeta, int pb, int n, float sc, float dx0,
-                                             float dx1, float dx2, int nx, int ny, int nz, int o0, int o1, int o2,
-                                             const TileGeom& tg, const float4 (&pp)[kDepositPre],
+                                             const int* __restrict__ start, int key0, int pb, int n, float sc,
+                                             float dx0, float dx1, float dx2, int nx, int ny, int nz, int o0, int o1,
+                                             int o2, const TileGeom& tg, const float4 (&pp)[kDepositPre],
                                              const int (&typ)[kDepositPre])
 {
     const float i0 = 1.0f / dx0, i1 = 1.0f / dx1, i2 = 1.0f / dx2;
@@ -504,8 +515,8 @@ __device__ __forceinline__ void deposit_tile(ACC* acc, int qchan, const float4* 
     for (int j = threadIdx.x + kDepositPre * kTileThreads; j < 64 * S; j += kTileThreads) {
         const int r = deposit_rank(j, n);
         if (r < 0) continue;
-        deposit_one<P, BM, ACC>(acc, qchan, spos[pb + r], smeta[pb + r].y, sc, dx0, dx1, dx2, i0, i1, i2, nx, ny,
-                                nz, o0, o1, o2, tg);
+        deposit_one<P, BM, ACC>(acc, qchan, spos[pb + r], species_in_brick(start, key0, tg.nspec, pb + r), sc, dx0,
+                                dx1, dx2, i0, i1, i2, nx, ny, nz, o0, o1, o2, tg);
     }
 }
 
@@ -514,7 +525,7 @@ __device__ __forceinline__ void deposit_tile(ACC* acc, int qchan, const float4* 
 // the tile is written as float times 1/gvol.
 template <int P, int BM>
 __global__ void __launch_bounds__(kTileThreads) k_density_tile(
-    float* __restrict__ tiles, int nchan, int qchan, const float4* __restrict__ spos, const int2* __restrict__ smeta,
+    float* __restrict__ tiles, int nchan, int qchan, const float4* __restrict__ spos,
     const int* __restrict__ start, TileGeom tg, int nx, int ny, int nz, float dx0, float dx1, float dx2,
     double inv_gvol)
 {
@@ -524,10 +535,9 @@ __global__ void __launch_bounds__(kTileThreads) k_density_tile(
     const int brick = brick_of_block(blockIdx.x, tg.nbricks);
     const int k0 = brick % tg.nb[0], k1 = (brick / tg.nb[0]) % tg.nb[1], k2 = brick / (tg.nb[0] * tg.nb[1]);
     const int o0 = k0 << tg.sb[0], o1 = k1 << tg.sb[1], o2 = k2 << tg.sb[2];
-    const int pb = start[(brick * tg.nspec) << 6], pe = start[((brick + 1) * tg.nspec) << 6];
+    const int key0 = (brick * tg.nspec) << 6;
+    const int pb = start[key0], pe = start[((brick + 1) * tg.nspec) << 6];
     const int n = pe - pb;
-    // largest power-of-two scale 2^sc with n (w_max 2^sc + 1/2) < 2^32, the
-    // bound of one point's sum of rounded deposits (checked for sc >= 16)
     // the first passes' particles are loaded before the tile is cleared, so
     // their latency overlaps the clearing and the barrier (which waits for LDS
     // only)
@@ -537,8 +547,10 @@ __global__ void __launch_bounds__(kTileThreads) k_density_tile(
     for (int q = 0; q < kDepositPre; ++q) {
         const int r = deposit_rank(threadIdx.x + q * kTileThreads, n);
         pp0[q] = r >= 0 ? spos[pb + r] : make_float4(0.f, 0.f, 0.f, 0.f);
-        typ0[q] = r >= 0 ? smeta[pb + r].y : 0;
+        typ0[q] = r >= 0 ? species_in_brick(start, key0, tg.nspec, pb + r) : 0;
     }
+    // largest power-of-two scale 2^sc with n (w_max 2^sc + 1/2) < 2^32, the
+    // bound of one point's sum of rounded deposits (checked for sc >= 16)
     const float bound = (float)n * (stencil_wmax(P) + 1.0f / 65536.0f) + 1.0f;
     int need = 1;
     while (need < 31 && (float)(1u << need) <= bound) ++need;
@@ -555,16 +567,16 @@ __global__ void __launch_bounds__(kTileThreads) k_density_tile(
     if (narrow) {
         for (int i = threadIdx.x; i < nchan * T3; i += kTileThreads) acc32[i] = 0u;
         __syncthreads();
-        deposit_tile<P, BM, unsigned int>(acc32, qchan, spos, smeta, pb, n, (float)(1u << sc), dx0, dx1, dx2, nx,
-                                          ny, nz, o0, o1, o2, tg, pp0, typ0);
+        deposit_tile<P, BM, unsigned int>(acc32, qchan, spos, start, key0, pb, n, (float)(1u << sc), dx0, dx1, dx2,
+                                          nx, ny, nz, o0, o1, o2, tg, pp0, typ0);
         __syncthreads();
         const double scale = inv_gvol / (double)(1u << sc);
         for (int i = threadIdx.x; i < nchan * T3; i += kTileThreads) *out(i) = (float)((double)acc32[i] * scale);
     } else {
         for (int i = threadIdx.x; i < nchan * T3; i += kTileThreads) acc64[i] = 0ull;
         __syncthreads();
-        deposit_tile<P, BM, unsigned long long>(acc64, qchan, spos, smeta, pb, n, 2147483648.0f, dx0, dx1, dx2, nx,
-                                                ny, nz, o0, o1, o2, tg, pp0, typ0);
+        deposit_tile<P, BM, unsigned long long>(acc64, qchan, spos, start, key0, pb, n, 2147483648.0f, dx0, dx1,
+                                                dx2, nx, ny, nz, o0, o1, o2, tg, pp0, typ0);
         __syncthreads();
         const double scale = inv_gvol / 2147483648.0;
         for (int i = threadIdx.x; i < nchan * T3; i += kTileThreads)
@@ -1667,7 +1679,7 @@ int tild_density(TildEngine* e)
     const double inv_gvol = 1.0 / (double)e->gvol;
 #define MFT_DENS(P_, B_)                                                                                  \
     k_density_tile<P_, B_><<<tg.nbricks, kTileThreads, e->dens_lds, e->stream>>>(                         \
-        e->tiles, e->nchan, e->qchan, e->X[e->cur], e->META[e->cur], e->start, tg, d.nx, d.ny, d.nz, e->dx[0], e->dx[1], \
+        e->tiles, e->nchan, e->qchan, e->X[e->cur], e->start, tg, d.nx, d.ny, d.nz, e->dx[0], e->dx[1], \
         e->dx[2], inv_gvol)
 #define MFT_DENS_B(P_)                                                                                    \
     if (BM == 8) MFT_DENS(P_, 8);                                                                         \
