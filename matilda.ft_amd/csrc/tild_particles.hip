@@ -520,6 +520,49 @@ __device__ __forceinline__ void deposit_tile(ACC* acc, int qchan, const float4* 
     }
 }
 
+// Walk of one thread over a brick's tile entries i, i + kTileThreads, ...
+// (i = c T3 + t0 + TP (t1 + TP t2)) with the entry's offset in the
+// halo-expanded tile array: the tile digits (t0, t1, t2) and the offset
+// advance by the mixed-radix digits of kTileThreads with carries, so no
+// entry needs an integer division or a 64-bit multiply.
+template <int TP>
+struct TileWalk {
+    static constexpr int T3 = TP * TP * TP;
+    static constexpr int DC = kTileThreads / T3, R = kTileThreads % T3;
+    static constexpr int D2 = R / (TP * TP), D1 = (R / TP) % TP, D0 = R % TP;
+    int t0, t1, t2;
+    long off;
+    long step;     // offset change with no carry
+    int j0, j1;    // extra offset of a carry out of t0, out of t1
+    long j2;       // ... out of t2 (into the channel)
+    __device__ TileWalk(int i, int ex0, int ex1, long cstride)
+    {
+        const int c = i / T3, t = i - c * T3;
+        t0 = t % TP;
+        t1 = (t / TP) % TP;
+        t2 = t / (TP * TP);
+        const int e01 = ex0 * ex1;
+        off = c * cstride + t0 + (long)ex0 * t1 + (long)e01 * t2;
+        step = D0 + (long)D1 * ex0 + (long)D2 * e01 + DC * cstride;
+        j0 = ex0 - TP;
+        j1 = e01 - TP * ex0;
+        j2 = cstride - (long)TP * e01;
+    }
+    __device__ __forceinline__ void advance()
+    {
+        t0 += D0;
+        const bool c0 = t0 >= TP;
+        t0 -= c0 ? TP : 0;
+        t1 += D1 + (c0 ? 1 : 0);
+        const bool c1 = t1 >= TP;
+        t1 -= c1 ? TP : 0;
+        t2 += D2 + (c1 ? 1 : 0);
+        const bool c2 = t2 >= TP;
+        t2 -= c2 ? TP : 0;
+        off += step + (c0 ? j0 : 0) + (c1 ? j1 : 0) + (c2 ? j2 : 0);
+    }
+};
+
 // One workgroup per brick: the brick's particles (contiguous in key order)
 // spread their (p+1)^3 weights over the brick's tile in LDS in fixed point;
 // the tile is written as float times 1/gvol.
@@ -559,11 +602,7 @@ __global__ void __launch_bounds__(kTileThreads) k_density_tile(
     // tile rows go to the halo-expanded grid: brick k at k * TP per axis
     const long cstride = (long)tg.ex[0] * tg.ex[1] * tg.ex[2];
     float* dst = tiles + (long)k0 * TP + (long)tg.ex[0] * (k1 * TP + (long)tg.ex[1] * (k2 * TP));
-    auto out = [&](int i) -> float* {
-        const int c = i / T3, t = i - c * T3;
-        const int t0 = t % TP, t12 = t / TP, t1 = t12 % TP, t2 = t12 / TP;
-        return dst + c * cstride + t0 + (long)tg.ex[0] * (t1 + (long)tg.ex[1] * t2);
-    };
+    const int nt = nchan * T3;
     if (narrow) {
         for (int i = threadIdx.x; i < nchan * T3; i += kTileThreads) acc32[i] = 0u;
         __syncthreads();
@@ -571,7 +610,9 @@ __global__ void __launch_bounds__(kTileThreads) k_density_tile(
                                           nx, ny, nz, o0, o1, o2, tg, pp0, typ0);
         __syncthreads();
         const double scale = inv_gvol / (double)(1u << sc);
-        for (int i = threadIdx.x; i < nchan * T3; i += kTileThreads) *out(i) = (float)((double)acc32[i] * scale);
+        TileWalk<TP> w(threadIdx.x, tg.ex[0], tg.ex[1], cstride);
+        for (int i = threadIdx.x; i < nt; i += kTileThreads, w.advance())
+            dst[w.off] = (float)((double)acc32[i] * scale);
     } else {
         for (int i = threadIdx.x; i < nchan * T3; i += kTileThreads) acc64[i] = 0ull;
         __syncthreads();
@@ -579,8 +620,9 @@ __global__ void __launch_bounds__(kTileThreads) k_density_tile(
                                                 dx2, nx, ny, nz, o0, o1, o2, tg, pp0, typ0);
         __syncthreads();
         const double scale = inv_gvol / 2147483648.0;
-        for (int i = threadIdx.x; i < nchan * T3; i += kTileThreads)
-            *out(i) = (float)((double)(long long)acc64[i] * scale);
+        TileWalk<TP> w(threadIdx.x, tg.ex[0], tg.ex[1], cstride);
+        for (int i = threadIdx.x; i < nt; i += kTileThreads, w.advance())
+            dst[w.off] = (float)((double)(long long)acc64[i] * scale);
     }
 }
 
