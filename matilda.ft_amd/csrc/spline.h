@@ -78,10 +78,18 @@ __device__ __forceinline__ int base_index(float x, float dx, int p)
     return (int)__fdiv_rn(x, dx);
 }
 
+// i mod n in [0, n) (n > 0).  Indices within one period of [0, n) -- every
+// base index of a coordinate inside the box, and the stencil and window
+// offsets from one -- take a compare and an add; only lanes further out
+// (a branch the wave skips when it has none) divide.
 __device__ __forceinline__ int wrap(int i, int n)
 {
-    i %= n;
-    return i < 0 ? i + n : i;
+    int r = i < 0 ? i + n : (i >= n ? i - n : i);
+    if ((unsigned)r >= (unsigned)n) {
+        r = i % n;
+        r = r < 0 ? r + n : r;
+    }
+    return r;
 }
 
 } // namespace mft
