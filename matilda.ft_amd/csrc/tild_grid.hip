@@ -126,7 +126,70 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd_tiles(const float* __restrict
     const float* tc = tiles + c * cstride;
     constexpr int PER = (B * N + 255) / 256;
     float sums[PER];
-    if (cover2) {
+    bool lane_cover = false;
+    if constexpr (N % 64 == 0) lane_cover = cover2 && cstride < (1L << 30);
+    if (lane_cover) {
+        // at most two covering tiles per axis, and a wavefront's 64 lanes on
+        // one line (N a multiple of 64): the line's z and y cover rows, and
+        // which of its (z, y) entries exist, are uniform over the wave (scalar
+        // loads and branches); only the x cover is per lane.  Entries a line
+        // lacks are not loaded; a lane without a second x entry re-reads its
+        // first.  Byte offsets within a channel fit 32 bits (cstride < 2^30),
+        // so every load is a base plus a 32-bit lane offset.
+        const char* tb = reinterpret_cast<const char*>(tc);
+        int cz[PER], cy[PER], oz[PER][2], oy[PER][2], cx[PER], bx[PER][2];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = threadIdx.x + 256 * k;
+            const int l = __builtin_amdgcn_readfirstlane(i / N);
+            const int x = i - l * N;
+            const int line = blockIdx.x * B + l;
+            const bool valid = l < B && line < nlines;
+            const int gy = valid ? line % ny : 0, gz = valid ? line / ny : 0;
+            const int* tz = tab + 5 * (N + ny + gz);
+            const int* ty = tab + 5 * (N + gy);
+            const int* tx = tabx + 5 * x;
+            cz[k] = valid ? tz[0] : 0;
+            cy[k] = ty[0];
+            oz[k][0] = tz[1];
+            oz[k][1] = tz[2];
+            oy[k][0] = ty[1];
+            oy[k][1] = ty[2];
+            cx[k] = tx[0];
+            bx[k][0] = 4 * tx[1];
+            bx[k][1] = 4 * (tx[0] > 1 ? tx[2] : tx[1]);
+        }
+        float v[PER][8];
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    float v0 = 0.f, v1 = 0.f;
+                    if (a < cz[k] && b < cy[k]) {
+                        const unsigned o = 4u * (unsigned)(oz[k][a] + oy[k][b]);
+                        v0 = *reinterpret_cast<const float*>(tb + (o + (unsigned)bx[k][0]));
+                        v1 = *reinterpret_cast<const float*>(tb + (o + (unsigned)bx[k][1]));
+                    }
+                    v[k][4 * a + 2 * b] = v0;
+                    v[k][4 * a + 2 * b + 1] = v1;
+                }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            float sum = 0.f;
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int d = 0; d < 2; ++d) {
+                        const bool in = (a < cz[k]) & (b < cy[k]) & (d < cx[k]);
+                        sum = in ? sum + v[k][4 * a + 2 * b + d] : sum;
+                    }
+            sums[k] = sum;
+        }
+    } else if (cover2) {
         // at most two covering tiles per axis: every point's cover rows are
         // read first, then all its (up to eight) tile entries are loaded
         // together, branch-free, and summed in k_tile_reduce's order
