@@ -979,7 +979,9 @@ int mft_tild_kernel_bytes(mft_tild* e, double* bytes)
     // under GJF the previous noise is regenerated, not carried (16 bytes in
     // and out of the permutation and of the integrator fewer)
     const double nz = e->d.integrator == MFT_INT_GJF ? 0.0 : 16.0;
-    bytes[MFT_TK_SCATTER] = N * ((4 + 4 + 4 + 12 + nz + 16 + 8 + 4 + 4 * mb) + (16 + 12 + nz + 8 + 4 + 4 * mb + 4) +
+    // bond types per slot only with more than one type (else none carried)
+    const double bt = e->bp.ntypes > 1 ? 4 * mb : 0.0;
+    bytes[MFT_TK_SCATTER] = N * ((4 + 4 + 4 + 12 + nz + 16 + 8 + 4 + bt) + (16 + 12 + nz + 8 + 4 + bt + 4) +
                                  (4 + 4 + 8 * mb + 4 * mb));
     bytes[MFT_TK_DENSITY] = N * 16 + M * 4 * C;               // key-ordered positions, rho
     const bool plane = e->plane_passes && tild_plane_supported(e->d.nx, e->d.ny);
@@ -995,7 +997,7 @@ int mft_tild_kernel_bytes(mft_tild* e, double* bytes)
     bytes[MFT_TK_CONV_Z] = M * 8 * (C + O);
     // per-slot grid forces and next positions as 12-byte records
     bytes[MFT_TK_GATHER] = N * (16 + 8 + 12) + M * 12 * O4;
-    bytes[MFT_TK_INTEGRATE] = N * ((16 + 12 + nz + 8 + 4 + 8 * mb + 12) + (12 + nz + 4 + 4));
+    bytes[MFT_TK_INTEGRATE] = N * ((16 + 12 + nz + 8 + 4 + 4 * mb + bt + 12) + (12 + nz + 4 + 4));
     return MFT_OK;
 }
 

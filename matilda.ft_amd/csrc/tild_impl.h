@@ -317,6 +317,11 @@ struct TildEngine {
     int64_t tk_n[MFT_TK_COUNT];
 };
 
+// Bond types of buffer c in key order, or null with at most one bond type:
+// every bond is then type 0 (set_topology checks the range), so the slots'
+// types are neither carried through the permutation nor read.
+inline int* slot_bond_types(const TildEngine* e, int c) { return e->bp.ntypes > 1 ? e->BT[c] : nullptr; }
+
 constexpr int kBlock = 256;
 inline unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 

@@ -350,7 +350,8 @@ __global__ void k_scan_apply(int* __restrict__ count, int M, const int* __restri
 
 // Particle state into the next step's key order: the slot's next position
 // and noise (from the integrator), its current position as the next previous
-// position, identity and bonds; map[slot] = new slot.
+// position, identity and bonds; map[slot] = new slot.  BT/BT2 null: a single
+// bond type, whose slots' types are not carried (every bond is type 0).
 __global__ void k_permute(int ns, int maxbonds, const int* __restrict__ key, const int* __restrict__ rank,
                           const int* __restrict__ start, int* __restrict__ map, const float4* __restrict__ XN,
                           const float4* __restrict__ NN, const float4* __restrict__ X, const int2* __restrict__ META,
@@ -369,7 +370,8 @@ __global__ void k_permute(int ns, int maxbonds, const int* __restrict__ key, con
     if (NN) NZ2[n] = NN[s];
     META2[n] = META[s];
     NB2[n] = NB[s];
-    for (int b = 0; b < maxbonds; ++b) BT2[n * maxbonds + b] = BT[s * maxbonds + b];
+    if (BT2)
+        for (int b = 0; b < maxbonds; ++b) BT2[n * maxbonds + b] = BT[s * maxbonds + b];
 }
 
 // bond partner slots into the new order (after every map entry is written)
@@ -1082,7 +1084,7 @@ __device__ __forceinline__ void bond_force_slots(int s, int ns, const float4* __
     if (nb > maxbonds) nb = maxbonds;
     for (int b = 0; b < nb; ++b) {
         const int s2 = BS[s * maxbonds + b];
-        const int bt = BT[s * maxbonds + b];
+        const int bt = BT ? BT[s * maxbonds + b] : 0;
         if (s2 < 0 || s2 >= ns || bt < 0 || bt >= bp.ntypes) continue;
         const float4 y = X[s2];
         const float d0 = min_image(x0 - y.x, L0, 0.5f * L0);
@@ -1386,7 +1388,7 @@ __global__ void __launch_bounds__(256) k_integrate(
         if constexpr (TWO) {
             nb = NB[s];
             bs = reinterpret_cast<const int2*>(BS)[s];
-            bt = reinterpret_cast<const int2*>(BT)[s];
+            if (BT) bt = reinterpret_cast<const int2*>(BT)[s];
         }
         const uint32_t step = dstep ? *dstep : A.step;
         float fo[3] = {fg.x, fg.y, fg.z};
@@ -1459,7 +1461,7 @@ __global__ void k_bond_virial_slots(float* __restrict__ vir, const float4* __res
     float v[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int b = 0; b < NB[s]; ++b) {
         const int s2 = BS[s * maxbonds + b];
-        const int bt = BT[s * maxbonds + b];
+        const int bt = BT ? BT[s * maxbonds + b] : 0;
         const float4 y = X[s2];
         const float yv[3] = {y.x, y.y, y.z};
         float d[3], r2 = 0.f;
@@ -1491,7 +1493,7 @@ int tild_bond_virial(TildEngine* e, float* vir)
 {
     const int c = e->cur;
     k_bond_virial_slots<<<nblk(e->N), kBlock, 0, e->stream>>>(vir, e->X[c], e->META[c], e->NB[c], e->BS[c],
-                                                               e->BT[c], e->bp, e->d.lx, e->d.ly, e->d.lz, e->N,
+                                                               slot_bond_types(e, c), e->bp, e->d.lx, e->d.ly, e->d.lz, e->N,
                                                                std::max(1, e->d.maxbonds));
     MFT_LAUNCH_CHECK();
     return MFT_OK;
@@ -1512,7 +1514,7 @@ __global__ void k_bond_energy_slots(double* __restrict__ partial, const float4* 
         const float xv[3] = {x.x, x.y, x.z};
         for (int b = 0; b < NB[s]; ++b) {
             const int s2 = BS[s * maxbonds + b];
-            const int bt = BT[s * maxbonds + b];
+            const int bt = BT ? BT[s * maxbonds + b] : 0;
             const float4 y = X[s2];
             const float yv[3] = {y.x, y.y, y.z};
             float r2 = 0.f;
@@ -1702,8 +1704,8 @@ int tild_permute(TildEngine* e)
     // under GJF the next step regenerates the previous noise: NZ stays behind
     const float4* nn = e->d.integrator == MFT_INT_GJF ? nullptr : e->NN;
     k_permute<<<nblk(N), kBlock, 0, e->stream>>>(N, mb, e->key, e->rank, e->start, e->map, e->XN, nn, e->X[c],
-                                                  e->META[c], e->NB[c], e->BT[c], e->X[n], e->XO[n], e->NZ[n],
-                                                  e->META[n], e->NB[n], e->BT[n]);
+                                                  e->META[c], e->NB[c], slot_bond_types(e, c), e->X[n], e->XO[n],
+                                                  e->NZ[n], e->META[n], e->NB[n], slot_bond_types(e, n));
     k_remap_bonds<<<nblk(N), kBlock, 0, e->stream>>>(N, mb, e->map, e->NB[c], e->BS[c], e->BS[n]);
     if (e->d.maxangles > 0)
         k_permute_angles<<<nblk(N), kBlock, 0, e->stream>>>(N, e->d.maxangles, e->map, e->NA[c], e->AT[c], e->AG[c],
@@ -1926,13 +1928,15 @@ int tild_integrate(TildEngine* e, bool want_f)
     const int c = e->cur;
     const uint32_t* ds = e->use_dstep ? e->d_step : nullptr;
     if (d.maxbonds == 2)
-        k_integrate<true><<<nblk(e->N), kBlock, 0, e->stream>>>(e->X[c], e->META[c], e->NB[c], e->BS[c], e->BT[c],
-                                                              e->fgrid, want_f ? e->ftot : nullptr, 2, e->bp,
+        k_integrate<true><<<nblk(e->N), kBlock, 0, e->stream>>>(e->X[c], e->META[c], e->NB[c], e->BS[c],
+                                                              slot_bond_types(e, c), e->fgrid,
+                                                              want_f ? e->ftot : nullptr, 2, e->bp,
                                                               step_args(e), e->N, e->count, e->key, e->rank, e->tg,
                                                               ds);
     else
-        k_integrate<false><<<nblk(e->N), kBlock, 0, e->stream>>>(e->X[c], e->META[c], e->NB[c], e->BS[c], e->BT[c],
-                                                               e->fgrid, want_f ? e->ftot : nullptr,
+        k_integrate<false><<<nblk(e->N), kBlock, 0, e->stream>>>(e->X[c], e->META[c], e->NB[c], e->BS[c],
+                                                               slot_bond_types(e, c), e->fgrid,
+                                                               want_f ? e->ftot : nullptr,
                                                                std::max(1, d.maxbonds), e->bp, step_args(e), e->N,
                                                                e->count, e->key, e->rank, e->tg, ds);
     MFT_LAUNCH_CHECK();
@@ -2027,7 +2031,7 @@ int tild_bond_energy(TildEngine* e)
     const mft_tild_desc& d = e->d;
     const int c = e->cur;
     unsigned nb = nblk(e->N);
-    k_bond_energy_slots<<<nb, kBlock, 0, e->stream>>>(e->bpart, e->X[c], e->NB[c], e->BS[c], e->BT[c], e->bp, d.lx,
+    k_bond_energy_slots<<<nb, kBlock, 0, e->stream>>>(e->bpart, e->X[c], e->NB[c], e->BS[c], slot_bond_types(e, c), e->bp, d.lx,
                                                        d.ly, d.lz, e->N, std::max(1, d.maxbonds));
     k_sum_partials<<<1, 1024, 0, e->stream>>>(e->eout + kMaxPots, e->bpart, nb);
     MFT_LAUNCH_CHECK();

@@ -710,6 +710,40 @@ def test_bond_virial_matches_oracle(gpu):
     eng.close()
 
 
+def test_two_bond_types_match_oracle(gpu):
+    """Bond types carried through the key order: a diblock whose A and B
+    blocks have their own harmonic bond (k, r0), so the permutation must move
+    every slot's types with it.  Three GJF steps, the bond energy and the bond
+    virial after the re-keying, against the oracle.  (With one bond type the
+    slots' types are not carried: every bond is type 0.)"""
+    cfg = systems.tild_diblock(nchains=60, NA=5, NB=5, box=6.0, grid=16, seed=21)
+    cfg["topo"] = systems.make_linear(60, [(5, 0), (5, 1)], block_bond_type=[1, 2])
+    cfg["bond_k"], cfg["bond_r0"], cfg["bond_style"] = [1.5, 2.5], [0.0, 0.4], [0, 0]
+    assert set(np.unique(cfg["topo"]["bondType"][cfg["topo"]["nBonds"].repeat(2) > 0])) == {0, 1}
+    say("case: two bond types")
+    eng, sys_o = _engine_vs_oracle(gpu, cfg)
+    ns = cfg["topo"]["type"].size
+    for step in range(3):
+        eng.step(1)
+        xi = orc.tild_noise(ns, cfg["seed"], step)
+        out = orc.tild_step(sys_o, xi)
+        sys_o.x = out["x"].astype(np.float32)
+        sys_o.xo = out["xo"].astype(np.float32)
+        sys_o.noise_old = out["noise_old"].astype(np.float32)
+        x_g, _, _ = eng.get_state()
+        assert np.max(periodic_dist(x_g, out["x"], cfg["L"])) < 1e-6 * max(cfg["L"]), step
+    topo = sys_o.topo
+    bonds = dict(nBonds=topo["nBonds"], bondedTo=topo["bondedTo"], bondType=topo["bondType"],
+                 bondReq=cfg["bond_r0"], bondK=cfg["bond_k"], bondStyle=cfg["bond_style"], maxbonds=2)
+    x, _, _ = eng.get_state()
+    _, be = eng.energies()
+    eb = orc.bond_energy(bonds, x, cfg["L"])
+    assert be == pytest.approx(0.5 * eb.sum(), rel=TOL)
+    v = eng.bond_virial()
+    assert rel(v, orc.bond_virial(bonds, x, cfg["L"])) < TOL
+    eng.close()
+
+
 @pytest.mark.parametrize("integrator", ["GJF", "EM"])
 def test_replica_group_graph_equals_engines_alone(gpu, integrator):
     """Replicas advanced by the group's HIP graph (eight steps of every engine
