@@ -182,7 +182,6 @@ static int tild_setup(TildEngine* e)
         if ((rc = dalloc(&e->XO[b], (size_t)N))) return rc;
         if ((rc = dalloc(&e->NZ[b], (size_t)N))) return rc;
         if ((rc = dalloc(&e->META[b], (size_t)N))) return rc;
-        if ((rc = dalloc(&e->NB[b], (size_t)N))) return rc;
         if ((rc = dalloc(&e->BS[b], (size_t)N * mb))) return rc;
         if ((rc = dalloc(&e->BT[b], (size_t)N * mb))) return rc;
     }
@@ -572,8 +571,8 @@ int mft_tild_create(mft_tild** out, const mft_tild_desc* desc)
 int mft_tild_destroy(mft_tild* e)
 {
     if (!e) return MFT_OK;
-    void* ptrs[] = {e->X[0], e->X[1], e->XO[0], e->XO[1], e->NZ[0], e->NZ[1], e->META[0], e->META[1], e->NB[0],
-                    e->NB[1], e->BS[0], e->BS[1], e->BT[0], e->BT[1], e->XN, e->NN, e->fgrid, e->ftot, e->key,
+    void* ptrs[] = {e->X[0], e->X[1], e->XO[0], e->XO[1], e->NZ[0], e->NZ[1], e->META[0], e->META[1],
+                    e->BS[0], e->BS[1], e->BT[0], e->BT[1], e->XN, e->NN, e->fgrid, e->ftot, e->key,
                     e->rank, e->map, e->slot_of, e->type, e->charge, e->nBonds, e->bondedTo, e->bondType, e->pio,
                     e->pio2, e->pio3, e->count, e->start, e->partial, e->rho, e->S, e->Y, e->F4, e->force_slot,
                     e->epart, e->eout, e->bpart, e->tiles, e->tile_tab, e->NA[0], e->NA[1], e->AT[0], e->AT[1],
@@ -974,15 +973,16 @@ int mft_tild_kernel_bytes(mft_tild* e, double* bytes)
     const double mb = e->d.maxbonds > 0 ? e->d.maxbonds : 1;
     bytes[MFT_TK_BIN] = 0;                                    // sorting happens when the state is set
     bytes[MFT_TK_SCAN] = (double)e->nkeys * (4 + 4 + 4);      // count read, start write, count clear
-    // permutation: key, rank, start, next x and noise, x, id/species, bonds
-    // in; x, previous x, noise, id/species, bonds, map out; bond slots remapped
+    // permutation: key, rank, start, next x and noise, x, id/species, bond
+    // types in; x, previous x, noise, id/species, bond types, map out; bond
+    // slots remapped (no bond count: a slot's bonds are its partners >= 0)
     // under GJF the previous noise is regenerated, not carried (16 bytes in
     // and out of the permutation and of the integrator fewer)
     const double nz = e->d.integrator == MFT_INT_GJF ? 0.0 : 16.0;
     // bond types per slot only with more than one type (else none carried)
     const double bt = e->bp.ntypes > 1 ? 4 * mb : 0.0;
-    bytes[MFT_TK_SCATTER] = N * ((4 + 4 + 4 + 12 + nz + 16 + 8 + 4 + bt) + (16 + 12 + nz + 8 + 4 + bt + 4) +
-                                 (4 + 4 + 8 * mb + 4 * mb));
+    bytes[MFT_TK_SCATTER] = N * ((4 + 4 + 4 + 12 + nz + 16 + 8 + bt) + (16 + 12 + nz + 8 + bt + 4) +
+                                 (4 + 8 * mb + 4 * mb));
     bytes[MFT_TK_DENSITY] = N * 16 + M * 4 * C;               // key-ordered positions, rho
     const bool plane = e->plane_passes && tild_plane_supported(e->d.nx, e->d.ny);
     if (plane) {
@@ -997,7 +997,7 @@ int mft_tild_kernel_bytes(mft_tild* e, double* bytes)
     bytes[MFT_TK_CONV_Z] = M * 8 * (C + O);
     // per-slot grid forces and next positions as 12-byte records
     bytes[MFT_TK_GATHER] = N * (16 + 8 + 12) + M * 12 * O4;
-    bytes[MFT_TK_INTEGRATE] = N * ((16 + 12 + nz + 8 + 4 + 4 * mb + bt + 12) + (12 + nz + 4 + 4));
+    bytes[MFT_TK_INTEGRATE] = N * ((16 + 12 + nz + 8 + 4 * mb + bt + 12) + (12 + nz + 4 + 4));
     return MFT_OK;
 }
 

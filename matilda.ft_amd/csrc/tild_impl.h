@@ -224,8 +224,7 @@ struct TildEngine {
     float4* XO[2] = {nullptr, nullptr};  // previous positions (GJF), 12 bytes a slot (ff_load / ff_store)
     float4* NZ[2] = {nullptr, nullptr};  // previous noise (GJF)
     int2* META[2] = {nullptr, nullptr};  // particle id, species
-    int* NB[2] = {nullptr, nullptr};     // bond count
-    int* BS[2] = {nullptr, nullptr};     // bond partners as slots [N][maxbonds]
+    int* BS[2] = {nullptr, nullptr};     // bond partners as slots [N][maxbonds], -1 past the last
     int* BT[2] = {nullptr, nullptr};     // bond types [N][maxbonds]
     int* NA[2] = {nullptr, nullptr};     // angle count (angles only)
     int* AT[2] = {nullptr, nullptr};     // angle types [N][maxangles]

@@ -224,11 +224,10 @@ __global__ void k_sort_count(const float* __restrict__ x, const int* __restrict_
 // Initial sort: every particle's state into its slot.
 __global__ void k_sort_place(const float* __restrict__ x, const float* __restrict__ xo,
                              const float* __restrict__ noise, const float* __restrict__ q,
-                             const int* __restrict__ type, const int* __restrict__ nBonds,
-                             const int* __restrict__ bondType, int maxbonds, int ns, const int* __restrict__ key,
+                             const int* __restrict__ type, const int* __restrict__ bondType, int maxbonds, int ns, const int* __restrict__ key,
                              const int* __restrict__ rank, const int* __restrict__ start, int* __restrict__ slot_of,
                              float4* __restrict__ X, float4* __restrict__ XO, float4* __restrict__ NZ,
-                             int2* __restrict__ META, int* __restrict__ NB, int* __restrict__ BT)
+                             int2* __restrict__ META, int* __restrict__ BT)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ns) return;
@@ -238,18 +237,18 @@ __global__ void k_sort_place(const float* __restrict__ x, const float* __restric
     ff_store(XO, s, xo[i * 3], xo[i * 3 + 1], xo[i * 3 + 2]);
     NZ[s] = make_float4(noise[i * 3], noise[i * 3 + 1], noise[i * 3 + 2], 0.f);
     META[s] = make_int2(i, type[i]);
-    NB[s] = nBonds[i];
     for (int b = 0; b < maxbonds; ++b) BT[s * maxbonds + b] = bondType[i * maxbonds + b];
 }
 
-// Initial sort: bond partners as slots.
+// Initial sort: bond partners as slots, -1 past the particle's bonds (the
+// key-ordered state keeps no bond count: a slot's bonds are its entries >= 0).
 __global__ void k_sort_bonds(const int* __restrict__ bondedTo, const int* __restrict__ nBonds, int maxbonds, int ns,
                              const int* __restrict__ slot_of, int* __restrict__ BS)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ns) return;
     const int s = slot_of[i];
-    for (int b = 0; b < maxbonds; ++b) BS[s * maxbonds + b] = b < nBonds[i] ? slot_of[bondedTo[i * maxbonds + b]] : 0;
+    for (int b = 0; b < maxbonds; ++b) BS[s * maxbonds + b] = b < nBonds[i] ? slot_of[bondedTo[i * maxbonds + b]] : -1;
 }
 
 // Initial sort of the angles: counts, types and members as slots.
@@ -355,9 +354,8 @@ __global__ void k_scan_apply(int* __restrict__ count, int M, const int* __restri
 __global__ void k_permute(int ns, int maxbonds, const int* __restrict__ key, const int* __restrict__ rank,
                           const int* __restrict__ start, int* __restrict__ map, const float4* __restrict__ XN,
                           const float4* __restrict__ NN, const float4* __restrict__ X, const int2* __restrict__ META,
-                          const int* __restrict__ NB, const int* __restrict__ BT, float4* __restrict__ X2,
-                          float4* __restrict__ XO2, float4* __restrict__ NZ2, int2* __restrict__ META2,
-                          int* __restrict__ NB2, int* __restrict__ BT2)
+                          const int* __restrict__ BT, float4* __restrict__ X2, float4* __restrict__ XO2,
+                          float4* __restrict__ NZ2, int2* __restrict__ META2, int* __restrict__ BT2)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
@@ -369,27 +367,29 @@ __global__ void k_permute(int ns, int maxbonds, const int* __restrict__ key, con
     ff_store(XO2, n, xc.x, xc.y, xc.z);
     if (NN) NZ2[n] = NN[s];
     META2[n] = META[s];
-    NB2[n] = NB[s];
     if (BT2)
         for (int b = 0; b < maxbonds; ++b) BT2[n * maxbonds + b] = BT[s * maxbonds + b];
 }
 
 // bond partner slots into the new order (after every map entry is written)
-__global__ void k_remap_bonds(int ns, int maxbonds, const int* __restrict__ map, const int* __restrict__ NB,
-                              const int* __restrict__ BS, int* __restrict__ BS2)
+__global__ void k_remap_bonds(int ns, int maxbonds, const int* __restrict__ map, const int* __restrict__ BS,
+                              int* __restrict__ BS2)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
-    const int n = map[s], nb = NB[s];
+    const int n = map[s];
     if (maxbonds == 2) {
         // both partners' map entries loaded together
         const int2 b = reinterpret_cast<const int2*>(BS)[s];
-        const int m0 = nb > 0 ? map[b.x] : 0;
-        const int m1 = nb > 1 ? map[b.y] : 0;
+        const int m0 = b.x >= 0 ? map[b.x] : -1;
+        const int m1 = b.y >= 0 ? map[b.y] : -1;
         reinterpret_cast<int2*>(BS2)[n] = make_int2(m0, m1);
         return;
     }
-    for (int b = 0; b < maxbonds; ++b) BS2[n * maxbonds + b] = b < nb ? map[BS[s * maxbonds + b]] : 0;
+    for (int b = 0; b < maxbonds; ++b) {
+        const int p = BS[s * maxbonds + b];
+        BS2[n * maxbonds + b] = p >= 0 ? map[p] : -1;
+    }
 }
 
 // angles into the new order (after every map entry is written)
@@ -1075,14 +1075,12 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_charge(
 // key order.  Writes the next position and noise of the slot and counts the
 // next key; integ = MFT_INT_NONE only writes the total force.
 __device__ __forceinline__ void bond_force_slots(int s, int ns, const float4* __restrict__ X,
-                                                 const int* __restrict__ NB, const int* __restrict__ BS,
+                                                 const int* __restrict__ BS,
                                                  const int* __restrict__ BT, int maxbonds, const BondParams& bp,
                                                  float L0, float L1, float L2, float x0, float x1, float x2,
                                                  float* fo)
 {
-    int nb = NB[s];
-    if (nb > maxbonds) nb = maxbonds;
-    for (int b = 0; b < nb; ++b) {
+    for (int b = 0; b < maxbonds; ++b) {
         const int s2 = BS[s * maxbonds + b];
         const int bt = BT ? BT[s * maxbonds + b] : 0;
         if (s2 < 0 || s2 >= ns || bt < 0 || bt >= bp.ntypes) continue;
@@ -1361,8 +1359,8 @@ __device__ __forceinline__ int integrate_slot(int s, float4 xc, int2 meta, const
 // step index on the device (graph replays), else null.
 template <bool TWO>
 __global__ void __launch_bounds__(256) k_integrate(
-    const float4* __restrict__ X, const int2* __restrict__ META, const int* __restrict__ NB,
-    const int* __restrict__ BS, const int* __restrict__ BT, const float4* __restrict__ fgrid,
+    const float4* __restrict__ X, const int2* __restrict__ META, const int* __restrict__ BS,
+    const int* __restrict__ BT, const float4* __restrict__ fgrid,
     float4* __restrict__ ftot, int maxbonds, BondParams bp, StepArgs A, int ns, int* __restrict__ count,
     int* __restrict__ key, int* __restrict__ rank, TileGeom tg, const uint32_t* __restrict__ dstep)
 {
@@ -1383,10 +1381,8 @@ __global__ void __launch_bounds__(256) k_integrate(
         const float3 xo3 = gjf ? ff_load(A.XO, s) : make_float3(0.f, 0.f, 0.f);
         const float4 xo4 = make_float4(xo3.x, xo3.y, xo3.z, 0.f);
         const float4 nz4 = need_nz ? A.NZ[s] : make_float4(0.f, 0.f, 0.f, 0.f);
-        int nb = 0;
-        int2 bs = make_int2(0, 0), bt = make_int2(0, 0);
+        int2 bs = make_int2(-1, -1), bt = make_int2(0, 0);
         if constexpr (TWO) {
-            nb = NB[s];
             bs = reinterpret_cast<const int2*>(BS)[s];
             if (BT) bt = reinterpret_cast<const int2*>(BT)[s];
         }
@@ -1394,8 +1390,8 @@ __global__ void __launch_bounds__(256) k_integrate(
         float fo[3] = {fg.x, fg.y, fg.z};
         SpeciesCoef sco;
         if constexpr (TWO) {
-            const bool v0 = (nb > 0) & (bs.x >= 0) & (bs.x < ns) & (bt.x >= 0) & (bt.x < bp.ntypes);
-            const bool v1 = (nb > 1) & (bs.y >= 0) & (bs.y < ns) & (bt.y >= 0) & (bt.y < bp.ntypes);
+            const bool v0 = (bs.x >= 0) & (bs.x < ns) & (bt.x >= 0) & (bt.x < bp.ntypes);
+            const bool v1 = (bs.y >= 0) & (bs.y < ns) & (bt.y >= 0) & (bt.y < bp.ntypes);
             // the bond types' parameters (indexed per lane, so read from
             // memory) are loaded with the partners' positions, not after them
             const int t0 = v0 ? bt.x : 0, t1 = v1 ? bt.y : 0;
@@ -1409,7 +1405,7 @@ __global__ void __launch_bounds__(256) k_integrate(
             if (v1) bond_force_params(st1, k1, q1, A.L[0], A.L[1], A.L[2], xc, y1, fo);
         } else {
             sco = species_coef(A, meta.y);
-            bond_force_slots(s, ns, X, NB, BS, BT, maxbonds, bp, A.L[0], A.L[1], A.L[2], xc.x, xc.y, xc.z, fo);
+            bond_force_slots(s, ns, X, BS, BT, maxbonds, bp, A.L[0], A.L[1], A.L[2], xc.x, xc.y, xc.z, fo);
         }
         if (A.NA) angle_force_slots(s, X, A.NA, A.AT, A.AG, A.ap, A.L[0], A.L[1], A.L[2], fo);
         if (A.walls.n) wall_forces(A.walls, meta.y, xc, fo);
@@ -1449,8 +1445,8 @@ __global__ void __launch_bounds__(256) k_integrate(
 // d_bondVirScratch); dr with the 0.5*Lh minimum image (:132-133); no term for
 // r^2 <= 1e-4 or for FENE at or beyond r0, where the reference leaves mf unset.
 __global__ void k_bond_virial_slots(float* __restrict__ vir, const float4* __restrict__ X,
-                                    const int2* __restrict__ META, const int* __restrict__ NB,
-                                    const int* __restrict__ BS, const int* __restrict__ BT, BondParams bp, float L0,
+                                    const int2* __restrict__ META, const int* __restrict__ BS,
+                                    const int* __restrict__ BT, BondParams bp, float L0,
                                     float L1, float L2, int ns, int maxbonds)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1459,8 +1455,9 @@ __global__ void k_bond_virial_slots(float* __restrict__ vir, const float4* __res
     const float4 x = X[s];
     const float xv[3] = {x.x, x.y, x.z};
     float v[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    for (int b = 0; b < NB[s]; ++b) {
+    for (int b = 0; b < maxbonds; ++b) {
         const int s2 = BS[s * maxbonds + b];
+        if (s2 < 0) continue;
         const int bt = BT ? BT[s * maxbonds + b] : 0;
         const float4 y = X[s2];
         const float yv[3] = {y.x, y.y, y.z};
@@ -1492,7 +1489,7 @@ __global__ void k_bond_virial_slots(float* __restrict__ vir, const float4* __res
 int tild_bond_virial(TildEngine* e, float* vir)
 {
     const int c = e->cur;
-    k_bond_virial_slots<<<nblk(e->N), kBlock, 0, e->stream>>>(vir, e->X[c], e->META[c], e->NB[c], e->BS[c],
+    k_bond_virial_slots<<<nblk(e->N), kBlock, 0, e->stream>>>(vir, e->X[c], e->META[c], e->BS[c],
                                                                slot_bond_types(e, c), e->bp, e->d.lx, e->d.ly, e->d.lz, e->N,
                                                                std::max(1, e->d.maxbonds));
     MFT_LAUNCH_CHECK();
@@ -1502,7 +1499,7 @@ int tild_bond_virial(TildEngine* e, float* vir)
 // d_bondStressEnergy energy part (ps_deviceBonds.cu:85-171) over slots, FP64
 // block sums, with the reference's 0.5*Lh minimum image (:132-133).
 __global__ void k_bond_energy_slots(double* __restrict__ partial, const float4* __restrict__ X,
-                                    const int* __restrict__ NB, const int* __restrict__ BS,
+                                    const int* __restrict__ BS,
                                     const int* __restrict__ BT, BondParams bp, float L0, float L1, float L2, int ns,
                                     int maxbonds)
 {
@@ -1512,8 +1509,9 @@ __global__ void k_bond_energy_slots(double* __restrict__ partial, const float4* 
         const float Ls[3] = {L0, L1, L2};
         const float4 x = X[s];
         const float xv[3] = {x.x, x.y, x.z};
-        for (int b = 0; b < NB[s]; ++b) {
+        for (int b = 0; b < maxbonds; ++b) {
             const int s2 = BS[s * maxbonds + b];
+            if (s2 < 0) continue;
             const int bt = BT ? BT[s * maxbonds + b] : 0;
             const float4 y = X[s2];
             const float yv[3] = {y.x, y.y, y.z};
@@ -1643,9 +1641,9 @@ int tild_sort_state(TildEngine* e)
     MFT_LAUNCH_CHECK();
     int rc;
     if ((rc = tild_scan(e))) return rc;
-    k_sort_place<<<nblk(N), kBlock, 0, e->stream>>>(e->pio, e->pio2, e->pio3, e->charge, e->type, e->nBonds,
+    k_sort_place<<<nblk(N), kBlock, 0, e->stream>>>(e->pio, e->pio2, e->pio3, e->charge, e->type,
                                                      e->bondType, mb, N, e->key, e->rank, e->start, e->slot_of,
-                                                     e->X[c], e->XO[c], e->NZ[c], e->META[c], e->NB[c], e->BT[c]);
+                                                     e->X[c], e->XO[c], e->NZ[c], e->META[c], e->BT[c]);
     k_sort_bonds<<<nblk(N), kBlock, 0, e->stream>>>(e->bondedTo, e->nBonds, mb, N, e->slot_of, e->BS[c]);
     if (d.maxangles > 0)
         k_sort_angles<<<nblk(N), kBlock, 0, e->stream>>>(e->nAngles, e->angleType, e->angleGroup, d.maxangles, N,
@@ -1704,9 +1702,9 @@ int tild_permute(TildEngine* e)
     // under GJF the next step regenerates the previous noise: NZ stays behind
     const float4* nn = e->d.integrator == MFT_INT_GJF ? nullptr : e->NN;
     k_permute<<<nblk(N), kBlock, 0, e->stream>>>(N, mb, e->key, e->rank, e->start, e->map, e->XN, nn, e->X[c],
-                                                  e->META[c], e->NB[c], slot_bond_types(e, c), e->X[n], e->XO[n],
-                                                  e->NZ[n], e->META[n], e->NB[n], slot_bond_types(e, n));
-    k_remap_bonds<<<nblk(N), kBlock, 0, e->stream>>>(N, mb, e->map, e->NB[c], e->BS[c], e->BS[n]);
+                                                  e->META[c], slot_bond_types(e, c), e->X[n], e->XO[n], e->NZ[n],
+                                                  e->META[n], slot_bond_types(e, n));
+    k_remap_bonds<<<nblk(N), kBlock, 0, e->stream>>>(N, mb, e->map, e->BS[c], e->BS[n]);
     if (e->d.maxangles > 0)
         k_permute_angles<<<nblk(N), kBlock, 0, e->stream>>>(N, e->d.maxangles, e->map, e->NA[c], e->AT[c], e->AG[c],
                                                             e->NA[n], e->AT[n], e->AG[n]);
@@ -1928,13 +1926,13 @@ int tild_integrate(TildEngine* e, bool want_f)
     const int c = e->cur;
     const uint32_t* ds = e->use_dstep ? e->d_step : nullptr;
     if (d.maxbonds == 2)
-        k_integrate<true><<<nblk(e->N), kBlock, 0, e->stream>>>(e->X[c], e->META[c], e->NB[c], e->BS[c],
+        k_integrate<true><<<nblk(e->N), kBlock, 0, e->stream>>>(e->X[c], e->META[c], e->BS[c],
                                                               slot_bond_types(e, c), e->fgrid,
                                                               want_f ? e->ftot : nullptr, 2, e->bp,
                                                               step_args(e), e->N, e->count, e->key, e->rank, e->tg,
                                                               ds);
     else
-        k_integrate<false><<<nblk(e->N), kBlock, 0, e->stream>>>(e->X[c], e->META[c], e->NB[c], e->BS[c],
+        k_integrate<false><<<nblk(e->N), kBlock, 0, e->stream>>>(e->X[c], e->META[c], e->BS[c],
                                                                slot_bond_types(e, c), e->fgrid,
                                                                want_f ? e->ftot : nullptr,
                                                                std::max(1, d.maxbonds), e->bp, step_args(e), e->N,
@@ -2031,7 +2029,7 @@ int tild_bond_energy(TildEngine* e)
     const mft_tild_desc& d = e->d;
     const int c = e->cur;
     unsigned nb = nblk(e->N);
-    k_bond_energy_slots<<<nb, kBlock, 0, e->stream>>>(e->bpart, e->X[c], e->NB[c], e->BS[c], slot_bond_types(e, c), e->bp, d.lx,
+    k_bond_energy_slots<<<nb, kBlock, 0, e->stream>>>(e->bpart, e->X[c], e->BS[c], slot_bond_types(e, c), e->bp, d.lx,
                                                        d.ly, d.lz, e->N, std::max(1, d.maxbonds));
     k_sum_partials<<<1, 1024, 0, e->stream>>>(e->eout + kMaxPots, e->bpart, nb);
     MFT_LAUNCH_CHECK();
