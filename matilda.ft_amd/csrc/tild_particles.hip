@@ -737,18 +737,25 @@ __device__ __forceinline__ void stage_window(Win3* ft, int* xg, int* yo, int* zo
                                              int e2, int nx, int ny, int nz)
 {
     constexpr int TP = BM + P, shift = P / 2;
-    if ((int)threadIdx.x < TP) {
-        const int t = threadIdx.x;
-        xg[t] = wrap(o0 - shift + t, nx);
-        yo[t] = nx * wrap(o1 - shift + t, ny);
-        zo[t] = nx * ny * wrap(o2 - shift + t, nz);
-    }
+    // a full window that does not wrap (most bricks): its points split by the
+    // compile-time TP and address the field directly, with no tables
+    const int g0 = o0 - shift, g1 = o1 - shift, g2 = o2 - shift;
+    const bool direct = e0 == TP && e1 == TP && e2 == TP && g0 >= 0 && g1 >= 0 && g2 >= 0 && g0 + TP <= nx &&
+                        g1 + TP <= ny && g2 + TP <= nz;
     const int nrows = e1 * e2;
-    for (int r = threadIdx.x; r < nrows; r += kTileThreads) {
-        rows[r][0] = (unsigned char)(r % e1);
-        rows[r][1] = (unsigned char)(r / e1);
+    if (!direct) {
+        if ((int)threadIdx.x < TP) {
+            const int t = threadIdx.x;
+            xg[t] = wrap(o0 - shift + t, nx);
+            yo[t] = nx * wrap(o1 - shift + t, ny);
+            zo[t] = nx * ny * wrap(o2 - shift + t, nz);
+        }
+        for (int r = threadIdx.x; r < nrows; r += kTileThreads) {
+            rows[r][0] = (unsigned char)(r % e1);
+            rows[r][1] = (unsigned char)(r / e1);
+        }
+        __syncthreads();
     }
-    __syncthreads();
     // the window's points dealt to every lane in turn (x fastest, so a wave
     // reads a few whole rows); points past the window re-load the last point
     // (in bounds) and skip the store, so the arrays stay in registers.  Row r
@@ -759,14 +766,25 @@ __device__ __forceinline__ void stage_window(Win3* ft, int* xg, int* yo, int* zo
     const float ie0 = 1.0f / (float)e0;
     float3 v[KMAX];
     int dofs[KMAX];
+    if (direct) {
+        const int base = g0 + nx * (g1 + ny * g2);
 #pragma unroll
-    for (int k = 0; k < KMAX; ++k) {
-        const int q = min((int)threadIdx.x + kTileThreads * k, npts - 1);
-        const int r = (int)(((float)q + 0.5f) * ie0);
-        const int x = q - r * e0;
-        const int t1 = rows[r][0], t2 = rows[r][1];
-        v[k] = ff_load(F, xg[x] + yo[t1] + zo[t2]);
-        dofs[k] = x + TP * (t1 + TP * t2);
+        for (int k = 0; k < KMAX; ++k) {
+            const int q = min((int)threadIdx.x + kTileThreads * k, npts - 1);
+            const int x = q % TP, t1 = (q / TP) % TP, t2 = q / (TP * TP);
+            v[k] = ff_load(F, base + x + nx * (t1 + ny * t2));
+            dofs[k] = q;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+            const int q = min((int)threadIdx.x + kTileThreads * k, npts - 1);
+            const int r = (int)(((float)q + 0.5f) * ie0);
+            const int x = q - r * e0;
+            const int t1 = rows[r][0], t2 = rows[r][1];
+            v[k] = ff_load(F, xg[x] + yo[t1] + zo[t2]);
+            dofs[k] = x + TP * (t1 + TP * t2);
+        }
     }
 #pragma unroll
     for (int k = 0; k < KMAX; ++k)
