@@ -744,6 +744,44 @@ def test_two_bond_types_match_oracle(gpu):
     eng.close()
 
 
+def test_three_bond_slots_match_oracle(gpu):
+    """MAXBONDS = 3 (the general bond path: each slot's partner list ends at
+    its first -1 entry; the two-bond path is specialised): the diblock's bonds
+    with an empty third slot per particle, against the oracle's two-slot
+    layout of the same bonds.  Three GJF steps (re-keyed every step), then the
+    bond energy and virial."""
+    from mft import TildEngine
+    cfg = small_system(False, seed=23)
+    topo = cfg["topo"]
+    ns = topo["type"].size
+    bto = np.zeros((ns, 3), np.int32)
+    bty = np.zeros((ns, 3), np.int32)
+    bto[:, :2] = topo["bondedTo"].reshape(ns, 2)
+    bty[:, :2] = topo["bondType"].reshape(ns, 2)
+    say("case: three bond slots")
+    eng = TildEngine(cfg["Nx"], cfg["L"], cfg["pmeorder"], len(cfg["species"]), ns, cfg["potentials"],
+                     cfg["bond_k"], cfg["bond_r0"], cfg["bond_style"], integrator="GJF", delt=cfg["delt"],
+                     mass=cfg["mass"], mobility=cfg["mobility"], seed=cfg["seed"], has_charges=False, maxbonds=3)
+    eng.set_topology(topo["type"], None, topo["nBonds"], bto.reshape(-1), bty.reshape(-1))
+    eng.set_state(cfg["x"])
+    sys_o = oracle_system(cfg)
+    for step in range(3):
+        eng.step(1)
+        out = orc.tild_step(sys_o, orc.tild_noise(ns, cfg["seed"], step))
+        sys_o.x = out["x"].astype(np.float32)
+        sys_o.xo = out["xo"].astype(np.float32)
+        sys_o.noise_old = out["noise_old"].astype(np.float32)
+        x_g, _, _ = eng.get_state()
+        assert np.max(periodic_dist(x_g, out["x"], cfg["L"])) < 1e-6 * max(cfg["L"]), step
+    bonds = dict(nBonds=topo["nBonds"], bondedTo=topo["bondedTo"], bondType=topo["bondType"],
+                 bondReq=cfg["bond_r0"], bondK=cfg["bond_k"], bondStyle=cfg["bond_style"], maxbonds=2)
+    x, _, _ = eng.get_state()
+    _, be = eng.energies()
+    assert be == pytest.approx(0.5 * orc.bond_energy(bonds, x, cfg["L"]).sum(), rel=TOL)
+    assert rel(eng.bond_virial(), orc.bond_virial(bonds, x, cfg["L"])) < TOL
+    eng.close()
+
+
 @pytest.mark.parametrize("integrator", ["GJF", "EM"])
 def test_replica_group_graph_equals_engines_alone(gpu, integrator):
     """Replicas advanced by the group's HIP graph (eight steps of every engine
