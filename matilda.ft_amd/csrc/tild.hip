@@ -49,6 +49,8 @@ static int tild_setup(TildEngine* e)
 {
     const mft_tild_desc& d = e->d;
     MFT_REQUIRE(d.nparticles > 0, MFT_ERR_INVALID, "Box created with no particles");
+    MFT_REQUIRE(d.nparticles < (1 << kMetaIdBits), MFT_ERR_UNSUPPORTED, "at most %d particles per engine",
+                (1 << kMetaIdBits) - 1);
     MFT_REQUIRE(d.nspecies >= 1 && d.nspecies <= kMaxSpecies, MFT_ERR_INVALID, "nspecies must be 1..%d",
                 kMaxSpecies);
     MFT_REQUIRE(d.pmeorder >= 0 && d.pmeorder <= 4, MFT_ERR_INVALID, "pmeorder must be 0..4");
@@ -981,7 +983,7 @@ int mft_tild_kernel_bytes(mft_tild* e, double* bytes)
     const double nz = e->d.integrator == MFT_INT_GJF ? 0.0 : 16.0;
     // bond types per slot only with more than one type (else none carried)
     const double bt = e->bp.ntypes > 1 ? 4 * mb : 0.0;
-    bytes[MFT_TK_SCATTER] = N * ((4 + 4 + 4 + 12 + nz + 16 + 8 + bt) + (16 + 12 + nz + 8 + bt + 4) +
+    bytes[MFT_TK_SCATTER] = N * ((4 + 4 + 4 + 12 + nz + 16 + 4 + bt) + (16 + 12 + nz + 4 + bt + 4) +
                                  (4 + 8 * mb + 4 * mb));
     bytes[MFT_TK_DENSITY] = N * 16 + M * 4 * C;               // key-ordered positions, rho
     const bool plane = e->plane_passes && tild_plane_supported(e->d.nx, e->d.ny);
@@ -997,7 +999,7 @@ int mft_tild_kernel_bytes(mft_tild* e, double* bytes)
     bytes[MFT_TK_CONV_Z] = M * 8 * (C + O);
     // per-slot grid forces and next positions as 12-byte records
     bytes[MFT_TK_GATHER] = N * (16 + 8 + 12) + M * 12 * O4;
-    bytes[MFT_TK_INTEGRATE] = N * ((16 + 12 + nz + 8 + 4 * mb + bt + 12) + (12 + nz + 4 + 4));
+    bytes[MFT_TK_INTEGRATE] = N * ((16 + 12 + nz + 4 + 4 * mb + bt + 12) + (12 + nz + 4 + 4));
     return MFT_OK;
 }
 

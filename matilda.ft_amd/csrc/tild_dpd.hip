@@ -21,12 +21,12 @@ __device__ __forceinline__ int dpd_cell1(float x, float cw, int n)
 }
 
 // cell of every slot of the group (-1 outside it) and the cell counts
-__global__ void k_dpd_count(const float4* __restrict__ X, const int2* __restrict__ META, DpdParams P, int ns,
+__global__ void k_dpd_count(const float4* __restrict__ X, const PMeta* __restrict__ META, DpdParams P, int ns,
                             int* __restrict__ count, int* __restrict__ cell)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
-    if (!((P.mask >> META[s].y) & 1u)) {
+    if (!((P.mask >> META[s].sp()) & 1u)) {
         cell[s] = -1;
         return;
     }
@@ -76,7 +76,7 @@ __global__ void k_dpd_fill(const int* __restrict__ cell, int ns, int* __restrict
 
 // d_DPD_forces (ps_potentialDPD.cu:63-137) for every pair of slot s within
 // rcut, found in the 27 cells around its own; V null means zero velocities
-__global__ void __launch_bounds__(256) k_dpd_force(const float4* __restrict__ X, const int2* __restrict__ META,
+__global__ void __launch_bounds__(256) k_dpd_force(const float4* __restrict__ X, const PMeta* __restrict__ META,
                                                    const float4* __restrict__ V, const int* __restrict__ cell,
                                                    const int* __restrict__ start, const int* __restrict__ slots,
                                                    DpdParams P, float L0, float L1, float L2, uint64_t seed,
@@ -91,7 +91,7 @@ __global__ void __launch_bounds__(256) k_dpd_force(const float4* __restrict__ X,
     }
     const float L[3] = {L0, L1, L2};
     const float Lh[3] = {0.5f * L0, 0.5f * L1, 0.5f * L2};
-    const int id = META[s].x;
+    const int id = META[s].id();
     const float4 x4 = X[s];
     const float xi[3] = {x4.x, x4.y, x4.z};
     float vi[3] = {0.f, 0.f, 0.f};
@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(256) k_dpd_force(const float4* __restrict__ X,
                     }
                     float vdot = 0.0f;
                     for (int d = 0; d < 3; ++d) vdot += (vi[d] - vj[d]) * (dr[d] * winv);
-                    const int jd = META[t].x;
+                    const int jd = META[t].id();
                     float z[4];
                     normal4((uint32_t)min(id, jd), (uint32_t)max(id, jd), TAG_DPD, step, seed, z);
                     const float Fmag = -P.gamma * w * w * vdot + P.sigma * w * z[0] * P.inv_sqrt_delt;

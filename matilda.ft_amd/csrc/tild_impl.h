@@ -32,6 +32,20 @@ namespace mft {
 const float2* fft_twiddles_device(int n); // fft.hip
 
 constexpr int kMaxSpecies = 16;
+
+// A slot's particle id (bits 0-26) and species (bits 27-31) in one word: the
+// permutation and the integrator move and read 4 bytes of identity per slot.
+constexpr int kMetaIdBits = 27;
+struct PMeta {
+    uint32_t v;
+    __host__ __device__ int id() const { return (int)(v & ((1u << kMetaIdBits) - 1u)); }
+    __host__ __device__ int sp() const { return (int)(v >> kMetaIdBits); }
+};
+__host__ __device__ inline PMeta make_meta(int id, int species)
+{
+    return PMeta{(uint32_t)id | ((uint32_t)species << kMetaIdBits)};
+}
+static_assert(kMaxSpecies <= (1 << (32 - kMetaIdBits)), "species field of PMeta");
 constexpr int kMaxPots = 32;
 constexpr int kMaxBondTypes = 16;
 constexpr int kMaxAngleTypes = 8;
@@ -223,7 +237,7 @@ struct TildEngine {
     float4* X[2] = {nullptr, nullptr};   // x, y, z, charge
     float4* XO[2] = {nullptr, nullptr};  // previous positions (GJF), 12 bytes a slot (ff_load / ff_store)
     float4* NZ[2] = {nullptr, nullptr};  // previous noise (GJF)
-    int2* META[2] = {nullptr, nullptr};  // particle id, species
+    PMeta* META[2] = {nullptr, nullptr}; // particle id, species
     int* BS[2] = {nullptr, nullptr};     // bond partners as slots [N][maxbonds], -1 past the last
     int* BT[2] = {nullptr, nullptr};     // bond types [N][maxbonds]
     int* NA[2] = {nullptr, nullptr};     // angle count (angles only)

@@ -227,7 +227,7 @@ __global__ void k_sort_place(const float* __restrict__ x, const float* __restric
                              const int* __restrict__ type, const int* __restrict__ bondType, int maxbonds, int ns, const int* __restrict__ key,
                              const int* __restrict__ rank, const int* __restrict__ start, int* __restrict__ slot_of,
                              float4* __restrict__ X, float4* __restrict__ XO, float4* __restrict__ NZ,
-                             int2* __restrict__ META, int* __restrict__ BT)
+                             PMeta* __restrict__ META, int* __restrict__ BT)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ns) return;
@@ -236,7 +236,7 @@ __global__ void k_sort_place(const float* __restrict__ x, const float* __restric
     X[s] = make_float4(x[i * 3], x[i * 3 + 1], x[i * 3 + 2], q ? q[i] : 0.f);
     ff_store(XO, s, xo[i * 3], xo[i * 3 + 1], xo[i * 3 + 2]);
     NZ[s] = make_float4(noise[i * 3], noise[i * 3 + 1], noise[i * 3 + 2], 0.f);
-    META[s] = make_int2(i, type[i]);
+    META[s] = make_meta(i, type[i]);
     for (int b = 0; b < maxbonds; ++b) BT[s * maxbonds + b] = bondType[i * maxbonds + b];
 }
 
@@ -269,7 +269,7 @@ __global__ void k_sort_angles(const int* __restrict__ nAngles, const int* __rest
 
 // key order -> particle-id order: out[3 id + j] = src[slot].j (packed: src
 // holds 12-byte records, ff_load)
-__global__ void k_unsort(float* __restrict__ out, const float4* __restrict__ src, const int2* __restrict__ META,
+__global__ void k_unsort(float* __restrict__ out, const float4* __restrict__ src, const PMeta* __restrict__ META,
                          int ns, int packed)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -281,7 +281,7 @@ __global__ void k_unsort(float* __restrict__ out, const float4* __restrict__ src
     } else {
         v = src[s];
     }
-    const int id = META[s].x;
+    const int id = META[s].id();
     out[id * 3 + 0] = v.x;
     out[id * 3 + 1] = v.y;
     out[id * 3 + 2] = v.z;
@@ -353,9 +353,9 @@ __global__ void k_scan_apply(int* __restrict__ count, int M, const int* __restri
 // bond type, whose slots' types are not carried (every bond is type 0).
 __global__ void k_permute(int ns, int maxbonds, const int* __restrict__ key, const int* __restrict__ rank,
                           const int* __restrict__ start, int* __restrict__ map, const float4* __restrict__ XN,
-                          const float4* __restrict__ NN, const float4* __restrict__ X, const int2* __restrict__ META,
+                          const float4* __restrict__ NN, const float4* __restrict__ X, const PMeta* __restrict__ META,
                           const int* __restrict__ BT, float4* __restrict__ X2, float4* __restrict__ XO2,
-                          float4* __restrict__ NZ2, int2* __restrict__ META2, int* __restrict__ BT2)
+                          float4* __restrict__ NZ2, PMeta* __restrict__ META2, int* __restrict__ BT2)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
@@ -657,16 +657,16 @@ __global__ void __launch_bounds__(64) k_tile_reduce(float* __restrict__ rho, con
 template <int P>
 __global__ void __launch_bounds__(256) k_gather(float4* __restrict__ fgrid, const float4* __restrict__ F4,
                                                 const int* __restrict__ force_slot, int qslot,
-                                                const float4* __restrict__ spos, const int2* __restrict__ smeta,
+                                                const float4* __restrict__ spos, const PMeta* __restrict__ smeta,
                                                 int ns, int nx, int ny, int nz, float dx0, float dx1, float dx2)
 {
     constexpr int p = P;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ns) return;
     float4 pp = spos[i];
-    int2 meta = smeta[i];
+    PMeta meta = smeta[i];
     const long M = (long)nx * ny * nz;
-    int fs = force_slot[meta.y];
+    int fs = force_slot[meta.sp()];
     bool charged = qslot >= 0 && pp.w != 0.f;
     float f[3] = {0.f, 0.f, 0.f};
     if (fs >= 0 || charged) {
@@ -829,7 +829,7 @@ struct GatherMap {
 template <int P, int BM>
 __global__ void __launch_bounds__(kTileThreads) k_gather_species(
     float4* __restrict__ fgrid, const float4* __restrict__ F4, GatherMap gm, const float4* __restrict__ spos,
-    const int2* __restrict__ smeta, const int* __restrict__ start, TileGeom tg, int nx, int ny, int nz, float dx0,
+    const PMeta* __restrict__ smeta, const int* __restrict__ start, TileGeom tg, int nx, int ny, int nz, float dx0,
     float dx1, float dx2)
 {
     constexpr int TP = BM + P, T3 = TP * TP * TP;
@@ -893,7 +893,7 @@ __device__ __forceinline__ void window_loads(float3 (&v)[((BM + P) * (BM + P) + 
 template <int P, int BM>
 __global__ void __launch_bounds__(kTileThreads) k_gather_species_pipe(
     float4* __restrict__ fgrid, const float4* __restrict__ F4, GatherMap gm, const float4* __restrict__ spos,
-    const int2* __restrict__ smeta, const int* __restrict__ start, TileGeom tg, int nx, int ny, int nz, float dx0,
+    const PMeta* __restrict__ smeta, const int* __restrict__ start, TileGeom tg, int nx, int ny, int nz, float dx0,
     float dx1, float dx2, int ntasks, int per)
 {
     constexpr int TP = BM + P, T3 = TP * TP * TP, RMAX = (TP * TP + 15) / 16;
@@ -982,7 +982,7 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_species_pipe(
 template <int P, int BM>
 __global__ void __launch_bounds__(kTileThreads) k_gather_brick(
     float4* __restrict__ fgrid, const float4* __restrict__ F4, GatherMap gm, const float4* __restrict__ spos,
-    const int2* __restrict__ smeta, const int* __restrict__ start, TileGeom tg, int nx, int ny, int nz, float dx0,
+    const PMeta* __restrict__ smeta, const int* __restrict__ start, TileGeom tg, int nx, int ny, int nz, float dx0,
     float dx1, float dx2)
 {
     constexpr int TP = BM + P, T3 = TP * TP * TP, shift = P / 2;
@@ -1039,7 +1039,7 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_brick(
     const float i0 = 1.0f / dx0, i1 = 1.0f / dx1, i2 = 1.0f / dx2;
     for (int it = pb + (int)threadIdx.x; it < pe; it += kTileThreads) {
         const float4 pp = spos[it];
-        const int sp = smeta[it].y;
+        const int sp = smeta[it].sp();
         float3 a = make_float3(0.f, 0.f, 0.f);
         if (gm.slot[sp] >= 0) {
             int w = 0;
@@ -1059,7 +1059,7 @@ __global__ void __launch_bounds__(kTileThreads) k_gather_brick(
 template <int P, int BM>
 __global__ void __launch_bounds__(kTileThreads) k_gather_charge(
     float4* __restrict__ fgrid, const float4* __restrict__ Fq, const float4* __restrict__ spos,
-    const int2* __restrict__ smeta, const int* __restrict__ start, TileGeom tg, int nx, int ny, int nz, float dx0,
+    const PMeta* __restrict__ smeta, const int* __restrict__ start, TileGeom tg, int nx, int ny, int nz, float dx0,
     float dx1, float dx2)
 {
     constexpr int TP = BM + P, T3 = TP * TP * TP;
@@ -1297,12 +1297,12 @@ __device__ __forceinline__ SpeciesCoef species_coef(const StepArgs& A, int ty)
     return SpeciesCoef{vv ? im : a, b, nm, cf, cn};
 }
 
-__device__ __forceinline__ int integrate_slot(int s, float4 xc, int2 meta, const float* fo, const StepArgs& A,
+__device__ __forceinline__ int integrate_slot(int s, float4 xc, PMeta meta, const float* fo, const StepArgs& A,
                                               uint32_t step, const TileGeom& tg, float4 xo4, float4 nz4,
                                               const SpeciesCoef& sc)
 {
     float z[4];
-    normal4((uint32_t)meta.x, step, TAG_TILD, 0u, A.seed, z);
+    normal4((uint32_t)meta.id(), step, TAG_TILD, 0u, A.seed, z);
     const float xcv[3] = {xc.x, xc.y, xc.z};
     float xn[3], nn[3] = {0.f, 0.f, 0.f};
     if (A.integ == MFT_INT_GJF) {
@@ -1311,7 +1311,7 @@ __device__ __forceinline__ int integrate_slot(int s, float4 xc, int2 meta, const
         if (A.nz_keyed) {
             // the previous step's nm * normal, the same bits as when it was drawn
             float zo[4];
-            normal4((uint32_t)meta.x, step - 1u, TAG_TILD, 0u, A.seed, zo);
+            normal4((uint32_t)meta.id(), step - 1u, TAG_TILD, 0u, A.seed, zo);
             for (int j = 0; j < 3; ++j) nov[j] = nm * zo[j];
         } else {
             nov[0] = nz4.x;
@@ -1365,7 +1365,7 @@ __device__ __forceinline__ int integrate_slot(int s, float4 xc, int2 meta, const
     }
     ff_store(A.XN, s, xn[0], xn[1], xn[2]);
     if (A.write_nn) A.NN[s] = make_float4(nn[0], nn[1], nn[2], 0.f);
-    return key_of(xn[0], xn[1], xn[2], meta.y, A.dx[0], A.dx[1], A.dx[2], A.p, A.nx, A.ny, A.nz, tg);
+    return key_of(xn[0], xn[1], xn[2], meta.sp(), A.dx[0], A.dx[1], A.dx[2], A.p, A.nx, A.ny, A.nz, tg);
 }
 
 // Bonds (ps_deviceBonds.cu:7-79) plus the integrator on group "all", one
@@ -1377,7 +1377,7 @@ __device__ __forceinline__ int integrate_slot(int s, float4 xc, int2 meta, const
 // step index on the device (graph replays), else null.
 template <bool TWO>
 __global__ void __launch_bounds__(256) k_integrate(
-    const float4* __restrict__ X, const int2* __restrict__ META, const int* __restrict__ BS,
+    const float4* __restrict__ X, const PMeta* __restrict__ META, const int* __restrict__ BS,
     const int* __restrict__ BT, const float4* __restrict__ fgrid,
     float4* __restrict__ ftot, int maxbonds, BondParams bp, StepArgs A, int ns, int* __restrict__ count,
     int* __restrict__ key, int* __restrict__ rank, TileGeom tg, const uint32_t* __restrict__ dstep)
@@ -1393,7 +1393,7 @@ __global__ void __launch_bounds__(256) k_integrate(
         // partners' positions next: two rounds of memory latency per slot
         const float4 xc = X[s];
         const float3 fg = ff_load(fgrid, s);
-        const int2 meta = META[s];
+        const PMeta meta = META[s];
         const bool gjf = A.integ == MFT_INT_GJF;
         const bool need_nz = (gjf && !A.nz_keyed) || A.integ == MFT_INT_VV || A.use_v;
         const float3 xo3 = gjf ? ff_load(A.XO, s) : make_float3(0.f, 0.f, 0.f);
@@ -1418,15 +1418,15 @@ __global__ void __launch_bounds__(256) k_integrate(
             const float4 y0 = X[v0 ? bs.x : s];
             const float4 y1 = X[v1 ? bs.y : s];
             // the species' coefficients in the same round of loads
-            sco = species_coef(A, meta.y);
+            sco = species_coef(A, meta.sp());
             if (v0) bond_force_params(st0, k0, q0, A.L[0], A.L[1], A.L[2], xc, y0, fo);
             if (v1) bond_force_params(st1, k1, q1, A.L[0], A.L[1], A.L[2], xc, y1, fo);
         } else {
-            sco = species_coef(A, meta.y);
+            sco = species_coef(A, meta.sp());
             bond_force_slots(s, ns, X, BS, BT, maxbonds, bp, A.L[0], A.L[1], A.L[2], xc.x, xc.y, xc.z, fo);
         }
         if (A.NA) angle_force_slots(s, X, A.NA, A.AT, A.AG, A.ap, A.L[0], A.L[1], A.L[2], fo);
-        if (A.walls.n) wall_forces(A.walls, meta.y, xc, fo);
+        if (A.walls.n) wall_forces(A.walls, meta.sp(), xc, fo);
         if (A.FD) {
             const float4 fd = A.FD[s];
             fo[0] += fd.x;
@@ -1444,7 +1444,7 @@ __global__ void __launch_bounds__(256) k_integrate(
             }
             // under VV these are the forces of the previous step (its noise)
             const uint32_t ls = (A.use_v && A.vv != 0) ? step - 1u : step;
-            langevin_forces(A.lang, meta.x, meta.y, v, A.seed, ls, fo);
+            langevin_forces(A.lang, meta.id(), meta.sp(), v, A.seed, ls, fo);
         }
         if (ftot) ftot[s] = make_float4(fo[0], fo[1], fo[2], 0.f);
         if (A.integ != MFT_INT_NONE) k = integrate_slot(s, xc, meta, fo, A, step, tg, xo4, nz4, sco);
@@ -1463,7 +1463,7 @@ __global__ void __launch_bounds__(256) k_integrate(
 // d_bondVirScratch); dr with the 0.5*Lh minimum image (:132-133); no term for
 // r^2 <= 1e-4 or for FENE at or beyond r0, where the reference leaves mf unset.
 __global__ void k_bond_virial_slots(float* __restrict__ vir, const float4* __restrict__ X,
-                                    const int2* __restrict__ META, const int* __restrict__ BS,
+                                    const PMeta* __restrict__ META, const int* __restrict__ BS,
                                     const int* __restrict__ BT, BondParams bp, float L0,
                                     float L1, float L2, int ns, int maxbonds)
 {
@@ -1500,7 +1500,7 @@ __global__ void k_bond_virial_slots(float* __restrict__ vir, const float4* __res
         v[4] += -mf * d[0] * d[2];
         v[5] += -mf * d[1] * d[2];
     }
-    const long id = META[s].x;
+    const long id = META[s].id();
     for (int c = 0; c < 6; ++c) vir[id * 6 + c] = v[c];
 }
 
@@ -1672,18 +1672,19 @@ int tild_sort_state(TildEngine* e)
 
 // previous GJF noise in particle-id order, regenerated from (particle,
 // step - 1): out[3 id + j] = nmag(type) * normal_j
-__global__ void k_unsort_keyed_noise(float* __restrict__ out, const int2* __restrict__ META, GjfParams gp, int ns,
+__global__ void k_unsort_keyed_noise(float* __restrict__ out, const PMeta* __restrict__ META, GjfParams gp, int ns,
                                      uint64_t seed, uint32_t prev_step)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
-    const int2 m = META[s];
+    const PMeta m = META[s];
+    const int id = m.id();
     float z[4];
-    normal4((uint32_t)m.x, prev_step, TAG_TILD, 0u, seed, z);
-    const float nm = gp.nmag[m.y];
-    out[m.x * 3 + 0] = nm * z[0];
-    out[m.x * 3 + 1] = nm * z[1];
-    out[m.x * 3 + 2] = nm * z[2];
+    normal4((uint32_t)id, prev_step, TAG_TILD, 0u, seed, z);
+    const float nm = gp.nmag[m.sp()];
+    out[id * 3 + 0] = nm * z[0];
+    out[id * 3 + 1] = nm * z[1];
+    out[id * 3 + 2] = nm * z[2];
 }
 
 // key order -> particle-id order into pio: what 0 positions, 1 previous
@@ -1961,11 +1962,11 @@ int tild_integrate(TildEngine* e, bool want_f)
 
 // d_wallHarmonicEnergy (ps_potentialWall.cu:129-147), FP64 block sums
 __global__ void k_wall_energy(double* __restrict__ partial, const float4* __restrict__ X,
-                              const int2* __restrict__ META, WallParams W, int w, int ns)
+                              const PMeta* __restrict__ META, WallParams W, int w, int ns)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     double v = 0.0;
-    if (s < ns && ((W.mask[w] >> META[s].y) & 1u)) {
+    if (s < ns && ((W.mask[w] >> META[s].sp()) & 1u)) {
         const float4 xc = X[s];
         const int a = W.axis[w];
         const float disp = W.pos[w] - (a == 0 ? xc.x : (a == 1 ? xc.y : xc.z));
