@@ -222,7 +222,8 @@ typedef struct {
 typedef struct mft_tild mft_tild;
 
 /* Engine with device-resident state.  create copies the description; the
- * state arrays below are host pointers copied in (set_*) or out (get_*). */
+ * state arrays below are host pointers copied in (set_*) or out (get_*).
+ * An engine holds at most 2^27 - 1 particles (MFT_ERR_UNSUPPORTED beyond). */
 int mft_tild_create(mft_tild** eng, const mft_tild_desc* desc);
 int mft_tild_destroy(mft_tild* eng);
 void* mft_tild_stream(mft_tild* eng);
